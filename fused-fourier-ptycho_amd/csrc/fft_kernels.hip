@@ -1,0 +1,360 @@
+// fft_kernels.hip -- batched 2D C2C transforms for gfx950.
+//
+// Column-major R x C pages (element (i, j) at j*R + i).  A 2D transform is
+//   contiguous pass : one LEN=R transform per column, a whole column in LDS;
+//   strided pass(es): one LEN=C transform per row, on tiles of 16 adjacent rows
+//                     (128-byte segments per column).  C <= 512 takes one
+//                     pass; larger C takes two (four-step, C = M1 M2) and the
+//                     frequency side is then stored column-permuted (ColPerm):
+//                     forward = contiguous, A (length M1, twiddle), B (length M2)
+//                     inverse = B^-1 (length M2, twiddle), A^-1 (length M1), contiguous.
+// Every pass reads and writes each element once (16 B / element / pass).
+// These replace the cufftPlan2d / cufftPlanMany transforms of
+// cuFPM_v3/cuFPM_pure_v3.cu:118-126 (unnormalised C2C, cuFFT sign convention).
+#include "fft_core.h"
+#include "common.h"
+#include "fft_kernels.h"
+#include "fpm_kernels.h"
+
+namespace fpm {
+
+// ------------------------------------------------------- contiguous pass
+template <int LEN> struct ContigCfg {
+    static constexpr int T = LEN >= 16384 ? 1024 : (LEN >= 8192 ? 512 : 256);
+    static constexpr int LINES = T * 16 / LEN;
+    using Map = LineSlowMap<LINES, LEN>;
+};
+
+template <int LEN, bool INV>
+__global__ __launch_bounds__(ContigCfg<LEN>::T) void fft_contig_kernel(const float2* __restrict__ in,
+                                                                      float2* __restrict__ out,
+                                                                      uint32_t nlines, Twiddles tw) {
+    using Cfg = ContigCfg<LEN>;
+    using Map = typename Cfg::Map;
+    using SL = Slots<LEN, INV, Cfg::T, Map>;
+    __shared__ float2 lds[Map::LDS_WORDS];
+    const int t = threadIdx.x;
+    const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16 / SL::R0; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::R0; ++s) {
+            int line, idx;
+            SL::in(t, q, s, line, idx);
+            const size_t gl = line0 + line;
+            v[q * SL::R0 + s] = gl < nlines ? in[gl * LEN + idx] : make_float2(0.f, 0.f);
+        }
+    first_pass_dft<LEN, INV>(v);
+    remaining_passes<LEN, INV, Cfg::T, Map>(v, lds, t, tw);
+#pragma unroll
+    for (int q = 0; q < 16 / SL::RL; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::RL; ++s) {
+            int line, idx;
+            SL::out(t, q, s, line, idx);
+            const size_t gl = line0 + line;
+            if (gl < nlines) out[gl * LEN + idx] = v[q * SL::RL + s];
+        }
+}
+
+// --------------------------------------------------------- strided pass
+// blockIdx.x: tile of 16 rows; blockIdx.y: group g; blockIdx.z: page.
+// Point t of row r is read from column g*in_g + t*in_s and written to column
+// g*out_g + t*out_s.  TWID multiplies output t by W_M^(+-g t) with
+// tw_step = 16384 / M.
+struct StridedArgs {
+    const float2* in;
+    float2* out;
+    uint32_t rows;
+    uint64_t page;  // elements per page
+    uint32_t in_g, in_s, out_g, out_s;
+    uint32_t tw_step;
+};
+
+template <int LEN> struct StridedCfg {
+    static constexpr int T = LEN;  // 16 rows x LEN points / 16 per thread
+    using Map = LineFastMap<kStridedRows, LEN>;
+};
+
+template <int LEN, bool INV, bool TWID>
+__global__ __launch_bounds__(StridedCfg<LEN>::T) void fft_strided_kernel(StridedArgs a, Twiddles tw) {
+    using Cfg = StridedCfg<LEN>;
+    using Map = typename Cfg::Map;
+    using SL = Slots<LEN, INV, Cfg::T, Map>;
+    __shared__ float2 lds[Map::LDS_WORDS];
+    const int t = threadIdx.x;
+    const uint32_t g = blockIdx.y;
+    const size_t base = (size_t)blockIdx.z * a.page + (size_t)blockIdx.x * kStridedRows;
+    const float2* __restrict__ in = a.in + base;
+    float2* __restrict__ out = a.out + base;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16 / SL::R0; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::R0; ++s) {
+            int line, idx;
+            SL::in(t, q, s, line, idx);
+            const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
+            v[q * SL::R0 + s] = in[col * a.rows + line];
+        }
+    first_pass_dft<LEN, INV>(v);
+    remaining_passes<LEN, INV, Cfg::T, Map>(v, lds, t, tw);
+#pragma unroll
+    for (int q = 0; q < 16 / SL::RL; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::RL; ++s) {
+            int line, idx;
+            SL::out(t, q, s, line, idx);
+            float2 r = v[q * SL::RL + s];
+            if constexpr (TWID) r = cmul(r, tw.get<INV>(g * (uint32_t)idx * a.tw_step));
+            const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
+            out[col * a.rows + line] = r;
+        }
+}
+
+// -------------------------------------------------------------- launchers
+template <bool INV>
+static void launch_contig_len(int len, const float2* in, float2* out, uint32_t nlines, Twiddles tw, hipStream_t st) {
+#define FPM_CONTIG_CASE(L)                                                                              \
+    case L: {                                                                                           \
+        dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                            \
+        hipLaunchKernelGGL((fft_contig_kernel<L, INV>), grid, dim3(ContigCfg<L>::T), 0, st, in, out, nlines, tw); \
+        break;                                                                                          \
+    }
+    switch (len) {
+        FPM_CONTIG_CASE(16)
+        FPM_CONTIG_CASE(32)
+        FPM_CONTIG_CASE(64)
+        FPM_CONTIG_CASE(128)
+        FPM_CONTIG_CASE(256)
+        FPM_CONTIG_CASE(512)
+        FPM_CONTIG_CASE(1024)
+        FPM_CONTIG_CASE(2048)
+        FPM_CONTIG_CASE(4096)
+        FPM_CONTIG_CASE(8192)
+        FPM_CONTIG_CASE(16384)
+        default: throw HipError{hipErrorInvalidValue, "contiguous FFT length " + std::to_string(len) + " unsupported"};
+    }
+#undef FPM_CONTIG_CASE
+    FPM_LAUNCH_CHECK();
+}
+
+template <bool INV, bool TWID>
+static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st) {
+#define FPM_STRIDED_CASE(L)                                                                             \
+    case L:                                                                                             \
+        hipLaunchKernelGGL((fft_strided_kernel<L, INV, TWID>), grid, dim3(StridedCfg<L>::T), 0, st, a, tw); \
+        break;
+    switch (len) {
+        FPM_STRIDED_CASE(16)
+        FPM_STRIDED_CASE(32)
+        FPM_STRIDED_CASE(64)
+        FPM_STRIDED_CASE(128)
+        FPM_STRIDED_CASE(256)
+        FPM_STRIDED_CASE(512)
+        default: throw HipError{hipErrorInvalidValue, "strided FFT length " + std::to_string(len) + " unsupported"};
+    }
+#undef FPM_STRIDED_CASE
+    FPM_LAUNCH_CHECK();
+}
+
+void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bool inverse, const float2* twid,
+                hipStream_t st) {
+    Twiddles tw{twid};
+    if (inverse) launch_contig_len<true>((int)len, in, out, nlines, tw, st);
+    else launch_contig_len<false>((int)len, in, out, nlines, tw, st);
+}
+
+// forward column transform(s): natural columns in `data` -> permuted (in
+// place) or natural (into nat_out) frequency columns
+void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
+                      hipStream_t st, float2* nat_out) {
+    Twiddles tw{twid};
+    StridedArgs a{};
+    a.rows = rows;
+    a.page = (uint64_t)rows * cols;
+    const uint32_t tiles = rows / kStridedRows;
+    if (p.M2 == 1) {
+        a.in = data;
+        a.out = nat_out ? nat_out : data;
+        a.in_g = a.out_g = 0;
+        a.in_s = a.out_s = 1;
+        launch_strided_len<false, false>((int)cols, a, dim3(tiles, 1, batch), tw, st);
+        return;
+    }
+    // pass A: length M1 over columns c2 + M2 c1, twiddle W_C^(c2 k1)
+    a.in = data;
+    a.out = data;
+    a.in_g = a.out_g = 1;
+    a.in_s = a.out_s = p.M2;
+    a.tw_step = kTwiddleLen / cols;
+    launch_strided_len<false, true>((int)p.M1, a, dim3(tiles, p.M2, batch), tw, st);
+    // pass B: length M2 over the contiguous block of group k1
+    a.in_g = p.M2;
+    a.in_s = 1;
+    if (nat_out) {
+        a.out = nat_out;
+        a.out_g = 1;
+        a.out_s = p.M1;
+    } else {
+        a.out_g = p.M2;
+        a.out_s = 1;
+    }
+    launch_strided_len<false, false>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st);
+}
+
+// inverse column transform(s): permuted (or natural, from nat_in) frequency
+// columns -> natural columns in `data`
+void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
+                      hipStream_t st, const float2* nat_in) {
+    Twiddles tw{twid};
+    StridedArgs a{};
+    a.rows = rows;
+    a.page = (uint64_t)rows * cols;
+    const uint32_t tiles = rows / kStridedRows;
+    if (p.M2 == 1) {
+        a.in = nat_in ? nat_in : data;
+        a.out = data;
+        a.in_g = a.out_g = 0;
+        a.in_s = a.out_s = 1;
+        launch_strided_len<true, false>((int)cols, a, dim3(tiles, 1, batch), tw, st);
+        return;
+    }
+    // pass B^-1: length M2 over k2 for group k1, twiddle W_C^-(c2 k1)
+    if (nat_in) {
+        a.in = nat_in;
+        a.in_g = 1;
+        a.in_s = p.M1;
+    } else {
+        a.in = data;
+        a.in_g = p.M2;
+        a.in_s = 1;
+    }
+    a.out = data;
+    a.out_g = p.M2;
+    a.out_s = 1;
+    a.tw_step = kTwiddleLen / cols;
+    launch_strided_len<true, true>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st);
+    // pass A^-1: length M1 over k1 for group c2
+    a.in = data;
+    a.in_g = a.out_g = 1;
+    a.in_s = a.out_s = p.M2;
+    launch_strided_len<true, false>((int)p.M1, a, dim3(tiles, p.M2, batch), tw, st);
+}
+
+void fft2_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
+                  hipStream_t st) {
+    fft_contig(data, data, rows, cols * batch, false, twid, st);
+    fft_cols_forward(data, rows, cols, batch, p, twid, st, nullptr);
+}
+
+void fft2_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
+                  hipStream_t st) {
+    fft_cols_inverse(data, rows, cols, batch, p, twid, st, nullptr);
+    fft_contig(data, data, rows, cols * batch, true, twid, st);
+}
+
+
+// ------------------------------------------- fused object update kernel
+// Finishes the inverse transform of the object-spectrum gradient along the
+// contiguous index (G holds the gradient after its column passes), applies
+// RMSprop to O (cuFPM_v3/cuda/kernels.cuh:364-382: m = beta m + (1-beta)|G|^2,
+// O -= lr G/(sqrt(m)+eps)) and, with FWD, starts the forward transform of the
+// updated O on the same registers, writing it back into G.  The gradient never
+// travels through HBM as a finished array and is never cleared: the next
+// step overwrites the buffer.
+template <int LEN, bool FWD>
+__global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2* __restrict__ G,
+                                                                        float2* __restrict__ O,
+                                                                        float* __restrict__ mO, uint32_t nlines,
+                                                                        RmsArgs rms, Twiddles tw) {
+    using Cfg = ContigCfg<LEN>;
+    using Map = typename Cfg::Map;
+    using SI = Slots<LEN, true, Cfg::T, Map>;
+    using SF = Slots<LEN, false, Cfg::T, Map>;
+    static_assert(SI::RL == SF::R0, "inverse last pass and forward first pass must share the slot map");
+    __shared__ float2 lds[Map::LDS_WORDS];
+    const int t = threadIdx.x;
+    const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16 / SI::R0; ++q)
+#pragma unroll
+        for (int s = 0; s < SI::R0; ++s) {
+            int line, idx;
+            SI::in(t, q, s, line, idx);
+            const size_t gl = line0 + line;
+            v[q * SI::R0 + s] = gl < nlines ? G[gl * LEN + idx] : make_float2(0.f, 0.f);
+        }
+    first_pass_dft<LEN, true>(v);
+    remaining_passes<LEN, true, Cfg::T, Map>(v, lds, t, tw);
+#pragma unroll
+    for (int q = 0; q < 16 / SI::RL; ++q)
+#pragma unroll
+        for (int s = 0; s < SI::RL; ++s) {
+            int line, idx;
+            SI::out(t, q, s, line, idx);
+            const size_t gl = line0 + line;
+            if (gl < nlines) {
+                const size_t k = gl * LEN + idx;
+                const float2 gr = v[q * SI::RL + s];
+                float mu = gr.x * gr.x + gr.y * gr.y;
+                mu = rms.beta * mO[k] + (1.0f - rms.beta) * mu;
+                const float sq = rms.lr / (sqrtf(mu) + rms.eps);
+                float2 w = O[k];
+                w.x -= gr.x * sq;
+                w.y -= gr.y * sq;
+                O[k] = w;
+                mO[k] = mu;
+                v[q * SI::RL + s] = w;
+            }
+        }
+    if constexpr (FWD) {
+        first_pass_dft<LEN, false>(v);
+        __syncthreads();
+        remaining_passes<LEN, false, Cfg::T, Map>(v, lds, t, tw);
+#pragma unroll
+        for (int q = 0; q < 16 / SF::RL; ++q)
+#pragma unroll
+            for (int s = 0; s < SF::RL; ++s) {
+                int line, idx;
+                SF::out(t, q, s, line, idx);
+                const size_t gl = line0 + line;
+                if (gl < nlines) G[gl * LEN + idx] = v[q * SF::RL + s];
+            }
+    }
+}
+
+void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
+                         const float2* twid, hipStream_t st) {
+    Twiddles tw{twid};
+    const uint32_t nlines = g.Nc;
+#define FPM_FUSED_CASE(L)                                                                                   \
+    case L: {                                                                                               \
+        dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                                \
+        if (forward_after)                                                                                  \
+            hipLaunchKernelGGL((fused_update_kernel<L, true>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, rms, tw); \
+        else                                                                                                \
+            hipLaunchKernelGGL((fused_update_kernel<L, false>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, rms, tw); \
+        break;                                                                                              \
+    }
+    switch (g.Nr) {
+        FPM_FUSED_CASE(16)
+        FPM_FUSED_CASE(32)
+        FPM_FUSED_CASE(64)
+        FPM_FUSED_CASE(128)
+        FPM_FUSED_CASE(256)
+        FPM_FUSED_CASE(512)
+        FPM_FUSED_CASE(1024)
+        FPM_FUSED_CASE(2048)
+        FPM_FUSED_CASE(4096)
+        FPM_FUSED_CASE(8192)
+        FPM_FUSED_CASE(16384)
+        default: throw HipError{hipErrorInvalidValue, "fused update length " + std::to_string(g.Nr) + " unsupported"};
+    }
+#undef FPM_FUSED_CASE
+    FPM_LAUNCH_CHECK();
+}
+
+}  // namespace fpm

@@ -1,0 +1,548 @@
+// fpm_api.hip -- host side of libfpm_amd: argument checking, buffers, the
+// epoch/minibatch loop and the C-ABI entry points declared in include/fpm_amd.h.
+//
+// The solve follows cuFPM_v3/cuFPM_pure_v3.cu:44-228: per minibatch (a
+// contiguous chunk of LEDs) Shat = DFT(O), crop x pupil, batched IDFT, loss
+// stencil, batched DFT, back-projection into the spectrum and pupil gradients,
+// IDFT of the spectrum gradient, RMSprop of O and P with the pupil clamp;
+// lr halves after every epoch with index > 12.  The MI355X schedule differs
+// in how the work is grouped, not in what is computed:
+//   * the object update is fused with the last inverse pass of the gradient
+//     and the first forward pass of the next step's DFT(O) (one HBM pass
+//     instead of three), so the spectrum of O is carried from step to step;
+//   * the pupil gradient is summed per pixel over the batch in registers and
+//     the pupil update runs in the back-projection kernel;
+//   * everything is queued on one stream with no host synchronisation inside
+//     the solve (the reference synchronises every step, cuFPM_pure_v3.cu:202).
+#include "../../include/fpm_amd.h"
+#include "common.h"
+#include "fft_core.h"
+#include "fft_kernels.h"
+#include "fpm_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace fpm {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* get_error() { return g_last_error.c_str(); }
+
+const float2* twiddle_table(int device) {
+    static std::mutex mu;
+    static std::map<int, float2*> tables;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = tables.find(device);
+    if (it != tables.end()) return it->second;
+    std::vector<float2> h(kTwiddleLen);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int m = 0; m < kTwiddleLen; ++m) {
+        const double a = -two_pi * (double)m / (double)kTwiddleLen;
+        h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    float2* d = nullptr;
+    FPM_HIP(hipMalloc(&d, sizeof(float2) * kTwiddleLen));
+    FPM_HIP(hipMemcpy(d, h.data(), sizeof(float2) * kTwiddleLen, hipMemcpyHostToDevice));
+    tables[device] = d;
+    return d;
+}
+
+namespace {
+
+struct InvalidArg {
+    int code;
+    std::string what;
+};
+
+[[noreturn]] void invalid(const std::string& w, int code = FPM_E_INVALID) { throw InvalidArg{code, w}; }
+
+// RAII device buffer
+struct DevBuf {
+    void* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T> T* as() const { return (T*)p; }
+    void alloc(size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw InvalidArg{FPM_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                              hipGetErrorString(e)};
+        }
+    }
+};
+
+// stream: caller's, or one created for the call (default priority)
+struct StreamHolder {
+    hipStream_t s = nullptr;
+    bool own = false;
+    explicit StreamHolder(void* user) {
+        if (user) {
+            s = (hipStream_t)user;
+        } else {
+            FPM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            own = true;
+        }
+    }
+    ~StreamHolder() {
+        if (own && s) (void)hipStreamDestroy(s);
+    }
+};
+
+fpm_opts merged_opts(const fpm_opts* user) {
+    fpm_opts o;
+    fpm_opts_init(&o);
+    if (user) {
+        size_t n = user->struct_size > 0 ? std::min<size_t>((size_t)user->struct_size, sizeof(fpm_opts)) : sizeof(fpm_opts);
+        std::memcpy(&o, user, n);
+        o.struct_size = sizeof(fpm_opts);
+    }
+    if (!(o.pupil_clamp > 0.0f)) o.pupil_clamp = 0.2f;
+    return o;
+}
+
+void check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        invalid("no HIP device available", FPM_E_NODEVICE);
+    }
+}
+
+void check_side(int64_t v, const char* what) {
+    if (!is_pow2(v) || v < 16 || v > 16384)
+        invalid(std::string(what) + " = " + std::to_string(v) +
+                    ": every image side must be a power of two in [16, 16384]",
+                FPM_E_UNSUPPORTED);
+}
+
+Geom make_geom(int64_t Nr, int64_t Nc, int64_t nr, int64_t nc) {
+    check_side(Nr, "N_rows");
+    check_side(Nc, "N_cols");
+    check_side(nr, "n_rows");
+    check_side(nc, "n_cols");
+    if (nr > Nr || nc > Nc) invalid("the pupil/images are larger than the object");
+    Geom g;
+    g.nr = (uint32_t)nr;
+    g.nc = (uint32_t)nc;
+    g.Nr = (uint32_t)Nr;
+    g.Nc = (uint32_t)Nc;
+    g.lnr = ilog2_host(g.nr);
+    g.pn = make_colperm(g.nc);
+    g.pN = make_colperm(g.Nc);
+    return g;
+}
+
+// device view of a caller array: the pointer itself, or an uploaded copy
+template <class T> const T* upload(const T* host_or_dev, size_t count, bool is_device, DevBuf& holder, hipStream_t st) {
+    if (is_device) return host_or_dev;
+    holder.alloc(sizeof(T) * count);
+    FPM_HIP(hipMemcpyAsync(holder.p, host_or_dev, sizeof(T) * count, hipMemcpyHostToDevice, st));
+    return holder.as<T>();
+}
+
+// image stack on the device, with the uint16 normalisation constants
+YStack prepare_y(const void* Y, int32_t y_dtype, size_t count, bool is_device, DevBuf& holder, hipStream_t st) {
+    YStack y{};
+    if (y_dtype == FPM_Y_F32_AMPLITUDE) {
+        y.data = upload((const float*)Y, count, is_device, holder, st);
+        y.u16 = 0;
+        return y;
+    }
+    if (y_dtype != FPM_Y_U16_RAW) invalid("y_dtype must be FPM_Y_F32_AMPLITUDE or FPM_Y_U16_RAW");
+    const uint16_t* d = upload((const uint16_t*)Y, count, is_device, holder, st);
+    DevBuf mm;
+    mm.alloc(2 * sizeof(uint32_t));
+    const uint32_t init[2] = {0xffffu, 0u};
+    FPM_HIP(hipMemcpyAsync(mm.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+    launch_u16_minmax(d, count, mm.as<uint32_t>(), st);
+    uint32_t h[2];
+    FPM_HIP(hipMemcpyAsync(h, mm.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    FPM_HIP(hipStreamSynchronize(st));
+    if (h[1] <= h[0]) invalid("uint16 image stack is constant (max == min): amplitudes undefined");
+    y.data = d;
+    y.u16 = 1;
+    y.mn = h[0];
+    y.range = (float)(h[1] - h[0]);
+    return y;
+}
+
+int run_guarded(const char* fn, const std::function<void()>& body) {
+    set_error("");
+    try {
+        body();
+        return FPM_OK;
+    } catch (const InvalidArg& e) {
+        set_error(std::string(fn) + ": " + e.what);
+        return e.code;
+    } catch (const HipError& e) {
+        set_error(std::string(fn) + ": " + e.what);
+        return e.code == hipErrorOutOfMemory ? FPM_E_NOMEM : FPM_E_HIP;
+    } catch (const std::exception& e) {
+        set_error(std::string(fn) + ": " + e.what());
+        return FPM_E_HIP;
+    }
+}
+
+// Per-minibatch gradient pipeline shared by the solve and the gradient entry.
+struct Workspace {
+    DevBuf lat, sub, rec;
+};
+
+// Shat (full spectrum, column-permuted) -> per-LED back-projection into Gs and
+// the pupil part (update or gradient output)
+void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2* led, uint32_t led0, uint32_t b,
+                        const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
+                        RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st) {
+    float2* lat = ws.lat.as<float2>();
+    float2* sub = ws.sub.as<float2>();
+    float2* rec = ws.rec.as<float2>();
+    launch_crop_pupil(g, Shat, P, led, led0, b, sub, lat, st);
+    fft2_inverse(lat, g.nr, g.nc, b, g.pn, tw, st);
+    launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st);
+    fft2_forward(rec, g.nr, g.nc, b, g.pn, tw, st);
+    if (stitch == FPM_STITCH_OWNER) {
+        launch_owner_stitch(g, rec, P, led, led0, b, Gs, st);
+        launch_backproj(g, rec, sub, led, led0, b, P, Gs, false, mP, A, rms, GP, st);
+    } else {
+        launch_zero(Gs, (size_t)g.Nr * g.Nc, st);
+        launch_backproj(g, rec, sub, led, led0, b, P, Gs, true, mP, A, rms, GP, st);
+    }
+}
+
+}  // namespace
+}  // namespace fpm
+
+using namespace fpm;
+
+extern "C" {
+
+void fpm_opts_init(fpm_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->struct_size = sizeof(fpm_opts);
+    o->stitch = FPM_STITCH_ATOMIC;
+    o->pointers_are_device = 0;
+    o->verbose = 0;
+    o->stream = nullptr;
+    o->loss_per_epoch = nullptr;
+    o->pupil_clamp = 0.2f;
+}
+
+const char* fpm_last_error(void) { return get_error(); }
+
+int32_t fpm_abi_version(void) { return FPM_AMD_ABI_VERSION; }
+
+int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32* P0, int64_t n_rows, int64_t n_cols,
+                const void* Y, int32_t y_dtype, int64_t L, const int32_t* led_idx, const float* A, double pratio,
+                double epochs, double batch, const double opt[3], fpm_c32* O_out, fpm_c32* P_out,
+                const fpm_opts* opts) {
+    return run_guarded("fpm_pure_v3", [&] {
+        const fpm_opts o = merged_opts(opts);
+        if (!O0 || !P0 || !Y || !led_idx || !A || !opt || !O_out || !P_out) invalid("null array argument");
+        const Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        if (L < 1) invalid("L (number of images) must be >= 1");
+        // doubles truncated to int as the mex does (cuFPM_pure_v3.cu:57-59)
+        const int pr = (int)pratio, ne = (int)epochs, bs = (int)batch;
+        if (pr < 1) invalid("pratio must be >= 1");
+        if (ne < 0) invalid("numEpochs must be >= 0");
+        if (bs < 1) invalid("batch must be >= 1");
+        if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
+        float lr = (float)opt[0];
+        const float beta = (float)opt[1], eps = (float)opt[2];
+        if (o.verbose) {
+            std::printf("prhs4: %d, prhs5: %d, prhs6: %d \n", pr, ne, bs);
+            std::printf("lr: %f, beta: %f, eps: %f \n", lr, beta, eps);
+        }
+        check_device();
+        int dev = 0;
+        FPM_HIP(hipGetDevice(&dev));
+        const float2* tw = twiddle_table(dev);
+        StreamHolder sh(o.stream);
+        hipStream_t st = sh.s;
+        const bool isdev = o.pointers_are_device != 0;
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        const uint32_t B = (uint32_t)std::min<int64_t>(bs, L);
+
+        // working O and P (outputs double as the working arrays)
+        DevBuf dO, dP;
+        float2 *O, *P;
+        if (isdev) {
+            O = (float2*)O_out;
+            P = (float2*)P_out;
+            if ((const void*)O0 != (void*)O_out)
+                FPM_HIP(hipMemcpyAsync(O, O0, NN * sizeof(float2), hipMemcpyDeviceToDevice, st));
+            if ((const void*)P0 != (void*)P_out)
+                FPM_HIP(hipMemcpyAsync(P, P0, nn * sizeof(float2), hipMemcpyDeviceToDevice, st));
+        } else {
+            dO.alloc(NN * sizeof(float2));
+            dP.alloc(nn * sizeof(float2));
+            O = dO.as<float2>();
+            P = dP.as<float2>();
+            FPM_HIP(hipMemcpyAsync(O, O0, NN * sizeof(float2), hipMemcpyHostToDevice, st));
+            FPM_HIP(hipMemcpyAsync(P, P0, nn * sizeof(float2), hipMemcpyHostToDevice, st));
+        }
+        DevBuf hY, hLed, hA;
+        const YStack y = prepare_y(Y, y_dtype, nn * (size_t)L, isdev, hY, st);
+        const int2* led = (const int2*)upload(led_idx, 2 * (size_t)L, isdev, hLed, st);
+        const float* dA = upload(A, nn, isdev, hA, st);
+
+        if (ne > 0) {
+            DevBuf S0, S1, mO, mP, lossd;
+            Workspace ws;
+            S0.alloc(NN * sizeof(float2));
+            S1.alloc(NN * sizeof(float2));
+            mO.alloc(NN * sizeof(float));
+            mP.alloc(nn * sizeof(float));
+            ws.lat.alloc(nn * B * sizeof(float2));
+            ws.sub.alloc(nn * B * sizeof(float2));
+            ws.rec.alloc(nn * B * sizeof(float2));
+            FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
+            FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
+            if (o.loss_per_epoch) {
+                lossd.alloc((size_t)ne * sizeof(double));
+                FPM_HIP(hipMemsetAsync(lossd.p, 0, (size_t)ne * sizeof(double), st));
+            }
+            float2* Shat = S0.as<float2>();
+            float2* Gs = S1.as<float2>();
+            // spectrum of the initial object
+            fft_contig(O, Shat, g.Nr, g.Nc, false, tw, st);
+            fft_cols_forward(Shat, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+            const uint32_t steps = (uint32_t)((L + B - 1) / B);
+            for (int e = 0; e < ne; ++e) {
+                const RmsArgs rms{lr, beta, eps, o.pupil_clamp};
+                double* lacc = o.loss_per_epoch ? lossd.as<double>() + e : nullptr;
+                for (uint32_t s = 0; s < steps; ++s) {
+                    const uint32_t led0 = s * B;
+                    const uint32_t b = std::min<uint32_t>(B, (uint32_t)L - led0);
+                    minibatch_backward(g, Shat, P, led, led0, b, y, pr, ws, Gs, o.stitch, mP.as<float>(), dA, rms,
+                                       nullptr, lacc, tw, st);
+                    fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+                    const bool last = (e == ne - 1) && (s == steps - 1);
+                    launch_fused_update(g, Gs, O, mO.as<float>(), rms, !last, tw, st);
+                    if (!last) {
+                        fft_cols_forward(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+                        std::swap(Shat, Gs);
+                    }
+                }
+                // cuFPM_pure_v3.cu:204-206 (float lr times double 0.5)
+                if (e > 12) lr = (float)((double)lr * 0.5);
+            }
+            if (o.loss_per_epoch)
+                FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, lossd.p, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost,
+                                       st));
+            FPM_HIP(hipStreamSynchronize(st));
+        }
+        if (!isdev) {
+            FPM_HIP(hipMemcpyAsync(O_out, O, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
+            FPM_HIP(hipMemcpyAsync(P_out, P, nn * sizeof(float2), hipMemcpyDeviceToHost, st));
+        }
+        FPM_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, const fpm_c32* P_in, int64_t n_rows,
+                       int64_t n_cols, const void* Y, int32_t y_dtype, int64_t L, const int32_t* led_idx,
+                       double pratio, fpm_c32* G_O, fpm_c32* G_P, double* loss_out, const fpm_opts* opts) {
+    return run_guarded("fpm_fused_gradient", [&] {
+        const fpm_opts o = merged_opts(opts);
+        if (!O_in || !P_in || !Y || !led_idx || !G_O || !G_P) invalid("null array argument");
+        const Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        if (L < 1) invalid("L (number of images) must be >= 1");
+        const int pr = (int)pratio;
+        if (pr < 1) invalid("pratio must be >= 1");
+        if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
+        check_device();
+        int dev = 0;
+        FPM_HIP(hipGetDevice(&dev));
+        const float2* tw = twiddle_table(dev);
+        StreamHolder sh(o.stream);
+        hipStream_t st = sh.s;
+        const bool isdev = o.pointers_are_device != 0;
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        DevBuf hO, hP, hY, hLed, pcopy, S0, S1, gp, gout, lossd;
+        const float2* O = (const float2*)upload((const fpm_c32*)O_in, NN, isdev, hO, st);
+        // P is read-only here; the back-projection takes a mutable pointer
+        pcopy.alloc(nn * sizeof(float2));
+        FPM_HIP(hipMemcpyAsync(pcopy.p, P_in, nn * sizeof(float2), isdev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               st));
+        const YStack y = prepare_y(Y, y_dtype, nn * (size_t)L, isdev, hY, st);
+        const int2* led = (const int2*)upload(led_idx, 2 * (size_t)L, isdev, hLed, st);
+        Workspace ws;
+        ws.lat.alloc(nn * L * sizeof(float2));
+        ws.sub.alloc(nn * L * sizeof(float2));
+        ws.rec.alloc(nn * L * sizeof(float2));
+        S0.alloc(NN * sizeof(float2));
+        S1.alloc(NN * sizeof(float2));
+        float2* GPd;
+        if (isdev) {
+            GPd = (float2*)G_P;
+        } else {
+            gp.alloc(nn * sizeof(float2));
+            GPd = gp.as<float2>();
+        }
+        if (loss_out) {
+            lossd.alloc(sizeof(double));
+            FPM_HIP(hipMemsetAsync(lossd.p, 0, sizeof(double), st));
+        }
+        fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st);
+        fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+        minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
+                           o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
+                           loss_out ? lossd.as<double>() : nullptr, tw, st);
+        fft_cols_inverse(S1.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+        float2* GOd;
+        if (isdev) {
+            GOd = (float2*)G_O;
+        } else {
+            gout.alloc(NN * sizeof(float2));
+            GOd = gout.as<float2>();
+        }
+        fft_contig(S1.as<float2>(), GOd, g.Nr, g.Nc, true, tw, st);
+        if (!isdev) {
+            FPM_HIP(hipMemcpyAsync(G_O, GOd, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
+            FPM_HIP(hipMemcpyAsync(G_P, GPd, nn * sizeof(float2), hipMemcpyDeviceToHost, st));
+        }
+        if (loss_out) FPM_HIP(hipMemcpyAsync(loss_out, lossd.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        FPM_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int fpm_forward_intensity(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, const fpm_c32* P_in, int64_t n_rows,
+                          int64_t n_cols, int64_t L, const int32_t* led_idx, double pratio, int32_t squared,
+                          float* out, int32_t pointers_are_device, void* stream) {
+    return run_guarded("fpm_forward_intensity", [&] {
+        if (!O_in || !P_in || !led_idx || !out) invalid("null array argument");
+        const Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        if (L < 1) invalid("L (number of images) must be >= 1");
+        const int pr = (int)pratio;
+        if (pr < 1) invalid("pratio must be >= 1");
+        check_device();
+        int dev = 0;
+        FPM_HIP(hipGetDevice(&dev));
+        const float2* tw = twiddle_table(dev);
+        StreamHolder sh(stream);
+        hipStream_t st = sh.s;
+        const bool isdev = pointers_are_device != 0;
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        DevBuf hO, hP, hLed, S0, lat, outd;
+        const float2* O = (const float2*)upload(O_in, NN, isdev, hO, st);
+        const float2* P = (const float2*)upload(P_in, nn, isdev, hP, st);
+        const int2* led = (const int2*)upload(led_idx, 2 * (size_t)L, isdev, hLed, st);
+        S0.alloc(NN * sizeof(float2));
+        const uint32_t chunk = (uint32_t)std::min<int64_t>(L, 64);
+        lat.alloc(nn * chunk * sizeof(float2));
+        float* od = out;
+        if (!isdev) {
+            outd.alloc(nn * (size_t)L * sizeof(float));
+            od = outd.as<float>();
+        }
+        fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st);
+        fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+        const float scale = (float)((double)nn * (double)pr * (double)pr);
+        for (int64_t l0 = 0; l0 < L; l0 += chunk) {
+            const uint32_t b = (uint32_t)std::min<int64_t>(chunk, L - l0);
+            launch_crop_pupil(g, S0.as<float2>(), P, led, (uint32_t)l0, b, nullptr, lat.as<float2>(), st);
+            fft2_inverse(lat.as<float2>(), g.nr, g.nc, b, g.pn, tw, st);
+            launch_abs_scaled(lat.as<float2>(), nn * b, scale, squared, od + nn * (size_t)l0, st);
+        }
+        if (!isdev) FPM_HIP(hipMemcpyAsync(out, od, nn * (size_t)L * sizeof(float), hipMemcpyDeviceToHost, st));
+        FPM_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int fpm_fft2(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols, int64_t batch, int32_t direction,
+             int32_t pointers_are_device, void* stream) {
+    return run_guarded("fpm_fft2", [&] {
+        if (!in || !out) invalid("null array argument");
+        check_side(rows, "rows");
+        check_side(cols, "cols");
+        if (batch < 1 || batch > 65535) invalid("batch must be in [1, 65535]");
+        if (direction != -1 && direction != 1) invalid("direction must be -1 (forward) or +1 (inverse)");
+        check_device();
+        int dev = 0;
+        FPM_HIP(hipGetDevice(&dev));
+        const float2* tw = twiddle_table(dev);
+        StreamHolder sh(stream);
+        hipStream_t st = sh.s;
+        const bool isdev = pointers_are_device != 0;
+        const size_t total = (size_t)rows * cols * batch;
+        const ColPerm p = make_colperm((uint32_t)cols);
+        DevBuf din, dout, tmp;
+        const float2* src = (const float2*)upload(in, total, isdev, din, st);
+        float2* dst;
+        if (isdev) {
+            dst = (float2*)out;
+        } else {
+            dout.alloc(total * sizeof(float2));
+            dst = dout.as<float2>();
+        }
+        const uint32_t R = (uint32_t)rows, C = (uint32_t)cols, Bt = (uint32_t)batch;
+        if (direction < 0) {
+            fft_contig(src, dst, R, C * Bt, false, tw, st);
+            if (p.M2 == 1) {
+                fft_cols_forward(dst, R, C, Bt, p, tw, st, nullptr);
+            } else {
+                tmp.alloc(total * sizeof(float2));
+                fft_cols_forward(dst, R, C, Bt, p, tw, st, tmp.as<float2>());
+                FPM_HIP(hipMemcpyAsync(dst, tmp.p, total * sizeof(float2), hipMemcpyDeviceToDevice, st));
+            }
+        } else {
+            if (p.M2 == 1) {
+                if (src != dst) FPM_HIP(hipMemcpyAsync(dst, src, total * sizeof(float2), hipMemcpyDeviceToDevice, st));
+                fft_cols_inverse(dst, R, C, Bt, p, tw, st, nullptr);
+            } else {
+                const float2* nat = src;
+                if (src == dst) {
+                    tmp.alloc(total * sizeof(float2));
+                    FPM_HIP(hipMemcpyAsync(tmp.p, src, total * sizeof(float2), hipMemcpyDeviceToDevice, st));
+                    nat = tmp.as<float2>();
+                }
+                fft_cols_inverse(dst, R, C, Bt, p, tw, st, nat);
+            }
+            fft_contig(dst, dst, R, C * Bt, true, tw, st);
+        }
+        if (!isdev) FPM_HIP(hipMemcpyAsync(out, dst, total * sizeof(float2), hipMemcpyDeviceToHost, st));
+        FPM_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int fpm_prepare_amplitude(const uint16_t* raw, int64_t count, float* amp_out, int32_t pointers_are_device,
+                          void* stream, uint16_t* min_out, uint16_t* max_out) {
+    return run_guarded("fpm_prepare_amplitude", [&] {
+        if (!raw || !amp_out) invalid("null array argument");
+        if (count < 1) invalid("count must be >= 1");
+        check_device();
+        StreamHolder sh(stream);
+        hipStream_t st = sh.s;
+        const bool isdev = pointers_are_device != 0;
+        DevBuf hraw, dout;
+        const YStack y = prepare_y(raw, FPM_Y_U16_RAW, (size_t)count, isdev, hraw, st);
+        float* od = amp_out;
+        if (!isdev) {
+            dout.alloc((size_t)count * sizeof(float));
+            od = dout.as<float>();
+        }
+        launch_u16_to_amp((const uint16_t*)y.data, (size_t)count, y.mn, y.range, od, st);
+        if (!isdev) FPM_HIP(hipMemcpyAsync(amp_out, od, (size_t)count * sizeof(float), hipMemcpyDeviceToHost, st));
+        FPM_HIP(hipStreamSynchronize(st));
+        if (min_out) *min_out = (uint16_t)y.mn;
+        if (max_out) *max_out = (uint16_t)(y.mn + (uint32_t)y.range);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// fpm_kernels.h -- host launchers of the FPM hot-path kernels.
+#pragma once
+#include "common.h"
+
+namespace fpm {
+
+// Geometry of one solve: object N_rows x N_cols, pupil/images n_rows x n_cols.
+struct Geom {
+    uint32_t nr, nc;   // low-res rows / cols
+    uint32_t Nr, Nc;   // high-res rows / cols
+    uint32_t lnr;      // log2 nr
+    ColPerm pn;        // column permutation of the n-side spectra
+    ColPerm pN;        // column permutation of the N-side spectrum
+    uint32_t plane() const { return nr * nc; }
+};
+
+// image stack reader: float amplitudes, or uint16 raw counts with
+// y = sqrt((I - mn) / (mx - mn)) in float
+struct YStack {
+    const void* data;
+    int32_t u16;       // 0: float amplitudes, 1: uint16 raw
+    uint32_t mn;       // raw minimum (u16 only)
+    float range;       // (float)(mx - mn) (u16 only)
+};
+
+// RMSprop + pupil constraint constants of one step
+struct RmsArgs {
+    float lr, beta, eps, clamp;
+};
+
+// crop of LED windows from the object spectrum, times the pupil
+// (cuFPM_v3/cuda/kernels.cuh:19-65): sub = S (natural), lat = ifftshift(S P)
+// in the n-side spectral layout.  led = int2 {kl, kt} array, LEDs led0..led0+b-1.
+void launch_crop_pupil(const Geom& g, const float2* Shat, const float2* P, const int2* led, uint32_t led0, uint32_t b,
+                       float2* sub, float2* lat, hipStream_t st);
+
+// feature-domain loss gradient (kernels.cuh:197-275): z (b pages, natural) and
+// images led0.. -> u.  loss_acc (device double, may be null) += sum |Dx e|+|Dy e|.
+void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio,
+                    float2* u, double* loss_acc, hipStream_t st);
+
+// back-projection (kernels.cuh:67-129) with FP32 atomics into Gs, pupil
+// gradient summed in registers over the batch in LED order.  If P_update
+// (mP, A non-null) the pupil RMSprop + clamp of kernels.cuh:384-410 is applied
+// to P in place; otherwise the pupil gradient is written to GP.
+// With do_atomics = false only the pupil part runs (owner-computes variant).
+void launch_backproj(const Geom& g, const float2* R, const float2* sub, const int2* led, uint32_t led0, uint32_t b,
+                     float2* P, float2* Gs, bool do_atomics, float* mP, const float* A, RmsArgs rms, float2* GP,
+                     hipStream_t st);
+
+// owner-computes stitch: every element of Gs written as the sum, in LED order,
+// of R_l . conj(P) / (Nr Nc) over the windows covering it (zero elsewhere).
+void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b,
+                         float2* Gs, hipStream_t st);
+
+// contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
+// [+ contiguous forward pass of the new O, written back into G]
+void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
+                         const float2* twid, hipStream_t st);
+
+// fill with zeros (count complex elements)
+void launch_zero(float2* p, size_t count, hipStream_t st);
+
+// |z| / (nr nc) (or its square) of b natural pages -> float pages
+void launch_abs_scaled(const float2* z, size_t count, float scale, int squared, float* out, hipStream_t st);
+
+// uint16 stack min/max (device) and conversion to float amplitudes
+void launch_u16_minmax(const uint16_t* raw, size_t count, uint32_t* minmax /* device [2] */, hipStream_t st);
+void launch_u16_to_amp(const uint16_t* raw, size_t count, uint32_t mn, float range, float* out, hipStream_t st);
+
+}  // namespace fpm

@@ -1,0 +1,361 @@
+// fpm_kernels.hip -- FPM hot-path kernels for gfx950 (crop x pupil, loss
+// stencil, back-projection + stitch, pupil update, helpers).
+//
+// Layouts: column-major pages; pixel (i, j) of an n-side page at j*nr + i.
+// Spectra (object spectrum, per-LED spectra) are stored column-permuted by the
+// FFT's ColPerm; the fftshift of cuFPM_v3 is folded into the index maps
+// (k + M/2) mod M, valid for even M (cuFPM_v3/cuda/kernels.cuh:45-47,62-64,92-93,109-111).
+#include "fft_core.h"
+#include "common.h"
+#include "fpm_kernels.h"
+#include <algorithm>
+
+namespace fpm {
+
+__device__ __forceinline__ float sgnf(float x) { return copysignf(1.0f, x); }
+
+// ------------------------------------------------------------ crop x pupil
+// cuFPM_v3/cuda/kernels.cuh:19-65 (fused_getSubpupil_shifted).  Window row of
+// local row i: (i + kt - 1 + Nr/2) mod Nr, evaluated in uint32 like the
+// reference (for power-of-two Nr the 2^32 wrap agrees with mod Nr).
+__global__ __launch_bounds__(256) void crop_pupil_kernel(Geom g, const float2* __restrict__ Shat,
+                                                         const float2* __restrict__ P, const int2* __restrict__ led,
+                                                         uint32_t led0, float2* __restrict__ sub,
+                                                         float2* __restrict__ lat) {
+    const uint32_t plane = g.nr * g.nc;
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= plane) return;
+    const uint32_t ell = blockIdx.y;
+    const uint32_t i = pix & (g.nr - 1), j = pix >> g.lnr;
+    const int2 k = led[led0 + ell];
+    const uint32_t srow = (i + (uint32_t)k.y - 1u + (g.Nr >> 1)) & (g.Nr - 1);
+    const uint32_t scol = (j + (uint32_t)k.x - 1u + (g.Nc >> 1)) & (g.Nc - 1);
+    const float2 S = Shat[(size_t)g.pN(scol) * g.Nr + srow];
+    const size_t page = (size_t)ell * plane;
+    if (sub) sub[page + pix] = S;
+    const float2 z = cmul(S, P[pix]);
+    const uint32_t di = (i + (g.nr >> 1)) & (g.nr - 1), dj = (j + (g.nc >> 1)) & (g.nc - 1);
+    lat[page + (size_t)g.pn(dj) * g.nr + di] = z;
+}
+
+void launch_crop_pupil(const Geom& g, const float2* Shat, const float2* P, const int2* led, uint32_t led0, uint32_t b,
+                       float2* sub, float2* lat, hipStream_t st) {
+    dim3 grid((g.plane() + 255) / 256, b);
+    hipLaunchKernelGGL(crop_pupil_kernel, grid, dim3(256), 0, st, g, Shat, P, led, led0, sub, lat);
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ loss stencil
+// cuFPM_v3/cuda/kernels.cuh:197-275 (differenceKernel_acc):
+//   e = |z|/(n_r n_c r^2) - y,  sx = sgn(e(i,j+1) - e(i,j)),  sy = sgn(e(i+1,j) - e(i,j))
+//   g = r^2 (sx(i,j-1) - sx(i,j) + sy(i-1,j) - sy(i,j)),  u = g z/|z|  (0 where |z| = 0)
+// all circular.  A workgroup owns a 64-row x 32-column tile; e of the tile and
+// a one-pixel halo is staged in LDS once, so every z and y is read once (plus
+// the halo) and the row-contiguous accesses are coalesced.
+constexpr int kStTR = 64, kStTC = 32, kStLD = kStTR + 2;
+
+__device__ __forceinline__ float read_y(const YStack& y, size_t off) {
+    if (y.u16) {
+        const uint32_t raw = ((const uint16_t*)y.data)[off];
+        return sqrtf((float)(raw - y.mn) / y.range);
+    }
+    return ((const float*)y.data)[off];
+}
+
+__global__ __launch_bounds__(256) void stencil_kernel(Geom g, const float2* __restrict__ z, YStack y, uint32_t led0,
+                                                      float ratio, float gscale, float2* __restrict__ u,
+                                                      double* __restrict__ loss_acc) {
+    __shared__ float es[(kStTC + 2) * kStLD];
+    __shared__ double red[4];
+    const int t = threadIdx.x;
+    const int r = t & 63, cg = t >> 6;
+    const uint32_t row0 = blockIdx.x * kStTR, col0 = blockIdx.y * kStTC;
+    const uint32_t ell = blockIdx.z;
+    const uint32_t plane = g.nr * g.nc;
+    const size_t page = (size_t)ell * plane;
+    const size_t ypage = (size_t)(led0 + ell) * plane;
+    const uint32_t rmask = g.nr - 1, cmask = g.nc - 1;
+
+    auto e_at = [&](uint32_t gi, uint32_t gj, float2* zout) -> float {
+        const uint32_t off = gj * g.nr + gi;
+        const float2 zz = z[page + off];
+        if (zout) *zout = zz;
+        return sqrtf(zz.x * zz.x + zz.y * zz.y) / ratio - read_y(y, ypage + off);
+    };
+
+    float2 zc[8];
+    const uint32_t gi = (row0 + r) & rmask;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int c = cg * 8 + k;
+        es[(c + 1) * kStLD + r + 1] = e_at(gi, (col0 + c) & cmask, &zc[k]);
+    }
+    if (t < 128) {  // left / right halo columns
+        const int c = t < 64 ? -1 : kStTC;
+        es[(c + 1) * kStLD + r + 1] = e_at(gi, (col0 + c) & cmask, nullptr);
+    } else if (t < 192) {  // top / bottom halo rows
+        const int q = t - 128;
+        const int hr = q < 32 ? -1 : kStTR;
+        const int c = q & 31;
+        es[(c + 1) * kStLD + hr + 1] = e_at((row0 + hr) & rmask, (col0 + c) & cmask, nullptr);
+    }
+    __syncthreads();
+
+    double loss = 0.0;
+    const bool row_ok = row0 + r < g.nr;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int c = cg * 8 + k;
+        const float* col = &es[(c + 1) * kStLD + r + 1];
+        const float e0 = col[0];
+        const float el = col[-kStLD], er = col[kStLD], eu = col[-1], ed = col[1];
+        const float gval = (sgnf(e0 - el) - sgnf(er - e0) + sgnf(e0 - eu) - sgnf(ed - e0)) * gscale;
+        const bool ok = row_ok && (col0 + c < g.nc);
+        if (ok) {
+            loss += (double)fabsf(er - e0) + (double)fabsf(ed - e0);
+            const float2 zz = zc[k];
+            const float mag = sqrtf(zz.x * zz.x + zz.y * zz.y);
+            float2 o = make_float2(0.f, 0.f);
+            if (mag > 0.0f) {
+                const float inv = 1.0f / mag;
+                o = make_float2(zz.x * inv * gval, zz.y * inv * gval);
+            }
+            u[page + (size_t)(col0 + c) * g.nr + (row0 + r)] = o;
+        }
+    }
+    if (loss_acc) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+        if ((t & 63) == 0) red[t >> 6] = loss;
+        __syncthreads();
+        if (t == 0) unsafeAtomicAdd(loss_acc, red[0] + red[1] + red[2] + red[3]);
+    }
+}
+
+void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio,
+                    float2* u, double* loss_acc, hipStream_t st) {
+    // the reference forms both scales in integer arithmetic then casts
+    const uint32_t plane = g.plane();
+    const float ratio = (float)(plane * (uint32_t)(pratio * pratio));
+    const float gscale = (float)(pratio * pratio);
+    dim3 grid((g.nr + kStTR - 1) / kStTR, (g.nc + kStTC - 1) / kStTC, b);
+    hipLaunchKernelGGL(stencil_kernel, grid, dim3(256), 0, st, g, z, y, led0, ratio, gscale, u, loss_acc);
+    FPM_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------- back-projection
+// cuFPM_v3/cuda/kernels.cuh:67-129 (fused_deconvPIE_shifted) and the pupil
+// half of constrained_RMSprop_both (kernels.cuh:384-410).  One thread per
+// pupil pixel walks the batch in LED order: the pupil gradient sum stays in a
+// register (no atomics, deterministic) and the pupil update needs no second
+// kernel, since only this thread reads P at that pixel.
+__global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __restrict__ R,
+                                                       const float2* __restrict__ sub, const int2* __restrict__ led,
+                                                       uint32_t led0, uint32_t b, float2* __restrict__ P,
+                                                       float2* __restrict__ Gs, int do_atomics, float* __restrict__ mP,
+                                                       const float* __restrict__ A, RmsArgs rms,
+                                                       float2* __restrict__ GP, float invN2) {
+    const uint32_t plane = g.nr * g.nc;
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= plane) return;
+    const uint32_t i = pix & (g.nr - 1), j = pix >> g.lnr;
+    const uint32_t di = (i + (g.nr >> 1)) & (g.nr - 1), dj = (j + (g.nc >> 1)) & (g.nc - 1);
+    const uint32_t rpos = g.pn(dj) * g.nr + di;
+    const float2 p = P[pix];
+    float2 gp = make_float2(0.f, 0.f);
+    for (uint32_t ell = 0; ell < b; ++ell) {
+        const size_t page = (size_t)ell * plane;
+        const float2 Rv = R[page + rpos];
+        if (do_atomics) {
+            const int2 k = led[led0 + ell];
+            const uint32_t srow = (i + (uint32_t)k.y - 1u + (g.Nr >> 1)) & (g.Nr - 1);
+            const uint32_t scol = (j + (uint32_t)k.x - 1u + (g.Nc >> 1)) & (g.Nc - 1);
+            const float2 gw = cmulc(Rv, p);
+            float* dst = (float*)(Gs + (size_t)g.pN(scol) * g.Nr + srow);
+            unsafeAtomicAdd(dst, gw.x * invN2);
+            unsafeAtomicAdd(dst + 1, gw.y * invN2);
+        }
+        const float2 gq = cmulc(Rv, sub[page + pix]);
+        gp.x += gq.x;
+        gp.y += gq.y;
+    }
+    if (mP) {
+        float mu = gp.x * gp.x + gp.y * gp.y;
+        mu = rms.beta * mP[pix] + (1.0f - rms.beta) * mu;
+        const float sq = rms.lr / (sqrtf(mu) + rms.eps);
+        float2 w = p;
+        w.x -= gp.x * sq;
+        w.y -= gp.y * sq;
+        const float a = A[pix];
+        const float ang = atan2f(w.y, w.x);
+        float ab = sqrtf(w.x * w.x + w.y * w.y);
+        ab = a * fmaxf(fminf(ab, a + rms.clamp), a - rms.clamp);
+        w.x = cosf(ang) * ab;
+        w.y = sinf(ang) * ab;
+        P[pix] = w;
+        mP[pix] = mu;
+    } else {
+        GP[pix] = gp;
+    }
+}
+
+void launch_backproj(const Geom& g, const float2* R, const float2* sub, const int2* led, uint32_t led0, uint32_t b,
+                     float2* P, float2* Gs, bool do_atomics, float* mP, const float* A, RmsArgs rms, float2* GP,
+                     hipStream_t st) {
+    const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
+    dim3 grid((g.plane() + 255) / 256);
+    hipLaunchKernelGGL(backproj_kernel, grid, dim3(256), 0, st, g, R, sub, led, led0, b, P, Gs, (int)do_atomics, mP, A,
+                       rms, GP, invN2);
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------- owner-computes stitch
+// Deterministic alternative to the atomics of kernels.cuh:112-116: each
+// workgroup owns a 64-row x 16-column tile of the (natural-order) spectrum,
+// finds the LED windows that meet the tile, and sums their contributions in
+// LED order.  Every element is written (zeros outside all windows), so the
+// gradient buffer needs no clearing.
+constexpr int kOwTR = 64, kOwTC = 16;
+
+__global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2* __restrict__ R,
+                                                           const float2* __restrict__ P, const int2* __restrict__ led,
+                                                           uint32_t led0, uint32_t b, float2* __restrict__ Gs,
+                                                           float invN2) {
+    __shared__ uint64_t hit[4];
+    const int t = threadIdx.x;
+    const int r = t & 63, cg = t >> 6;
+    const uint32_t ra = blockIdx.x * kOwTR, ca = blockIdx.y * kOwTC;
+    const uint32_t Rm = g.Nr - 1, Cm = g.Nc - 1;
+    const uint32_t th = g.Nr < kOwTR ? g.Nr : kOwTR, tw = g.Nc < kOwTC ? g.Nc : kOwTC;
+    const uint32_t plane = g.nr * g.nc;
+    const uint32_t row = ra + r;
+    float2 acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] = make_float2(0.f, 0.f);
+
+    for (uint32_t c0 = 0; c0 < b; c0 += 256) {
+        if (t < 4) hit[t] = 0;
+        __syncthreads();
+        const uint32_t me = c0 + t;
+        if (me < b) {
+            const int2 k = led[led0 + me];
+            const uint32_t wr = ((uint32_t)k.y - 1u + (g.Nr >> 1)) & Rm;
+            const uint32_t wc = ((uint32_t)k.x - 1u + (g.Nc >> 1)) & Cm;
+            const bool rows = ((ra - wr) & Rm) < g.nr || ((wr - ra) & Rm) < th;
+            const bool cols = ((ca - wc) & Cm) < g.nc || ((wc - ca) & Cm) < tw;
+            if (rows && cols) atomicOr((unsigned long long*)&hit[t >> 6], 1ull << (t & 63));
+        }
+        __syncthreads();
+        for (int w = 0; w < 4; ++w) {
+            uint64_t m = hit[w];
+            while (m) {
+                const int bit = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const uint32_t ell = c0 + w * 64 + bit;
+                const int2 k = led[led0 + ell];
+                const uint32_t wr = ((uint32_t)k.y - 1u + (g.Nr >> 1)) & Rm;
+                const uint32_t wc = ((uint32_t)k.x - 1u + (g.Nc >> 1)) & Cm;
+                const uint32_t li = (row - wr) & Rm;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t col = ca + cg * 4 + q;
+                    const uint32_t lj = (col - wc) & Cm;
+                    if (li < g.nr && lj < g.nc) {
+                        const uint32_t di = (li + (g.nr >> 1)) & (g.nr - 1);
+                        const uint32_t dj = (lj + (g.nc >> 1)) & (g.nc - 1);
+                        const float2 Rv = R[(size_t)ell * plane + g.pn(dj) * g.nr + di];
+                        const float2 gw = cmulc(Rv, P[lj * g.nr + li]);
+                        acc[q].x += gw.x * invN2;
+                        acc[q].y += gw.y * invN2;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (row < g.Nr) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t col = ca + cg * 4 + q;
+            if (col < g.Nc) Gs[(size_t)g.pN(col) * g.Nr + row] = acc[q];
+        }
+    }
+}
+
+void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b,
+                         float2* Gs, hipStream_t st) {
+    const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
+    dim3 grid((g.Nr + kOwTR - 1) / kOwTR, (g.Nc + kOwTC - 1) / kOwTC);
+    hipLaunchKernelGGL(owner_stitch_kernel, grid, dim3(256), 0, st, g, R, P, led, led0, b, Gs, invN2);
+    FPM_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- helpers
+__global__ __launch_bounds__(256) void zero_kernel(float4* __restrict__ p, size_t n4) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256)
+        p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+void launch_zero(float2* p, size_t count, hipStream_t st) {
+    // count complex = count/2 float4 (all buffers here hold an even count)
+    const size_t n4 = count / 2;
+    const size_t blocks = std::min<size_t>((n4 + 255) / 256, 1u << 20);
+    if (n4) hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (float4*)p, n4);
+    if (count & 1) FPM_HIP(hipMemsetAsync(p + count - 1, 0, sizeof(float2), st));
+    FPM_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void abs_scaled_kernel(const float2* __restrict__ z, size_t n, float scale,
+                                                         int squared, float* __restrict__ out) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const float2 v = z[k];
+        const float a = sqrtf(v.x * v.x + v.y * v.y) / scale;
+        out[k] = squared ? a * a : a;
+    }
+}
+
+void launch_abs_scaled(const float2* z, size_t count, float scale, int squared, float* out, hipStream_t st) {
+    const size_t blocks = std::min<size_t>((count + 255) / 256, 1u << 20);
+    hipLaunchKernelGGL(abs_scaled_kernel, dim3((unsigned)blocks), dim3(256), 0, st, z, count, scale, squared, out);
+    FPM_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void u16_minmax_kernel(const uint16_t* __restrict__ raw, size_t n,
+                                                         uint32_t* __restrict__ mm) {
+    uint32_t lo = 0xffffu, hi = 0u;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const uint32_t v = raw[k];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t l2 = __shfl_down(lo, off, 64), h2 = __shfl_down(hi, off, 64);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&mm[0], lo);
+        atomicMax(&mm[1], hi);
+    }
+}
+
+void launch_u16_minmax(const uint16_t* raw, size_t count, uint32_t* minmax, hipStream_t st) {
+    const size_t blocks = std::min<size_t>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(u16_minmax_kernel, dim3((unsigned)blocks), dim3(256), 0, st, raw, count, minmax);
+    FPM_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void u16_to_amp_kernel(const uint16_t* __restrict__ raw, size_t n, uint32_t mn,
+                                                         float range, float* __restrict__ out) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+        out[k] = sqrtf((float)((uint32_t)raw[k] - mn) / range);
+}
+
+void launch_u16_to_amp(const uint16_t* raw, size_t count, uint32_t mn, float range, float* out, hipStream_t st) {
+    const size_t blocks = std::min<size_t>((count + 255) / 256, 1u << 20);
+    hipLaunchKernelGGL(u16_to_amp_kernel, dim3((unsigned)blocks), dim3(256), 0, st, raw, count, mn, range, out);
+    FPM_LAUNCH_CHECK();
+}
+
+}  // namespace fpm

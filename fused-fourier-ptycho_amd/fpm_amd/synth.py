@@ -1,0 +1,163 @@
+"""Seeded synthetic FPM datasets (the caller side of the solve).
+
+The reference ships no data and no generator for this setting (its datasets are
+external, README.md:17,23; benchmark/gen_img_batch.m writes amplitudes of a
+ring-LED model).  This generator produces the workload the solve is measured
+on:
+
+* optics: wavelength 0.532 um, NA 0.1, effective pixel 1.625 um, 19 x 19 LEDs
+  at 4 mm pitch, 90 mm high, centre-out spiral order; LED windows from
+  :func:`fpm_amd.led_offsets` (FP64, init_environment_rgb.m:33-134);
+* pupil mask A: the NA disk on the low-res frequency grid
+  (init_environment_rgb.m:33-40); true pupil A exp(i 0.5 sqrt3 (2 rho^2 - 1));
+* object (N x N): amplitude = uniform noise low-passed by a disk at
+  NA 0.1 + max illumination NA, scaled to [0.3, 1]; phase = sum of 20
+  Gaussians (centres uniform, sigma in [N/64, N/16], heights in [-pi/2, pi/2]);
+* images: I = amp^2 of the solver's image model (fpm_forward_intensity),
+  scaled to a brightfield peak of 40000, Poisson noise, + 100, clipped to
+  uint16;
+* init: O0 = bilinear upsampling of the prepared centre amplitude, P0 = A.
+
+Two backends: ``numpy`` (any size the host handles; the image model is
+evaluated with numpy's FFT) and ``device`` (torch tensors on the GPU, the
+image model and transforms through libfpm_amd) for the large workloads.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import led_offsets
+
+WAVELENGTH_UM = 0.532
+NA = 0.1
+PIXEL_UM = 1.625
+LED_PITCH_UM = 4000.0
+LED_HEIGHT_UM = 90000.0
+LED_GRID = 19
+PEAK_COUNTS = 40000.0
+OFFSET_COUNTS = 100.0
+
+
+@dataclass
+class Dataset:
+    n: int
+    pratio: int
+    led: np.ndarray          # int32 2 x L {kl; kt}
+    A: np.ndarray            # n x n float32 pupil mask
+    P_true: np.ndarray       # n x n complex64
+    O_true: Optional[np.ndarray]  # N x N complex64 (numpy backend only)
+    raw: object              # n x n x L uint16 (numpy array or device tensor view)
+    amp: object              # n x n x L float32 prepared amplitudes
+    O0: object               # N x N complex64 initial object
+    P0: np.ndarray           # n x n complex64 initial pupil
+    na_illum: np.ndarray
+
+
+def freq_grid(n: int, pixel_um: float = PIXEL_UM):
+    """Centred angular-frequency grid of an n x n image (init_environment_rgb.m:33-35)."""
+    f = np.arange(-(n // 2), n // 2, dtype=np.float64) / (n * pixel_um) * (2 * np.pi)
+    Fx, Fy = np.meshgrid(f, f)
+    return Fx, Fy
+
+
+def pupil(n: int, defocus_rad: float = 0.5):
+    """(A, P_true): NA disk (init_environment_rgb.m:39-40) and a defocused pupil."""
+    Fx, Fy = freq_grid(n)
+    kmax = NA * 2 * np.pi / WAVELENGTH_UM
+    r2 = Fx * Fx + Fy * Fy
+    A = (r2 <= kmax * kmax).astype(np.float32)
+    rho2 = r2 / (kmax * kmax)
+    P = A * np.exp(1j * defocus_rad * np.sqrt(3.0) * (2 * rho2 - 1))
+    # the solve's pupil is unshifted-centred like the reference's (centre = n/2+1)
+    return np.asfortranarray(A), np.asfortranarray(P.astype(np.complex64))
+
+
+def bilinear_upsample(img: np.ndarray, r: int) -> np.ndarray:
+    """Bilinear upsampling by an integer factor, pixel-centre aligned
+    (x_in = (x_out + 0.5)/r - 0.5, clamped at the borders)."""
+    n0, n1 = img.shape
+
+    def axis(n):
+        x = (np.arange(n * r) + 0.5) / r - 0.5
+        x = np.clip(x, 0, n - 1)
+        i0 = np.floor(x).astype(np.int64)
+        i1 = np.minimum(i0 + 1, n - 1)
+        w = x - i0
+        return i0, i1, w
+
+    r0, r1, wr = axis(n0)
+    c0, c1, wc = axis(n1)
+    a = img[np.ix_(r0, c0)] * (1 - wr)[:, None] * (1 - wc)[None, :]
+    a += img[np.ix_(r1, c0)] * wr[:, None] * (1 - wc)[None, :]
+    a += img[np.ix_(r0, c1)] * (1 - wr)[:, None] * wc[None, :]
+    a += img[np.ix_(r1, c1)] * wr[:, None] * wc[None, :]
+    return a
+
+
+def object_numpy(N: int, pratio: int, na_illum_max: float, seed: int) -> np.ndarray:
+    """Seeded synthetic object (N x N complex128, Fortran order)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    noise = rng.random((N, N))
+    # band limit: disk of radius (NA + NA_illum_max) k0 on the high-res grid,
+    # whose pixel is PIXEL_UM / pratio
+    k0 = 2 * np.pi / WAVELENGTH_UM
+    f = np.fft.fftfreq(N, d=PIXEL_UM / pratio) * 2 * np.pi
+    FX, FY = np.meshgrid(f, f)
+    cut = (NA + na_illum_max) * k0
+    spec = np.fft.fft2(noise) * ((FX * FX + FY * FY) <= cut * cut)
+    amp = np.real(np.fft.ifft2(spec))
+    amp = 0.3 + 0.7 * (amp - amp.min()) / (amp.max() - amp.min())
+    yy, xx = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    phase = np.zeros((N, N))
+    for _ in range(20):
+        cy, cx = rng.random(2) * N
+        sig = N / 64 + rng.random() * (N / 16 - N / 64)
+        h = (rng.random() * 2 - 1) * np.pi / 2
+        phase += h * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sig * sig))
+    return np.asfortranarray(amp * np.exp(1j * phase))
+
+
+def image_model_numpy(O: np.ndarray, P: np.ndarray, led: np.ndarray, pratio: int) -> np.ndarray:
+    """|IDFT_n(ifftshift(crop(DFT_N O) . P))| / (n^2 r^2) for every LED (float64),
+    the amplitude the solve compares with Y (cuFPM_v3/cuda/kernels.cuh:19-65,223)."""
+    Nr, Nc = O.shape
+    nr, nc = P.shape
+    S = np.fft.fft2(O)
+    out = np.empty((nr, nc, led.shape[1]))
+    ii = np.arange(nr)[:, None]
+    jj = np.arange(nc)[None, :]
+    for l in range(led.shape[1]):
+        kl, kt = int(led[0, l]), int(led[1, l])
+        rows = (ii + kt - 1 + Nr // 2) % Nr
+        cols = (jj + kl - 1 + Nc // 2) % Nc
+        sub = S[rows, cols] * P
+        z = np.fft.ifft2(np.fft.ifftshift(sub)) * (nr * nc)
+        out[:, :, l] = np.abs(z) / (nr * nc * pratio * pratio)
+    return out
+
+
+def make_numpy(n: int, pratio: int, seed: int = 20261203, led_grid: int = LED_GRID, defocus_rad: float = 0.5,
+               noise: bool = True) -> Dataset:
+    """Whole dataset on the host (small and medium sizes)."""
+    N = n * pratio
+    led, _, nai = led_offsets(n, pratio, wavelength_um=WAVELENGTH_UM, na=NA, pixel_um=PIXEL_UM,
+                              led_pitch_um=LED_PITCH_UM, led_height_um=LED_HEIGHT_UM, led_grid=led_grid)
+    A, P_true = pupil(n, defocus_rad)
+    O = object_numpy(N, pratio, float(nai.max()), seed)
+    amp = image_model_numpy(O, P_true, led, pratio)
+    I = amp * amp
+    I = I * (PEAK_COUNTS / I[:, :, 0].max())
+    rng = np.random.Generator(np.random.Philox(seed + 1))
+    if noise:
+        I = rng.poisson(I).astype(np.float64)
+    raw = np.clip(np.rint(I + OFFSET_COUNTS), 0, 65535).astype(np.uint16)
+    raw = np.asfortranarray(raw)
+    Y = raw.astype(np.float32) - np.float32(raw.min())
+    Y = np.sqrt(Y / np.float32(Y.max())).astype(np.float32)
+    O0 = bilinear_upsample(Y[:, :, 0].astype(np.float64), pratio).astype(np.complex64)
+    return Dataset(n=n, pratio=pratio, led=led, A=A, P_true=P_true, O_true=O.astype(np.complex64), raw=raw,
+                   amp=np.asfortranarray(Y), O0=np.asfortranarray(O0), P0=np.asfortranarray(A.astype(np.complex64)),
+                   na_illum=nai)

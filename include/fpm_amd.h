@@ -1,0 +1,161 @@
+/*
+ * fpm_amd.h -- C-ABI of the MI355X (gfx950) fused Fourier ptychographic
+ * microscopy (FPM) reconstruction library, libfpm_amd.so.
+ *
+ * Every array is column-major exactly as MATLAB lays it out: element
+ * (row i, column j) of an R x C array sits at j*R + i.  Complex values are
+ * interleaved {re, im} float pairs.  All entry points return 0 on success or
+ * a negative FPM_E* code; fpm_last_error() then returns a thread-local text.
+ * No entry point mutates its inputs.  Outputs are caller-allocated.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository
+ * THUHoloLab/Fused-Fourier-Ptycho):
+ *   fpm_pure_v3          <- mexFunction of cuFPM_v3/cuFPM_pure_v3.cu:44-228
+ *                           (argument list prhs[0..8], outputs plhs[0..1])
+ *   fpm_fused_gradient   <- mexFunction of benchmark/cuFPM/fullyfusedFPM.cu:21-162
+ *                           (w1, w2, Y, ledIdx, pratio) -> (dldw1, dldw2), with
+ *                           the v3 loss of cuFPM_v3/cuda/kernels.cuh:197-275
+ *   fpm_led_offsets      <- codes_v2/func_ware/init_environment_rgb.m:33-134 and
+ *                           codes_v2/main_run_FPM.m:23,27-31,53-55
+ *   fpm_prepare_amplitude<- codes_v2/main_run_FPM.m:40-45
+ *   fpm_forward_intensity<- the forward half of benchmark/func_ware/fpm_forward.m:25-35
+ *                           (|IFFT(crop(FFT O) . P)|, the image model)
+ *   fpm_fft2             <- the cufftExecC2C call sites of cuFPM_v3/cuFPM_pure_v3.cu:118-187
+ *                           (unnormalised 2D C2C transforms)
+ */
+#ifndef FPM_AMD_H
+#define FPM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FPM_AMD_ABI_VERSION 1
+
+/* status codes */
+#define FPM_OK              0
+#define FPM_E_INVALID      -1  /* bad argument (null pointer, bad shape, batch <= 0 ...) */
+#define FPM_E_UNSUPPORTED  -2  /* shape outside what the kernels handle (see below) */
+#define FPM_E_HIP          -3  /* a HIP runtime call failed (text says which) */
+#define FPM_E_NOMEM        -4  /* device allocation failed */
+#define FPM_E_NODEVICE     -5  /* no usable gfx950 device */
+
+/* shape rules: every image side (rows and columns of the object, pupil and
+ * images) is a power of two in [16, 16384]; the object is at least as large
+ * as the pupil in each dimension.  The reference accepts any even side
+ * (cuFFT); sides that are not powers of two return FPM_E_UNSUPPORTED. */
+
+typedef struct { float re, im; } fpm_c32;
+
+/* type of the image stack Y */
+#define FPM_Y_F32_AMPLITUDE 0  /* n x n x L single amplitudes: exactly what v3 takes */
+#define FPM_Y_U16_RAW       1  /* n x n x L uint16 raw counts; the library applies
+                                  Y = sqrt((I - min)/(max - min)) in float, as
+                                  codes_v2/main_run_FPM.m:43-45 does in single */
+
+/* how the object-spectrum gradient is stitched from the per-LED back-projections */
+#define FPM_STITCH_ATOMIC 0  /* FP32 atomics into the spectrum (v3 kernels.cuh:115-116) */
+#define FPM_STITCH_OWNER  1  /* owner-computes gather in fixed LED order: deterministic */
+
+typedef struct {
+    int32_t struct_size;          /* sizeof(fpm_opts); lets the ABI grow */
+    int32_t stitch;               /* FPM_STITCH_*; default ATOMIC */
+    int32_t pointers_are_device;  /* 1: every array argument is a device pointer */
+    int32_t verbose;              /* 1: print the two parameter lines v3 prints */
+    void*   stream;               /* hipStream_t to run on; NULL = library stream */
+    double* loss_per_epoch;       /* optional host array [epochs]: FP64 sum over the
+                                     epoch's minibatches of sum|Dx e| + |Dy e| */
+    float   pupil_clamp;          /* pupil amplitude band (v3 hard-codes 0.2);
+                                     <= 0 selects 0.2 */
+    int32_t reserved[7];
+} fpm_opts;
+
+/* Fills *o with the v3 defaults (atomic stitch, host pointers, silent,
+ * library stream, no loss, clamp 0.2). */
+void fpm_opts_init(fpm_opts* o);
+
+/* The drop-in solve: [O, P] = cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio,
+ * numEpochs, batch, [lr beta eps]).
+ *   O0      N_rows x N_cols complex single, SPATIAL-domain object
+ *   P0      n_rows x n_cols complex single pupil
+ *   Y       n_rows x n_cols x L images (y_dtype says float amplitude or uint16 raw)
+ *   led_idx int32 2 x L, column l = {kl (1-based column start), kt (1-based row
+ *           start)} of LED l's window in the centred object spectrum
+ *   A       n_rows x n_cols single pupil mask
+ *   pratio, epochs, batch: doubles truncated to int as the mex does
+ *   opt     {learning rate, beta, eps}, cast to float
+ *   O_out, P_out: results (may alias O0 / P0)
+ * opts may be NULL (v3 semantics, host pointers). */
+int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols,
+                const fpm_c32* P0, int64_t n_rows, int64_t n_cols,
+                const void* Y, int32_t y_dtype, int64_t L,
+                const int32_t* led_idx, const float* A,
+                double pratio, double epochs, double batch, const double opt[3],
+                fpm_c32* O_out, fpm_c32* P_out, const fpm_opts* opts);
+
+/* One fused gradient over all L images (one minibatch):
+ *   G_O = IDFT_N(scatter(R . conj(P) / (N_rows N_cols))),  G_P = sum_l R_l . conj(S_l)
+ * where R_l = fftshift(DFT_n(u_l)) and u_l is the feature-domain loss
+ * gradient of v3 (kernels.cuh:197-275).  Equals r^2 grad_O Loss and
+ * n^2 r^4 grad_P Loss for N = n r.  loss_out (optional, host) receives the
+ * FP64 loss sum|Dx e| + |Dy e| at (O, P). */
+int fpm_fused_gradient(const fpm_c32* O, int64_t N_rows, int64_t N_cols,
+                       const fpm_c32* P, int64_t n_rows, int64_t n_cols,
+                       const void* Y, int32_t y_dtype, int64_t L,
+                       const int32_t* led_idx, double pratio,
+                       fpm_c32* G_O, fpm_c32* G_P, double* loss_out,
+                       const fpm_opts* opts);
+
+/* Optics set-up: LED illumination wave-vectors -> integer window offsets,
+ * computed in FP64 (init_environment_rgb.m:33-134, main_run_FPM.m:23,53-55).
+ *   wavelength_um, na, pixel_um (camera pixel / magnification, in um),
+ *   led_pitch_um, led_height_um, led_grid (odd, e.g. 19), n (image side),
+ *   pratio (upsampling) ; spiral = 1 orders LEDs in the centre-out spiral of
+ *   init_environment_rgb.m:110-134, 0 keeps row-major grid order.
+ * Outputs: led_idx (int32 2 x L, {kl, kt}), grid_index (int32 L, 1-based grid
+ * number k(i,j) = j + (i-1)*led_grid of each entry, may be NULL),
+ * na_illum (double L, may be NULL).  L = led_grid^2.
+ * Returns FPM_E_INVALID if an argmin is not unique (a tie). */
+int fpm_led_offsets(double wavelength_um, double na, double pixel_um,
+                    double led_pitch_um, double led_height_um,
+                    int32_t led_grid, int64_t n, int32_t pratio, int32_t spiral,
+                    int32_t* led_idx, int32_t* grid_index, double* na_illum);
+
+/* uint16 raw stack -> float amplitudes sqrt((I - min)/(max - min)) with min and
+ * max taken over the whole stack (main_run_FPM.m:43-45).  count = n*n*L.
+ * Pointers are host or device per pointers_are_device; min/max outputs may be NULL. */
+int fpm_prepare_amplitude(const uint16_t* raw, int64_t count, float* amp_out,
+                          int32_t pointers_are_device, void* stream,
+                          uint16_t* min_out, uint16_t* max_out);
+
+/* Image model of the solver: amp_l = |IDFT_n(ifftshift(crop_l(DFT_N(O)) . P))|
+ * / (n_rows n_cols r^2) with r = (int)pratio -- the amplitude v3 compares with
+ * Y (kernels.cuh:223,233) -- for l in [0, L), written as n_rows x n_cols x L
+ * float.  With squared = 1 the intensities amp^2 are written instead.  The
+ * crop follows the same led_idx convention as fpm_pure_v3. */
+int fpm_forward_intensity(const fpm_c32* O, int64_t N_rows, int64_t N_cols,
+                          const fpm_c32* P, int64_t n_rows, int64_t n_cols,
+                          int64_t L, const int32_t* led_idx, double pratio,
+                          int32_t squared, float* out,
+                          int32_t pointers_are_device, void* stream);
+
+/* Unnormalised 2D C2C transform of `batch` column-major rows x cols pages,
+ * out-of-place or in place (out == in), natural order in and out.
+ * direction -1 = forward (exp(-2 pi i ...)), +1 = inverse (no 1/N scaling). */
+int fpm_fft2(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
+             int64_t batch, int32_t direction, int32_t pointers_are_device,
+             void* stream);
+
+/* Thread-local text of the last error ("" if none). */
+const char* fpm_last_error(void);
+
+/* ABI version (FPM_AMD_ABI_VERSION). */
+int32_t fpm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FPM_AMD_H */

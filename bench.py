@@ -1,0 +1,216 @@
+"""Benchmark of the fused FPM solve on one MI355X.
+
+Workload: the reference's headline case -- 361 uint16 images of 2048 x 2048,
+upsampling 8 -> a 16384 x 16384 complex64 object, batch 26, FP32 -- on
+synthetic data (fpm_amd.synth.make_device; the reference's data are external).
+A "step" is one epoch (one pass over all 361 images: 14 minibatch steps).
+The timed region is one fpm_pure_v3 call of K epochs on device-resident
+inputs, bracketed by device synchronisation, exactly as the reference's
+mex call is timed around cuFPM_pure_v3 (README.md:36 quotes the whole call).
+
+Also reported:
+  stages     per-stage device time (HIP events on the solve's stream, inside
+             the timed region) and GB/s against the dense algorithmic byte
+             model (per step 104 N^2, per LED 138 n^2 with uint16 Y);
+  roofline   the dominant stage against MI355X's 8 TB/s HBM peak;
+  cpu_baseline the FP32 CPU restatement (oracle) on the host cores, one
+             minibatch step of the same workload scaled to s/epoch.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fused-fourier-ptycho_amd"))
+
+METRIC = "s/epoch 2048²×361→16384²; HBM GB/s vs peak; time-to-loss vs plain @rel-L2 1e-3"
+HBM_PEAK_GBPS = 8000.0
+STAGE_NAMES = ["spectrum_init", "crop", "ifft_n_cols", "ifft_n_contig", "stencil", "fft_n_contig", "fft_n_cols",
+               "zero", "stitch_owner", "backproj", "ifft_N_cols", "fused_update", "fft_N_cols"]
+
+
+def stage_alg_bytes(n, N, b_avg, y_bytes, stitch):
+    """Dense algorithmic bytes of ONE execution of each stage.
+
+    Large-array terms per step (c64 = 8 B, f32 = 4 B): DFT(O) 32 N^2 and
+    IDFT(G) 32 N^2 as two passes of (8 read + 8 write), RMSprop with gradient
+    zeroing 40 N^2 (read G, O, m; write O, m, G=0).  Per-LED terms: crop 24 n^2
+    (read S, write sub + latent), each batched transform 32 n^2, stencil
+    16 n^2 + Y, back-projection 32 n^2 (read R, sub; read-modify-write G)."""
+    NN, nn = float(N) * N, float(n) * n
+    lo = b_avg * nn
+    d = {
+        "spectrum_init": 32 * NN,
+        "crop": 24 * lo,
+        "ifft_n_cols": 16 * lo,
+        "ifft_n_contig": 16 * lo,
+        "stencil": (16 + y_bytes) * lo,
+        "fft_n_contig": 16 * lo,
+        "fft_n_cols": 16 * lo,
+        "zero": 8 * NN,
+        # owner-computes: reads R once, writes every spectrum element (the
+        # clearing it replaces); the back-projection then reads R and sub
+        "stitch_owner": 8 * lo + 8 * NN,
+        "backproj": (32 if stitch == "atomic" else 16) * lo,
+        "ifft_N_cols": 16 * NN,
+        # contiguous IDFT pass 16 + RMSprop without the zeroing 32 + contiguous DFT pass 16
+        "fused_update": 64 * NN,
+        "fft_N_cols": 16 * NN,
+    }
+    return d
+
+
+def run_cpu_baseline(ds, args):
+    """FP32 oracle (OpenMP) on one minibatch step of the same workload."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    n, N, B = ds["n"], ds["N"], args.batch
+    raw = ds["raw_dev"][:B].cpu().numpy()  # (B, n, n) row-major = column-major pages
+    mn, mx = ds["raw_min"], ds["raw_max"]
+    Y = np.sqrt((raw.astype(np.float32) - np.float32(mn)) / np.float32(mx - mn)).astype(np.float32)
+    Y = np.asfortranarray(Y.transpose(2, 1, 0))
+    O0 = np.asfortranarray(ds["O0_dev"].cpu().numpy().T)
+    P0 = np.asfortranarray(ds["P0_dev"].cpu().numpy().T)
+    led = np.asfortranarray(ds["led"][:, :B])
+    A = np.asfortranarray(ds["A"])
+    oracle.lib()
+    t0 = time.perf_counter()
+    oracle.solve(O0, P0, Y, led, A, ds["pratio"], 1, B, (args.lr, 0.999, 1e-4), prec="f32", max_steps=1)
+    dt = time.perf_counter() - t0
+    steps = (ds["L"] + B - 1) // B
+    return {"value": dt * steps, "unit": "s/epoch", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"one minibatch step ({B} of {ds['L']} images, FP32 C restatement, OpenMP) of the "
+                      f"{n}^2 x {ds['L']} -> {N}^2 workload, timed {dt:.2f} s, scaled x{steps} steps/epoch"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="epochs timed (one call)")
+    ap.add_argument("--warmup", type=int, default=1, help="epochs of the untimed warm-up call")
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--pratio", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=26)
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--stitch", choices=["atomic", "owner"], default="atomic")
+    ap.add_argument("--y", choices=["u16", "f32"], default="u16")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stages", action="store_true", help="time without per-stage events")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for rocprof FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}),
+              flush=True)
+        return
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs on one GPU: the solve is single-device (the reference "
+                                   "is single-GPU too); --gpus above 1 is not supported"}), flush=True)
+        sys.exit(2)
+
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import fpm_amd
+    from fpm_amd import synth
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t_gen = time.perf_counter()
+    ds = synth.make_device(args.n, args.pratio, device="cuda:0")
+    print(f"[bench] synthetic data ready in {time.perf_counter() - t_gen:.1f} s "
+          f"(n={ds['n']} N={ds['N']} L={ds['L']})", file=sys.stderr, flush=True)
+    n, N, L = ds["n"], ds["N"], ds["L"]
+    if args.y == "u16":
+        Yd, y_u16 = ds["raw_dev"], True
+    else:
+        Yd = torch.sqrt((ds["raw_dev"].to(torch.float32) - ds["raw_min"]) / float(ds["raw_max"] - ds["raw_min"]))
+        y_u16 = False
+    Od = torch.empty_like(ds["O0_dev"])
+    Pd = torch.empty_like(ds["P0_dev"])
+    stream = torch.cuda.current_stream(dev)
+    opt = (args.lr, 0.999, 1e-4)
+    stitch = fpm_amd.STITCH_ATOMIC if args.stitch == "atomic" else fpm_amd.STITCH_OWNER
+
+    def solve(epochs, stage_times=None):
+        optv = (ctypes.c_double * 3)(*opt)
+        o = fpm_amd._opts(stitch=stitch, device=True, stream=stream.cuda_stream)
+        if stage_times is not None:
+            o.stage_times = ctypes.pointer(stage_times)
+        fpm_amd._check(fpm_amd.lib().fpm_pure_v3(
+            ds["O0_dev"].data_ptr(), N, N, ds["P0_dev"].data_ptr(), n, n, Yd.data_ptr(),
+            fpm_amd.Y_U16_RAW if y_u16 else fpm_amd.Y_F32_AMPLITUDE, L, ds["led_dev"].data_ptr(),
+            ds["A_dev"].data_ptr(), float(args.pratio), float(epochs), float(args.batch), optv,
+            Od.data_ptr(), Pd.data_ptr(), ctypes.byref(o)))
+
+    if args.warmup > 0:
+        solve(args.warmup)
+        torch.cuda.synchronize()
+    st = None if args.no_stages else fpm_amd.StageTimes()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solve(args.steps, st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sec_per_epoch = (t1 - t0) / args.steps
+    print(f"[bench] {args.steps} epochs in {t1 - t0:.3f} s", file=sys.stderr, flush=True)
+    assert torch.isfinite(torch.view_as_real(Od)).all().item(), "non-finite object"
+
+    steps_per_epoch = (L + args.batch - 1) // args.batch
+    b_avg = L / steps_per_epoch
+    y_bytes = 2 if y_u16 else 4
+    epoch_bytes = steps_per_epoch * 104.0 * N * N + L * (136.0 + y_bytes) * n * n
+    out = {
+        "metric": METRIC, "value": sec_per_epoch, "unit": "s/epoch", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sec_per_epoch * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"FPM solve, {L} uint16 images {n}x{n} -> {N}x{N} complex64 object, "
+                               f"batch {args.batch}, {args.stitch} stitch, {args.steps} epochs in one call",
+                   "n": n, "N": N, "L": L, "batch": args.batch, "y": args.y, "stitch": args.stitch,
+                   "solve_wall_s": t1 - t0,
+                   "reference_wall_s_rtx3090_full_solve": 22.31},
+        "epoch_model_GB": epoch_bytes / 1e9,
+        "epoch_model_GBps": epoch_bytes / 1e9 / sec_per_epoch,
+    }
+    if st is not None:
+        alg = stage_alg_bytes(n, N, b_avg, y_bytes, args.stitch)
+        stages = {}
+        for k, name in enumerate(STAGE_NAMES):
+            cnt = int(st.count[k])
+            if cnt == 0:
+                continue
+            ms_exec = st.ms[k] / cnt
+            stages[name] = {"ms_total": round(st.ms[k], 4), "execs": cnt, "launches": int(st.launches[k]),
+                            "ms_per_exec": round(ms_exec, 5),
+                            "alg_GBps": round(alg[name] / 1e9 / (ms_exec / 1e3), 1)}
+        out["stages"] = stages
+        dom = max(stages, key=lambda s: stages[s]["ms_total"])
+        ach = stages[dom]["alg_GBps"]
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                           "alg_bytes_per_exec": alg[dom], "ms_per_exec": stages[dom]["ms_per_exec"]}
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle, one minibatch step)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = run_cpu_baseline(ds, args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

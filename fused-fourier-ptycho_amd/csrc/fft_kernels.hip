@@ -72,20 +72,24 @@ struct StridedArgs {
     uint32_t tw_step;
 };
 
-template <int LEN> struct StridedCfg {
-    static constexpr int T = LEN;  // 16 rows x LEN points / 16 per thread
-    using Map = LineFastMap<kStridedRows, LEN>;
+// TR adjacent rows per tile: 16 (128 B per column) for long lines, more for
+// short ones so that every workgroup has LEN*TR/16 >= 256 threads.
+constexpr int strided_rows_pref(int len) { return len >= 256 ? 16 : (4096 / len > 128 ? 128 : 4096 / len); }
+
+template <int LEN, int TR> struct StridedCfg {
+    static constexpr int T = LEN * TR / 16;  // TR rows x LEN points / 16 per thread
+    using Map = LineFastMap<TR, LEN>;
 };
 
-template <int LEN, bool INV, bool TWID>
-__global__ __launch_bounds__(StridedCfg<LEN>::T) void fft_strided_kernel(StridedArgs a, Twiddles tw) {
-    using Cfg = StridedCfg<LEN>;
+template <int LEN, int TR, bool INV, bool TWID>
+__global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs a, Twiddles tw) {
+    using Cfg = StridedCfg<LEN, TR>;
     using Map = typename Cfg::Map;
     using SL = Slots<LEN, INV, Cfg::T, Map>;
     __shared__ float2 lds[Map::LDS_WORDS];
     const int t = threadIdx.x;
     const uint32_t g = blockIdx.y;
-    const size_t base = (size_t)blockIdx.z * a.page + (size_t)blockIdx.x * kStridedRows;
+    const size_t base = (size_t)blockIdx.z * a.page + (size_t)blockIdx.x * TR;
     const float2* __restrict__ in = a.in + base;
     float2* __restrict__ out = a.out + base;
     float2 v[16];
@@ -140,11 +144,24 @@ static void launch_contig_len(int len, const float2* in, float2* out, uint32_t n
     FPM_LAUNCH_CHECK();
 }
 
+// grid.x is given in 16-row tiles; a wide-tile kernel covers TR/16 of them
+template <int L, bool INV, bool TWID>
+static void launch_strided_tr(const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st) {
+    constexpr int TRP = strided_rows_pref(L);
+    const uint32_t rows = grid.x * kStridedRows;
+    if (TRP > 16 && rows % TRP == 0) {
+        dim3 g2(rows / TRP, grid.y, grid.z);
+        hipLaunchKernelGGL((fft_strided_kernel<L, TRP, INV, TWID>), g2, dim3(StridedCfg<L, TRP>::T), 0, st, a, tw);
+    } else {
+        hipLaunchKernelGGL((fft_strided_kernel<L, 16, INV, TWID>), grid, dim3(StridedCfg<L, 16>::T), 0, st, a, tw);
+    }
+}
+
 template <bool INV, bool TWID>
 static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st) {
-#define FPM_STRIDED_CASE(L)                                                                             \
-    case L:                                                                                             \
-        hipLaunchKernelGGL((fft_strided_kernel<L, INV, TWID>), grid, dim3(StridedCfg<L>::T), 0, st, a, tw); \
+#define FPM_STRIDED_CASE(L)                              \
+    case L:                                              \
+        launch_strided_tr<L, INV, TWID>(a, grid, tw, st); \
         break;
     switch (len) {
         FPM_STRIDED_CASE(16)

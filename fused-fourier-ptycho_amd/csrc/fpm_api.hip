@@ -198,6 +198,71 @@ int run_guarded(const char* fn, const std::function<void()>& body) {
     }
 }
 
+// Optional per-stage timing with events recorded on the call's stream.
+struct StageTimer {
+    fpm_stage_times* out = nullptr;
+    hipStream_t st = nullptr;
+    struct Rec {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    int open_stage = -1;
+    hipEvent_t open_ev = nullptr;
+    StageTimer(fpm_stage_times* o, hipStream_t s) : out(o), st(s) {
+        if (out) std::memset(out, 0, sizeof(*out));
+    }
+    ~StageTimer() {
+        for (auto& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t ev() {
+        hipEvent_t e;
+        if (!pool.empty()) {
+            e = pool.back();
+            pool.pop_back();
+        } else {
+            FPM_HIP(hipEventCreate(&e));
+        }
+        return e;
+    }
+    void begin(int stage) {
+        if (!out) return;
+        open_stage = stage;
+        open_ev = ev();
+        FPM_HIP(hipEventRecord(open_ev, st));
+    }
+    void end(int launches) {
+        if (!out) return;
+        hipEvent_t e = ev();
+        FPM_HIP(hipEventRecord(e, st));
+        recs.push_back({open_stage, open_ev, e});
+        out->launches[open_stage] = launches;
+        // bound the number of live events: fold finished records in
+        if (recs.size() >= 512) flush();
+    }
+    void flush() {
+        if (!out) return;
+        FPM_HIP(hipStreamSynchronize(st));
+        for (auto& r : recs) {
+            float ms = 0.f;
+            FPM_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            out->ms[r.stage] += ms;
+            out->count[r.stage] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+};
+
+// kernel launches of the column passes of a C-column transform
+inline int col_launches(const ColPerm& p) { return p.M2 == 1 ? 1 : 2; }
+
 // Per-minibatch gradient pipeline shared by the solve and the gradient entry.
 struct Workspace {
     DevBuf lat, sub, rec;
@@ -207,20 +272,43 @@ struct Workspace {
 // the pupil part (update or gradient output)
 void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2* led, uint32_t led0, uint32_t b,
                         const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
-                        RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st) {
+                        RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
+                        StageTimer& tm) {
     float2* lat = ws.lat.as<float2>();
     float2* sub = ws.sub.as<float2>();
     float2* rec = ws.rec.as<float2>();
+    tm.begin(FPM_STAGE_CROP);
     launch_crop_pupil(g, Shat, P, led, led0, b, sub, lat, st);
-    fft2_inverse(lat, g.nr, g.nc, b, g.pn, tw, st);
+    tm.end(1);
+    tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
+    fft_cols_inverse(lat, g.nr, g.nc, b, g.pn, tw, st, nullptr);
+    tm.end(col_launches(g.pn));
+    tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
+    fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st);
+    tm.end(1);
+    tm.begin(FPM_STAGE_STENCIL);
     launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st);
-    fft2_forward(rec, g.nr, g.nc, b, g.pn, tw, st);
+    tm.end(1);
+    tm.begin(FPM_STAGE_FFT_N_CONTIG_LO);
+    fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st);
+    tm.end(1);
+    tm.begin(FPM_STAGE_FFT_N_COLS_LO);
+    fft_cols_forward(rec, g.nr, g.nc, b, g.pn, tw, st, nullptr);
+    tm.end(col_launches(g.pn));
     if (stitch == FPM_STITCH_OWNER) {
+        tm.begin(FPM_STAGE_STITCH_OWNER);
         launch_owner_stitch(g, rec, P, led, led0, b, Gs, st);
+        tm.end(1);
+        tm.begin(FPM_STAGE_BACKPROJ);
         launch_backproj(g, rec, sub, led, led0, b, P, Gs, false, mP, A, rms, GP, st);
+        tm.end(1);
     } else {
+        tm.begin(FPM_STAGE_ZERO);
         launch_zero(Gs, (size_t)g.Nr * g.Nc, st);
+        tm.end(1);
+        tm.begin(FPM_STAGE_BACKPROJ);
         launch_backproj(g, rec, sub, led, led0, b, P, Gs, true, mP, A, rms, GP, st);
+        tm.end(1);
     }
 }
 
@@ -319,9 +407,12 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             }
             float2* Shat = S0.as<float2>();
             float2* Gs = S1.as<float2>();
+            StageTimer tm(o.stage_times, st);
             // spectrum of the initial object
+            tm.begin(FPM_STAGE_SPECTRUM_INIT);
             fft_contig(O, Shat, g.Nr, g.Nc, false, tw, st);
             fft_cols_forward(Shat, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+            tm.end(1 + col_launches(g.pN));
             const uint32_t steps = (uint32_t)((L + B - 1) / B);
             for (int e = 0; e < ne; ++e) {
                 const RmsArgs rms{lr, beta, eps, o.pupil_clamp};
@@ -330,12 +421,18 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
                     const uint32_t led0 = s * B;
                     const uint32_t b = std::min<uint32_t>(B, (uint32_t)L - led0);
                     minibatch_backward(g, Shat, P, led, led0, b, y, pr, ws, Gs, o.stitch, mP.as<float>(), dA, rms,
-                                       nullptr, lacc, tw, st);
+                                       nullptr, lacc, tw, st, tm);
+                    tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
                     fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+                    tm.end(col_launches(g.pN));
                     const bool last = (e == ne - 1) && (s == steps - 1);
+                    tm.begin(FPM_STAGE_FUSED_UPDATE);
                     launch_fused_update(g, Gs, O, mO.as<float>(), rms, !last, tw, st);
+                    tm.end(1);
                     if (!last) {
+                        tm.begin(FPM_STAGE_FFT_N_COLS_HI);
                         fft_cols_forward(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+                        tm.end(col_launches(g.pN));
                         std::swap(Shat, Gs);
                     }
                 }
@@ -346,6 +443,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
                 FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, lossd.p, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost,
                                        st));
             FPM_HIP(hipStreamSynchronize(st));
+            tm.flush();
         }
         if (!isdev) {
             FPM_HIP(hipMemcpyAsync(O_out, O, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
@@ -399,11 +497,14 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             lossd.alloc(sizeof(double));
             FPM_HIP(hipMemsetAsync(lossd.p, 0, sizeof(double), st));
         }
+        StageTimer tm(o.stage_times, st);
+        tm.begin(FPM_STAGE_SPECTRUM_INIT);
         fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st);
         fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+        tm.end(1 + col_launches(g.pN));
         minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
                            o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
-                           loss_out ? lossd.as<double>() : nullptr, tw, st);
+                           loss_out ? lossd.as<double>() : nullptr, tw, st, tm);
         fft_cols_inverse(S1.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
         float2* GOd;
         if (isdev) {
@@ -419,6 +520,7 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         }
         if (loss_out) FPM_HIP(hipMemcpyAsync(loss_out, lossd.p, sizeof(double), hipMemcpyDeviceToHost, st));
         FPM_HIP(hipStreamSynchronize(st));
+        tm.flush();
     });
 }
 

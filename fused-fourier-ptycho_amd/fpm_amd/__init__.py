@@ -54,6 +54,18 @@ class FPMError(RuntimeError):
         self.code = code
 
 
+STAGE_COUNT = 13
+
+
+class StageTimes(ctypes.Structure):
+    """fpm_stage_times: per-stage device milliseconds, executions, launches."""
+    _fields_ = [
+        ("ms", ctypes.c_double * STAGE_COUNT),
+        ("count", ctypes.c_int64 * STAGE_COUNT),
+        ("launches", ctypes.c_int32 * STAGE_COUNT),
+    ]
+
+
 class _Opts(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_int32),
@@ -63,7 +75,8 @@ class _Opts(ctypes.Structure):
         ("stream", ctypes.c_void_p),
         ("loss_per_epoch", ctypes.POINTER(ctypes.c_double)),
         ("pupil_clamp", ctypes.c_float),
-        ("reserved", ctypes.c_int32 * 7),
+        ("stage_times", ctypes.POINTER(StageTimes)),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 

@@ -139,6 +139,83 @@ def image_model_numpy(O: np.ndarray, P: np.ndarray, led: np.ndarray, pratio: int
     return out
 
 
+def fortran_empty(shape, dtype, device):
+    """torch tensor laid out column-major for MATLAB shape `shape`: returned as
+    the row-major buffer of reversed shape (element (i, j[, l]) of the MATLAB
+    array is t[l, j, i])."""
+    import torch
+
+    return torch.empty(tuple(reversed(shape)), dtype=dtype, device=device)
+
+
+def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda", led_grid: int = LED_GRID,
+                defocus_rad: float = 0.5):
+    """The same generator on the GPU for large workloads (torch tensors in
+    column-major layout, transforms and the image model through libfpm_amd).
+    Returns a dict of device tensors + host LED/mask arrays."""
+    import torch
+    import torch.nn.functional as F
+
+    from . import _check, lib
+
+    N = n * pratio
+    dev = torch.device(device)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    led, _, nai = led_offsets(n, pratio, wavelength_um=WAVELENGTH_UM, na=NA, pixel_um=PIXEL_UM,
+                              led_pitch_um=LED_PITCH_UM, led_height_um=LED_HEIGHT_UM, led_grid=led_grid)
+    L = led.shape[1]
+    A, P_true = pupil(n, defocus_rad)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    # amplitude: band-limited noise (the filter is even, so the layout
+    # transpose of the torch view does not matter)
+    noise = torch.rand((N, N), generator=gen, device=dev, dtype=torch.float32).to(torch.complex64)
+    spec = torch.empty_like(noise)
+    _check(lib().fpm_fft2(noise.data_ptr(), spec.data_ptr(), N, N, 1, -1, 1, stream))
+    f = torch.fft.fftfreq(N, d=PIXEL_UM / pratio, device=dev, dtype=torch.float64) * 2 * np.pi
+    cut = (NA + float(nai.max())) * 2 * np.pi / WAVELENGTH_UM
+    mask = ((f[:, None] ** 2 + f[None, :] ** 2) <= cut * cut).to(torch.float32)
+    spec *= mask
+    del mask
+    _check(lib().fpm_fft2(spec.data_ptr(), noise.data_ptr(), N, N, 1, 1, 1, stream))
+    amp = noise.real.clone()
+    del noise, spec
+    amp = 0.3 + 0.7 * (amp - amp.min()) / (amp.max() - amp.min())
+    # phase: 20 Gaussians (separable outer products)
+    phase = torch.zeros((N, N), device=dev, dtype=torch.float32)
+    g = np.random.Generator(np.random.Philox(seed))
+    ax = torch.arange(N, device=dev, dtype=torch.float32)
+    for _ in range(20):
+        cy, cx = g.random(2) * N
+        sig = N / 64 + g.random() * (N / 16 - N / 64)
+        h = (g.random() * 2 - 1) * np.pi / 2
+        gy = torch.exp(-((ax - float(cy)) ** 2) / (2 * sig * sig))
+        gx = torch.exp(-((ax - float(cx)) ** 2) / (2 * sig * sig))
+        phase.add_(h * torch.outer(gx, gy))
+    O_true = torch.polar(amp, phase)
+    del amp, phase
+    Pt = torch.from_numpy(np.ascontiguousarray(P_true.T)).to(dev)
+    ledd = torch.from_numpy(np.ascontiguousarray(led.T)).to(dev)
+    I = fortran_empty((n, n, L), torch.float32, dev)
+    _check(lib().fpm_forward_intensity(O_true.data_ptr(), N, N, Pt.data_ptr(), n, n, L, ledd.data_ptr(),
+                                       float(pratio), 1, I.data_ptr(), 1, stream))
+    del O_true
+    I *= PEAK_COUNTS / float(I[0].max())
+    I = torch.poisson(I, generator=gen)
+    I += OFFSET_COUNTS
+    raw = I.round_().clamp_(0, 65535).to(torch.int32).to(torch.uint16)
+    del I
+    mn, mx = int(raw.min()), int(raw.max())
+    centre = torch.sqrt((raw[0].to(torch.float32) - mn) / float(mx - mn))
+    O0 = F.interpolate(centre[None, None], scale_factor=pratio, mode="bilinear", align_corners=False)[0, 0]
+    O0 = O0.to(torch.complex64).contiguous()
+    P0 = torch.from_numpy(np.ascontiguousarray(A.T.astype(np.complex64))).to(dev)
+    Ad = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
+    torch.cuda.synchronize(dev)
+    return {"n": n, "N": N, "L": L, "pratio": pratio, "led": led, "led_dev": ledd, "A": A, "A_dev": Ad,
+            "raw_dev": raw, "O0_dev": O0, "P0_dev": P0, "na_illum": nai, "raw_min": mn, "raw_max": mx}
+
+
 def make_numpy(n: int, pratio: int, seed: int = 20261203, led_grid: int = LED_GRID, defocus_rad: float = 0.5,
                noise: bool = True) -> Dataset:
     """Whole dataset on the host (small and medium sizes)."""

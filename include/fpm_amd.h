@@ -59,6 +59,31 @@ typedef struct { float re, im; } fpm_c32;
 #define FPM_STITCH_ATOMIC 0  /* FP32 atomics into the spectrum (v3 kernels.cuh:115-116) */
 #define FPM_STITCH_OWNER  1  /* owner-computes gather in fixed LED order: deterministic */
 
+/* Per-stage device time of one call, measured with HIP events recorded on the
+ * call's stream around each stage (opts.stage_times; costs a few us per step). */
+enum {
+    FPM_STAGE_SPECTRUM_INIT = 0, /* DFT of the initial object (contiguous + column passes) */
+    FPM_STAGE_CROP,              /* crop x pupil (kernels.cuh:19-65) */
+    FPM_STAGE_IFFT_N_COLS_LO,    /* batched inverse DFT, column pass(es) */
+    FPM_STAGE_IFFT_N_CONTIG_LO,  /* batched inverse DFT, contiguous pass */
+    FPM_STAGE_STENCIL,           /* loss stencil (kernels.cuh:197-275) */
+    FPM_STAGE_FFT_N_CONTIG_LO,   /* batched forward DFT, contiguous pass */
+    FPM_STAGE_FFT_N_COLS_LO,     /* batched forward DFT, column pass(es) */
+    FPM_STAGE_ZERO,              /* clearing the spectrum gradient (atomic stitch) */
+    FPM_STAGE_STITCH_OWNER,      /* owner-computes stitch */
+    FPM_STAGE_BACKPROJ,          /* back-projection (+ atomics) + pupil update */
+    FPM_STAGE_IFFT_N_COLS_HI,    /* inverse DFT of the spectrum gradient, column passes */
+    FPM_STAGE_FUSED_UPDATE,      /* contiguous inverse pass + RMSprop of O + contiguous forward pass */
+    FPM_STAGE_FFT_N_COLS_HI,     /* forward DFT of the new object, column passes */
+    FPM_STAGE_COUNT
+};
+
+typedef struct fpm_stage_times {
+    double  ms[FPM_STAGE_COUNT];      /* summed device time per stage */
+    int64_t count[FPM_STAGE_COUNT];   /* number of stage executions */
+    int32_t launches[FPM_STAGE_COUNT];/* kernel launches per execution */
+} fpm_stage_times;
+
 typedef struct {
     int32_t struct_size;          /* sizeof(fpm_opts); lets the ABI grow */
     int32_t stitch;               /* FPM_STITCH_*; default ATOMIC */
@@ -69,7 +94,8 @@ typedef struct {
                                      epoch's minibatches of sum|Dx e| + |Dy e| */
     float   pupil_clamp;          /* pupil amplitude band (v3 hard-codes 0.2);
                                      <= 0 selects 0.2 */
-    int32_t reserved[7];
+    fpm_stage_times* stage_times; /* optional: filled with per-stage device time */
+    int32_t reserved[5];
 } fpm_opts;
 
 /* Fills *o with the v3 defaults (atomic stitch, host pointers, silent,

@@ -1,0 +1,56 @@
+"""Shared test helpers: small problems with guaranteed sign margins."""
+import numpy as np
+
+import fpm_amd
+from fpm_amd import synth
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def small_leds(n, r, grid=3, spiral=True):
+    """LED windows of a grid x grid array (the standard optics, fewer LEDs)."""
+    led, _, _ = fpm_amd.led_offsets(n, r, led_grid=grid, spiral=spiral)
+    return led
+
+
+def small_leds_oracle(n, r, grid=3, spiral=True):
+    import oracle
+    led, _, _ = oracle.led_offsets_matlab(n, r, led_grid=grid, spiral=spiral)
+    return led
+
+
+def margin_field(shape, rng, step=0.01, levels=10):
+    """Integer-level field k * step with every circular right/down neighbour
+    pair different, so every sign decision of the loss stencil has a margin
+    of at least `step`."""
+    nr, nc = shape
+    k = rng.integers(0, levels, size=shape)
+    for _ in range(1000):
+        bad = (k == np.roll(k, -1, axis=1)) | (k == np.roll(k, -1, axis=0)) | \
+              (k == np.roll(k, 1, axis=1)) | (k == np.roll(k, 1, axis=0))
+        if not bad.any():
+            break
+        k[bad] = rng.integers(0, levels, size=int(bad.sum()))
+    assert not ((k == np.roll(k, -1, axis=1)) | (k == np.roll(k, -1, axis=0))).any()
+    return k * step
+
+
+def margin_problem(n, r, grid=3, seed=7, spiral=True):
+    """(O, P, Y, led) whose residual e = model - Y is a margin field: every
+    loss-stencil sign is decided by >= 0.01, far above FP32 rounding."""
+    rng = np.random.default_rng(seed)
+    N = n * r
+    led = small_leds(n, r, grid, spiral)
+    _, P = synth.pupil(n)
+    amp = synth.object_numpy(N, r, 0.3, seed)
+    O = np.asfortranarray(amp.astype(np.complex128))
+    model = synth.image_model_numpy(O, P.astype(np.complex128), led, r)
+    L = led.shape[1]
+    Y = np.empty((n, n, L), dtype=np.float32, order="F")
+    for l in range(L):
+        Y[:, :, l] = (model[:, :, l] - margin_field((n, n), rng)).astype(np.float32)
+    return O.astype(np.complex64), P, Y, led

@@ -1,0 +1,89 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, and rejects bad arguments before touching a device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import fpm_amd
+from helpers import small_leds_oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fpm_amd.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(fpm_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_python_symbol_list():
+    assert _declared() == sorted(fpm_amd.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = fpm_amd.lib()
+    for s in _declared():
+        assert hasattr(lib, s), s
+    out = subprocess.run(["nm", "-D", "--defined-only", fpm_amd.library_path], capture_output=True, text=True).stdout
+    for s in _declared():
+        assert re.search(rf"\bT {s}\b", out), s
+
+
+def test_abi_version_and_opts_layout():
+    assert fpm_amd.lib().fpm_abi_version() == 1
+    o = fpm_amd._Opts()
+    fpm_amd.lib().fpm_opts_init(ctypes.byref(o))
+    assert o.struct_size == ctypes.sizeof(fpm_amd._Opts)
+    assert abs(o.pupil_clamp - 0.2) < 1e-7 and o.stitch == 0
+
+
+def _call_solve(nr=16, Nr=32, L=4, batch=2, led=None, epochs=1):
+    O = np.zeros((Nr, Nr), np.complex64, order="F")
+    P = np.zeros((nr, nr), np.complex64, order="F")
+    Y = np.ones((nr, nr, L), np.float32, order="F")
+    A = np.ones((nr, nr), np.float32, order="F")
+    if led is None:
+        led = np.ones((2, L), np.int32, order="F")
+    return fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, 2, epochs, batch, (1e-3, 0.999, 1e-4))
+
+
+def test_rejects_non_power_of_two():
+    with pytest.raises(fpm_amd.FPMError) as e:
+        _call_solve(nr=24, Nr=48)
+    assert e.value.code == -2
+
+
+def test_rejects_bad_batch():
+    with pytest.raises(fpm_amd.FPMError) as e:
+        _call_solve(batch=0)
+    assert e.value.code == -1 and "batch" in str(e.value)
+
+
+def test_rejects_pupil_larger_than_object():
+    with pytest.raises(fpm_amd.FPMError) as e:
+        _call_solve(nr=64, Nr=32)
+    assert e.value.code == -1
+
+
+def test_rejects_negative_epochs():
+    with pytest.raises(fpm_amd.FPMError) as e:
+        _call_solve(epochs=-1)
+    assert e.value.code == -1
+
+
+def test_led_offsets_match_oracle_restatement():
+    for n, r, grid in [(128, 4, 19), (512, 8, 19), (2048, 8, 19), (64, 2, 5)]:
+        a, _, _ = fpm_amd.led_offsets(n, r, led_grid=grid)
+        assert np.array_equal(a, small_leds_oracle(n, r, grid)), (n, r, grid)
+    a, _, _ = fpm_amd.led_offsets(128, 4, spiral=False)
+    b = small_leds_oracle(128, 4, 19, spiral=False)
+    assert np.array_equal(a, b)
+
+
+def test_led_offsets_rejects_even_grid():
+    with pytest.raises(fpm_amd.FPMError):
+        fpm_amd.led_offsets(128, 4, led_grid=4)

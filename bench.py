@@ -141,7 +141,9 @@ def main():
     if args.y == "u16":
         Yd, y_u16 = ds["raw_dev"], True
     else:
-        Yd = torch.sqrt((ds["raw_dev"].to(torch.float32) - ds["raw_min"]) / float(ds["raw_max"] - ds["raw_min"]))
+        Yd = torch.empty(ds["raw_dev"].shape, dtype=torch.float32, device=dev)
+        fpm_amd._check(fpm_amd.lib().fpm_prepare_amplitude(ds["raw_dev"].data_ptr(), ds["raw_dev"].numel(),
+                                                           Yd.data_ptr(), 1, None, None, None))
         y_u16 = False
     Od = torch.empty_like(ds["O0_dev"])
     Pd = torch.empty_like(ds["P0_dev"])
@@ -191,6 +193,9 @@ def main():
     }
     if st is not None:
         alg = stage_alg_bytes(n, N, b_avg, y_bytes, args.stitch)
+        if int(st.count[STAGE_NAMES.index("crop")]) == 0:
+            # the crop runs inside the first inverse column pass
+            alg["ifft_n_cols"] += alg["crop"]
         stages = {}
         for k, name in enumerate(STAGE_NAMES):
             cnt = int(st.count[k])

@@ -41,14 +41,52 @@ template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a) {
 constexpr int kTwiddleLen = 16384;
 constexpr int kTwiddleLog = 14;
 
-// W_{kTwiddleLen}^m, FP64-generated and rounded once to FP32 (forward sign)
+// W_{kTwiddleLen}^m.  `tab` is the FP64-generated, FP32-rounded table in
+// global memory (kernel argument); inside a kernel the factors are read from a
+// two-level copy in LDS, W^m = W^(128 h) W^l with m = 128 h + l (256 entries,
+// 2 KiB), so a twiddle costs two LDS reads and one complex product (<= 2 ulp
+// beyond the table's rounding) instead of an L2 gather from a 128 KiB table.
+constexpr int kTwLo = 128;
 struct Twiddles {
-    const float2* __restrict__ tab;
+    const float2* __restrict__ tab;   // global table, kTwiddleLen entries
+    const float2* lds = nullptr;      // [0,128): W^(128 h), [128,256): W^l
     template <bool INV> __device__ __forceinline__ float2 get(uint32_t m) const {
-        float2 w = tab[m & (kTwiddleLen - 1)];
+        m &= (kTwiddleLen - 1);
+        const float2 w = cmul(lds[m >> 7], lds[kTwLo + (m & (kTwLo - 1))]);
         return INV ? cconj(w) : w;
     }
 };
+
+// Fills the LDS copy (256 words) and binds it; ends with a barrier.
+__device__ __forceinline__ Twiddles bind_twiddles(Twiddles tw, float2* lds_tw, int t, int T) {
+    for (int k = t; k < 2 * kTwLo; k += T)
+        lds_tw[k] = k < kTwLo ? tw.tab[k * (kTwiddleLen / kTwLo)] : tw.tab[k - kTwLo];
+    __syncthreads();
+    tw.lds = lds_tw;
+    return tw;
+}
+
+// W^(s m) for s = 1..R-1 from 4 lookups (m, 2m, 4m, 8m) and at most two
+// products more (error <= 3 products of table-grade values)
+template <int R, bool INV>
+__device__ __forceinline__ void twiddle_powers(const Twiddles& tw, uint32_t m, float2 (&w)[16]) {
+    w[1] = tw.get<INV>(m);
+    if constexpr (R > 2) {
+        w[2] = tw.get<INV>(2 * m);
+        w[3] = cmul(w[1], w[2]);
+    }
+    if constexpr (R > 4) {
+        w[4] = tw.get<INV>(4 * m);
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+    }
+    if constexpr (R > 8) {
+        w[8] = tw.get<INV>(8 * m);
+#pragma unroll
+        for (int s = 9; s < 16; ++s) w[s] = cmul(w[8], w[s - 8]);
+    }
+}
 
 // ----------------------------------------------------------- small DFTs
 template <bool INV> __device__ __forceinline__ void dft2(float2& a0, float2& a1) {
@@ -205,8 +243,10 @@ __device__ __forceinline__ void pass_from_lds(float2 (&v)[16], const float2* lds
         if constexpr (L > 1) {
             const uint32_t k = (uint32_t)(j % L);
             constexpr uint32_t step = kTwiddleLen / (L * R);
+            float2 w[16];
+            twiddle_powers<R, INV>(tw, k * step, w);
 #pragma unroll
-            for (int s = 1; s < R; ++s) v[q * R + s] = cmul(v[q * R + s], tw.get<INV>(s * k * step));
+            for (int s = 1; s < R; ++s) v[q * R + s] = cmul(v[q * R + s], w[s]);
         }
         dft<R, INV>(&v[q * R]);
     }

@@ -4,6 +4,32 @@
 
 namespace fpm {
 
+// Crop of LED windows from the column-permuted N-side spectrum times the
+// pupil, formed inside the first inverse column pass of the batched transform
+// (see fft_cols_inverse).  Page l of the batch is LED led0 + l.
+struct CropArgs {
+    const float2* Shat;
+    const float2* P;
+    const int2* led;
+    uint32_t led0;
+    uint32_t nr, nc, Nr, Nc;
+    ColPerm pN;
+};
+
+// Owner-computes stitch formed inside the first inverse column pass of the
+// object-spectrum gradient: element (row, natural col) = sum over the LEDs
+// led0..led0+b-1 (b <= 64) whose window covers it, in LED order, of
+// R_l conj(P) / (Nr Nc), R = the column-permuted per-LED spectra.
+struct StitchArgs {
+    const float2* R;
+    const float2* P;
+    const int2* led;
+    uint32_t led0, b;
+    uint32_t nr, nc, Nr, Nc;
+    ColPerm pn;
+    float invN2;
+};
+
 // one LEN-point transform per contiguous line (LEN = len, nlines lines)
 void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bool inverse, const float2* twid,
                 hipStream_t st);
@@ -11,8 +37,11 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // transforms along the columns index of rows x cols pages; see fft_kernels.hip
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, float2* nat_out);
+// With crop, the input is not read: the first pass gathers ifftshift(S P)
+// of each page's LED window (natural frequency columns) instead.
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, const float2* nat_in);
+                      hipStream_t st, const float2* nat_in, const CropArgs* crop = nullptr,
+                      const StitchArgs* stitch = nullptr);
 
 // full 2D transforms, in place; the frequency side is column-permuted by p
 void fft2_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,

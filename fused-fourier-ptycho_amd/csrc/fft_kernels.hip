@@ -33,7 +33,9 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fft_contig_kernel(const flo
     using Map = typename Cfg::Map;
     using SL = Slots<LEN, INV, Cfg::T, Map>;
     __shared__ float2 lds[Map::LDS_WORDS];
+    __shared__ float2 twl[2 * kTwLo];
     const int t = threadIdx.x;
+    tw = bind_twiddles(tw, twl, t, Cfg::T);
     const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
     float2 v[16];
 #pragma unroll
@@ -81,18 +83,59 @@ template <int LEN, int TR> struct StridedCfg {
     using Map = LineFastMap<TR, LEN>;
 };
 
-template <int LEN, int TR, bool INV, bool TWID>
-__global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs a, Twiddles tw) {
+// Sources of the first pass (SRC):
+//  SRC_PLAIN  a.in
+//  SRC_CROP   point (row di, natural column dj) of page l is formed on the fly
+//             as ifftshift(S P) of LED led0 + l -- the crop of
+//             cuFPM_v3/cuda/kernels.cuh:38-64 -- gathered from the
+//             column-permuted N-side spectrum, so the cropped spectrum never
+//             makes a round trip to HBM;
+//  SRC_STITCH point (row, natural column) of the object-spectrum gradient is
+//             summed on the fly, in LED order, from the per-LED
+//             back-projections R_l conj(P) / (Nr Nc) of the windows covering
+//             it (the owner-computes stitch), so the gradient spectrum is
+//             never written, read back or cleared.
+enum { SRC_PLAIN = 0, SRC_CROP = 1, SRC_STITCH = 2 };
+constexpr int kMaxStitchLeds = 64;  // one bit per LED of the batch
+
+template <int LEN, int TR, bool INV, bool TWID, int SRC = SRC_PLAIN>
+__global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs a, CropArgs c, StitchArgs sa,
+                                                                   Twiddles tw) {
     using Cfg = StridedCfg<LEN, TR>;
     using Map = typename Cfg::Map;
     using SL = Slots<LEN, INV, Cfg::T, Map>;
     __shared__ float2 lds[Map::LDS_WORDS];
+    __shared__ float2 twl[2 * kTwLo];
     const int t = threadIdx.x;
+    tw = bind_twiddles(tw, twl, t, Cfg::T);
     const uint32_t g = blockIdx.y;
     const size_t base = (size_t)blockIdx.z * a.page + (size_t)blockIdx.x * TR;
     const float2* __restrict__ in = a.in + base;
     float2* __restrict__ out = a.out + base;
     float2 v[16];
+    uint32_t kl = 0, kt = 0;
+    if constexpr (SRC == SRC_CROP) {
+        const int2 k = c.led[c.led0 + blockIdx.z];
+        kl = (uint32_t)k.x;
+        kt = (uint32_t)k.y;
+    }
+    // stitch: window corners of the batch and this thread's row mask
+    uint64_t rowhits = 0;
+    uint32_t srow = 0;
+    __shared__ uint32_t win[SRC == SRC_STITCH ? kMaxStitchLeds : 1];
+    if constexpr (SRC == SRC_STITCH) {
+        const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
+        for (uint32_t l = t; l < sa.b; l += Cfg::T) {
+            const int2 k = sa.led[sa.led0 + l];
+            const uint32_t wr = ((uint32_t)k.y - 1u + (sa.Nr >> 1)) & Rm;
+            const uint32_t wc = ((uint32_t)k.x - 1u + (sa.Nc >> 1)) & Cm;
+            win[l] = wr | (wc << 16);
+        }
+        __syncthreads();
+        srow = blockIdx.x * TR + (uint32_t)(t % TR);  // LineFast: one row per thread
+        for (uint32_t l = 0; l < sa.b; ++l)
+            if (((srow - (win[l] & 0xffffu)) & Rm) < sa.nr) rowhits |= 1ull << l;
+    }
 #pragma unroll
     for (int q = 0; q < 16 / SL::R0; ++q)
 #pragma unroll
@@ -100,7 +143,37 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             int line, idx;
             SL::in(t, q, s, line, idx);
             const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
-            v[q * SL::R0 + s] = in[col * a.rows + line];
+            if constexpr (SRC == SRC_CROP) {
+                const uint32_t di = blockIdx.x * TR + line, dj = (uint32_t)col;
+                const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
+                const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
+                const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
+                const float2 S = c.Shat[(size_t)c.pN(scl) * c.Nr + srw];
+                v[q * SL::R0 + s] = cmul(S, c.P[j * c.nr + i]);
+            } else if constexpr (SRC == SRC_STITCH) {
+                const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
+                const uint32_t plane = sa.nr * sa.nc;
+                float2 acc = make_float2(0.f, 0.f);
+                uint64_t m = rowhits;
+                while (m) {
+                    const int l = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const uint32_t w = win[l];
+                    const uint32_t lj = ((uint32_t)col - (w >> 16)) & Cm;
+                    if (lj < sa.nc) {
+                        const uint32_t li = (srow - (w & 0xffffu)) & Rm;
+                        const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
+                        const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
+                        const float2 Rv = sa.R[(size_t)l * plane + sa.pn(dj) * sa.nr + di];
+                        const float2 gw = cmulc(Rv, sa.P[lj * sa.nr + li]);
+                        acc.x += gw.x * sa.invN2;
+                        acc.y += gw.y * sa.invN2;
+                    }
+                }
+                v[q * SL::R0 + s] = acc;
+            } else {
+                v[q * SL::R0 + s] = in[col * a.rows + line];
+            }
         }
     first_pass_dft<LEN, INV>(v);
     remaining_passes<LEN, INV, Cfg::T, Map>(v, lds, t, tw);
@@ -145,23 +218,46 @@ static void launch_contig_len(int len, const float2* in, float2* out, uint32_t n
 }
 
 // grid.x is given in 16-row tiles; a wide-tile kernel covers TR/16 of them
+struct Gather {
+    const CropArgs* crop = nullptr;
+    const StitchArgs* stitch = nullptr;
+};
+
+template <int L, int TR, bool INV, bool TWID>
+static void launch_strided_one(const StridedArgs& a, Gather gs, dim3 grid, Twiddles tw, hipStream_t st) {
+    const CropArgs c = gs.crop ? *gs.crop : CropArgs{};
+    const StitchArgs s = gs.stitch ? *gs.stitch : StitchArgs{};
+    const dim3 blk(StridedCfg<L, TR>::T);
+    if constexpr (INV) {
+        if (gs.crop) {
+            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_CROP>), grid, blk, 0, st, a, c, s, tw);
+            return;
+        }
+        if (gs.stitch) {
+            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_STITCH>), grid, blk, 0, st, a, c, s, tw);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_PLAIN>), grid, blk, 0, st, a, c, s, tw);
+}
+
 template <int L, bool INV, bool TWID>
-static void launch_strided_tr(const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st) {
+static void launch_strided_tr(const StridedArgs& a, Gather gs, dim3 grid, Twiddles tw, hipStream_t st) {
     constexpr int TRP = strided_rows_pref(L);
     const uint32_t rows = grid.x * kStridedRows;
     if (TRP > 16 && rows % TRP == 0) {
-        dim3 g2(rows / TRP, grid.y, grid.z);
-        hipLaunchKernelGGL((fft_strided_kernel<L, TRP, INV, TWID>), g2, dim3(StridedCfg<L, TRP>::T), 0, st, a, tw);
+        launch_strided_one<L, TRP, INV, TWID>(a, gs, dim3(rows / TRP, grid.y, grid.z), tw, st);
     } else {
-        hipLaunchKernelGGL((fft_strided_kernel<L, 16, INV, TWID>), grid, dim3(StridedCfg<L, 16>::T), 0, st, a, tw);
+        launch_strided_one<L, 16, INV, TWID>(a, gs, grid, tw, st);
     }
 }
 
 template <bool INV, bool TWID>
-static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st) {
-#define FPM_STRIDED_CASE(L)                              \
-    case L:                                              \
-        launch_strided_tr<L, INV, TWID>(a, grid, tw, st); \
+static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st,
+                               Gather gs = Gather{}) {
+#define FPM_STRIDED_CASE(L)                                  \
+    case L:                                                  \
+        launch_strided_tr<L, INV, TWID>(a, gs, grid, tw, st); \
         break;
     switch (len) {
         FPM_STRIDED_CASE(16)
@@ -224,18 +320,23 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
 // inverse column transform(s): permuted (or natural, from nat_in) frequency
 // columns -> natural columns in `data`
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, const float2* nat_in) {
+                      hipStream_t st, const float2* nat_in, const CropArgs* crop, const StitchArgs* stitch) {
     Twiddles tw{twid};
     StridedArgs a{};
     a.rows = rows;
     a.page = (uint64_t)rows * cols;
     const uint32_t tiles = rows / kStridedRows;
+    Gather gs{crop, stitch};
+    if (stitch && (stitch->b > (uint32_t)kMaxStitchLeds || batch != 1))
+        throw HipError{hipErrorInvalidValue, "fused stitch supports one page and at most 64 LEDs"};
+    // crop and stitch sources are addressed by natural column, like nat_in
+    if (crop || stitch) nat_in = data;
     if (p.M2 == 1) {
         a.in = nat_in ? nat_in : data;
         a.out = data;
         a.in_g = a.out_g = 0;
         a.in_s = a.out_s = 1;
-        launch_strided_len<true, false>((int)cols, a, dim3(tiles, 1, batch), tw, st);
+        launch_strided_len<true, false>((int)cols, a, dim3(tiles, 1, batch), tw, st, gs);
         return;
     }
     // pass B^-1: length M2 over k2 for group k1, twiddle W_C^-(c2 k1)
@@ -252,7 +353,7 @@ void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
     a.out_g = p.M2;
     a.out_s = 1;
     a.tw_step = kTwiddleLen / cols;
-    launch_strided_len<true, true>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st);
+    launch_strided_len<true, true>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st, gs);
     // pass A^-1: length M1 over k1 for group c2
     a.in = data;
     a.in_g = a.out_g = 1;
@@ -268,7 +369,7 @@ void fft2_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, co
 
 void fft2_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                   hipStream_t st) {
-    fft_cols_inverse(data, rows, cols, batch, p, twid, st, nullptr);
+    fft_cols_inverse(data, rows, cols, batch, p, twid, st, nullptr, nullptr, nullptr);
     fft_contig(data, data, rows, cols * batch, true, twid, st);
 }
 
@@ -292,7 +393,9 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
     using SF = Slots<LEN, false, Cfg::T, Map>;
     static_assert(SI::RL == SF::R0, "inverse last pass and forward first pass must share the slot map");
     __shared__ float2 lds[Map::LDS_WORDS];
+    __shared__ float2 twl[2 * kTwLo];
     const int t = threadIdx.x;
+    tw = bind_twiddles(tw, twl, t, Cfg::T);
     const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
     float2 v[16];
 #pragma unroll

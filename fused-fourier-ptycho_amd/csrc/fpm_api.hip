@@ -265,7 +265,7 @@ inline int col_launches(const ColPerm& p) { return p.M2 == 1 ? 1 : 2; }
 
 // Per-minibatch gradient pipeline shared by the solve and the gradient entry.
 struct Workspace {
-    DevBuf lat, sub, rec;
+    DevBuf lat, rec;
 };
 
 // Shat (full spectrum, column-permuted) -> per-LED back-projection into Gs and
@@ -275,13 +275,11 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
                         RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
                         StageTimer& tm) {
     float2* lat = ws.lat.as<float2>();
-    float2* sub = ws.sub.as<float2>();
     float2* rec = ws.rec.as<float2>();
-    tm.begin(FPM_STAGE_CROP);
-    launch_crop_pupil(g, Shat, P, led, led0, b, sub, lat, st);
-    tm.end(1);
+    // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
+    const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
     tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
-    fft_cols_inverse(lat, g.nr, g.nc, b, g.pn, tw, st, nullptr);
+    fft_cols_inverse(lat, g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca);
     tm.end(col_launches(g.pn));
     tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
     fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st);
@@ -295,21 +293,37 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     tm.begin(FPM_STAGE_FFT_N_COLS_LO);
     fft_cols_forward(rec, g.nr, g.nc, b, g.pn, tw, st, nullptr);
     tm.end(col_launches(g.pn));
+    // stitch into Gs, pupil gradient / update, then the column passes of IDFT_N(Gs)
+    if (stitch == FPM_STITCH_OWNER && b <= 64) {
+        // owner-computes stitch inside the first column pass; the pupil update
+        // must come after it (the stitch reads the pupil of this step)
+        const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.pn, 1.0f / (float)(g.Nr * g.Nc)};
+        tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
+        fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa);
+        tm.end(col_launches(g.pN));
+        tm.begin(FPM_STAGE_BACKPROJ);
+        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, false, mP, A, rms, GP, st);
+        tm.end(1);
+        return;
+    }
     if (stitch == FPM_STITCH_OWNER) {
         tm.begin(FPM_STAGE_STITCH_OWNER);
         launch_owner_stitch(g, rec, P, led, led0, b, Gs, st);
         tm.end(1);
         tm.begin(FPM_STAGE_BACKPROJ);
-        launch_backproj(g, rec, sub, led, led0, b, P, Gs, false, mP, A, rms, GP, st);
+        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, false, mP, A, rms, GP, st);
         tm.end(1);
     } else {
         tm.begin(FPM_STAGE_ZERO);
         launch_zero(Gs, (size_t)g.Nr * g.Nc, st);
         tm.end(1);
         tm.begin(FPM_STAGE_BACKPROJ);
-        launch_backproj(g, rec, sub, led, led0, b, P, Gs, true, mP, A, rms, GP, st);
+        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, true, mP, A, rms, GP, st);
         tm.end(1);
     }
+    tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
+    fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+    tm.end(col_launches(g.pN));
 }
 
 }  // namespace
@@ -397,7 +411,6 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             mO.alloc(NN * sizeof(float));
             mP.alloc(nn * sizeof(float));
             ws.lat.alloc(nn * B * sizeof(float2));
-            ws.sub.alloc(nn * B * sizeof(float2));
             ws.rec.alloc(nn * B * sizeof(float2));
             FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
             FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
@@ -422,9 +435,6 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
                     const uint32_t b = std::min<uint32_t>(B, (uint32_t)L - led0);
                     minibatch_backward(g, Shat, P, led, led0, b, y, pr, ws, Gs, o.stitch, mP.as<float>(), dA, rms,
                                        nullptr, lacc, tw, st, tm);
-                    tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
-                    fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
-                    tm.end(col_launches(g.pN));
                     const bool last = (e == ne - 1) && (s == steps - 1);
                     tm.begin(FPM_STAGE_FUSED_UPDATE);
                     launch_fused_update(g, Gs, O, mO.as<float>(), rms, !last, tw, st);
@@ -482,7 +492,6 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         const int2* led = (const int2*)upload(led_idx, 2 * (size_t)L, isdev, hLed, st);
         Workspace ws;
         ws.lat.alloc(nn * L * sizeof(float2));
-        ws.sub.alloc(nn * L * sizeof(float2));
         ws.rec.alloc(nn * L * sizeof(float2));
         S0.alloc(NN * sizeof(float2));
         S1.alloc(NN * sizeof(float2));
@@ -505,7 +514,6 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
                            o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
                            loss_out ? lossd.as<double>() : nullptr, tw, st, tm);
-        fft_cols_inverse(S1.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
         float2* GOd;
         if (isdev) {
             GOd = (float2*)G_O;
@@ -558,8 +566,9 @@ int fpm_forward_intensity(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, c
         const float scale = (float)((double)nn * (double)pr * (double)pr);
         for (int64_t l0 = 0; l0 < L; l0 += chunk) {
             const uint32_t b = (uint32_t)std::min<int64_t>(chunk, L - l0);
-            launch_crop_pupil(g, S0.as<float2>(), P, led, (uint32_t)l0, b, nullptr, lat.as<float2>(), st);
-            fft2_inverse(lat.as<float2>(), g.nr, g.nc, b, g.pn, tw, st);
+            const CropArgs ca{S0.as<float2>(), P, led, (uint32_t)l0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
+            fft_cols_inverse(lat.as<float2>(), g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca);
+            fft_contig(lat.as<float2>(), lat.as<float2>(), g.nr, g.nc * b, true, tw, st);
             launch_abs_scaled(lat.as<float2>(), nn * b, scale, squared, od + nn * (size_t)l0, st);
         }
         if (!isdev) FPM_HIP(hipMemcpyAsync(out, od, nn * (size_t)L * sizeof(float), hipMemcpyDeviceToHost, st));

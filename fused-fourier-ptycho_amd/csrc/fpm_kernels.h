@@ -28,23 +28,18 @@ struct RmsArgs {
     float lr, beta, eps, clamp;
 };
 
-// crop of LED windows from the object spectrum, times the pupil
-// (cuFPM_v3/cuda/kernels.cuh:19-65): sub = S (natural), lat = ifftshift(S P)
-// in the n-side spectral layout.  led = int2 {kl, kt} array, LEDs led0..led0+b-1.
-void launch_crop_pupil(const Geom& g, const float2* Shat, const float2* P, const int2* led, uint32_t led0, uint32_t b,
-                       float2* sub, float2* lat, hipStream_t st);
-
 // feature-domain loss gradient (kernels.cuh:197-275): z (b pages, natural) and
 // images led0.. -> u.  loss_acc (device double, may be null) += sum |Dx e|+|Dy e|.
 void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio,
                     float2* u, double* loss_acc, hipStream_t st);
 
 // back-projection (kernels.cuh:67-129) with FP32 atomics into Gs, pupil
-// gradient summed in registers over the batch in LED order.  If P_update
+// gradient R conj(S) summed in registers over the batch in LED order (S read
+// back from the spectrum Shat at the window element).  If P_update
 // (mP, A non-null) the pupil RMSprop + clamp of kernels.cuh:384-410 is applied
 // to P in place; otherwise the pupil gradient is written to GP.
 // With do_atomics = false only the pupil part runs (owner-computes variant).
-void launch_backproj(const Geom& g, const float2* R, const float2* sub, const int2* led, uint32_t led0, uint32_t b,
+void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const int2* led, uint32_t led0, uint32_t b,
                      float2* P, float2* Gs, bool do_atomics, float* mP, const float* A, RmsArgs rms, float2* GP,
                      hipStream_t st);
 

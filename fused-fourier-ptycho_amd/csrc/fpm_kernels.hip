@@ -14,37 +14,6 @@ namespace fpm {
 
 __device__ __forceinline__ float sgnf(float x) { return copysignf(1.0f, x); }
 
-// ------------------------------------------------------------ crop x pupil
-// cuFPM_v3/cuda/kernels.cuh:19-65 (fused_getSubpupil_shifted).  Window row of
-// local row i: (i + kt - 1 + Nr/2) mod Nr, evaluated in uint32 like the
-// reference (for power-of-two Nr the 2^32 wrap agrees with mod Nr).
-__global__ __launch_bounds__(256) void crop_pupil_kernel(Geom g, const float2* __restrict__ Shat,
-                                                         const float2* __restrict__ P, const int2* __restrict__ led,
-                                                         uint32_t led0, float2* __restrict__ sub,
-                                                         float2* __restrict__ lat) {
-    const uint32_t plane = g.nr * g.nc;
-    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
-    if (pix >= plane) return;
-    const uint32_t ell = blockIdx.y;
-    const uint32_t i = pix & (g.nr - 1), j = pix >> g.lnr;
-    const int2 k = led[led0 + ell];
-    const uint32_t srow = (i + (uint32_t)k.y - 1u + (g.Nr >> 1)) & (g.Nr - 1);
-    const uint32_t scol = (j + (uint32_t)k.x - 1u + (g.Nc >> 1)) & (g.Nc - 1);
-    const float2 S = Shat[(size_t)g.pN(scol) * g.Nr + srow];
-    const size_t page = (size_t)ell * plane;
-    if (sub) sub[page + pix] = S;
-    const float2 z = cmul(S, P[pix]);
-    const uint32_t di = (i + (g.nr >> 1)) & (g.nr - 1), dj = (j + (g.nc >> 1)) & (g.nc - 1);
-    lat[page + (size_t)g.pn(dj) * g.nr + di] = z;
-}
-
-void launch_crop_pupil(const Geom& g, const float2* Shat, const float2* P, const int2* led, uint32_t led0, uint32_t b,
-                       float2* sub, float2* lat, hipStream_t st) {
-    dim3 grid((g.plane() + 255) / 256, b);
-    hipLaunchKernelGGL(crop_pupil_kernel, grid, dim3(256), 0, st, g, Shat, P, led, led0, sub, lat);
-    FPM_LAUNCH_CHECK();
-}
-
 // ------------------------------------------------------------ loss stencil
 // cuFPM_v3/cuda/kernels.cuh:197-275 (differenceKernel_acc):
 //   e = |z|/(n_r n_c r^2) - y,  sx = sgn(e(i,j+1) - e(i,j)),  sy = sgn(e(i+1,j) - e(i,j))
@@ -150,7 +119,7 @@ void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t le
 // register (no atomics, deterministic) and the pupil update needs no second
 // kernel, since only this thread reads P at that pixel.
 __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __restrict__ R,
-                                                       const float2* __restrict__ sub, const int2* __restrict__ led,
+                                                       const float2* __restrict__ Shat, const int2* __restrict__ led,
                                                        uint32_t led0, uint32_t b, float2* __restrict__ P,
                                                        float2* __restrict__ Gs, int do_atomics, float* __restrict__ mP,
                                                        const float* __restrict__ A, RmsArgs rms,
@@ -166,16 +135,19 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
     for (uint32_t ell = 0; ell < b; ++ell) {
         const size_t page = (size_t)ell * plane;
         const float2 Rv = R[page + rpos];
+        // the window element: where S was cropped from and where the
+        // spectrum gradient goes (kernels.cuh:43-47 and :107-113 agree)
+        const int2 k = led[led0 + ell];
+        const uint32_t srow = (i + (uint32_t)k.y - 1u + (g.Nr >> 1)) & (g.Nr - 1);
+        const uint32_t scol = (j + (uint32_t)k.x - 1u + (g.Nc >> 1)) & (g.Nc - 1);
+        const size_t spos = (size_t)g.pN(scol) * g.Nr + srow;
         if (do_atomics) {
-            const int2 k = led[led0 + ell];
-            const uint32_t srow = (i + (uint32_t)k.y - 1u + (g.Nr >> 1)) & (g.Nr - 1);
-            const uint32_t scol = (j + (uint32_t)k.x - 1u + (g.Nc >> 1)) & (g.Nc - 1);
             const float2 gw = cmulc(Rv, p);
-            float* dst = (float*)(Gs + (size_t)g.pN(scol) * g.Nr + srow);
+            float* dst = (float*)(Gs + spos);
             unsafeAtomicAdd(dst, gw.x * invN2);
             unsafeAtomicAdd(dst + 1, gw.y * invN2);
         }
-        const float2 gq = cmulc(Rv, sub[page + pix]);
+        const float2 gq = cmulc(Rv, Shat[spos]);
         gp.x += gq.x;
         gp.y += gq.y;
     }
@@ -199,12 +171,12 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
     }
 }
 
-void launch_backproj(const Geom& g, const float2* R, const float2* sub, const int2* led, uint32_t led0, uint32_t b,
+void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const int2* led, uint32_t led0, uint32_t b,
                      float2* P, float2* Gs, bool do_atomics, float* mP, const float* A, RmsArgs rms, float2* GP,
                      hipStream_t st) {
     const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
     dim3 grid((g.plane() + 255) / 256);
-    hipLaunchKernelGGL(backproj_kernel, grid, dim3(256), 0, st, g, R, sub, led, led0, b, P, Gs, (int)do_atomics, mP, A,
+    hipLaunchKernelGGL(backproj_kernel, grid, dim3(256), 0, st, g, R, Shat, led, led0, b, P, Gs, (int)do_atomics, mP, A,
                        rms, GP, invN2);
     FPM_LAUNCH_CHECK();
 }

@@ -203,10 +203,12 @@ def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda",
     I *= PEAK_COUNTS / float(I[0].max())
     I = torch.poisson(I, generator=gen)
     I += OFFSET_COUNTS
-    raw = I.round_().clamp_(0, 65535).to(torch.int32).to(torch.uint16)
+    counts = I.round_().clamp_(0, 65535).to(torch.int32)
     del I
-    mn, mx = int(raw.min()), int(raw.max())
-    centre = torch.sqrt((raw[0].to(torch.float32) - mn) / float(mx - mn))
+    mn, mx = int(counts.min()), int(counts.max())
+    centre = torch.sqrt((counts[0].to(torch.float32) - mn) / float(mx - mn))
+    raw = counts.to(torch.uint16)
+    del counts
     O0 = F.interpolate(centre[None, None], scale_factor=pratio, mode="bilinear", align_corners=False)[0, 0]
     O0 = O0.to(torch.complex64).contiguous()
     P0 = torch.from_numpy(np.ascontiguousarray(A.T.astype(np.complex64))).to(dev)

@@ -98,7 +98,7 @@ def main():
     ap.add_argument("--pratio", type=int, default=8)
     ap.add_argument("--batch", type=int, default=26)
     ap.add_argument("--lr", type=float, default=5e-4)
-    ap.add_argument("--stitch", choices=["atomic", "owner"], default="atomic")
+    ap.add_argument("--stitch", choices=["atomic", "owner"], default="owner")
     ap.add_argument("--y", choices=["u16", "f32"], default="u16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stages", action="store_true", help="time without per-stage events")

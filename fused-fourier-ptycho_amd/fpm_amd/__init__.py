@@ -122,7 +122,7 @@ def _check(rc: int) -> None:
         raise FPMError(rc, lib().fpm_last_error().decode(errors="replace"))
 
 
-def _opts(stitch=STITCH_ATOMIC, device=False, stream=None, loss_buf=None, clamp=0.2, verbose=False) -> _Opts:
+def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0.2, verbose=False) -> _Opts:
     o = _Opts()
     lib().fpm_opts_init(ctypes.byref(o))
     o.stitch = int(stitch)
@@ -145,7 +145,7 @@ def _ptr(a: np.ndarray) -> int:
 
 
 def cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio, numEpochs, batch, opt: Sequence[float],
-                  *, stitch: int = STITCH_ATOMIC, loss: bool = False, clamp: float = 0.2,
+                  *, stitch: int = STITCH_OWNER, loss: bool = False, clamp: float = 0.2,
                   verbose: bool = False):
     """The drop-in solve (host arrays).  Returns (O, P), or (O, P, loss_per_epoch).
 
@@ -183,7 +183,7 @@ def cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio, numEpochs, batch, opt: Sequence[
     return O, P
 
 
-def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_ATOMIC):
+def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER):
     """One fused gradient over all images: (G_O, G_P, loss).
 
     The gradient-only call of benchmark/cuFPM/fullyfusedFPM.cu:21-162 with the
@@ -261,7 +261,7 @@ def fft2(x, inverse: bool = False) -> np.ndarray:
 # ----------------------------------------------------------------- device API
 def solve_device(O0_ptr: int, P0_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int, A_ptr: int,
                  N: Tuple[int, int], n: Tuple[int, int], L: int, pratio: float, epochs: float, batch: float,
-                 opt: Sequence[float], O_out_ptr: int, P_out_ptr: int, *, stitch: int = STITCH_ATOMIC,
+                 opt: Sequence[float], O_out_ptr: int, P_out_ptr: int, *, stitch: int = STITCH_OWNER,
                  stream: Optional[int] = None, loss: bool = False, clamp: float = 0.2):
     """fpm_pure_v3 on device pointers (column-major buffers already in HBM).
     Returns the per-epoch loss array if ``loss`` else None."""
@@ -276,7 +276,7 @@ def solve_device(O0_ptr: int, P0_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int
 
 
 def gradient_device(O_ptr: int, P_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int, N, n, L: int, pratio: float,
-                    GO_ptr: int, GP_ptr: int, *, stitch: int = STITCH_ATOMIC, stream: Optional[int] = None) -> float:
+                    GO_ptr: int, GP_ptr: int, *, stitch: int = STITCH_OWNER, stream: Optional[int] = None) -> float:
     """fpm_fused_gradient on device pointers; returns the loss."""
     lossv = ctypes.c_double(0.0)
     o = _opts(stitch=stitch, device=True, stream=stream)

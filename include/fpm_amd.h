@@ -57,7 +57,9 @@ typedef struct { float re, im; } fpm_c32;
 
 /* how the object-spectrum gradient is stitched from the per-LED back-projections */
 #define FPM_STITCH_ATOMIC 0  /* FP32 atomics into the spectrum (v3 kernels.cuh:115-116) */
-#define FPM_STITCH_OWNER  1  /* owner-computes gather in fixed LED order: deterministic */
+#define FPM_STITCH_OWNER  1  /* owner-computes gather in fixed LED order: deterministic;
+                                the default (faster on MI355X: it is formed inside the
+                                first column pass of the gradient's inverse transform) */
 
 /* Per-stage device time of one call, measured with HIP events recorded on the
  * call's stream around each stage (opts.stage_times; costs a few us per step). */
@@ -86,7 +88,7 @@ typedef struct fpm_stage_times {
 
 typedef struct {
     int32_t struct_size;          /* sizeof(fpm_opts); lets the ABI grow */
-    int32_t stitch;               /* FPM_STITCH_*; default ATOMIC */
+    int32_t stitch;               /* FPM_STITCH_*; default OWNER */
     int32_t pointers_are_device;  /* 1: every array argument is a device pointer */
     int32_t verbose;              /* 1: print the two parameter lines v3 prints */
     void*   stream;               /* hipStream_t to run on; NULL = library stream */
@@ -98,7 +100,7 @@ typedef struct {
     int32_t reserved[5];
 } fpm_opts;
 
-/* Fills *o with the v3 defaults (atomic stitch, host pointers, silent,
+/* Fills *o with the defaults (owner-computes stitch, host pointers, silent,
  * library stream, no loss, clamp 0.2). */
 void fpm_opts_init(fpm_opts* o);
 

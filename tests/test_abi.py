@@ -38,7 +38,7 @@ def test_abi_version_and_opts_layout():
     o = fpm_amd._Opts()
     fpm_amd.lib().fpm_opts_init(ctypes.byref(o))
     assert o.struct_size == ctypes.sizeof(fpm_amd._Opts)
-    assert abs(o.pupil_clamp - 0.2) < 1e-7 and o.stitch == 0
+    assert abs(o.pupil_clamp - 0.2) < 1e-7 and o.stitch == fpm_amd.STITCH_OWNER
 
 
 def _call_solve(nr=16, Nr=32, L=4, batch=2, led=None, epochs=1):

@@ -62,6 +62,24 @@ def stage_alg_bytes(n, N, b_avg, y_bytes, stitch):
     return d
 
 
+# stage -> kernel whose per-dispatch HBM traffic stands for it in the PMC summary
+STAGE_KERNEL = {"fused_update": "fused_update_kernel<16384, true>"}
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_fpm_solve_16384_b26.md")
+
+
+def measured_traffic(stage, n, N):
+    """Read + write bytes per dispatch of the stage's kernel from the committed
+    rocprofv3 PMC summary (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
+    kern = STAGE_KERNEL.get(stage)
+    if kern is None or (n, N) != (2048, 16384) or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0].endswith(kern):
+            return (float(cells[2]) + float(cells[3])) * 1e9
+    return None
+
+
 def run_cpu_baseline(ds, args):
     """FP32 oracle (OpenMP) on one minibatch step of the same workload."""
     import numpy as np
@@ -208,9 +226,12 @@ def main():
         out["stages"] = stages
         dom = max(stages, key=lambda s: stages[s]["ms_total"])
         ach = stages[dom]["alg_GBps"]
+        traffic = measured_traffic(dom, n, N)
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                           "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                           "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": traffic,
                            "alg_bytes_per_exec": alg[dom], "ms_per_exec": stages[dom]["ms_per_exec"]}
+        if traffic:
+            out["roofline"]["traffic_GBps"] = round(traffic / 1e9 / (stages[dom]["ms_per_exec"] / 1e3), 1)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, one minibatch step)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = run_cpu_baseline(ds, args)

@@ -76,7 +76,13 @@ struct StridedArgs {
 
 // TR adjacent rows per tile: 16 (128 B per column) for long lines, more for
 // short ones so that every workgroup has LEN*TR/16 >= 256 threads.
-constexpr int strided_rows_pref(int len) { return len >= 256 ? 16 : (4096 / len > 128 ? 128 : 4096 / len); }
+#ifndef FPM_STRIDED_POINTS
+#define FPM_STRIDED_POINTS 4096  // points per strided workgroup (16 per thread)
+#endif
+constexpr int strided_rows_pref(int len) {
+    return len * 16 >= FPM_STRIDED_POINTS ? 16
+                                          : (FPM_STRIDED_POINTS / len > 128 ? 128 : FPM_STRIDED_POINTS / len);
+}
 
 template <int LEN, int TR> struct StridedCfg {
     static constexpr int T = LEN * TR / 16;  // TR rows x LEN points / 16 per thread

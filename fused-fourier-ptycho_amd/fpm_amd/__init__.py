@@ -43,7 +43,9 @@ EXPORTED_SYMBOLS = (
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-library_path = os.path.join(_HERE, "libfpm_amd.so")
+# FPM_AMD_LIBRARY selects another build of the same library (e.g. a variant
+# compiled with different tile constants); default: the in-tree build
+library_path = os.environ.get("FPM_AMD_LIBRARY") or os.path.join(_HERE, "libfpm_amd.so")
 
 
 class FPMError(RuntimeError):

@@ -18,8 +18,12 @@ struct ColPerm {
     }
 };
 
-// longest column transform done in a single strided pass
-constexpr int kMaxStridedLen = 512;
+// longest column transform done in a single strided pass (LDS-bound: a
+// 2048-point pass uses 8-row tiles, 147 KiB)
+#ifndef FPM_MAX_STRIDED_LEN
+#define FPM_MAX_STRIDED_LEN 512
+#endif
+constexpr int kMaxStridedLen = FPM_MAX_STRIDED_LEN;
 // rows per strided tile (16 complex floats = 128 bytes)
 constexpr int kStridedRows = 16;
 

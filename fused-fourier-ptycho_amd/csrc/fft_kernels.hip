@@ -61,7 +61,8 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fft_contig_kernel(const flo
 }
 
 // --------------------------------------------------------- strided pass
-// blockIdx.x: tile of 16 rows; blockIdx.y: group g; blockIdx.z: page.
+// blockIdx.x: tile of TR rows; blockIdx.y: group g; blockIdx.z: page (x and z
+// swapped for the crop source, see below).
 // Point t of row r is read from column g*in_g + t*in_s and written to column
 // g*out_g + t*out_s.  TWID multiplies output t by W_M^(+-g t) with
 // tw_step = 16384 / M.
@@ -80,8 +81,10 @@ struct StridedArgs {
 #define FPM_STRIDED_POINTS 4096  // points per strided workgroup (16 per thread)
 #endif
 constexpr int strided_rows_pref(int len) {
-    return len * 16 >= FPM_STRIDED_POINTS ? 16
-                                          : (FPM_STRIDED_POINTS / len > 128 ? 128 : FPM_STRIDED_POINTS / len);
+    return len >= 2048 ? 8  // 2048 x 9 x 8 B = 147 KiB of LDS
+                       : len * 16 >= FPM_STRIDED_POINTS
+                             ? 16
+                             : (FPM_STRIDED_POINTS / len > 128 ? 128 : FPM_STRIDED_POINTS / len);
 }
 
 template <int LEN, int TR> struct StridedCfg {
@@ -115,13 +118,17 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     const int t = threadIdx.x;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     const uint32_t g = blockIdx.y;
-    const size_t base = (size_t)blockIdx.z * a.page + (size_t)blockIdx.x * TR;
+    // the crop launches pages fastest (grid x) so the LED pages that share a
+    // pupil tile run back to back and the pupil tile stays in cache
+    const uint32_t tile = SRC == SRC_CROP ? blockIdx.z : blockIdx.x;
+    const uint32_t page = SRC == SRC_CROP ? blockIdx.x : blockIdx.z;
+    const size_t base = (size_t)page * a.page + (size_t)tile * TR;
     const float2* __restrict__ in = a.in + base;
     float2* __restrict__ out = a.out + base;
     float2 v[16];
     uint32_t kl = 0, kt = 0;
     if constexpr (SRC == SRC_CROP) {
-        const int2 k = c.led[c.led0 + blockIdx.z];
+        const int2 k = c.led[c.led0 + page];
         kl = (uint32_t)k.x;
         kt = (uint32_t)k.y;
     }
@@ -138,7 +145,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             win[l] = wr | (wc << 16);
         }
         __syncthreads();
-        srow = blockIdx.x * TR + (uint32_t)(t % TR);  // LineFast: one row per thread
+        srow = tile * TR + (uint32_t)(t % TR);  // LineFast: one row per thread
         for (uint32_t l = 0; l < sa.b; ++l)
             if (((srow - (win[l] & 0xffffu)) & Rm) < sa.nr) rowhits |= 1ull << l;
     }
@@ -150,7 +157,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             SL::in(t, q, s, line, idx);
             const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
             if constexpr (SRC == SRC_CROP) {
-                const uint32_t di = blockIdx.x * TR + line, dj = (uint32_t)col;
+                const uint32_t di = tile * TR + line, dj = (uint32_t)col;
                 const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
                 const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
                 const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
@@ -236,7 +243,9 @@ static void launch_strided_one(const StridedArgs& a, Gather gs, dim3 grid, Twidd
     const dim3 blk(StridedCfg<L, TR>::T);
     if constexpr (INV) {
         if (gs.crop) {
-            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_CROP>), grid, blk, 0, st, a, c, s, tw);
+            const dim3 pages_first(grid.z, grid.y, grid.x);
+            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_CROP>), pages_first, blk, 0, st, a, c, s,
+                               tw);
             return;
         }
         if (gs.stitch) {
@@ -250,11 +259,12 @@ static void launch_strided_one(const StridedArgs& a, Gather gs, dim3 grid, Twidd
 template <int L, bool INV, bool TWID>
 static void launch_strided_tr(const StridedArgs& a, Gather gs, dim3 grid, Twiddles tw, hipStream_t st) {
     constexpr int TRP = strided_rows_pref(L);
+    constexpr int TRF = TRP < 16 ? TRP : 16;  // fallback tile when rows % TRP != 0
     const uint32_t rows = grid.x * kStridedRows;
-    if (TRP > 16 && rows % TRP == 0) {
+    if (rows % TRP == 0) {
         launch_strided_one<L, TRP, INV, TWID>(a, gs, dim3(rows / TRP, grid.y, grid.z), tw, st);
     } else {
-        launch_strided_one<L, 16, INV, TWID>(a, gs, grid, tw, st);
+        launch_strided_one<L, TRF, INV, TWID>(a, gs, dim3(rows / TRF, grid.y, grid.z), tw, st);
     }
 }
 
@@ -272,6 +282,10 @@ static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddle
         FPM_STRIDED_CASE(128)
         FPM_STRIDED_CASE(256)
         FPM_STRIDED_CASE(512)
+#if FPM_MAX_STRIDED_LEN >= 2048
+        FPM_STRIDED_CASE(1024)
+        FPM_STRIDED_CASE(2048)
+#endif
         default: throw HipError{hipErrorInvalidValue, "strided FFT length " + std::to_string(len) + " unsupported"};
     }
 #undef FPM_STRIDED_CASE

@@ -295,14 +295,14 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     tm.end(col_launches(g.pn));
     // stitch into Gs, pupil gradient / update, then the column passes of IDFT_N(Gs)
     if (stitch == FPM_STITCH_OWNER && b <= 64) {
-        // owner-computes stitch inside the first column pass; the pupil update
-        // must come after it (the stitch reads the pupil of this step)
+        // owner-computes stitch inside the first column pass of IDFT_N; the
+        // pupil update must come after it (the stitch reads this step's pupil)
         const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.pn, 1.0f / (float)(g.Nr * g.Nc)};
         tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
         fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa);
         tm.end(col_launches(g.pN));
         tm.begin(FPM_STAGE_BACKPROJ);
-        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, false, mP, A, rms, GP, st);
+        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, kBackprojPupil, mP, A, rms, GP, st);
         tm.end(1);
         return;
     }
@@ -311,14 +311,14 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
         launch_owner_stitch(g, rec, P, led, led0, b, Gs, st);
         tm.end(1);
         tm.begin(FPM_STAGE_BACKPROJ);
-        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, false, mP, A, rms, GP, st);
+        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, kBackprojPupil, mP, A, rms, GP, st);
         tm.end(1);
     } else {
         tm.begin(FPM_STAGE_ZERO);
         launch_zero(Gs, (size_t)g.Nr * g.Nc, st);
         tm.end(1);
         tm.begin(FPM_STAGE_BACKPROJ);
-        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, true, mP, A, rms, GP, st);
+        launch_backproj(g, rec, Shat, led, led0, b, P, Gs, kBackprojAtomic, mP, A, rms, GP, st);
         tm.end(1);
     }
     tm.begin(FPM_STAGE_IFFT_N_COLS_HI);

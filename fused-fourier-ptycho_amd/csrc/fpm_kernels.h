@@ -33,14 +33,16 @@ struct RmsArgs {
 void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio,
                     float2* u, double* loss_acc, hipStream_t st);
 
-// back-projection (kernels.cuh:67-129) with FP32 atomics into Gs, pupil
-// gradient R conj(S) summed in registers over the batch in LED order (S read
-// back from the spectrum Shat at the window element).  If P_update
-// (mP, A non-null) the pupil RMSprop + clamp of kernels.cuh:384-410 is applied
-// to P in place; otherwise the pupil gradient is written to GP.
-// With do_atomics = false only the pupil part runs (owner-computes variant).
+// back-projection (kernels.cuh:67-129): pupil gradient R conj(S) summed in
+// registers over the batch in LED order (S read back from the spectrum Shat
+// at the window element) and, in kBackprojAtomic mode, FP32 atomics of
+// R conj(P) / (Nr Nc) into Gs (kBackprojPupil: the owner-computes stitch is
+// done elsewhere).  If mP and A are non-null the pupil RMSprop + clamp of
+// kernels.cuh:384-410 is applied to P in place; otherwise the pupil gradient
+// is written to GP.
+enum { kBackprojPupil = 0, kBackprojAtomic = 1 };
 void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const int2* led, uint32_t led0, uint32_t b,
-                     float2* P, float2* Gs, bool do_atomics, float* mP, const float* A, RmsArgs rms, float2* GP,
+                     float2* P, float2* Gs, int mode, float* mP, const float* A, RmsArgs rms, float2* GP,
                      hipStream_t st);
 
 // owner-computes stitch: every element of Gs written as the sum, in LED order,

@@ -121,7 +121,7 @@ void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t le
 __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __restrict__ R,
                                                        const float2* __restrict__ Shat, const int2* __restrict__ led,
                                                        uint32_t led0, uint32_t b, float2* __restrict__ P,
-                                                       float2* __restrict__ Gs, int do_atomics, float* __restrict__ mP,
+                                                       float2* __restrict__ Gs, int mode, float* __restrict__ mP,
                                                        const float* __restrict__ A, RmsArgs rms,
                                                        float2* __restrict__ GP, float invN2) {
     const uint32_t plane = g.nr * g.nc;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
         const uint32_t srow = (i + (uint32_t)k.y - 1u + (g.Nr >> 1)) & (g.Nr - 1);
         const uint32_t scol = (j + (uint32_t)k.x - 1u + (g.Nc >> 1)) & (g.Nc - 1);
         const size_t spos = (size_t)g.pN(scol) * g.Nr + srow;
-        if (do_atomics) {
+        if (mode == kBackprojAtomic) {
             const float2 gw = cmulc(Rv, p);
             float* dst = (float*)(Gs + spos);
             unsafeAtomicAdd(dst, gw.x * invN2);
@@ -172,12 +172,12 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
 }
 
 void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const int2* led, uint32_t led0, uint32_t b,
-                     float2* P, float2* Gs, bool do_atomics, float* mP, const float* A, RmsArgs rms, float2* GP,
+                     float2* P, float2* Gs, int mode, float* mP, const float* A, RmsArgs rms, float2* GP,
                      hipStream_t st) {
     const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
     dim3 grid((g.plane() + 255) / 256);
-    hipLaunchKernelGGL(backproj_kernel, grid, dim3(256), 0, st, g, R, Shat, led, led0, b, P, Gs, (int)do_atomics, mP, A,
-                       rms, GP, invN2);
+    hipLaunchKernelGGL(backproj_kernel, grid, dim3(256), 0, st, g, R, Shat, led, led0, b, P, Gs, mode, mP, A, rms, GP,
+                       invN2);
     FPM_LAUNCH_CHECK();
 }
 

@@ -214,6 +214,9 @@ def main():
         if int(st.count[STAGE_NAMES.index("crop")]) == 0:
             # the crop runs inside the first inverse column pass
             alg["ifft_n_cols"] += alg["crop"]
+        if int(st.count[STAGE_NAMES.index("ifft_n_contig")]) == 0:
+            # the contiguous passes run inside the stencil kernel
+            alg["stencil"] += alg["ifft_n_contig"] + alg["fft_n_contig"]
         stages = {}
         for k, name in enumerate(STAGE_NAMES):
             cnt = int(st.count[k])

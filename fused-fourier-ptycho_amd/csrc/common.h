@@ -26,6 +26,8 @@ struct ColPerm {
 constexpr int kMaxStridedLen = FPM_MAX_STRIDED_LEN;
 // rows per strided tile (16 complex floats = 128 bytes)
 constexpr int kStridedRows = 16;
+// rows per entry of the window row masks (byte per 16 rows)
+constexpr int kMaskRows = 16;
 
 inline int ilog2_host(uint64_t v) {
     int l = 0;

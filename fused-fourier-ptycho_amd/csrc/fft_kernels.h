@@ -34,14 +34,17 @@ struct StitchArgs {
 void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bool inverse, const float2* twid,
                 hipStream_t st);
 
-// transforms along the columns index of rows x cols pages; see fft_kernels.hip
+// transforms along the columns index of rows x cols pages; see fft_kernels.hip.
+// rowmask (optional, one byte per 16 rows): row tiles whose bytes are all 0
+// are skipped -- their output is left as it was (forward: rows nobody reads;
+// inverse: rows whose input and so output are zero, masked again by the reader).
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, float2* nat_out);
+                      hipStream_t st, float2* nat_out, const uint8_t* rowmask = nullptr);
 // With crop, the input is not read: the first pass gathers ifftshift(S P)
 // of each page's LED window (natural frequency columns) instead.
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, const float2* nat_in, const CropArgs* crop = nullptr,
-                      const StitchArgs* stitch = nullptr);
+                      const StitchArgs* stitch = nullptr, const uint8_t* rowmask = nullptr);
 
 // full 2D transforms, in place; the frequency side is column-permuted by p
 void fft2_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,

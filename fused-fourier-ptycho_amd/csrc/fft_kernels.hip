@@ -73,6 +73,7 @@ struct StridedArgs {
     uint64_t page;  // elements per page
     uint32_t in_g, in_s, out_g, out_s;
     uint32_t tw_step;
+    const uint8_t* rowmask;  // optional: skip row tiles whose 16-row entries are all 0
 };
 
 // TR adjacent rows per tile: 16 (128 B per column) for long lines, more for
@@ -116,12 +117,18 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     __shared__ float2 lds[Map::LDS_WORDS];
     __shared__ float2 twl[2 * kTwLo];
     const int t = threadIdx.x;
-    tw = bind_twiddles(tw, twl, t, Cfg::T);
-    const uint32_t g = blockIdx.y;
     // the crop launches pages fastest (grid x) so the LED pages that share a
     // pupil tile run back to back and the pupil tile stays in cache
     const uint32_t tile = SRC == SRC_CROP ? blockIdx.z : blockIdx.x;
     const uint32_t page = SRC == SRC_CROP ? blockIdx.x : blockIdx.z;
+    if (a.rowmask) {  // rows outside every LED window of the batch: nothing to do
+        bool any = false;
+        for (uint32_t k = (tile * TR) / kMaskRows; k <= (tile * TR + TR - 1) / kMaskRows; ++k)
+            any |= a.rowmask[k] != 0;
+        if (!any) return;  // uniform over the workgroup
+    }
+    tw = bind_twiddles(tw, twl, t, Cfg::T);
+    const uint32_t g = blockIdx.y;
     const size_t base = (size_t)page * a.page + (size_t)tile * TR;
     const float2* __restrict__ in = a.in + base;
     float2* __restrict__ out = a.out + base;
@@ -302,11 +309,12 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // forward column transform(s): natural columns in `data` -> permuted (in
 // place) or natural (into nat_out) frequency columns
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, float2* nat_out) {
+                      hipStream_t st, float2* nat_out, const uint8_t* rowmask) {
     Twiddles tw{twid};
     StridedArgs a{};
     a.rows = rows;
     a.page = (uint64_t)rows * cols;
+    a.rowmask = rowmask;
     const uint32_t tiles = rows / kStridedRows;
     if (p.M2 == 1) {
         a.in = data;
@@ -340,11 +348,13 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
 // inverse column transform(s): permuted (or natural, from nat_in) frequency
 // columns -> natural columns in `data`
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, const float2* nat_in, const CropArgs* crop, const StitchArgs* stitch) {
+                      hipStream_t st, const float2* nat_in, const CropArgs* crop, const StitchArgs* stitch,
+                      const uint8_t* rowmask) {
     Twiddles tw{twid};
     StridedArgs a{};
     a.rows = rows;
     a.page = (uint64_t)rows * cols;
+    a.rowmask = rowmask;
     const uint32_t tiles = rows / kStridedRows;
     Gather gs{crop, stitch};
     if (stitch && (stitch->b > (uint32_t)kMaxStitchLeds || batch != 1))
@@ -406,7 +416,8 @@ template <int LEN, bool FWD>
 __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2* __restrict__ G,
                                                                         float2* __restrict__ O,
                                                                         float* __restrict__ mO, uint32_t nlines,
-                                                                        RmsArgs rms, Twiddles tw) {
+                                                                        RmsArgs rms, const uint8_t* __restrict__ rowmask,
+                                                                        Twiddles tw) {
     using Cfg = ContigCfg<LEN>;
     using Map = typename Cfg::Map;
     using SI = Slots<LEN, true, Cfg::T, Map>;
@@ -414,7 +425,11 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
     static_assert(SI::RL == SF::R0, "inverse last pass and forward first pass must share the slot map");
     __shared__ float2 lds[Map::LDS_WORDS];
     __shared__ float2 twl[2 * kTwLo];
+    // rows of the gradient known to be zero (outside every LED window of the
+    // batch): not read, taken as 0 -- the column passes skipped them
+    __shared__ uint8_t rm[LEN / kMaskRows];
     const int t = threadIdx.x;
+    for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rowmask ? rowmask[k] : 1;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
     float2 v[16];
@@ -425,7 +440,7 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
             int line, idx;
             SI::in(t, q, s, line, idx);
             const size_t gl = line0 + line;
-            v[q * SI::R0 + s] = gl < nlines ? G[gl * LEN + idx] : make_float2(0.f, 0.f);
+            v[q * SI::R0 + s] = (gl < nlines && rm[idx / kMaskRows]) ? G[gl * LEN + idx] : make_float2(0.f, 0.f);
         }
     first_pass_dft<LEN, true>(v);
     remaining_passes<LEN, true, Cfg::T, Map>(v, lds, t, tw);
@@ -467,16 +482,18 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
 }
 
 void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                         const float2* twid, hipStream_t st) {
+                         const uint8_t* rowmask, const float2* twid, hipStream_t st) {
     Twiddles tw{twid};
     const uint32_t nlines = g.Nc;
 #define FPM_FUSED_CASE(L)                                                                                   \
     case L: {                                                                                               \
         dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                                \
         if (forward_after)                                                                                  \
-            hipLaunchKernelGGL((fused_update_kernel<L, true>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, rms, tw); \
+            hipLaunchKernelGGL((fused_update_kernel<L, true>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
+                               rms, rowmask, tw);                                                           \
         else                                                                                                \
-            hipLaunchKernelGGL((fused_update_kernel<L, false>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, rms, tw); \
+            hipLaunchKernelGGL((fused_update_kernel<L, false>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
+                               rms, rowmask, tw);                                                           \
         break;                                                                                              \
     }
     switch (g.Nr) {
@@ -494,6 +511,152 @@ void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs
         default: throw HipError{hipErrorInvalidValue, "fused update length " + std::to_string(g.Nr) + " unsupported"};
     }
 #undef FPM_FUSED_CASE
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------ per-LED contiguous passes fused with the stencil
+// One workgroup owns K = LT - 2 interior columns of one LED page plus the
+// column on each side, which the stencil needs as neighbours and which are
+// recomputed here.  Steps: contiguous inverse pass of every column (z stays
+// in registers); e = |z|/(n_r n_c r^2) - y into LDS; the loss stencil of
+// cuFPM_v3/cuda/kernels.cuh:197-275 on the interior columns, u = g z/|z|;
+// the contiguous forward pass of u on the same registers (the inverse's last
+// pass and the forward's first pass share the slot map).  z and u never
+// reach HBM: 21 B per pixel instead of 50 for three separate kernels.
+template <int LEN> struct StencilCfg {
+    static constexpr int T = 1024;
+    static constexpr int LT = T * 16 / LEN;  // columns per workgroup
+    static constexpr int K = LT - 2;          // interior columns
+    using Map = LineSlowMap<LT, LEN>;
+};
+
+template <int LEN>
+__global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
+                                                            Geom g, YStack y, uint32_t led0, float ratio,
+                                                            float gscale, double* __restrict__ loss_acc,
+                                                            Twiddles tw) {
+    using Cfg = StencilCfg<LEN>;
+    using Map = typename Cfg::Map;
+    using SI = Slots<LEN, true, Cfg::T, Map>;
+    using SF = Slots<LEN, false, Cfg::T, Map>;
+    static_assert(SI::RL == SF::R0, "inverse last pass and forward first pass must share the slot map");
+    static_assert(Cfg::K >= 2, "too few interior columns per workgroup");
+    __shared__ float2 lds[Map::LDS_WORDS];
+    __shared__ float2 twl[2 * kTwLo];
+    __shared__ double red[Cfg::T / 64];
+    const int t = threadIdx.x;
+    tw = bind_twiddles(tw, twl, t, Cfg::T);
+    const uint32_t page = blockIdx.y;
+    const uint32_t j0 = blockIdx.x * Cfg::K;  // first interior column; line l is column j0 - 1 + l
+    const uint32_t cmask = g.nc - 1;
+    const size_t plane = (size_t)g.nr * g.nc;
+    const float2* __restrict__ src = in + (size_t)page * plane;
+    float2* __restrict__ dst = out + (size_t)page * plane;
+    const size_t ybase = (size_t)(led0 + page) * plane;
+    auto column = [&](int line) -> uint32_t { return (j0 - 1u + (uint32_t)line) & cmask; };
+    auto interior = [&](int line) -> bool { return line >= 1 && line <= Cfg::K && j0 + (uint32_t)line - 1u < g.nc; };
+
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16 / SI::R0; ++q)
+#pragma unroll
+        for (int s = 0; s < SI::R0; ++s) {
+            int line, idx;
+            SI::in(t, q, s, line, idx);
+            v[q * SI::R0 + s] = src[(size_t)column(line) * LEN + idx];
+        }
+    first_pass_dft<LEN, true>(v);
+    remaining_passes<LEN, true, Cfg::T, Map>(v, lds, t, tw);
+    __syncthreads();
+    // e of every column of the workgroup, in LDS (the exchange buffer as floats)
+    float* ef = (float*)lds;
+#pragma unroll
+    for (int q = 0; q < 16 / SI::RL; ++q)
+#pragma unroll
+        for (int s = 0; s < SI::RL; ++s) {
+            int line, idx;
+            SI::out(t, q, s, line, idx);
+            const float2 z = v[q * SI::RL + s];
+            ef[line * LEN + idx] = sqrtf(z.x * z.x + z.y * z.y) / ratio -
+                                   read_y(y, ybase + (size_t)column(line) * LEN + idx);
+        }
+    __syncthreads();
+    float loss32 = 0.0f;  // 32 terms per thread at most; the workgroup sum is FP64
+#pragma unroll
+    for (int q = 0; q < 16 / SI::RL; ++q)
+#pragma unroll
+        for (int s = 0; s < SI::RL; ++s) {
+            int line, idx;
+            SI::out(t, q, s, line, idx);
+            float2 o = make_float2(0.f, 0.f);
+            if (interior(line)) {
+                const float* e = ef + line * LEN;
+                const float e0 = e[idx];
+                const float el = e[idx - LEN], er = e[idx + LEN];
+                const float eu = e[(idx - 1) & (LEN - 1)], ed = e[(idx + 1) & (LEN - 1)];
+                const float gval = (copysignf(1.0f, e0 - el) - copysignf(1.0f, er - e0) + copysignf(1.0f, e0 - eu) -
+                                    copysignf(1.0f, ed - e0)) * gscale;
+                loss32 += fabsf(er - e0) + fabsf(ed - e0);
+                const float2 z = v[q * SI::RL + s];
+                const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                if (mag > 0.0f) {
+                    const float inv = 1.0f / mag;
+                    o = make_float2(z.x * inv * gval, z.y * inv * gval);
+                }
+            }
+            v[q * SI::RL + s] = o;
+        }
+    first_pass_dft<LEN, false>(v);
+    __syncthreads();
+    remaining_passes<LEN, false, Cfg::T, Map>(v, lds, t, tw);
+#pragma unroll
+    for (int q = 0; q < 16 / SF::RL; ++q)
+#pragma unroll
+        for (int s = 0; s < SF::RL; ++s) {
+            int line, idx;
+            SF::out(t, q, s, line, idx);
+            if (interior(line)) dst[(size_t)column(line) * LEN + idx] = v[q * SF::RL + s];
+        }
+    if (loss_acc) {
+        double loss = (double)loss32;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+        if ((t & 63) == 0) red[t >> 6] = loss;
+        __syncthreads();
+        if (t == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < Cfg::T / 64; ++w) sum += red[w];
+            unsafeAtomicAdd(loss_acc, sum);
+        }
+    }
+}
+
+bool stencil_fused_supported(const Geom& g) { return g.nr <= 2048; }
+
+void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
+                          float2* u_out, double* loss_acc, const float2* twid, hipStream_t st) {
+    Twiddles tw{twid};
+    const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
+    const float gscale = (float)(pratio * pratio);
+#define FPM_STF_CASE(L)                                                                                   \
+    case L: {                                                                                             \
+        dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                                   \
+        hipLaunchKernelGGL((stencil_fused_kernel<L>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, led0, ratio, \
+                           gscale, loss_acc, tw);                                                          \
+        break;                                                                                            \
+    }
+    switch (g.nr) {
+        FPM_STF_CASE(16)
+        FPM_STF_CASE(32)
+        FPM_STF_CASE(64)
+        FPM_STF_CASE(128)
+        FPM_STF_CASE(256)
+        FPM_STF_CASE(512)
+        FPM_STF_CASE(1024)
+        FPM_STF_CASE(2048)
+        default: throw HipError{hipErrorInvalidValue, "fused stencil length " + std::to_string(g.nr) + " unsupported"};
+    }
+#undef FPM_STF_CASE
     FPM_LAUNCH_CHECK();
 }
 

@@ -268,12 +268,15 @@ struct Workspace {
     DevBuf lat, rec;
 };
 
-// Shat (full spectrum, column-permuted) -> per-LED back-projection into Gs and
-// the pupil part (update or gradient output)
+// Shat (full spectrum, column-permuted) -> per-LED back-projection, the
+// pupil part (update or gradient output) and Gs after the column passes of
+// its inverse transform.  rowmask (optional): the batch's window rows; the
+// column passes then skip the other row tiles of Gs, whose contents are left
+// stale and must be read as zero (launch_fused_update does).
 void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2* led, uint32_t led0, uint32_t b,
                         const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
                         RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
-                        StageTimer& tm) {
+                        StageTimer& tm, const uint8_t* rowmask) {
     float2* lat = ws.lat.as<float2>();
     float2* rec = ws.rec.as<float2>();
     // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
@@ -281,15 +284,22 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
     fft_cols_inverse(lat, g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca);
     tm.end(col_launches(g.pn));
-    tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
-    fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st);
-    tm.end(1);
-    tm.begin(FPM_STAGE_STENCIL);
-    launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st);
-    tm.end(1);
-    tm.begin(FPM_STAGE_FFT_N_CONTIG_LO);
-    fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st);
-    tm.end(1);
+    if (stencil_fused_supported(g)) {
+        // contiguous inverse pass + stencil + contiguous forward pass in one kernel
+        tm.begin(FPM_STAGE_STENCIL);
+        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, tw, st);
+        tm.end(1);
+    } else {
+        tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
+        fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st);
+        tm.end(1);
+        tm.begin(FPM_STAGE_STENCIL);
+        launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st);
+        tm.end(1);
+        tm.begin(FPM_STAGE_FFT_N_CONTIG_LO);
+        fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st);
+        tm.end(1);
+    }
     tm.begin(FPM_STAGE_FFT_N_COLS_LO);
     fft_cols_forward(rec, g.nr, g.nc, b, g.pn, tw, st, nullptr);
     tm.end(col_launches(g.pn));
@@ -299,7 +309,7 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
         // pupil update must come after it (the stitch reads this step's pupil)
         const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.pn, 1.0f / (float)(g.Nr * g.Nc)};
         tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
-        fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa);
+        fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa, rowmask);
         tm.end(col_launches(g.pN));
         tm.begin(FPM_STAGE_BACKPROJ);
         launch_backproj(g, rec, Shat, led, led0, b, P, Gs, kBackprojPupil, mP, A, rms, GP, st);
@@ -322,7 +332,7 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
         tm.end(1);
     }
     tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
-    fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+    fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, nullptr, rowmask);
     tm.end(col_launches(g.pN));
 }
 
@@ -423,29 +433,49 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             float2* Shat = S0.as<float2>();
             float2* Gs = S1.as<float2>();
             StageTimer tm(o.stage_times, st);
-            // spectrum of the initial object
+            // Row masks of the window rows of the current and the next batch:
+            // outside them the spectrum gradient is zero and the spectrum is
+            // never read, so the N-side column passes skip those row tiles.
+            const uint32_t steps = (uint32_t)((L + B - 1) / B);
+            const size_t mrows = g.Nr / kMaskRows;
+            DevBuf masks;
+            masks.alloc(2 * mrows);
+            uint8_t* mk[2] = {masks.as<uint8_t>(), masks.as<uint8_t>() + mrows};
+            int cur = 0;
+            auto batch_of = [&](uint32_t s, uint32_t& led0, uint32_t& b) {
+                led0 = s * B;
+                b = std::min<uint32_t>(B, (uint32_t)L - led0);
+            };
+            uint32_t led0, b;
+            batch_of(0, led0, b);
+            launch_row_mask(led, led0, b, g.Nr, g.nr, mk[cur], st);
+            // spectrum of the initial object (rows of the first batch)
             tm.begin(FPM_STAGE_SPECTRUM_INIT);
             fft_contig(O, Shat, g.Nr, g.Nc, false, tw, st);
-            fft_cols_forward(Shat, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+            fft_cols_forward(Shat, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[cur]);
             tm.end(1 + col_launches(g.pN));
-            const uint32_t steps = (uint32_t)((L + B - 1) / B);
             for (int e = 0; e < ne; ++e) {
                 const RmsArgs rms{lr, beta, eps, o.pupil_clamp};
                 double* lacc = o.loss_per_epoch ? lossd.as<double>() + e : nullptr;
                 for (uint32_t s = 0; s < steps; ++s) {
-                    const uint32_t led0 = s * B;
-                    const uint32_t b = std::min<uint32_t>(B, (uint32_t)L - led0);
+                    batch_of(s, led0, b);
                     minibatch_backward(g, Shat, P, led, led0, b, y, pr, ws, Gs, o.stitch, mP.as<float>(), dA, rms,
-                                       nullptr, lacc, tw, st, tm);
+                                       nullptr, lacc, tw, st, tm, mk[cur]);
                     const bool last = (e == ne - 1) && (s == steps - 1);
+                    if (!last) {  // the next minibatch: s + 1, or the first one of the next epoch
+                        uint32_t nled0, nb;
+                        batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
+                        launch_row_mask(led, nled0, nb, g.Nr, g.nr, mk[1 - cur], st);
+                    }
                     tm.begin(FPM_STAGE_FUSED_UPDATE);
-                    launch_fused_update(g, Gs, O, mO.as<float>(), rms, !last, tw, st);
+                    launch_fused_update(g, Gs, O, mO.as<float>(), rms, !last, mk[cur], tw, st);
                     tm.end(1);
                     if (!last) {
                         tm.begin(FPM_STAGE_FFT_N_COLS_HI);
-                        fft_cols_forward(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+                        fft_cols_forward(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[1 - cur]);
                         tm.end(col_launches(g.pN));
                         std::swap(Shat, Gs);
+                        cur = 1 - cur;
                     }
                 }
                 // cuFPM_pure_v3.cu:204-206 (float lr times double 0.5)
@@ -515,7 +545,7 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         tm.end(1 + col_launches(g.pN));
         minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
                            o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
-                           loss_out ? lossd.as<double>() : nullptr, tw, st, tm);
+                           loss_out ? lossd.as<double>() : nullptr, tw, st, tm, nullptr);
         float2* GOd;
         if (isdev) {
             GOd = (float2*)G_O;

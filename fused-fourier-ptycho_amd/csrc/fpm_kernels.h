@@ -23,6 +23,16 @@ struct YStack {
     float range;       // (float)(mx - mn) (u16 only)
 };
 
+// amplitude at element `off` of the stack (uint16: (I - min)/(max - min) in
+// float, then sqrt -- the single-precision steps of main_run_FPM.m:43-45)
+__device__ __forceinline__ float read_y(const YStack& y, size_t off) {
+    if (y.u16) {
+        const uint32_t raw = ((const uint16_t*)y.data)[off];
+        return sqrtf((float)(raw - y.mn) / y.range);
+    }
+    return ((const float*)y.data)[off];
+}
+
 // RMSprop + pupil constraint constants of one step
 struct RmsArgs {
     float lr, beta, eps, clamp;
@@ -50,10 +60,24 @@ void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const i
 void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b,
                          float2* Gs, hipStream_t st);
 
+// contiguous inverse pass + loss stencil + contiguous forward pass of a batch
+// of n-side pages in one kernel (z_cols: pages after their inverse column
+// passes; u_out: pages before their forward column passes); nr <= 2048.
+bool stencil_fused_supported(const Geom& g);
+void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
+                          float2* u_out, double* loss_acc, const float2* twid, hipStream_t st);
+
 // contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
 // [+ contiguous forward pass of the new O, written back into G]
+// rowmask (optional, Nr/16 bytes): gradient rows of 0-entries are taken as zero
 void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
+                         const uint8_t* rowmask,
                          const float2* twid, hipStream_t st);
+
+// per 16-row tile of the N-side spectrum: 1 if any LED window of the batch
+// covers a row of the tile (Nr / 16 bytes)
+void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, uint8_t* mask,
+                     hipStream_t st);
 
 // fill with zeros (count complex elements)
 void launch_zero(float2* p, size_t count, hipStream_t st);

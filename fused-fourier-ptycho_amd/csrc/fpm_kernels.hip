@@ -23,14 +23,6 @@ __device__ __forceinline__ float sgnf(float x) { return copysignf(1.0f, x); }
 // the halo) and the row-contiguous accesses are coalesced.
 constexpr int kStTR = 64, kStTC = 32, kStLD = kStTR + 2;
 
-__device__ __forceinline__ float read_y(const YStack& y, size_t off) {
-    if (y.u16) {
-        const uint32_t raw = ((const uint16_t*)y.data)[off];
-        return sqrtf((float)(raw - y.mn) / y.range);
-    }
-    return ((const float*)y.data)[off];
-}
-
 __global__ __launch_bounds__(256) void stencil_kernel(Geom g, const float2* __restrict__ z, YStack y, uint32_t led0,
                                                       float ratio, float gscale, float2* __restrict__ u,
                                                       double* __restrict__ loss_acc) {
@@ -259,6 +251,31 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
     const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
     dim3 grid((g.Nr + kOwTR - 1) / kOwTR, (g.Nc + kOwTC - 1) / kOwTC);
     hipLaunchKernelGGL(owner_stitch_kernel, grid, dim3(256), 0, st, g, R, P, led, led0, b, Gs, invN2);
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------ window row masks
+// mask[t] = 1 iff rows [16t, 16t+16) of the N-side spectrum meet the row
+// window of any LED led0..led0+b-1 (rows (kt - 1 + Nr/2 + i) mod Nr, i < nr).
+// Outside those rows the spectrum gradient of the batch is exactly zero and
+// the spectrum is never read, so the column passes skip them.
+__global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ led, uint32_t led0, uint32_t b,
+                                                       uint32_t Nr, uint32_t nr, uint8_t* __restrict__ mask) {
+    const uint32_t tile = blockIdx.x * 256u + threadIdx.x;
+    if (tile >= Nr / kMaskRows) return;
+    const uint32_t Rm = Nr - 1, ra = tile * kMaskRows;
+    uint8_t hit = 0;
+    for (uint32_t l = 0; l < b; ++l) {
+        const uint32_t wr = ((uint32_t)led[led0 + l].y - 1u + (Nr >> 1)) & Rm;
+        if (((ra - wr) & Rm) < nr || ((wr - ra) & Rm) < kMaskRows) hit = 1;
+    }
+    mask[tile] = hit;
+}
+
+void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, uint8_t* mask,
+                     hipStream_t st) {
+    const uint32_t tiles = Nr / kMaskRows;
+    hipLaunchKernelGGL(row_mask_kernel, dim3((tiles + 255) / 256), dim3(256), 0, st, led, led0, b, Nr, nr, mask);
     FPM_LAUNCH_CHECK();
 }
 

@@ -168,8 +168,10 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
                 const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
                 const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
-                const float2 S = c.Shat[(size_t)c.pN(scl) * c.Nr + srw];
-                v[q * SL::R0 + s] = cmul(S, c.P[j * c.nr + i]);
+                const float2 pp = c.P[j * c.nr + i];
+                float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
+                if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(c.Shat[(size_t)c.pN(scl) * c.Nr + srw], pp);
+                v[q * SL::R0 + s] = z;
             } else if constexpr (SRC == SRC_STITCH) {
                 const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
                 const uint32_t plane = sa.nr * sa.nc;
@@ -184,10 +186,13 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                         const uint32_t li = (srow - (w & 0xffffu)) & Rm;
                         const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
                         const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
-                        const float2 Rv = sa.R[(size_t)l * plane + sa.pn(dj) * sa.nr + di];
-                        const float2 gw = cmulc(Rv, sa.P[lj * sa.nr + li]);
-                        acc.x += gw.x * sa.invN2;
-                        acc.y += gw.y * sa.invN2;
+                        const float2 pp = sa.P[lj * sa.nr + li];
+                        if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0; R not read
+                            const float2 Rv = sa.R[(size_t)l * plane + sa.pn(dj) * sa.nr + di];
+                            const float2 gw = cmulc(Rv, pp);
+                            acc.x += gw.x * sa.invN2;
+                            acc.y += gw.y * sa.invN2;
+                        }
                     }
                 }
                 v[q * SL::R0 + s] = acc;
@@ -534,7 +539,7 @@ template <int LEN>
 __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,
                                                             float gscale, double* __restrict__ loss_acc,
-                                                            Twiddles tw) {
+                                                            const uint8_t* __restrict__ rowmask, Twiddles tw) {
     using Cfg = StencilCfg<LEN>;
     using Map = typename Cfg::Map;
     using SI = Slots<LEN, true, Cfg::T, Map>;
@@ -544,7 +549,10 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
     __shared__ float2 lds[Map::LDS_WORDS];
     __shared__ float2 twl[2 * kTwLo];
     __shared__ double red[Cfg::T / 64];
+    // spectrum rows known to be zero (pupil rows without support): not read
+    __shared__ uint8_t rm[LEN / kMaskRows];
     const int t = threadIdx.x;
+    for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rowmask ? rowmask[k] : 1;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     const uint32_t page = blockIdx.y;
     const uint32_t j0 = blockIdx.x * Cfg::K;  // first interior column; line l is column j0 - 1 + l
@@ -563,7 +571,7 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
         for (int s = 0; s < SI::R0; ++s) {
             int line, idx;
             SI::in(t, q, s, line, idx);
-            v[q * SI::R0 + s] = src[(size_t)column(line) * LEN + idx];
+            v[q * SI::R0 + s] = rm[idx / kMaskRows] ? src[(size_t)column(line) * LEN + idx] : make_float2(0.f, 0.f);
         }
     first_pass_dft<LEN, true>(v);
     remaining_passes<LEN, true, Cfg::T, Map>(v, lds, t, tw);
@@ -634,7 +642,8 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
 bool stencil_fused_supported(const Geom& g) { return g.nr <= 2048; }
 
 void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
-                          float2* u_out, double* loss_acc, const float2* twid, hipStream_t st) {
+                          float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
+                          hipStream_t st) {
     Twiddles tw{twid};
     const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
     const float gscale = (float)(pratio * pratio);
@@ -642,7 +651,7 @@ void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, 
     case L: {                                                                                             \
         dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                                   \
         hipLaunchKernelGGL((stencil_fused_kernel<L>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, led0, ratio, \
-                           gscale, loss_acc, tw);                                                          \
+                           gscale, loss_acc, rowmask, tw);                                                 \
         break;                                                                                            \
     }
     switch (g.nr) {

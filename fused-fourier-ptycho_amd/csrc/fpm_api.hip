@@ -272,22 +272,32 @@ struct Workspace {
 // pupil part (update or gradient output) and Gs after the column passes of
 // its inverse transform.  rowmask (optional): the batch's window rows; the
 // column passes then skip the other row tiles of Gs, whose contents are left
-// stale and must be read as zero (launch_fused_update does).
+// stale and must be read as zero (launch_fused_update does).  pmask
+// (optional, 2 x nr/16 bytes): scratch for the pupil-support row masks of
+// the per-LED spectra; with it the per-LED column passes skip pupil rows
+// without support (update mode only: the pupil kernel then skips them too).
 void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2* led, uint32_t led0, uint32_t b,
                         const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
                         RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
-                        StageTimer& tm, const uint8_t* rowmask) {
+                        StageTimer& tm, const uint8_t* rowmask, uint8_t* pmask) {
     float2* lat = ws.lat.as<float2>();
     float2* rec = ws.rec.as<float2>();
+    const bool fused = stencil_fused_supported(g);
+    // pupil-support pruning needs the masked read of the fused stencil kernel
+    // and the skip of the pupil kernel (update mode, A given)
+    const bool prune = pmask && fused && mP && A;
+    uint8_t* lat_mask = prune ? pmask : nullptr;
+    uint8_t* rec_mask = prune ? pmask + g.nr / kMaskRows : nullptr;
+    if (prune) launch_pupil_row_mask(P, A, g.nr, g.nc, lat_mask, rec_mask, st);
     // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
     const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
     tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
-    fft_cols_inverse(lat, g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca);
+    fft_cols_inverse(lat, g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca, nullptr, lat_mask);
     tm.end(col_launches(g.pn));
-    if (stencil_fused_supported(g)) {
+    if (fused) {
         // contiguous inverse pass + stencil + contiguous forward pass in one kernel
         tm.begin(FPM_STAGE_STENCIL);
-        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, tw, st);
+        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, lat_mask, tw, st);
         tm.end(1);
     } else {
         tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
@@ -301,7 +311,7 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
         tm.end(1);
     }
     tm.begin(FPM_STAGE_FFT_N_COLS_LO);
-    fft_cols_forward(rec, g.nr, g.nc, b, g.pn, tw, st, nullptr);
+    fft_cols_forward(rec, g.nr, g.nc, b, g.pn, tw, st, nullptr, rec_mask);
     tm.end(col_launches(g.pn));
     // stitch into Gs, pupil gradient / update, then the column passes of IDFT_N(Gs)
     if (stitch == FPM_STITCH_OWNER && b <= 64) {
@@ -439,7 +449,8 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             const uint32_t steps = (uint32_t)((L + B - 1) / B);
             const size_t mrows = g.Nr / kMaskRows;
             DevBuf masks;
-            masks.alloc(2 * mrows);
+            masks.alloc(2 * mrows + 2 * (g.nr / kMaskRows));
+            uint8_t* pmask = masks.as<uint8_t>() + 2 * mrows;  // pupil-support row masks
             uint8_t* mk[2] = {masks.as<uint8_t>(), masks.as<uint8_t>() + mrows};
             int cur = 0;
             auto batch_of = [&](uint32_t s, uint32_t& led0, uint32_t& b) {
@@ -460,7 +471,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
                 for (uint32_t s = 0; s < steps; ++s) {
                     batch_of(s, led0, b);
                     minibatch_backward(g, Shat, P, led, led0, b, y, pr, ws, Gs, o.stitch, mP.as<float>(), dA, rms,
-                                       nullptr, lacc, tw, st, tm, mk[cur]);
+                                       nullptr, lacc, tw, st, tm, mk[cur], pmask);
                     const bool last = (e == ne - 1) && (s == steps - 1);
                     if (!last) {  // the next minibatch: s + 1, or the first one of the next epoch
                         uint32_t nled0, nb;
@@ -545,7 +556,7 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         tm.end(1 + col_launches(g.pN));
         minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
                            o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
-                           loss_out ? lossd.as<double>() : nullptr, tw, st, tm, nullptr);
+                           loss_out ? lossd.as<double>() : nullptr, tw, st, tm, nullptr, nullptr);
         float2* GOd;
         if (isdev) {
             GOd = (float2*)G_O;

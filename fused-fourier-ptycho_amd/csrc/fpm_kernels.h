@@ -63,9 +63,11 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 // contiguous inverse pass + loss stencil + contiguous forward pass of a batch
 // of n-side pages in one kernel (z_cols: pages after their inverse column
 // passes; u_out: pages before their forward column passes); nr <= 2048.
+// rowmask (optional, nr/16 bytes): input rows of 0-entries are taken as zero.
 bool stencil_fused_supported(const Geom& g);
 void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
-                          float2* u_out, double* loss_acc, const float2* twid, hipStream_t st);
+                          float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
+                          hipStream_t st);
 
 // contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
 // [+ contiguous forward pass of the new O, written back into G]
@@ -78,6 +80,11 @@ void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs
 // covers a row of the tile (Nr / 16 bytes)
 void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, uint8_t* mask,
                      hipStream_t st);
+
+// row masks (nr/16 bytes each) of the per-LED spectra from the pupil support:
+// lat -- rows where S P can be nonzero; rec -- rows where R is read
+void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* lat, uint8_t* rec,
+                           hipStream_t st);
 
 // fill with zeros (count complex elements)
 void launch_zero(float2* p, size_t count, hipStream_t st);

@@ -123,8 +123,14 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
     const uint32_t di = (i + (g.nr >> 1)) & (g.nr - 1), dj = (j + (g.nc >> 1)) & (g.nc - 1);
     const uint32_t rpos = g.pn(dj) * g.nr + di;
     const float2 p = P[pix];
+    const bool p_zero = p.x == 0.0f && p.y == 0.0f;
+    // Where P = 0 the stitch contribution R conj(P) is zero, and where also
+    // A = 0 the clamped pupil update below is zero whatever the gradient: the
+    // spectra R are then not read (the forward column passes may have skipped
+    // those rows).  The gradient-only call (mP == nullptr) computes every pixel.
+    const bool skip = mP && p_zero && A[pix] == 0.0f;
     float2 gp = make_float2(0.f, 0.f);
-    for (uint32_t ell = 0; ell < b; ++ell) {
+    for (uint32_t ell = 0; ell < (skip ? 0u : b); ++ell) {
         const size_t page = (size_t)ell * plane;
         const float2 Rv = R[page + rpos];
         // the window element: where S was cropped from and where the
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
         const uint32_t srow = (i + (uint32_t)k.y - 1u + (g.Nr >> 1)) & (g.Nr - 1);
         const uint32_t scol = (j + (uint32_t)k.x - 1u + (g.Nc >> 1)) & (g.Nc - 1);
         const size_t spos = (size_t)g.pN(scol) * g.Nr + srow;
-        if (mode == kBackprojAtomic) {
+        if (mode == kBackprojAtomic && !p_zero) {
             const float2 gw = cmulc(Rv, p);
             float* dst = (float*)(Gs + spos);
             unsafeAtomicAdd(dst, gw.x * invN2);
@@ -227,10 +233,13 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
                     if (li < g.nr && lj < g.nc) {
                         const uint32_t di = (li + (g.nr >> 1)) & (g.nr - 1);
                         const uint32_t dj = (lj + (g.nc >> 1)) & (g.nc - 1);
-                        const float2 Rv = R[(size_t)ell * plane + g.pn(dj) * g.nr + di];
-                        const float2 gw = cmulc(Rv, P[lj * g.nr + li]);
-                        acc[q].x += gw.x * invN2;
-                        acc[q].y += gw.y * invN2;
+                        const float2 pp = P[lj * g.nr + li];
+                        if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0
+                            const float2 Rv = R[(size_t)ell * plane + g.pn(dj) * g.nr + di];
+                            const float2 gw = cmulc(Rv, pp);
+                            acc[q].x += gw.x * invN2;
+                            acc[q].y += gw.y * invN2;
+                        }
                     }
                 }
             }
@@ -276,6 +285,44 @@ void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, ui
                      hipStream_t st) {
     const uint32_t tiles = Nr / kMaskRows;
     hipLaunchKernelGGL(row_mask_kernel, dim3((tiles + 255) / 256), dim3(256), 0, st, led, led0, b, Nr, nr, mask);
+    FPM_LAUNCH_CHECK();
+}
+
+// Row masks of the per-LED spectra from the pupil support.  The spectra are
+// ifftshifted: spectrum row di holds pupil row i = (di + nr/2) mod nr.
+//   lat[t] = 1 iff pupil rows of spectrum rows [16t, 16t+16) hold a P != 0:
+//            elsewhere S P is zero, so the inverse column passes skip them;
+//   rec[t] = 1 iff they hold a P != 0 or an A != 0: the back-projection only
+//            reads R there (P = 0 contributes nothing to the stitch, and
+//            where A = 0 the clamped pupil update is 0 whatever the gradient),
+//            so the forward column passes skip the rest.
+__global__ __launch_bounds__(256) void pupil_row_mask_kernel(const float2* __restrict__ P, const float* __restrict__ A,
+                                                             uint32_t nr, uint32_t nc, uint8_t* __restrict__ lat,
+                                                             uint8_t* __restrict__ rec) {
+    __shared__ int any_p, any_a;
+    const uint32_t t = threadIdx.x, tile = blockIdx.x;
+    if (t == 0) any_p = any_a = 0;
+    __syncthreads();
+    int hp = 0, ha = 0;
+    for (uint32_t k = t; k < kMaskRows * nc; k += 256) {
+        const uint32_t di = tile * kMaskRows + (k % kMaskRows), j = k / kMaskRows;
+        const uint32_t i = (di + (nr >> 1)) & (nr - 1);
+        const float2 p = P[(size_t)j * nr + i];
+        hp |= (p.x != 0.0f) | (p.y != 0.0f);
+        ha |= A[(size_t)j * nr + i] != 0.0f;
+    }
+    if (hp) atomicOr(&any_p, 1);
+    if (ha) atomicOr(&any_a, 1);
+    __syncthreads();
+    if (t == 0) {
+        lat[tile] = (uint8_t)any_p;
+        rec[tile] = (uint8_t)(any_p | any_a);
+    }
+}
+
+void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* lat, uint8_t* rec,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(pupil_row_mask_kernel, dim3(nr / kMaskRows), dim3(256), 0, st, P, A, nr, nc, lat, rec);
     FPM_LAUNCH_CHECK();
 }
 

@@ -156,50 +156,68 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
         for (uint32_t l = 0; l < sa.b; ++l)
             if (((srow - (win[l] & 0xffffu)) & Rm) < sa.nr) rowhits |= 1ull << l;
     }
+    if constexpr (SRC == SRC_STITCH) {
+        // A thread's slots of butterfly q sit at columns col0 + s*D (s < R0),
+        // D = (LEN/R0) * in_s: loop over the LEDs covering the thread's row
+        // (in LED order) and, per LED, test each slot's column against the
+        // window -- the LED's row offset and page are computed once.
+        const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
+        const uint32_t plane = sa.nr * sa.nc;
+        constexpr uint32_t NB0 = LEN / SL::R0;
 #pragma unroll
-    for (int q = 0; q < 16 / SL::R0; ++q)
+        for (int k = 0; k < 16; ++k) v[k] = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int s = 0; s < SL::R0; ++s) {
-            int line, idx;
-            SL::in(t, q, s, line, idx);
-            const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
-            if constexpr (SRC == SRC_CROP) {
-                const uint32_t di = tile * TR + line, dj = (uint32_t)col;
-                const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
-                const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
-                const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
-                const float2 pp = c.P[j * c.nr + i];
-                float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
-                if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(c.Shat[(size_t)c.pN(scl) * c.Nr + srw], pp);
-                v[q * SL::R0 + s] = z;
-            } else if constexpr (SRC == SRC_STITCH) {
-                const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
-                const uint32_t plane = sa.nr * sa.nc;
-                float2 acc = make_float2(0.f, 0.f);
-                uint64_t m = rowhits;
-                while (m) {
-                    const int l = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
-                    const uint32_t w = win[l];
-                    const uint32_t lj = ((uint32_t)col - (w >> 16)) & Cm;
+        for (int q = 0; q < 16 / SL::R0; ++q) {
+            int line, idx0;
+            SL::in(t, q, 0, line, idx0);
+            const uint32_t col0 = g * a.in_g + (uint32_t)idx0 * a.in_s;
+            const uint32_t D = NB0 * a.in_s;
+            uint64_t m = rowhits;
+            while (m) {
+                const int l = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const uint32_t w = win[l];
+                const uint32_t li = (srow - (w & 0xffffu)) & Rm;
+                const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
+                const uint32_t d = (col0 - (w >> 16)) & Cm;
+                const float2* __restrict__ Rl = sa.R + (size_t)l * plane + di;
+#pragma unroll
+                for (int s = 0; s < SL::R0; ++s) {
+                    const uint32_t lj = (d + (uint32_t)s * D) & Cm;
                     if (lj < sa.nc) {
-                        const uint32_t li = (srow - (w & 0xffffu)) & Rm;
-                        const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
-                        const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
                         const float2 pp = sa.P[lj * sa.nr + li];
                         if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0; R not read
-                            const float2 Rv = sa.R[(size_t)l * plane + sa.pn(dj) * sa.nr + di];
-                            const float2 gw = cmulc(Rv, pp);
-                            acc.x += gw.x * sa.invN2;
-                            acc.y += gw.y * sa.invN2;
+                            const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
+                            const float2 gw = cmulc(Rl[sa.pn(dj) * sa.nr], pp);
+                            v[q * SL::R0 + s].x += gw.x * sa.invN2;
+                            v[q * SL::R0 + s].y += gw.y * sa.invN2;
                         }
                     }
                 }
-                v[q * SL::R0 + s] = acc;
-            } else {
-                v[q * SL::R0 + s] = in[col * a.rows + line];
             }
         }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16 / SL::R0; ++q)
+#pragma unroll
+            for (int s = 0; s < SL::R0; ++s) {
+                int line, idx;
+                SL::in(t, q, s, line, idx);
+                const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
+                if constexpr (SRC == SRC_CROP) {
+                    const uint32_t di = tile * TR + line, dj = (uint32_t)col;
+                    const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
+                    const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
+                    const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
+                    const float2 pp = c.P[j * c.nr + i];
+                    float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
+                    if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(c.Shat[(size_t)c.pN(scl) * c.Nr + srw], pp);
+                    v[q * SL::R0 + s] = z;
+                } else {
+                    v[q * SL::R0 + s] = in[col * a.rows + line];
+                }
+            }
+    }
     first_pass_dft<LEN, INV>(v);
     remaining_passes<LEN, INV, Cfg::T, Map>(v, lds, t, tw);
 #pragma unroll

@@ -273,22 +273,21 @@ struct Workspace {
 // its inverse transform.  rowmask (optional): the batch's window rows; the
 // column passes then skip the other row tiles of Gs, whose contents are left
 // stale and must be read as zero (launch_fused_update does).  pmask
-// (optional, 2 x nr/16 bytes): scratch for the pupil-support row masks of
-// the per-LED spectra; with it the per-LED column passes skip pupil rows
+// (optional, nr/16 bytes): the pupil-support row mask of the per-LED spectra
+// (launch_pupil_row_mask); with it the per-LED column passes skip pupil rows
 // without support (update mode only: the pupil kernel then skips them too).
 void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2* led, uint32_t led0, uint32_t b,
                         const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
                         RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
-                        StageTimer& tm, const uint8_t* rowmask, uint8_t* pmask) {
+                        StageTimer& tm, const uint8_t* rowmask, const uint8_t* pmask) {
     float2* lat = ws.lat.as<float2>();
     float2* rec = ws.rec.as<float2>();
     const bool fused = stencil_fused_supported(g);
     // pupil-support pruning needs the masked read of the fused stencil kernel
     // and the skip of the pupil kernel (update mode, A given)
     const bool prune = pmask && fused && mP && A;
-    uint8_t* lat_mask = prune ? pmask : nullptr;
-    uint8_t* rec_mask = prune ? pmask + g.nr / kMaskRows : nullptr;
-    if (prune) launch_pupil_row_mask(P, A, g.nr, g.nc, lat_mask, rec_mask, st);
+    const uint8_t* lat_mask = prune ? pmask : nullptr;
+    const uint8_t* rec_mask = prune ? pmask : nullptr;
     // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
     const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
     tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
@@ -449,9 +448,12 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             const uint32_t steps = (uint32_t)((L + B - 1) / B);
             const size_t mrows = g.Nr / kMaskRows;
             DevBuf masks;
-            masks.alloc(2 * mrows + 2 * (g.nr / kMaskRows));
-            uint8_t* pmask = masks.as<uint8_t>() + 2 * mrows;  // pupil-support row masks
+            masks.alloc(2 * mrows + g.nr / kMaskRows);
+            uint8_t* pmask = masks.as<uint8_t>() + 2 * mrows;  // pupil-support row mask
             uint8_t* mk[2] = {masks.as<uint8_t>(), masks.as<uint8_t>() + mrows};
+            // rows where P != 0 or A != 0: the update keeps P = 0 wherever
+            // A = 0, so the mask of the initial pupil holds for every step
+            launch_pupil_row_mask(P, dA, g.nr, g.nc, pmask, st);
             int cur = 0;
             auto batch_of = [&](uint32_t s, uint32_t& led0, uint32_t& b) {
                 led0 = s * B;

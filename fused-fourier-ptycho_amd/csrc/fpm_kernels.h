@@ -81,10 +81,10 @@ void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs
 void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, uint8_t* mask,
                      hipStream_t st);
 
-// row masks (nr/16 bytes each) of the per-LED spectra from the pupil support:
-// lat -- rows where S P can be nonzero; rec -- rows where R is read
-void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* lat, uint8_t* rec,
-                           hipStream_t st);
+// row mask (nr/16 bytes) of the per-LED spectra from the pupil support: the
+// rows where P != 0 or A != 0 -- the only rows where S P can be nonzero and
+// where R is read; valid for every step of a solve with the clamped update
+void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, hipStream_t st);
 
 // fill with zeros (count complex elements)
 void launch_zero(float2* p, size_t count, hipStream_t st);

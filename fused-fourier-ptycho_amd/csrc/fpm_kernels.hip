@@ -288,41 +288,35 @@ void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, ui
     FPM_LAUNCH_CHECK();
 }
 
-// Row masks of the per-LED spectra from the pupil support.  The spectra are
+// Row mask of the per-LED spectra from the pupil support.  The spectra are
 // ifftshifted: spectrum row di holds pupil row i = (di + nr/2) mod nr.
-//   lat[t] = 1 iff pupil rows of spectrum rows [16t, 16t+16) hold a P != 0:
-//            elsewhere S P is zero, so the inverse column passes skip them;
-//   rec[t] = 1 iff they hold a P != 0 or an A != 0: the back-projection only
-//            reads R there (P = 0 contributes nothing to the stitch, and
-//            where A = 0 the clamped pupil update is 0 whatever the gradient),
-//            so the forward column passes skip the rest.
-__global__ __launch_bounds__(256) void pupil_row_mask_kernel(const float2* __restrict__ P, const float* __restrict__ A,
-                                                             uint32_t nr, uint32_t nc, uint8_t* __restrict__ lat,
-                                                             uint8_t* __restrict__ rec) {
-    __shared__ int any_p, any_a;
+// mask[t] = 1 iff pupil rows of spectrum rows [16t, 16t+16) hold a P != 0 or
+// an A != 0.  Outside them S P is zero (the inverse column passes skip those
+// rows) and R is never read (P = 0 contributes nothing to the stitch, and
+// where A = 0 the clamped pupil update is 0 whatever the gradient, so the
+// forward column passes skip them too).  Since every update leaves P = 0
+// wherever A = 0, the mask of the initial pupil holds for every later step.
+__global__ __launch_bounds__(1024) void pupil_row_mask_kernel(const float2* __restrict__ P,
+                                                              const float* __restrict__ A, uint32_t nr, uint32_t nc,
+                                                              uint8_t* __restrict__ mask) {
+    __shared__ int any;
     const uint32_t t = threadIdx.x, tile = blockIdx.x;
-    if (t == 0) any_p = any_a = 0;
+    if (t == 0) any = 0;
     __syncthreads();
-    int hp = 0, ha = 0;
-    for (uint32_t k = t; k < kMaskRows * nc; k += 256) {
+    int h = 0;
+    for (uint32_t k = t; k < kMaskRows * nc; k += 1024) {
         const uint32_t di = tile * kMaskRows + (k % kMaskRows), j = k / kMaskRows;
         const uint32_t i = (di + (nr >> 1)) & (nr - 1);
         const float2 p = P[(size_t)j * nr + i];
-        hp |= (p.x != 0.0f) | (p.y != 0.0f);
-        ha |= A[(size_t)j * nr + i] != 0.0f;
+        h |= (p.x != 0.0f) | (p.y != 0.0f) | (A[(size_t)j * nr + i] != 0.0f);
     }
-    if (hp) atomicOr(&any_p, 1);
-    if (ha) atomicOr(&any_a, 1);
+    if (h) atomicOr(&any, 1);
     __syncthreads();
-    if (t == 0) {
-        lat[tile] = (uint8_t)any_p;
-        rec[tile] = (uint8_t)(any_p | any_a);
-    }
+    if (t == 0) mask[tile] = (uint8_t)any;
 }
 
-void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* lat, uint8_t* rec,
-                           hipStream_t st) {
-    hipLaunchKernelGGL(pupil_row_mask_kernel, dim3(nr / kMaskRows), dim3(256), 0, st, P, A, nr, nc, lat, rec);
+void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, hipStream_t st) {
+    hipLaunchKernelGGL(pupil_row_mask_kernel, dim3(nr / kMaskRows), dim3(1024), 0, st, P, A, nr, nc, mask);
     FPM_LAUNCH_CHECK();
 }
 

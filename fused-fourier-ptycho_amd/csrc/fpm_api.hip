@@ -448,12 +448,13 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             const uint32_t steps = (uint32_t)((L + B - 1) / B);
             const size_t mrows = g.Nr / kMaskRows;
             DevBuf masks;
-            masks.alloc(2 * mrows + g.nr / kMaskRows);
-            uint8_t* pmask = masks.as<uint8_t>() + 2 * mrows;  // pupil-support row mask
+            masks.alloc(2 * mrows + g.nr / kMaskRows + g.nr);
+            uint8_t* pmask = masks.as<uint8_t>() + 2 * mrows;  // pupil-support row mask (n side)
+            uint8_t* psup = pmask + g.nr / kMaskRows;          // pupil-support rows
             uint8_t* mk[2] = {masks.as<uint8_t>(), masks.as<uint8_t>() + mrows};
             // rows where P != 0 or A != 0: the update keeps P = 0 wherever
-            // A = 0, so the mask of the initial pupil holds for every step
-            launch_pupil_row_mask(P, dA, g.nr, g.nc, pmask, st);
+            // A = 0, so the masks of the initial pupil hold for every step
+            launch_pupil_row_mask(P, dA, g.nr, g.nc, pmask, psup, st);
             int cur = 0;
             auto batch_of = [&](uint32_t s, uint32_t& led0, uint32_t& b) {
                 led0 = s * B;
@@ -461,7 +462,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             };
             uint32_t led0, b;
             batch_of(0, led0, b);
-            launch_row_mask(led, led0, b, g.Nr, g.nr, mk[cur], st);
+            launch_row_mask(led, led0, b, g.Nr, g.nr, psup, mk[cur], st);
             // spectrum of the initial object (rows of the first batch)
             tm.begin(FPM_STAGE_SPECTRUM_INIT);
             fft_contig(O, Shat, g.Nr, g.Nc, false, tw, st);
@@ -478,7 +479,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
                     if (!last) {  // the next minibatch: s + 1, or the first one of the next epoch
                         uint32_t nled0, nb;
                         batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
-                        launch_row_mask(led, nled0, nb, g.Nr, g.nr, mk[1 - cur], st);
+                        launch_row_mask(led, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st);
                     }
                     tm.begin(FPM_STAGE_FUSED_UPDATE);
                     launch_fused_update(g, Gs, O, mO.as<float>(), rms, !last, mk[cur], tw, st);

@@ -76,15 +76,18 @@ void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs
                          const uint8_t* rowmask,
                          const float2* twid, hipStream_t st);
 
-// per 16-row tile of the N-side spectrum: 1 if any LED window of the batch
-// covers a row of the tile (Nr / 16 bytes)
-void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, uint8_t* mask,
-                     hipStream_t st);
+// per 16-row tile of the N-side spectrum: 1 if a LED window of the batch
+// covers a row of the tile at a pupil row i with sup[i] != 0 (sup: nr bytes,
+// or nullptr for every pupil row); Nr / 16 bytes
+void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, const uint8_t* sup,
+                     uint8_t* mask, hipStream_t st);
 
-// row mask (nr/16 bytes) of the per-LED spectra from the pupil support: the
-// rows where P != 0 or A != 0 -- the only rows where S P can be nonzero and
-// where R is read; valid for every step of a solve with the clamped update
-void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, hipStream_t st);
+// pupil support: sup[i] (nr bytes) = 1 iff pupil row i holds P != 0 or A != 0;
+// mask (nr/16 bytes): the same per 16-row tile of the ifftshifted per-LED
+// spectra -- the only rows where S P can be nonzero and where R is read.
+// Valid for every step of a solve with the clamped update.
+void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, uint8_t* sup,
+                           hipStream_t st);
 
 // fill with zeros (count complex elements)
 void launch_zero(float2* p, size_t count, hipStream_t st);

@@ -264,59 +264,75 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 }
 
 // ------------------------------------------------------ window row masks
-// mask[t] = 1 iff rows [16t, 16t+16) of the N-side spectrum meet the row
-// window of any LED led0..led0+b-1 (rows (kt - 1 + Nr/2 + i) mod Nr, i < nr).
-// Outside those rows the spectrum gradient of the batch is exactly zero and
-// the spectrum is never read, so the column passes skip them.
+// mask[t] = 1 iff rows [16t, 16t+16) of the N-side spectrum meet the window
+// of any LED led0..led0+b-1 at a pupil row i with sup[i] != 0 (window row
+// (kt - 1 + Nr/2 + i) mod Nr, i < nr; sup == nullptr: every pupil row).
+// Outside those rows the spectrum gradient of the batch is exactly zero (the
+// windows' P rows there are all 0) and the spectrum is never read (the crop
+// and the pupil kernel read it only where P != 0 or A != 0), so the column
+// passes skip them.
 __global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ led, uint32_t led0, uint32_t b,
-                                                       uint32_t Nr, uint32_t nr, uint8_t* __restrict__ mask) {
+                                                       uint32_t Nr, uint32_t nr, const uint8_t* __restrict__ sup,
+                                                       uint8_t* __restrict__ mask) {
     const uint32_t tile = blockIdx.x * 256u + threadIdx.x;
     if (tile >= Nr / kMaskRows) return;
     const uint32_t Rm = Nr - 1, ra = tile * kMaskRows;
     uint8_t hit = 0;
-    for (uint32_t l = 0; l < b; ++l) {
+    for (uint32_t l = 0; l < b && !hit; ++l) {
         const uint32_t wr = ((uint32_t)led[led0 + l].y - 1u + (Nr >> 1)) & Rm;
-        if (((ra - wr) & Rm) < nr || ((wr - ra) & Rm) < kMaskRows) hit = 1;
+        for (uint32_t r = 0; r < kMaskRows; ++r) {
+            const uint32_t i = (ra + r - wr) & Rm;
+            if (i < nr && (!sup || sup[i])) hit = 1;
+        }
     }
     mask[tile] = hit;
 }
 
-void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, uint8_t* mask,
-                     hipStream_t st) {
+void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, const uint8_t* sup,
+                     uint8_t* mask, hipStream_t st) {
     const uint32_t tiles = Nr / kMaskRows;
-    hipLaunchKernelGGL(row_mask_kernel, dim3((tiles + 255) / 256), dim3(256), 0, st, led, led0, b, Nr, nr, mask);
+    hipLaunchKernelGGL(row_mask_kernel, dim3((tiles + 255) / 256), dim3(256), 0, st, led, led0, b, Nr, nr, sup,
+                       mask);
     FPM_LAUNCH_CHECK();
 }
 
-// Row mask of the per-LED spectra from the pupil support.  The spectra are
-// ifftshifted: spectrum row di holds pupil row i = (di + nr/2) mod nr.
-// mask[t] = 1 iff pupil rows of spectrum rows [16t, 16t+16) hold a P != 0 or
-// an A != 0.  Outside them S P is zero (the inverse column passes skip those
-// rows) and R is never read (P = 0 contributes nothing to the stitch, and
-// where A = 0 the clamped pupil update is 0 whatever the gradient, so the
-// forward column passes skip them too).  Since every update leaves P = 0
-// wherever A = 0, the mask of the initial pupil holds for every later step.
+// Row masks from the pupil support.  sup[i] = 1 iff pupil row i holds a
+// P != 0 or an A != 0.  The per-LED spectra are ifftshifted (spectrum row di
+// holds pupil row i = (di + nr/2) mod nr): mask[t] = 1 iff spectrum rows
+// [16t, 16t+16) hold a pupil row with sup = 1.  Outside them S P is zero (the
+// inverse column passes skip those rows) and R is never read (P = 0
+// contributes nothing to the stitch, and where A = 0 the clamped pupil update
+// is 0 whatever the gradient, so the forward column passes skip them too).
+// Since every update leaves P = 0 wherever A = 0, the masks of the initial
+// pupil hold for every later step.
 __global__ __launch_bounds__(1024) void pupil_row_mask_kernel(const float2* __restrict__ P,
                                                               const float* __restrict__ A, uint32_t nr, uint32_t nc,
-                                                              uint8_t* __restrict__ mask) {
-    __shared__ int any;
+                                                              uint8_t* __restrict__ mask, uint8_t* __restrict__ sup) {
+    __shared__ int any[kMaskRows];
     const uint32_t t = threadIdx.x, tile = blockIdx.x;
-    if (t == 0) any = 0;
+    if (t < kMaskRows) any[t] = 0;
     __syncthreads();
+    // 1024 is a multiple of 16: every thread stays on spectrum row t mod 16
+    const uint32_t di = tile * kMaskRows + t % kMaskRows;
+    const uint32_t i = (di + (nr >> 1)) & (nr - 1);
     int h = 0;
-    for (uint32_t k = t; k < kMaskRows * nc; k += 1024) {
-        const uint32_t di = tile * kMaskRows + (k % kMaskRows), j = k / kMaskRows;
-        const uint32_t i = (di + (nr >> 1)) & (nr - 1);
+    for (uint32_t j = t / kMaskRows; j < nc; j += 1024 / kMaskRows) {
         const float2 p = P[(size_t)j * nr + i];
         h |= (p.x != 0.0f) | (p.y != 0.0f) | (A[(size_t)j * nr + i] != 0.0f);
     }
-    if (h) atomicOr(&any, 1);
+    if (h) atomicOr(&any[t % kMaskRows], 1);
     __syncthreads();
-    if (t == 0) mask[tile] = (uint8_t)any;
+    if (t < kMaskRows) sup[(tile * kMaskRows + t + (nr >> 1)) & (nr - 1)] = (uint8_t)any[t];
+    if (t == 0) {
+        int m = 0;
+        for (int r = 0; r < kMaskRows; ++r) m |= any[r];
+        mask[tile] = (uint8_t)m;
+    }
 }
 
-void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, hipStream_t st) {
-    hipLaunchKernelGGL(pupil_row_mask_kernel, dim3(nr / kMaskRows), dim3(1024), 0, st, P, A, nr, nc, mask);
+void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, uint8_t* sup,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(pupil_row_mask_kernel, dim3(nr / kMaskRows), dim3(1024), 0, st, P, A, nr, nc, mask, sup);
     FPM_LAUNCH_CHECK();
 }
 

@@ -50,6 +50,55 @@ inline ColPerm make_colperm(uint32_t C) {
     return p;
 }
 
+// Layout of a stack of rows x cols pages: element (r, c) of a page at
+//   (r >> sh) * tstride + perm(c) * ld + (r & rmask)
+//   column-major: sh = 31, tstride = 0, ld = rows, rmask = ~0   (r < 2^31)
+//   8-row tiles:  sh = 3, tstride = 8 cols, ld = 8, rmask = 7, perm = identity
+// The tiles keep the 8 rows x cols block of a one-pass column transform
+// contiguous in HBM (a 2048-point column pass of 8 rows streams 128 KiB),
+// while a pass down one column still reads 64-byte runs.
+struct PageLayout {
+    uint32_t sh, rmask, ld;
+    uint64_t tstride;
+    ColPerm perm;
+    __host__ __device__ __forceinline__ size_t row_off(uint32_t r) const {
+        return (size_t)(r >> sh) * tstride + (r & rmask);
+    }
+    __host__ __device__ __forceinline__ size_t off(uint32_t r, uint32_t c) const {
+        return row_off(r) + (size_t)perm(c) * ld;
+    }
+    __host__ __device__ __forceinline__ bool tiled() const { return sh != 31; }
+};
+constexpr int kTileRows = 8;
+// per-LED minibatch pages in 8-row tiles where possible (0: always column-major)
+#ifndef FPM_TILED_PAGES
+#define FPM_TILED_PAGES 1
+#endif
+constexpr bool kTiledPages = FPM_TILED_PAGES != 0;
+
+inline PageLayout colmajor_layout(uint32_t rows, const ColPerm& p) {
+    PageLayout l;
+    l.sh = 31;
+    l.rmask = 0xffffffffu;
+    l.ld = rows;
+    l.tstride = 0;
+    l.perm = p;
+    return l;
+}
+
+inline PageLayout tiled_layout(uint32_t cols) {
+    PageLayout l;
+    l.sh = 3;
+    l.rmask = kTileRows - 1;
+    l.ld = kTileRows;
+    l.tstride = (uint64_t)kTileRows * cols;
+    l.perm.C = cols;
+    l.perm.M1 = cols;
+    l.perm.M2 = 1;
+    l.perm.log2M1 = ilog2_host(cols);
+    return l;
+}
+
 // error plumbing (thread-local last error text)
 void set_error(const std::string& msg);
 const char* get_error();

@@ -26,7 +26,7 @@ struct StitchArgs {
     const int2* led;
     uint32_t led0, b;
     uint32_t nr, nc, Nr, Nc;
-    ColPerm pn;
+    PageLayout nl;  // layout of the R pages
     float invN2;
 };
 
@@ -38,13 +38,17 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // rowmask (optional, one byte per 16 rows): row tiles whose bytes are all 0
 // are skipped -- their output is left as it was (forward: rows nobody reads;
 // inverse: rows whose input and so output are zero, masked again by the reader).
+// lay (optional): the pages' layout (default column-major); a tiled layout
+// needs p = identity (one pass, cols <= 2048).
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, float2* nat_out, const uint8_t* rowmask = nullptr);
+                      hipStream_t st, float2* nat_out, const uint8_t* rowmask = nullptr,
+                      const PageLayout* lay = nullptr);
 // With crop, the input is not read: the first pass gathers ifftshift(S P)
 // of each page's LED window (natural frequency columns) instead.
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, const float2* nat_in, const CropArgs* crop = nullptr,
-                      const StitchArgs* stitch = nullptr, const uint8_t* rowmask = nullptr);
+                      const StitchArgs* stitch = nullptr, const uint8_t* rowmask = nullptr,
+                      const PageLayout* lay = nullptr);
 
 // full 2D transforms, in place; the frequency side is column-permuted by p
 void fft2_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,

@@ -144,6 +144,9 @@ Geom make_geom(int64_t Nr, int64_t Nc, int64_t nr, int64_t nc) {
     g.lnr = ilog2_host(g.nr);
     g.pn = make_colperm(g.nc);
     g.pN = make_colperm(g.Nc);
+    // per-LED pages of a minibatch: 8-row tiles when a column transform fits
+    // one pass and the stencil is fused (both sides <= 2048), else column-major
+    g.nl = (kTiledPages && g.nr <= 2048 && g.nc <= 2048) ? tiled_layout(g.nc) : colmajor_layout(g.nr, g.pn);
     return g;
 }
 
@@ -291,8 +294,8 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
     const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
     tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
-    fft_cols_inverse(lat, g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca, nullptr, lat_mask);
-    tm.end(col_launches(g.pn));
+    fft_cols_inverse(lat, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, &ca, nullptr, lat_mask, &g.nl);
+    tm.end(col_launches(g.nl.perm));
     if (fused) {
         // contiguous inverse pass + stencil + contiguous forward pass in one kernel
         tm.begin(FPM_STAGE_STENCIL);
@@ -310,13 +313,13 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
         tm.end(1);
     }
     tm.begin(FPM_STAGE_FFT_N_COLS_LO);
-    fft_cols_forward(rec, g.nr, g.nc, b, g.pn, tw, st, nullptr, rec_mask);
-    tm.end(col_launches(g.pn));
+    fft_cols_forward(rec, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, rec_mask, &g.nl);
+    tm.end(col_launches(g.nl.perm));
     // stitch into Gs, pupil gradient / update, then the column passes of IDFT_N(Gs)
     if (stitch == FPM_STITCH_OWNER && b <= 64) {
         // owner-computes stitch inside the first column pass of IDFT_N; the
         // pupil update must come after it (the stitch reads this step's pupil)
-        const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.pn, 1.0f / (float)(g.Nr * g.Nc)};
+        const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.nl, 1.0f / (float)(g.Nr * g.Nc)};
         tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
         fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa, rowmask);
         tm.end(col_launches(g.pN));

@@ -9,8 +9,9 @@ struct Geom {
     uint32_t nr, nc;   // low-res rows / cols
     uint32_t Nr, Nc;   // high-res rows / cols
     uint32_t lnr;      // log2 nr
-    ColPerm pn;        // column permutation of the n-side spectra
+    ColPerm pn;        // column permutation of n-side spectra (column-major pages)
     ColPerm pN;        // column permutation of the N-side spectrum
+    PageLayout nl;     // layout of the per-LED pages of a minibatch (tiled or column-major with pn)
     uint32_t plane() const { return nr * nc; }
 };
 

@@ -1,10 +1,13 @@
 // fpm_kernels.hip -- FPM hot-path kernels for gfx950 (crop x pupil, loss
 // stencil, back-projection + stitch, pupil update, helpers).
 //
-// Layouts: column-major pages; pixel (i, j) of an n-side page at j*nr + i.
-// Spectra (object spectrum, per-LED spectra) are stored column-permuted by the
-// FFT's ColPerm; the fftshift of cuFPM_v3 is folded into the index maps
-// (k + M/2) mod M, valid for even M (cuFPM_v3/cuda/kernels.cuh:45-47,62-64,92-93,109-111).
+// Layouts: caller arrays (object, pupil, images) are column-major, pixel
+// (i, j) at j*rows + i.  The object spectrum is column-major and stored
+// column-permuted by the FFT's ColPerm; the per-LED pages of a minibatch use
+// Geom::nl (8-row tiles, natural columns, when both sides are <= 2048;
+// column-major with ColPerm pn otherwise).  The fftshift of cuFPM_v3 is folded
+// into the index maps (k + M/2) mod M, valid for even M
+// (cuFPM_v3/cuda/kernels.cuh:45-47,62-64,92-93,109-111).
 #include "fft_core.h"
 #include "common.h"
 #include "fpm_kernels.h"
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
     if (pix >= plane) return;
     const uint32_t i = pix & (g.nr - 1), j = pix >> g.lnr;
     const uint32_t di = (i + (g.nr >> 1)) & (g.nr - 1), dj = (j + (g.nc >> 1)) & (g.nc - 1);
-    const uint32_t rpos = g.pn(dj) * g.nr + di;
+    const size_t rpos = g.nl.off(di, dj);
     const float2 p = P[pix];
     const bool p_zero = p.x == 0.0f && p.y == 0.0f;
     // Where P = 0 the stitch contribution R conj(P) is zero, and where also
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
                         const uint32_t dj = (lj + (g.nc >> 1)) & (g.nc - 1);
                         const float2 pp = P[lj * g.nr + li];
                         if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0
-                            const float2 Rv = R[(size_t)ell * plane + g.pn(dj) * g.nr + di];
+                            const float2 Rv = R[(size_t)ell * plane + g.nl.off(di, dj)];
                             const float2 gw = cmulc(Rv, pp);
                             acc[q].x += gw.x * invN2;
                             acc[q].y += gw.y * invN2;

@@ -120,6 +120,10 @@ def main():
     ap.add_argument("--y", choices=["u16", "f32"], default="u16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stages", action="store_true", help="time without per-stage events")
+    ap.add_argument("--engine", choices=["f32", "split"], default="f32",
+                    help="FFT radix-16 stages: FP32 butterflies or split-FP16 MFMA products")
+    ap.add_argument("--graph", action="store_true",
+                    help="capture each epoch as a hipGraph and replay it (per-stage events off)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -171,7 +175,8 @@ def main():
 
     def solve(epochs, stage_times=None):
         optv = (ctypes.c_double * 3)(*opt)
-        o = fpm_amd._opts(stitch=stitch, device=True, stream=stream.cuda_stream)
+        o = fpm_amd._opts(stitch=stitch, device=True, stream=stream.cuda_stream, graph=args.graph,
+                          engine=fpm_amd.FFT_SPLIT_F16 if args.engine == "split" else fpm_amd.FFT_F32)
         if stage_times is not None:
             o.stage_times = ctypes.pointer(stage_times)
         fpm_amd._check(fpm_amd.lib().fpm_pure_v3(
@@ -183,7 +188,7 @@ def main():
     if args.warmup > 0:
         solve(args.warmup)
         torch.cuda.synchronize()
-    st = None if args.no_stages else fpm_amd.StageTimes()
+    st = None if (args.no_stages or args.graph) else fpm_amd.StageTimes()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     solve(args.steps, st)
@@ -204,6 +209,8 @@ def main():
         "config": {"workload": f"FPM solve, {L} uint16 images {n}x{n} -> {N}x{N} complex64 object, "
                                f"batch {args.batch}, {args.stitch} stitch, {args.steps} epochs in one call",
                    "n": n, "N": N, "L": L, "batch": args.batch, "y": args.y, "stitch": args.stitch,
+                   "launch": "hipGraph per epoch" if args.graph else "eager",
+                   "fft_engine": "split-fp16 MFMA (hi+lo, fp32 accumulate)" if args.engine == "split" else "fp32",
                    "solve_wall_s": t1 - t0,
                    "reference_wall_s_rtx3090_full_solve": 22.31},
         "epoch_model_GB": epoch_bytes / 1e9,

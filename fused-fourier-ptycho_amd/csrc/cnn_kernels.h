@@ -1,0 +1,87 @@
+// cnn_kernels.h -- launchers of the warm-start network's kernels (BF16
+// matrix-core products with FP32 accumulation, gfx950).
+//
+// Tensors are NHWC: element (b, y, x, c) of a B x H x W x C tensor sits at
+// ((b H + y) W + x) C + c.  Activations and the conv operands are BF16
+// (stored as uint16 bit patterns); accumulations, weight gradients, master
+// weights and the gradients between layers are FP32.
+#pragma once
+#include "common.h"
+
+namespace fpm {
+namespace cnn {
+
+typedef uint16_t bf16_t;
+
+// 3x3 convolution, zero padding 1, as an implicit GEMM on the matrix cores:
+//   out(b, p, co) = bias(co) + sum_{tap, ci} x(b, p + off(tap), ci) w(co, tap, ci)
+// w is Cout x 9 x Cin (tap = 3 (dy+1) + (dx+1)); Cin and Cout multiples of 32.
+// Epilogue: + bias (if non-null), ReLU (if relu), then stored as BF16 (y) and
+// / or FP32 (yf) NHWC.
+struct ConvArgs {
+    const bf16_t* x;
+    const bf16_t* w;
+    const float* bias;
+    bf16_t* y;
+    float* yf;
+    int B, H, W, Cin, Cout;
+    int relu;
+};
+void conv3x3(const ConvArgs& a, hipStream_t st);
+
+// C(M x N) += A(M x K) B(N x K)^T, A and B row-major BF16 with K contiguous,
+// C row-major FP32, K split over workgroups of ksplit elements each whose
+// partial tiles are added with FP32 atomics (C must be cleared first).
+// M, N multiples of 32; K any (zero beyond K).
+void gemm_nt_atomic(const bf16_t* A, const bf16_t* Bm, float* C, int M, int N, int64_t K, int64_t ksplit,
+                    hipStream_t st);
+
+// xcol(tap Cin + ci, q) = x(b, y+dy, x+dx, ci) or 0 off the image, q = (b H + y) W + x;
+// a (9 Cin) x Q BF16 matrix with Q = B H W contiguous.
+void im2col_t(const bf16_t* x, bf16_t* xcol, int B, int H, int W, int C, hipStream_t st);
+
+// weights for the input gradient: wt(ci, tap, co) = w(co, 8 - tap, ci)
+void flip_transpose_w(const bf16_t* w, bf16_t* wt, int Cout, int Cin, hipStream_t st);
+
+// gradient of a conv output before its backward products:
+//   gz = gy (times [y > 0] when relu), as BF16 NHWC (for the input gradient)
+//   and BF16 C x Q (gz_t, for the weight gradient); db(c) += sum_q gz(q, c)
+// (FP32, db must be cleared first).
+void grad_prep(const float* gy, const bf16_t* y, int relu, bf16_t* gz, bf16_t* gz_t, float* db, int64_t Q, int C,
+               hipStream_t st);
+
+// 2x2 average pooling (H, W even) and its adjoint (gx (+)= gy / 4 broadcast)
+void avgpool2_fwd(const bf16_t* x, bf16_t* y, int B, int H, int W, int C, hipStream_t st);
+void avgpool2_bwd(const float* gy, float* gx, int B, int H, int W, int C, int accumulate, hipStream_t st);
+// nearest 2x upsampling (h, w -> 2h, 2w) and its adjoint (sum of the 4 children)
+void upsample2_fwd(const bf16_t* x, bf16_t* y, int B, int h, int w, int C, hipStream_t st);
+void upsample2_bwd(const float* gy, float* gx, int B, int h, int w, int C, hipStream_t st);
+// y = a + b (BF16 in and out, FP32 sum)
+void add_bf16(const bf16_t* a, const bf16_t* b, bf16_t* y, int64_t count, hipStream_t st);
+// y (+)= x
+void add_f32(const float* x, float* y, int64_t count, int accumulate, hipStream_t st);
+
+// planar FP32 features (B x Cf x H x W) -> BF16 NHWC with C >= Cf channels (zero pad)
+void features_to_nhwc(const float* f, bf16_t* x, int B, int Cf, int H, int W, int C, hipStream_t st);
+
+// Head: yf (B x H x W x 2 u^2, FP32) -> planar output out (B x 2 x uH x uW) by
+// depth-to-space, channel c u^2 + sy u + sx -> plane c, pixel (u y + sy, u x + sx).
+// With target: loss_acc (FP64) += sum (out - target)^2 and
+// gyf = 2 (out - target) / count mapped back to the head layout.
+void head_shuffle(const float* yf, float* out, int B, int H, int W, int u, hipStream_t st);
+void head_mse(const float* yf, const float* target, float* gyf, double* loss_acc, int B, int H, int W, int u,
+              double inv_count, hipStream_t st);
+
+// Adam (no weight decay): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+// p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps); w16 = bf16(p)
+// for the first n16 entries of each layer segment (nullptr: none).
+struct AdamArgs {
+    float lr, b1, b2, eps, c1, c2;  // c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)
+};
+void adam_update(float* p, float* m, float* v, const float* g, int64_t count, AdamArgs a, hipStream_t st);
+// BF16 copy of FP32 values
+void to_bf16(const float* p, bf16_t* w, int64_t count, hipStream_t st);
+void fill_f32(float* p, int64_t count, float v, hipStream_t st);
+
+}  // namespace cnn
+}  // namespace fpm

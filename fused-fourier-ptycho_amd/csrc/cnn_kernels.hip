@@ -1,0 +1,466 @@
+// cnn_kernels.hip -- kernels of the warm-start network on gfx950.
+//
+// The three products of a 3x3 convolution layer run on the matrix cores as
+// v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32 accumulation):
+//   forward        out = x * w            implicit GEMM, M = pixels, N = Cout, K = 9 Cin
+//   input grad     gx = gz * flip(w)^T    the same kernel on transposed weights
+//   weight grad    gw = gz^T im2col(x)    NT GEMM, K = B H W, split over workgroups
+// Operand fragments are 8 BF16 (16 bytes) per lane, loaded straight from
+// global memory: channel runs of NHWC tensors for the implicit GEMM, rows of
+// K-contiguous matrices for the NT GEMM.  Lane layout of the 16x16x32
+// product (lane l, i = l mod 16, g = l / 16): A row i, columns 8g..8g+7;
+// B column i, rows 8g..8g+7; D column i, rows 4g..4g+3.
+#include "cnn_kernels.h"
+
+#include <algorithm>
+
+namespace fpm {
+namespace cnn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bf16x8 ld8(const bf16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
+__device__ __forceinline__ f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+static unsigned blocks_for(int64_t n, int t) { return (unsigned)std::min<int64_t>((n + t - 1) / t, 1 << 20); }
+
+// --------------------------------------------------------------- conv 3x3
+// One wave per 32 pixels (flattened index within an image) x 32 output
+// channels: 2 x 2 tiles of 16 x 16, K loop over 9 taps x Cin / 32.
+__global__ __launch_bounds__(64) void conv3x3_kernel(ConvArgs a) {
+    const int l = threadIdx.x;
+    const int i = l & 15, g = l >> 4;
+    const int P = a.H * a.W;
+    const int p0 = blockIdx.x * 32;
+    const int c0 = blockIdx.y * 32;
+    const int b = blockIdx.z;
+    const bf16_t* xb = a.x + (size_t)b * P * a.Cin;
+    int py[2], px[2];
+    bool pin[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int p = p0 + 16 * mt + i;
+        pin[mt] = p < P;
+        py[mt] = p / a.W;
+        px[mt] = p % a.W;
+    }
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const bf16_t* src[2];
+        bool ok[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            const int yy = py[mt] + dy, xx = px[mt] + dx;
+            ok[mt] = pin[mt] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            src[mt] = xb + ((size_t)(ok[mt] ? yy : 0) * a.W + (ok[mt] ? xx : 0)) * a.Cin + 8 * g;
+        }
+        const bf16_t* wr[2];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) wr[nt] = a.w + ((size_t)(c0 + 16 * nt + i) * 9 + tap) * a.Cin + 8 * g;
+        for (int ci = 0; ci < a.Cin; ci += 32) {
+            const bf16x8 A0 = ok[0] ? ld8(src[0] + ci) : zero8();
+            const bf16x8 A1 = ok[1] ? ld8(src[1] + ci) : zero8();
+            const bf16x8 B0 = ld8(wr[0] + ci);
+            const bf16x8 B1 = ld8(wr[1] + ci);
+            acc[0][0] = mma(A0, B0, acc[0][0]);
+            acc[0][1] = mma(A0, B1, acc[0][1]);
+            acc[1][0] = mma(A1, B0, acc[1][0]);
+            acc[1][1] = mma(A1, B1, acc[1][1]);
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int co = c0 + 16 * nt + i;
+        const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = p0 + 16 * mt + 4 * g + r;
+                if (p >= P) continue;
+                float v = acc[mt][nt][r] + bias;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                const size_t o = ((size_t)b * P + p) * a.Cout + co;
+                if (a.y) a.y[o] = f2bf(v);
+                if (a.yf) a.yf[o] = v;
+            }
+    }
+}
+
+void conv3x3(const ConvArgs& a, hipStream_t st) {
+    if (a.Cin % 32 || a.Cout % 32) throw HipError{hipErrorInvalidValue, "conv3x3: channels must be multiples of 32"};
+    const int P = a.H * a.W;
+    dim3 grid((P + 31) / 32, a.Cout / 32, a.B);
+    hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(64), 0, st, a);
+    FPM_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------- NT GEMM
+__global__ __launch_bounds__(64) void gemm_nt_kernel(const bf16_t* __restrict__ A, const bf16_t* __restrict__ Bm,
+                                                     float* __restrict__ C, int M, int N, int64_t K, int64_t ks) {
+    const int l = threadIdx.x;
+    const int i = l & 15, g = l >> 4;
+    const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+    const int64_t kb = (int64_t)blockIdx.z * ks;
+    const int64_t ke = std::min<int64_t>(K, kb + ks);
+    const bf16_t* ar[2];
+    const bf16_t* br[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        ar[t] = A + (size_t)(m0 + 16 * t + i) * K + 8 * g;
+        br[t] = Bm + (size_t)(n0 + 16 * t + i) * K + 8 * g;
+    }
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t k = kb; k < ke; k += 32) {
+        bf16x8 a0, a1, b0, b1;
+        if (k + 32 <= ke) {
+            a0 = ld8(ar[0] + k);
+            a1 = ld8(ar[1] + k);
+            b0 = ld8(br[0] + k);
+            b1 = ld8(br[1] + k);
+        } else {  // tail: element-wise, zero beyond ke
+            a0 = a1 = b0 = b1 = zero8();
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (k + 8 * g + e < ke) {
+                    a0[e] = __builtin_bit_cast(__bf16, ar[0][k + e]);
+                    a1[e] = __builtin_bit_cast(__bf16, ar[1][k + e]);
+                    b0[e] = __builtin_bit_cast(__bf16, br[0][k + e]);
+                    b1[e] = __builtin_bit_cast(__bf16, br[1][k + e]);
+                }
+            }
+        }
+        acc[0][0] = mma(a0, b0, acc[0][0]);
+        acc[0][1] = mma(a0, b1, acc[0][1]);
+        acc[1][0] = mma(a1, b0, acc[1][0]);
+        acc[1][1] = mma(a1, b1, acc[1][1]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + 16 * mt + 4 * g + r, col = n0 + 16 * nt + i;
+                unsafeAtomicAdd(&C[(size_t)row * N + col], acc[mt][nt][r]);
+            }
+}
+
+void gemm_nt_atomic(const bf16_t* A, const bf16_t* Bm, float* C, int M, int N, int64_t K, int64_t ksplit,
+                    hipStream_t st) {
+    if (M % 32 || N % 32 || K % 8 || ksplit % 32 || ksplit <= 0)
+        throw HipError{hipErrorInvalidValue, "gemm_nt_atomic: M, N multiples of 32, K of 8, ksplit of 32"};
+    dim3 grid(M / 32, N / 32, (unsigned)((K + ksplit - 1) / ksplit));
+    hipLaunchKernelGGL(gemm_nt_kernel, grid, dim3(64), 0, st, A, Bm, C, M, N, K, ksplit);
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ im2col^T
+// thread: one pixel q and 8 channels; writes 9 x 8 rows of xcol at column q
+__global__ __launch_bounds__(256) void im2col_t_kernel(const bf16_t* __restrict__ x, bf16_t* __restrict__ xcol, int B,
+                                                       int H, int W, int C) {
+    const int64_t Q = (int64_t)B * H * W;
+    const int c8 = C / 8;
+    const int64_t total = Q * c8;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int64_t q = t % Q;
+        const int cg = (int)(t / Q);
+        const int xx = (int)(q % W);
+        const int yy = (int)((q / W) % H);
+        const int64_t b = q / ((int64_t)H * W);
+        for (int tap = 0; tap < 9; ++tap) {
+            const int sy = yy + tap / 3 - 1, sx = xx + tap % 3 - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (sy >= 0 && sy < H && sx >= 0 && sx < W)
+                v = *(const uint4*)(x + ((b * H + sy) * W + sx) * C + 8 * cg);
+            const uint16_t* e = (const uint16_t*)&v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) xcol[((size_t)tap * C + 8 * cg + k) * Q + q] = e[k];
+        }
+    }
+}
+
+void im2col_t(const bf16_t* x, bf16_t* xcol, int B, int H, int W, int C, hipStream_t st) {
+    if (C % 8) throw HipError{hipErrorInvalidValue, "im2col_t: channels must be a multiple of 8"};
+    const int64_t total = (int64_t)B * H * W * (C / 8);
+    hipLaunchKernelGGL(im2col_t_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, x, xcol, B, H, W, C);
+    FPM_LAUNCH_CHECK();
+}
+
+__global__ void flip_transpose_w_kernel(const bf16_t* __restrict__ w, bf16_t* __restrict__ wt, int Cout, int Cin) {
+    const int64_t total = (int64_t)Cout * 9 * Cin;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int co = (int)(t % Cout);
+        const int tap = (int)((t / Cout) % 9);
+        const int ci = (int)(t / ((int64_t)Cout * 9));
+        wt[t] = w[((size_t)co * 9 + (8 - tap)) * Cin + ci];  // wt(ci, tap, co)
+    }
+}
+
+void flip_transpose_w(const bf16_t* w, bf16_t* wt, int Cout, int Cin, hipStream_t st) {
+    const int64_t total = (int64_t)Cout * 9 * Cin;
+    hipLaunchKernelGGL(flip_transpose_w_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, w, wt, Cout, Cin);
+    FPM_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------- grad prep
+// workgroup: 64 consecutive q, all C (<= 256) channels through an LDS tile
+constexpr int kGpQ = 64;
+__global__ __launch_bounds__(256) void grad_prep_kernel(const float* __restrict__ gy, const bf16_t* __restrict__ y,
+                                                        int relu, bf16_t* __restrict__ gz, bf16_t* __restrict__ gz_t,
+                                                        float* __restrict__ db, int64_t Q, int C) {
+    __shared__ bf16_t tile[kGpQ * 257];
+    __shared__ float dbs[256];
+    const int t = threadIdx.x;
+    const int64_t q0 = (int64_t)blockIdx.x * kGpQ;
+    for (int c = t; c < C; c += 256) dbs[c] = 0.0f;
+    __syncthreads();
+    for (int e = t; e < kGpQ * C; e += 256) {
+        const int qq = e / C, c = e % C;
+        const int64_t q = q0 + qq;
+        bf16_t h = 0;
+        if (q < Q) {
+            const size_t o = (size_t)q * C + c;
+            float v = gy[o];
+            if (relu && !(bf2f(y[o]) > 0.0f)) v = 0.0f;
+            h = f2bf(v);
+            gz[o] = h;
+        }
+        tile[qq * 257 + c] = h;
+    }
+    __syncthreads();
+    // per-channel sums of the rounded values; one atomic per channel per workgroup
+    for (int c = t; c < C; c += 256) {
+        float s = 0.0f;
+        for (int qq = 0; qq < kGpQ; ++qq) s += bf2f(tile[qq * 257 + c]);
+        unsafeAtomicAdd(&db[c], s);
+    }
+    for (int e = t; e < kGpQ * C; e += 256) {
+        const int c = e / kGpQ, qq = e % kGpQ;
+        const int64_t q = q0 + qq;
+        if (q < Q) gz_t[(size_t)c * Q + q] = tile[qq * 257 + c];
+    }
+}
+
+void grad_prep(const float* gy, const bf16_t* y, int relu, bf16_t* gz, bf16_t* gz_t, float* db, int64_t Q, int C,
+               hipStream_t st) {
+    if (C > 256) throw HipError{hipErrorInvalidValue, "grad_prep: at most 256 channels"};
+    hipLaunchKernelGGL(grad_prep_kernel, dim3((unsigned)((Q + kGpQ - 1) / kGpQ)), dim3(256), 0, st, gy, y, relu, gz,
+                       gz_t, db, Q, C);
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------- pooling / upsampling
+__global__ void avgpool2_fwd_kernel(const bf16_t* __restrict__ x, bf16_t* __restrict__ y, int B, int H, int W, int C) {
+    const int h = H / 2, w = W / 2;
+    const int64_t total = (int64_t)B * h * w * C;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int c = (int)(t % C);
+        const int xx = (int)((t / C) % w);
+        const int yy = (int)((t / ((int64_t)C * w)) % h);
+        const int64_t b = t / ((int64_t)C * w * h);
+        const size_t r0 = ((b * H + 2 * yy) * W + 2 * xx) * C + c;
+        const size_t r1 = r0 + (size_t)W * C;
+        const float s = (bf2f(x[r0]) + bf2f(x[r0 + C])) + (bf2f(x[r1]) + bf2f(x[r1 + C]));
+        y[t] = f2bf(s * 0.25f);
+    }
+}
+
+__global__ void avgpool2_bwd_kernel(const float* __restrict__ gy, float* __restrict__ gx, int B, int H, int W, int C,
+                                    int acc) {
+    const int64_t total = (int64_t)B * H * W * C;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int c = (int)(t % C);
+        const int xx = (int)((t / C) % W);
+        const int yy = (int)((t / ((int64_t)C * W)) % H);
+        const int64_t b = t / ((int64_t)C * W * H);
+        const float v = gy[((b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c] * 0.25f;
+        gx[t] = acc ? gx[t] + v : v;
+    }
+}
+
+__global__ void upsample2_fwd_kernel(const bf16_t* __restrict__ x, bf16_t* __restrict__ y, int B, int h, int w,
+                                     int C) {
+    const int H = 2 * h, W = 2 * w;
+    const int64_t total = (int64_t)B * H * W * C;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int c = (int)(t % C);
+        const int xx = (int)((t / C) % W);
+        const int yy = (int)((t / ((int64_t)C * W)) % H);
+        const int64_t b = t / ((int64_t)C * W * H);
+        y[t] = x[((b * h + yy / 2) * w + xx / 2) * C + c];
+    }
+}
+
+__global__ void upsample2_bwd_kernel(const float* __restrict__ gy, float* __restrict__ gx, int B, int h, int w, int C) {
+    const int W = 2 * w;
+    const int64_t total = (int64_t)B * h * w * C;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int c = (int)(t % C);
+        const int xx = (int)((t / C) % w);
+        const int yy = (int)((t / ((int64_t)C * w)) % h);
+        const int64_t b = t / ((int64_t)C * w * h);
+        const size_t r0 = ((b * 2 * h + 2 * yy) * W + 2 * xx) * C + c;
+        const size_t r1 = r0 + (size_t)W * C;
+        gx[t] = (gy[r0] + gy[r0 + C]) + (gy[r1] + gy[r1 + C]);
+    }
+}
+
+void avgpool2_fwd(const bf16_t* x, bf16_t* y, int B, int H, int W, int C, hipStream_t st) {
+    const int64_t total = (int64_t)B * (H / 2) * (W / 2) * C;
+    hipLaunchKernelGGL(avgpool2_fwd_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, x, y, B, H, W, C);
+    FPM_LAUNCH_CHECK();
+}
+void avgpool2_bwd(const float* gy, float* gx, int B, int H, int W, int C, int accumulate, hipStream_t st) {
+    const int64_t total = (int64_t)B * H * W * C;
+    hipLaunchKernelGGL(avgpool2_bwd_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, gy, gx, B, H, W, C,
+                       accumulate);
+    FPM_LAUNCH_CHECK();
+}
+void upsample2_fwd(const bf16_t* x, bf16_t* y, int B, int h, int w, int C, hipStream_t st) {
+    const int64_t total = (int64_t)B * 4 * h * w * C;
+    hipLaunchKernelGGL(upsample2_fwd_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, x, y, B, h, w, C);
+    FPM_LAUNCH_CHECK();
+}
+void upsample2_bwd(const float* gy, float* gx, int B, int h, int w, int C, hipStream_t st) {
+    const int64_t total = (int64_t)B * h * w * C;
+    hipLaunchKernelGGL(upsample2_bwd_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, gy, gx, B, h, w, C);
+    FPM_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------- elementwise
+__global__ void add_bf16_kernel(const bf16_t* __restrict__ a, const bf16_t* __restrict__ b, bf16_t* __restrict__ y,
+                                int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        y[t] = f2bf(bf2f(a[t]) + bf2f(b[t]));
+}
+__global__ void add_f32_kernel(const float* __restrict__ x, float* __restrict__ y, int64_t n, int acc) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        y[t] = acc ? y[t] + x[t] : x[t];
+}
+void add_bf16(const bf16_t* a, const bf16_t* b, bf16_t* y, int64_t count, hipStream_t st) {
+    hipLaunchKernelGGL(add_bf16_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, a, b, y, count);
+    FPM_LAUNCH_CHECK();
+}
+void add_f32(const float* x, float* y, int64_t count, int accumulate, hipStream_t st) {
+    hipLaunchKernelGGL(add_f32_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, x, y, count, accumulate);
+    FPM_LAUNCH_CHECK();
+}
+
+__global__ void features_kernel(const float* __restrict__ f, bf16_t* __restrict__ x, int B, int Cf, int H, int W,
+                                int C) {
+    const int64_t total = (int64_t)B * H * W * C;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
+        const int c = (int)(t % C);
+        const int64_t p = (t / C) % ((int64_t)H * W);
+        const int64_t b = t / ((int64_t)C * H * W);
+        x[t] = c < Cf ? f2bf(f[(b * Cf + c) * H * W + p]) : (bf16_t)0;
+    }
+}
+void features_to_nhwc(const float* f, bf16_t* x, int B, int Cf, int H, int W, int C, hipStream_t st) {
+    const int64_t total = (int64_t)B * H * W * C;
+    hipLaunchKernelGGL(features_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, f, x, B, Cf, H, W, C);
+    FPM_LAUNCH_CHECK();
+}
+
+// -------------------------------------------------------------- head
+// head element e = ((b H + y) W + x) 2u^2 + ch, ch = c u^2 + sy u + sx
+__device__ __forceinline__ size_t head_out_index(int64_t e, int H, int W, int u) {
+    const int U2 = u * u, CH = 2 * U2;
+    const int ch = (int)(e % CH);
+    const int64_t pix = e / CH;
+    const int x = (int)(pix % W);
+    const int y = (int)((pix / W) % H);
+    const int64_t b = pix / ((int64_t)W * H);
+    const int c = ch / U2, sy = (ch % U2) / u, sx = ch % u;
+    const int64_t OH = (int64_t)u * H, OW = (int64_t)u * W;
+    return (size_t)(((b * 2 + c) * OH + (int64_t)u * y + sy) * OW + (int64_t)u * x + sx);
+}
+
+__global__ void head_shuffle_kernel(const float* __restrict__ yf, float* __restrict__ out, int B, int H, int W, int u) {
+    const int64_t total = (int64_t)B * H * W * 2 * u * u;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256)
+        out[head_out_index(e, H, W, u)] = yf[e];
+}
+
+__global__ __launch_bounds__(256) void head_mse_kernel(const float* __restrict__ yf, const float* __restrict__ target,
+                                                       float* __restrict__ gyf, double* __restrict__ loss_acc, int B,
+                                                       int H, int W, int u, double inv_count) {
+    __shared__ double red[4];
+    const int64_t total = (int64_t)B * H * W * 2 * u * u;
+    double s = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const float d = yf[e] - target[head_out_index(e, H, W, u)];
+        s += (double)d * (double)d;
+        gyf[e] = (float)(2.0 * (double)d * inv_count);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0 && loss_acc) unsafeAtomicAdd(loss_acc, red[0] + red[1] + red[2] + red[3]);
+}
+
+void head_shuffle(const float* yf, float* out, int B, int H, int W, int u, hipStream_t st) {
+    const int64_t total = (int64_t)B * H * W * 2 * u * u;
+    hipLaunchKernelGGL(head_shuffle_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, yf, out, B, H, W, u);
+    FPM_LAUNCH_CHECK();
+}
+void head_mse(const float* yf, const float* target, float* gyf, double* loss_acc, int B, int H, int W, int u,
+              double inv_count, hipStream_t st) {
+    const int64_t total = (int64_t)B * H * W * 2 * u * u;
+    hipLaunchKernelGGL(head_mse_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, yf, target, gyf, loss_acc, B,
+                       H, W, u, inv_count);
+    FPM_LAUNCH_CHECK();
+}
+
+// -------------------------------------------------------------- Adam
+__global__ void adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, int64_t n, AdamArgs a) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+        const float gr = g[t];
+        const float mm = a.b1 * m[t] + (1.0f - a.b1) * gr;
+        const float vv = a.b2 * v[t] + (1.0f - a.b2) * gr * gr;
+        m[t] = mm;
+        v[t] = vv;
+        p[t] -= a.lr * (mm * a.c1) / (sqrtf(vv * a.c2) + a.eps);
+    }
+}
+void adam_update(float* p, float* m, float* v, const float* g, int64_t count, AdamArgs a, hipStream_t st) {
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, p, m, v, g, count, a);
+    FPM_LAUNCH_CHECK();
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ p, bf16_t* __restrict__ w, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) w[t] = f2bf(p[t]);
+}
+void to_bf16(const float* p, bf16_t* w, int64_t count, hipStream_t st) {
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, p, w, count);
+    FPM_LAUNCH_CHECK();
+}
+
+__global__ void fill_kernel(float* __restrict__ p, int64_t n, float v) {
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) p[t] = v;
+}
+void fill_f32(float* p, int64_t count, float v, hipStream_t st) {
+    hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, p, count, v);
+    FPM_LAUNCH_CHECK();
+}
+
+}  // namespace cnn
+}  // namespace fpm

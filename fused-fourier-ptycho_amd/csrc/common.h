@@ -3,9 +3,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <cmath>
 #include <string>
 
 namespace fpm {
+
+// FFT engines (see fft_core.h): FP32 butterflies on the vector ALUs, or the
+// radix-16 DFTs as split-FP16 matrix-core products with FP32 accumulation
+enum { ENG_F32 = 0, ENG_SPLIT16 = 1 };
 
 // Column permutation of a frequency-domain array whose column count C is
 // split C = M1 * M2 for the two-pass (four-step) column transform: natural
@@ -35,6 +40,12 @@ inline int ilog2_host(uint64_t v) {
     return l;
 }
 inline bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+// 1/x when x is a power of two (then x * (1/x) scaling equals division by x
+// exactly), else 0
+inline float exact_inverse_or_zero(float x) {
+    int e = 0;
+    return (x > 0.0f && std::frexp(x, &e) == 0.5f) ? 1.0f / x : 0.0f;
+}
 
 inline ColPerm make_colperm(uint32_t C) {
     ColPerm p;

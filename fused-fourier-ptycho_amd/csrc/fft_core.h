@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.h"
+
 namespace fpm {
 
 // ------------------------------------------------------------------ complex
@@ -227,99 +229,255 @@ template <int NLINES, int LEN> struct LineFastMap {
     __device__ static __forceinline__ int lds(int line, int idx) { return idx * (NLINES + 1) + line; }
 };
 
+// ------------------------------------------------------------------ engines
+// ENG_F32:     every butterfly in FP32 on the vector ALUs (thread = butterfly).
+// ENG_SPLIT16: the radix-16 DFTs run on the matrix cores as FP16 products
+//              with split hi + lo operands and FP32 accumulation; twiddles,
+//              small-radix passes and everything around the transform stay
+//              FP32.  A wave holds 64 butterflies of a radix-16 pass as four
+//              groups of 16; lane (j = lane mod 16, g = lane / 16) keeps
+//              points 4g..4g+3 of butterfly j of each group, and the MFMA
+//              returns outputs 4g..4g+3 to the same lane (same slot map in
+//              and out, so fused kernels keep their register hand-offs).
+
+typedef _Float16 fpm_half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 fpm_half2 __attribute__((ext_vector_type(2)));
+typedef float fpm_float4 __attribute__((ext_vector_type(4)));
+typedef float fpm_float2v __attribute__((ext_vector_type(2)));
+
+// The forward DFT-16 as two 16 x 32 real matrices (rows: re/im of 8 outputs,
+// columns: re/im of the 16 inputs), split into FP16 hi + lo, in the A-operand
+// layout of v_mfma_f32_16x16x32_f16 (lane l: row l mod 16, columns
+// 8 (l/16) .. +7).  Matrix h (0, 1) yields outputs 4 (p/2) + 2h + (p mod 2)
+// for row pair p, so lane group g receives outputs 4g..4g+3.
+struct Dft16Mats {
+    fpm_half8 hi[2], lo[2];
+};
+
+__device__ __forceinline__ Dft16Mats make_dft16_mats() {
+    // cos and sin of 2 pi m / 16, rounded from the exact values
+    constexpr float C[16] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                             0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+                             -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
+                             0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
+    const int l = threadIdx.x & 63;
+    const int i = l & 15, p = i >> 1, c = i & 1;
+    Dft16Mats m;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int kout = 4 * (p >> 1) + 2 * h + (p & 1);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int kk = 8 * (l >> 4) + t;
+            const int s = kk >> 1, cp = kk & 1;
+            const int e = (kout * s) & 15;
+            // W16^(kout s) = cos(2 pi e/16) - i sin(2 pi e/16) (forward sign)
+            const float co = C[e], si = -C[(e + 12) & 15];
+            // real form of y = w x: re = co xr - si xi, im = si xr + co xi
+            const float a = c == 0 ? (cp == 0 ? co : -si) : (cp == 0 ? si : co);
+            const _Float16 ah = (_Float16)a;
+            m.hi[h][t] = ah;
+            m.lo[h][t] = (_Float16)(a - (float)ah);
+        }
+    }
+    return m;
+}
+
+// DFT-16 of the four points v[0..3] this lane holds (points 4g..4g+3 of the
+// butterfly of lane j = lane mod 16); on return v[0..3] are outputs 4g..4g+3.
+// Block floating point per butterfly: the inputs are scaled by 2^k so that
+// the largest |component| lies in [2^14, 2^15), split exactly into
+// hi = fp16(x) and lo = fp16(x - hi) (22 significant bits together), the
+// product hi*hi + hi*lo + lo*hi accumulates in FP32 and the result is scaled
+// back by 2^-k (exact).  The inverse is conj(DFT(conj x)).
+template <bool INV> __device__ __forceinline__ void dft16_mfma(float2* v, const Dft16Mats& M) {
+    float mx = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) mx = fmaxf(mx, fmaxf(fabsf(v[r].x), fabsf(v[r].y)));
+    mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    const int k = 15 - __builtin_amdgcn_frexp_expf(mx);
+    fpm_half8 bh, bl;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        fpm_float2v x;
+        x.x = __builtin_amdgcn_ldexpf(v[r].x, k);
+        x.y = __builtin_amdgcn_ldexpf(INV ? -v[r].y : v[r].y, k);
+        const fpm_half2 h = __builtin_convertvector(x, fpm_half2);
+        const fpm_float2v back = __builtin_convertvector(h, fpm_float2v);
+        const fpm_half2 lo = __builtin_convertvector(x - back, fpm_half2);
+        bh[2 * r] = h.x;
+        bh[2 * r + 1] = h.y;
+        bl[2 * r] = lo.x;
+        bl[2 * r + 1] = lo.y;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        fpm_float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(M.lo[h], bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(M.hi[h], bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(M.hi[h], bh, acc, 0, 0, 0);
+        v[2 * h].x = __builtin_amdgcn_ldexpf(acc[0], -k);
+        v[2 * h].y = __builtin_amdgcn_ldexpf(INV ? -acc[1] : acc[1], -k);
+        v[2 * h + 1].x = __builtin_amdgcn_ldexpf(acc[2], -k);
+        v[2 * h + 1].y = __builtin_amdgcn_ldexpf(INV ? -acc[3] : acc[3], -k);
+    }
+}
+
+// Engine context of a kernel: the twiddles and (split engine) the DFT-16 matrices
+template <int ENG> struct EngCtx;
+template <> struct EngCtx<ENG_F32> {
+    __device__ __forceinline__ void init() {}
+};
+template <> struct EngCtx<ENG_SPLIT16> {
+    Dft16Mats M;
+    __device__ __forceinline__ void init() { M = make_dft16_mats(); }
+};
+
+// Slot layout of pass P: registers v[q*GS + s] (q < NG, s < GS) hold point
+// first + s of butterfly q's line index j, i.e. element j + (first + s) NB.
+template <int LEN, bool INV, int P, int T, class Map, int ENG> struct PassSlots {
+    static constexpr int R = Plan<LEN, INV>::radix(P);
+    static constexpr int NB = LEN / R;
+    // the split layout needs whole waves; smaller workgroups keep FP32 butterflies
+    static constexpr bool MF = ENG == ENG_SPLIT16 && R == 16 && T % 64 == 0;
+    static constexpr int NG = MF ? 4 : 16 / R;
+    static constexpr int GS = MF ? 4 : R;
+    __device__ static __forceinline__ void group(int t, int q, int& line, int& j, int& first) {
+        int b;
+        if constexpr (MF) {
+            b = (t >> 6) * 64 + q * 16 + (t & 15);
+            first = 4 * ((t >> 4) & 3);
+        } else {
+            b = t + T * q;
+            first = 0;
+        }
+        Map::split(b, NB, line, j);
+    }
+};
+
+// W^(s m) for s = first .. first+3 (split engine): 3 lookups + 4 products
+template <bool INV>
+__device__ __forceinline__ void twiddle_quad(const Twiddles& tw, uint32_t m, int first, float2 (&w)[4]) {
+    const float2 w1 = tw.get<INV>(m), w2 = tw.get<INV>(2 * m);
+    const float2 wf = tw.get<INV>((uint32_t)first * m);
+    w[0] = wf;
+    w[1] = cmul(wf, w1);
+    w[2] = cmul(wf, w2);
+    w[3] = cmul(wf, cmul(w1, w2));
+}
+
+template <int R, bool INV, bool MF, int ENG>
+__device__ __forceinline__ void dft_group(float2* a, const EngCtx<ENG>& ctx) {
+    if constexpr (MF) dft16_mfma<INV>(a, ctx.M);
+    else dft<R, INV>(a);
+}
+
 // Gather the inputs of pass P from LDS into registers, apply twiddles, DFT.
-template <int LEN, bool INV, int P, int T, class Map>
-__device__ __forceinline__ void pass_from_lds(float2 (&v)[16], const float2* lds, int t, const Twiddles& tw) {
-    using PL = Plan<LEN, INV>;
-    constexpr int R = PL::radix(P);
-    constexpr int L = PL::span(P);
-    constexpr int NB = LEN / R;  // butterflies per line
+template <int LEN, bool INV, int P, int T, class Map, int ENG = ENG_F32>
+__device__ __forceinline__ void pass_from_lds(float2 (&v)[16], const float2* lds, int t, const Twiddles& tw,
+                                              const EngCtx<ENG>& ctx) {
+    using PS = PassSlots<LEN, INV, P, T, Map, ENG>;
+    constexpr int R = PS::R;
+    constexpr int L = Plan<LEN, INV>::span(P);
+    constexpr int NB = PS::NB;
 #pragma unroll
-    for (int q = 0; q < 16 / R; ++q) {
-        int line, j;
-        Map::split(t + T * q, NB, line, j);
+    for (int q = 0; q < PS::NG; ++q) {
+        int line, j, first;
+        PS::group(t, q, line, j, first);
 #pragma unroll
-        for (int s = 0; s < R; ++s) v[q * R + s] = lds[Map::lds(line, j + s * NB)];
+        for (int s = 0; s < PS::GS; ++s) v[q * PS::GS + s] = lds[Map::lds(line, j + (first + s) * NB)];
         if constexpr (L > 1) {
             const uint32_t k = (uint32_t)(j % L);
             constexpr uint32_t step = kTwiddleLen / (L * R);
-            float2 w[16];
-            twiddle_powers<R, INV>(tw, k * step, w);
+            if constexpr (PS::MF) {
+                float2 w[4];
+                twiddle_quad<INV>(tw, k * step, first, w);
 #pragma unroll
-            for (int s = 1; s < R; ++s) v[q * R + s] = cmul(v[q * R + s], w[s]);
+                for (int s = 0; s < 4; ++s) v[q * 4 + s] = cmul(v[q * 4 + s], w[s]);
+            } else {
+                float2 w[16];
+                twiddle_powers<R, INV>(tw, k * step, w);
+#pragma unroll
+                for (int s = 1; s < R; ++s) v[q * R + s] = cmul(v[q * R + s], w[s]);
+            }
         }
-        dft<R, INV>(&v[q * R]);
+        dft_group<R, INV, PS::MF, ENG>(&v[q * PS::GS], ctx);
     }
 }
 
 // Scatter the outputs of pass P (held in registers) to LDS in Stockham order.
-template <int LEN, bool INV, int P, int T, class Map>
+template <int LEN, bool INV, int P, int T, class Map, int ENG = ENG_F32>
 __device__ __forceinline__ void pass_to_lds(const float2 (&v)[16], float2* lds, int t) {
-    using PL = Plan<LEN, INV>;
-    constexpr int R = PL::radix(P);
-    constexpr int L = PL::span(P);
-    constexpr int NB = LEN / R;
+    using PS = PassSlots<LEN, INV, P, T, Map, ENG>;
+    constexpr int R = PS::R;
+    constexpr int L = Plan<LEN, INV>::span(P);
 #pragma unroll
-    for (int q = 0; q < 16 / R; ++q) {
-        int line, j;
-        Map::split(t + T * q, NB, line, j);
+    for (int q = 0; q < PS::NG; ++q) {
+        int line, j, first;
+        PS::group(t, q, line, j, first);
         const int base = (j / L) * L * R + (j % L);
 #pragma unroll
-        for (int s = 0; s < R; ++s) lds[Map::lds(line, base + s * L)] = v[q * R + s];
+        for (int s = 0; s < PS::GS; ++s) lds[Map::lds(line, base + (first + s) * L)] = v[q * PS::GS + s];
     }
 }
 
-// DFT of the first pass on registers loaded as v[q*R + s] = x[line][j + s*LEN/R]
+// DFT of the first pass on registers loaded in the slot map of pass 0
 // (pass 0 has span 1, so it has no twiddles).
-template <int LEN, bool INV>
-__device__ __forceinline__ void first_pass_dft(float2 (&v)[16]) {
+template <int LEN, bool INV, int T, int ENG = ENG_F32>
+__device__ __forceinline__ void first_pass_dft(float2 (&v)[16], const EngCtx<ENG>& ctx = EngCtx<ENG>{}) {
     constexpr int R = Plan<LEN, INV>::radix(0);
+    constexpr bool MF = ENG == ENG_SPLIT16 && R == 16 && T % 64 == 0;
+    constexpr int NG = MF ? 4 : 16 / R, GS = MF ? 4 : R;
 #pragma unroll
-    for (int q = 0; q < 16 / R; ++q) dft<R, INV>(&v[q * R]);
+    for (int q = 0; q < NG; ++q) dft_group<R, INV, MF, ENG>(&v[q * GS], ctx);
 }
 
 // Runs passes 1..NPASS-1 (pass 0's DFT must already be done on v).  On
 // return v holds the outputs of the last pass (see Slots::out).  The caller
 // must __syncthreads() before calling if the LDS was in use.
-template <int LEN, bool INV, int T, class Map>
-__device__ __forceinline__ void remaining_passes(float2 (&v)[16], float2* lds, int t, const Twiddles& tw) {
+template <int LEN, bool INV, int T, class Map, int ENG = ENG_F32>
+__device__ __forceinline__ void remaining_passes(float2 (&v)[16], float2* lds, int t, const Twiddles& tw,
+                                                 const EngCtx<ENG>& ctx = EngCtx<ENG>{}) {
     using PL = Plan<LEN, INV>;
     static_assert(PL::NPASS <= 4, "LEN above 65536 is not supported");
     if constexpr (PL::NPASS > 1) {
-        pass_to_lds<LEN, INV, 0, T, Map>(v, lds, t);
+        pass_to_lds<LEN, INV, 0, T, Map, ENG>(v, lds, t);
         __syncthreads();
-        pass_from_lds<LEN, INV, 1, T, Map>(v, lds, t, tw);
+        pass_from_lds<LEN, INV, 1, T, Map, ENG>(v, lds, t, tw, ctx);
     }
     if constexpr (PL::NPASS > 2) {
         __syncthreads();
-        pass_to_lds<LEN, INV, 1, T, Map>(v, lds, t);
+        pass_to_lds<LEN, INV, 1, T, Map, ENG>(v, lds, t);
         __syncthreads();
-        pass_from_lds<LEN, INV, 2, T, Map>(v, lds, t, tw);
+        pass_from_lds<LEN, INV, 2, T, Map, ENG>(v, lds, t, tw, ctx);
     }
     if constexpr (PL::NPASS > 3) {
         __syncthreads();
-        pass_to_lds<LEN, INV, 2, T, Map>(v, lds, t);
+        pass_to_lds<LEN, INV, 2, T, Map, ENG>(v, lds, t);
         __syncthreads();
-        pass_from_lds<LEN, INV, 3, T, Map>(v, lds, t, tw);
+        pass_from_lds<LEN, INV, 3, T, Map, ENG>(v, lds, t, tw, ctx);
     }
 }
 
-// slot (q, s) of thread t -> (line, index) for the first pass input and the
-// last pass output
-template <int LEN, bool INV, int T, class Map> struct Slots {
-    static constexpr int R0 = Plan<LEN, INV>::radix(0);
-    static constexpr int RL = Plan<LEN, INV>::radix(Plan<LEN, INV>::NPASS - 1);
+// Slot maps of the first pass input and the last pass output: register
+// v[q*GS + s] (q < NG, s < GS) holds element (line, idx); for a fixed q the
+// index steps by NB per s (idx(q, s) = idx(q, 0) + s*NB).
+template <int LEN, bool INV, int T, class Map, int ENG = ENG_F32> struct Slots {
+    using First = PassSlots<LEN, INV, 0, T, Map, ENG>;
+    using Last = PassSlots<LEN, INV, Plan<LEN, INV>::NPASS - 1, T, Map, ENG>;
+    static constexpr int R0 = First::R, NG0 = First::NG, GS0 = First::GS, NB0 = First::NB;
+    static constexpr int RL = Last::R, NGL = Last::NG, GSL = Last::GS, NBL = Last::NB;
+    static constexpr bool MF0 = First::MF, MFL = Last::MF;
     __device__ static __forceinline__ void in(int t, int q, int s, int& line, int& idx) {
-        constexpr int NB = LEN / R0;
-        int j;
-        Map::split(t + T * q, NB, line, j);
-        idx = j + s * NB;
+        int j, first;
+        First::group(t, q, line, j, first);
+        idx = j + (first + s) * NB0;
     }
     __device__ static __forceinline__ void out(int t, int q, int s, int& line, int& idx) {
-        constexpr int NB = LEN / RL;
-        int j;
-        Map::split(t + T * q, NB, line, j);
-        idx = j + s * NB;
+        int j, first;
+        Last::group(t, q, line, j, first);
+        idx = j + (first + s) * NBL;
     }
 };
 

@@ -31,8 +31,9 @@ struct StitchArgs {
 };
 
 // one LEN-point transform per contiguous line (LEN = len, nlines lines)
+// eng: ENG_F32 or ENG_SPLIT16 (every launcher below)
 void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bool inverse, const float2* twid,
-                hipStream_t st);
+                hipStream_t st, int eng = ENG_F32);
 
 // transforms along the columns index of rows x cols pages; see fft_kernels.hip.
 // rowmask (optional, one byte per 16 rows): row tiles whose bytes are all 0
@@ -42,18 +43,18 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // needs p = identity (one pass, cols <= 2048).
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, float2* nat_out, const uint8_t* rowmask = nullptr,
-                      const PageLayout* lay = nullptr);
+                      const PageLayout* lay = nullptr, int eng = ENG_F32);
 // With crop, the input is not read: the first pass gathers ifftshift(S P)
 // of each page's LED window (natural frequency columns) instead.
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, const float2* nat_in, const CropArgs* crop = nullptr,
                       const StitchArgs* stitch = nullptr, const uint8_t* rowmask = nullptr,
-                      const PageLayout* lay = nullptr);
+                      const PageLayout* lay = nullptr, int eng = ENG_F32);
 
 // full 2D transforms, in place; the frequency side is column-permuted by p
 void fft2_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                  hipStream_t st);
+                  hipStream_t st, int eng = ENG_F32);
 void fft2_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                  hipStream_t st);
+                  hipStream_t st, int eng = ENG_F32);
 
 }  // namespace fpm

@@ -27,38 +27,40 @@ template <int LEN> struct ContigCfg {
     using Map = LineSlowMap<LINES, LEN>;
 };
 
-template <int LEN, bool INV>
+template <int LEN, bool INV, int ENG>
 __global__ __launch_bounds__(ContigCfg<LEN>::T) void fft_contig_kernel(const float2* __restrict__ in,
                                                                       float2* __restrict__ out,
                                                                       uint32_t nlines, Twiddles tw) {
     using Cfg = ContigCfg<LEN>;
     using Map = typename Cfg::Map;
-    using SL = Slots<LEN, INV, Cfg::T, Map>;
+    using SL = Slots<LEN, INV, Cfg::T, Map, ENG>;
     __shared__ float2 lds[Map::LDS_WORDS];
     __shared__ float2 twl[2 * kTwLo];
     const int t = threadIdx.x;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
+    EngCtx<ENG> ctx;
+    ctx.init();
     const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
     float2 v[16];
 #pragma unroll
-    for (int q = 0; q < 16 / SL::R0; ++q)
+    for (int q = 0; q < SL::NG0; ++q)
 #pragma unroll
-        for (int s = 0; s < SL::R0; ++s) {
+        for (int s = 0; s < SL::GS0; ++s) {
             int line, idx;
             SL::in(t, q, s, line, idx);
             const size_t gl = line0 + line;
-            v[q * SL::R0 + s] = gl < nlines ? in[gl * LEN + idx] : make_float2(0.f, 0.f);
+            v[q * SL::GS0 + s] = gl < nlines ? in[gl * LEN + idx] : make_float2(0.f, 0.f);
         }
-    first_pass_dft<LEN, INV>(v);
-    remaining_passes<LEN, INV, Cfg::T, Map>(v, lds, t, tw);
+    first_pass_dft<LEN, INV, Cfg::T, ENG>(v, ctx);
+    remaining_passes<LEN, INV, Cfg::T, Map, ENG>(v, lds, t, tw, ctx);
 #pragma unroll
-    for (int q = 0; q < 16 / SL::RL; ++q)
+    for (int q = 0; q < SL::NGL; ++q)
 #pragma unroll
-        for (int s = 0; s < SL::RL; ++s) {
+        for (int s = 0; s < SL::GSL; ++s) {
             int line, idx;
             SL::out(t, q, s, line, idx);
             const size_t gl = line0 + line;
-            if (gl < nlines) out[gl * LEN + idx] = v[q * SL::RL + s];
+            if (gl < nlines) out[gl * LEN + idx] = v[q * SL::GSL + s];
         }
 }
 
@@ -119,12 +121,16 @@ template <int LEN, int TR> struct StridedCfg {
 enum { SRC_PLAIN = 0, SRC_CROP = 1, SRC_STITCH = 2 };
 constexpr int kMaxStitchLeds = 64;  // one bit per LED of the batch
 
-template <int LEN, int TR, bool INV, bool TWID, int SRC = SRC_PLAIN>
+template <int LEN, int TR, bool INV, bool TWID, int SRC, int ENG>
 __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs a, CropArgs c, StitchArgs sa,
                                                                    Twiddles tw) {
     using Cfg = StridedCfg<LEN, TR>;
     using Map = typename Cfg::Map;
-    using SL = Slots<LEN, INV, Cfg::T, Map>;
+    using SL = Slots<LEN, INV, Cfg::T, Map, ENG>;
+    // every slot of thread t lies in row t mod TR: true of the FP32 map
+    // (butterfly t + T q) and of the split map (16 consecutive lanes walk 16
+    // consecutive butterflies) when TR divides 16
+    static_assert(!(SL::MF0 || SL::MFL) || 16 % TR == 0, "split engine: strided tiles of at most 16 rows");
     __shared__ float2 lds[Map::LDS_WORDS];
     __shared__ float2 twl[2 * kTwLo];
     const int t = threadIdx.x;
@@ -139,6 +145,8 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
         if (!any) return;  // uniform over the workgroup
     }
     tw = bind_twiddles(tw, twl, t, Cfg::T);
+    EngCtx<ENG> ctx;
+    ctx.init();
     const uint32_t g = blockIdx.y;
     const size_t base = (size_t)page * a.page;
     const float2* __restrict__ in = a.in + base;
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
 #pragma unroll
         for (int k = 0; k < 16; ++k) v[k] = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int q = 0; q < 16 / SL::R0; ++q) {
+        for (int q = 0; q < SL::NG0; ++q) {
             int line, idx0;
             SL::in(t, q, 0, line, idx0);
             const uint32_t col0 = g * a.in_g + (uint32_t)idx0 * a.in_s;
@@ -196,15 +204,15 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 const uint32_t d = (col0 - (w >> 16)) & Cm;
                 const float2* __restrict__ Rl = sa.R + (size_t)l * plane + sa.nl.row_off(di);
 #pragma unroll
-                for (int s = 0; s < SL::R0; ++s) {
+                for (int s = 0; s < SL::GS0; ++s) {
                     const uint32_t lj = (d + (uint32_t)s * D) & Cm;
                     if (lj < sa.nc) {
                         const float2 pp = sa.P[lj * sa.nr + li];
                         if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0; R not read
                             const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
                             const float2 gw = cmulc(Rl[(size_t)sa.nl.perm(dj) * sa.nl.ld], pp);
-                            v[q * SL::R0 + s].x += gw.x * sa.invN2;
-                            v[q * SL::R0 + s].y += gw.y * sa.invN2;
+                            v[q * SL::GS0 + s].x += gw.x * sa.invN2;
+                            v[q * SL::GS0 + s].y += gw.y * sa.invN2;
                         }
                     }
                 }
@@ -212,9 +220,9 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < 16 / SL::R0; ++q)
+        for (int q = 0; q < SL::NG0; ++q)
 #pragma unroll
-            for (int s = 0; s < SL::R0; ++s) {
+            for (int s = 0; s < SL::GS0; ++s) {
                 int line, idx;
                 SL::in(t, q, s, line, idx);
                 const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
@@ -226,21 +234,21 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                     const float2 pp = c.P[j * c.nr + i];
                     float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
                     if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(c.Shat[(size_t)c.pN(scl) * c.Nr + srw], pp);
-                    v[q * SL::R0 + s] = z;
+                    v[q * SL::GS0 + s] = z;
                 } else {
-                    v[q * SL::R0 + s] = in[roff + col * a.ld];
+                    v[q * SL::GS0 + s] = in[roff + col * a.ld];
                 }
             }
     }
-    first_pass_dft<LEN, INV>(v);
-    remaining_passes<LEN, INV, Cfg::T, Map>(v, lds, t, tw);
+    first_pass_dft<LEN, INV, Cfg::T, ENG>(v, ctx);
+    remaining_passes<LEN, INV, Cfg::T, Map, ENG>(v, lds, t, tw, ctx);
 #pragma unroll
-    for (int q = 0; q < 16 / SL::RL; ++q)
+    for (int q = 0; q < SL::NGL; ++q)
 #pragma unroll
-        for (int s = 0; s < SL::RL; ++s) {
+        for (int s = 0; s < SL::GSL; ++s) {
             int line, idx;
             SL::out(t, q, s, line, idx);
-            float2 r = v[q * SL::RL + s];
+            float2 r = v[q * SL::GSL + s];
             if constexpr (TWID) r = cmul(r, tw.get<INV>(g * (uint32_t)idx * a.tw_step));
             const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
             out[roff + col * a.ld] = r;
@@ -248,12 +256,13 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
 }
 
 // -------------------------------------------------------------- launchers
-template <bool INV>
+template <bool INV, int ENG>
 static void launch_contig_len(int len, const float2* in, float2* out, uint32_t nlines, Twiddles tw, hipStream_t st) {
 #define FPM_CONTIG_CASE(L)                                                                              \
     case L: {                                                                                           \
         dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                            \
-        hipLaunchKernelGGL((fft_contig_kernel<L, INV>), grid, dim3(ContigCfg<L>::T), 0, st, in, out, nlines, tw); \
+        hipLaunchKernelGGL((fft_contig_kernel<L, INV, ENG>), grid, dim3(ContigCfg<L>::T), 0, st, in, out, nlines, \
+                           tw);                                                                         \
         break;                                                                                          \
     }
     switch (len) {
@@ -280,7 +289,7 @@ struct Gather {
     const StitchArgs* stitch = nullptr;
 };
 
-template <int L, int TR, bool INV, bool TWID>
+template <int L, int TR, bool INV, bool TWID, int ENG>
 static void launch_strided_one(const StridedArgs& a, Gather gs, dim3 grid, Twiddles tw, hipStream_t st) {
     const CropArgs c = gs.crop ? *gs.crop : CropArgs{};
     const StitchArgs s = gs.stitch ? *gs.stitch : StitchArgs{};
@@ -288,36 +297,38 @@ static void launch_strided_one(const StridedArgs& a, Gather gs, dim3 grid, Twidd
     if constexpr (INV) {
         if (gs.crop) {
             const dim3 pages_first(grid.z, grid.y, grid.x);
-            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_CROP>), pages_first, blk, 0, st, a, c, s,
+            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_CROP, ENG>), pages_first, blk, 0, st, a, c, s,
                                tw);
             return;
         }
         if (gs.stitch) {
-            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_STITCH>), grid, blk, 0, st, a, c, s, tw);
+            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_STITCH, ENG>), grid, blk, 0, st, a, c, s, tw);
             return;
         }
     }
-    hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_PLAIN>), grid, blk, 0, st, a, c, s, tw);
+    hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_PLAIN, ENG>), grid, blk, 0, st, a, c, s, tw);
 }
 
-template <int L, bool INV, bool TWID>
+template <int L, bool INV, bool TWID, int ENG>
 static void launch_strided_tr(const StridedArgs& a, Gather gs, dim3 grid, Twiddles tw, hipStream_t st) {
-    constexpr int TRP = strided_rows_pref(L);
+    // the split engine's slot map keeps one row per thread only for tiles of
+    // at most 16 rows (the kernel asserts it)
+    constexpr int TRP0 = strided_rows_pref(L);
+    constexpr int TRP = (ENG == ENG_SPLIT16 && TRP0 > 16) ? 16 : TRP0;
     constexpr int TRF = TRP < 16 ? TRP : 16;  // fallback tile when rows % TRP != 0
     const uint32_t rows = grid.x * kStridedRows;
     if (rows % TRP == 0) {
-        launch_strided_one<L, TRP, INV, TWID>(a, gs, dim3(rows / TRP, grid.y, grid.z), tw, st);
+        launch_strided_one<L, TRP, INV, TWID, ENG>(a, gs, dim3(rows / TRP, grid.y, grid.z), tw, st);
     } else {
-        launch_strided_one<L, TRF, INV, TWID>(a, gs, dim3(rows / TRF, grid.y, grid.z), tw, st);
+        launch_strided_one<L, TRF, INV, TWID, ENG>(a, gs, dim3(rows / TRF, grid.y, grid.z), tw, st);
     }
 }
 
-template <bool INV, bool TWID>
-static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st,
-                               Gather gs = Gather{}) {
-#define FPM_STRIDED_CASE(L)                                  \
-    case L:                                                  \
-        launch_strided_tr<L, INV, TWID>(a, gs, grid, tw, st); \
+template <bool INV, bool TWID, int ENG>
+static void launch_strided_eng(int len, const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st, Gather gs) {
+#define FPM_STRIDED_CASE(L)                                       \
+    case L:                                                       \
+        launch_strided_tr<L, INV, TWID, ENG>(a, gs, grid, tw, st); \
         break;
     switch (len) {
         FPM_STRIDED_CASE(16)
@@ -334,17 +345,29 @@ static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddle
     FPM_LAUNCH_CHECK();
 }
 
+template <bool INV, bool TWID>
+static void launch_strided_len(int len, const StridedArgs& a, dim3 grid, Twiddles tw, hipStream_t st, int eng,
+                               Gather gs = Gather{}) {
+    if (eng == ENG_SPLIT16) launch_strided_eng<INV, TWID, ENG_SPLIT16>(len, a, grid, tw, st, gs);
+    else launch_strided_eng<INV, TWID, ENG_F32>(len, a, grid, tw, st, gs);
+}
+
 void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bool inverse, const float2* twid,
-                hipStream_t st) {
+                hipStream_t st, int eng) {
     Twiddles tw{twid};
-    if (inverse) launch_contig_len<true>((int)len, in, out, nlines, tw, st);
-    else launch_contig_len<false>((int)len, in, out, nlines, tw, st);
+    if (eng == ENG_SPLIT16) {
+        if (inverse) launch_contig_len<true, ENG_SPLIT16>((int)len, in, out, nlines, tw, st);
+        else launch_contig_len<false, ENG_SPLIT16>((int)len, in, out, nlines, tw, st);
+    } else {
+        if (inverse) launch_contig_len<true, ENG_F32>((int)len, in, out, nlines, tw, st);
+        else launch_contig_len<false, ENG_F32>((int)len, in, out, nlines, tw, st);
+    }
 }
 
 // forward column transform(s): natural columns in `data` -> permuted (in
 // place) or natural (into nat_out) frequency columns
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, float2* nat_out, const uint8_t* rowmask, const PageLayout* lay) {
+                      hipStream_t st, float2* nat_out, const uint8_t* rowmask, const PageLayout* lay, int eng) {
     Twiddles tw{twid};
     StridedArgs a{};
     set_layout(a, lay ? *lay : colmajor_layout(rows, p));
@@ -356,7 +379,7 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
         a.out = nat_out ? nat_out : data;
         a.in_g = a.out_g = 0;
         a.in_s = a.out_s = 1;
-        launch_strided_len<false, false>((int)cols, a, dim3(tiles, 1, batch), tw, st);
+        launch_strided_len<false, false>((int)cols, a, dim3(tiles, 1, batch), tw, st, eng);
         return;
     }
     // pass A: length M1 over columns c2 + M2 c1, twiddle W_C^(c2 k1)
@@ -365,7 +388,7 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
     a.in_g = a.out_g = 1;
     a.in_s = a.out_s = p.M2;
     a.tw_step = kTwiddleLen / cols;
-    launch_strided_len<false, true>((int)p.M1, a, dim3(tiles, p.M2, batch), tw, st);
+    launch_strided_len<false, true>((int)p.M1, a, dim3(tiles, p.M2, batch), tw, st, eng);
     // pass B: length M2 over the contiguous block of group k1
     a.in_g = p.M2;
     a.in_s = 1;
@@ -377,14 +400,14 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
         a.out_g = p.M2;
         a.out_s = 1;
     }
-    launch_strided_len<false, false>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st);
+    launch_strided_len<false, false>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st, eng);
 }
 
 // inverse column transform(s): permuted (or natural, from nat_in) frequency
 // columns -> natural columns in `data`
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, const float2* nat_in, const CropArgs* crop, const StitchArgs* stitch,
-                      const uint8_t* rowmask, const PageLayout* lay) {
+                      const uint8_t* rowmask, const PageLayout* lay, int eng) {
     Twiddles tw{twid};
     StridedArgs a{};
     set_layout(a, lay ? *lay : colmajor_layout(rows, p));
@@ -401,7 +424,7 @@ void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
         a.out = data;
         a.in_g = a.out_g = 0;
         a.in_s = a.out_s = 1;
-        launch_strided_len<true, false>((int)cols, a, dim3(tiles, 1, batch), tw, st, gs);
+        launch_strided_len<true, false>((int)cols, a, dim3(tiles, 1, batch), tw, st, eng, gs);
         return;
     }
     // pass B^-1: length M2 over k2 for group k1, twiddle W_C^-(c2 k1)
@@ -418,24 +441,24 @@ void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
     a.out_g = p.M2;
     a.out_s = 1;
     a.tw_step = kTwiddleLen / cols;
-    launch_strided_len<true, true>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st, gs);
+    launch_strided_len<true, true>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st, eng, gs);
     // pass A^-1: length M1 over k1 for group c2
     a.in = data;
     a.in_g = a.out_g = 1;
     a.in_s = a.out_s = p.M2;
-    launch_strided_len<true, false>((int)p.M1, a, dim3(tiles, p.M2, batch), tw, st);
+    launch_strided_len<true, false>((int)p.M1, a, dim3(tiles, p.M2, batch), tw, st, eng);
 }
 
 void fft2_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                  hipStream_t st) {
-    fft_contig(data, data, rows, cols * batch, false, twid, st);
-    fft_cols_forward(data, rows, cols, batch, p, twid, st, nullptr);
+                  hipStream_t st, int eng) {
+    fft_contig(data, data, rows, cols * batch, false, twid, st, eng);
+    fft_cols_forward(data, rows, cols, batch, p, twid, st, nullptr, nullptr, nullptr, eng);
 }
 
 void fft2_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                  hipStream_t st) {
-    fft_cols_inverse(data, rows, cols, batch, p, twid, st, nullptr, nullptr, nullptr);
-    fft_contig(data, data, rows, cols * batch, true, twid, st);
+                  hipStream_t st, int eng) {
+    fft_cols_inverse(data, rows, cols, batch, p, twid, st, nullptr, nullptr, nullptr, nullptr, nullptr, eng);
+    fft_contig(data, data, rows, cols * batch, true, twid, st, eng);
 }
 
 
@@ -480,7 +503,7 @@ template <bool NT> __device__ __forceinline__ void st_s(float* p, float v) {
     else *p = v;
 }
 
-template <int LEN, bool FWD>
+template <int LEN, bool FWD, int ENG>
 __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2* __restrict__ G,
                                                                         float2* __restrict__ O,
                                                                         float* __restrict__ mO, uint32_t nlines,
@@ -488,8 +511,8 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
                                                                         Twiddles tw) {
     using Cfg = ContigCfg<LEN>;
     using Map = typename Cfg::Map;
-    using SI = Slots<LEN, true, Cfg::T, Map>;
-    using SF = Slots<LEN, false, Cfg::T, Map>;
+    using SI = Slots<LEN, true, Cfg::T, Map, ENG>;
+    using SF = Slots<LEN, false, Cfg::T, Map, ENG>;
     static_assert(SI::RL == SF::R0, "inverse last pass and forward first pass must share the slot map");
     __shared__ float2 lds[Map::LDS_WORDS];
     __shared__ float2 twl[2 * kTwLo];
@@ -499,69 +522,73 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
     const int t = threadIdx.x;
     for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rowmask ? rowmask[k] : 1;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
+    EngCtx<ENG> ctx;
+    ctx.init();
+    const float lr = rms_lr(rms);
     const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
     float2 v[16];
 #pragma unroll
-    for (int q = 0; q < 16 / SI::R0; ++q)
+    for (int q = 0; q < SI::NG0; ++q)
 #pragma unroll
-        for (int s = 0; s < SI::R0; ++s) {
+        for (int s = 0; s < SI::GS0; ++s) {
             int line, idx;
             SI::in(t, q, s, line, idx);
             const size_t gl = line0 + line;
-            v[q * SI::R0 + s] = (gl < nlines && rm[idx / kMaskRows]) ? G[gl * LEN + idx] : make_float2(0.f, 0.f);
+            v[q * SI::GS0 + s] = (gl < nlines && rm[idx / kMaskRows]) ? G[gl * LEN + idx] : make_float2(0.f, 0.f);
         }
-    first_pass_dft<LEN, true>(v);
-    remaining_passes<LEN, true, Cfg::T, Map>(v, lds, t, tw);
+    first_pass_dft<LEN, true, Cfg::T, ENG>(v, ctx);
+    remaining_passes<LEN, true, Cfg::T, Map, ENG>(v, lds, t, tw, ctx);
 #pragma unroll
-    for (int q = 0; q < 16 / SI::RL; ++q)
+    for (int q = 0; q < SI::NGL; ++q)
 #pragma unroll
-        for (int s = 0; s < SI::RL; ++s) {
+        for (int s = 0; s < SI::GSL; ++s) {
             int line, idx;
             SI::out(t, q, s, line, idx);
             const size_t gl = line0 + line;
             if (gl < nlines) {
                 const size_t k = gl * LEN + idx;
-                const float2 gr = v[q * SI::RL + s];
+                const float2 gr = v[q * SI::GSL + s];
                 constexpr bool NT = FPM_NT_UPDATE >= 1;
                 float mu = gr.x * gr.x + gr.y * gr.y;
                 mu = rms.beta * ld_s<NT>(mO + k) + (1.0f - rms.beta) * mu;
-                const float sq = rms.lr / (sqrtf(mu) + rms.eps);
+                const float sq = lr / (sqrtf(mu) + rms.eps);
                 float2 w = ld_s<NT>(O + k);
                 w.x -= gr.x * sq;
                 w.y -= gr.y * sq;
                 st_s<NT>(O + k, w);
                 st_s<NT>(mO + k, mu);
-                v[q * SI::RL + s] = w;
+                v[q * SI::GSL + s] = w;
             }
         }
     if constexpr (FWD) {
-        first_pass_dft<LEN, false>(v);
+        first_pass_dft<LEN, false, Cfg::T, ENG>(v, ctx);
         __syncthreads();
-        remaining_passes<LEN, false, Cfg::T, Map>(v, lds, t, tw);
+        remaining_passes<LEN, false, Cfg::T, Map, ENG>(v, lds, t, tw, ctx);
 #pragma unroll
-        for (int q = 0; q < 16 / SF::RL; ++q)
+        for (int q = 0; q < SF::NGL; ++q)
 #pragma unroll
-            for (int s = 0; s < SF::RL; ++s) {
+            for (int s = 0; s < SF::GSL; ++s) {
                 int line, idx;
                 SF::out(t, q, s, line, idx);
                 const size_t gl = line0 + line;
-                if (gl < nlines) st_s<(FPM_NT_UPDATE >= 2)>(G + gl * LEN + idx, v[q * SF::RL + s]);
+                if (gl < nlines) st_s<(FPM_NT_UPDATE >= 2)>(G + gl * LEN + idx, v[q * SF::GSL + s]);
             }
     }
 }
 
-void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                         const uint8_t* rowmask, const float2* twid, hipStream_t st) {
+template <int ENG>
+static void launch_fused_update_eng(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
+                                    const uint8_t* rowmask, const float2* twid, hipStream_t st) {
     Twiddles tw{twid};
     const uint32_t nlines = g.Nc;
 #define FPM_FUSED_CASE(L)                                                                                   \
     case L: {                                                                                               \
         dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                                \
         if (forward_after)                                                                                  \
-            hipLaunchKernelGGL((fused_update_kernel<L, true>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
+            hipLaunchKernelGGL((fused_update_kernel<L, true, ENG>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
                                rms, rowmask, tw);                                                           \
         else                                                                                                \
-            hipLaunchKernelGGL((fused_update_kernel<L, false>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
+            hipLaunchKernelGGL((fused_update_kernel<L, false, ENG>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
                                rms, rowmask, tw);                                                           \
         break;                                                                                              \
     }
@@ -583,6 +610,12 @@ void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs
     FPM_LAUNCH_CHECK();
 }
 
+void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
+                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng) {
+    if (eng == ENG_SPLIT16) launch_fused_update_eng<ENG_SPLIT16>(g, G, O, mO, rms, forward_after, rowmask, twid, st);
+    else launch_fused_update_eng<ENG_F32>(g, G, O, mO, rms, forward_after, rowmask, twid, st);
+}
+
 // ------------------------ per-LED contiguous passes fused with the stencil
 // One workgroup owns K = LT - 2 interior columns of one LED page plus the
 // column on each side, which the stencil needs as neighbours and which are
@@ -594,6 +627,9 @@ void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs
 // reach HBM: 21 B per pixel instead of 50 for three separate kernels.
 // The pages may be column-major or in 8-row tiles (Geom::nl): either way a
 // wave walks 64 points down one column, 8 runs of 64 bytes in the tiled case.
+#ifndef FPM_STENCIL_MAG_LDS
+#define FPM_STENCIL_MAG_LDS 0
+#endif
 template <int LEN> struct StencilCfg {
     static constexpr int T = 1024;
     static constexpr int LT = T * 16 / LEN;  // columns per workgroup
@@ -601,15 +637,16 @@ template <int LEN> struct StencilCfg {
     using Map = LineSlowMap<LT, LEN>;
 };
 
-template <int LEN>
+template <int LEN, int ENG>
 __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,
-                                                            float gscale, double* __restrict__ loss_acc,
+                                                            float inv_ratio, float gscale,
+                                                            double* __restrict__ loss_acc,
                                                             const uint8_t* __restrict__ rowmask, Twiddles tw) {
     using Cfg = StencilCfg<LEN>;
     using Map = typename Cfg::Map;
-    using SI = Slots<LEN, true, Cfg::T, Map>;
-    using SF = Slots<LEN, false, Cfg::T, Map>;
+    using SI = Slots<LEN, true, Cfg::T, Map, ENG>;
+    using SF = Slots<LEN, false, Cfg::T, Map, ENG>;
     static_assert(SI::RL == SF::R0, "inverse last pass and forward first pass must share the slot map");
     static_assert(Cfg::K >= 2, "too few interior columns per workgroup");
     __shared__ float2 lds[Map::LDS_WORDS];
@@ -620,6 +657,8 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
     const int t = threadIdx.x;
     for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rowmask ? rowmask[k] : 1;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
+    EngCtx<ENG> ctx;
+    ctx.init();
     const uint32_t page = blockIdx.y;
     const uint32_t j0 = blockIdx.x * Cfg::K;  // first interior column; line l is column j0 - 1 + l
     const uint32_t cmask = g.nc - 1;
@@ -634,34 +673,39 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
 
     float2 v[16];
 #pragma unroll
-    for (int q = 0; q < 16 / SI::R0; ++q)
+    for (int q = 0; q < SI::NG0; ++q)
 #pragma unroll
-        for (int s = 0; s < SI::R0; ++s) {
+        for (int s = 0; s < SI::GS0; ++s) {
             int line, idx;
             SI::in(t, q, s, line, idx);
-            v[q * SI::R0 + s] = rm[idx / kMaskRows] ? src[at(idx, line)] : make_float2(0.f, 0.f);
+            v[q * SI::GS0 + s] = rm[idx / kMaskRows] ? src[at(idx, line)] : make_float2(0.f, 0.f);
         }
-    first_pass_dft<LEN, true>(v);
-    remaining_passes<LEN, true, Cfg::T, Map>(v, lds, t, tw);
+    first_pass_dft<LEN, true, Cfg::T, ENG>(v, ctx);
+    remaining_passes<LEN, true, Cfg::T, Map, ENG>(v, lds, t, tw, ctx);
     __syncthreads();
-    // e of every column of the workgroup, in LDS (the exchange buffer as floats)
+    // e of every column of the workgroup, in LDS (the exchange buffer as
+    // floats); with FPM_STENCIL_MAG_LDS also |z| (LT*LEN words each, within
+    // the LT*PADLEN float2 of the buffer), else |z| is formed again below
     float* ef = (float*)lds;
+    float* mf = ef + Cfg::LT * LEN;
+    static_assert(2 * Cfg::LT * LEN <= 2 * Map::LDS_WORDS, "e and |z| must fit the exchange buffer");
 #pragma unroll
-    for (int q = 0; q < 16 / SI::RL; ++q)
+    for (int q = 0; q < SI::NGL; ++q)
 #pragma unroll
-        for (int s = 0; s < SI::RL; ++s) {
+        for (int s = 0; s < SI::GSL; ++s) {
             int line, idx;
             SI::out(t, q, s, line, idx);
-            const float2 z = v[q * SI::RL + s];
-            ef[line * LEN + idx] = sqrtf(z.x * z.x + z.y * z.y) / ratio -
-                                   read_y(y, ybase + (size_t)column(line) * LEN + idx);
+            const float2 z = v[q * SI::GSL + s];
+            const float mag = sqrtf(z.x * z.x + z.y * z.y);
+            if constexpr (FPM_STENCIL_MAG_LDS) mf[line * LEN + idx] = mag;
+            ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) - read_y(y, ybase + (size_t)column(line) * LEN + idx);
         }
     __syncthreads();
     float loss32 = 0.0f;  // 32 terms per thread at most; the workgroup sum is FP64
 #pragma unroll
-    for (int q = 0; q < 16 / SI::RL; ++q)
+    for (int q = 0; q < SI::NGL; ++q)
 #pragma unroll
-        for (int s = 0; s < SI::RL; ++s) {
+        for (int s = 0; s < SI::GSL; ++s) {
             int line, idx;
             SI::out(t, q, s, line, idx);
             float2 o = make_float2(0.f, 0.f);
@@ -673,25 +717,25 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
                 const float gval = (copysignf(1.0f, e0 - el) - copysignf(1.0f, er - e0) + copysignf(1.0f, e0 - eu) -
                                     copysignf(1.0f, ed - e0)) * gscale;
                 loss32 += fabsf(er - e0) + fabsf(ed - e0);
-                const float2 z = v[q * SI::RL + s];
-                const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                const float2 z = v[q * SI::GSL + s];
+                const float mag = FPM_STENCIL_MAG_LDS ? mf[line * LEN + idx] : sqrtf(z.x * z.x + z.y * z.y);
                 if (mag > 0.0f) {
                     const float inv = 1.0f / mag;
                     o = make_float2(z.x * inv * gval, z.y * inv * gval);
                 }
             }
-            v[q * SI::RL + s] = o;
+            v[q * SI::GSL + s] = o;
         }
-    first_pass_dft<LEN, false>(v);
+    first_pass_dft<LEN, false, Cfg::T, ENG>(v, ctx);
     __syncthreads();
-    remaining_passes<LEN, false, Cfg::T, Map>(v, lds, t, tw);
+    remaining_passes<LEN, false, Cfg::T, Map, ENG>(v, lds, t, tw, ctx);
 #pragma unroll
-    for (int q = 0; q < 16 / SF::RL; ++q)
+    for (int q = 0; q < SF::NGL; ++q)
 #pragma unroll
-        for (int s = 0; s < SF::RL; ++s) {
+        for (int s = 0; s < SF::GSL; ++s) {
             int line, idx;
             SF::out(t, q, s, line, idx);
-            if (interior(line)) dst[at(idx, line)] = v[q * SF::RL + s];
+            if (interior(line)) dst[at(idx, line)] = v[q * SF::GSL + s];
         }
     if (loss_acc) {
         double loss = (double)loss32;
@@ -709,17 +753,19 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
 
 bool stencil_fused_supported(const Geom& g) { return g.nr <= 2048; }
 
-void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
-                          float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
-                          hipStream_t st) {
+template <int ENG>
+static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b,
+                                     int pratio, float2* u_out, double* loss_acc, const uint8_t* rowmask,
+                                     const float2* twid, hipStream_t st) {
     Twiddles tw{twid};
     const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
     const float gscale = (float)(pratio * pratio);
+    const float inv_ratio = exact_inverse_or_zero(ratio);
 #define FPM_STF_CASE(L)                                                                                   \
     case L: {                                                                                             \
         dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                                   \
-        hipLaunchKernelGGL((stencil_fused_kernel<L>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, led0, ratio, \
-                           gscale, loss_acc, rowmask, tw);                                                 \
+        hipLaunchKernelGGL((stencil_fused_kernel<L, ENG>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, led0, ratio, \
+                           inv_ratio, gscale, loss_acc, rowmask, tw);                                      \
         break;                                                                                            \
     }
     switch (g.nr) {
@@ -735,6 +781,15 @@ void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, 
     }
 #undef FPM_STF_CASE
     FPM_LAUNCH_CHECK();
+}
+
+void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
+                          float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
+                          hipStream_t st, int eng) {
+    if (eng == ENG_SPLIT16)
+        launch_stencil_fused_eng<ENG_SPLIT16>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st);
+    else
+        launch_stencil_fused_eng<ENG_F32>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st);
 }
 
 }  // namespace fpm

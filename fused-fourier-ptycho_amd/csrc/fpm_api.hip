@@ -181,7 +181,14 @@ YStack prepare_y(const void* Y, int32_t y_dtype, size_t count, bool is_device, D
     y.u16 = 1;
     y.mn = h[0];
     y.range = (float)(h[1] - h[0]);
+    y.rcp_range = 1.0f / y.range;
     return y;
+}
+
+int engine_of(int32_t fft_engine) {
+    if (fft_engine == FPM_FFT_F32) return ENG_F32;
+    if (fft_engine == FPM_FFT_SPLIT_F16) return ENG_SPLIT16;
+    invalid("fft_engine must be FPM_FFT_F32 or FPM_FFT_SPLIT_F16");
 }
 
 int run_guarded(const char* fn, const std::function<void()>& body) {
@@ -263,6 +270,36 @@ struct StageTimer {
     }
 };
 
+// Instantiated epoch graphs of one solve, keyed by (spectrum buffer parity,
+// last epoch): capture runs the enqueue function on the stream in capture
+// mode (nothing executes), then the graph is instantiated once and replayed.
+struct GraphCache {
+    hipGraph_t graph[4] = {};
+    hipGraphExec_t exec[4] = {};
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache() {
+        for (int k = 0; k < 4; ++k) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        }
+    }
+    void capture(int key, hipStream_t st, const std::function<void()>& enqueue) {
+        FPM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FPM_HIP(hipStreamEndCapture(st, &graph[key]));
+        FPM_HIP(hipGraphInstantiate(&exec[key], graph[key], nullptr, nullptr, 0));
+    }
+};
+
 // kernel launches of the column passes of a C-column transform
 inline int col_launches(const ColPerm& p) { return p.M2 == 1 ? 1 : 2; }
 
@@ -294,26 +331,26 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
     const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
     tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
-    fft_cols_inverse(lat, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, &ca, nullptr, lat_mask, &g.nl);
+    fft_cols_inverse(lat, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, &ca, nullptr, lat_mask, &g.nl, g.eng);
     tm.end(col_launches(g.nl.perm));
     if (fused) {
         // contiguous inverse pass + stencil + contiguous forward pass in one kernel
         tm.begin(FPM_STAGE_STENCIL);
-        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, lat_mask, tw, st);
+        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, lat_mask, tw, st, g.eng);
         tm.end(1);
     } else {
         tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
-        fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st);
+        fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st, g.eng);
         tm.end(1);
         tm.begin(FPM_STAGE_STENCIL);
         launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st);
         tm.end(1);
         tm.begin(FPM_STAGE_FFT_N_CONTIG_LO);
-        fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st);
+        fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st, g.eng);
         tm.end(1);
     }
     tm.begin(FPM_STAGE_FFT_N_COLS_LO);
-    fft_cols_forward(rec, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, rec_mask, &g.nl);
+    fft_cols_forward(rec, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, rec_mask, &g.nl, g.eng);
     tm.end(col_launches(g.nl.perm));
     // stitch into Gs, pupil gradient / update, then the column passes of IDFT_N(Gs)
     if (stitch == FPM_STITCH_OWNER && b <= 64) {
@@ -321,7 +358,7 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
         // pupil update must come after it (the stitch reads this step's pupil)
         const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.nl, 1.0f / (float)(g.Nr * g.Nc)};
         tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
-        fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa, rowmask);
+        fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa, rowmask, nullptr, g.eng);
         tm.end(col_launches(g.pN));
         tm.begin(FPM_STAGE_BACKPROJ);
         launch_backproj(g, rec, Shat, led, led0, b, P, Gs, kBackprojPupil, mP, A, rms, GP, st);
@@ -344,7 +381,7 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
         tm.end(1);
     }
     tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
-    fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, nullptr, rowmask);
+    fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, nullptr, rowmask, nullptr, g.eng);
     tm.end(col_launches(g.pN));
 }
 
@@ -380,7 +417,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
     return run_guarded("fpm_pure_v3", [&] {
         const fpm_opts o = merged_opts(opts);
         if (!O0 || !P0 || !Y || !led_idx || !A || !opt || !O_out || !P_out) invalid("null array argument");
-        const Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        (void)make_geom(N_rows, N_cols, n_rows, n_cols);  // shape errors before argument errors
         if (L < 1) invalid("L (number of images) must be >= 1");
         // doubles truncated to int as the mex does (cuFPM_pure_v3.cu:57-59)
         const int pr = (int)pratio, ne = (int)epochs, bs = (int)batch;
@@ -388,7 +425,9 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         if (ne < 0) invalid("numEpochs must be >= 0");
         if (bs < 1) invalid("batch must be >= 1");
         if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
-        float lr = (float)opt[0];
+        const float lr = (float)opt[0];
+        Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        g.eng = engine_of(o.fft_engine);
         const float beta = (float)opt[1], eps = (float)opt[2];
         if (o.verbose) {
             std::printf("prhs4: %d, prhs5: %d, prhs6: %d \n", pr, ne, bs);
@@ -428,7 +467,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         const float* dA = upload(A, nn, isdev, hA, st);
 
         if (ne > 0) {
-            DevBuf S0, S1, mO, mP, lossd;
+            DevBuf S0, S1, mO, mP, lossd, state;
             Workspace ws;
             S0.alloc(NN * sizeof(float2));
             S1.alloc(NN * sizeof(float2));
@@ -438,13 +477,20 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             ws.rec.alloc(nn * B * sizeof(float2));
             FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
             FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
+            // device-resident solve state: lr, epoch counter, FP64 loss slots
+            state.alloc(64 + (o.loss_per_epoch ? (size_t)ne * sizeof(double) : 0));
+            SolveState ss{};
+            ss.loss_cur = (double*)state.p;
+            ss.lr = (float*)(state.as<char>() + 8);
+            ss.epoch = (uint32_t*)(state.as<char>() + 16);
             if (o.loss_per_epoch) {
-                lossd.alloc((size_t)ne * sizeof(double));
-                FPM_HIP(hipMemsetAsync(lossd.p, 0, (size_t)ne * sizeof(double), st));
+                ss.loss_hist = (double*)(state.as<char>() + 64);
+            } else {
+                ss.loss_cur = nullptr;
             }
-            float2* Shat = S0.as<float2>();
-            float2* Gs = S1.as<float2>();
+            launch_solve_state_init(ss, lr, st);
             StageTimer tm(o.stage_times, st);
+            const bool use_graph = o.graph != 0 && o.stage_times == nullptr;
             // Row masks of the window rows of the current and the next batch:
             // outside them the spectrum gradient is zero and the spectrum is
             // never read, so the N-side column passes skip those row tiles.
@@ -455,52 +501,69 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             uint8_t* pmask = masks.as<uint8_t>() + 2 * mrows;  // pupil-support row mask (n side)
             uint8_t* psup = pmask + g.nr / kMaskRows;          // pupil-support rows
             uint8_t* mk[2] = {masks.as<uint8_t>(), masks.as<uint8_t>() + mrows};
+            float2* spec[2] = {S0.as<float2>(), S1.as<float2>()};
             // rows where P != 0 or A != 0: the update keeps P = 0 wherever
             // A = 0, so the masks of the initial pupil hold for every step
             launch_pupil_row_mask(P, dA, g.nr, g.nc, pmask, psup, st);
-            int cur = 0;
             auto batch_of = [&](uint32_t s, uint32_t& led0, uint32_t& b) {
                 led0 = s * B;
                 b = std::min<uint32_t>(B, (uint32_t)L - led0);
             };
             uint32_t led0, b;
             batch_of(0, led0, b);
-            launch_row_mask(led, led0, b, g.Nr, g.nr, psup, mk[cur], st);
+            launch_row_mask(led, led0, b, g.Nr, g.nr, psup, mk[0], st);
             // spectrum of the initial object (rows of the first batch)
             tm.begin(FPM_STAGE_SPECTRUM_INIT);
-            fft_contig(O, Shat, g.Nr, g.Nc, false, tw, st);
-            fft_cols_forward(Shat, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[cur]);
+            fft_contig(O, spec[0], g.Nr, g.Nc, false, tw, st, g.eng);
+            fft_cols_forward(spec[0], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[0], nullptr, g.eng);
             tm.end(1 + col_launches(g.pN));
-            for (int e = 0; e < ne; ++e) {
-                const RmsArgs rms{lr, beta, eps, o.pupil_clamp};
-                double* lacc = o.loss_per_epoch ? lossd.as<double>() + e : nullptr;
+            const RmsArgs rms{lr, beta, eps, o.pupil_clamp, ss.lr};
+            // One epoch's launches, given which of the two spectrum buffers
+            // (and row masks) holds the current spectrum: every step reads
+            // spec[cur] and, unless it is the solve's last step, leaves the
+            // next step's spectrum in spec[1 - cur].  The launches depend only
+            // on (cur, last): lr, the epoch index and the loss slot live in
+            // device memory, so an epoch can be captured and replayed.
+            auto enqueue_epoch = [&](int cur, bool last_epoch) {
                 for (uint32_t s = 0; s < steps; ++s) {
                     batch_of(s, led0, b);
-                    minibatch_backward(g, Shat, P, led, led0, b, y, pr, ws, Gs, o.stitch, mP.as<float>(), dA, rms,
-                                       nullptr, lacc, tw, st, tm, mk[cur], pmask);
-                    const bool last = (e == ne - 1) && (s == steps - 1);
+                    minibatch_backward(g, spec[cur], P, led, led0, b, y, pr, ws, spec[1 - cur], o.stitch,
+                                       mP.as<float>(), dA, rms, nullptr, ss.loss_cur, tw, st, tm, mk[cur], pmask);
+                    const bool last = last_epoch && (s == steps - 1);
                     if (!last) {  // the next minibatch: s + 1, or the first one of the next epoch
                         uint32_t nled0, nb;
                         batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
                         launch_row_mask(led, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st);
                     }
                     tm.begin(FPM_STAGE_FUSED_UPDATE);
-                    launch_fused_update(g, Gs, O, mO.as<float>(), rms, !last, mk[cur], tw, st);
+                    launch_fused_update(g, spec[1 - cur], O, mO.as<float>(), rms, !last, mk[cur], tw, st, g.eng);
                     tm.end(1);
                     if (!last) {
                         tm.begin(FPM_STAGE_FFT_N_COLS_HI);
-                        fft_cols_forward(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[1 - cur]);
+                        fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng);
                         tm.end(col_launches(g.pN));
-                        std::swap(Shat, Gs);
                         cur = 1 - cur;
                     }
                 }
-                // cuFPM_pure_v3.cu:204-206 (float lr times double 0.5)
-                if (e > 12) lr = (float)((double)lr * 0.5);
+                // loss of the epoch to its slot; lr halving (cuFPM_pure_v3.cu:204-206)
+                launch_epoch_end(ss, st);
+            };
+            GraphCache graphs;
+            int cur = 0;
+            for (int e = 0; e < ne; ++e) {
+                const bool last_epoch = e == ne - 1;
+                if (use_graph) {
+                    const int key = cur * 2 + (last_epoch ? 1 : 0);
+                    if (!graphs.exec[key]) graphs.capture(key, st, [&] { enqueue_epoch(cur, last_epoch); });
+                    FPM_HIP(hipGraphLaunch(graphs.exec[key], st));
+                } else {
+                    enqueue_epoch(cur, last_epoch);
+                }
+                cur = (int)((cur + steps) & 1u);
             }
             if (o.loss_per_epoch)
-                FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, lossd.p, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost,
-                                       st));
+                FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, ss.loss_hist, (size_t)ne * sizeof(double),
+                                       hipMemcpyDeviceToHost, st));
             FPM_HIP(hipStreamSynchronize(st));
             tm.flush();
         }
@@ -518,7 +581,8 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
     return run_guarded("fpm_fused_gradient", [&] {
         const fpm_opts o = merged_opts(opts);
         if (!O_in || !P_in || !Y || !led_idx || !G_O || !G_P) invalid("null array argument");
-        const Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        g.eng = engine_of(o.fft_engine);
         if (L < 1) invalid("L (number of images) must be >= 1");
         const int pr = (int)pratio;
         if (pr < 1) invalid("pratio must be >= 1");
@@ -557,8 +621,8 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         }
         StageTimer tm(o.stage_times, st);
         tm.begin(FPM_STAGE_SPECTRUM_INIT);
-        fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st);
-        fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+        fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st, g.eng);
+        fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, nullptr, g.eng);
         tm.end(1 + col_launches(g.pN));
         minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
                            o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
@@ -570,7 +634,7 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             gout.alloc(NN * sizeof(float2));
             GOd = gout.as<float2>();
         }
-        fft_contig(S1.as<float2>(), GOd, g.Nr, g.Nc, true, tw, st);
+        fft_contig(S1.as<float2>(), GOd, g.Nr, g.Nc, true, tw, st, g.eng);
         if (!isdev) {
             FPM_HIP(hipMemcpyAsync(G_O, GOd, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
             FPM_HIP(hipMemcpyAsync(G_P, GPd, nn * sizeof(float2), hipMemcpyDeviceToHost, st));
@@ -627,7 +691,13 @@ int fpm_forward_intensity(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, c
 
 int fpm_fft2(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols, int64_t batch, int32_t direction,
              int32_t pointers_are_device, void* stream) {
+    return fpm_fft2_engine(in, out, rows, cols, batch, direction, FPM_FFT_F32, pointers_are_device, stream);
+}
+
+int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols, int64_t batch, int32_t direction,
+                    int32_t engine, int32_t pointers_are_device, void* stream) {
     return run_guarded("fpm_fft2", [&] {
+        const int eng = engine_of(engine);
         if (!in || !out) invalid("null array argument");
         check_side(rows, "rows");
         check_side(cols, "cols");
@@ -653,18 +723,18 @@ int fpm_fft2(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols, int64_
         }
         const uint32_t R = (uint32_t)rows, C = (uint32_t)cols, Bt = (uint32_t)batch;
         if (direction < 0) {
-            fft_contig(src, dst, R, C * Bt, false, tw, st);
+            fft_contig(src, dst, R, C * Bt, false, tw, st, eng);
             if (p.M2 == 1) {
-                fft_cols_forward(dst, R, C, Bt, p, tw, st, nullptr);
+                fft_cols_forward(dst, R, C, Bt, p, tw, st, nullptr, nullptr, nullptr, eng);
             } else {
                 tmp.alloc(total * sizeof(float2));
-                fft_cols_forward(dst, R, C, Bt, p, tw, st, tmp.as<float2>());
+                fft_cols_forward(dst, R, C, Bt, p, tw, st, tmp.as<float2>(), nullptr, nullptr, eng);
                 FPM_HIP(hipMemcpyAsync(dst, tmp.p, total * sizeof(float2), hipMemcpyDeviceToDevice, st));
             }
         } else {
             if (p.M2 == 1) {
                 if (src != dst) FPM_HIP(hipMemcpyAsync(dst, src, total * sizeof(float2), hipMemcpyDeviceToDevice, st));
-                fft_cols_inverse(dst, R, C, Bt, p, tw, st, nullptr);
+                fft_cols_inverse(dst, R, C, Bt, p, tw, st, nullptr, nullptr, nullptr, nullptr, nullptr, eng);
             } else {
                 const float2* nat = src;
                 if (src == dst) {
@@ -672,9 +742,9 @@ int fpm_fft2(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols, int64_
                     FPM_HIP(hipMemcpyAsync(tmp.p, src, total * sizeof(float2), hipMemcpyDeviceToDevice, st));
                     nat = tmp.as<float2>();
                 }
-                fft_cols_inverse(dst, R, C, Bt, p, tw, st, nat);
+                fft_cols_inverse(dst, R, C, Bt, p, tw, st, nat, nullptr, nullptr, nullptr, nullptr, eng);
             }
-            fft_contig(dst, dst, R, C * Bt, true, tw, st);
+            fft_contig(dst, dst, R, C * Bt, true, tw, st, eng);
         }
         if (!isdev) FPM_HIP(hipMemcpyAsync(out, dst, total * sizeof(float2), hipMemcpyDeviceToHost, st));
         FPM_HIP(hipStreamSynchronize(st));

@@ -12,6 +12,7 @@ struct Geom {
     ColPerm pn;        // column permutation of n-side spectra (column-major pages)
     ColPerm pN;        // column permutation of the N-side spectrum
     PageLayout nl;     // layout of the per-LED pages of a minibatch (tiled or column-major with pn)
+    int eng = ENG_F32; // FFT engine of every transform of the solve
     uint32_t plane() const { return nr * nc; }
 };
 
@@ -22,22 +23,61 @@ struct YStack {
     int32_t u16;       // 0: float amplitudes, 1: uint16 raw
     uint32_t mn;       // raw minimum (u16 only)
     float range;       // (float)(mx - mn) (u16 only)
+    float rcp_range;   // 1.0f / range rounded to nearest (u16 only)
 };
+
+// a / b rounded to nearest for integers 0 <= a < 2^16, 1 <= b < 2^16, given
+// rb = RN(1/b): q = RN(a rb) is within an ulp, the remainder a - q b is
+// exact under FMA and one correction q + r rb rounds correctly (Markstein's
+// division theorem).  Three operations instead of the IEEE division
+// sequence; equality with a / b for every such (a, b) is checked
+// exhaustively by the test suite.
+__device__ __forceinline__ float div_u16(float a, float b, float rb) {
+    const float q = a * rb;
+    const float r = fmaf(-q, b, a);
+    return fmaf(r, rb, q);
+}
 
 // amplitude at element `off` of the stack (uint16: (I - min)/(max - min) in
 // float, then sqrt -- the single-precision steps of main_run_FPM.m:43-45)
 __device__ __forceinline__ float read_y(const YStack& y, size_t off) {
     if (y.u16) {
         const uint32_t raw = ((const uint16_t*)y.data)[off];
-        return sqrtf((float)(raw - y.mn) / y.range);
+        return sqrtf(div_u16((float)(raw - y.mn), y.range, y.rcp_range));
     }
     return ((const float*)y.data)[off];
 }
 
-// RMSprop + pupil constraint constants of one step
+// |z| / ratio as the reference's absC (cuFPM_v3/cuda/kernels.cuh:5-9); when
+// ratio = n_r n_c r^2 is a power of two, inv_ratio = 1/ratio is exact and the
+// product equals the quotient; otherwise inv_ratio is 0 and the division runs
+__device__ __forceinline__ float scaled_mag(float mag, float ratio, float inv_ratio) {
+    return inv_ratio > 0.0f ? mag * inv_ratio : mag / ratio;
+}
+
+// RMSprop + pupil constraint constants of one step.  lr_dev (optional): the
+// learning rate lives in device memory (the solve keeps it there so that the
+// epoch-end kernel can halve it and every epoch's launches are identical,
+// which lets an epoch be captured once as a graph and replayed); else lr.
 struct RmsArgs {
     float lr, beta, eps, clamp;
+    const float* lr_dev = nullptr;
 };
+__device__ __forceinline__ float rms_lr(const RmsArgs& r) { return r.lr_dev ? *r.lr_dev : r.lr; }
+
+// solve state in device memory: learning rate, epoch counter, FP64 loss of
+// the running epoch and the per-epoch loss history
+struct SolveState {
+    float* lr;
+    uint32_t* epoch;
+    double* loss_cur;   // may be null (no loss requested)
+    double* loss_hist;  // [epochs], may be null
+};
+// lr = lr0, epoch = 0, loss_cur = 0 (a kernel, not a memset: graph-safe)
+void launch_solve_state_init(const SolveState& s, float lr0, hipStream_t st);
+// end of an epoch: loss_hist[epoch] = loss_cur, loss_cur = 0, lr halved after
+// every epoch with index > 12 (cuFPM_v3/cuFPM_pure_v3.cu:204-206), epoch += 1
+void launch_epoch_end(const SolveState& s, hipStream_t st);
 
 // feature-domain loss gradient (kernels.cuh:197-275): z (b pages, natural) and
 // images led0.. -> u.  loss_acc (device double, may be null) += sum |Dx e|+|Dy e|.
@@ -68,14 +108,13 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 bool stencil_fused_supported(const Geom& g);
 void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
                           float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
-                          hipStream_t st);
+                          hipStream_t st, int eng = ENG_F32);
 
 // contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
 // [+ contiguous forward pass of the new O, written back into G]
 // rowmask (optional, Nr/16 bytes): gradient rows of 0-entries are taken as zero
 void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                         const uint8_t* rowmask,
-                         const float2* twid, hipStream_t st);
+                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng = ENG_F32);
 
 // per 16-row tile of the N-side spectrum: 1 if a LED window of the batch
 // covers a row of the tile at a pupil row i with sup[i] != 0 (sup: nr bytes,

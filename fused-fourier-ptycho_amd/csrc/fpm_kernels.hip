@@ -27,8 +27,8 @@ __device__ __forceinline__ float sgnf(float x) { return copysignf(1.0f, x); }
 constexpr int kStTR = 64, kStTC = 32, kStLD = kStTR + 2;
 
 __global__ __launch_bounds__(256) void stencil_kernel(Geom g, const float2* __restrict__ z, YStack y, uint32_t led0,
-                                                      float ratio, float gscale, float2* __restrict__ u,
-                                                      double* __restrict__ loss_acc) {
+                                                      float ratio, float inv_ratio, float gscale,
+                                                      float2* __restrict__ u, double* __restrict__ loss_acc) {
     __shared__ float es[(kStTC + 2) * kStLD];
     __shared__ double red[4];
     const int t = threadIdx.x;
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void stencil_kernel(Geom g, const float2* __re
         const uint32_t off = gj * g.nr + gi;
         const float2 zz = z[page + off];
         if (zout) *zout = zz;
-        return sqrtf(zz.x * zz.x + zz.y * zz.y) / ratio - read_y(y, ypage + off);
+        return scaled_mag(sqrtf(zz.x * zz.x + zz.y * zz.y), ratio, inv_ratio) - read_y(y, ypage + off);
     };
 
     float2 zc[8];
@@ -103,7 +103,8 @@ void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t le
     const float ratio = (float)(plane * (uint32_t)(pratio * pratio));
     const float gscale = (float)(pratio * pratio);
     dim3 grid((g.nr + kStTR - 1) / kStTR, (g.nc + kStTC - 1) / kStTC, b);
-    hipLaunchKernelGGL(stencil_kernel, grid, dim3(256), 0, st, g, z, y, led0, ratio, gscale, u, loss_acc);
+    hipLaunchKernelGGL(stencil_kernel, grid, dim3(256), 0, st, g, z, y, led0, ratio, exact_inverse_or_zero(ratio), gscale,
+                       u, loss_acc);
     FPM_LAUNCH_CHECK();
 }
 
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
     if (mP) {
         float mu = gp.x * gp.x + gp.y * gp.y;
         mu = rms.beta * mP[pix] + (1.0f - rms.beta) * mu;
-        const float sq = rms.lr / (sqrtf(mu) + rms.eps);
+        const float sq = rms_lr(rms) / (sqrtf(mu) + rms.eps);
         float2 w = p;
         w.x -= gp.x * sq;
         w.y -= gp.y * sq;
@@ -340,17 +341,19 @@ void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_
 }
 
 // ---------------------------------------------------------------- helpers
-__global__ __launch_bounds__(256) void zero_kernel(float4* __restrict__ p, size_t n4) {
+// a fill kernel rather than hipMemsetAsync: the call may sit in a captured graph
+__global__ __launch_bounds__(256) void zero_kernel(float4* __restrict__ p, size_t n4, float2* __restrict__ tail) {
     for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256)
         p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tail && blockIdx.x == 0 && threadIdx.x == 0) *tail = make_float2(0.f, 0.f);
 }
 
 void launch_zero(float2* p, size_t count, hipStream_t st) {
-    // count complex = count/2 float4 (all buffers here hold an even count)
+    // count complex = count/2 float4 (+ one float2 when count is odd)
     const size_t n4 = count / 2;
-    const size_t blocks = std::min<size_t>((n4 + 255) / 256, 1u << 20);
-    if (n4) hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (float4*)p, n4);
-    if (count & 1) FPM_HIP(hipMemsetAsync(p + count - 1, 0, sizeof(float2), st));
+    const size_t blocks = std::max<size_t>(1, std::min<size_t>((n4 + 255) / 256, 1u << 20));
+    if (count) hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (float4*)p, n4,
+                                  (count & 1) ? p + count - 1 : nullptr);
     FPM_LAUNCH_CHECK();
 }
 
@@ -396,14 +399,47 @@ void launch_u16_minmax(const uint16_t* raw, size_t count, uint32_t* minmax, hipS
 }
 
 __global__ __launch_bounds__(256) void u16_to_amp_kernel(const uint16_t* __restrict__ raw, size_t n, uint32_t mn,
-                                                         float range, float* __restrict__ out) {
+                                                         float range, float rcp_range, float* __restrict__ out) {
     for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
-        out[k] = sqrtf((float)((uint32_t)raw[k] - mn) / range);
+        out[k] = sqrtf(div_u16((float)((uint32_t)raw[k] - mn), range, rcp_range));
 }
 
 void launch_u16_to_amp(const uint16_t* raw, size_t count, uint32_t mn, float range, float* out, hipStream_t st) {
     const size_t blocks = std::min<size_t>((count + 255) / 256, 1u << 20);
-    hipLaunchKernelGGL(u16_to_amp_kernel, dim3((unsigned)blocks), dim3(256), 0, st, raw, count, mn, range, out);
+    hipLaunchKernelGGL(u16_to_amp_kernel, dim3((unsigned)blocks), dim3(256), 0, st, raw, count, mn, range, 1.0f / range,
+                       out);
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ solve state
+__global__ void solve_state_init_kernel(SolveState s, float lr0) {
+    if (threadIdx.x == 0) {
+        *s.lr = lr0;
+        *s.epoch = 0u;
+        if (s.loss_cur) *s.loss_cur = 0.0;
+    }
+}
+
+__global__ void epoch_end_kernel(SolveState s) {
+    if (threadIdx.x == 0) {
+        const uint32_t e = *s.epoch;
+        if (s.loss_cur) {
+            if (s.loss_hist) s.loss_hist[e] = *s.loss_cur;
+            *s.loss_cur = 0.0;
+        }
+        // float lr times double 0.5, as the reference's host loop does
+        if (e > 12u) *s.lr = (float)((double)*s.lr * 0.5);
+        *s.epoch = e + 1u;
+    }
+}
+
+void launch_solve_state_init(const SolveState& s, float lr0, hipStream_t st) {
+    hipLaunchKernelGGL(solve_state_init_kernel, dim3(1), dim3(64), 0, st, s, lr0);
+    FPM_LAUNCH_CHECK();
+}
+
+void launch_epoch_end(const SolveState& s, hipStream_t st) {
+    hipLaunchKernelGGL(epoch_end_kernel, dim3(1), dim3(64), 0, st, s);
     FPM_LAUNCH_CHECK();
 }
 

@@ -33,13 +33,17 @@ __all__ = [
 
 STITCH_ATOMIC = 0
 STITCH_OWNER = 1
+FFT_F32 = 0        # FP32 butterflies (default)
+FFT_SPLIT_F16 = 1  # radix-16 DFTs as split hi+lo FP16 MFMA products, FP32 accumulation
 Y_F32_AMPLITUDE = 0
 Y_U16_RAW = 1
 
 # every symbol include/fpm_amd.h declares
 EXPORTED_SYMBOLS = (
     "fpm_opts_init", "fpm_pure_v3", "fpm_fused_gradient", "fpm_led_offsets", "fpm_prepare_amplitude",
-    "fpm_forward_intensity", "fpm_fft2", "fpm_last_error", "fpm_abi_version",
+    "fpm_forward_intensity", "fpm_fft2", "fpm_fft2_engine", "fpm_last_error", "fpm_abi_version",
+    "fpm_cnn_create", "fpm_cnn_destroy", "fpm_cnn_num_params", "fpm_cnn_get_params", "fpm_cnn_set_params",
+    "fpm_cnn_forward", "fpm_cnn_gradient", "fpm_cnn_train_step",
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -78,7 +82,9 @@ class _Opts(ctypes.Structure):
         ("loss_per_epoch", ctypes.POINTER(ctypes.c_double)),
         ("pupil_clamp", ctypes.c_float),
         ("stage_times", ctypes.POINTER(StageTimes)),
-        ("reserved", ctypes.c_int32 * 5),
+        ("graph", ctypes.c_int32),
+        ("fft_engine", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -111,6 +117,8 @@ def lib() -> ctypes.CDLL:
     L.fpm_forward_intensity.restype = ctypes.c_int
     L.fpm_fft2.argtypes = [vp, vp, i64, i64, i64, i32, i32, vp]
     L.fpm_fft2.restype = ctypes.c_int
+    L.fpm_fft2_engine.argtypes = [vp, vp, i64, i64, i64, i32, i32, i32, vp]
+    L.fpm_fft2_engine.restype = ctypes.c_int
     L.fpm_last_error.argtypes = []
     L.fpm_last_error.restype = ctypes.c_char_p
     L.fpm_abi_version.argtypes = []
@@ -124,7 +132,8 @@ def _check(rc: int) -> None:
         raise FPMError(rc, lib().fpm_last_error().decode(errors="replace"))
 
 
-def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0.2, verbose=False) -> _Opts:
+def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0.2, verbose=False,
+          graph=False, engine=FFT_F32) -> _Opts:
     o = _Opts()
     lib().fpm_opts_init(ctypes.byref(o))
     o.stitch = int(stitch)
@@ -134,6 +143,8 @@ def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0
     if loss_buf is not None:
         o.loss_per_epoch = loss_buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     o.pupil_clamp = float(clamp)
+    o.graph = 1 if graph else 0
+    o.fft_engine = int(engine)
     return o
 
 
@@ -148,8 +159,11 @@ def _ptr(a: np.ndarray) -> int:
 
 def cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio, numEpochs, batch, opt: Sequence[float],
                   *, stitch: int = STITCH_OWNER, loss: bool = False, clamp: float = 0.2,
-                  verbose: bool = False):
+                  verbose: bool = False, graph: bool = False, engine: int = FFT_F32):
     """The drop-in solve (host arrays).  Returns (O, P), or (O, P, loss_per_epoch).
+
+    ``graph=True`` captures each epoch once as a hipGraph and replays it;
+    ``engine=FFT_SPLIT_F16`` runs the radix-16 DFT stages on the matrix cores.
 
     ``Y`` may be float32 amplitudes (exactly what v3 takes) or uint16 raw
     counts (normalised on the device as codes_v2/main_run_FPM.m:43-45 does).
@@ -176,7 +190,7 @@ def cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio, numEpochs, batch, opt: Sequence[
     P = np.empty_like(P0, order="F")
     optv = (ctypes.c_double * 3)(*[float(v) for v in opt[:3]])
     lossb = np.zeros(max(ne, 1), dtype=np.float64) if loss else None
-    o = _opts(stitch=stitch, loss_buf=lossb, clamp=clamp, verbose=verbose)
+    o = _opts(stitch=stitch, loss_buf=lossb, clamp=clamp, verbose=verbose, graph=graph, engine=engine)
     _check(lib().fpm_pure_v3(_ptr(O0), O0.shape[0], O0.shape[1], _ptr(P0), nr, nc, _ptr(Yarr), ytype, L,
                              _ptr(led), _ptr(A), float(pratio), float(numEpochs), float(batch), optv,
                              _ptr(O), _ptr(P), ctypes.byref(o)))
@@ -185,7 +199,7 @@ def cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio, numEpochs, batch, opt: Sequence[
     return O, P
 
 
-def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER):
+def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER, engine: int = FFT_F32):
     """One fused gradient over all images: (G_O, G_P, loss).
 
     The gradient-only call of benchmark/cuFPM/fullyfusedFPM.cu:21-162 with the
@@ -205,7 +219,7 @@ def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER):
     GO = np.empty_like(O, order="F")
     GP = np.empty_like(P, order="F")
     lossv = ctypes.c_double(0.0)
-    o = _opts(stitch=stitch)
+    o = _opts(stitch=stitch, engine=engine)
     _check(lib().fpm_fused_gradient(_ptr(O), O.shape[0], O.shape[1], _ptr(P), nr, nc, _ptr(Yarr), ytype, L,
                                     _ptr(led), float(pratio), _ptr(GO), _ptr(GP), ctypes.byref(lossv),
                                     ctypes.byref(o)))
@@ -248,7 +262,7 @@ def forward_intensity(O, P, ledIdx, pratio, *, squared=False) -> np.ndarray:
     return out
 
 
-def fft2(x, inverse: bool = False) -> np.ndarray:
+def fft2(x, inverse: bool = False, engine: int = FFT_F32) -> np.ndarray:
     """Unnormalised 2D C2C transform of each (rows, cols) page of a Fortran array."""
     a = _fc(x, np.complex64)
     if a.ndim == 2:
@@ -256,7 +270,8 @@ def fft2(x, inverse: bool = False) -> np.ndarray:
     else:
         shape, batch = a.shape[:2], int(np.prod(a.shape[2:]))
     out = np.empty_like(a, order="F")
-    _check(lib().fpm_fft2(_ptr(a), _ptr(out), shape[0], shape[1], batch, 1 if inverse else -1, 0, None))
+    _check(lib().fpm_fft2_engine(_ptr(a), _ptr(out), shape[0], shape[1], batch, 1 if inverse else -1, int(engine),
+                                 0, None))
     return out
 
 
@@ -264,13 +279,14 @@ def fft2(x, inverse: bool = False) -> np.ndarray:
 def solve_device(O0_ptr: int, P0_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int, A_ptr: int,
                  N: Tuple[int, int], n: Tuple[int, int], L: int, pratio: float, epochs: float, batch: float,
                  opt: Sequence[float], O_out_ptr: int, P_out_ptr: int, *, stitch: int = STITCH_OWNER,
-                 stream: Optional[int] = None, loss: bool = False, clamp: float = 0.2):
+                 stream: Optional[int] = None, loss: bool = False, clamp: float = 0.2, graph: bool = False,
+                 engine: int = FFT_F32):
     """fpm_pure_v3 on device pointers (column-major buffers already in HBM).
     Returns the per-epoch loss array if ``loss`` else None."""
     optv = (ctypes.c_double * 3)(*[float(v) for v in opt[:3]])
     ne = int(epochs)
     lossb = np.zeros(max(ne, 1), dtype=np.float64) if loss else None
-    o = _opts(stitch=stitch, device=True, stream=stream, loss_buf=lossb, clamp=clamp)
+    o = _opts(stitch=stitch, device=True, stream=stream, loss_buf=lossb, clamp=clamp, graph=graph, engine=engine)
     _check(lib().fpm_pure_v3(O0_ptr, N[0], N[1], P0_ptr, n[0], n[1], Y_ptr, Y_U16_RAW if y_u16 else Y_F32_AMPLITUDE,
                              L, led_ptr, A_ptr, float(pratio), float(epochs), float(batch), optv, O_out_ptr,
                              P_out_ptr, ctypes.byref(o)))

@@ -149,10 +149,13 @@ def fortran_empty(shape, dtype, device):
 
 
 def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda", led_grid: int = LED_GRID,
-                defocus_rad: float = 0.5):
+                defocus_rad: float = 0.5, keep_object: bool = False):
     """The same generator on the GPU for large workloads (torch tensors in
     column-major layout, transforms and the image model through libfpm_amd).
-    Returns a dict of device tensors + host LED/mask arrays."""
+    Returns a dict of device tensors + host LED/mask arrays.  keep_object
+    adds the true object in the solve's scale (O_true_dev = kappa O, where
+    kappa maps the object to the prepared amplitudes sqrt((I - min) /
+    (max - min))) and its phase field (phase_true_dev)."""
     import torch
     import torch.nn.functional as F
 
@@ -193,14 +196,17 @@ def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda",
         gx = torch.exp(-((ax - float(cx)) ** 2) / (2 * sig * sig))
         phase.add_(h * torch.outer(gx, gy))
     O_true = torch.polar(amp, phase)
+    phase_keep = phase if keep_object else None
     del amp, phase
     Pt = torch.from_numpy(np.ascontiguousarray(P_true.T)).to(dev)
     ledd = torch.from_numpy(np.ascontiguousarray(led.T)).to(dev)
     I = fortran_empty((n, n, L), torch.float32, dev)
     _check(lib().fpm_forward_intensity(O_true.data_ptr(), N, N, Pt.data_ptr(), n, n, L, ledd.data_ptr(),
                                        float(pratio), 1, I.data_ptr(), 1, stream))
-    del O_true
-    I *= PEAK_COUNTS / float(I[0].max())
+    if not keep_object:
+        del O_true
+    scale = PEAK_COUNTS / float(I[0].max())
+    I *= scale
     I = torch.poisson(I, generator=gen)
     I += OFFSET_COUNTS
     counts = I.round_().clamp_(0, 65535).to(torch.int32)
@@ -213,9 +219,15 @@ def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda",
     O0 = O0.to(torch.complex64).contiguous()
     P0 = torch.from_numpy(np.ascontiguousarray(A.T.astype(np.complex64))).to(dev)
     Ad = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
+    out = {"n": n, "N": N, "L": L, "pratio": pratio, "led": led, "led_dev": ledd, "A": A, "A_dev": Ad,
+           "raw_dev": raw, "O0_dev": O0, "P0_dev": P0, "na_illum": nai, "raw_min": mn, "raw_max": mx}
+    if keep_object:
+        kappa = float(np.sqrt(scale / float(mx - mn)))
+        out["O_true_dev"] = O_true * kappa
+        out["phase_true_dev"] = phase_keep
+        out["kappa"] = kappa
     torch.cuda.synchronize(dev)
-    return {"n": n, "N": N, "L": L, "pratio": pratio, "led": led, "led_dev": ledd, "A": A, "A_dev": Ad,
-            "raw_dev": raw, "O0_dev": O0, "P0_dev": P0, "na_illum": nai, "raw_min": mn, "raw_max": mx}
+    return out
 
 
 def make_numpy(n: int, pratio: int, seed: int = 20261203, led_grid: int = LED_GRID, defocus_rad: float = 0.5,

@@ -1,0 +1,225 @@
+"""Learned warm start of the FPM solve.
+
+No counterpart in the reference, whose solve starts from the bilinear
+upsampling of the centre image (codes_v2/main_run_FPM.m:48).  Here a U-Net
+(libfpm_amd's fpm_cnn_*: BF16 matrix-core kernels, FP32 accumulation, trained
+with the library's own forward and backward kernels) maps low-resolution
+features of the image stack to the object's amplitude and phase at the
+solve's resolution; the plain RMSprop solve then starts from that object and
+finishes the reconstruction, so it both judges and bounds the network's
+contribution: the solve is the same whatever the network predicts.
+
+Features (26 planes, n x n): the prepared amplitudes of the 25 LEDs of the
+central 5 x 5 block (the first 25 of the centre-out spiral), and the mean
+darkfield intensity (LEDs whose illumination NA exceeds the objective NA)
+scaled by 16.  Targets (2 planes, N x N): amplitude and phase of the true
+object in the solve's scale -- the factor kappa that maps the object to the
+prepared amplitudes Y = sqrt((I - min) / (max - min)).
+
+All planes use the device layout of fpm_amd.synth (the row-major buffer of a
+column-major MATLAB array), so features, targets, network outputs and the
+solve's O0 share one orientation.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import FPMError, _check, lib
+
+N_BRIGHT = 25
+DARK_SCALE = 16.0
+
+
+class CnnDesc(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("base_channels", ctypes.c_int32), ("in_channels", ctypes.c_int32),
+                ("n", ctypes.c_int32), ("up", ctypes.c_int32), ("max_batch", ctypes.c_int32)]
+
+
+_bound = False
+
+
+def _cnn_lib():
+    global _bound
+    L = lib()
+    if not _bound:
+        vp = ctypes.c_void_p
+        L.fpm_cnn_create.argtypes = [ctypes.POINTER(CnnDesc), ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.fpm_cnn_create.restype = ctypes.c_int
+        L.fpm_cnn_destroy.argtypes = [vp]
+        L.fpm_cnn_destroy.restype = None
+        L.fpm_cnn_num_params.argtypes = [vp]
+        L.fpm_cnn_num_params.restype = ctypes.c_int64
+        L.fpm_cnn_get_params.argtypes = [vp, vp]
+        L.fpm_cnn_get_params.restype = ctypes.c_int
+        L.fpm_cnn_set_params.argtypes = [vp, vp]
+        L.fpm_cnn_set_params.restype = ctypes.c_int
+        L.fpm_cnn_forward.argtypes = [vp, vp, ctypes.c_int64, vp]
+        L.fpm_cnn_forward.restype = ctypes.c_int
+        L.fpm_cnn_gradient.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), vp]
+        L.fpm_cnn_gradient.restype = ctypes.c_int
+        L.fpm_cnn_train_step.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+        L.fpm_cnn_train_step.restype = ctypes.c_int
+        _bound = True
+    return L
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+class WarmStartNet:
+    """Handle of one network (parameters, Adam state and workspaces on the GPU)."""
+
+    def __init__(self, n: int, up: int, *, levels: int = 4, base: int = 32, in_channels: int = N_BRIGHT + 1,
+                 max_batch: int = 8, seed: int = 0):
+        self.desc = CnnDesc(levels, base, in_channels, n, up, max_batch)
+        h = ctypes.c_void_p()
+        _check(_cnn_lib().fpm_cnn_create(ctypes.byref(self.desc), int(seed), ctypes.byref(h)))
+        self._h = h
+        self.nparam = int(_cnn_lib().fpm_cnn_num_params(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _cnn_lib().fpm_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.nparam, dtype=np.float32)
+        _check(_cnn_lib().fpm_cnn_get_params(self._h, p.ctypes.data))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparam:
+            raise FPMError(-1, f"expected {self.nparam} parameters, got {p.size}")
+        _check(_cnn_lib().fpm_cnn_set_params(self._h, p.ctypes.data))
+
+    def forward(self, feats):
+        """feats: CUDA float32 (B, in_channels, n, n) -> (B, 2, up n, up n)."""
+        import torch
+
+        feats = feats.contiguous()
+        B = feats.shape[0]
+        N = self.desc.up * self.desc.n
+        out = torch.empty((B, 2, N, N), dtype=torch.float32, device=feats.device)
+        _sync()
+        _check(_cnn_lib().fpm_cnn_forward(self._h, feats.data_ptr(), B, out.data_ptr()))
+        return out
+
+    def gradient(self, feats, targets):
+        feats, targets = feats.contiguous(), targets.contiguous()
+        g = np.empty(self.nparam, dtype=np.float32)
+        loss = ctypes.c_double()
+        _sync()
+        _check(_cnn_lib().fpm_cnn_gradient(self._h, feats.data_ptr(), targets.data_ptr(), feats.shape[0],
+                                           ctypes.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def train_step(self, feats, targets, lr: float) -> float:
+        feats, targets = feats.contiguous(), targets.contiguous()
+        loss = ctypes.c_double()
+        _sync()
+        _check(_cnn_lib().fpm_cnn_train_step(self._h, feats.data_ptr(), targets.data_ptr(), feats.shape[0],
+                                             float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def save(self, path: str) -> None:
+        d = self.desc
+        np.savez(path, params=self.get_params(),
+                 desc=np.array([d.levels, d.base_channels, d.in_channels, d.n, d.up, d.max_batch], np.int64))
+
+    @classmethod
+    def load(cls, path: str, max_batch: Optional[int] = None) -> "WarmStartNet":
+        z = np.load(path)  # plain arrays only (allow_pickle stays False)
+        lv, base, cin, n, up, mb = (int(v) for v in z["desc"])
+        net = cls(n, up, levels=lv, base=base, in_channels=cin, max_batch=max_batch or mb)
+        net.set_params(z["params"])
+        return net
+
+
+# ------------------------------------------------------------ features
+def features(amp, na_illum, na: float = 0.1):
+    """(26, n, n) CUDA float32 features of one prepared amplitude stack
+    amp (L, n, n) (device layout) with LEDs in centre-out spiral order."""
+    import torch
+
+    dark = torch.from_numpy(np.nonzero(np.asarray(na_illum) > na)[0]).to(amp.device)
+    f = torch.empty((N_BRIGHT + 1,) + tuple(amp.shape[1:]), dtype=torch.float32, device=amp.device)
+    f[:N_BRIGHT] = amp[:N_BRIGHT]
+    f[N_BRIGHT] = (amp.index_select(0, dark) ** 2).mean(0) * DARK_SCALE if dark.numel() else 0.0
+    return f
+
+
+def warm_start_object(net: WarmStartNet, feats):
+    """Network output -> complex object O0 = amp exp(i phase) (N, N) in the
+    solve's device layout (CUDA complex64)."""
+    import torch
+
+    out = net.forward(feats[None])[0]
+    amp = torch.clamp(out[0], min=0.0)
+    return torch.polar(amp, out[1]).contiguous()
+
+
+# ------------------------------------------------------------- training
+def make_sample(n: int, pratio: int, seed: int, device: str = "cuda", led_grid: int = 19):
+    """One synthetic object through the forward model: (features (26, n, n),
+    target (2, N, N), dataset dict of synth.make_device)."""
+    import torch
+
+    from . import synth
+
+    ds = synth.make_device(n, pratio, seed=seed, device=device, led_grid=led_grid, keep_object=True)
+    raw = ds["raw_dev"]
+    amp = torch.sqrt((raw.to(torch.float32) - ds["raw_min"]) / float(ds["raw_max"] - ds["raw_min"]))
+    feats = features(amp, ds["na_illum"])
+    O = ds["O_true_dev"]
+    target = torch.stack([O.abs(), ds["phase_true_dev"]])
+    return feats, target, ds
+
+
+def make_training_set(count: int, n: int, pratio: int, seed0: int = 1000, device: str = "cuda",
+                      led_grid: int = 19, log=None):
+    """count samples (features, targets) stacked on the device."""
+    import torch
+
+    F = torch.empty((count, N_BRIGHT + 1, n, n), dtype=torch.float32, device=device)
+    T = torch.empty((count, 2, n * pratio, n * pratio), dtype=torch.float32, device=device)
+    for k in range(count):
+        f, t, _ = make_sample(n, pratio, seed0 + k, device, led_grid)
+        F[k] = f
+        T[k] = t
+        if log and (k + 1) % max(1, count // 8) == 0:
+            log(f"[warmstart] generated {k + 1}/{count} samples")
+    return F, T
+
+
+def train(net: WarmStartNet, F, T, *, epochs: int, batch: int, lr: float = 1e-3, seed: int = 0, log=None):
+    """Adam on minibatches of (F, T); returns the mean loss of each epoch."""
+    import torch
+
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    count = F.shape[0]
+    hist = []
+    for e in range(epochs):
+        order = torch.randperm(count, generator=g)
+        tot, nb = 0.0, 0
+        for k0 in range(0, count - batch + 1, batch):
+            idx = order[k0:k0 + batch].to(F.device)
+            tot += net.train_step(F.index_select(0, idx), T.index_select(0, idx), lr)
+            nb += 1
+        hist.append(tot / max(nb, 1))
+        if log:
+            log(f"[warmstart] epoch {e + 1}/{epochs} loss {hist[-1]:.5f}")
+    return hist

@@ -61,6 +61,13 @@ typedef struct { float re, im; } fpm_c32;
                                 the default (faster on MI355X: it is formed inside the
                                 first column pass of the gradient's inverse transform) */
 
+/* arithmetic of the radix-16 DFT stages of every transform of a call */
+#define FPM_FFT_F32       0  /* FP32 butterflies on the vector ALUs (the default) */
+#define FPM_FFT_SPLIT_F16 1  /* radix-16 DFTs as FP16 matrix-core products (MFMA) with
+                                split hi + lo operands, per-butterfly power-of-two
+                                scaling and FP32 accumulation; twiddles, small-radix
+                                passes and all non-FFT arithmetic stay FP32 */
+
 /* Per-stage device time of one call, measured with HIP events recorded on the
  * call's stream around each stage (opts.stage_times; costs a few us per step). */
 enum {
@@ -97,11 +104,16 @@ typedef struct {
     float   pupil_clamp;          /* pupil amplitude band (v3 hard-codes 0.2);
                                      <= 0 selects 0.2 */
     fpm_stage_times* stage_times; /* optional: filled with per-stage device time */
-    int32_t reserved[5];
+    int32_t graph;                /* 1: each epoch's launches are captured once as a
+                                     hipGraph and replayed (at most four distinct
+                                     epoch graphs per call); ignored when stage_times
+                                     is set (per-stage events need eager launches) */
+    int32_t fft_engine;           /* FPM_FFT_*; default FPM_FFT_F32 */
+    int32_t reserved[3];
 } fpm_opts;
 
 /* Fills *o with the defaults (owner-computes stitch, host pointers, silent,
- * library stream, no loss, clamp 0.2). */
+ * library stream, no loss, clamp 0.2, eager launches, FP32 transforms). */
 void fpm_opts_init(fpm_opts* o);
 
 /* The drop-in solve: [O, P] = cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio,
@@ -175,6 +187,54 @@ int fpm_forward_intensity(const fpm_c32* O, int64_t N_rows, int64_t N_cols,
 int fpm_fft2(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
              int64_t batch, int32_t direction, int32_t pointers_are_device,
              void* stream);
+
+/* fpm_fft2 with the arithmetic of the radix-16 stages chosen by engine
+ * (FPM_FFT_F32 or FPM_FFT_SPLIT_F16).  No counterpart in the reference: it
+ * exposes the split-precision transform for accuracy checks against FP64. */
+int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
+                    int64_t batch, int32_t direction, int32_t engine,
+                    int32_t pointers_are_device, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Learned warm start.  No counterpart in the reference (its solve starts from
+ * the upsampled centre image, codes_v2/main_run_FPM.m:48): a U-Net of 3x3
+ * convolutions maps low-resolution features of the image stack to the
+ * initial amplitude and phase of the object; the plain solve then starts from
+ * it and finishes the reconstruction.  Training and inference run on the
+ * library's own BF16 matrix-core kernels with FP32 accumulation.
+ * Images here are planes in row-major order (element (y, x) at y*W + x);
+ * feats, targets and out are DEVICE pointers whose producers have finished;
+ * every call synchronises the work it queued before returning.
+ * ------------------------------------------------------------------------ */
+typedef struct fpm_cnn fpm_cnn;
+typedef struct {
+    int32_t levels;        /* U-Net levels (resolutions n, n/2, ...), 1..6 */
+    int32_t base_channels; /* channels of level 0; level l has base * 2^l (<= 256) */
+    int32_t in_channels;   /* feature planes per sample */
+    int32_t n;             /* feature side in pixels */
+    int32_t up;            /* output side = up * n; 2 up^2 a multiple of 32 */
+    int32_t max_batch;     /* largest batch of one call (sizes the workspaces) */
+} fpm_cnn_desc;
+
+/* Network with He-normal weights from `seed`, zero biases, Adam state zero. */
+int fpm_cnn_create(const fpm_cnn_desc* desc, uint64_t seed, fpm_cnn** out);
+void fpm_cnn_destroy(fpm_cnn* net);
+/* Number of FP32 parameters: per conv (encoder levels, decoder levels, head)
+ * W (Cout x 9 x Cin, Cin of the first conv rounded up to 32) then b (Cout). */
+int64_t fpm_cnn_num_params(const fpm_cnn* net);
+int fpm_cnn_get_params(fpm_cnn* net, float* host_out);
+/* Sets the parameters (host array) and clears the optimiser state. */
+int fpm_cnn_set_params(fpm_cnn* net, const float* host_in);
+/* feats: batch x in_channels x n x n; out: batch x 2 x (up n) x (up n)
+ * (plane 0 amplitude, plane 1 phase) */
+int fpm_cnn_forward(fpm_cnn* net, const float* feats, int64_t batch, float* out);
+/* Mean squared error against targets (batch x 2 x (up n)^2) and its gradient
+ * with respect to every parameter (host_grad may be NULL). */
+int fpm_cnn_gradient(fpm_cnn* net, const float* feats, const float* targets, int64_t batch,
+                     double* loss_out, float* host_grad);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error. */
+int fpm_cnn_train_step(fpm_cnn* net, const float* feats, const float* targets, int64_t batch,
+                       float lr, double* loss_out);
 
 /* Thread-local text of the last error ("" if none). */
 const char* fpm_last_error(void);

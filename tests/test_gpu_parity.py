@@ -126,6 +126,26 @@ def test_solve_20_epochs_matches_fp32_oracle(stitch):
     assert lg[-1] < lg[0]
 
 
+@pytest.mark.parametrize("batch", [10, 9])
+def test_graph_replay_equals_eager_launches(batch):
+    """opts.graph: each epoch captured once and replayed gives the same bits
+    as eager launches -- O, P and every epoch's loss -- over 16 epochs (lr
+    halving from epoch 14).  49 images in batches of 9 make 6 steps per epoch
+    (even) and in batches of 10 make 5 (odd: the spectrum buffers swap roles
+    between epochs, so both parity graphs are replayed in turn)."""
+    n, r, grid = 32, 4, 7
+    O0, P0, Y, led, A = _dataset(n, r, grid)
+    opt = (5e-4, 0.999, 1e-4)
+    Oe, Pe, le = fpm_amd.cuFPM_pure_v3(O0, P0, Y, led, A, r, 16, batch, opt, loss=True)
+    Og, Pg, lg = fpm_amd.cuFPM_pure_v3(O0, P0, Y, led, A, r, 16, batch, opt, loss=True, graph=True)
+    assert np.array_equal(Oe, Og) and np.array_equal(Pe, Pg)
+    assert np.array_equal(le, lg)
+    # eager work in between, then a second graph solve: same bits again
+    fpm_amd.fft2(_cplx((32, 32)))
+    Og2, Pg2, lg2 = fpm_amd.cuFPM_pure_v3(O0, P0, Y, led, A, r, 16, batch, opt, loss=True, graph=True)
+    assert np.array_equal(Og, Og2) and np.array_equal(Pg, Pg2) and np.array_equal(lg, lg2)
+
+
 def test_solve_u16_path_equals_prepared_f32():
     """Y as uint16 raw counts (normalised on the device) gives the same
     bits as Y prepared on the host the way MATLAB does in single."""

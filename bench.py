@@ -107,6 +107,85 @@ def run_cpu_baseline(ds, args):
                       f"{n}^2 x {ds['L']} -> {N}^2 workload, timed {dt:.2f} s, scaled x{steps} steps/epoch"}
 
 
+def run_time_to_loss(args):
+    """Plain solve vs learned warm start + solve on the warm-start workload
+    (361 uint16 images of 256 x 256 -> 1024 x 1024, batch 26, 20 epochs):
+    the network (4 levels, base 32) is trained here on synthetic objects
+    through the forward model, the warm form's schedule is chosen on a
+    validation object, and both forms are timed on a held-out object."""
+    import torch
+
+    import fpm_amd
+    from fpm_amd import warmstart
+
+    n, r = 256, 4
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    F, T = warmstart.make_training_set(args.ttl_samples, n, r, seed0=1000)
+    t_gen = time.perf_counter() - t0
+    net = warmstart.WarmStartNet(n, r, levels=4, base=32, max_batch=8, seed=0)
+    t0 = time.perf_counter()
+    hist = warmstart.train(net, F, T, epochs=args.ttl_epochs, batch=8, lr=5e-4, lr_final=5e-5, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del F, T
+    torch.cuda.empty_cache()
+    sched = warmstart.tune_warm_schedule(net, n, r, seed=77000)
+    log(f"[bench] warm schedule (epochs, lr halving from) = {sched}")
+    res = {}
+    for name, eng in (("fp32", fpm_amd.FFT_F32), ("split", fpm_amd.FFT_SPLIT_F16)):
+        rr = warmstart.time_to_loss(net, n, r, seed=77001, warm_engine=eng, schedule=sched)
+        rr.pop("loss_per_epoch")
+        res[name] = rr
+        log(f"[bench] time to loss ({name} warm solve): ratio {rr['ratio_vs_plain']:.3f}, "
+            f"reached {rr['warm_reaches_plain_loss']}")
+    net.close()
+    return {"workload": f"361 uint16 images {n}x{n} -> {n * r}x{n * r}, batch 26, plain 20 epochs (FP32 engine, "
+                        f"reference schedule) vs U-Net warm start + solve",
+            "training": {"objects": args.ttl_samples, "epochs": args.ttl_epochs, "batch": 8, "generate_s": t_gen,
+                         "train_s": t_train, "loss_first_last": [hist[0], hist[-1]]},
+            "warm_schedule_tuned_on": "validation object seed 77000", "test_object": "seed 77001",
+            "warm_fft_engine": res}
+
+
+def run_tiles(args):
+    """Uneven field-of-view tiles: 48 tiles of 128 and 256 low-res pixels
+    (upsample 4) over a 1024-pixel field, 19 x 19 LEDs seen from each tile's
+    centre, per-tile Zernike pupils (sigma 0.4 rad), batch 26, reconstructed
+    jointly (one hipGraph per epoch, autotuned stream count) against the same
+    tiles solved one after another with fpm_pure_v3."""
+    import torch
+
+    import fpm_amd
+    from fpm_amd import tiles as ft
+
+    t0 = time.perf_counter()
+    tl = ft.make_tile_set(1024, (128, 256), 7, 4, seed=11, count=48)
+    t_gen = time.perf_counter() - t0
+    ep, opt = args.tile_epochs, (args.lr, 0.999, 1e-4)
+    # warm-up call (first launches of every kernel shape), then the timed joint call
+    ft.tiles_solve(tl, 4, 1, args.batch, opt, streams=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep, _ = ft.tiles_solve(tl, 4, ep, args.batch, opt, streams=0)
+    torch.cuda.synchronize()
+    t_joint = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for t in tl:
+        N, n = t["O0"].shape[0], t["P0"].shape[0]
+        fpm_amd.solve_device(t["O0"].data_ptr(), t["P0"].data_ptr(), t["Y"].data_ptr(), True, t["led"].data_ptr(),
+                             t["A"].data_ptr(), (N, N), (n, n), t["Y"].shape[0], 4, ep, args.batch, opt,
+                             t["O"].data_ptr(), t["P"].data_ptr())
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    return {"workload": "48 tiles of 128/256 px (upsample 4) over a 1024 px field, 361 LEDs per tile seen from its "
+                        f"centre, per-tile pupils, batch {args.batch}, {ep} epochs",
+            "joint_s_per_epoch": t_joint / ep, "sequential_s_per_epoch": t_seq / ep,
+            "speedup": t_seq / t_joint, "streams_chosen": rep["streams"],
+            "autotune_epoch_ms": rep["candidates"], "generate_s": t_gen,
+            "note": "tile sides are powers of two; 192 px tiles need a radix-3 transform (not built)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +203,11 @@ def main():
                     help="FFT radix-16 stages: FP32 butterflies or split-FP16 MFMA products")
     ap.add_argument("--graph", action="store_true",
                     help="capture each epoch as a hipGraph and replay it (per-stage events off)")
+    ap.add_argument("--no-time-to-loss", action="store_true", help="skip the warm-start time-to-loss leg")
+    ap.add_argument("--ttl-samples", type=int, default=2048, help="synthetic training objects of the warm start")
+    ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
+    ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
+    ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -245,6 +329,13 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, one minibatch step)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = run_cpu_baseline(ds, args)
+    if not (args.no_time_to_loss and args.no_tiles):
+        del ds, Yd, Od, Pd
+        torch.cuda.empty_cache()
+    if not args.no_time_to_loss:
+        out["time_to_loss"] = run_time_to_loss(args)
+    if not args.no_tiles:
+        out["tiles"] = run_tiles(args)
     print(json.dumps(out), flush=True)
 
 

@@ -339,7 +339,10 @@ int fpm_cnn_create(const fpm_cnn_desc* desc, uint64_t seed, fpm_cnn** out) {
         for (size_t k = 0; k < net->convs.size(); ++k) {
             const Conv& c = net->convs[k];
             const int real_cin = k == 0 ? desc->in_channels : c.cin;
-            const float sd = std::sqrt(2.0f / (9.0f * (float)real_cin)) * (c.relu ? 1.0f : 0.5f);
+            // the head starts at zero: the untrained network reproduces the plain
+            // init exactly, and the first updates need not silence random output
+            // (which they do by driving ReLUs dead)
+            const float sd = c.relu ? std::sqrt(2.0f / (9.0f * (float)real_cin)) : 0.0f;
             for (int co = 0; co < c.cout; ++co)
                 for (int tap = 0; tap < 9; ++tap)
                     for (int ci = 0; ci < c.cin; ++ci)

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -270,33 +271,43 @@ struct StageTimer {
     }
 };
 
-// Instantiated epoch graphs of one solve, keyed by (spectrum buffer parity,
-// last epoch): capture runs the enqueue function on the stream in capture
-// mode (nothing executes), then the graph is instantiated once and replayed.
+// Instantiated epoch graphs, keyed by the caller (a solve: spectrum buffer
+// parity and last epoch; the multi-tile solve adds the stream count):
+// capture runs the enqueue function on the stream in capture mode (nothing
+// executes), then the graph is instantiated once and replayed.
 struct GraphCache {
-    hipGraph_t graph[4] = {};
-    hipGraphExec_t exec[4] = {};
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> g;
     GraphCache() = default;
     GraphCache(const GraphCache&) = delete;
     GraphCache& operator=(const GraphCache&) = delete;
     ~GraphCache() {
-        for (int k = 0; k < 4; ++k) {
-            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
-            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        for (auto& kv : g) {
+            if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+            if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
         }
     }
-    void capture(int key, hipStream_t st, const std::function<void()>& enqueue) {
+    hipGraphExec_t get(int key, hipStream_t st, const std::function<void()>& enqueue) {
+        auto it = g.find(key);
+        if (it != g.end()) return it->second.second;
         FPM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
             enqueue();
         } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(st, &g);
-            if (g) (void)hipGraphDestroy(g);
+            hipGraph_t gr = nullptr;
+            (void)hipStreamEndCapture(st, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
             throw;
         }
-        FPM_HIP(hipStreamEndCapture(st, &graph[key]));
-        FPM_HIP(hipGraphInstantiate(&exec[key], graph[key], nullptr, nullptr, 0));
+        hipGraph_t gr = nullptr;
+        FPM_HIP(hipStreamEndCapture(st, &gr));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(gr);
+            FPM_HIP(e);
+        }
+        g[key] = {gr, ex};
+        return ex;
     }
 };
 
@@ -385,6 +396,115 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     tm.end(col_launches(g.pN));
 }
 
+// One solve's device state and the launches of its epochs: the body of
+// fpm_pure_v3, and one per tile of the multi-tile solve.
+struct Solve {
+    Geom g{};
+    YStack y{};
+    const int2* led = nullptr;
+    const float* A = nullptr;
+    float2* O = nullptr;  // working object (= output)
+    float2* P = nullptr;  // working pupil (= output)
+    uint32_t L = 0, B = 0, steps = 0;
+    int pr = 1, stitch = FPM_STITCH_OWNER;
+    const float2* tw = nullptr;
+    RmsArgs rms{};
+    SolveState ss{};
+    DevBuf S0, S1, mO, mP, state, masks;
+    Workspace ws;
+    float2* spec[2] = {nullptr, nullptr};
+    uint8_t* mk[2] = {nullptr, nullptr};
+    uint8_t* pmask = nullptr;  // pupil-support row mask (n side)
+    uint8_t* psup = nullptr;   // pupil-support rows
+
+    void batch_of(uint32_t s, uint32_t& led0, uint32_t& b) const {
+        led0 = s * B;
+        b = std::min<uint32_t>(B, L - led0);
+    }
+
+    // buffers, zeroed moments, device state, row masks and the spectrum of
+    // the initial object (rows of the first batch)
+    void init(int ne, bool want_loss, float lr, float beta, float eps, float clamp, int32_t halve_from,
+              hipStream_t st, StageTimer& tm) {
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        steps = (L + B - 1) / B;
+        S0.alloc(NN * sizeof(float2));
+        S1.alloc(NN * sizeof(float2));
+        mO.alloc(NN * sizeof(float));
+        mP.alloc(nn * sizeof(float));
+        ws.lat.alloc(nn * B * sizeof(float2));
+        ws.rec.alloc(nn * B * sizeof(float2));
+        FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
+        FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
+        // device-resident solve state: lr, epoch counter, FP64 loss slots
+        state.alloc(64 + (want_loss ? (size_t)ne * sizeof(double) : 0));
+        ss = SolveState{};
+        ss.lr = (float*)(state.as<char>() + 8);
+        ss.epoch = (uint32_t*)(state.as<char>() + 16);
+        ss.loss_cur = want_loss ? (double*)state.p : nullptr;
+        ss.loss_hist = want_loss ? (double*)(state.as<char>() + 64) : nullptr;
+        ss.halve_from = halve_from > 0 ? (uint32_t)halve_from : 13u;
+        launch_solve_state_init(ss, lr, st);
+        rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
+        // Row masks of the window rows of the current and the next batch:
+        // outside them the spectrum gradient is zero and the spectrum is
+        // never read, so the N-side column passes skip those row tiles.
+        const size_t mrows = g.Nr / kMaskRows;
+        masks.alloc(2 * mrows + g.nr / kMaskRows + g.nr);
+        pmask = masks.as<uint8_t>() + 2 * mrows;
+        psup = pmask + g.nr / kMaskRows;
+        mk[0] = masks.as<uint8_t>();
+        mk[1] = masks.as<uint8_t>() + mrows;
+        spec[0] = S0.as<float2>();
+        spec[1] = S1.as<float2>();
+        // rows where P != 0 or A != 0: the update keeps P = 0 wherever
+        // A = 0, so the masks of the initial pupil hold for every step
+        launch_pupil_row_mask(P, A, g.nr, g.nc, pmask, psup, st);
+        uint32_t led0, b;
+        batch_of(0, led0, b);
+        launch_row_mask(led, led0, b, g.Nr, g.nr, psup, mk[0], st);
+        tm.begin(FPM_STAGE_SPECTRUM_INIT);
+        fft_contig(O, spec[0], g.Nr, g.Nc, false, tw, st, g.eng);
+        fft_cols_forward(spec[0], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[0], nullptr, g.eng);
+        tm.end(1 + col_launches(g.pN));
+    }
+
+    // One epoch's launches, given which of the two spectrum buffers (and row
+    // masks) holds the current spectrum: every step reads spec[cur] and,
+    // unless it is the solve's last step, leaves the next step's spectrum in
+    // spec[1 - cur].  The launches depend only on (cur, last): lr, the epoch
+    // index and the loss slot live in device memory, so an epoch can be
+    // captured and replayed.
+    void enqueue_epoch(int cur, bool last_epoch, hipStream_t st, StageTimer& tm) {
+        uint32_t led0, b;
+        for (uint32_t s = 0; s < steps; ++s) {
+            batch_of(s, led0, b);
+            minibatch_backward(g, spec[cur], P, led, led0, b, y, pr, ws, spec[1 - cur], stitch, mP.as<float>(), A,
+                               rms, nullptr, ss.loss_cur, tw, st, tm, mk[cur], pmask);
+            const bool last = last_epoch && (s == steps - 1);
+            if (!last) {  // the next minibatch: s + 1, or the first one of the next epoch
+                uint32_t nled0, nb;
+                batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
+                launch_row_mask(led, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st);
+            }
+            tm.begin(FPM_STAGE_FUSED_UPDATE);
+            launch_fused_update(g, spec[1 - cur], O, mO.as<float>(), rms, !last, mk[cur], tw, st, g.eng);
+            tm.end(1);
+            if (!last) {
+                tm.begin(FPM_STAGE_FFT_N_COLS_HI);
+                fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng);
+                tm.end(col_launches(g.pN));
+                cur = 1 - cur;
+            }
+        }
+        // loss of the epoch to its slot; lr halving (cuFPM_pure_v3.cu:204-206)
+        launch_epoch_end(ss, st);
+    }
+
+    // spectrum buffer parity at the start of the next epoch
+    int parity_after(int cur) const { return (int)((cur + steps) & 1u); }
+};
+
 }  // namespace
 }  // namespace fpm
 
@@ -425,6 +545,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         if (ne < 0) invalid("numEpochs must be >= 0");
         if (bs < 1) invalid("batch must be >= 1");
         if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
+        if (o.lr_halve_from < 0) invalid("lr_halve_from must be >= 0");
         const float lr = (float)opt[0];
         Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
         g.eng = engine_of(o.fft_engine);
@@ -467,102 +588,36 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         const float* dA = upload(A, nn, isdev, hA, st);
 
         if (ne > 0) {
-            DevBuf S0, S1, mO, mP, lossd, state;
-            Workspace ws;
-            S0.alloc(NN * sizeof(float2));
-            S1.alloc(NN * sizeof(float2));
-            mO.alloc(NN * sizeof(float));
-            mP.alloc(nn * sizeof(float));
-            ws.lat.alloc(nn * B * sizeof(float2));
-            ws.rec.alloc(nn * B * sizeof(float2));
-            FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
-            FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
-            // device-resident solve state: lr, epoch counter, FP64 loss slots
-            state.alloc(64 + (o.loss_per_epoch ? (size_t)ne * sizeof(double) : 0));
-            SolveState ss{};
-            ss.loss_cur = (double*)state.p;
-            ss.lr = (float*)(state.as<char>() + 8);
-            ss.epoch = (uint32_t*)(state.as<char>() + 16);
-            if (o.loss_per_epoch) {
-                ss.loss_hist = (double*)(state.as<char>() + 64);
-            } else {
-                ss.loss_cur = nullptr;
-            }
-            launch_solve_state_init(ss, lr, st);
+            Solve sv;
+            sv.g = g;
+            sv.y = y;
+            sv.led = led;
+            sv.A = dA;
+            sv.O = O;
+            sv.P = P;
+            sv.L = (uint32_t)L;
+            sv.B = B;
+            sv.pr = pr;
+            sv.stitch = o.stitch;
+            sv.tw = tw;
             StageTimer tm(o.stage_times, st);
+            sv.init(ne, o.loss_per_epoch != nullptr, lr, beta, eps, o.pupil_clamp, o.lr_halve_from, st, tm);
             const bool use_graph = o.graph != 0 && o.stage_times == nullptr;
-            // Row masks of the window rows of the current and the next batch:
-            // outside them the spectrum gradient is zero and the spectrum is
-            // never read, so the N-side column passes skip those row tiles.
-            const uint32_t steps = (uint32_t)((L + B - 1) / B);
-            const size_t mrows = g.Nr / kMaskRows;
-            DevBuf masks;
-            masks.alloc(2 * mrows + g.nr / kMaskRows + g.nr);
-            uint8_t* pmask = masks.as<uint8_t>() + 2 * mrows;  // pupil-support row mask (n side)
-            uint8_t* psup = pmask + g.nr / kMaskRows;          // pupil-support rows
-            uint8_t* mk[2] = {masks.as<uint8_t>(), masks.as<uint8_t>() + mrows};
-            float2* spec[2] = {S0.as<float2>(), S1.as<float2>()};
-            // rows where P != 0 or A != 0: the update keeps P = 0 wherever
-            // A = 0, so the masks of the initial pupil hold for every step
-            launch_pupil_row_mask(P, dA, g.nr, g.nc, pmask, psup, st);
-            auto batch_of = [&](uint32_t s, uint32_t& led0, uint32_t& b) {
-                led0 = s * B;
-                b = std::min<uint32_t>(B, (uint32_t)L - led0);
-            };
-            uint32_t led0, b;
-            batch_of(0, led0, b);
-            launch_row_mask(led, led0, b, g.Nr, g.nr, psup, mk[0], st);
-            // spectrum of the initial object (rows of the first batch)
-            tm.begin(FPM_STAGE_SPECTRUM_INIT);
-            fft_contig(O, spec[0], g.Nr, g.Nc, false, tw, st, g.eng);
-            fft_cols_forward(spec[0], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[0], nullptr, g.eng);
-            tm.end(1 + col_launches(g.pN));
-            const RmsArgs rms{lr, beta, eps, o.pupil_clamp, ss.lr};
-            // One epoch's launches, given which of the two spectrum buffers
-            // (and row masks) holds the current spectrum: every step reads
-            // spec[cur] and, unless it is the solve's last step, leaves the
-            // next step's spectrum in spec[1 - cur].  The launches depend only
-            // on (cur, last): lr, the epoch index and the loss slot live in
-            // device memory, so an epoch can be captured and replayed.
-            auto enqueue_epoch = [&](int cur, bool last_epoch) {
-                for (uint32_t s = 0; s < steps; ++s) {
-                    batch_of(s, led0, b);
-                    minibatch_backward(g, spec[cur], P, led, led0, b, y, pr, ws, spec[1 - cur], o.stitch,
-                                       mP.as<float>(), dA, rms, nullptr, ss.loss_cur, tw, st, tm, mk[cur], pmask);
-                    const bool last = last_epoch && (s == steps - 1);
-                    if (!last) {  // the next minibatch: s + 1, or the first one of the next epoch
-                        uint32_t nled0, nb;
-                        batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
-                        launch_row_mask(led, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st);
-                    }
-                    tm.begin(FPM_STAGE_FUSED_UPDATE);
-                    launch_fused_update(g, spec[1 - cur], O, mO.as<float>(), rms, !last, mk[cur], tw, st, g.eng);
-                    tm.end(1);
-                    if (!last) {
-                        tm.begin(FPM_STAGE_FFT_N_COLS_HI);
-                        fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng);
-                        tm.end(col_launches(g.pN));
-                        cur = 1 - cur;
-                    }
-                }
-                // loss of the epoch to its slot; lr halving (cuFPM_pure_v3.cu:204-206)
-                launch_epoch_end(ss, st);
-            };
             GraphCache graphs;
             int cur = 0;
             for (int e = 0; e < ne; ++e) {
                 const bool last_epoch = e == ne - 1;
                 if (use_graph) {
                     const int key = cur * 2 + (last_epoch ? 1 : 0);
-                    if (!graphs.exec[key]) graphs.capture(key, st, [&] { enqueue_epoch(cur, last_epoch); });
-                    FPM_HIP(hipGraphLaunch(graphs.exec[key], st));
+                    hipGraphExec_t ex = graphs.get(key, st, [&] { sv.enqueue_epoch(cur, last_epoch, st, tm); });
+                    FPM_HIP(hipGraphLaunch(ex, st));
                 } else {
-                    enqueue_epoch(cur, last_epoch);
+                    sv.enqueue_epoch(cur, last_epoch, st, tm);
                 }
-                cur = (int)((cur + steps) & 1u);
+                cur = sv.parity_after(cur);
             }
             if (o.loss_per_epoch)
-                FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, ss.loss_hist, (size_t)ne * sizeof(double),
+                FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, sv.ss.loss_hist, (size_t)ne * sizeof(double),
                                        hipMemcpyDeviceToHost, st));
             FPM_HIP(hipStreamSynchronize(st));
             tm.flush();
@@ -570,6 +625,180 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         if (!isdev) {
             FPM_HIP(hipMemcpyAsync(O_out, O, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
             FPM_HIP(hipMemcpyAsync(P_out, P, nn * sizeof(float2), hipMemcpyDeviceToHost, st));
+        }
+        FPM_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pratio, double epochs, double batch,
+                    const double opt[3], int32_t streams, const fpm_opts* opts, fpm_tiles_report* report) {
+    return run_guarded("fpm_tiles_solve", [&] {
+        const fpm_opts o = merged_opts(opts);
+        if (!tiles || !opt) invalid("null argument");
+        if (ntiles < 1 || ntiles > 4096) invalid("ntiles must be in [1, 4096]");
+        if (L < 1) invalid("L (number of images) must be >= 1");
+        const int pr = (int)pratio, ne = (int)epochs, bs = (int)batch;
+        if (pr < 1) invalid("pratio must be >= 1");
+        if (ne < 0) invalid("numEpochs must be >= 0");
+        if (bs < 1) invalid("batch must be >= 1");
+        if (streams < 0 || streams > 64) invalid("streams must be in [0, 64]");
+        if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
+        if (o.lr_halve_from < 0) invalid("lr_halve_from must be >= 0");
+        const int eng = engine_of(o.fft_engine);
+        std::vector<Geom> geo(ntiles);
+        for (int t = 0; t < ntiles; ++t) {
+            const fpm_tile& tl = tiles[t];
+            if (!tl.O0 || !tl.O_out || !tl.P0 || !tl.P_out || !tl.Y || !tl.led_idx || !tl.A)
+                invalid("tile " + std::to_string(t) + ": null array argument");
+            geo[t] = make_geom(tl.N, tl.N, tl.n, tl.n);
+            geo[t].eng = eng;
+        }
+        const float lr = (float)opt[0], beta = (float)opt[1], eps = (float)opt[2];
+        check_device();
+        int dev = 0;
+        FPM_HIP(hipGetDevice(&dev));
+        const float2* tw = twiddle_table(dev);
+        StreamHolder sh(o.stream);
+        hipStream_t st = sh.s;
+        const uint32_t B = (uint32_t)std::min<int64_t>(bs, L);
+        StageTimer none(nullptr, st);
+        std::vector<std::unique_ptr<Solve>> sv(ntiles);
+        std::vector<std::unique_ptr<DevBuf>> ybuf(ntiles);
+        std::vector<double> work(ntiles);
+        for (int t = 0; t < ntiles; ++t) {
+            const fpm_tile& tl = tiles[t];
+            const Geom& g = geo[t];
+            const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+            if ((const void*)tl.O0 != (void*)tl.O_out)
+                FPM_HIP(hipMemcpyAsync(tl.O_out, tl.O0, NN * sizeof(float2), hipMemcpyDeviceToDevice, st));
+            if ((const void*)tl.P0 != (void*)tl.P_out)
+                FPM_HIP(hipMemcpyAsync(tl.P_out, tl.P0, nn * sizeof(float2), hipMemcpyDeviceToDevice, st));
+            ybuf[t] = std::make_unique<DevBuf>();
+            sv[t] = std::make_unique<Solve>();
+            Solve& s = *sv[t];
+            s.g = g;
+            s.y = prepare_y(tl.Y, tl.y_dtype, nn * (size_t)L, true, *ybuf[t], st);
+            s.led = (const int2*)tl.led_idx;
+            s.A = tl.A;
+            s.O = (float2*)tl.O_out;
+            s.P = (float2*)tl.P_out;
+            s.L = (uint32_t)L;
+            s.B = B;
+            s.pr = pr;
+            s.stitch = o.stitch;
+            s.tw = tw;
+            if (ne > 0) s.init(ne, tl.loss_per_epoch != nullptr, lr, beta, eps, o.pupil_clamp, o.lr_halve_from, st, none);
+            work[t] = (double)NN + (double)B * (double)nn;
+        }
+        if (report) std::memset(report, 0, sizeof(*report));
+        if (ne > 0) {
+            // candidate stream counts; tiles largest first, each to the least loaded stream
+            std::vector<int> cands;
+            if (streams > 0) {
+                cands.push_back(std::min(streams, ntiles));
+            } else {
+                for (int k : {1, 2, 4, 8})
+                    if (k <= ntiles && (cands.empty() || cands.back() != k)) cands.push_back(k);
+                if (cands.back() < std::min(8, ntiles)) cands.push_back(std::min(8, ntiles));
+            }
+            const int kmax = *std::max_element(cands.begin(), cands.end());
+            std::vector<int> order(ntiles);
+            for (int t = 0; t < ntiles; ++t) order[t] = t;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+            struct Streams {
+                std::vector<hipStream_t> s;
+                std::vector<hipEvent_t> e;
+                ~Streams() {
+                    for (auto x : s) (void)hipStreamDestroy(x);
+                    for (auto x : e) (void)hipEventDestroy(x);
+                }
+            } xs;
+            for (int k = 1; k < kmax; ++k) {
+                hipStream_t x;
+                FPM_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+                xs.s.push_back(x);
+            }
+            for (int k = 0; k < kmax; ++k) {
+                hipEvent_t e;
+                FPM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                xs.e.push_back(e);
+            }
+            auto stream_of = [&](int k) { return k == 0 ? st : xs.s[k - 1]; };
+            // all tiles' epoch on K streams: fork from st, join back into st
+            auto enqueue_all = [&](int K, int cur, bool last) {
+                std::vector<std::vector<int>> bucket(K);
+                std::vector<double> load(K, 0.0);
+                for (int t : order) {
+                    const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                    bucket[k].push_back(t);
+                    load[k] += work[t];
+                }
+                if (K > 1) {
+                    FPM_HIP(hipEventRecord(xs.e[0], st));
+                    for (int k = 1; k < K; ++k) FPM_HIP(hipStreamWaitEvent(stream_of(k), xs.e[0], 0));
+                }
+                for (int k = 0; k < K; ++k)
+                    for (int t : bucket[k]) sv[t]->enqueue_epoch(cur, last, stream_of(k), none);
+                if (K > 1) {
+                    for (int k = 1; k < K; ++k) {
+                        FPM_HIP(hipEventRecord(xs.e[k], stream_of(k)));
+                        FPM_HIP(hipStreamWaitEvent(st, xs.e[k], 0));
+                    }
+                }
+            };
+            hipEvent_t ta, tb;
+            FPM_HIP(hipEventCreate(&ta));
+            FPM_HIP(hipEventCreate(&tb));
+            struct EvPair {
+                hipEvent_t a, b;
+                ~EvPair() {
+                    (void)hipEventDestroy(a);
+                    (void)hipEventDestroy(b);
+                }
+            } evp{ta, tb};
+            GraphCache graphs;
+            const bool tune = cands.size() > 1;
+            std::vector<double> cand_ms(cands.size(), 0.0);
+            int K = tune ? -1 : cands[0];
+            int cur = 0;
+            for (int e = 0; e < ne; ++e) {
+                const bool last = e == ne - 1;
+                const bool timing = K < 0 && e < (int)cands.size();
+                const int Ke = timing ? cands[e] : K;
+                const int key = (Ke * 2 + cur) * 2 + (last ? 1 : 0);
+                hipGraphExec_t ex = graphs.get(key, st, [&] { enqueue_all(Ke, cur, last); });
+                if (timing) {
+                    FPM_HIP(hipEventRecord(ta, st));
+                    FPM_HIP(hipGraphLaunch(ex, st));
+                    FPM_HIP(hipEventRecord(tb, st));
+                    FPM_HIP(hipEventSynchronize(tb));
+                    float ms = 0.f;
+                    FPM_HIP(hipEventElapsedTime(&ms, ta, tb));
+                    cand_ms[e] = ms;
+                    if (e == (int)cands.size() - 1 || last) {
+                        const int nt = e + 1;
+                        K = cands[(int)(std::min_element(cand_ms.begin(), cand_ms.begin() + nt) - cand_ms.begin())];
+                    }
+                } else {
+                    FPM_HIP(hipGraphLaunch(ex, st));
+                }
+                cur = sv[0]->parity_after(cur);
+            }
+            for (int t = 0; t < ntiles; ++t)
+                if (tiles[t].loss_per_epoch)
+                    FPM_HIP(hipMemcpyAsync(tiles[t].loss_per_epoch, sv[t]->ss.loss_hist, (size_t)ne * sizeof(double),
+                                           hipMemcpyDeviceToHost, st));
+            if (report) {
+                report->streams = K;
+                report->tuned = tune ? (int32_t)std::min<size_t>(cands.size(), (size_t)ne) : 0;
+                for (size_t k = 0; k < cands.size() && k < 4; ++k) {
+                    report->cand[k] = cands[k];
+                    report->cand_ms[k] = cand_ms[k];
+                }
+            }
+            // the graphs, streams and events of this block are released on
+            // leaving it: nothing of theirs may still run
+            FPM_HIP(hipStreamSynchronize(st));
         }
         FPM_HIP(hipStreamSynchronize(st));
     });

@@ -72,11 +72,13 @@ struct SolveState {
     uint32_t* epoch;
     double* loss_cur;   // may be null (no loss requested)
     double* loss_hist;  // [epochs], may be null
+    uint32_t halve_from = 13;  // lr halves after every epoch with index >= halve_from
 };
 // lr = lr0, epoch = 0, loss_cur = 0 (a kernel, not a memset: graph-safe)
 void launch_solve_state_init(const SolveState& s, float lr0, hipStream_t st);
 // end of an epoch: loss_hist[epoch] = loss_cur, loss_cur = 0, lr halved after
-// every epoch with index > 12 (cuFPM_v3/cuFPM_pure_v3.cu:204-206), epoch += 1
+// every epoch with index >= halve_from (13: cuFPM_v3/cuFPM_pure_v3.cu:204-206),
+// epoch += 1
 void launch_epoch_end(const SolveState& s, hipStream_t st);
 
 // feature-domain loss gradient (kernels.cuh:197-275): z (b pages, natural) and

@@ -428,7 +428,7 @@ __global__ void epoch_end_kernel(SolveState s) {
             *s.loss_cur = 0.0;
         }
         // float lr times double 0.5, as the reference's host loop does
-        if (e > 12u) *s.lr = (float)((double)*s.lr * 0.5);
+        if (e >= s.halve_from) *s.lr = (float)((double)*s.lr * 0.5);
         *s.epoch = e + 1u;
     }
 }

@@ -45,9 +45,13 @@ int argmin_unique(const std::vector<double>& grid, double target) {
 
 }  // namespace
 
-extern "C" int fpm_led_offsets(double wavelength_um, double na, double pixel_um, double led_pitch_um,
-                               double led_height_um, int32_t led_grid, int64_t n, int32_t pratio, int32_t spiral,
-                               int32_t* led_idx, int32_t* grid_index, double* na_illum) {
+// LED (i, j) seen from the field area at (objdx, objdy) of the sample plane:
+// the lateral offsets above become (-v_i D + objdx, -h_j D + objdy)
+// (init_environment_rgb.m:26-29,83-84, where the reference fixes objdx =
+// objdy = 0).
+static int led_offsets_from(double objdx_um, double objdy_um, double wavelength_um, double na, double pixel_um,
+                            double led_pitch_um, double led_height_um, int32_t led_grid, int64_t n, int32_t pratio,
+                            int32_t spiral, int32_t* led_idx, int32_t* grid_index, double* na_illum) {
     fpm::set_error("");
     (void)na;  // the pupil radius does not enter the window offsets
     if (!led_idx) {
@@ -72,7 +76,7 @@ extern "C" int fpm_led_offsets(double wavelength_um, double na, double pixel_um,
     for (int i = 0; i < g; ++i) {
         for (int j = 0; j < g; ++j) {
             const double vv = (double)i - c, hh = (double)j - c;
-            const double a = -vv * led_pitch_um, b = -hh * led_pitch_um;
+            const double a = -vv * led_pitch_um + objdx_um, b = -hh * led_pitch_um + objdy_um;
             const double den = std::sqrt(a * a + b * b + led_height_um * led_height_um);
             const double uu = a / den;    // -v_i D / den
             const double v = -b / den;    //  h_j D / den
@@ -129,4 +133,23 @@ extern "C" int fpm_led_offsets(double wavelength_um, double na, double pixel_um,
         if (na_illum) na_illum[l] = nai[q];
     }
     return FPM_OK;
+}
+
+extern "C" int fpm_led_offsets(double wavelength_um, double na, double pixel_um, double led_pitch_um,
+                               double led_height_um, int32_t led_grid, int64_t n, int32_t pratio, int32_t spiral,
+                               int32_t* led_idx, int32_t* grid_index, double* na_illum) {
+    return led_offsets_from(0.0, 0.0, wavelength_um, na, pixel_um, led_pitch_um, led_height_um, led_grid, n, pratio,
+                            spiral, led_idx, grid_index, na_illum);
+}
+
+extern "C" int fpm_led_offsets_at(double objdx_um, double objdy_um, double wavelength_um, double na,
+                                  double pixel_um, double led_pitch_um, double led_height_um, int32_t led_grid,
+                                  int64_t n, int32_t pratio, int32_t spiral, int32_t* led_idx, int32_t* grid_index,
+                                  double* na_illum) {
+    if (!std::isfinite(objdx_um) || !std::isfinite(objdy_um)) {
+        fpm::set_error("fpm_led_offsets_at: non-finite field offset");
+        return FPM_E_INVALID;
+    }
+    return led_offsets_from(objdx_um, objdy_um, wavelength_um, na, pixel_um, led_pitch_um, led_height_um, led_grid, n,
+                            pratio, spiral, led_idx, grid_index, na_illum);
 }

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "fpm_opts_init", "fpm_pure_v3", "fpm_fused_gradient", "fpm_led_offsets", "fpm_prepare_amplitude",
     "fpm_forward_intensity", "fpm_fft2", "fpm_fft2_engine", "fpm_last_error", "fpm_abi_version",
     "fpm_cnn_create", "fpm_cnn_destroy", "fpm_cnn_num_params", "fpm_cnn_get_params", "fpm_cnn_set_params",
-    "fpm_cnn_forward", "fpm_cnn_gradient", "fpm_cnn_train_step",
+    "fpm_cnn_forward", "fpm_cnn_gradient", "fpm_cnn_train_step", "fpm_led_offsets_at", "fpm_tiles_solve",
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -84,7 +84,8 @@ class _Opts(ctypes.Structure):
         ("stage_times", ctypes.POINTER(StageTimes)),
         ("graph", ctypes.c_int32),
         ("fft_engine", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("lr_halve_from", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -133,7 +134,7 @@ def _check(rc: int) -> None:
 
 
 def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0.2, verbose=False,
-          graph=False, engine=FFT_F32) -> _Opts:
+          graph=False, engine=FFT_F32, lr_halve_from=0) -> _Opts:
     o = _Opts()
     lib().fpm_opts_init(ctypes.byref(o))
     o.stitch = int(stitch)
@@ -145,6 +146,7 @@ def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0
     o.pupil_clamp = float(clamp)
     o.graph = 1 if graph else 0
     o.fft_engine = int(engine)
+    o.lr_halve_from = int(lr_halve_from)
     return o
 
 
@@ -280,13 +282,14 @@ def solve_device(O0_ptr: int, P0_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int
                  N: Tuple[int, int], n: Tuple[int, int], L: int, pratio: float, epochs: float, batch: float,
                  opt: Sequence[float], O_out_ptr: int, P_out_ptr: int, *, stitch: int = STITCH_OWNER,
                  stream: Optional[int] = None, loss: bool = False, clamp: float = 0.2, graph: bool = False,
-                 engine: int = FFT_F32):
+                 engine: int = FFT_F32, lr_halve_from: int = 0):
     """fpm_pure_v3 on device pointers (column-major buffers already in HBM).
     Returns the per-epoch loss array if ``loss`` else None."""
     optv = (ctypes.c_double * 3)(*[float(v) for v in opt[:3]])
     ne = int(epochs)
     lossb = np.zeros(max(ne, 1), dtype=np.float64) if loss else None
-    o = _opts(stitch=stitch, device=True, stream=stream, loss_buf=lossb, clamp=clamp, graph=graph, engine=engine)
+    o = _opts(stitch=stitch, device=True, stream=stream, loss_buf=lossb, clamp=clamp, graph=graph, engine=engine,
+              lr_halve_from=lr_halve_from)
     _check(lib().fpm_pure_v3(O0_ptr, N[0], N[1], P0_ptr, n[0], n[1], Y_ptr, Y_U16_RAW if y_u16 else Y_F32_AMPLITUDE,
                              L, led_ptr, A_ptr, float(pratio), float(epochs), float(batch), optv, O_out_ptr,
                              P_out_ptr, ctypes.byref(o)))

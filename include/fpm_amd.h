@@ -17,6 +17,8 @@
  *                           the v3 loss of cuFPM_v3/cuda/kernels.cuh:197-275
  *   fpm_led_offsets      <- codes_v2/func_ware/init_environment_rgb.m:33-134 and
  *                           codes_v2/main_run_FPM.m:23,27-31,53-55
+ *   fpm_led_offsets_at   <- the same with the field-area offsets objdx, objdy
+ *                           of init_environment_rgb.m:26-29,83-84
  *   fpm_prepare_amplitude<- codes_v2/main_run_FPM.m:40-45
  *   fpm_forward_intensity<- the forward half of benchmark/func_ware/fpm_forward.m:25-35
  *                           (|IFFT(crop(FFT O) . P)|, the image model)
@@ -109,7 +111,10 @@ typedef struct {
                                      epoch graphs per call); ignored when stage_times
                                      is set (per-stage events need eager launches) */
     int32_t fft_engine;           /* FPM_FFT_*; default FPM_FFT_F32 */
-    int32_t reserved[3];
+    int32_t lr_halve_from;        /* lr halves after every epoch with index >= this;
+                                     0 selects 13, the reference's schedule
+                                     (cuFPM_pure_v3.cu:204-206: index > 12) */
+    int32_t reserved[2];
 } fpm_opts;
 
 /* Fills *o with the defaults (owner-computes stitch, host pointers, silent,
@@ -163,6 +168,16 @@ int fpm_led_offsets(double wavelength_um, double na, double pixel_um,
                     int32_t led_grid, int64_t n, int32_t pratio, int32_t spiral,
                     int32_t* led_idx, int32_t* grid_index, double* na_illum);
 
+/* fpm_led_offsets for the field area at (objdx_um, objdy_um) of the sample
+ * plane: the LED lateral offsets (-v D, -h D) become (-v D + objdx, -h D + objdy)
+ * as in codes_v2/func_ware/init_environment_rgb.m:26-29,83-84 (the reference
+ * fixes objdx = objdy = 0 there, i.e. fpm_led_offsets).  Gives each
+ * field-of-view tile of fpm_tiles_solve its own illumination angles. */
+int fpm_led_offsets_at(double objdx_um, double objdy_um, double wavelength_um, double na,
+                       double pixel_um, double led_pitch_um, double led_height_um,
+                       int32_t led_grid, int64_t n, int32_t pratio, int32_t spiral,
+                       int32_t* led_idx, int32_t* grid_index, double* na_illum);
+
 /* uint16 raw stack -> float amplitudes sqrt((I - min)/(max - min)) with min and
  * max taken over the whole stack (main_run_FPM.m:43-45).  count = n*n*L.
  * Pointers are host or device per pointers_are_device; min/max outputs may be NULL. */
@@ -194,6 +209,46 @@ int fpm_fft2(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
 int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
                     int64_t batch, int32_t direction, int32_t engine,
                     int32_t pointers_are_device, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Multi-tile solve.  No counterpart in the reference (one field of view per
+ * mex call): many field-of-view tiles, each with its own object, pupil,
+ * images and LED windows (its own illumination angles), reconstructed
+ * jointly with the update of fpm_pure_v3.  Every tile has the same number of
+ * images L and uses the same pratio, epochs, batch and RMSprop constants;
+ * tile sides are powers of two (the transforms' rule).  All tile arrays are
+ * DEVICE pointers (opts->pointers_are_device is ignored).  Each epoch of all
+ * tiles is captured once as one hipGraph whose tiles are spread over
+ * `streams` concurrent streams, largest tiles first (longest-processing-time
+ * assignment); streams = 0 autotunes: the first epochs run with 1, 2, 4 and 8
+ * streams in turn, each timed, and the fastest count runs the rest.  The
+ * stream count changes only the overlap, never the arithmetic: results are
+ * the same bits for every count.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    const fpm_c32* O0;       /* N x N spatial object */
+    fpm_c32* O_out;          /* N x N result (may alias O0) */
+    int64_t N;
+    const fpm_c32* P0;       /* n x n pupil */
+    fpm_c32* P_out;          /* n x n result (may alias P0) */
+    int64_t n;
+    const void* Y;           /* n x n x L images */
+    int32_t y_dtype;         /* FPM_Y_* */
+    const int32_t* led_idx;  /* int32 2 x L {kl, kt} of this tile */
+    const float* A;          /* n x n pupil mask */
+    double* loss_per_epoch;  /* optional HOST array [epochs] */
+} fpm_tile;
+
+typedef struct {
+    int32_t streams;         /* stream count of the epochs after tuning */
+    int32_t tuned;           /* candidates timed (0: streams was given) */
+    int32_t cand[4];         /* candidate stream counts */
+    double  cand_ms[4];      /* measured epoch time of each candidate */
+} fpm_tiles_report;
+
+int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pratio,
+                    double epochs, double batch, const double opt[3], int32_t streams,
+                    const fpm_opts* opts, fpm_tiles_report* report);
 
 /* ------------------------------------------------------------------------
  * Learned warm start.  No counterpart in the reference (its solve starts from

@@ -43,7 +43,7 @@ def lib() -> ctypes.CDLL:
             getattr(L, f"oracle_loss_{s}").argtypes = [vp, i, i, vp, i, i, vp, i, vp, i]
             getattr(L, f"oracle_loss_{s}").restype = d
             getattr(L, f"oracle_solve_{s}").argtypes = [vp, i, i, vp, i, i, vp, i, vp, vp, i, i, i, f, f, f, f,
-                                                        vp, vp, vp, i]
+                                                        vp, vp, vp, i, i]
         L.oracle_prepare_amplitude.argtypes = [vp, ctypes.c_long, vp]
         L.oracle_num_threads.restype = ctypes.c_int
         _lib = L
@@ -99,8 +99,9 @@ def loss(O, P, Y, led, pratio, prec="f64"):
                                                  P.shape[1], Y.ctypes.data, Y.shape[2], led.ctypes.data, int(pratio))
 
 
-def solve(O0, P0, Y, led, A, pratio, epochs, batch, opt, prec="f32", clamp=0.2, max_steps=0):
-    """The v3 solve restated on the CPU: (O, P, loss_per_epoch)."""
+def solve(O0, P0, Y, led, A, pratio, epochs, batch, opt, prec="f32", clamp=0.2, max_steps=0, halve_from=13):
+    """The v3 solve restated on the CPU: (O, P, loss_per_epoch); lr halves
+    after every epoch with index >= halve_from (13: the reference)."""
     O0 = _fa(O0, _ct(prec))
     P0 = _fa(P0, _ct(prec))
     Y = _fa(Y, np.float32)
@@ -113,7 +114,7 @@ def solve(O0, P0, Y, led, A, pratio, epochs, batch, opt, prec="f32", clamp=0.2, 
                                            P0.shape[1], Y.ctypes.data, Y.shape[2], led.ctypes.data, A.ctypes.data,
                                            int(pratio), int(epochs), int(batch), float(opt[0]), float(opt[1]),
                                            float(opt[2]), float(clamp), O.ctypes.data, P.ctypes.data,
-                                           lpe.ctypes.data, int(max_steps))
+                                           lpe.ctypes.data, int(max_steps), int(halve_from))
     return O, P, lpe[: int(epochs)]
 
 
@@ -125,7 +126,7 @@ def prepare_amplitude(raw):
 
 
 def led_offsets_matlab(n, pratio, wavelength_um=0.532, pixel_um=1.625, led_pitch_um=4000.0,
-                       led_height_um=90000.0, led_grid=19, spiral=True):
+                       led_height_um=90000.0, led_grid=19, spiral=True, objdx=0.0, objdy=0.0):
     """numpy restatement of init_environment_rgb.m:33-134 + main_run_FPM.m:23,27-31,53-55.
 
     Mirrors MATLAB step by step, including sin/cos(pi/2*d) + round in the
@@ -143,9 +144,10 @@ def led_offsets_matlab(n, pratio, wavelength_um=0.532, pixel_um=1.625, led_pitch
     vled = np.arange(g) - lit_cen
     hled = np.arange(g) - lit_cen
     hhled, vvled = np.meshgrid(hled, vled)
-    den = np.sqrt((-vvled * led_pitch_um + 0) ** 2 + (-hhled * led_pitch_um + 0) ** 2 + led_height_um ** 2)
-    v = (-vvled * led_pitch_um + 0) / den
-    u = -(-hhled * led_pitch_um + 0) / den
+    # init_environment_rgb.m:83-84 (objdx, objdy: the field area, :26-29)
+    den = np.sqrt((-vvled * led_pitch_um + objdx) ** 2 + (-hhled * led_pitch_um + objdy) ** 2 + led_height_um ** 2)
+    v = (-vvled * led_pitch_um + objdx) / den
+    u = -(-hhled * led_pitch_um + objdy) / den
     u, v = v, u  # init_environment_rgb.m:86-88
     pos = np.zeros((g, g, 2), dtype=np.int64)
     margin = np.inf

@@ -239,11 +239,13 @@ double FN(oracle_loss)(const CT* O, int Nr, int Nc, const CT* P, int nr, int nc,
 }
 
 /* The solve of cuFPM_pure_v3.cu:128-207 with v3's update rule.  max_steps > 0
- * stops after that many minibatch steps (a bounded sample for timing). */
+ * stops after that many minibatch steps (a bounded sample for timing).  lr
+ * halves after every epoch with index >= halve_from (13 is the reference's
+ * `ep > 12`, :204-206). */
 void FN(oracle_solve)(const CT* O0, int Nr, int Nc, const CT* P0, int nr, int nc, const float* Y, int L,
                       const int32_t* led, const float* A, int pratio, int epochs, int batch, float lr_in,
                       float beta_in, float eps_in, float clamp_in, CT* O, CT* P, double* loss_per_epoch,
-                      int max_steps) {
+                      int max_steps, int halve_from) {
     const long NN = (long)Nr * Nc, nn = (long)nr * nc;
     memcpy(O, O0, sizeof(CT) * NN);
     memcpy(P, P0, sizeof(CT) * nn);
@@ -299,7 +301,7 @@ void FN(oracle_solve)(const CT* O0, int Nr, int Nc, const CT* P0, int nr, int nc
             ++steps_done;
         }
         if (loss_per_epoch) loss_per_epoch[ep] = lsum;
-        if (ep > 12) lr = (float)((double)lr * 0.5);
+        if (ep >= halve_from) lr = (float)((double)lr * 0.5);
     }
     free(mO);
     free(mP);

@@ -84,6 +84,22 @@ def test_led_offsets_match_oracle_restatement():
     assert np.array_equal(a, b)
 
 
+def test_led_offsets_at_field_areas_match_oracle_restatement():
+    """fpm_led_offsets_at equals the numpy restatement of
+    init_environment_rgb.m:83-84 with objdx, objdy at several field areas,
+    and equals fpm_led_offsets at (0, 0)."""
+    import oracle
+    from fpm_amd import tiles as ft
+    for n, r, grid, dx, dy in [(128, 4, 19, 0.0, 0.0), (128, 4, 19, -600.5, 250.0), (64, 4, 7, 1300.0, -975.0),
+                               (256, 8, 19, -2000.0, -1500.25)]:
+        a, _, nai = ft.led_offsets_at(dx, dy, n, r, led_grid=grid)
+        b, _, _ = oracle.led_offsets_matlab(n, r, led_grid=grid, objdx=dx, objdy=dy)
+        assert np.array_equal(a, b), (n, r, grid, dx, dy)
+        if dx == 0.0 and dy == 0.0:
+            c, _, _ = fpm_amd.led_offsets(n, r, led_grid=grid)
+            assert np.array_equal(a, c)
+
+
 def test_led_offsets_rejects_even_grid():
     with pytest.raises(fpm_amd.FPMError):
         fpm_amd.led_offsets(128, 4, led_grid=4)

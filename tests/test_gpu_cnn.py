@@ -14,8 +14,17 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _net(levels=3, n=16, batch=2, seed=1):
-    return warmstart.WarmStartNet(n, 4, levels=levels, base=32, in_channels=26, max_batch=batch, seed=seed)
+def _net(levels=3, n=16, batch=2, seed=1, random_head=True):
+    """Network as created (He-normal convs, zero head) with, by default, a
+    random head, so that every layer's output and gradient is non-trivial."""
+    net = warmstart.WarmStartNet(n, 4, levels=levels, base=32, in_channels=26, max_batch=batch, seed=seed)
+    if random_head:
+        p = net.get_params()
+        ci, co, _ = cnn_oracle.layer_shapes(levels, 32, 26, 4)[-1]
+        k = p.size - (co * 9 * ci + co)
+        p[k:k + co * 9 * ci] = np.random.default_rng(seed).standard_normal(co * 9 * ci) * np.sqrt(2.0 / (9 * ci))
+        net.set_params(p)
+    return net
 
 
 def _inputs(n, batch, seed):
@@ -68,16 +77,18 @@ def test_cnn_gradient_matches_oracle(levels, n):
 
 def test_cnn_training_fits_a_batch():
     """Adam steps through the library's own backward kernels drive the loss
-    of a fixed batch down by 10x (the gradient is a descent direction end to end)."""
-    net = _net(3, 16, seed=3)
-    f, t = _inputs(16, 2, 21)
-    t = (0.5 * np.tanh(t)).astype(np.float32)
-    F, T = torch.from_numpy(f).cuda(), torch.from_numpy(t).cuda()
+    of a fixed batch down by 5x in 150 steps (the gradient is a descent
+    direction end to end).  The target is learnable by the network: the
+    first two feature planes upsampled 4x (nearest), centred."""
+    net = _net(3, 16, seed=3, random_head=False)
+    f, _ = _inputs(16, 2, 21)
+    t = np.repeat(np.repeat(f[:, :2] - 0.5, 4, axis=2), 4, axis=3).astype(np.float32)
+    F, T = torch.from_numpy(f).cuda(), torch.from_numpy(np.ascontiguousarray(t)).cuda()
     l0 = net.train_step(F, T, 2e-3)
     for _ in range(150):
         l = net.train_step(F, T, 2e-3)
     print(f"[cnn-train] loss {l0:.4f} -> {l:.4f}", flush=True)
-    assert l < 0.1 * l0
+    assert l < 0.2 * l0
     net.close()
 
 
@@ -106,7 +117,7 @@ def test_warm_start_gains_over_plain_init():
     for k in range(4):
         feats, target, ds = warmstart.make_sample(n, r, 90000 + k, led_grid=grid)
         O_true = ds["O_true_dev"].cpu().numpy()
-        O_net = warmstart.warm_start_object(net, feats)
+        O_net = warmstart.warm_start_object(net, feats, ds["O0_dev"].real)
         for O0, errs, losses in ((ds["O0_dev"], e_plain, l_plain), (O_net, e_warm, l_warm)):
             errs.append(_phase_aligned_rel_l2(O0.cpu().numpy(), O_true))
             Od, Pd = torch.empty_like(O0), torch.empty_like(ds["P0_dev"])

@@ -146,6 +146,36 @@ def test_graph_replay_equals_eager_launches(batch):
     assert np.array_equal(Og, Og2) and np.array_equal(Pg, Pg2) and np.array_equal(lg, lg2)
 
 
+def test_lr_schedule_option_matches_fp32_oracle():
+    """opts.lr_halve_from: 0 and 13 are the reference's schedule (same bits);
+    halving from epoch 3 over 8 epochs matches the FP32 oracle run with the
+    same schedule (rel-L2 <= 1e-3) and differs from the reference schedule."""
+    n, r, grid = 32, 4, 7
+    O0, P0, Y, led, A = _dataset(n, r, grid)
+    opt = (5e-4, 0.999, 1e-4)
+    N = n * r
+    import torch
+    dev = [torch.from_numpy(np.ascontiguousarray(a.T)).cuda() for a in (O0, P0, Y, led, A)]
+    dO, dP, dY, dled, dA = dev
+
+    def run(h):
+        Od, Pd = torch.empty_like(dO), torch.empty_like(dP)
+        torch.cuda.synchronize()
+        loss = fpm_amd.solve_device(dO.data_ptr(), dP.data_ptr(), dY.data_ptr(), False, dled.data_ptr(),
+                                    dA.data_ptr(), (N, N), (n, n), Y.shape[2], r, 8, 10, opt, Od.data_ptr(),
+                                    Pd.data_ptr(), loss=True, lr_halve_from=h)
+        return Od.cpu().numpy().T, Pd.cpu().numpy().T, loss
+
+    O0h, P0h, l0 = run(0)
+    O13, P13, l13 = run(13)
+    O3, P3, l3 = run(3)
+    assert np.array_equal(O0h, O13) and np.array_equal(P0h, P13) and np.array_equal(l0, l13)
+    Oo, Po, lo = oracle.solve(O0, P0, Y, led, A, r, 8, 10, opt, prec="f32", halve_from=3)
+    assert rel_l2(O3, Oo) <= 1e-3 and rel_l2(P3, Po) <= 1e-3
+    assert np.allclose(l3, lo, rtol=1e-3)
+    assert not np.array_equal(O3, O13)
+
+
 def test_solve_u16_path_equals_prepared_f32():
     """Y as uint16 raw counts (normalised on the device) gives the same
     bits as Y prepared on the host the way MATLAB does in single."""

@@ -63,7 +63,7 @@ def stage_alg_bytes(n, N, b_avg, y_bytes, stitch):
 
 
 # stage -> kernel whose per-dispatch HBM traffic stands for it in the PMC summary
-STAGE_KERNEL = {"fused_update": "fused_update_kernel<16384, true>"}
+STAGE_KERNEL = {"fused_update": "fused_update_kernel<16384, true, 0>"}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_fpm_solve_16384_b26.md")
 
 

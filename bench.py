@@ -107,6 +107,26 @@ def run_cpu_baseline(ds, args):
                       f"{n}^2 x {ds['L']} -> {N}^2 workload, timed {dt:.2f} s, scaled x{steps} steps/epoch"}
 
 
+def fp64_check(result, sched):
+    """The FP64 oracle (host cores) run from the same warm start, schedule,
+    images and LED windows as the GPU's warm-started solve: rel-L2 of the GPU
+    object and pupil against it."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    O0, P0, raw, led, A, r, batch, opt, O_gpu, P_gpu = result
+    mn, mx = int(raw.min()), int(raw.max())
+    Y = np.sqrt((raw.astype(np.float32) - np.float32(mn)) / np.float32(mx - mn)).astype(np.float32)
+    t0 = time.perf_counter()
+    Oo, Po, _ = oracle.solve(O0, P0, np.asfortranarray(Y), led, A, r, sched[0], batch, opt, prec="f64",
+                             halve_from=sched[1])
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    return {"O": rel(O_gpu, Oo), "P": rel(P_gpu, Po), "oracle_s": time.perf_counter() - t0,
+            "oracle_threads": oracle.num_threads(), "epochs": sched[0], "lr_halve_from": sched[1]}
+
+
 def run_time_to_loss(args):
     """Plain solve vs learned warm start + solve on the warm-start workload
     (361 uint16 images of 256 x 256 -> 1024 x 1024, batch 26, 20 epochs):
@@ -134,8 +154,11 @@ def run_time_to_loss(args):
     log(f"[bench] warm schedule (epochs, lr halving from) = {sched}")
     res = {}
     for name, eng in (("fp32", fpm_amd.FFT_F32), ("split", fpm_amd.FFT_SPLIT_F16)):
-        rr = warmstart.time_to_loss(net, n, r, seed=77001, warm_engine=eng, schedule=sched)
+        rr = warmstart.time_to_loss(net, n, r, seed=77001, warm_engine=eng, schedule=sched,
+                                    keep_result=args.ttl_fp64 and name == "fp32")
         rr.pop("loss_per_epoch")
+        if "result" in rr:
+            rr["warm_rel_l2_vs_fp64_oracle"] = fp64_check(rr.pop("result"), sched)
         res[name] = rr
         log(f"[bench] time to loss ({name} warm solve): ratio {rr['ratio_vs_plain']:.3f}, "
             f"reached {rr['warm_reaches_plain_loss']}")
@@ -204,7 +227,10 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="capture each epoch as a hipGraph and replay it (per-stage events off)")
     ap.add_argument("--no-time-to-loss", action="store_true", help="skip the warm-start time-to-loss leg")
-    ap.add_argument("--ttl-samples", type=int, default=2048, help="synthetic training objects of the warm start")
+    ap.add_argument("--ttl-samples", type=int, default=4096, help="synthetic training objects of the warm start")
+    ap.add_argument("--ttl-fp64", action="store_true",
+                    help="also run the FP64 oracle from the warm start (host cores) and report the GPU result's "
+                         "rel-L2 against it")
     ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
     ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
     ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")

@@ -275,7 +275,7 @@ def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0):
                         epochs, batch, opt, Od.data_ptr(), Pd.data_ptr(), loss=True, engine=engine,
                         lr_halve_from=halve_from)
     torch.cuda.synchronize()
-    return time.perf_counter() - t0, loss, Od
+    return time.perf_counter() - t0, loss, Od, Pd
 
 
 # (epochs, lr halving from epoch index) tried for the warm-started solve, shortest first;
@@ -295,10 +295,10 @@ def tune_warm_schedule(net: WarmStartNet, n: int, pratio: int, seed: int, *, epo
     (within LOSS_RTOL) on a validation object (not the object the time is
     then measured on)."""
     feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid)
-    _, loss_plain, _ = _timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
+    _, loss_plain, _, _ = _timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
     O_net = warm_start_object(net, feats, ds["O0_dev"].real)
     for e, h in WARM_SCHEDULES:
-        _, loss_w, _ = _timed_solve(ds, O_net, e, batch, opt, engine, h)
+        _, loss_w, _, _ = _timed_solve(ds, O_net, e, batch, opt, engine, h)
         if loss_w[-1] <= loss_plain[-1] * (1.0 + LOSS_RTOL):
             return e, h
     return WARM_SCHEDULES[-1]
@@ -306,14 +306,16 @@ def tune_warm_schedule(net: WarmStartNet, n: int, pratio: int, seed: int, *, epo
 
 def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: int = 20, batch: int = 26,
                  opt=(5e-4, 0.999, 1e-4), engine: int = 0, warm_engine: Optional[int] = None, led_grid: int = 19,
-                 repeats: int = 3, schedule=None):
+                 repeats: int = 3, schedule=None, keep_result: bool = False):
     """Plain solve from the reference's init (upsampled centre image, the
     reference's 20-epoch schedule) against network inference + the solve from
     the network's object with `schedule` = (epochs, lr_halve_from) (from
     tune_warm_schedule on another object; default: the reference's), on one
     held-out synthetic object.  The plain form's final loss is the target.
     engine / warm_engine: FFT engine of the plain / warm-started solve.
-    Times are the best of `repeats` runs."""
+    Times are the best of `repeats` runs.  keep_result adds "result": the
+    warm solve's inputs and outputs as host arrays in MATLAB orientation
+    (O0, P0, raw uint16 stack, LED windows, A, pratio, batch, opt, O, P)."""
     warm_engine = engine if warm_engine is None else warm_engine
     import time
 
@@ -321,8 +323,8 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
 
     feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid)
     O_true = ds["O_true_dev"].cpu().numpy()
-    t_plain, loss_plain, O_plain = min((_timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
-                                        for _ in range(repeats)), key=lambda r: r[0])
+    t_plain, loss_plain, O_plain, _ = min((_timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
+                                           for _ in range(repeats)), key=lambda r: r[0])
     t_inf = float("inf")
     for _ in range(repeats):
         torch.cuda.synchronize()
@@ -331,14 +333,14 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
         torch.cuda.synchronize()
         t_inf = min(t_inf, time.perf_counter() - t0)
     we, wh = schedule if schedule else (epochs, 13)
-    t_warm, loss_warm, O_warm = min((_timed_solve(ds, O_net, we, batch, opt, warm_engine, wh)
-                                     for _ in range(repeats)), key=lambda r: r[0])
+    t_warm, loss_warm, O_warm, P_warm = min((_timed_solve(ds, O_net, we, batch, opt, warm_engine, wh)
+                                             for _ in range(repeats)), key=lambda r: r[0])
     target = float(loss_plain[-1])
     reached = bool(loss_warm[-1] <= target * (1.0 + LOSS_RTOL))
     ttl = t_inf + t_warm
     e_init = phase_aligned_rel_l2(O_net.cpu().numpy(), O_true)
     e_plain = phase_aligned_rel_l2(O_plain.cpu().numpy(), O_true)
-    return {
+    out = {
         "plain_epochs": epochs, "plain_final_loss": target, "plain_s": t_plain,
         "warm_schedule": {"epochs": we, "lr_halve_from": wh},
         "warm_inference_s": t_inf, "warm_solve_s": t_warm, "warm_final_loss": float(loss_warm[-1]),
@@ -354,3 +356,10 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
         "warm_time_to_plain_error_s": t_inf if e_init <= e_plain else None,
         "loss_per_epoch": {"plain": [float(v) for v in loss_plain], "warm": [float(v) for v in loss_warm]},
     }
+    if keep_result:
+        # device layout is the transpose of the MATLAB orientation the oracle takes
+        t = lambda x: np.asfortranarray(x.cpu().numpy().T)  # noqa: E731
+        raw = np.asfortranarray(ds["raw_dev"].cpu().numpy().transpose(2, 1, 0))
+        out["result"] = (t(O_net), t(ds["P0_dev"]), raw, ds["led"], ds["A"], pratio, batch, opt, t(O_warm),
+                         t(P_warm))
+    return out

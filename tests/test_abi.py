@@ -103,3 +103,57 @@ def test_led_offsets_at_field_areas_match_oracle_restatement():
 def test_led_offsets_rejects_even_grid():
     with pytest.raises(fpm_amd.FPMError):
         fpm_amd.led_offsets(128, 4, led_grid=4)
+
+
+def test_opts_new_fields_default_and_bad_values():
+    """graph / fft_engine / lr_halve_from default to the reference's behaviour;
+    an unknown engine and a negative schedule are rejected before any device work."""
+    o = fpm_amd._Opts()
+    fpm_amd.lib().fpm_opts_init(ctypes.byref(o))
+    assert o.graph == 0 and o.fft_engine == fpm_amd.FFT_F32 and o.lr_halve_from == 0
+    Oz = np.zeros((32, 32), np.complex64, order="F")
+    Pz = np.zeros((16, 16), np.complex64, order="F")
+    Y = np.ones((16, 16, 4), np.float32, order="F")
+    led = np.ones((2, 4), np.int32, order="F")
+    A = np.ones((16, 16), np.float32, order="F")
+    for field, val, word in (("fft_engine", 7, "fft_engine"), ("lr_halve_from", -1, "lr_halve_from")):
+        o = fpm_amd._opts()
+        setattr(o, field, val)
+        optv = (ctypes.c_double * 3)(1e-3, 0.999, 1e-4)
+        Oo, Po = np.empty_like(Oz), np.empty_like(Pz)
+        rc = fpm_amd.lib().fpm_pure_v3(Oz.ctypes.data, 32, 32, Pz.ctypes.data, 16, 16, Y.ctypes.data, 0, 4,
+                                       led.ctypes.data, A.ctypes.data, 2.0, 1.0, 2.0, optv, Oo.ctypes.data,
+                                       Po.ctypes.data, ctypes.byref(o))
+        assert rc == -1 and word in fpm_amd.lib().fpm_last_error().decode()
+
+
+def test_tiles_solve_rejects_bad_arguments():
+    from fpm_amd import tiles as ft
+    L = ft._tiles_lib()
+    optv = (ctypes.c_double * 3)(1e-3, 0.999, 1e-4)
+    arr = (ft.Tile * 1)()
+    # null arrays of a tile, no tiles, bad stream count: FPM_E_INVALID before any device work
+    assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 0, None, None) == -1
+    assert "null" in L.fpm_last_error().decode()
+    assert L.fpm_tiles_solve(arr, 0, 4, 2.0, 1.0, 2.0, optv, 0, None, None) == -1
+    assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 65, None, None) == -1
+    # a tile side that is not a power of two
+    buf = np.zeros(4096, np.complex64)
+    p = buf.ctypes.data
+    arr[0] = ft.Tile(p, p, 48, p, p, 24, p, 0, p, p, None)
+    assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 1, None, None) == -2
+
+
+def test_cnn_create_rejects_bad_descriptors():
+    from fpm_amd import warmstart as ws
+    L = ws._cnn_lib()
+    h = ctypes.c_void_p()
+    for desc in (ws.CnnDesc(0, 32, 26, 64, 4, 1),     # levels
+                 ws.CnnDesc(4, 48, 26, 64, 4, 1),     # base not a multiple of 32
+                 ws.CnnDesc(5, 32, 26, 64, 4, 1),     # 32 * 2^4 channels > 256
+                 ws.CnnDesc(4, 32, 26, 12, 4, 1),     # n not divisible by 2^(levels-1)
+                 ws.CnnDesc(4, 32, 26, 64, 2, 1),     # 2 up^2 not a multiple of 32
+                 ws.CnnDesc(4, 32, 26, 64, 4, 0)):    # max_batch
+        assert L.fpm_cnn_create(ctypes.byref(desc), 0, ctypes.byref(h)) == -1
+        assert not h.value
+    assert L.fpm_cnn_num_params(None) == -1

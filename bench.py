@@ -155,7 +155,7 @@ def run_time_to_loss(args):
     res = {}
     for name, eng in (("fp32", fpm_amd.FFT_F32), ("split", fpm_amd.FFT_SPLIT_F16)):
         rr = warmstart.time_to_loss(net, n, r, seed=77001, warm_engine=eng, schedule=sched,
-                                    keep_result=args.ttl_fp64 and name == "fp32")
+                                    keep_result=not args.no_ttl_fp64 and name == "fp32")
         rr.pop("loss_per_epoch")
         if "result" in rr:
             rr["warm_rel_l2_vs_fp64_oracle"] = fp64_check(rr.pop("result"), sched)
@@ -228,9 +228,9 @@ def main():
                     help="capture each epoch as a hipGraph and replay it (per-stage events off)")
     ap.add_argument("--no-time-to-loss", action="store_true", help="skip the warm-start time-to-loss leg")
     ap.add_argument("--ttl-samples", type=int, default=4096, help="synthetic training objects of the warm start")
-    ap.add_argument("--ttl-fp64", action="store_true",
-                    help="also run the FP64 oracle from the warm start (host cores) and report the GPU result's "
-                         "rel-L2 against it")
+    ap.add_argument("--no-ttl-fp64", action="store_true",
+                    help="skip the FP64 oracle run (host cores) from the warm start that the GPU's warm-started "
+                         "result is compared with")
     ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
     ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
     ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")

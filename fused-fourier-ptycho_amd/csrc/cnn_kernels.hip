@@ -28,7 +28,11 @@ __device__ __forceinline__ f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-static unsigned blocks_for(int64_t n, int t) { return (unsigned)std::min<int64_t>((n + t - 1) / t, 1 << 20); }
+static unsigned blocks_for(int64_t n, int t) {
+    // the element-wise kernels decode indices in 32 bits
+    if (n >= ((int64_t)1 << 31)) throw HipError{hipErrorInvalidValue, "cnn: tensor of 2^31 or more elements"};
+    return (unsigned)std::min<int64_t>((n + t - 1) / t, 1 << 20);
+}
 
 // --------------------------------------------------------------- conv 3x3
 // One wave per 32 pixels (flattened index within an image) x 32 output
@@ -178,16 +182,19 @@ __global__ __launch_bounds__(256) void im2col_t_kernel(const bf16_t* __restrict_
     const int c8 = C / 8;
     const int64_t total = Q * c8;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
-        const int64_t q = t % Q;
-        const int cg = (int)(t / Q);
-        const int xx = (int)(q % W);
-        const int yy = (int)((q / W) % H);
-        const int64_t b = q / ((int64_t)H * W);
+        // 32-bit index arithmetic (the launcher checks the sizes): 64-bit
+        // division is a long instruction sequence on the GPU
+        const uint32_t q = (uint32_t)t % (uint32_t)Q;
+        const uint32_t cg = (uint32_t)t / (uint32_t)Q;
+        const uint32_t xx = q % (uint32_t)W;
+        const uint32_t qw = q / (uint32_t)W;
+        const uint32_t yy = qw % (uint32_t)H;
+        const uint32_t b = qw / (uint32_t)H;
         for (int tap = 0; tap < 9; ++tap) {
-            const int sy = yy + tap / 3 - 1, sx = xx + tap % 3 - 1;
+            const int sy = (int)yy + tap / 3 - 1, sx = (int)xx + tap % 3 - 1;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (sy >= 0 && sy < H && sx >= 0 && sx < W)
-                v = *(const uint4*)(x + ((b * H + sy) * W + sx) * C + 8 * cg);
+                v = *(const uint4*)(x + (((size_t)b * H + sy) * W + sx) * C + 8 * cg);
             const uint16_t* e = (const uint16_t*)&v;
 #pragma unroll
             for (int k = 0; k < 8; ++k) xcol[((size_t)tap * C + 8 * cg + k) * Q + q] = e[k];
@@ -205,9 +212,10 @@ void im2col_t(const bf16_t* x, bf16_t* xcol, int B, int H, int W, int C, hipStre
 __global__ void flip_transpose_w_kernel(const bf16_t* __restrict__ w, bf16_t* __restrict__ wt, int Cout, int Cin) {
     const int64_t total = (int64_t)Cout * 9 * Cin;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
-        const int co = (int)(t % Cout);
-        const int tap = (int)((t / Cout) % 9);
-        const int ci = (int)(t / ((int64_t)Cout * 9));
+        const uint32_t u = (uint32_t)t, uc = u / (uint32_t)Cout;
+        const int co = (int)(u % (uint32_t)Cout);
+        const int tap = (int)(uc % 9u);
+        const int ci = (int)(uc / 9u);
         wt[t] = w[((size_t)co * 9 + (8 - tap)) * Cin + ci];  // wt(ci, tap, co)
     }
 }
@@ -270,10 +278,11 @@ __global__ void avgpool2_fwd_kernel(const bf16_t* __restrict__ x, bf16_t* __rest
     const int h = H / 2, w = W / 2;
     const int64_t total = (int64_t)B * h * w * C;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
-        const int c = (int)(t % C);
-        const int xx = (int)((t / C) % w);
-        const int yy = (int)((t / ((int64_t)C * w)) % h);
-        const int64_t b = t / ((int64_t)C * w * h);
+        const uint32_t u = (uint32_t)t, u1 = u / (uint32_t)C, u2 = u1 / (uint32_t)w;
+        const int c = (int)(u % (uint32_t)C);
+        const int xx = (int)(u1 % (uint32_t)w);
+        const int yy = (int)(u2 % (uint32_t)h);
+        const int64_t b = (int64_t)(u2 / (uint32_t)h);
         const size_t r0 = ((b * H + 2 * yy) * W + 2 * xx) * C + c;
         const size_t r1 = r0 + (size_t)W * C;
         const float s = (bf2f(x[r0]) + bf2f(x[r0 + C])) + (bf2f(x[r1]) + bf2f(x[r1 + C]));
@@ -285,10 +294,11 @@ __global__ void avgpool2_bwd_kernel(const float* __restrict__ gy, float* __restr
                                     int acc) {
     const int64_t total = (int64_t)B * H * W * C;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
-        const int c = (int)(t % C);
-        const int xx = (int)((t / C) % W);
-        const int yy = (int)((t / ((int64_t)C * W)) % H);
-        const int64_t b = t / ((int64_t)C * W * H);
+        const uint32_t u = (uint32_t)t, u1 = u / (uint32_t)C, u2 = u1 / (uint32_t)W;
+        const int c = (int)(u % (uint32_t)C);
+        const int xx = (int)(u1 % (uint32_t)W);
+        const int yy = (int)(u2 % (uint32_t)H);
+        const int64_t b = (int64_t)(u2 / (uint32_t)H);
         const float v = gy[((b * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c] * 0.25f;
         gx[t] = acc ? gx[t] + v : v;
     }
@@ -299,10 +309,11 @@ __global__ void upsample2_fwd_kernel(const bf16_t* __restrict__ x, bf16_t* __res
     const int H = 2 * h, W = 2 * w;
     const int64_t total = (int64_t)B * H * W * C;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
-        const int c = (int)(t % C);
-        const int xx = (int)((t / C) % W);
-        const int yy = (int)((t / ((int64_t)C * W)) % H);
-        const int64_t b = t / ((int64_t)C * W * H);
+        const uint32_t u = (uint32_t)t, u1 = u / (uint32_t)C, u2 = u1 / (uint32_t)W;
+        const int c = (int)(u % (uint32_t)C);
+        const int xx = (int)(u1 % (uint32_t)W);
+        const int yy = (int)(u2 % (uint32_t)H);
+        const int64_t b = (int64_t)(u2 / (uint32_t)H);
         y[t] = x[((b * h + yy / 2) * w + xx / 2) * C + c];
     }
 }
@@ -311,10 +322,11 @@ __global__ void upsample2_bwd_kernel(const float* __restrict__ gy, float* __rest
     const int W = 2 * w;
     const int64_t total = (int64_t)B * h * w * C;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
-        const int c = (int)(t % C);
-        const int xx = (int)((t / C) % w);
-        const int yy = (int)((t / ((int64_t)C * w)) % h);
-        const int64_t b = t / ((int64_t)C * w * h);
+        const uint32_t u = (uint32_t)t, u1 = u / (uint32_t)C, u2 = u1 / (uint32_t)w;
+        const int c = (int)(u % (uint32_t)C);
+        const int xx = (int)(u1 % (uint32_t)w);
+        const int yy = (int)(u2 % (uint32_t)h);
+        const int64_t b = (int64_t)(u2 / (uint32_t)h);
         const size_t r0 = ((b * 2 * h + 2 * yy) * W + 2 * xx) * C + c;
         const size_t r1 = r0 + (size_t)W * C;
         gx[t] = (gy[r0] + gy[r0 + C]) + (gy[r1] + gy[r1 + C]);
@@ -366,9 +378,10 @@ __global__ void features_kernel(const float* __restrict__ f, bf16_t* __restrict_
                                 int C) {
     const int64_t total = (int64_t)B * H * W * C;
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
-        const int c = (int)(t % C);
-        const int64_t p = (t / C) % ((int64_t)H * W);
-        const int64_t b = t / ((int64_t)C * H * W);
+        const uint32_t u = (uint32_t)t, u1 = u / (uint32_t)C;
+        const int c = (int)(u % (uint32_t)C);
+        const int64_t p = (int64_t)(u1 % (uint32_t)(H * W));
+        const int64_t b = (int64_t)(u1 / (uint32_t)(H * W));
         x[t] = c < Cf ? f2bf(f[(b * Cf + c) * H * W + p]) : (bf16_t)0;
     }
 }
@@ -381,15 +394,16 @@ void features_to_nhwc(const float* f, bf16_t* x, int B, int Cf, int H, int W, in
 // -------------------------------------------------------------- head
 // head element e = ((b H + y) W + x) 2u^2 + ch, ch = c u^2 + sy u + sx
 __device__ __forceinline__ size_t head_out_index(int64_t e, int H, int W, int u) {
-    const int U2 = u * u, CH = 2 * U2;
-    const int ch = (int)(e % CH);
-    const int64_t pix = e / CH;
-    const int x = (int)(pix % W);
-    const int y = (int)((pix / W) % H);
-    const int64_t b = pix / ((int64_t)W * H);
-    const int c = ch / U2, sy = (ch % U2) / u, sx = ch % u;
-    const int64_t OH = (int64_t)u * H, OW = (int64_t)u * W;
-    return (size_t)(((b * 2 + c) * OH + (int64_t)u * y + sy) * OW + (int64_t)u * x + sx);
+    // 32-bit index arithmetic (the launchers check the sizes)
+    const uint32_t U2 = (uint32_t)(u * u), CH = 2u * U2;
+    const uint32_t ev = (uint32_t)e, pix = ev / CH, pw = pix / (uint32_t)W;
+    const uint32_t ch = ev % CH;
+    const uint32_t x = pix % (uint32_t)W;
+    const uint32_t y = pw % (uint32_t)H;
+    const uint32_t b = pw / (uint32_t)H;
+    const uint32_t c = ch / U2, sy = (ch % U2) / (uint32_t)u, sx = ch % (uint32_t)u;
+    const size_t OH = (size_t)u * H, OW = (size_t)u * W;
+    return (((size_t)b * 2 + c) * OH + (size_t)u * y + sy) * OW + (size_t)u * x + sx;
 }
 
 __global__ void head_shuffle_kernel(const float* __restrict__ yf, float* __restrict__ out, int B, int H, int W, int u) {

@@ -92,6 +92,21 @@ def test_cnn_training_fits_a_batch():
     net.close()
 
 
+def test_cnn_save_load_roundtrip(tmp_path):
+    """Parameters survive save/load (plain npz arrays) and give the same output."""
+    net = _net(2, 16, seed=4)
+    f, _ = _inputs(16, 2, 30)
+    F = torch.from_numpy(f).cuda()
+    out = net.forward(F).cpu().numpy()
+    path = str(tmp_path / "net.npz")
+    net.save(path)
+    net2 = warmstart.WarmStartNet.load(path)
+    assert np.array_equal(net2.get_params(), net.get_params())
+    assert np.array_equal(net2.forward(F).cpu().numpy(), out)
+    net.close()
+    net2.close()
+
+
 def _phase_aligned_rel_l2(a, b):
     """min over a global phase factor of ||a e^{i phi} - b|| / ||b||"""
     a = a.astype(np.complex128).ravel()

@@ -103,3 +103,23 @@ def test_split_solve_20_epochs_matches_fp32_oracle():
     assert rel_l2(Os, Oo) <= 1e-3
     assert rel_l2(Ps, Po) <= 1e-3
     assert np.allclose(ls, lo, rtol=1e-3)
+
+
+def test_split_engine_with_graph_replay_and_atomic_stitch():
+    """The split engine composes with the other options: graph replay gives
+    the same bits as eager launches, and the atomic stitch stays within the
+    20-epoch bar of the FP32 oracle."""
+    n, r, grid = 32, 4, 7
+    led = small_leds(n, r, grid)
+    A, Ptrue = synth.pupil(n)
+    O = synth.object_numpy(n * r, r, 0.3, 12)
+    amp = np.asfortranarray(synth.image_model_numpy(O, Ptrue.astype(np.complex128), led, r).astype(np.float32))
+    O0 = np.asfortranarray(synth.bilinear_upsample(amp[:, :, 0].astype(np.float64), r).astype(np.complex64))
+    P0 = np.asfortranarray(A.astype(np.complex64))
+    opt = (5e-4, 0.999, 1e-4)
+    a = fpm_amd.cuFPM_pure_v3(O0, P0, amp, led, A, r, 15, 10, opt, loss=True, engine=SPLIT)
+    b = fpm_amd.cuFPM_pure_v3(O0, P0, amp, led, A, r, 15, 10, opt, loss=True, engine=SPLIT, graph=True)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    Oa, Pa = fpm_amd.cuFPM_pure_v3(O0, P0, amp, led, A, r, 20, 10, opt, engine=SPLIT, stitch=fpm_amd.STITCH_ATOMIC)
+    Oo, Po, _ = oracle.solve(O0, P0, amp, led, A, r, 20, 10, opt, prec="f32")
+    assert rel_l2(Oa, Oo) <= 1e-3 and rel_l2(Pa, Po) <= 1e-3

@@ -227,49 +227,56 @@ void flip_transpose_w(const bf16_t* w, bf16_t* wt, int Cout, int Cin, hipStream_
 }
 
 // ---------------------------------------------------------- grad prep
-// workgroup: 64 consecutive q, all C (<= 256) channels through an LDS tile
+// workgroup: chunks of 64 consecutive q (grid-stride), all C (<= 256)
+// channels through an LDS tile; per-channel bias sums stay in LDS across the
+// chunks and end in one atomic per channel per workgroup
 constexpr int kGpQ = 64;
+constexpr unsigned kGpMaxBlocks = 1024;
 __global__ __launch_bounds__(256) void grad_prep_kernel(const float* __restrict__ gy, const bf16_t* __restrict__ y,
                                                         int relu, bf16_t* __restrict__ gz, bf16_t* __restrict__ gz_t,
                                                         float* __restrict__ db, int64_t Q, int C) {
     __shared__ bf16_t tile[kGpQ * 257];
     __shared__ float dbs[256];
     const int t = threadIdx.x;
-    const int64_t q0 = (int64_t)blockIdx.x * kGpQ;
+    const int elems = kGpQ * C;  // <= 16384
     for (int c = t; c < C; c += 256) dbs[c] = 0.0f;
-    __syncthreads();
-    for (int e = t; e < kGpQ * C; e += 256) {
-        const int qq = e / C, c = e % C;
-        const int64_t q = q0 + qq;
-        bf16_t h = 0;
-        if (q < Q) {
-            const size_t o = (size_t)q * C + c;
-            float v = gy[o];
-            if (relu && !(bf2f(y[o]) > 0.0f)) v = 0.0f;
-            h = f2bf(v);
-            gz[o] = h;
+    for (int64_t q0 = (int64_t)blockIdx.x * kGpQ; q0 < Q; q0 += (int64_t)gridDim.x * kGpQ) {
+        __syncthreads();  // the previous chunk's transposed reads are done
+        for (int e = t; e < elems; e += 256) {
+            const int qq = e / C, c = e - qq * C;
+            const int64_t q = q0 + qq;
+            bf16_t h = 0;
+            if (q < Q) {
+                const size_t o = (size_t)q * C + c;
+                float v = gy[o];
+                if (relu && !(bf2f(y[o]) > 0.0f)) v = 0.0f;
+                h = f2bf(v);
+                gz[o] = h;
+            }
+            tile[qq * 257 + c] = h;
         }
-        tile[qq * 257 + c] = h;
+        __syncthreads();
+        // per-channel sums of the rounded values (thread c owns channel c)
+        for (int c = t; c < C; c += 256) {
+            float s = 0.0f;
+            for (int qq = 0; qq < kGpQ; ++qq) s += bf2f(tile[qq * 257 + c]);
+            dbs[c] += s;
+        }
+        for (int e = t; e < elems; e += 256) {
+            const int c = e / kGpQ, qq = e % kGpQ;
+            const int64_t q = q0 + qq;
+            if (q < Q) gz_t[(size_t)c * Q + q] = tile[qq * 257 + c];
+        }
     }
-    __syncthreads();
-    // per-channel sums of the rounded values; one atomic per channel per workgroup
-    for (int c = t; c < C; c += 256) {
-        float s = 0.0f;
-        for (int qq = 0; qq < kGpQ; ++qq) s += bf2f(tile[qq * 257 + c]);
-        unsafeAtomicAdd(&db[c], s);
-    }
-    for (int e = t; e < kGpQ * C; e += 256) {
-        const int c = e / kGpQ, qq = e % kGpQ;
-        const int64_t q = q0 + qq;
-        if (q < Q) gz_t[(size_t)c * Q + q] = tile[qq * 257 + c];
-    }
+    for (int c = t; c < C; c += 256) unsafeAtomicAdd(&db[c], dbs[c]);
 }
 
 void grad_prep(const float* gy, const bf16_t* y, int relu, bf16_t* gz, bf16_t* gz_t, float* db, int64_t Q, int C,
                hipStream_t st) {
     if (C > 256) throw HipError{hipErrorInvalidValue, "grad_prep: at most 256 channels"};
-    hipLaunchKernelGGL(grad_prep_kernel, dim3((unsigned)((Q + kGpQ - 1) / kGpQ)), dim3(256), 0, st, gy, y, relu, gz,
-                       gz_t, db, Q, C);
+    if (Q <= 0) return;
+    const unsigned blocks = (unsigned)std::min<int64_t>((Q + kGpQ - 1) / kGpQ, kGpMaxBlocks);
+    hipLaunchKernelGGL(grad_prep_kernel, dim3(blocks), dim3(256), 0, st, gy, y, relu, gz, gz_t, db, Q, C);
     FPM_LAUNCH_CHECK();
 }
 
@@ -438,8 +445,11 @@ void head_shuffle(const float* yf, float* out, int B, int H, int W, int u, hipSt
 void head_mse(const float* yf, const float* target, float* gyf, double* loss_acc, int B, int H, int W, int u,
               double inv_count, hipStream_t st) {
     const int64_t total = (int64_t)B * H * W * 2 * u * u;
-    hipLaunchKernelGGL(head_mse_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, yf, target, gyf, loss_acc, B,
-                       H, W, u, inv_count);
+    // every workgroup ends in one FP64 atomic on the same address: few
+    // workgroups, each striding over many elements
+    const unsigned blocks = std::min(blocks_for(total, 256), 1024u);
+    hipLaunchKernelGGL(head_mse_kernel, dim3(blocks), dim3(256), 0, st, yf, target, gyf, loss_acc, B, H, W, u,
+                       inv_count);
     FPM_LAUNCH_CHECK();
 }
 

@@ -130,8 +130,9 @@ def fp64_check(result, sched):
 def run_time_to_loss(args):
     """Plain solve vs learned warm start + solve on the warm-start workload
     (361 uint16 images of 256 x 256 -> 1024 x 1024, batch 26, 20 epochs):
-    the network (4 levels, base 32) is trained here on synthetic objects
-    through the forward model, the warm form's schedule is chosen on a
+    the network (4 levels, base 32, object and 5 pupil Zernike coefficients)
+    is trained here on synthetic objects imaged through random pupils
+    (coefficients N(0, 0.4) rad), the warm form's schedule is chosen on a
     validation object, and both forms are timed on a held-out object."""
     import torch
 
@@ -141,14 +142,15 @@ def run_time_to_loss(args):
     n, r = 256, 4
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
-    F, T = warmstart.make_training_set(args.ttl_samples, n, r, seed0=1000)
+    F, T, Z = warmstart.make_training_set(args.ttl_samples, n, r, seed0=1000, pupil_sigma=warmstart.PUPIL_SIGMA)
     t_gen = time.perf_counter() - t0
-    net = warmstart.WarmStartNet(n, r, levels=4, base=32, max_batch=8, seed=0)
+    net = warmstart.WarmStartNet(n, r, levels=4, base=32, max_batch=8, seed=0, pupil_terms=warmstart.PUPIL_TERMS,
+                                 pupil_weight=args.ttl_pupil_weight)
     t0 = time.perf_counter()
-    hist = warmstart.train(net, F, T, epochs=args.ttl_epochs, batch=8, lr=5e-4, lr_final=5e-5, log=log)
+    hist = warmstart.train(net, F, T, Z, epochs=args.ttl_epochs, batch=8, lr=5e-4, lr_final=5e-5, log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    del F, T
+    del F, T, Z
     torch.cuda.empty_cache()
     sched = warmstart.tune_warm_schedule(net, n, r, seed=77000)
     log(f"[bench] warm schedule (epochs, lr halving from) = {sched}")
@@ -163,10 +165,12 @@ def run_time_to_loss(args):
         log(f"[bench] time to loss ({name} warm solve): ratio {rr['ratio_vs_plain']:.3f}, "
             f"reached {rr['warm_reaches_plain_loss']}")
     net.close()
-    return {"workload": f"361 uint16 images {n}x{n} -> {n * r}x{n * r}, batch 26, plain 20 epochs (FP32 engine, "
-                        f"reference schedule) vs U-Net warm start + solve",
+    return {"workload": f"361 uint16 images {n}x{n} -> {n * r}x{n * r}, batch 26, pupils with 5 Zernike terms "
+                        f"~ N(0, {warmstart.PUPIL_SIGMA}) rad, plain 20 epochs (FP32 engine, reference schedule, "
+                        f"ideal-pupil init) vs U-Net warm start (object + pupil) + solve",
             "training": {"objects": args.ttl_samples, "epochs": args.ttl_epochs, "batch": 8, "generate_s": t_gen,
-                         "train_s": t_train, "loss_first_last": [hist[0], hist[-1]]},
+                         "train_s": t_train, "pupil_weight": args.ttl_pupil_weight,
+                         "loss_first_last": [hist[0], hist[-1]]},
             "warm_schedule_tuned_on": "validation object seed 77000", "test_object": "seed 77001",
             "warm_fft_engine": res}
 
@@ -232,6 +236,8 @@ def main():
                     help="skip the FP64 oracle run (host cores) from the warm start that the GPU's warm-started "
                          "result is compared with")
     ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
+    ap.add_argument("--ttl-pupil-weight", type=float, default=1.0,
+                    help="weight of the pupil coefficients' squared error in the warm start's training loss")
     ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
     ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")
     ap.add_argument("--calibrate-copy", action="store_true",

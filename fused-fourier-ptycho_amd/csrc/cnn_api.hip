@@ -10,8 +10,12 @@
 //   decoder level l = L-2..0:
 //       upsample 2x -> conv+ReLU (d_l,a) ; s_l = d_l,a + e_l ; conv+ReLU (d_l,b)
 //   head: conv (C -> 2 u^2) -> depth-to-space u -> amplitude, phase at u n.
+//   pupil head (K = pupil_terms > 0): mean of e_{L-1} over its pixels ->
+//       64 hidden (ReLU) -> K coefficients of the pupil's phase
 // Parameters (FP32, one vector): per conv in the order above, W (Cout x 9 x
-// Cin, Cin of the first conv rounded up to a multiple of 32) then b (Cout).
+// Cin, Cin of the first conv rounded up to a multiple of 32) then b (Cout);
+// then, with a pupil head, W1 (64 x Cb), b1 (64), W2 (K x 64), b2 (K), where
+// Cb = C 2^(L-1) is the bottom level's channel count.
 #include "../../include/fpm_amd.h"
 #include "cnn_kernels.h"
 
@@ -76,7 +80,11 @@ struct Net {
     int t_in = -1;
     std::vector<int> t_pool, t_ea, t_eb, t_up, t_da, t_s, t_db;
     Buf head_f, head_g;  // FP32 head output and its gradient
-    Buf xcol, gz, gzt, wt, lossd;
+    Buf xcol, gz, gzt, wt, lossd;  // lossd: image loss sum, pupil loss
+    int K = 0;                     // pupil terms (0: no pupil head)
+    int cb = 0;                    // channels of the bottom level
+    int64_t poff = -1;             // offset of the pupil head's parameters
+    Buf pws, pgpool;               // pupil head workspace and pooled gradient
     hipStream_t st = nullptr;
 
     int add_tensor(int H, int W, int C) {
@@ -107,6 +115,9 @@ void check_desc(const fpm_cnn_desc& d) {
         bad("n must be divisible by 2^(levels-1) with at least 8 pixels at the coarsest level");
     if (d.up < 1 || 2 * d.up * d.up % 32) bad("2 up^2 must be a multiple of 32 (up = 4, 8, ...)");
     if (d.max_batch < 1) bad("max_batch must be >= 1");
+    if (d.pupil_terms < 0 || d.pupil_terms > kPupilMaxTerms) bad("pupil_terms must be in [0, 16]");
+    if (d.pupil_terms > 0 && d.max_batch > kPupilMaxBatch) bad("max_batch must be <= 64 with a pupil head");
+    if (!(d.pupil_weight >= 0.0f) || !std::isfinite(d.pupil_weight)) bad("pupil_weight must be finite and >= 0");
 }
 
 // builds the layer list (convs in parameter order) and the tensors
@@ -140,6 +151,26 @@ void build(Net& net) {
         prev_c = c;
     }
     net.add_conv(C, 2 * d.up * d.up, 0, n, n);  // head
+    net.K = d.pupil_terms;
+    net.cb = C << (L - 1);
+    if (net.K > 0) {
+        net.poff = net.nparam;
+        net.nparam += (int64_t)kPupilHidden * net.cb + kPupilHidden + (int64_t)net.K * kPupilHidden + net.K;
+    }
+}
+
+// pupil head parameters (FP32 master copy) or their gradients
+struct PupilParams {
+    float *W1, *b1, *W2, *b2;
+};
+PupilParams pupil_params(const Net& net, float* base) {
+    float* p = base + net.poff;
+    PupilParams q;
+    q.W1 = p;
+    q.b1 = q.W1 + (size_t)kPupilHidden * net.cb;
+    q.W2 = q.b1 + kPupilHidden;
+    q.b2 = q.W2 + (size_t)net.K * kPupilHidden;
+    return q;
 }
 
 // conv index helpers (parameter order: 2 per encoder level, 2 per decoder level, head)
@@ -169,6 +200,10 @@ void alloc_batch(Net& net, int B) {
     net.gz.alloc(max_gz * 2);
     net.gzt.alloc(max_gz * 2);
     net.wt.alloc(max_wt * 2);
+    if (net.K > 0) {
+        net.pws.alloc((size_t)B * (net.cb + kPupilHidden + net.K) * 4);
+        net.pgpool.alloc((size_t)B * net.cb * 4);
+    }
 }
 
 const float* bias_of(const Net& net, const Conv& c) { return net.params.as<float>() + c.boff; }
@@ -183,8 +218,10 @@ void run_conv(Net& net, int ci, const bf16_t* x, bf16_t* y, float* yf, int B) {
 bf16_t* act(Net& net, int t) { return net.T[t]->act.as<bf16_t>(); }
 float* grad(Net& net, int t) { return net.T[t]->grad.as<float>(); }
 
-// forward of a batch already in the input tensor; head output in head_f
-void forward(Net& net, int B) {
+// forward of a batch already in the input tensor; head output in head_f,
+// pupil coefficients (with a pupil head) in the workspace and coef_out (may
+// be null)
+void forward(Net& net, int B, float* coef_out = nullptr) {
     const int L = net.d.levels;
     hipStream_t st = net.st;
     const bf16_t* x = act(net, net.t_in);
@@ -198,6 +235,12 @@ void forward(Net& net, int B) {
         run_conv(net, conv_enc(net, l, 0), x, act(net, net.t_ea[l]), nullptr, B);
         run_conv(net, conv_enc(net, l, 1), act(net, net.t_ea[l]), act(net, net.t_eb[l]), nullptr, B);
         x = act(net, net.t_eb[l]);
+    }
+    if (net.K > 0) {
+        const int hb = net.d.n >> (L - 1);
+        const PupilParams q = pupil_params(net, net.params.as<float>());
+        pupil_head_fwd(act(net, net.t_eb[L - 1]), B, hb * hb, net.cb, q.W1, q.b1, q.W2, q.b2, net.K,
+                       net.pws.as<float>(), coef_out, st);
     }
     int prev = net.t_eb[L - 1];
     for (int l = L - 2; l >= 0; --l) {
@@ -235,8 +278,9 @@ void conv_backward(Net& net, int ci, const bf16_t* x, const bf16_t* y, const flo
     }
 }
 
-// backward through the whole net from head_g; parameter gradients in grads
-void backward(Net& net, int B) {
+// backward through the whole net from head_g (and, with a pupil head, from
+// the coefficient targets); parameter gradients in grads
+void backward(Net& net, int B, const float* pupil_targets) {
     const int L = net.d.levels;
     hipStream_t st = net.st;
     fill_f32(net.grads.as<float>(), net.nparam, 0.0f, st);
@@ -254,6 +298,15 @@ void backward(Net& net, int B) {
         const int prev = l == L - 2 ? net.t_eb[L - 1] : net.t_db[l + 1];
         const int h = net.d.n >> l;
         upsample2_bwd(grad(net, net.t_up[l]), grad(net, prev), B, h / 2, h / 2, net.T[prev]->C, st);
+    }
+    if (net.K > 0) {
+        // the bottom level's output also feeds the pupil head
+        const int hb = net.d.n >> (L - 1);
+        const PupilParams q = pupil_params(net, net.params.as<float>());
+        const PupilParams g = pupil_params(net, net.grads.as<float>());
+        pupil_head_bwd(net.pws.as<float>(), pupil_targets, B, hb * hb, net.cb, q.W1, q.W2, net.K, net.d.pupil_weight,
+                       g.W1, g.b1, g.W2, g.b2, net.pgpool.as<float>(), grad(net, net.t_eb[L - 1]),
+                       net.lossd.as<double>() + 1, st);
     }
     for (int l = L - 1; l >= 0; --l) {
         const bf16_t* xin = l > 0 ? act(net, net.t_pool[l]) : act(net, net.t_in);
@@ -274,18 +327,26 @@ void load_features(Net& net, const float* feats, int B) {
     features_to_nhwc(feats, act(net, net.t_in), B, net.d.in_channels, net.d.n, net.d.n, net.cin_pad, net.st);
 }
 
-double loss_and_grad(Net& net, const float* feats, const float* targets, int B) {
+double loss_and_grad(Net& net, const float* feats, const float* targets, const float* pupil_targets, int B) {
     load_features(net, feats, B);
     forward(net, B);
-    FPM_HIP(hipMemsetAsync(net.lossd.p, 0, sizeof(double), net.st));
+    FPM_HIP(hipMemsetAsync(net.lossd.p, 0, 2 * sizeof(double), net.st));
     const int64_t count = (int64_t)B * 2 * (int64_t)net.d.up * net.d.n * net.d.up * net.d.n;
     head_mse(net.head_f.as<float>(), targets, net.head_g.as<float>(), net.lossd.as<double>(), B, net.d.n, net.d.n,
              net.d.up, 1.0 / (double)count, net.st);
-    backward(net, B);
-    double loss = 0.0;
-    FPM_HIP(hipMemcpyAsync(&loss, net.lossd.p, sizeof(double), hipMemcpyDeviceToHost, net.st));
+    backward(net, B, pupil_targets);
+    double loss[2] = {0.0, 0.0};
+    FPM_HIP(hipMemcpyAsync(loss, net.lossd.p, 2 * sizeof(double), hipMemcpyDeviceToHost, net.st));
     FPM_HIP(hipStreamSynchronize(net.st));
-    return loss / (double)count;
+    return loss[0] / (double)count + loss[1];
+}
+
+void check_call(const Net* net, const float* feats, const float* targets, const float* pupil_targets, int64_t batch) {
+    if (!net || !feats || !targets) throw HipError{hipErrorInvalidValue, "null argument"};
+    if (batch < 1 || batch > net->d.max_batch) throw HipError{hipErrorInvalidValue, "batch outside [1, max_batch]"};
+    if ((net->K > 0) != (pupil_targets != nullptr))
+        throw HipError{hipErrorInvalidValue, net->K > 0 ? "the network has a pupil head: pupil_targets required"
+                                                        : "pupil_targets given to a network without a pupil head"};
 }
 
 int guarded(const char* fn, const std::function<void()>& body) {
@@ -330,7 +391,7 @@ int fpm_cnn_create(const fpm_cnn_desc* desc, uint64_t seed, fpm_cnn** out) {
         net->v.alloc(net->nparam * 4);
         net->grads.alloc(net->nparam * 4);
         net->wbf.alloc(net->nparam * 2);
-        net->lossd.alloc(sizeof(double));
+        net->lossd.alloc(2 * sizeof(double));
         // He-normal weights (std sqrt(2 / fan_in) over the real input
         // channels), zero biases, from a seeded host generator
         std::vector<float> h(net->nparam, 0.0f);
@@ -347,6 +408,12 @@ int fpm_cnn_create(const fpm_cnn_desc* desc, uint64_t seed, fpm_cnn** out) {
                 for (int tap = 0; tap < 9; ++tap)
                     for (int ci = 0; ci < c.cin; ++ci)
                         h[c.woff + ((int64_t)co * 9 + tap) * c.cin + ci] = ci < real_cin ? sd * nd(rng) : 0.0f;
+        }
+        if (net->K > 0) {
+            // pupil head: He-normal hidden layer; zero output layer (the
+            // untrained head predicts the ideal pupil, the plain init)
+            const float sd = std::sqrt(2.0f / (float)net->cb);
+            for (int64_t i = 0; i < (int64_t)kPupilHidden * net->cb; ++i) h[net->poff + i] = sd * nd(rng);
         }
         FPM_HIP(hipMemcpy(net->params.p, h.data(), net->nparam * 4, hipMemcpyHostToDevice));
         fill_f32(net->m.as<float>(), net->nparam, 0.0f, net->st);
@@ -392,26 +459,27 @@ int fpm_cnn_set_params(fpm_cnn* h, const float* in) {
     });
 }
 
-int fpm_cnn_forward(fpm_cnn* h, const float* feats, int64_t batch, float* out) {
+int fpm_cnn_forward(fpm_cnn* h, const float* feats, int64_t batch, float* out, float* pupil_out) {
     return guarded("fpm_cnn_forward", [&] {
         Net* net = (Net*)h;
         if (!net || !feats || !out) throw fpm::HipError{hipErrorInvalidValue, "null argument"};
         if (batch < 1 || batch > net->d.max_batch) throw fpm::HipError{hipErrorInvalidValue, "batch outside [1, max_batch]"};
         const int B = (int)batch;
+        if (pupil_out && net->K == 0)
+            throw fpm::HipError{hipErrorInvalidValue, "pupil_out given to a network without a pupil head"};
         load_features(*net, feats, B);
-        forward(*net, B);
+        forward(*net, B, pupil_out);
         head_shuffle(net->head_f.as<float>(), out, B, net->d.n, net->d.n, net->d.up, net->st);
         FPM_HIP(hipStreamSynchronize(net->st));
     });
 }
 
-int fpm_cnn_gradient(fpm_cnn* h, const float* feats, const float* targets, int64_t batch, double* loss_out,
-                     float* grad_out) {
+int fpm_cnn_gradient(fpm_cnn* h, const float* feats, const float* targets, const float* pupil_targets, int64_t batch,
+                     double* loss_out, float* grad_out) {
     return guarded("fpm_cnn_gradient", [&] {
         Net* net = (Net*)h;
-        if (!net || !feats || !targets) throw fpm::HipError{hipErrorInvalidValue, "null argument"};
-        if (batch < 1 || batch > net->d.max_batch) throw fpm::HipError{hipErrorInvalidValue, "batch outside [1, max_batch]"};
-        const double loss = loss_and_grad(*net, feats, targets, (int)batch);
+        check_call(net, feats, targets, pupil_targets, batch);
+        const double loss = loss_and_grad(*net, feats, targets, pupil_targets, (int)batch);
         if (loss_out) *loss_out = loss;
         if (grad_out) {
             FPM_HIP(hipMemcpyAsync(grad_out, net->grads.p, net->nparam * 4, hipMemcpyDeviceToHost, net->st));
@@ -420,13 +488,12 @@ int fpm_cnn_gradient(fpm_cnn* h, const float* feats, const float* targets, int64
     });
 }
 
-int fpm_cnn_train_step(fpm_cnn* h, const float* feats, const float* targets, int64_t batch, float lr,
-                       double* loss_out) {
+int fpm_cnn_train_step(fpm_cnn* h, const float* feats, const float* targets, const float* pupil_targets,
+                       int64_t batch, float lr, double* loss_out) {
     return guarded("fpm_cnn_train_step", [&] {
         Net* net = (Net*)h;
-        if (!net || !feats || !targets) throw fpm::HipError{hipErrorInvalidValue, "null argument"};
-        if (batch < 1 || batch > net->d.max_batch) throw fpm::HipError{hipErrorInvalidValue, "batch outside [1, max_batch]"};
-        const double loss = loss_and_grad(*net, feats, targets, (int)batch);
+        check_call(net, feats, targets, pupil_targets, batch);
+        const double loss = loss_and_grad(*net, feats, targets, pupil_targets, (int)batch);
         net->step += 1;
         const float b1 = 0.9f, b2 = 0.999f;
         AdamArgs a{lr, b1, b2, 1e-8f, (float)(1.0 / (1.0 - std::pow((double)b1, (double)net->step))),

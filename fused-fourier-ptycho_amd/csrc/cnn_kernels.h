@@ -72,6 +72,26 @@ void head_shuffle(const float* yf, float* out, int B, int H, int W, int u, hipSt
 void head_mse(const float* yf, const float* target, float* gyf, double* loss_acc, int B, int H, int W, int u,
               double inv_count, hipStream_t st);
 
+// Pupil head (FP32 throughout): per sample b, pooled(c) = mean over the HW
+// pixels of e (B x HW x C, BF16, C <= 256), hid = ReLU(W1 pooled + b1)
+// (W1: kPupilHidden x C), coef = W2 hid + b2 (W2: K x kPupilHidden, K <= 16).
+// ws (B x (C + kPupilHidden + K)) keeps pooled, hid and coef for the
+// backward; coef_out (B x K) may be null.
+constexpr int kPupilHidden = 64;
+constexpr int kPupilMaxTerms = 16;
+constexpr int kPupilMaxBatch = 64;
+void pupil_head_fwd(const bf16_t* e, int B, int HW, int C, const float* W1, const float* b1, const float* W2,
+                    const float* b2, int K, float* ws, float* coef_out, hipStream_t st);
+// Backward from target (B x K): with d = coef - target,
+// loss_acc (FP64) += lam sum d^2 / (B K); g_coef = 2 lam d / (B K);
+// gW2 = g_coef^T hid, gb2 = sum_b g_coef, g_hid = (g_coef W2) [hid > 0],
+// gW1 = g_hid^T pooled, gb1 = sum_b g_hid (all written, not accumulated);
+// ge (B x HW x C, FP32) += (g_hid W1) / HW at every pixel.  B <= kPupilMaxBatch.
+// gpool (B x C) is scratch.
+void pupil_head_bwd(const float* ws, const float* target, int B, int HW, int C, const float* W1, const float* W2,
+                    int K, float lam, float* gW1, float* gb1, float* gW2, float* gb2, float* gpool, float* ge,
+                    double* loss_acc, hipStream_t st);
+
 // Adam (no weight decay): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
 // p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps); w16 = bf16(p)
 // for the first n16 entries of each layer segment (nullptr: none).

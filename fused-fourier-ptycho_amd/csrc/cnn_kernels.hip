@@ -453,6 +453,137 @@ void head_mse(const float* yf, const float* target, float* gyf, double* loss_acc
     FPM_LAUNCH_CHECK();
 }
 
+// --------------------------------------------------------- pupil head
+// one workgroup per sample; thread c sums channel c over the pixels
+// (consecutive threads read consecutive channels of a pixel)
+__global__ __launch_bounds__(256) void pupil_head_fwd_kernel(const bf16_t* __restrict__ e, int HW, int C,
+                                                             const float* __restrict__ W1, const float* __restrict__ b1,
+                                                             const float* __restrict__ W2, const float* __restrict__ b2,
+                                                             int K, float* __restrict__ ws,
+                                                             float* __restrict__ coef_out) {
+    __shared__ float pooled[256];
+    __shared__ float hid[kPupilHidden];
+    const int t = threadIdx.x, b = blockIdx.x;
+    const int stride = C + kPupilHidden + K;
+    float* w = ws + (size_t)b * stride;
+    if (t < C) {
+        const bf16_t* eb = e + (size_t)b * HW * C + t;
+        float s = 0.0f;
+        for (int p = 0; p < HW; ++p) s += bf2f(eb[(size_t)p * C]);
+        pooled[t] = s / (float)HW;
+        w[t] = pooled[t];
+    }
+    __syncthreads();
+    if (t < kPupilHidden) {
+        float s = b1[t];
+        for (int c = 0; c < C; ++c) s = fmaf(W1[t * C + c], pooled[c], s);
+        hid[t] = fmaxf(s, 0.0f);
+        w[C + t] = hid[t];
+    }
+    __syncthreads();
+    if (t < K) {
+        float s = b2[t];
+        for (int j = 0; j < kPupilHidden; ++j) s = fmaf(W2[t * kPupilHidden + j], hid[j], s);
+        w[C + kPupilHidden + t] = s;
+        if (coef_out) coef_out[b * K + t] = s;
+    }
+}
+
+// one workgroup for the whole batch: every parameter gradient is a sum over
+// the B samples in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void pupil_head_bwd_kernel(const float* __restrict__ ws,
+                                                             const float* __restrict__ target, int B, int HW, int C,
+                                                             const float* __restrict__ W1, const float* __restrict__ W2,
+                                                             int K, float lam, float* __restrict__ gW1,
+                                                             float* __restrict__ gb1, float* __restrict__ gW2,
+                                                             float* __restrict__ gb2, float* __restrict__ gpool,
+                                                             double* __restrict__ loss_acc) {
+    __shared__ float gc[kPupilMaxBatch * kPupilMaxTerms];
+    __shared__ float gh[kPupilMaxBatch * kPupilHidden];
+    const int t = threadIdx.x;
+    const int stride = C + kPupilHidden + K;
+    const float inv = 1.0f / (float)(B * K);
+    for (int i = t; i < B * K; i += 256) {
+        const int b = i / K, k = i - b * K;
+        gc[i] = ws[(size_t)b * stride + C + kPupilHidden + k] - target[i];  // d, scaled below
+    }
+    __syncthreads();
+    if (t == 0 && loss_acc) {
+        double s = 0.0;
+        for (int i = 0; i < B * K; ++i) s += (double)gc[i] * (double)gc[i];
+        *loss_acc += (double)lam * s / (double)(B * K);
+    }
+    __syncthreads();
+    for (int i = t; i < B * K; i += 256) gc[i] *= 2.0f * lam * inv;
+    __syncthreads();
+    for (int i = t; i < B * kPupilHidden; i += 256) {
+        const int b = i / kPupilHidden, j = i - b * kPupilHidden;
+        float s = 0.0f;
+        if (ws[(size_t)b * stride + C + j] > 0.0f)
+            for (int k = 0; k < K; ++k) s = fmaf(gc[b * K + k], W2[k * kPupilHidden + j], s);
+        gh[i] = s;
+    }
+    __syncthreads();
+    for (int i = t; i < K * kPupilHidden; i += 256) {
+        const int k = i / kPupilHidden, j = i - k * kPupilHidden;
+        float s = 0.0f;
+        for (int b = 0; b < B; ++b) s = fmaf(gc[b * K + k], ws[(size_t)b * stride + C + j], s);
+        gW2[i] = s;
+    }
+    for (int k = t; k < K; k += 256) {
+        float s = 0.0f;
+        for (int b = 0; b < B; ++b) s += gc[b * K + k];
+        gb2[k] = s;
+    }
+    for (int i = t; i < kPupilHidden * C; i += 256) {
+        const int j = i / C, c = i - j * C;
+        float s = 0.0f;
+        for (int b = 0; b < B; ++b) s = fmaf(gh[b * kPupilHidden + j], ws[(size_t)b * stride + c], s);
+        gW1[i] = s;
+    }
+    for (int j = t; j < kPupilHidden; j += 256) {
+        float s = 0.0f;
+        for (int b = 0; b < B; ++b) s += gh[b * kPupilHidden + j];
+        gb1[j] = s;
+    }
+    for (int i = t; i < B * C; i += 256) {
+        const int b = i / C, c = i - b * C;
+        float s = 0.0f;
+        for (int j = 0; j < kPupilHidden; ++j) s = fmaf(gh[b * kPupilHidden + j], W1[j * C + c], s);
+        gpool[i] = s / (float)HW;
+    }
+}
+
+// ge(b, p, c) += gpool(b, c)
+__global__ void pupil_pool_bwd_kernel(const float* __restrict__ gpool, float* __restrict__ ge, int B, int HW, int C) {
+    const uint32_t total = (uint32_t)B * (uint32_t)HW * (uint32_t)C;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < total; i += gridDim.x * 256u) {
+        const uint32_t c = i % (uint32_t)C, b = i / ((uint32_t)HW * (uint32_t)C);
+        ge[i] += gpool[b * (uint32_t)C + c];
+    }
+}
+
+void pupil_head_fwd(const bf16_t* e, int B, int HW, int C, const float* W1, const float* b1, const float* W2,
+                    const float* b2, int K, float* ws, float* coef_out, hipStream_t st) {
+    if (C > 256 || K < 1 || K > kPupilMaxTerms || B < 1 || HW < 1)
+        throw HipError{hipErrorInvalidValue, "pupil_head_fwd: C <= 256, 1 <= K <= 16, B, HW >= 1"};
+    hipLaunchKernelGGL(pupil_head_fwd_kernel, dim3(B), dim3(256), 0, st, e, HW, C, W1, b1, W2, b2, K, ws, coef_out);
+    FPM_LAUNCH_CHECK();
+}
+
+void pupil_head_bwd(const float* ws, const float* target, int B, int HW, int C, const float* W1, const float* W2,
+                    int K, float lam, float* gW1, float* gb1, float* gW2, float* gb2, float* gpool, float* ge,
+                    double* loss_acc, hipStream_t st) {
+    if (C > 256 || K < 1 || K > kPupilMaxTerms || B < 1 || B > kPupilMaxBatch || HW < 1)
+        throw HipError{hipErrorInvalidValue, "pupil_head_bwd: C <= 256, 1 <= K <= 16, 1 <= B <= 64, HW >= 1"};
+    hipLaunchKernelGGL(pupil_head_bwd_kernel, dim3(1), dim3(256), 0, st, ws, target, B, HW, C, W1, W2, K, lam, gW1,
+                       gb1, gW2, gb2, gpool, loss_acc);
+    FPM_LAUNCH_CHECK();
+    const int64_t total = (int64_t)B * HW * C;
+    hipLaunchKernelGGL(pupil_pool_bwd_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, st, gpool, ge, B, HW, C);
+    FPM_LAUNCH_CHECK();
+}
+
 // -------------------------------------------------------------- Adam
 __global__ void adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
                             const float* __restrict__ g, int64_t n, AdamArgs a) {

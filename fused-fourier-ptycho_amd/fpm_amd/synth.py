@@ -9,7 +9,9 @@ on:
   at 4 mm pitch, 90 mm high, centre-out spiral order; LED windows from
   :func:`fpm_amd.led_offsets` (FP64, init_environment_rgb.m:33-134);
 * pupil mask A: the NA disk on the low-res frequency grid
-  (init_environment_rgb.m:33-40); true pupil A exp(i 0.5 sqrt3 (2 rho^2 - 1));
+  (init_environment_rgb.m:33-40); true pupil A exp(i 0.5 sqrt3 (2 rho^2 - 1))
+  (0.5 rad of defocus), or A exp(i Z) for given Zernike coefficients
+  (:func:`zernike_pupil`);
 * object (N x N): amplitude = uniform noise low-passed by a disk at
   NA 0.1 + max illumination NA, scaled to [0.3, 1]; phase = sum of 20
   Gaussians (centres uniform, sigma in [N/64, N/16], heights in [-pi/2, pi/2]);
@@ -73,6 +75,30 @@ def pupil(n: int, defocus_rad: float = 0.5):
     P = A * np.exp(1j * defocus_rad * np.sqrt(3.0) * (2 * rho2 - 1))
     # the solve's pupil is unshifted-centred like the reference's (centre = n/2+1)
     return np.asfortranarray(A), np.asfortranarray(P.astype(np.complex64))
+
+
+ZERNIKE_TERMS = ("defocus", "astigmatism 0", "astigmatism 45", "coma x", "coma y")
+
+
+def zernike_basis(n: int) -> np.ndarray:
+    """(5, n, n) float64 phase maps (Noll-normalised over the NA disk, rho =
+    |k| / k_max; the pupil mask A cuts them off), in :data:`ZERNIKE_TERMS`
+    order: sqrt3 (2 rho^2 - 1),
+    sqrt6 rho^2 cos 2t, sqrt6 rho^2 sin 2t, sqrt8 (3 rho^3 - 2 rho) cos t,
+    sqrt8 (3 rho^3 - 2 rho) sin t, on the grid of :func:`freq_grid`."""
+    Fx, Fy = freq_grid(n)
+    kmax = NA * 2 * np.pi / WAVELENGTH_UM
+    rho = np.sqrt(Fx * Fx + Fy * Fy) / kmax
+    th = np.arctan2(Fy, Fx)
+    return np.stack([np.sqrt(3) * (2 * rho ** 2 - 1), np.sqrt(6) * rho ** 2 * np.cos(2 * th),
+                  np.sqrt(6) * rho ** 2 * np.sin(2 * th), np.sqrt(8) * (3 * rho ** 3 - 2 * rho) * np.cos(th),
+                  np.sqrt(8) * (3 * rho ** 3 - 2 * rho) * np.sin(th)])
+
+
+def zernike_pupil(n: int, coeffs, A: np.ndarray) -> np.ndarray:
+    """A exp(i sum_k c_k Z_k) (n x n complex64, column-major like :func:`pupil`)."""
+    Z = np.tensordot(np.asarray(coeffs, dtype=np.float64), zernike_basis(n), axes=1)
+    return np.asfortranarray((A * np.exp(1j * Z)).astype(np.complex64))
 
 
 def bilinear_upsample(img: np.ndarray, r: int) -> np.ndarray:
@@ -149,13 +175,15 @@ def fortran_empty(shape, dtype, device):
 
 
 def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda", led_grid: int = LED_GRID,
-                defocus_rad: float = 0.5, keep_object: bool = False):
+                defocus_rad: float = 0.5, keep_object: bool = False, pupil_coeffs=None):
     """The same generator on the GPU for large workloads (torch tensors in
     column-major layout, transforms and the image model through libfpm_amd).
     Returns a dict of device tensors + host LED/mask arrays.  keep_object
     adds the true object in the solve's scale (O_true_dev = kappa O, where
     kappa maps the object to the prepared amplitudes sqrt((I - min) /
-    (max - min))) and its phase field (phase_true_dev)."""
+    (max - min))) and its phase field (phase_true_dev).  pupil_coeffs (5
+    Zernike coefficients, rad) replaces the defocus-only true pupil; the
+    dict's "P_true_dev" holds the true pupil in the device layout."""
     import torch
     import torch.nn.functional as F
 
@@ -168,6 +196,8 @@ def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda",
                               led_pitch_um=LED_PITCH_UM, led_height_um=LED_HEIGHT_UM, led_grid=led_grid)
     L = led.shape[1]
     A, P_true = pupil(n, defocus_rad)
+    if pupil_coeffs is not None:
+        P_true = zernike_pupil(n, pupil_coeffs, A)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
     # amplitude: band-limited noise (the filter is even, so the layout
@@ -220,7 +250,8 @@ def make_device(n: int, pratio: int, seed: int = 20261203, device: str = "cuda",
     P0 = torch.from_numpy(np.ascontiguousarray(A.T.astype(np.complex64))).to(dev)
     Ad = torch.from_numpy(np.ascontiguousarray(A.T)).to(dev)
     out = {"n": n, "N": N, "L": L, "pratio": pratio, "led": led, "led_dev": ledd, "A": A, "A_dev": Ad,
-           "raw_dev": raw, "O0_dev": O0, "P0_dev": P0, "na_illum": nai, "raw_min": mn, "raw_max": mx}
+           "raw_dev": raw, "O0_dev": O0, "P0_dev": P0, "P_true_dev": Pt, "na_illum": nai, "raw_min": mn,
+           "raw_max": mx}
     if keep_object:
         kappa = float(np.sqrt(scale / float(mx - mn)))
         out["O_true_dev"] = O_true * kappa

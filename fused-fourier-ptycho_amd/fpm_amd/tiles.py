@@ -104,23 +104,6 @@ def tiles_solve(tiles: Sequence[dict], pratio: int, epochs: int, batch: int, opt
 
 
 # --------------------------------------------------------- generator
-def _zernike_pupil(n: int, coeffs, A: np.ndarray) -> np.ndarray:
-    """A exp(i Z), Z = c_def sqrt3 (2 rho^2 - 1) + c_a0 sqrt6 rho^2 cos 2t
-    + c_a45 sqrt6 rho^2 sin 2t + c_cx sqrt8 (3 rho^3 - 2 rho) cos t
-    + c_cy sqrt8 (3 rho^3 - 2 rho) sin t over the NA disk (Noll-normalised)."""
-    from . import synth
-
-    Fx, Fy = synth.freq_grid(n)
-    kmax = synth.NA * 2 * np.pi / synth.WAVELENGTH_UM
-    rho = np.sqrt(Fx * Fx + Fy * Fy) / kmax
-    th = np.arctan2(Fy, Fx)
-    d, a0, a45, cx, cy = coeffs
-    Z = (d * np.sqrt(3) * (2 * rho ** 2 - 1) + a0 * np.sqrt(6) * rho ** 2 * np.cos(2 * th)
-         + a45 * np.sqrt(6) * rho ** 2 * np.sin(2 * th)
-         + cx * np.sqrt(8) * (3 * rho ** 3 - 2 * rho) * np.cos(th) + cy * np.sqrt(8) * (3 * rho ** 3 - 2 * rho) * np.sin(th))
-    return np.asfortranarray((A * np.exp(1j * Z)).astype(np.complex64))
-
-
 def make_tile_set(field_n: int, sizes: Sequence[int], grid: int, pratio: int, *, seed: int = 7, sigma: float = 0.4,
                   led_grid: int = 19, count: int = 0, device: str = "cuda") -> List[dict]:
     """Tiles of a synthetic field of field_n x field_n low-res pixels
@@ -177,7 +160,7 @@ def make_tile_set(field_n: int, sizes: Sequence[int], grid: int, pratio: int, *,
         objdy = -(r0 + n / 2 - field_n / 2) * synth.PIXEL_UM
         led, _, _ = led_offsets_at(objdx, objdy, n, pratio, led_grid=led_grid)
         A, _ = synth.pupil(n)
-        P_true = _zernike_pupil(n, rng.normal(0.0, sigma, 5), A)
+        P_true = synth.zernike_pupil(n, rng.normal(0.0, sigma, 5), A)
         # device layout: row-major buffer of the column-major array (t[j, i] = M(i, j))
         Ot = O_field[c0 * pratio:c0 * pratio + N, r0 * pratio:r0 * pratio + N].contiguous()
         Pt = torch.from_numpy(np.ascontiguousarray(P_true.T)).to(dev)

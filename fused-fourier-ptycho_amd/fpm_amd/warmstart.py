@@ -5,9 +5,12 @@ upsampling of the centre image (codes_v2/main_run_FPM.m:48).  Here a U-Net
 (libfpm_amd's fpm_cnn_*: BF16 matrix-core kernels, FP32 accumulation, trained
 with the library's own forward and backward kernels) maps low-resolution
 features of the image stack to the object's amplitude and phase at the
-solve's resolution; the plain RMSprop solve then starts from that object and
-finishes the reconstruction, so it both judges and bounds the network's
-contribution: the solve is the same whatever the network predicts.
+solve's resolution and, from the mean of its bottom level's features, to the
+five low-order Zernike coefficients of the pupil's phase (defocus,
+astigmatism 0 / 45, coma x / y); the plain RMSprop solve then starts from that
+object and pupil and finishes the reconstruction, so it both judges and
+bounds the network's contribution: the solve is the same whatever the
+network predicts.
 
 Features (26 planes, n x n): the prepared amplitudes of the 25 LEDs of the
 central 5 x 5 block (the first 25 of the centre-out spiral), and the mean
@@ -15,10 +18,12 @@ darkfield intensity (LEDs whose illumination NA exceeds the objective NA)
 scaled by 16.  The network predicts a correction to the reference's own init:
 plane 0 is added to the bilinear upsampling of the centre amplitude, plane 1
 is the phase (the reference's init has none), so a network that outputs zero
-reproduces the plain method exactly.  Training targets are the amplitude
-residual and the phase of the true object in the solve's scale (the factor
-kappa that maps the object to the prepared amplitudes
-Y = sqrt((I - min) / (max - min))).
+reproduces the plain method exactly (zero coefficients give the reference's
+ideal pupil).  Training targets are the amplitude residual and the phase of
+the true object in the solve's scale (the factor kappa that maps the object
+to the prepared amplitudes Y = sqrt((I - min) / (max - min))) and the true
+pupil's coefficients; training objects are imaged through pupils whose
+coefficients are drawn from N(0, sigma).
 
 All planes use the device layout of fpm_amd.synth (the row-major buffer of a
 column-major MATLAB array), so features, targets, network outputs and the
@@ -35,11 +40,14 @@ from . import FPMError, _check, lib
 
 N_BRIGHT = 25
 DARK_SCALE = 16.0
+PUPIL_TERMS = 5  # fpm_amd.synth.ZERNIKE_TERMS
+PUPIL_SIGMA = 0.4  # rad, spread of the training pupils' coefficients
 
 
 class CnnDesc(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("base_channels", ctypes.c_int32), ("in_channels", ctypes.c_int32),
-                ("n", ctypes.c_int32), ("up", ctypes.c_int32), ("max_batch", ctypes.c_int32)]
+                ("n", ctypes.c_int32), ("up", ctypes.c_int32), ("max_batch", ctypes.c_int32),
+                ("pupil_terms", ctypes.c_int32), ("pupil_weight", ctypes.c_float)]
 
 
 _bound = False
@@ -60,11 +68,12 @@ def _cnn_lib():
         L.fpm_cnn_get_params.restype = ctypes.c_int
         L.fpm_cnn_set_params.argtypes = [vp, vp]
         L.fpm_cnn_set_params.restype = ctypes.c_int
-        L.fpm_cnn_forward.argtypes = [vp, vp, ctypes.c_int64, vp]
+        L.fpm_cnn_forward.argtypes = [vp, vp, ctypes.c_int64, vp, vp]
         L.fpm_cnn_forward.restype = ctypes.c_int
-        L.fpm_cnn_gradient.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), vp]
+        L.fpm_cnn_gradient.argtypes = [vp, vp, vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), vp]
         L.fpm_cnn_gradient.restype = ctypes.c_int
-        L.fpm_cnn_train_step.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+        L.fpm_cnn_train_step.argtypes = [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_float,
+                                         ctypes.POINTER(ctypes.c_double)]
         L.fpm_cnn_train_step.restype = ctypes.c_int
         _bound = True
     return L
@@ -80,8 +89,8 @@ class WarmStartNet:
     """Handle of one network (parameters, Adam state and workspaces on the GPU)."""
 
     def __init__(self, n: int, up: int, *, levels: int = 4, base: int = 32, in_channels: int = N_BRIGHT + 1,
-                 max_batch: int = 8, seed: int = 0):
-        self.desc = CnnDesc(levels, base, in_channels, n, up, max_batch)
+                 max_batch: int = 8, seed: int = 0, pupil_terms: int = 0, pupil_weight: float = 0.1):
+        self.desc = CnnDesc(levels, base, in_channels, n, up, max_batch, pupil_terms, pupil_weight)
         h = ctypes.c_void_p()
         _check(_cnn_lib().fpm_cnn_create(ctypes.byref(self.desc), int(seed), ctypes.byref(h)))
         self._h = h
@@ -109,45 +118,56 @@ class WarmStartNet:
             raise FPMError(-1, f"expected {self.nparam} parameters, got {p.size}")
         _check(_cnn_lib().fpm_cnn_set_params(self._h, p.ctypes.data))
 
-    def forward(self, feats):
-        """feats: CUDA float32 (B, in_channels, n, n) -> (B, 2, up n, up n)."""
+    def forward(self, feats, pupil: bool = False):
+        """feats: CUDA float32 (B, in_channels, n, n) -> (B, 2, up n, up n);
+        with pupil=True also the (B, pupil_terms) coefficients."""
         import torch
 
         feats = feats.contiguous()
         B = feats.shape[0]
         N = self.desc.up * self.desc.n
         out = torch.empty((B, 2, N, N), dtype=torch.float32, device=feats.device)
+        coef = torch.empty((B, self.desc.pupil_terms), dtype=torch.float32, device=feats.device) if pupil else None
         _sync()
-        _check(_cnn_lib().fpm_cnn_forward(self._h, feats.data_ptr(), B, out.data_ptr()))
-        return out
+        _check(_cnn_lib().fpm_cnn_forward(self._h, feats.data_ptr(), B, out.data_ptr(),
+                                          coef.data_ptr() if pupil else None))
+        return (out, coef) if pupil else out
 
-    def gradient(self, feats, targets):
+    def _pt(self, pupil_targets):
+        return pupil_targets.contiguous().data_ptr() if pupil_targets is not None else None
+
+    def gradient(self, feats, targets, pupil_targets=None):
         feats, targets = feats.contiguous(), targets.contiguous()
+        pt = pupil_targets.contiguous() if pupil_targets is not None else None
         g = np.empty(self.nparam, dtype=np.float32)
         loss = ctypes.c_double()
         _sync()
-        _check(_cnn_lib().fpm_cnn_gradient(self._h, feats.data_ptr(), targets.data_ptr(), feats.shape[0],
-                                           ctypes.byref(loss), g.ctypes.data))
+        _check(_cnn_lib().fpm_cnn_gradient(self._h, feats.data_ptr(), targets.data_ptr(), self._pt(pt),
+                                           feats.shape[0], ctypes.byref(loss), g.ctypes.data))
         return loss.value, g
 
-    def train_step(self, feats, targets, lr: float) -> float:
+    def train_step(self, feats, targets, lr: float, pupil_targets=None) -> float:
         feats, targets = feats.contiguous(), targets.contiguous()
+        pt = pupil_targets.contiguous() if pupil_targets is not None else None
         loss = ctypes.c_double()
         _sync()
-        _check(_cnn_lib().fpm_cnn_train_step(self._h, feats.data_ptr(), targets.data_ptr(), feats.shape[0],
-                                             float(lr), ctypes.byref(loss)))
+        _check(_cnn_lib().fpm_cnn_train_step(self._h, feats.data_ptr(), targets.data_ptr(), self._pt(pt),
+                                             feats.shape[0], float(lr), ctypes.byref(loss)))
         return loss.value
 
     def save(self, path: str) -> None:
         d = self.desc
         np.savez(path, params=self.get_params(),
-                 desc=np.array([d.levels, d.base_channels, d.in_channels, d.n, d.up, d.max_batch], np.int64))
+                 desc=np.array([d.levels, d.base_channels, d.in_channels, d.n, d.up, d.max_batch, d.pupil_terms],
+                               np.int64),
+                 pupil_weight=np.array([d.pupil_weight], np.float64))
 
     @classmethod
     def load(cls, path: str, max_batch: Optional[int] = None) -> "WarmStartNet":
         z = np.load(path)  # plain arrays only (allow_pickle stays False)
-        lv, base, cin, n, up, mb = (int(v) for v in z["desc"])
-        net = cls(n, up, levels=lv, base=base, in_channels=cin, max_batch=max_batch or mb)
+        lv, base, cin, n, up, mb, pk = (int(v) for v in z["desc"])
+        net = cls(n, up, levels=lv, base=base, in_channels=cin, max_batch=max_batch or mb, pupil_terms=pk,
+                  pupil_weight=float(z["pupil_weight"][0]))
         net.set_params(z["params"])
         return net
 
@@ -177,24 +197,70 @@ def warm_start_object(net: WarmStartNet, feats, base_amp=None):
     """Network output -> complex object O0 = (base + amp) exp(i phase) (N, N)
     in the solve's device layout (CUDA complex64); base_amp defaults to the
     plain init of the centre amplitude feature."""
-    import torch
-
-    if base_amp is None:
-        base_amp = plain_init_amplitude(feats[0], net.desc.up)
-    out = net.forward(feats[None])[0]
-    amp = torch.clamp(base_amp + out[0], min=0.0)
-    return torch.polar(amp, out[1]).contiguous()
+    return warm_start(net, feats, base_amp)[0]
 
 
-# ------------------------------------------------------------- training
-def make_sample(n: int, pratio: int, seed: int, device: str = "cuda", led_grid: int = 19):
-    """One synthetic object through the forward model: (features (26, n, n),
-    target (2, N, N), dataset dict of synth.make_device)."""
+_basis_cache = {}
+
+
+def pupil_from_coeffs(coef, A_dev):
+    """A exp(i sum_k c_k Z_k) (n, n) in the device layout (CUDA complex64)
+    from the coefficients (CUDA (K,)) and the device pupil mask."""
     import torch
 
     from . import synth
 
-    ds = synth.make_device(n, pratio, seed=seed, device=device, led_grid=led_grid, keep_object=True)
+    n = A_dev.shape[0]
+    key = (n, A_dev.device)
+    if key not in _basis_cache:
+        # device layout: the transpose of the column-major (MATLAB-oriented) maps
+        Z = synth.zernike_basis(n).transpose(0, 2, 1)
+        _basis_cache[key] = torch.from_numpy(np.ascontiguousarray(Z, dtype=np.float32)).to(A_dev.device)
+    Z = _basis_cache[key][:coef.shape[0]]
+    phase = torch.tensordot(coef.to(torch.float32), Z, dims=1)
+    return torch.polar(A_dev.to(torch.float32), phase).contiguous()
+
+
+def warm_start(net: WarmStartNet, feats, base_amp=None, A_dev=None):
+    """(O0, P0): the object of warm_start_object and, when the network has a
+    pupil head and the pupil mask A_dev is given, the pupil built from the
+    predicted coefficients (else None: the caller keeps the plain P0 = A)."""
+    import torch
+
+    if base_amp is None:
+        base_amp = plain_init_amplitude(feats[0], net.desc.up)
+    want_p = net.desc.pupil_terms > 0 and A_dev is not None
+    res = net.forward(feats[None], pupil=want_p)
+    out, coef = res if want_p else (res, None)
+    amp = torch.clamp(base_amp + out[0][0], min=0.0)
+    O0 = torch.polar(amp, out[0][1]).contiguous()
+    return O0, (pupil_from_coeffs(coef[0], A_dev) if want_p else None)
+
+
+# ------------------------------------------------------------- training
+def sample_pupil_coeffs(seed: int, sigma: Optional[float]):
+    """The true pupil's Zernike coefficients of sample `seed`: drawn from
+    N(0, sigma) (own stream of the seed), or, with sigma None, the
+    generator's fixed pupil (0.5 rad of defocus)."""
+    if sigma is None:
+        return np.array([0.5, 0.0, 0.0, 0.0, 0.0])
+    return np.random.Generator(np.random.Philox(key=[seed, 0x5EED])).normal(0.0, sigma, PUPIL_TERMS)
+
+
+def make_sample(n: int, pratio: int, seed: int, device: str = "cuda", led_grid: int = 19,
+                pupil_sigma: Optional[float] = None):
+    """One synthetic object through the forward model: (features (26, n, n),
+    target (2, N, N), dataset dict of synth.make_device); the dict's
+    "pupil_coeffs" holds the true pupil's coefficients
+    (sample_pupil_coeffs(seed, pupil_sigma))."""
+    import torch
+
+    from . import synth
+
+    coeffs = sample_pupil_coeffs(seed, pupil_sigma)
+    ds = synth.make_device(n, pratio, seed=seed, device=device, led_grid=led_grid, keep_object=True,
+                           pupil_coeffs=coeffs)
+    ds["pupil_coeffs"] = coeffs
     raw = ds["raw_dev"]
     amp = torch.sqrt((raw.to(torch.float32) - ds["raw_min"]) / float(ds["raw_max"] - ds["raw_min"]))
     feats = features(amp, ds["na_illum"])
@@ -204,25 +270,29 @@ def make_sample(n: int, pratio: int, seed: int, device: str = "cuda", led_grid: 
 
 
 def make_training_set(count: int, n: int, pratio: int, seed0: int = 1000, device: str = "cuda",
-                      led_grid: int = 19, log=None):
-    """count samples (features, targets) stacked on the device."""
+                      led_grid: int = 19, log=None, pupil_sigma: Optional[float] = PUPIL_SIGMA):
+    """count samples stacked on the device: features F, targets T and the
+    true pupils' coefficients Z (count, 5)."""
     import torch
 
     F = torch.empty((count, N_BRIGHT + 1, n, n), dtype=torch.float32, device=device)
     T = torch.empty((count, 2, n * pratio, n * pratio), dtype=torch.float32, device=device)
+    Z = torch.empty((count, PUPIL_TERMS), dtype=torch.float32, device=device)
     for k in range(count):
-        f, t, _ = make_sample(n, pratio, seed0 + k, device, led_grid)
+        f, t, ds = make_sample(n, pratio, seed0 + k, device, led_grid, pupil_sigma)
         F[k] = f
         T[k] = t
+        Z[k] = torch.from_numpy(ds["pupil_coeffs"].astype(np.float32))
         if log and (k + 1) % max(1, count // 8) == 0:
             log(f"[warmstart] generated {k + 1}/{count} samples")
-    return F, T
+    return F, T, Z
 
 
-def train(net: WarmStartNet, F, T, *, epochs: int, batch: int, lr: float = 1e-3, lr_final: Optional[float] = None,
-          seed: int = 0, log=None):
-    """Adam on minibatches of (F, T), learning rate cosine-annealed from lr
-    to lr_final (default lr / 10); returns the mean loss of each epoch."""
+def train(net: WarmStartNet, F, T, Z=None, *, epochs: int, batch: int, lr: float = 1e-3,
+          lr_final: Optional[float] = None, seed: int = 0, log=None):
+    """Adam on minibatches of (F, T) (and the coefficient targets Z for a
+    network with a pupil head), learning rate cosine-annealed from lr to
+    lr_final (default lr / 10); returns the mean loss of each epoch."""
     import math
 
     import torch
@@ -240,7 +310,8 @@ def train(net: WarmStartNet, F, T, *, epochs: int, batch: int, lr: float = 1e-3,
         for k0 in range(0, count - batch + 1, batch):
             idx = order[k0:k0 + batch].to(F.device)
             lr_k = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / max(steps, 1)))
-            tot += net.train_step(F.index_select(0, idx), T.index_select(0, idx), lr_k)
+            zt = Z.index_select(0, idx) if net.desc.pupil_terms > 0 else None
+            tot += net.train_step(F.index_select(0, idx), T.index_select(0, idx), lr_k, zt)
             step += 1
             nb += 1
         hist.append(tot / max(nb, 1))
@@ -259,7 +330,7 @@ def phase_aligned_rel_l2(a, b) -> float:
     return float(np.linalg.norm(a * ph - b) / np.linalg.norm(b))
 
 
-def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0):
+def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0, P0=None):
     import time
 
     import torch
@@ -267,10 +338,11 @@ def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0):
     from . import solve_device
 
     n, N = ds["n"], ds["N"]
-    Od, Pd = torch.empty_like(O0), torch.empty_like(ds["P0_dev"])
+    P0 = ds["P0_dev"] if P0 is None else P0
+    Od, Pd = torch.empty_like(O0), torch.empty_like(P0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    loss = solve_device(O0.data_ptr(), ds["P0_dev"].data_ptr(), ds["raw_dev"].data_ptr(), True,
+    loss = solve_device(O0.data_ptr(), P0.data_ptr(), ds["raw_dev"].data_ptr(), True,
                         ds["led_dev"].data_ptr(), ds["A_dev"].data_ptr(), (N, N), (n, n), ds["L"], ds["pratio"],
                         epochs, batch, opt, Od.data_ptr(), Pd.data_ptr(), loss=True, engine=engine,
                         lr_halve_from=halve_from)
@@ -280,7 +352,8 @@ def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0):
 
 # (epochs, lr halving from epoch index) tried for the warm-started solve, shortest first;
 # the last entry is the reference's own schedule
-WARM_SCHEDULES = ((2, 1), (3, 1), (3, 2), (4, 2), (5, 3), (6, 3), (8, 4), (10, 6), (12, 8), (16, 11), (20, 13))
+WARM_SCHEDULES = ((1, 1), (2, 1), (3, 1), (3, 2), (4, 2), (5, 3), (6, 3), (8, 4), (10, 6), (12, 8), (16, 11),
+                  (20, 13))
 
 
 # a solve "reaches" the plain form's final loss when its final loss is within
@@ -289,16 +362,17 @@ LOSS_RTOL = 1e-3
 
 
 def tune_warm_schedule(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: int = 20, batch: int = 26,
-                       opt=(5e-4, 0.999, 1e-4), engine: int = 0, led_grid: int = 19):
+                       opt=(5e-4, 0.999, 1e-4), engine: int = 0, led_grid: int = 19,
+                       pupil_sigma: Optional[float] = PUPIL_SIGMA):
     """Shortest (epochs, lr_halve_from) of WARM_SCHEDULES with which the
     warm-started solve reaches the final loss of the plain 20-epoch solve
     (within LOSS_RTOL) on a validation object (not the object the time is
     then measured on)."""
-    feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid)
+    feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid, pupil_sigma=pupil_sigma)
     _, loss_plain, _, _ = _timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
-    O_net = warm_start_object(net, feats, ds["O0_dev"].real)
+    O_net, P_net = warm_start(net, feats, ds["O0_dev"].real, ds["A_dev"])
     for e, h in WARM_SCHEDULES:
-        _, loss_w, _, _ = _timed_solve(ds, O_net, e, batch, opt, engine, h)
+        _, loss_w, _, _ = _timed_solve(ds, O_net, e, batch, opt, engine, h, P_net)
         if loss_w[-1] <= loss_plain[-1] * (1.0 + LOSS_RTOL):
             return e, h
     return WARM_SCHEDULES[-1]
@@ -306,12 +380,14 @@ def tune_warm_schedule(net: WarmStartNet, n: int, pratio: int, seed: int, *, epo
 
 def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: int = 20, batch: int = 26,
                  opt=(5e-4, 0.999, 1e-4), engine: int = 0, warm_engine: Optional[int] = None, led_grid: int = 19,
-                 repeats: int = 3, schedule=None, keep_result: bool = False):
-    """Plain solve from the reference's init (upsampled centre image, the
-    reference's 20-epoch schedule) against network inference + the solve from
-    the network's object with `schedule` = (epochs, lr_halve_from) (from
-    tune_warm_schedule on another object; default: the reference's), on one
-    held-out synthetic object.  The plain form's final loss is the target.
+                 repeats: int = 3, schedule=None, keep_result: bool = False,
+                 pupil_sigma: Optional[float] = PUPIL_SIGMA):
+    """Plain solve from the reference's init (upsampled centre image, ideal
+    pupil, the reference's 20-epoch schedule) against network inference + the
+    solve from the network's object and pupil with `schedule` = (epochs,
+    lr_halve_from) (from tune_warm_schedule on another object; default: the
+    reference's), on one held-out synthetic object whose pupil is drawn like
+    the training pupils.  The plain form's final loss is the target.
     engine / warm_engine: FFT engine of the plain / warm-started solve.
     Times are the best of `repeats` runs.  keep_result adds "result": the
     warm solve's inputs and outputs as host arrays in MATLAB orientation
@@ -321,7 +397,7 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
 
     import torch
 
-    feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid)
+    feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid, pupil_sigma=pupil_sigma)
     O_true = ds["O_true_dev"].cpu().numpy()
     t_plain, loss_plain, O_plain, _ = min((_timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
                                            for _ in range(repeats)), key=lambda r: r[0])
@@ -329,17 +405,21 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
     for _ in range(repeats):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        O_net = warm_start_object(net, feats, ds["O0_dev"].real)
+        O_net, P_net = warm_start(net, feats, ds["O0_dev"].real, ds["A_dev"])
         torch.cuda.synchronize()
         t_inf = min(t_inf, time.perf_counter() - t0)
+    coef = None
+    if net.desc.pupil_terms > 0:
+        coef = net.forward(feats[None], pupil=True)[1][0].cpu().numpy().astype(np.float64)
     we, wh = schedule if schedule else (epochs, 13)
-    t_warm, loss_warm, O_warm, P_warm = min((_timed_solve(ds, O_net, we, batch, opt, warm_engine, wh)
+    t_warm, loss_warm, O_warm, P_warm = min((_timed_solve(ds, O_net, we, batch, opt, warm_engine, wh, P_net)
                                              for _ in range(repeats)), key=lambda r: r[0])
     target = float(loss_plain[-1])
     reached = bool(loss_warm[-1] <= target * (1.0 + LOSS_RTOL))
     ttl = t_inf + t_warm
     e_init = phase_aligned_rel_l2(O_net.cpu().numpy(), O_true)
     e_plain = phase_aligned_rel_l2(O_plain.cpu().numpy(), O_true)
+    P0w = ds["P0_dev"] if P_net is None else P_net
     out = {
         "plain_epochs": epochs, "plain_final_loss": target, "plain_s": t_plain,
         "warm_schedule": {"epochs": we, "lr_halve_from": wh},
@@ -350,6 +430,8 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
         "init_rel_l2_vs_truth": {"plain": phase_aligned_rel_l2(ds["O0_dev"].cpu().numpy(), O_true), "warm": e_init},
         "final_rel_l2_vs_truth": {"plain": e_plain,
                                   "warm": phase_aligned_rel_l2(O_warm.cpu().numpy(), O_true)},
+        "pupil_coeffs": {"true": [float(v) for v in ds["pupil_coeffs"]],
+                         "predicted": None if coef is None else [float(v) for v in coef]},
         # the network's object alone is already closer to the truth than the
         # plain solve's result: the warm form reaches the plain form's error
         # against the true object at inference time
@@ -360,6 +442,5 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
         # device layout is the transpose of the MATLAB orientation the oracle takes
         t = lambda x: np.asfortranarray(x.cpu().numpy().T)  # noqa: E731
         raw = np.asfortranarray(ds["raw_dev"].cpu().numpy().transpose(2, 1, 0))
-        out["result"] = (t(O_net), t(ds["P0_dev"]), raw, ds["led"], ds["A"], pratio, batch, opt, t(O_warm),
-                         t(P_warm))
+        out["result"] = (t(O_net), t(P0w), raw, ds["led"], ds["A"], pratio, batch, opt, t(O_warm), t(P_warm))
     return out

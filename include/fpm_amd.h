@@ -268,28 +268,37 @@ typedef struct {
     int32_t in_channels;   /* feature planes per sample */
     int32_t n;             /* feature side in pixels */
     int32_t up;            /* output side = up * n; 2 up^2 a multiple of 32 */
-    int32_t max_batch;     /* largest batch of one call (sizes the workspaces) */
+    int32_t max_batch;     /* largest batch of one call (sizes the workspaces; <= 64 with a pupil head) */
+    int32_t pupil_terms;   /* K coefficients of the pupil phase predicted from the mean of the bottom
+                              level's features (64 hidden, ReLU); 0: no pupil head; <= 16 */
+    float pupil_weight;    /* weight of the coefficients' mean squared error in the training loss */
 } fpm_cnn_desc;
 
 /* Network with He-normal weights from `seed`, zero biases, Adam state zero. */
 int fpm_cnn_create(const fpm_cnn_desc* desc, uint64_t seed, fpm_cnn** out);
 void fpm_cnn_destroy(fpm_cnn* net);
 /* Number of FP32 parameters: per conv (encoder levels, decoder levels, head)
- * W (Cout x 9 x Cin, Cin of the first conv rounded up to 32) then b (Cout). */
+ * W (Cout x 9 x Cin, Cin of the first conv rounded up to 32) then b (Cout);
+ * then, with a pupil head, W1 (64 x Cb), b1 (64), W2 (K x 64), b2 (K), Cb the
+ * bottom level's channel count. */
 int64_t fpm_cnn_num_params(const fpm_cnn* net);
 int fpm_cnn_get_params(fpm_cnn* net, float* host_out);
 /* Sets the parameters (host array) and clears the optimiser state. */
 int fpm_cnn_set_params(fpm_cnn* net, const float* host_in);
 /* feats: batch x in_channels x n x n; out: batch x 2 x (up n) x (up n)
- * (plane 0 amplitude, plane 1 phase) */
-int fpm_cnn_forward(fpm_cnn* net, const float* feats, int64_t batch, float* out);
-/* Mean squared error against targets (batch x 2 x (up n)^2) and its gradient
- * with respect to every parameter (host_grad may be NULL). */
-int fpm_cnn_gradient(fpm_cnn* net, const float* feats, const float* targets, int64_t batch,
-                     double* loss_out, float* host_grad);
-/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error. */
-int fpm_cnn_train_step(fpm_cnn* net, const float* feats, const float* targets, int64_t batch,
-                       float lr, double* loss_out);
+ * (plane 0 amplitude, plane 1 phase); pupil_out: batch x pupil_terms
+ * coefficients (NULL: not written; must be NULL without a pupil head) */
+int fpm_cnn_forward(fpm_cnn* net, const float* feats, int64_t batch, float* out, float* pupil_out);
+/* Mean squared error against targets (batch x 2 x (up n)^2), plus
+ * pupil_weight x the mean squared error of the coefficients against
+ * pupil_targets (batch x pupil_terms; required with a pupil head, NULL
+ * without), and its gradient with respect to every parameter (host_grad may
+ * be NULL). */
+int fpm_cnn_gradient(fpm_cnn* net, const float* feats, const float* targets, const float* pupil_targets,
+                     int64_t batch, double* loss_out, float* host_grad);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss. */
+int fpm_cnn_train_step(fpm_cnn* net, const float* feats, const float* targets, const float* pupil_targets,
+                       int64_t batch, float lr, double* loss_out);
 
 /* Thread-local text of the last error ("" if none). */
 const char* fpm_last_error(void);

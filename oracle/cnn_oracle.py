@@ -9,10 +9,14 @@ float64, exactly what fused-fourier-ptycho_amd/csrc/cnn_api.hip computes:
   decoder level l = L-2..0:  upsample 2x (nearest) -> conv3x3+ReLU (d_a);
                              s = d_a + e_l; conv3x3+ReLU (d_b)
   head: conv3x3 (no ReLU) -> depth-to-space by u -> (amplitude, phase) planes
-  loss: mean squared error over all output values
+  pupil head (K = pupil_terms > 0): mean of e_{L-1} over its pixels (Cb
+      values) -> W1 (64 x Cb) + b1, ReLU -> W2 (K x 64) + b2: K coefficients
+  loss: mean squared error over all output values, plus pupil_weight x the
+      mean squared error of the K coefficients
 
 Parameters use the library's vector layout: per conv W (Cout x 9 x Cin, tap
-= 3 (dy+1) + (dx+1), Cin of the first conv rounded up to 32) then b (Cout).
+= 3 (dy+1) + (dx+1), Cin of the first conv rounded up to 32) then b (Cout);
+then, with a pupil head, W1, b1, W2, b2 (FP32 in the library, no BF16).
 
 With ``bf16=True`` values are rounded to BF16 (round to nearest even) where
 the library stores BF16: the input features, every activation, the weights
@@ -125,14 +129,47 @@ def space_to_depth(o, u):
     return o.reshape(B, 2, H, u, W, u).transpose(0, 2, 4, 1, 3, 5).reshape(B, H, W, 2 * u * u)
 
 
+PUPIL_HIDDEN = 64
+
+
+def pupil_head_forward(e, W1, b1, W2, b2):
+    """e (B, h, w, Cb) -> (pooled (B, Cb), hidden (B, 64), coefficients (B, K))"""
+    pooled = e.mean(axis=(1, 2))
+    hid = np.maximum(pooled @ W1.T + b1, 0.0)
+    return pooled, hid, hid @ W2.T + b2
+
+
+def pupil_head_backward(g_coef, pooled, hid, W1, W2, hw):
+    """gradients (gW1, gb1, gW2, gb2) and the gradient of every pixel of e
+    (B, Cb): each of the hw pixels receives g_pooled / hw"""
+    gW2, gb2 = g_coef.T @ hid, g_coef.sum(axis=0)
+    g_hid = (g_coef @ W2) * (hid > 0)
+    gW1, gb1 = g_hid.T @ pooled, g_hid.sum(axis=0)
+    return gW1, gb1, gW2, gb2, (g_hid @ W1) / hw
+
+
 class Net:
     """Forward / backward of the network for a parameter vector."""
 
-    def __init__(self, levels, base, in_ch, up, bf16=True):
+    def __init__(self, levels, base, in_ch, up, bf16=True, pupil_terms=0, pupil_weight=1.0):
         self.L, self.C, self.in_ch, self.u, self.bf16 = levels, base, in_ch, up, bf16
         self.cin_pad = (in_ch + 31) // 32 * 32
         self.convs = layer_shapes(levels, base, in_ch, up)
-        self.nparam = sum(co * 9 * ci + co for ci, co, _ in self.convs)
+        self.K, self.lam = pupil_terms, pupil_weight
+        self.cb = base << (levels - 1)
+        self.nconv = sum(co * 9 * ci + co for ci, co, _ in self.convs)
+        self.npupil = (PUPIL_HIDDEN * self.cb + PUPIL_HIDDEN + self.K * PUPIL_HIDDEN + self.K) if self.K else 0
+        self.nparam = self.nconv + self.npupil
+
+    def pupil_params(self, params):
+        """(W1, b1, W2, b2) views of the pupil head's parameters"""
+        p = np.asarray(params, dtype=np.float64)[self.nconv:]
+        H, cb, K = PUPIL_HIDDEN, self.cb, self.K
+        W1 = p[:H * cb].reshape(H, cb)
+        b1 = p[H * cb:H * cb + H]
+        W2 = p[H * cb + H:H * cb + H + K * H].reshape(K, H)
+        b2 = p[H * cb + H + K * H:]
+        return W1, b1, W2, b2
 
     def _r(self, x):
         return round_bf16(x) if self.bf16 else np.asarray(x, dtype=np.float64)
@@ -145,7 +182,9 @@ class Net:
         return self._r(x)
 
     def forward(self, params, feats):
-        P = split_params(np.asarray(params, dtype=np.float64), self.convs)
+        """(output planes (B, 2, uN, uN), cache); cache["coef"] holds the
+        pupil coefficients (B, K) when the network has a pupil head"""
+        P = split_params(np.asarray(params, dtype=np.float64)[:self.nconv], self.convs)
         Wb = [(self._r(w), b) for w, b in P]
         cache = {"W": Wb}
         r = self._r
@@ -173,13 +212,25 @@ class Net:
             prev = db
         yf = conv3x3(prev, *Wb[k])
         cache.update(enc=enc, dec=dec, last=prev, khead=k, yf=yf)
+        if self.K:
+            cache["pupil"] = pupil_head_forward(enc[-1]["eb"], *self.pupil_params(params))
+            cache["coef"] = cache["pupil"][2]
         return depth_to_space(yf, self.u), cache
 
-    def loss_and_grad(self, params, feats, targets):
+    def loss_and_grad(self, params, feats, targets, pupil_targets=None):
         out, c = self.forward(params, feats)
         d = out - targets
         count = d.size
         loss = float(np.sum(d * d) / count)
+        g_pupil = None
+        if self.K:
+            dp = c["coef"] - pupil_targets
+            loss += self.lam * float(np.sum(dp * dp)) / dp.size
+            pooled, hid, _ = c["pupil"]
+            W1, _, W2, _ = self.pupil_params(params)
+            e = c["enc"][-1]["eb"]
+            g_pupil = pupil_head_backward(2.0 * self.lam * dp / dp.size, pooled, hid, W1, W2,
+                                          e.shape[1] * e.shape[2])
         gy_head = space_to_depth(2.0 * d / count, self.u)
         grads = [None] * len(self.convs)
         r, Wb = self._r, c["W"]
@@ -202,6 +253,8 @@ class Net:
             gu = conv_bwd(dl["k"], dl["up"], dl["da"], gs)
             gprev = upsample2_adj(gu)
         g[("eb", self.L - 1)] = gprev
+        if g_pupil is not None:
+            g[("eb", self.L - 1)] = gprev + g_pupil[4][:, None, None, :]
         for l in range(self.L - 1, -1, -1):
             e = enc[l]
             gea = conv_bwd(e["k"] + 1, e["ea"], e["eb"], g[("eb", l)])
@@ -211,4 +264,6 @@ class Net:
         flat = []
         for gw, gb in grads:
             flat += [gw.ravel(), gb.ravel()]
+        if g_pupil is not None:
+            flat += [t.ravel() for t in g_pupil[:4]]
         return loss, np.concatenate(flat), out

@@ -153,7 +153,12 @@ def test_cnn_create_rejects_bad_descriptors():
                  ws.CnnDesc(5, 32, 26, 64, 4, 1),     # 32 * 2^4 channels > 256
                  ws.CnnDesc(4, 32, 26, 12, 4, 1),     # n not divisible by 2^(levels-1)
                  ws.CnnDesc(4, 32, 26, 64, 2, 1),     # 2 up^2 not a multiple of 32
-                 ws.CnnDesc(4, 32, 26, 64, 4, 0)):    # max_batch
+                 ws.CnnDesc(4, 32, 26, 64, 4, 0),     # max_batch
+                 ws.CnnDesc(4, 32, 26, 64, 4, 8, -1, 0.1),   # pupil terms < 0
+                 ws.CnnDesc(4, 32, 26, 64, 4, 8, 17, 0.1),   # pupil terms > 16
+                 ws.CnnDesc(4, 32, 26, 64, 4, 65, 5, 0.1),   # batch > 64 with a pupil head
+                 ws.CnnDesc(4, 32, 26, 64, 4, 8, 5, -1.0),   # negative pupil weight
+                 ws.CnnDesc(4, 32, 26, 64, 4, 8, 5, float("nan"))):
         assert L.fpm_cnn_create(ctypes.byref(desc), 0, ctypes.byref(h)) == -1
         assert not h.value
     assert L.fpm_cnn_num_params(None) == -1

@@ -79,14 +79,39 @@ def test_depth_to_space_layout():
                         assert o[0, c, u * y + sy, u * x + sx] == yf[0, y, x, c * u * u + sy * u + sx]
 
 
-@pytest.mark.parametrize("levels", [1, 2, 3])
-def test_gradient_matches_finite_differences(levels):
+def test_pupil_head_adjoint_and_brute_force():
+    """The pupil head's forward against explicit loops, and its backward's
+    feature gradient as the adjoint of the linearised map (mean pool ->
+    masked hidden layer -> output layer)."""
+    rng = np.random.default_rng(9)
+    B, h, w, C, H, K = 2, 3, 4, 5, co.PUPIL_HIDDEN, 3
+    e = rng.standard_normal((B, h, w, C))
+    W1, b1 = rng.standard_normal((H, C)), rng.standard_normal(H)
+    W2, b2 = rng.standard_normal((K, H)), rng.standard_normal(K)
+    pooled, hid, coef = co.pupil_head_forward(e, W1, b1, W2, b2)
+    for b in range(B):
+        pl = [sum(e[b, y, x, c] for y in range(h) for x in range(w)) / (h * w) for c in range(C)]
+        hd = [max(0.0, b1[j] + sum(W1[j, c] * pl[c] for c in range(C))) for j in range(H)]
+        for k in range(K):
+            assert abs(coef[b, k] - (b2[k] + sum(W2[k, j] * hd[j] for j in range(H)))) <= 1e-9 * (1 + abs(coef[b, k]))
+    # <J de, g> = <de, J^T g> with J the derivative of coef in e (ReLU mask fixed)
+    g = rng.standard_normal((B, K))
+    de = rng.standard_normal(e.shape)
+    mask = hid > 0
+    J_de = ((de.mean(axis=(1, 2)) @ W1.T) * mask) @ W2.T
+    *_, gpool = co.pupil_head_backward(g, pooled, hid, W1, W2, h * w)
+    JT_g = np.broadcast_to(gpool[:, None, None, :], e.shape)
+    assert abs(np.sum(J_de * g) - np.sum(de * JT_g)) <= 1e-10 * max(1.0, abs(np.sum(J_de * g)))
+
+
+@pytest.mark.parametrize("levels,pk", [(1, 0), (2, 0), (3, 0), (1, 3), (2, 5)])
+def test_gradient_matches_finite_differences(levels, pk):
     """Central differences of the FP64 loss (no BF16 rounding) against the
-    analytic gradient, on parameters of every conv; inputs with margins so
-    no ReLU flips inside the difference step."""
-    net = co.Net(levels, 32, 3, 4, bf16=False)
+    analytic gradient, on parameters of every conv and of the pupil head;
+    inputs with margins so no ReLU flips inside the difference step."""
+    net = co.Net(levels, 32, 3, 4, bf16=False, pupil_terms=pk, pupil_weight=0.7)
     n = 4 << (levels - 1)
-    rng = np.random.default_rng(3 + levels)
+    rng = np.random.default_rng(3 + levels + 10 * pk)
     p = np.zeros(net.nparam)
     k = 0
     for ci, cout, _ in net.convs:
@@ -94,20 +119,26 @@ def test_gradient_matches_finite_differences(levels):
         k += cout * 9 * ci
         p[k:k + cout] = 0.1 * rng.standard_normal(cout)
         k += cout
+    p[k:] = 0.2 * rng.standard_normal(net.nparam - k)
     feats = rng.standard_normal((2, 3, n, n))
     targets = rng.standard_normal((2, 2, 4 * n, 4 * n))
-    loss, g, _ = net.loss_and_grad(p, feats, targets)
-    # probe a few weights and biases of every conv
+    zt = rng.normal(0.0, 0.4, (2, pk)) if pk else None
+    loss, g, _ = net.loss_and_grad(p, feats, targets, zt)
+    # probe a few weights and biases of every conv and of the pupil head's layers
     idx, k = [], 0
+    sizes = []
     for ci, cout, _ in net.convs:
-        idx += list(rng.integers(k, k + cout * 9 * ci, 3))
-        k += cout * 9 * ci
-        idx.append(int(rng.integers(k, k + cout)))
-        k += cout
+        sizes += [cout * 9 * ci, cout]
+    if pk:
+        sizes += [co.PUPIL_HIDDEN * net.cb, co.PUPIL_HIDDEN, pk * co.PUPIL_HIDDEN, pk]
+    for size in sizes:
+        idx += list(rng.integers(k, k + size, 3 if size > 64 else 1))
+        k += size
+    assert k == net.nparam
     h = 1e-6
     for i in idx:
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (net.loss_and_grad(pp, feats, targets)[0] - net.loss_and_grad(pm, feats, targets)[0]) / (2 * h)
+        fd = (net.loss_and_grad(pp, feats, targets, zt)[0] - net.loss_and_grad(pm, feats, targets, zt)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g).max()) + 1e-4 * abs(fd), (i, fd, g[i])

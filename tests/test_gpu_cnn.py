@@ -14,17 +14,37 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _net(levels=3, n=16, batch=2, seed=1, random_head=True):
-    """Network as created (He-normal convs, zero head) with, by default, a
-    random head, so that every layer's output and gradient is non-trivial."""
-    net = warmstart.WarmStartNet(n, 4, levels=levels, base=32, in_channels=26, max_batch=batch, seed=seed)
+def _net(levels=3, n=16, batch=2, seed=1, random_head=True, pupil_terms=0):
+    """Network as created (He-normal convs, zero head, zero pupil output
+    layer) with, by default, random heads, so that every layer's output and
+    gradient is non-trivial."""
+    net = warmstart.WarmStartNet(n, 4, levels=levels, base=32, in_channels=26, max_batch=batch, seed=seed,
+                                 pupil_terms=pupil_terms, pupil_weight=0.5)
     if random_head:
         p = net.get_params()
-        ci, co, _ = cnn_oracle.layer_shapes(levels, 32, 26, 4)[-1]
-        k = p.size - (co * 9 * ci + co)
-        p[k:k + co * 9 * ci] = np.random.default_rng(seed).standard_normal(co * 9 * ci) * np.sqrt(2.0 / (9 * ci))
+        rng = np.random.default_rng(seed)
+        onet = cnn_oracle.Net(levels, 32, 26, 4, pupil_terms=pupil_terms)
+        ci, co, _ = onet.convs[-1]
+        k = onet.nconv - (co * 9 * ci + co)
+        p[k:k + co * 9 * ci] = rng.standard_normal(co * 9 * ci) * np.sqrt(2.0 / (9 * ci))
+        if pupil_terms:
+            h = cnn_oracle.PUPIL_HIDDEN
+            k2 = onet.nconv + h * onet.cb + h
+            p[k2:k2 + pupil_terms * h] = rng.standard_normal(pupil_terms * h) * np.sqrt(1.0 / h)
+            p[onet.nconv + h * onet.cb:k2] = 0.1 * rng.standard_normal(h)  # b1: some hidden units on
         net.set_params(p)
     return net
+
+
+def _tensor_sizes(onet):
+    """sizes of the parameter tensors in vector order"""
+    sizes = []
+    for ci, cout, _ in onet.convs:
+        sizes += [cout * 9 * ci, cout]
+    if onet.K:
+        h = cnn_oracle.PUPIL_HIDDEN
+        sizes += [h * onet.cb, h, onet.K * h, onet.K]
+    return sizes
 
 
 def _inputs(n, batch, seed):
@@ -34,42 +54,57 @@ def _inputs(n, batch, seed):
     return f, t
 
 
-@pytest.mark.parametrize("levels,n", [(1, 8), (3, 16), (4, 32)])
-def test_cnn_forward_matches_oracle(levels, n):
+CASES = [(1, 8, 0), (3, 16, 0), (4, 32, 0), (1, 8, 5), (3, 16, 5), (4, 32, 5)]
+
+
+@pytest.mark.parametrize("levels,n,pk", CASES)
+def test_cnn_forward_matches_oracle(levels, n, pk):
     """Network output vs the oracle with the same parameters: relative L2
     within 1e-2 (BF16 activations: an FP32 accumulation that lands on the
-    other side of a BF16 rounding boundary moves a value by 2^-8)."""
-    net = _net(levels, n)
+    other side of a BF16 rounding boundary moves a value by 2^-8); the pupil
+    coefficients (FP32 head on BF16 features) within 1e-2 too."""
+    net = _net(levels, n, pupil_terms=pk)
     p = net.get_params()
     f, _ = _inputs(n, 2, levels)
-    got = net.forward(torch.from_numpy(f).cuda()).cpu().numpy()
-    ref, _ = cnn_oracle.Net(levels, 32, 26, 4, bf16=True).forward(p.astype(np.float64), f.astype(np.float64))
-    print(f"[cnn-fwd] levels {levels} n {n}: rel-L2 {rel_l2(got, ref):.2e}", flush=True)
+    res = net.forward(torch.from_numpy(f).cuda(), pupil=pk > 0)
+    got = (res[0] if pk else res).cpu().numpy()
+    ref, cache = cnn_oracle.Net(levels, 32, 26, 4, bf16=True, pupil_terms=pk).forward(p.astype(np.float64),
+                                                                                        f.astype(np.float64))
+    msg = f"[cnn-fwd] levels {levels} n {n} pupil {pk}: rel-L2 {rel_l2(got, ref):.2e}"
+    if pk:
+        ec = rel_l2(res[1].cpu().numpy(), cache["coef"])
+        msg += f", coefficients {ec:.2e}"
+        assert ec <= 1e-2
+    print(msg, flush=True)
     assert rel_l2(got, ref) <= 1e-2
     net.close()
 
 
-@pytest.mark.parametrize("levels,n", [(1, 8), (3, 16), (4, 32)])
-def test_cnn_gradient_matches_oracle(levels, n):
-    """Loss within 1e-3 and the gradient of every conv's weights and biases
-    within 5e-2 relative L2 of the oracle's (BF16 gradients entering each
-    product, FP32 accumulation against FP64)."""
-    net = _net(levels, n)
+@pytest.mark.parametrize("levels,n,pk", CASES)
+def test_cnn_gradient_matches_oracle(levels, n, pk):
+    """Loss within 1e-3 and the gradient of every parameter tensor (conv
+    weights and biases, pupil head layers) within 5e-2 relative L2 of the
+    oracle's (BF16 gradients entering each conv product, FP32 accumulation
+    against FP64)."""
+    net = _net(levels, n, pupil_terms=pk)
     p = net.get_params()
     f, t = _inputs(n, 2, 10 + levels)
-    loss, g = net.gradient(torch.from_numpy(f).cuda(), torch.from_numpy(t).cuda())
-    onet = cnn_oracle.Net(levels, 32, 26, 4, bf16=True)
-    lref, gref, _ = onet.loss_and_grad(p.astype(np.float64), f.astype(np.float64), t.astype(np.float64))
+    z = np.random.default_rng(20 + levels).normal(0.0, 0.4, (2, pk)).astype(np.float32) if pk else None
+    loss, g = net.gradient(torch.from_numpy(f).cuda(), torch.from_numpy(t).cuda(),
+                           torch.from_numpy(z).cuda() if pk else None)
+    onet = cnn_oracle.Net(levels, 32, 26, 4, bf16=True, pupil_terms=pk, pupil_weight=0.5)
+    lref, gref, _ = onet.loss_and_grad(p.astype(np.float64), f.astype(np.float64), t.astype(np.float64),
+                                       z.astype(np.float64) if pk else None)
     assert abs(loss - lref) <= 1e-3 * lref
     k = 0
     worst = 0.0
-    for ci, cout, _ in onet.convs:
-        for size in (cout * 9 * ci, cout):
-            e = rel_l2(g[k:k + size], gref[k:k + size])
-            worst = max(worst, e)
-            k += size
-    print(f"[cnn-grad] levels {levels} n {n}: loss {loss:.6f} vs {lref:.6f}, worst per-tensor rel-L2 {worst:.2e}, "
-          f"whole {rel_l2(g, gref):.2e}", flush=True)
+    for size in _tensor_sizes(onet):
+        e = rel_l2(g[k:k + size], gref[k:k + size])
+        worst = max(worst, e)
+        k += size
+    assert k == g.size
+    print(f"[cnn-grad] levels {levels} n {n} pupil {pk}: loss {loss:.6f} vs {lref:.6f}, worst per-tensor rel-L2 "
+          f"{worst:.2e}, whole {rel_l2(g, gref):.2e}", flush=True)
     assert rel_l2(g, gref) <= 2e-2
     assert worst <= 5e-2
     net.close()
@@ -78,29 +113,39 @@ def test_cnn_gradient_matches_oracle(levels, n):
 def test_cnn_training_fits_a_batch():
     """Adam steps through the library's own backward kernels drive the loss
     of a fixed batch down by 5x in 150 steps (the gradient is a descent
-    direction end to end).  The target is learnable by the network: the
-    first two feature planes upsampled 4x (nearest), centred."""
-    net = _net(3, 16, seed=3, random_head=False)
+    direction end to end), and the pupil head's coefficient error down by
+    5x.  The targets are learnable by the network: the first two feature
+    planes upsampled 4x (nearest), centred; coefficients = 4 x (the mean of
+    feature planes 2..6 - 0.5)."""
+    net = _net(3, 16, seed=3, random_head=False, pupil_terms=5)
     f, _ = _inputs(16, 2, 21)
+    f[1, 2:7] *= 0.5  # different plane means in the two samples
     t = np.repeat(np.repeat(f[:, :2] - 0.5, 4, axis=2), 4, axis=3).astype(np.float32)
+    z = (4.0 * (f[:, 2:7].mean(axis=(2, 3)) - 0.5)).astype(np.float32)
     F, T = torch.from_numpy(f).cuda(), torch.from_numpy(np.ascontiguousarray(t)).cuda()
-    l0 = net.train_step(F, T, 2e-3)
+    Z = torch.from_numpy(z).cuda()
+    c0 = np.linalg.norm(net.forward(F, pupil=True)[1].cpu().numpy() - z)
+    l0 = net.train_step(F, T, 2e-3, Z)
     for _ in range(150):
-        l = net.train_step(F, T, 2e-3)
-    print(f"[cnn-train] loss {l0:.4f} -> {l:.4f}", flush=True)
+        l = net.train_step(F, T, 2e-3, Z)
+    c = np.linalg.norm(net.forward(F, pupil=True)[1].cpu().numpy() - z)
+    print(f"[cnn-train] loss {l0:.4f} -> {l:.4f}, coefficient error {c0:.4f} -> {c:.4f}", flush=True)
     assert l < 0.2 * l0
+    assert c < 0.2 * c0
     net.close()
 
 
 def test_cnn_save_load_roundtrip(tmp_path):
-    """Parameters survive save/load (plain npz arrays) and give the same output."""
-    net = _net(2, 16, seed=4)
+    """Parameters and the pupil head survive save/load (plain npz arrays) and
+    give the same output."""
+    net = _net(2, 16, seed=4, pupil_terms=5)
     f, _ = _inputs(16, 2, 30)
     F = torch.from_numpy(f).cuda()
     out = net.forward(F).cpu().numpy()
     path = str(tmp_path / "net.npz")
     net.save(path)
     net2 = warmstart.WarmStartNet.load(path)
+    assert net2.desc.pupil_terms == 5 and net2.desc.pupil_weight == net.desc.pupil_weight
     assert np.array_equal(net2.get_params(), net.get_params())
     assert np.array_equal(net2.forward(F).cpu().numpy(), out)
     net.close()
@@ -117,33 +162,41 @@ def _phase_aligned_rel_l2(a, b):
 
 
 def test_warm_start_gains_over_plain_init():
-    """Train a small network (3 levels, 32 x 32 features -> 128 x 128 object,
-    7 x 7 LEDs) on 384 synthetic objects, then on 4 held-out objects compare
-    the solve from the network's object with the solve from the reference's
-    init (upsampled centre image): the initial error against the true object
-    and the loss after 4 epochs must both be lower with the network."""
+    """Train a small network (3 levels, 32 x 32 features -> 128 x 128 object
+    and 5 pupil coefficients, 7 x 7 LEDs) on 384 synthetic objects imaged
+    through random pupils, then on 4 held-out objects compare the solve from
+    the network's object and pupil with the solve from the reference's init
+    (upsampled centre image, ideal pupil): the initial error against the
+    true object and the loss after 4 epochs must both be lower with the
+    network.  (At this size the pupil coefficients are not learned: their
+    error is printed, not asserted; the 256^2 workload of the time-to-loss
+    measurement learns them.)"""
     n, r, grid = 32, 4, 7
-    net = warmstart.WarmStartNet(n, r, levels=3, base=32, max_batch=8, seed=5)
-    F, T = warmstart.make_training_set(384, n, r, seed0=5000, led_grid=grid)
-    hist = warmstart.train(net, F, T, epochs=12, batch=8, lr=2e-3, seed=1)
+    net = warmstart.WarmStartNet(n, r, levels=3, base=32, max_batch=8, seed=5, pupil_terms=5)
+    F, T, Z = warmstart.make_training_set(384, n, r, seed0=5000, led_grid=grid)
+    hist = warmstart.train(net, F, T, Z, epochs=12, batch=8, lr=2e-3, seed=1)
     print(f"[warmstart] training loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
     assert hist[-1] < hist[0]
-    e_plain, e_warm, l_plain, l_warm = [], [], [], []
+    e_plain, e_warm, l_plain, l_warm, c_err, c_zero = [], [], [], [], [], []
     for k in range(4):
-        feats, target, ds = warmstart.make_sample(n, r, 90000 + k, led_grid=grid)
+        feats, target, ds = warmstart.make_sample(n, r, 90000 + k, led_grid=grid, pupil_sigma=warmstart.PUPIL_SIGMA)
         O_true = ds["O_true_dev"].cpu().numpy()
-        O_net = warmstart.warm_start_object(net, feats, ds["O0_dev"].real)
-        for O0, errs, losses in ((ds["O0_dev"], e_plain, l_plain), (O_net, e_warm, l_warm)):
+        O_net, P_net = warmstart.warm_start(net, feats, ds["O0_dev"].real, ds["A_dev"])
+        coef = net.forward(feats[None], pupil=True)[1][0].cpu().numpy()
+        c_err.append(np.linalg.norm(coef - ds["pupil_coeffs"]))
+        c_zero.append(np.linalg.norm(ds["pupil_coeffs"]))
+        for O0, P0, errs, losses in ((ds["O0_dev"], ds["P0_dev"], e_plain, l_plain), (O_net, P_net, e_warm, l_warm)):
             errs.append(_phase_aligned_rel_l2(O0.cpu().numpy(), O_true))
-            Od, Pd = torch.empty_like(O0), torch.empty_like(ds["P0_dev"])
+            Od, Pd = torch.empty_like(O0), torch.empty_like(P0)
             torch.cuda.synchronize()
-            lossv = fpm_amd.solve_device(O0.data_ptr(), ds["P0_dev"].data_ptr(), ds["raw_dev"].data_ptr(), True,
+            lossv = fpm_amd.solve_device(O0.data_ptr(), P0.data_ptr(), ds["raw_dev"].data_ptr(), True,
                                          ds["led_dev"].data_ptr(), ds["A_dev"].data_ptr(), (n * r, n * r), (n, n),
                                          ds["L"], r, 4, 26, (2e-3, 0.999, 1e-4), Od.data_ptr(), Pd.data_ptr(),
                                          loss=True)
             losses.append(lossv[-1])
-    print(f"[warmstart] init error plain {np.mean(e_plain):.3f} net {np.mean(e_warm):.3f}; "
-          f"loss after 4 epochs plain {np.mean(l_plain):.4e} net {np.mean(l_warm):.4e}", flush=True)
+    print(f"[warmstart] init error plain {np.mean(e_plain):.3f} net {np.mean(e_warm):.3f}; pupil coefficient error "
+          f"{np.mean(c_err):.3f} (ideal pupil {np.mean(c_zero):.3f}); loss after 4 epochs plain "
+          f"{np.mean(l_plain):.4e} net {np.mean(l_warm):.4e}", flush=True)
     assert np.mean(e_warm) < np.mean(e_plain)
     assert np.mean(l_warm) < np.mean(l_plain)
     net.close()

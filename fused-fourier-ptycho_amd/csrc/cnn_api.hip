@@ -506,4 +506,18 @@ int fpm_cnn_train_step(fpm_cnn* h, const float* feats, const float* targets, con
     });
 }
 
+int fpm_parallax_shifts(const float* planes, int64_t batch, int64_t planes_per_sample, int64_t rows, int64_t cols,
+                        int32_t ref, const int32_t* which, int32_t count, int32_t iters, float* shifts, void* stream) {
+    return guarded("fpm_parallax_shifts", [&] {
+        if (!planes || !which || !shifts) throw fpm::HipError{hipErrorInvalidValue, "null argument"};
+        if (batch < 1 || batch > 65535 || planes_per_sample < 1 || planes_per_sample > (1 << 20) || rows < 3 ||
+            cols < 3 || rows * cols >= ((int64_t)1 << 31))
+            throw fpm::HipError{hipErrorInvalidValue, "batch in [1, 65535], planes of at least 3 x 3"};
+        hipStream_t st = (hipStream_t)stream;
+        parallax_shifts(planes, (int)batch, (int)planes_per_sample, (int)rows, (int)cols, ref, which, count, iters,
+                        shifts, st);
+        FPM_HIP(hipStreamSynchronize(st));
+    });
+}
+
 }  // extern "C"

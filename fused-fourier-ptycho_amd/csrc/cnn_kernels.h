@@ -92,6 +92,23 @@ void pupil_head_bwd(const float* ws, const float* target, int B, int HW, int C, 
                     int K, float lam, float* gW1, float* gb1, float* gW2, float* gb2, float* gpool, float* ge,
                     double* loss_acc, hipStream_t st);
 
+// Parallax shifts of the brightfield images (the pupil estimate of the warm
+// start).  planes: B x P planes of H x W FP32 (row-major, plane (b, p) at
+// ((b P + p) H W)).  For sample b and j < count: d(b, j) = (dy, dx), the
+// global shift with  img(y, x) ~ ref(y - dy, x - dx), img = plane which[j],
+// ref = plane ref, by Lucas-Kanade iterations on the mean/std-normalised
+// planes: w(y, x) = img(y + dy, x + dx) (bilinear, coordinates clamped to the
+// plane), over the interior pixels g = (central differences of (ref + w) / 2),
+// e = w - ref, [sum g g^T] delta = -[sum g e] (FP64 sums), d += delta.
+// which: host array of count (<= 64) plane indices; d: device B x count x 2;
+// W <= 2730 (a band of rows plus its two halo rows is staged in LDS).
+constexpr int kParallaxMaxPlanes = 64;
+struct PlaneList {
+    int32_t idx[kParallaxMaxPlanes];
+};
+void parallax_shifts(const float* planes, int B, int P, int H, int W, int ref, const int32_t* which, int count,
+                     int iters, float* d, hipStream_t st);
+
 // Adam (no weight decay): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
 // p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps); w16 = bf16(p)
 // for the first n16 entries of each layer segment (nullptr: none).

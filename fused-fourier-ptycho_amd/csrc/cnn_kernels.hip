@@ -584,6 +584,130 @@ void pupil_head_bwd(const float* ws, const float* target, int B, int HW, int C, 
     FPM_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------ parallax shifts
+constexpr int kPxThreads = 1024;
+constexpr int kPxLdsFloats = 8192;  // per staged plane (32 KiB): (band + 2) x W values
+
+__device__ __forceinline__ double block_sum_px(double v, double* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __syncthreads();  // red is free (previous readers are done)
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kPxThreads / 64; ++k) s += red[k];
+    return s;
+}
+
+// bilinear sample of plane q (H, W >= 2) at (y, x), coordinates clamped to
+// the plane: the cell's top-left corner is (min(floor y, H-2), min(floor x, W-2))
+__device__ __forceinline__ float bilinear_clamped(const float* __restrict__ q, int H, int W, float y, float x) {
+    y = fminf(fmaxf(y, 0.0f), (float)(H - 1));
+    x = fminf(fmaxf(x, 0.0f), (float)(W - 1));
+    const int y0 = min((int)y, H - 2), x0 = min((int)x, W - 2);
+    const float fy = y - (float)y0, fx = x - (float)x0;
+    const float* r0 = q + (size_t)y0 * W + x0;
+    const float* r1 = r0 + W;
+    const float a = fmaf(fx, r0[1] - r0[0], r0[0]);
+    const float b = fmaf(fx, r1[1] - r1[0], r1[0]);
+    return fmaf(fy, b - a, a);
+}
+
+// One workgroup per (image j, sample b).  Each iteration walks the interior
+// rows in bands: the normalised reference rows and the warped, normalised
+// image rows of a band (plus one halo row above and below) are staged in
+// LDS, so every warped value is sampled once and its neighbours come from LDS.
+__global__ __launch_bounds__(kPxThreads) void parallax_shift_kernel(const float* __restrict__ planes, int P, int H,
+                                                                    int W, int ref, PlaneList which, int count,
+                                                                    int iters, float* __restrict__ dout) {
+    __shared__ double red[kPxThreads / 64];
+    __shared__ float rs[kPxLdsFloats];
+    __shared__ float ws[kPxLdsFloats];
+    const int j = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
+    const size_t HW = (size_t)H * W;
+    const float* R = planes + ((size_t)b * P + ref) * HW;
+    const float* I = planes + ((size_t)b * P + which.idx[j]) * HW;
+    // mean and standard deviation of both planes (FP64)
+    double sr = 0.0, sr2 = 0.0, si = 0.0, si2 = 0.0;
+    for (size_t k = t; k < HW; k += kPxThreads) {
+        const double r = R[k], v = I[k];
+        sr += r;
+        sr2 += r * r;
+        si += v;
+        si2 += v * v;
+    }
+    sr = block_sum_px(sr, red);
+    sr2 = block_sum_px(sr2, red);
+    si = block_sum_px(si, red);
+    si2 = block_sum_px(si2, red);
+    const double inv = 1.0 / (double)HW;
+    const double mr = sr * inv, mi = si * inv;
+    const double vr = sr2 * inv - mr * mr, vi = si2 * inv - mi * mi;
+    const float ar = vr > 0.0 ? (float)(1.0 / sqrt(vr)) : 0.0f, ai = vi > 0.0 ? (float)(1.0 / sqrt(vi)) : 0.0f;
+    const float br = (float)mr, bi = (float)mi;
+    const int band = kPxLdsFloats / W - 2;  // interior rows per band (>= 1: W <= kPxLdsFloats / 3)
+    const int iw = W - 2;
+    double dy = 0.0, dx = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        const float fy = (float)dy, fx = (float)dx;
+        double s00 = 0.0, s01 = 0.0, s11 = 0.0, b0 = 0.0, b1 = 0.0;
+        for (int y0 = 1; y0 < H - 1; y0 += band) {
+            const int rows = min(band, H - 1 - y0);  // interior rows y0 .. y0 + rows - 1
+            const int staged = (rows + 2) * W;        // rows y0 - 1 .. y0 + rows
+            __syncthreads();                          // the previous band's readers are done
+            for (int k = t; k < staged; k += kPxThreads) {
+                const int yy = y0 - 1 + k / W, xx = k % W;
+                rs[k] = (R[(size_t)yy * W + xx] - br) * ar;
+                ws[k] = (bilinear_clamped(I, H, W, (float)yy + fy, (float)xx + fx) - bi) * ai;
+            }
+            __syncthreads();
+            for (int k = t; k < rows * iw; k += kPxThreads) {
+                const int ly = 1 + k / iw, x = 1 + k % iw;  // LDS row of the pixel, column
+                const int c = ly * W + x;
+                const double g0 = 0.25 * ((double)(rs[c + W] - rs[c - W]) + (double)(ws[c + W] - ws[c - W]));
+                const double g1 = 0.25 * ((double)(rs[c + 1] - rs[c - 1]) + (double)(ws[c + 1] - ws[c - 1]));
+                const double e = (double)ws[c] - (double)rs[c];
+                s00 += g0 * g0;
+                s01 += g0 * g1;
+                s11 += g1 * g1;
+                b0 += g0 * e;
+                b1 += g1 * e;
+            }
+        }
+        s00 = block_sum_px(s00, red);
+        s01 = block_sum_px(s01, red);
+        s11 = block_sum_px(s11, red);
+        b0 = block_sum_px(b0, red);
+        b1 = block_sum_px(b1, red);
+        const double det = s00 * s11 - s01 * s01;
+        if (!(det > 0.0)) break;  // no texture: keep the current estimate (every thread sees the same sums)
+        dy -= (s11 * b0 - s01 * b1) / det;
+        dx -= (s00 * b1 - s01 * b0) / det;
+    }
+    if (t == 0) {
+        dout[((size_t)b * count + j) * 2 + 0] = (float)dy;
+        dout[((size_t)b * count + j) * 2 + 1] = (float)dx;
+    }
+}
+
+void parallax_shifts(const float* planes, int B, int P, int H, int W, int ref, const int32_t* which, int count,
+                     int iters, float* d, hipStream_t st) {
+    if (B < 1 || B > 65535 || P < 1 || H < 3 || W < 3 || W > kPxLdsFloats / 3 || ref < 0 || ref >= P || count < 1 ||
+        count > kParallaxMaxPlanes || iters < 0 || (int64_t)H * W >= ((int64_t)1 << 31))
+        throw HipError{hipErrorInvalidValue,
+                       "parallax_shifts: B in [1, 65535], planes of at least 3 x 3 with at most 2730 columns, "
+                       "0 <= ref < P, 1 <= count <= 64, iters >= 0"};
+    PlaneList pl{};
+    for (int j = 0; j < count; ++j) {
+        if (which[j] < 0 || which[j] >= P) throw HipError{hipErrorInvalidValue, "parallax_shifts: plane index >= P"};
+        pl.idx[j] = which[j];
+    }
+    hipLaunchKernelGGL(parallax_shift_kernel, dim3(count, B), dim3(kPxThreads), 0, st, planes, P, H, W, ref, pl, count,
+                       iters, d);
+    FPM_LAUNCH_CHECK();
+}
+
 // -------------------------------------------------------------- Adam
 __global__ void adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
                             const float* __restrict__ g, int64_t n, AdamArgs a) {

@@ -546,6 +546,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         if (bs < 1) invalid("batch must be >= 1");
         if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
         if (o.lr_halve_from < 0) invalid("lr_halve_from must be >= 0");
+        if (!(o.stop_loss >= 0.0)) invalid("stop_loss must be >= 0 (0: off)");
         const float lr = (float)opt[0];
         Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
         g.eng = engine_of(o.fft_engine);
@@ -587,6 +588,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         const int2* led = (const int2*)upload(led_idx, 2 * (size_t)L, isdev, hLed, st);
         const float* dA = upload(A, nn, isdev, hA, st);
 
+        int ran = 0;
         if (ne > 0) {
             Solve sv;
             sv.g = g;
@@ -601,7 +603,8 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
             sv.stitch = o.stitch;
             sv.tw = tw;
             StageTimer tm(o.stage_times, st);
-            sv.init(ne, o.loss_per_epoch != nullptr, lr, beta, eps, o.pupil_clamp, o.lr_halve_from, st, tm);
+            const bool stop = o.stop_loss > 0.0;
+            sv.init(ne, o.loss_per_epoch != nullptr || stop, lr, beta, eps, o.pupil_clamp, o.lr_halve_from, st, tm);
             const bool use_graph = o.graph != 0 && o.stage_times == nullptr;
             GraphCache graphs;
             int cur = 0;
@@ -615,13 +618,24 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
                     sv.enqueue_epoch(cur, last_epoch, st, tm);
                 }
                 cur = sv.parity_after(cur);
+                ran = e + 1;
+                if (stop && !last_epoch) {
+                    // the epoch-end kernel has written this epoch's loss
+                    double lv = 0.0;
+                    FPM_HIP(hipMemcpyAsync(&lv, sv.ss.loss_hist + e, sizeof(double), hipMemcpyDeviceToHost, st));
+                    FPM_HIP(hipStreamSynchronize(st));
+                    if (lv <= o.stop_loss) break;
+                }
             }
-            if (o.loss_per_epoch)
-                FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, sv.ss.loss_hist, (size_t)ne * sizeof(double),
+            if (o.loss_per_epoch) {
+                FPM_HIP(hipMemcpyAsync(o.loss_per_epoch, sv.ss.loss_hist, (size_t)ran * sizeof(double),
                                        hipMemcpyDeviceToHost, st));
+                for (int e = ran; e < ne; ++e) o.loss_per_epoch[e] = std::nan("");
+            }
             FPM_HIP(hipStreamSynchronize(st));
             tm.flush();
         }
+        if (o.epochs_run) *o.epochs_run = ran;
         if (!isdev) {
             FPM_HIP(hipMemcpyAsync(O_out, O, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
             FPM_HIP(hipMemcpyAsync(P_out, P, nn * sizeof(float2), hipMemcpyDeviceToHost, st));
@@ -644,6 +658,7 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
         if (streams < 0 || streams > 64) invalid("streams must be in [0, 64]");
         if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
         if (o.lr_halve_from < 0) invalid("lr_halve_from must be >= 0");
+        if (o.stop_loss != 0.0 || o.epochs_run) invalid("stop_loss / epochs_run are options of fpm_pure_v3 only");
         const int eng = engine_of(o.fft_engine);
         std::vector<Geom> geo(ntiles);
         for (int t = 0; t < ntiles; ++t) {

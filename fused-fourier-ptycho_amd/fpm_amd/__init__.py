@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "fpm_forward_intensity", "fpm_fft2", "fpm_fft2_engine", "fpm_last_error", "fpm_abi_version",
     "fpm_cnn_create", "fpm_cnn_destroy", "fpm_cnn_num_params", "fpm_cnn_get_params", "fpm_cnn_set_params",
     "fpm_cnn_forward", "fpm_cnn_gradient", "fpm_cnn_train_step", "fpm_led_offsets_at", "fpm_tiles_solve",
+    "fpm_parallax_shifts",
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -86,6 +87,8 @@ class _Opts(ctypes.Structure):
         ("fft_engine", ctypes.c_int32),
         ("lr_halve_from", ctypes.c_int32),
         ("reserved", ctypes.c_int32 * 2),
+        ("stop_loss", ctypes.c_double),
+        ("epochs_run", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 
@@ -134,7 +137,7 @@ def _check(rc: int) -> None:
 
 
 def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0.2, verbose=False,
-          graph=False, engine=FFT_F32, lr_halve_from=0) -> _Opts:
+          graph=False, engine=FFT_F32, lr_halve_from=0, stop_loss=0.0, epochs_run=None) -> _Opts:
     o = _Opts()
     lib().fpm_opts_init(ctypes.byref(o))
     o.stitch = int(stitch)
@@ -147,6 +150,9 @@ def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0
     o.graph = 1 if graph else 0
     o.fft_engine = int(engine)
     o.lr_halve_from = int(lr_halve_from)
+    o.stop_loss = float(stop_loss)
+    if epochs_run is not None:
+        o.epochs_run = ctypes.pointer(epochs_run)
     return o
 
 
@@ -282,18 +288,21 @@ def solve_device(O0_ptr: int, P0_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int
                  N: Tuple[int, int], n: Tuple[int, int], L: int, pratio: float, epochs: float, batch: float,
                  opt: Sequence[float], O_out_ptr: int, P_out_ptr: int, *, stitch: int = STITCH_OWNER,
                  stream: Optional[int] = None, loss: bool = False, clamp: float = 0.2, graph: bool = False,
-                 engine: int = FFT_F32, lr_halve_from: int = 0):
+                 engine: int = FFT_F32, lr_halve_from: int = 0, stop_loss: float = 0.0):
     """fpm_pure_v3 on device pointers (column-major buffers already in HBM).
-    Returns the per-epoch loss array if ``loss`` else None."""
+    Returns the per-epoch loss array of the epochs run if ``loss`` else
+    None; with stop_loss > 0 the solve stops after the first epoch whose
+    loss is <= stop_loss."""
     optv = (ctypes.c_double * 3)(*[float(v) for v in opt[:3]])
     ne = int(epochs)
     lossb = np.zeros(max(ne, 1), dtype=np.float64) if loss else None
+    ran = ctypes.c_int32(0)
     o = _opts(stitch=stitch, device=True, stream=stream, loss_buf=lossb, clamp=clamp, graph=graph, engine=engine,
-              lr_halve_from=lr_halve_from)
+              lr_halve_from=lr_halve_from, stop_loss=stop_loss, epochs_run=ran)
     _check(lib().fpm_pure_v3(O0_ptr, N[0], N[1], P0_ptr, n[0], n[1], Y_ptr, Y_U16_RAW if y_u16 else Y_F32_AMPLITUDE,
                              L, led_ptr, A_ptr, float(pratio), float(epochs), float(batch), optv, O_out_ptr,
                              P_out_ptr, ctypes.byref(o)))
-    return lossb[:ne] if loss else None
+    return lossb[:ran.value] if loss else None
 
 
 def gradient_device(O_ptr: int, P_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int, N, n, L: int, pratio: float,

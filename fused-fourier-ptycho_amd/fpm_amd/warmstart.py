@@ -75,6 +75,9 @@ def _cnn_lib():
         L.fpm_cnn_train_step.argtypes = [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_float,
                                          ctypes.POINTER(ctypes.c_double)]
         L.fpm_cnn_train_step.restype = ctypes.c_int
+        L.fpm_parallax_shifts.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.c_int32, vp, ctypes.c_int32, ctypes.c_int32, vp, vp]
+        L.fpm_parallax_shifts.restype = ctypes.c_int
         _bound = True
     return L
 
@@ -221,20 +224,80 @@ def pupil_from_coeffs(coef, A_dev):
     return torch.polar(A_dev.to(torch.float32), phase).contiguous()
 
 
-def warm_start(net: WarmStartNet, feats, base_amp=None, A_dev=None):
-    """(O0, P0): the object of warm_start_object and, when the network has a
-    pupil head and the pupil mask A_dev is given, the pupil built from the
-    predicted coefficients (else None: the caller keeps the plain P0 = A)."""
+def warm_start(net: WarmStartNet, feats, base_amp=None, A_dev=None, pupil_model: "Optional[ParallaxPupil]" = None):
+    """(O0, P0): the object of warm_start_object and, given the pupil mask
+    A_dev, the pupil built from the coefficients of pupil_model (parallax
+    shifts of the brightfield feature planes) or else of the network's pupil
+    head; None when neither is available (the caller keeps the plain P0 = A)."""
     import torch
 
     if base_amp is None:
         base_amp = plain_init_amplitude(feats[0], net.desc.up)
-    want_p = net.desc.pupil_terms > 0 and A_dev is not None
-    res = net.forward(feats[None], pupil=want_p)
-    out, coef = res if want_p else (res, None)
+    head = net.desc.pupil_terms > 0 and A_dev is not None and pupil_model is None
+    res = net.forward(feats[None], pupil=head)
+    out, coef = res if head else (res, None)
     amp = torch.clamp(base_amp + out[0][0], min=0.0)
     O0 = torch.polar(amp, out[0][1]).contiguous()
-    return O0, (pupil_from_coeffs(coef[0], A_dev) if want_p else None)
+    if A_dev is not None and pupil_model is not None:
+        coef = pupil_model.predict(feats[None])
+    return O0, (pupil_from_coeffs(coef[0], A_dev) if coef is not None and A_dev is not None else None)
+
+
+# ---------------------------------------------------- parallax pupil
+def brightfield_planes(na_illum, na: float = 0.1):
+    """Indices of the feature planes (the first N_BRIGHT LEDs of the spiral)
+    whose LED lies inside the objective NA; plane 0 (centre LED) first."""
+    return [i for i in range(N_BRIGHT) if float(na_illum[i]) < na]
+
+
+def parallax_shifts(feats, planes, iters: int = 4):
+    """Shifts (dy, dx) of feature planes planes[1:] against plane planes[0]
+    (libfpm_amd's fpm_parallax_shifts): feats CUDA float32 (B, C, n, n) ->
+    CUDA (B, 2 (len(planes) - 1))."""
+    import torch
+
+    feats = feats.contiguous()
+    B, C, H, W = feats.shape
+    which = np.ascontiguousarray(planes[1:], dtype=np.int32)
+    out = torch.empty((B, 2 * which.size), dtype=torch.float32, device=feats.device)
+    _sync()
+    _check(_cnn_lib().fpm_parallax_shifts(feats.data_ptr(), B, C, H, W, int(planes[0]), which.ctypes.data,
+                                          int(which.size), int(iters), out.data_ptr(), None))
+    return out
+
+
+class ParallaxPupil:
+    """The pupil's Zernike coefficients from the parallax of the brightfield
+    images.  Under tilted illumination k_i the pupil phase Z shifts the image
+    by grad Z(k_i) (the linear term of Z around k_i), so the shifts of the
+    brightfield images against the centre image are linear in the
+    coefficients to first order; the map (shifts, 1) -> coefficients is
+    fitted by least squares (FP64) on training samples, which also absorbs
+    the scale, the axis conventions and the first-order model's bias."""
+
+    def __init__(self, planes, iters: int = 4):
+        self.planes = list(planes)
+        self.iters = iters
+        self.W = None
+
+    def features(self, feats):
+        return parallax_shifts(feats, self.planes, self.iters)
+
+    def fit(self, F, Z, chunk: int = 256):
+        """F: CUDA (M, C, n, n) feature planes, Z: (M, K) true coefficients."""
+        X = np.concatenate([self.features(F[k:k + chunk]).cpu().numpy() for k in range(0, F.shape[0], chunk)])
+        Z = Z.cpu().numpy() if hasattr(Z, "cpu") else np.asarray(Z)
+        Xa = np.hstack([X.astype(np.float64), np.ones((X.shape[0], 1))])
+        self.W = np.linalg.lstsq(Xa, Z.astype(np.float64), rcond=None)[0]
+        return float(np.sqrt(np.mean((Xa @ self.W - Z) ** 2)))
+
+    def predict(self, feats):
+        """CUDA (B, C, n, n) -> CUDA (B, K) coefficients."""
+        import torch
+
+        X = self.features(feats).to(torch.float64)
+        W = torch.from_numpy(self.W).to(X.device)
+        return (X @ W[:-1] + W[-1]).to(torch.float32)
 
 
 # ------------------------------------------------------------- training
@@ -330,7 +393,7 @@ def phase_aligned_rel_l2(a, b) -> float:
     return float(np.linalg.norm(a * ph - b) / np.linalg.norm(b))
 
 
-def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0, P0=None):
+def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0, P0=None, stop_loss=0.0):
     import time
 
     import torch
@@ -345,53 +408,31 @@ def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0, P0=None):
     loss = solve_device(O0.data_ptr(), P0.data_ptr(), ds["raw_dev"].data_ptr(), True,
                         ds["led_dev"].data_ptr(), ds["A_dev"].data_ptr(), (N, N), (n, n), ds["L"], ds["pratio"],
                         epochs, batch, opt, Od.data_ptr(), Pd.data_ptr(), loss=True, engine=engine,
-                        lr_halve_from=halve_from)
+                        lr_halve_from=halve_from, stop_loss=stop_loss)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, loss, Od, Pd
 
 
-# (epochs, lr halving from epoch index) tried for the warm-started solve, shortest first;
-# the last entry is the reference's own schedule
-WARM_SCHEDULES = ((1, 1), (2, 1), (3, 1), (3, 2), (4, 2), (5, 3), (6, 3), (8, 4), (10, 6), (12, 8), (16, 11),
-                  (20, 13))
-
-
-# a solve "reaches" the plain form's final loss when its final loss is within
-# this relative distance of it (the metric's rel-L2 1e-3)
+# a solve "reaches" the plain form's final loss when its loss is within this
+# relative distance of it (the metric's rel-L2 1e-3)
 LOSS_RTOL = 1e-3
-
-
-def tune_warm_schedule(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: int = 20, batch: int = 26,
-                       opt=(5e-4, 0.999, 1e-4), engine: int = 0, led_grid: int = 19,
-                       pupil_sigma: Optional[float] = PUPIL_SIGMA):
-    """Shortest (epochs, lr_halve_from) of WARM_SCHEDULES with which the
-    warm-started solve reaches the final loss of the plain 20-epoch solve
-    (within LOSS_RTOL) on a validation object (not the object the time is
-    then measured on)."""
-    feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid, pupil_sigma=pupil_sigma)
-    _, loss_plain, _, _ = _timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
-    O_net, P_net = warm_start(net, feats, ds["O0_dev"].real, ds["A_dev"])
-    for e, h in WARM_SCHEDULES:
-        _, loss_w, _, _ = _timed_solve(ds, O_net, e, batch, opt, engine, h, P_net)
-        if loss_w[-1] <= loss_plain[-1] * (1.0 + LOSS_RTOL):
-            return e, h
-    return WARM_SCHEDULES[-1]
 
 
 def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: int = 20, batch: int = 26,
                  opt=(5e-4, 0.999, 1e-4), engine: int = 0, warm_engine: Optional[int] = None, led_grid: int = 19,
-                 repeats: int = 3, schedule=None, keep_result: bool = False,
-                 pupil_sigma: Optional[float] = PUPIL_SIGMA):
+                 repeats: int = 3, keep_result: bool = False, pupil_sigma: Optional[float] = PUPIL_SIGMA,
+                 pupil_model: Optional[ParallaxPupil] = None):
     """Plain solve from the reference's init (upsampled centre image, ideal
-    pupil, the reference's 20-epoch schedule) against network inference + the
-    solve from the network's object and pupil with `schedule` = (epochs,
-    lr_halve_from) (from tune_warm_schedule on another object; default: the
-    reference's), on one held-out synthetic object whose pupil is drawn like
-    the training pupils.  The plain form's final loss is the target.
-    engine / warm_engine: FFT engine of the plain / warm-started solve.
-    Times are the best of `repeats` runs.  keep_result adds "result": the
-    warm solve's inputs and outputs as host arrays in MATLAB orientation
-    (O0, P0, raw uint16 stack, LED windows, A, pratio, batch, opt, O, P)."""
+    pupil, the reference's 20-epoch schedule) against the warm form: network
+    inference (object, and the pupil from pupil_model or the network's pupil
+    head) + the same solve from that object and pupil, stopped after the
+    first epoch whose loss reaches the plain form's final loss (within
+    LOSS_RTOL; fpm_opts.stop_loss), on one held-out synthetic object whose
+    pupil is drawn like the training pupils.  engine / warm_engine: FFT
+    engine of the plain / warm-started solve.  Times are the best of
+    `repeats` runs.  keep_result adds "result": the warm solve's inputs and
+    outputs as host arrays in MATLAB orientation (O0, P0, raw uint16 stack,
+    LED windows, A, pratio, batch, opt, O, P) and its epoch count."""
     warm_engine = engine if warm_engine is None else warm_engine
     import time
 
@@ -405,26 +446,30 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
     for _ in range(repeats):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        O_net, P_net = warm_start(net, feats, ds["O0_dev"].real, ds["A_dev"])
+        O_net, P_net = warm_start(net, feats, ds["O0_dev"].real, ds["A_dev"], pupil_model)
         torch.cuda.synchronize()
         t_inf = min(t_inf, time.perf_counter() - t0)
     coef = None
-    if net.desc.pupil_terms > 0:
+    if pupil_model is not None:
+        coef = pupil_model.predict(feats[None])[0].cpu().numpy().astype(np.float64)
+    elif net.desc.pupil_terms > 0:
         coef = net.forward(feats[None], pupil=True)[1][0].cpu().numpy().astype(np.float64)
-    we, wh = schedule if schedule else (epochs, 13)
-    t_warm, loss_warm, O_warm, P_warm = min((_timed_solve(ds, O_net, we, batch, opt, warm_engine, wh, P_net)
-                                             for _ in range(repeats)), key=lambda r: r[0])
     target = float(loss_plain[-1])
-    reached = bool(loss_warm[-1] <= target * (1.0 + LOSS_RTOL))
+    stop = target * (1.0 + LOSS_RTOL)
+    t_warm, loss_warm, O_warm, P_warm = min((_timed_solve(ds, O_net, epochs, batch, opt, warm_engine, 0, P_net,
+                                                          stop_loss=stop) for _ in range(repeats)),
+                                            key=lambda r: r[0])
+    ran = len(loss_warm)
+    reached = bool(loss_warm[-1] <= stop)
     ttl = t_inf + t_warm
     e_init = phase_aligned_rel_l2(O_net.cpu().numpy(), O_true)
     e_plain = phase_aligned_rel_l2(O_plain.cpu().numpy(), O_true)
     P0w = ds["P0_dev"] if P_net is None else P_net
     out = {
-        "plain_epochs": epochs, "plain_final_loss": target, "plain_s": t_plain,
-        "warm_schedule": {"epochs": we, "lr_halve_from": wh},
-        "warm_inference_s": t_inf, "warm_solve_s": t_warm, "warm_final_loss": float(loss_warm[-1]),
-        "warm_reaches_plain_loss": reached, "epochs_saved": epochs - we if reached else 0,
+        "seed": seed, "plain_epochs": epochs, "plain_final_loss": target, "plain_s": t_plain,
+        "warm_inference_s": t_inf, "warm_solve_s": t_warm, "warm_epochs_run": ran,
+        "warm_final_loss": float(loss_warm[-1]), "warm_reaches_plain_loss": reached,
+        "epochs_saved": epochs - ran if reached else 0,
         "warm_time_s": ttl, "ratio_vs_plain": ttl / t_plain,
         "loss_rtol": LOSS_RTOL,
         "init_rel_l2_vs_truth": {"plain": phase_aligned_rel_l2(ds["O0_dev"].cpu().numpy(), O_true), "warm": e_init},
@@ -442,5 +487,5 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
         # device layout is the transpose of the MATLAB orientation the oracle takes
         t = lambda x: np.asfortranarray(x.cpu().numpy().T)  # noqa: E731
         raw = np.asfortranarray(ds["raw_dev"].cpu().numpy().transpose(2, 1, 0))
-        out["result"] = (t(O_net), t(P0w), raw, ds["led"], ds["A"], pratio, batch, opt, t(O_warm), t(P_warm))
+        out["result"] = (t(O_net), t(P0w), raw, ds["led"], ds["A"], pratio, batch, opt, t(O_warm), t(P_warm), ran)
     return out

@@ -115,6 +115,12 @@ typedef struct {
                                      0 selects 13, the reference's schedule
                                      (cuFPM_pure_v3.cu:204-206: index > 12) */
     int32_t reserved[2];
+    double  stop_loss;            /* > 0: fpm_pure_v3 stops after the first epoch whose
+                                     loss (as loss_per_epoch defines it) is <= stop_loss;
+                                     the host reads each epoch's loss (one stream
+                                     synchronisation per epoch); loss_per_epoch entries
+                                     past the last epoch run are set to NaN */
+    int32_t* epochs_run;          /* optional host int: epochs fpm_pure_v3 ran */
 } fpm_opts;
 
 /* Fills *o with the defaults (owner-computes stitch, host pointers, silent,
@@ -299,6 +305,23 @@ int fpm_cnn_gradient(fpm_cnn* net, const float* feats, const float* targets, con
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss. */
 int fpm_cnn_train_step(fpm_cnn* net, const float* feats, const float* targets, const float* pupil_targets,
                        int64_t batch, float lr, double* loss_out);
+
+/* Pupil estimate of the warm start (no counterpart in the reference): under
+ * tilted illumination k_i the pupil phase's gradient at k_i shifts the
+ * brightfield image, so the brightfield images' shifts against the centre
+ * image measure the pupil's aberrations (their Zernike coefficients follow by
+ * a linear map fitted on training data, fpm_amd.warmstart.ParallaxPupil).
+ * For sample b and j < count: shifts[(b count + j) 2 + {0, 1}] = (dy, dx),
+ * the global shift with plane which[j](y, x) ~ plane ref(y - dy, x - dx), by
+ * `iters` Lucas-Kanade iterations on the mean/std-normalised planes
+ * (bilinear warp with clamped coordinates, interior pixels, FP64 sums).
+ * planes: DEVICE, batch x planes_per_sample row-major rows x cols FP32 planes
+ * (3 <= rows, 3 <= cols <= 2730);
+ * which: HOST, count (<= 64) plane indices; shifts: DEVICE; stream: a
+ * hipStream_t or NULL; synchronises it before returning. */
+int fpm_parallax_shifts(const float* planes, int64_t batch, int64_t planes_per_sample, int64_t rows, int64_t cols,
+                        int32_t ref, const int32_t* which, int32_t count, int32_t iters, float* shifts,
+                        void* stream);
 
 /* Thread-local text of the last error ("" if none). */
 const char* fpm_last_error(void);

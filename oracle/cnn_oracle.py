@@ -18,6 +18,9 @@ Parameters use the library's vector layout: per conv W (Cout x 9 x Cin, tap
 = 3 (dy+1) + (dx+1), Cin of the first conv rounded up to 32) then b (Cout);
 then, with a pupil head, W1, b1, W2, b2 (FP32 in the library, no BF16).
 
+parallax_shift restates the warm start's pupil measurement (global
+Lucas-Kanade shifts of the brightfield images, fpm_parallax_shifts).
+
 With ``bf16=True`` values are rounded to BF16 (round to nearest even) where
 the library stores BF16: the input features, every activation, the weights
 used in the products, the gradients entering each conv's products.  With
@@ -267,3 +270,48 @@ class Net:
         if g_pupil is not None:
             flat += [t.ravel() for t in g_pupil[:4]]
         return loss, np.concatenate(flat), out
+
+
+# ------------------------------------------------------- parallax shifts
+def _bilinear_clamped(q, y, x):
+    """bilinear samples of plane q at (y, x) (arrays), coordinates clamped to
+    the plane, cell corner (min(floor y, H-2), min(floor x, W-2))"""
+    H, W = q.shape
+    y = np.clip(y, 0.0, H - 1.0)
+    x = np.clip(x, 0.0, W - 1.0)
+    y0 = np.minimum(np.floor(y).astype(np.int64), H - 2)
+    x0 = np.minimum(np.floor(x).astype(np.int64), W - 2)
+    fy, fx = y - y0, x - x0
+    a = q[y0, x0] + fx * (q[y0, x0 + 1] - q[y0, x0])
+    b = q[y0 + 1, x0] + fx * (q[y0 + 1, x0 + 1] - q[y0 + 1, x0])
+    return a + fy * (b - a)
+
+
+def parallax_shift(ref, img, iters):
+    """(dy, dx) with img(y, x) ~ ref(y - dy, x - dx): Lucas-Kanade on the
+    mean/std-normalised planes, w(y, x) = img(y + dy, x + dx) bilinear with
+    clamped coordinates, over the interior pixels g = central differences of
+    (ref + w) / 2, e = w - ref, [sum g g^T] delta = -[sum g e], d += delta
+    (what libfpm_amd's fpm_parallax_shifts computes, in float64)."""
+    def normalised(a):
+        a = np.asarray(a, dtype=np.float64)
+        m, v = a.mean(), a.var()
+        return (a - m) / np.sqrt(v) if v > 0 else a * 0.0
+    r, q = normalised(ref), normalised(img)
+    H, W = r.shape
+    yy, xx = np.meshgrid(np.arange(1, H - 1, dtype=np.float64), np.arange(1, W - 1, dtype=np.float64), indexing="ij")
+    rc = r[1:-1, 1:-1]
+    d = np.zeros(2)
+    for _ in range(iters):
+        w = lambda oy, ox: _bilinear_clamped(q, yy + d[0] + oy, xx + d[1] + ox)  # noqa: E731
+        wc = w(0, 0)
+        g0 = 0.25 * ((r[2:, 1:-1] - r[:-2, 1:-1]) + (w(1, 0) - w(-1, 0)))
+        g1 = 0.25 * ((r[1:-1, 2:] - r[1:-1, :-2]) + (w(0, 1) - w(0, -1)))
+        e = wc - rc
+        s00, s01, s11 = np.sum(g0 * g0), np.sum(g0 * g1), np.sum(g1 * g1)
+        b0, b1 = np.sum(g0 * e), np.sum(g1 * e)
+        det = s00 * s11 - s01 * s01
+        if not det > 0:
+            break
+        d = d - np.array([s11 * b0 - s01 * b1, s00 * b1 - s01 * b0]) / det
+    return d

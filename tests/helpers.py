@@ -54,3 +54,16 @@ def margin_problem(n, r, grid=3, seed=7, spiral=True):
     for l in range(L):
         Y[:, :, l] = (model[:, :, l] - margin_field((n, n), rng)).astype(np.float32)
     return O.astype(np.complex64), P, Y, led
+
+
+def texture(n, seed, shift=(0.0, 0.0), width=0.06):
+    """Smooth periodic random texture (Gaussian low-pass of white noise,
+    frequency width `width` cycles/pixel) translated by shift = (dy, dx)
+    through the Fourier shift theorem (exact for any sub-pixel amount)."""
+    rng = np.random.default_rng(seed)
+    f = np.fft.fft2(rng.standard_normal((n, n)))
+    ky = np.fft.fftfreq(n)[:, None]
+    kx = np.fft.fftfreq(n)[None, :]
+    f *= np.exp(-(ky ** 2 + kx ** 2) / (2 * width ** 2))
+    f *= np.exp(-2j * np.pi * (ky * shift[0] + kx * shift[1]))
+    return np.real(np.fft.ifft2(f))

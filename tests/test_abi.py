@@ -106,8 +106,9 @@ def test_led_offsets_rejects_even_grid():
 
 
 def test_opts_new_fields_default_and_bad_values():
-    """graph / fft_engine / lr_halve_from default to the reference's behaviour;
-    an unknown engine and a negative schedule are rejected before any device work."""
+    """graph / fft_engine / lr_halve_from / stop_loss default to the
+    reference's behaviour; an unknown engine, a negative schedule and a
+    negative or NaN stop loss are rejected before any device work."""
     o = fpm_amd._Opts()
     fpm_amd.lib().fpm_opts_init(ctypes.byref(o))
     assert o.graph == 0 and o.fft_engine == fpm_amd.FFT_F32 and o.lr_halve_from == 0
@@ -116,7 +117,9 @@ def test_opts_new_fields_default_and_bad_values():
     Y = np.ones((16, 16, 4), np.float32, order="F")
     led = np.ones((2, 4), np.int32, order="F")
     A = np.ones((16, 16), np.float32, order="F")
-    for field, val, word in (("fft_engine", 7, "fft_engine"), ("lr_halve_from", -1, "lr_halve_from")):
+    assert o.stop_loss == 0.0 and not o.epochs_run
+    for field, val, word in (("fft_engine", 7, "fft_engine"), ("lr_halve_from", -1, "lr_halve_from"),
+                             ("stop_loss", -1.0, "stop_loss"), ("stop_loss", float("nan"), "stop_loss")):
         o = fpm_amd._opts()
         setattr(o, field, val)
         optv = (ctypes.c_double * 3)(1e-3, 0.999, 1e-4)
@@ -162,3 +165,23 @@ def test_cnn_create_rejects_bad_descriptors():
         assert L.fpm_cnn_create(ctypes.byref(desc), 0, ctypes.byref(h)) == -1
         assert not h.value
     assert L.fpm_cnn_num_params(None) == -1
+
+
+def test_parallax_shifts_rejects_bad_arguments():
+    """fpm_parallax_shifts: null arrays, planes smaller than 3 x 3, a
+    reference or plane index outside the sample's planes, more than 64
+    planes, a batch of 0: FPM_E_INVALID before any device work."""
+    from fpm_amd import warmstart as ws
+    L = ws._cnn_lib()
+    buf = np.zeros(64, np.float32)
+    p = buf.ctypes.data
+    which = np.array([1, 2], np.int32)
+    many = np.arange(65, dtype=np.int32)
+    w = which.ctypes.data
+    cases = [(None, 1, 3, 8, 8, 0, w, 2, 4, p), (p, 1, 3, 8, 8, 0, None, 2, 4, p), (p, 1, 3, 8, 8, 0, w, 2, 4, None),
+             (p, 1, 3, 2, 8, 0, w, 2, 4, p), (p, 1, 3, 8, 8, 3, w, 2, 4, p), (p, 1, 2, 8, 8, 0, w, 2, 4, p),
+             (p, 1, 70, 8, 8, 0, many.ctypes.data, 65, 4, p), (p, 0, 3, 8, 8, 0, w, 2, 4, p),
+             (p, 1, 3, 8, 8, 0, w, 2, -1, p)]
+    for c in cases:
+        assert L.fpm_parallax_shifts(*c, None) == -1, c
+        assert "fpm_parallax_shifts" in fpm_amd.lib().fpm_last_error().decode()

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import cnn_oracle as co
+from helpers import texture
 
 
 def test_round_bf16_ties_to_even_and_exact_values():
@@ -142,3 +143,35 @@ def test_gradient_matches_finite_differences(levels, pk):
         pm[i] -= h
         fd = (net.loss_and_grad(pp, feats, targets, zt)[0] - net.loss_and_grad(pm, feats, targets, zt)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g).max()) + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_parallax_shift_zero_and_known_shifts():
+    """Identical planes give exactly zero; a texture translated by a known
+    sub-pixel amount (Fourier shift theorem, so the truth does not depend on
+    any interpolation) is recovered within 0.02 px; swapping the planes
+    negates the estimate; an affine change of brightness does not matter."""
+    ref = texture(96, 1)
+    assert np.array_equal(co.parallax_shift(ref, ref, 4), [0.0, 0.0])
+    for d in ((0.3, -0.45), (-0.8, 0.1), (1.6, 0.7)):
+        img = texture(96, 1, d)
+        est = co.parallax_shift(ref, img, 6)
+        assert np.all(np.abs(est - d) <= 0.02), (d, est)
+        back = co.parallax_shift(img, ref, 6)
+        assert np.all(np.abs(back + est) <= 0.02), (est, back)
+        assert np.allclose(co.parallax_shift(ref, 3.0 * img + 7.0, 6), est, atol=1e-9)
+
+
+def test_parallax_bilinear_matches_brute_force():
+    """The oracle's clamped bilinear sampler against explicit loops."""
+    rng = np.random.default_rng(4)
+    q = rng.standard_normal((5, 7))
+    ys = rng.uniform(-1.5, 5.5, 40)
+    xs = rng.uniform(-1.5, 7.5, 40)
+    got = co._bilinear_clamped(q, ys, xs)
+    for k in range(40):
+        y, x = min(max(ys[k], 0.0), 4.0), min(max(xs[k], 0.0), 6.0)
+        y0, x0 = min(int(np.floor(y)), 3), min(int(np.floor(x)), 5)
+        fy, fx = y - y0, x - x0
+        ref = ((1 - fy) * (1 - fx) * q[y0, x0] + (1 - fy) * fx * q[y0, x0 + 1] + fy * (1 - fx) * q[y0 + 1, x0]
+               + fy * fx * q[y0 + 1, x0 + 1])
+        assert abs(got[k] - ref) <= 1e-12

@@ -176,6 +176,55 @@ def test_lr_schedule_option_matches_fp32_oracle():
     assert not np.array_equal(O3, O13)
 
 
+def test_stop_loss_stops_after_the_epoch_that_reaches_it():
+    """opts.stop_loss: with the loss of epoch 3 of a 10-epoch solve as the
+    threshold the solve stops after epoch 3 (epochs_run 4, the losses of the
+    epochs run equal the full solve's), its P is bit-identical to a 4-epoch
+    solve's and its O within rel-L2 1e-6 (a planned last step runs the
+    fused update's variant without the next step's forward pass, a separate
+    instantiation whose update may round differently in the last bit), and
+    epochs not run report NaN; a threshold no epoch reaches runs all epochs."""
+    n, r, grid = 32, 4, 7
+    O0, P0, Y, led, A = _dataset(n, r, grid)
+    opt = (5e-4, 0.999, 1e-4)
+    N = n * r
+    import torch
+    dev = [torch.from_numpy(np.ascontiguousarray(a.T)).cuda() for a in (O0, P0, Y, led, A)]
+    dO, dP, dY, dled, dA = dev
+
+    def run(epochs, stop=0.0, graph=False):
+        Od, Pd = torch.empty_like(dO), torch.empty_like(dP)
+        torch.cuda.synchronize()
+        loss = fpm_amd.solve_device(dO.data_ptr(), dP.data_ptr(), dY.data_ptr(), False, dled.data_ptr(),
+                                    dA.data_ptr(), (N, N), (n, n), Y.shape[2], r, epochs, 10, opt, Od.data_ptr(),
+                                    Pd.data_ptr(), loss=True, stop_loss=stop, graph=graph)
+        return Od.cpu().numpy(), Pd.cpu().numpy(), loss
+
+    O10, P10, l10 = run(10)
+    assert np.all(np.diff(l10[:5]) < 0)  # the first epochs' losses fall: epoch 3 is the first <= l10[3]
+    O4, P4, l4 = run(4)
+    for graph in (False, True):
+        Os, Ps, ls = run(10, stop=float(l10[3]), graph=graph)
+        assert len(ls) == 4 and np.array_equal(ls, l10[:4])
+        print(f"[stop-loss] graph {graph}: O rel-L2 vs 4-epoch solve {rel_l2(Os, O4):.2e}, "
+              f"P identical {np.array_equal(Ps, P4)}", flush=True)
+        assert np.array_equal(Ps, P4)
+        assert rel_l2(Os, O4) <= 1e-6
+    Oa, Pa, la = run(10, stop=float(l10[-1]) * 0.5)
+    assert len(la) == 10 and np.array_equal(Oa, O10) and np.array_equal(la, l10)
+    # the host-array form marks the epochs not run
+    lossb = np.zeros(10)
+    ran = __import__("ctypes").c_int32(0)
+    o = fpm_amd._opts(loss_buf=lossb, stop_loss=float(l10[3]), epochs_run=ran)
+    optv = (__import__("ctypes").c_double * 3)(*opt)
+    Oh, Ph = np.empty_like(O0), np.empty_like(P0)
+    rc = fpm_amd.lib().fpm_pure_v3(O0.ctypes.data, N, N, P0.ctypes.data, n, n, Y.ctypes.data, 0, Y.shape[2],
+                                   led.ctypes.data, A.ctypes.data, float(r), 10.0, 10.0, optv, Oh.ctypes.data,
+                                   Ph.ctypes.data, __import__("ctypes").byref(o))
+    assert rc == 0 and ran.value == 4
+    assert np.array_equal(lossb[:4], l10[:4]) and np.all(np.isnan(lossb[4:]))
+
+
 def test_solve_u16_path_equals_prepared_f32():
     """Y as uint16 raw counts (normalised on the device) gives the same
     bits as Y prepared on the host the way MATLAB does in single."""

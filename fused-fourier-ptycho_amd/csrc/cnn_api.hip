@@ -80,7 +80,7 @@ struct Net {
     int t_in = -1;
     std::vector<int> t_pool, t_ea, t_eb, t_up, t_da, t_s, t_db;
     Buf head_f, head_g;  // FP32 head output and its gradient
-    Buf xcol, gz, gzt, wt, lossd;  // lossd: image loss sum, pupil loss
+    Buf xcol, xt, gz, gzt, wt, lossd;  // lossd: image loss sum, pupil loss
     int K = 0;                     // pupil terms (0: no pupil head)
     int cb = 0;                    // channels of the bottom level
     int64_t poff = -1;             // offset of the pupil head's parameters
@@ -181,7 +181,7 @@ int conv_head(const Net& net) { return (int)net.convs.size() - 1; }
 void alloc_batch(Net& net, int B) {
     if (B <= net.B) return;
     net.B = B;
-    size_t max_xcol = 0, max_gz = 0, max_wt = 0;
+    size_t max_xcol = 0, max_xt = 0, max_gz = 0, max_wt = 0;
     for (auto& t : net.T) {
         const size_t cnt = (size_t)B * t->H * t->W * t->C;
         t->act.alloc(cnt * 2);
@@ -189,7 +189,9 @@ void alloc_batch(Net& net, int B) {
     }
     for (auto& c : net.convs) {
         const size_t Q = (size_t)B * c.H * c.W;
-        max_xcol = std::max(max_xcol, (size_t)9 * c.cin * Q);
+        // the im2col matrix only for layers the implicit weight gradient does not cover
+        if (wgrad3x3_supported(c.W, c.cin, c.cout)) max_xt = std::max(max_xt, (size_t)c.cin * Q);
+        else max_xcol = std::max(max_xcol, (size_t)9 * c.cin * Q);
         max_gz = std::max(max_gz, (size_t)c.cout * Q);
         max_wt = std::max(max_wt, (size_t)9 * c.cin * c.cout);
     }
@@ -197,6 +199,7 @@ void alloc_batch(Net& net, int B) {
     net.head_f.alloc(nh * 4);
     net.head_g.alloc(nh * 4);
     net.xcol.alloc(max_xcol * 2);
+    net.xt.alloc(max_xt * 2);
     net.gz.alloc(max_gz * 2);
     net.gzt.alloc(max_gz * 2);
     net.wt.alloc(max_wt * 2);
@@ -268,9 +271,14 @@ void conv_backward(Net& net, int ci, const bf16_t* x, const bf16_t* y, const flo
     bf16_t* gz = net.gz.as<bf16_t>();
     bf16_t* gzt = net.gzt.as<bf16_t>();
     grad_prep(gy, y, c.relu, gz, gzt, gb, Q, c.cout, st);
-    im2col_t(x, net.xcol.as<bf16_t>(), B, c.H, c.W, c.cin, st);
-    const int64_t ks = std::max<int64_t>(1024, (Q / 64 + 31) / 32 * 32);
-    gemm_nt_atomic(gzt, net.xcol.as<bf16_t>(), gW, c.cout, 9 * c.cin, Q, ks, st);
+    if (wgrad3x3_supported(c.W, c.cin, c.cout)) {
+        to_channel_major(x, net.xt.as<bf16_t>(), Q, c.cin, st);
+        wgrad3x3(gzt, net.xt.as<bf16_t>(), gW, B, c.H, c.W, c.cin, c.cout, st);
+    } else {
+        im2col_t(x, net.xcol.as<bf16_t>(), B, c.H, c.W, c.cin, st);
+        const int64_t ks = std::max<int64_t>(1024, (Q / 64 + 31) / 32 * 32);
+        gemm_nt_atomic(gzt, net.xcol.as<bf16_t>(), gW, c.cout, 9 * c.cin, Q, ks, st);
+    }
     if (gx) {
         flip_transpose_w(w16_of(net, c), net.wt.as<bf16_t>(), c.cout, c.cin, st);
         ConvArgs a{gz, net.wt.as<bf16_t>(), nullptr, nullptr, gx, B, c.H, c.W, c.cout, c.cin, 0};

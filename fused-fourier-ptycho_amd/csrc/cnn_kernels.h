@@ -40,6 +40,18 @@ void gemm_nt_atomic(const bf16_t* A, const bf16_t* Bm, float* C, int M, int N, i
 // a (9 Cin) x Q BF16 matrix with Q = B H W contiguous.
 void im2col_t(const bf16_t* x, bf16_t* xcol, int B, int H, int W, int C, hipStream_t st);
 
+// Implicit weight gradient of a 3x3 convolution (no im2col matrix):
+//   gW(co, tap, ci) += sum_q gz_t(co, q) xt(ci, q + off(tap))   (0 off the image)
+// gz_t: Cout x Q and xt: Cin x Q BF16 (channel-major, Q = B H W contiguous),
+// gW FP32 (cleared first; partial sums over pixel ranges added with FP32
+// atomics).  Needs W % 8 == 0 (runs of 8 pixels stay in one image row) and
+// Cin, Cout multiples of 32.
+bool wgrad3x3_supported(int W, int Cin, int Cout);
+void wgrad3x3(const bf16_t* gz_t, const bf16_t* xt, float* gW, int B, int H, int W, int Cin, int Cout,
+              hipStream_t st);
+// channel-major copy of an NHWC tensor: xt(c, q) = x(q, c), Q pixels, C <= 256 channels
+void to_channel_major(const bf16_t* x, bf16_t* xt, int64_t Q, int C, hipStream_t st);
+
 // weights for the input gradient: wt(ci, tap, co) = w(co, 8 - tap, ci)
 void flip_transpose_w(const bf16_t* w, bf16_t* wt, int Cout, int Cin, hipStream_t st);
 

@@ -35,36 +35,38 @@ static unsigned blocks_for(int64_t n, int t) {
 }
 
 // --------------------------------------------------------------- conv 3x3
-// One wave per 32 pixels (flattened index within an image) x 32 output
-// channels: 2 x 2 tiles of 16 x 16, K loop over 9 taps x Cin / 32.
+// One wave per 16 MT pixels (flattened index within an image) x 32 output
+// channels: MT x 2 tiles of 16 x 16, K loop over 9 taps x Cin / 32.  Each
+// K step loads MT + 2 fragments for 2 MT products.
+template <int MT>
 __global__ __launch_bounds__(64) void conv3x3_kernel(ConvArgs a) {
     const int l = threadIdx.x;
     const int i = l & 15, g = l >> 4;
     const int P = a.H * a.W;
-    const int p0 = blockIdx.x * 32;
+    const int p0 = blockIdx.x * 16 * MT;
     const int c0 = blockIdx.y * 32;
     const int b = blockIdx.z;
     const bf16_t* xb = a.x + (size_t)b * P * a.Cin;
-    int py[2], px[2];
-    bool pin[2];
+    int py[MT], px[MT];
+    bool pin[MT];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
+    for (int mt = 0; mt < MT; ++mt) {
         const int p = p0 + 16 * mt + i;
         pin[mt] = p < P;
         py[mt] = p / a.W;
         px[mt] = p % a.W;
     }
-    f32x4 acc[2][2];
+    f32x4 acc[MT][2];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int tap = 0; tap < 9; ++tap) {
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const bf16_t* src[2];
-        bool ok[2];
+        const bf16_t* src[MT];
+        bool ok[MT];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
+        for (int mt = 0; mt < MT; ++mt) {
             const int yy = py[mt] + dy, xx = px[mt] + dx;
             ok[mt] = pin[mt] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
             src[mt] = xb + ((size_t)(ok[mt] ? yy : 0) * a.W + (ok[mt] ? xx : 0)) * a.Cin + 8 * g;
@@ -73,14 +75,14 @@ __global__ __launch_bounds__(64) void conv3x3_kernel(ConvArgs a) {
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) wr[nt] = a.w + ((size_t)(c0 + 16 * nt + i) * 9 + tap) * a.Cin + 8 * g;
         for (int ci = 0; ci < a.Cin; ci += 32) {
-            const bf16x8 A0 = ok[0] ? ld8(src[0] + ci) : zero8();
-            const bf16x8 A1 = ok[1] ? ld8(src[1] + ci) : zero8();
             const bf16x8 B0 = ld8(wr[0] + ci);
             const bf16x8 B1 = ld8(wr[1] + ci);
-            acc[0][0] = mma(A0, B0, acc[0][0]);
-            acc[0][1] = mma(A0, B1, acc[0][1]);
-            acc[1][0] = mma(A1, B0, acc[1][0]);
-            acc[1][1] = mma(A1, B1, acc[1][1]);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const bf16x8 A = ok[mt] ? ld8(src[mt] + ci) : zero8();
+                acc[mt][0] = mma(A, B0, acc[mt][0]);
+                acc[mt][1] = mma(A, B1, acc[mt][1]);
+            }
         }
     }
 #pragma unroll
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(64) void conv3x3_kernel(ConvArgs a) {
         const int co = c0 + 16 * nt + i;
         const float bias = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int p = p0 + 16 * mt + 4 * g + r;
@@ -105,8 +107,15 @@ __global__ __launch_bounds__(64) void conv3x3_kernel(ConvArgs a) {
 void conv3x3(const ConvArgs& a, hipStream_t st) {
     if (a.Cin % 32 || a.Cout % 32) throw HipError{hipErrorInvalidValue, "conv3x3: channels must be multiples of 32"};
     const int P = a.H * a.W;
-    dim3 grid((P + 31) / 32, a.Cout / 32, a.B);
-    hipLaunchKernelGGL(conv3x3_kernel, grid, dim3(64), 0, st, a);
+    // 64-pixel waves once there are enough of them to fill the chip, else 32
+    const int64_t waves64 = (int64_t)((P + 63) / 64) * (a.Cout / 32) * a.B;
+    if (waves64 >= 2048) {
+        dim3 grid((P + 63) / 64, a.Cout / 32, a.B);
+        hipLaunchKernelGGL(conv3x3_kernel<4>, grid, dim3(64), 0, st, a);
+    } else {
+        dim3 grid((P + 31) / 32, a.Cout / 32, a.B);
+        hipLaunchKernelGGL(conv3x3_kernel<2>, grid, dim3(64), 0, st, a);
+    }
     FPM_LAUNCH_CHECK();
 }
 
@@ -171,6 +180,138 @@ void gemm_nt_atomic(const bf16_t* A, const bf16_t* Bm, float* C, int M, int N, i
         throw HipError{hipErrorInvalidValue, "gemm_nt_atomic: M, N multiples of 32, K of 8, ksplit of 32"};
     dim3 grid(M / 32, N / 32, (unsigned)((K + ksplit - 1) / ksplit));
     hipLaunchKernelGGL(gemm_nt_kernel, grid, dim3(64), 0, st, A, Bm, C, M, N, K, ksplit);
+    FPM_LAUNCH_CHECK();
+}
+
+// --------------------------------------------- implicit weight gradient
+// One wave per (32 output channels, kernel row dy, 32 input channels, pixel
+// range): the three taps (dy, -1..+1) of the row share the wave's operands.
+// Lane (i, g) holds, per 32-pixel chunk, the run of 8 pixels q .. q+7
+// (q = q0 + 8g) of gz_t row co and of xt row ci at row y + dy; the runs for
+// dx = -1 / +1 are the aligned run funnel-shifted by one BF16 with its
+// neighbour run (zero beyond the row's ends: that is the zero padding).
+__device__ __forceinline__ uint4 ld16(const bf16_t* p) { return *(const uint4*)p; }
+
+__global__ __launch_bounds__(64) void wgrad3x3_kernel(const bf16_t* __restrict__ gzt, const bf16_t* __restrict__ xt,
+                                                      float* __restrict__ gW, int H, int W, int Cin, int64_t Q,
+                                                      int64_t ks) {
+    const int l = threadIdx.x;
+    const int i = l & 15, g = l >> 4;
+    const int co0 = blockIdx.x * 32;
+    const int dy = (int)(blockIdx.y % 3) - 1;
+    const int ci0 = (int)(blockIdx.y / 3) * 32;
+    const int64_t kb = (int64_t)blockIdx.z * ks;
+    const int64_t ke = std::min<int64_t>(Q, kb + ks);
+    const bf16_t* arow[2];
+    const bf16_t* brow[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        arow[t] = gzt + (size_t)(co0 + 16 * t + i) * Q;
+        brow[t] = xt + (size_t)(ci0 + 16 * t + i) * Q;
+    }
+    f32x4 acc[3][2][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) acc[d][mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint32_t HWu = (uint32_t)H * (uint32_t)W;
+    for (int64_t q0 = kb; q0 < ke; q0 += 32) {
+        const int64_t q = q0 + 8 * g;
+        bf16x8 a[2];
+        uint4 xv[2][3];  // per input-channel tile: runs for dx = -1, 0, +1
+        if (q < ke) {
+            const uint32_t pix = (uint32_t)(q % (int64_t)HWu);
+            const int y = (int)(pix / (uint32_t)W), x0 = (int)(pix % (uint32_t)W);
+            const int yy = y + dy;
+            const bool rowok = yy >= 0 && yy < H;
+            const int64_t src = q + (int64_t)dy * W;  // same image, row y + dy
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = ld8(arow[t] + q);
+                uint4 vp = make_uint4(0u, 0u, 0u, 0u), v0 = vp, v1 = vp;
+                if (rowok) {
+                    v0 = ld16(brow[t] + src);
+                    if (x0 > 0) vp = ld16(brow[t] + src - 8);
+                    if (x0 + 8 < W) v1 = ld16(brow[t] + src + 8);
+                }
+                xv[t][0] = make_uint4(__builtin_amdgcn_alignbyte(v0.x, vp.w, 2), __builtin_amdgcn_alignbyte(v0.y, v0.x, 2),
+                                      __builtin_amdgcn_alignbyte(v0.z, v0.y, 2), __builtin_amdgcn_alignbyte(v0.w, v0.z, 2));
+                xv[t][1] = v0;
+                xv[t][2] = make_uint4(__builtin_amdgcn_alignbyte(v0.y, v0.x, 2), __builtin_amdgcn_alignbyte(v0.z, v0.y, 2),
+                                      __builtin_amdgcn_alignbyte(v0.w, v0.z, 2), __builtin_amdgcn_alignbyte(v1.x, v0.w, 2));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                a[t] = zero8();
+                xv[t][0] = xv[t][1] = xv[t][2] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const bf16x8 b = __builtin_bit_cast(bf16x8, xv[nt][d]);
+                acc[d][0][nt] = mma(a[0], b, acc[d][0][nt]);
+                acc[d][1][nt] = mma(a[1], b, acc[d][1][nt]);
+            }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int tap = 3 * (dy + 1) + d;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int co = co0 + 16 * mt + 4 * g + r, ci = ci0 + 16 * nt + i;
+                    unsafeAtomicAdd(&gW[((size_t)co * 9 + tap) * Cin + ci], acc[d][mt][nt][r]);
+                }
+    }
+}
+
+bool wgrad3x3_supported(int W, int Cin, int Cout) { return W % 8 == 0 && Cin % 32 == 0 && Cout % 32 == 0; }
+
+void wgrad3x3(const bf16_t* gz_t, const bf16_t* xt, float* gW, int B, int H, int W, int Cin, int Cout,
+              hipStream_t st) {
+    if (!wgrad3x3_supported(W, Cin, Cout) || B < 1 || H < 1)
+        throw HipError{hipErrorInvalidValue, "wgrad3x3: W a multiple of 8, channels multiples of 32"};
+    const int64_t Q = (int64_t)B * H * W;
+    // pixel ranges of 32-pixel chunks: about 64 ranges per output tile
+    const int64_t ks = std::max<int64_t>(512, (Q / 64 + 31) / 32 * 32);
+    dim3 grid(Cout / 32, 3 * (Cin / 32), (unsigned)((Q + ks - 1) / ks));
+    hipLaunchKernelGGL(wgrad3x3_kernel, grid, dim3(64), 0, st, gz_t, xt, gW, H, W, Cin, Q, ks);
+    FPM_LAUNCH_CHECK();
+}
+
+// channel-major copy through an LDS tile of 64 pixels x C channels
+__global__ __launch_bounds__(256) void to_channel_major_kernel(const bf16_t* __restrict__ x, bf16_t* __restrict__ xt,
+                                                               int64_t Q, int C) {
+    __shared__ bf16_t tile[64 * 257];
+    const int t = threadIdx.x;
+    const int elems = 64 * C;
+    for (int64_t q0 = (int64_t)blockIdx.x * 64; q0 < Q; q0 += (int64_t)gridDim.x * 64) {
+        __syncthreads();
+        for (int e = t; e < elems; e += 256) {
+            const int qq = e / C, c = e - qq * C;
+            tile[qq * 257 + c] = q0 + qq < Q ? x[(size_t)(q0 + qq) * C + c] : (bf16_t)0;
+        }
+        __syncthreads();
+        for (int e = t; e < elems; e += 256) {
+            const int c = e / 64, qq = e % 64;
+            if (q0 + qq < Q) xt[(size_t)c * Q + q0 + qq] = tile[qq * 257 + c];
+        }
+    }
+}
+
+void to_channel_major(const bf16_t* x, bf16_t* xt, int64_t Q, int C, hipStream_t st) {
+    if (C < 1 || C > 256) throw HipError{hipErrorInvalidValue, "to_channel_major: 1 <= C <= 256"};
+    if (Q <= 0) return;
+    const unsigned blocks = (unsigned)std::min<int64_t>((Q + 63) / 64, 4096);
+    hipLaunchKernelGGL(to_channel_major_kernel, dim3(blocks), dim3(256), 0, st, x, xt, Q, C);
     FPM_LAUNCH_CHECK();
 }
 
@@ -585,7 +726,7 @@ void pupil_head_bwd(const float* ws, const float* target, int B, int HW, int C, 
 }
 
 // ------------------------------------------------------ parallax shifts
-constexpr int kPxThreads = 1024;
+constexpr int kPxThreads = 512;
 constexpr int kPxLdsFloats = 8192;  // per staged plane (32 KiB): (band + 2) x W values
 
 __device__ __forceinline__ double block_sum_px(double v, double* red) {

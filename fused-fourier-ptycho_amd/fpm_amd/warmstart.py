@@ -393,7 +393,7 @@ def phase_aligned_rel_l2(a, b) -> float:
     return float(np.linalg.norm(a * ph - b) / np.linalg.norm(b))
 
 
-def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0, P0=None, stop_loss=0.0):
+def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0, P0=None, stop_loss=0.0, graph=False):
     import time
 
     import torch
@@ -408,7 +408,7 @@ def _timed_solve(ds, O0, epochs, batch, opt, engine, halve_from=0, P0=None, stop
     loss = solve_device(O0.data_ptr(), P0.data_ptr(), ds["raw_dev"].data_ptr(), True,
                         ds["led_dev"].data_ptr(), ds["A_dev"].data_ptr(), (N, N), (n, n), ds["L"], ds["pratio"],
                         epochs, batch, opt, Od.data_ptr(), Pd.data_ptr(), loss=True, engine=engine,
-                        lr_halve_from=halve_from, stop_loss=stop_loss)
+                        lr_halve_from=halve_from, stop_loss=stop_loss, graph=graph)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, loss, Od, Pd
 
@@ -428,7 +428,9 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
     head) + the same solve from that object and pupil, stopped after the
     first epoch whose loss reaches the plain form's final loss (within
     LOSS_RTOL; fpm_opts.stop_loss), on one held-out synthetic object whose
-    pupil is drawn like the training pupils.  engine / warm_engine: FFT
+    pupil is drawn like the training pupils.  Both solves launch eagerly
+    (hipGraph replay would capture its epoch graphs again in every call,
+    which 2-7 epochs do not amortise).  engine / warm_engine: FFT
     engine of the plain / warm-started solve.  Times are the best of
     `repeats` runs.  keep_result adds "result": the warm solve's inputs and
     outputs as host arrays in MATLAB orientation (O0, P0, raw uint16 stack,

@@ -145,6 +145,10 @@ def test_tiles_solve_rejects_bad_arguments():
     p = buf.ctypes.data
     arr[0] = ft.Tile(p, p, 48, p, p, 24, p, 0, p, p, None)
     assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 1, None, None) == -2
+    # stop_loss is an option of the single solve only
+    o = fpm_amd._opts(stop_loss=1.0)
+    assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 1, ctypes.byref(o), None) == -1
+    assert "stop_loss" in L.fpm_last_error().decode()
 
 
 def test_cnn_create_rejects_bad_descriptors():
@@ -185,3 +189,28 @@ def test_parallax_shifts_rejects_bad_arguments():
     for c in cases:
         assert L.fpm_parallax_shifts(*c, None) == -1, c
         assert "fpm_parallax_shifts" in fpm_amd.lib().fpm_last_error().decode()
+
+
+def test_parallax_pupil_fit_recovers_a_linear_map():
+    """ParallaxPupil's least-squares map (FP64, with a bias column) recovers
+    an exact linear relation between shift features and coefficients, and
+    predict applies it; the features are stubbed (no device needed)."""
+    torch = pytest.importorskip("torch")
+    from fpm_amd import warmstart as ws
+
+    class Stub(ws.ParallaxPupil):
+        def features(self, feats):
+            return feats.reshape(feats.shape[0], -1)
+
+    rng = np.random.default_rng(5)
+    Wt = rng.standard_normal((6, 5))
+    bias = rng.standard_normal(5)
+    X = rng.standard_normal((300, 6))
+    Z = X @ Wt + bias
+    m = Stub([0, 1, 2, 3])
+    rms = m.fit(torch.from_numpy(X.reshape(300, 6, 1, 1)), torch.from_numpy(Z), chunk=64)
+    assert rms < 1e-10
+    assert np.allclose(m.W[:-1], Wt) and np.allclose(m.W[-1], bias)
+    Xt = rng.standard_normal((4, 6))
+    got = m.predict(torch.from_numpy(Xt.reshape(4, 6, 1, 1))).numpy()
+    assert np.allclose(got, Xt @ Wt + bias, atol=1e-5)

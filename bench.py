@@ -12,7 +12,8 @@ Also reported:
   stages     per-stage device time (HIP events on the solve's stream, inside
              the timed region) and GB/s against the dense algorithmic byte
              model (per step 104 N^2, per LED 138 n^2 with uint16 Y);
-  roofline   the dominant stage against MI355X's 8 TB/s HBM peak;
+  roofline   the dominant stage against MI355X's 8 TB/s HBM peak, and its
+             measured HBM traffic against a plain device copy timed here;
   cpu_baseline the FP32 CPU restatement (oracle) on the host cores, one
              minibatch step of the same workload scaled to s/epoch.
 """
@@ -65,6 +66,27 @@ def stage_alg_bytes(n, N, b_avg, y_bytes, stitch):
 # stage -> kernel whose per-dispatch HBM traffic stands for it in the PMC summary
 STAGE_KERNEL = {"fused_update": "fused_update_kernel<16384, true, 0>"}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_fpm_solve_16384_b26.md")
+
+
+def copy_bandwidth_gbps(nbytes=1 << 30, reps=5):
+    """HBM bandwidth of a plain device-to-device copy of nbytes (read + write
+    counted), best of `reps`, timed with HIP events on the current stream."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) / 1e3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / best / 1e9
 
 
 def measured_traffic(stage, n, N):
@@ -390,6 +412,10 @@ def main():
                            "alg_bytes_per_exec": alg[dom], "ms_per_exec": stages[dom]["ms_per_exec"]}
         if traffic:
             out["roofline"]["traffic_GBps"] = round(traffic / 1e9 / (stages[dom]["ms_per_exec"] / 1e3), 1)
+            # the measured denominator beside the spec peak: a plain device copy
+            cp = copy_bandwidth_gbps()
+            out["roofline"]["copy_GBps"] = round(cp, 1)
+            out["roofline"]["traffic_frac_of_copy"] = round(out["roofline"]["traffic_GBps"] / cp, 4)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, one minibatch step)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = run_cpu_baseline(ds, args)

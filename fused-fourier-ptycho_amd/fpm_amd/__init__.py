@@ -306,10 +306,11 @@ def solve_device(O0_ptr: int, P0_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int
 
 
 def gradient_device(O_ptr: int, P_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int, N, n, L: int, pratio: float,
-                    GO_ptr: int, GP_ptr: int, *, stitch: int = STITCH_OWNER, stream: Optional[int] = None) -> float:
+                    GO_ptr: int, GP_ptr: int, *, stitch: int = STITCH_OWNER, stream: Optional[int] = None,
+                    engine: int = FFT_F32) -> float:
     """fpm_fused_gradient on device pointers; returns the loss."""
     lossv = ctypes.c_double(0.0)
-    o = _opts(stitch=stitch, device=True, stream=stream)
+    o = _opts(stitch=stitch, device=True, stream=stream, engine=engine)
     _check(lib().fpm_fused_gradient(O_ptr, N[0], N[1], P_ptr, n[0], n[1], Y_ptr, Y_U16_RAW if y_u16 else Y_F32_AMPLITUDE,
                                     L, led_ptr, float(pratio), GO_ptr, GP_ptr, ctypes.byref(lossv), ctypes.byref(o)))
     return lossv.value

@@ -199,7 +199,7 @@ def fp64_check(result):
             "oracle_threads": oracle.num_threads(), "epochs": epochs}
 
 
-TTL_TEST_SEEDS = (77000, 77001, 77002, 77003)
+TTL_TEST_SEEDS = tuple(range(77000, 77008))
 
 
 def run_time_to_loss(args):

@@ -291,6 +291,19 @@ class ParallaxPupil:
         self.W = np.linalg.lstsq(Xa, Z.astype(np.float64), rcond=None)[0]
         return float(np.sqrt(np.mean((Xa @ self.W - Z) ** 2)))
 
+    def save(self, path: str) -> None:
+        """The fitted map, its planes and iteration count (plain npz arrays)."""
+        if self.W is None:
+            raise FPMError(-1, "ParallaxPupil.save: not fitted")
+        np.savez(path, W=self.W, planes=np.asarray(self.planes, np.int64), iters=np.array([self.iters], np.int64))
+
+    @classmethod
+    def load(cls, path: str) -> "ParallaxPupil":
+        z = np.load(path)  # plain arrays only (allow_pickle stays False)
+        m = cls([int(v) for v in z["planes"]], int(z["iters"][0]))
+        m.W = np.asarray(z["W"], dtype=np.float64)
+        return m
+
     def predict(self, feats):
         """CUDA (B, C, n, n) -> CUDA (B, K) coefficients."""
         import torch

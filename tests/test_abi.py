@@ -214,3 +214,15 @@ def test_parallax_pupil_fit_recovers_a_linear_map():
     Xt = rng.standard_normal((4, 6))
     got = m.predict(torch.from_numpy(Xt.reshape(4, 6, 1, 1))).numpy()
     assert np.allclose(got, Xt @ Wt + bias, atol=1e-5)
+
+
+def test_parallax_pupil_save_load_roundtrip(tmp_path):
+    from fpm_amd import warmstart as ws
+    m = ws.ParallaxPupil([0, 2, 5], iters=3)
+    with pytest.raises(fpm_amd.FPMError):
+        m.save(str(tmp_path / "unfitted.npz"))
+    m.W = np.random.default_rng(1).standard_normal((5, 5))
+    path = str(tmp_path / "pupil.npz")
+    m.save(path)
+    m2 = ws.ParallaxPupil.load(path)
+    assert m2.planes == [0, 2, 5] and m2.iters == 3 and np.array_equal(m2.W, m.W)

@@ -328,6 +328,8 @@ def main():
     ap.add_argument("--stitch", choices=["atomic", "owner"], default="owner")
     ap.add_argument("--y", choices=["u16", "f32"], default="u16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-sweep", default="16,26,32",
+                    help="comma-separated batch sizes also timed (same epochs, s/epoch each); empty: none")
     ap.add_argument("--no-full-check", action="store_true",
                     help="skip the full-scale one-minibatch gradient check against the FP64 oracle (host cores)")
     ap.add_argument("--no-stages", action="store_true", help="time without per-stage events")
@@ -392,7 +394,8 @@ def main():
     opt = (args.lr, 0.999, 1e-4)
     stitch = fpm_amd.STITCH_ATOMIC if args.stitch == "atomic" else fpm_amd.STITCH_OWNER
 
-    def solve(epochs, stage_times=None):
+    def solve(epochs, stage_times=None, batch=None):
+        batch = args.batch if batch is None else batch
         optv = (ctypes.c_double * 3)(*opt)
         o = fpm_amd._opts(stitch=stitch, device=True, stream=stream.cuda_stream, graph=args.graph,
                           engine=fpm_amd.FFT_SPLIT_F16 if args.engine == "split" else fpm_amd.FFT_F32)
@@ -401,7 +404,7 @@ def main():
         fpm_amd._check(fpm_amd.lib().fpm_pure_v3(
             ds["O0_dev"].data_ptr(), N, N, ds["P0_dev"].data_ptr(), n, n, Yd.data_ptr(),
             fpm_amd.Y_U16_RAW if y_u16 else fpm_amd.Y_F32_AMPLITUDE, L, ds["led_dev"].data_ptr(),
-            ds["A_dev"].data_ptr(), float(args.pratio), float(epochs), float(args.batch), optv,
+            ds["A_dev"].data_ptr(), float(args.pratio), float(epochs), float(batch), optv,
             Od.data_ptr(), Pd.data_ptr(), ctypes.byref(o)))
 
     if args.warmup > 0:
@@ -465,6 +468,22 @@ def main():
             cp = copy_bandwidth_gbps()
             out["roofline"]["copy_GBps"] = round(cp, 1)
             out["roofline"]["traffic_frac_of_copy"] = round(out["roofline"]["traffic_GBps"] / cp, 4)
+    if args.batch_sweep:
+        # the full-scale configuration's batch sizes: same call, same number of epochs, launched
+        # without per-stage events (so every size, the bench line's included, is timed alike)
+        sweep = {}
+        for bs in (int(v) for v in args.batch_sweep.split(",")):
+            solve(1, batch=bs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            solve(args.steps, batch=bs)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+            spe = (L + bs - 1) // bs
+            sweep[str(bs)] = {"s_per_epoch": dt, "steps_per_epoch": spe,
+                              "epoch_model_GBps": (spe * 104.0 * N * N + L * (136.0 + y_bytes) * n * n) / 1e9 / dt}
+            print(f"[bench] batch {bs}: {dt * 1e3:.1f} ms/epoch", file=sys.stderr, flush=True)
+        out["batch_sweep"] = sweep
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, one minibatch step)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = run_cpu_baseline(ds, args)

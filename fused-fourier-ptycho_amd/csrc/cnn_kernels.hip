@@ -104,8 +104,80 @@ __global__ __launch_bounds__(64) void conv3x3_kernel(ConvArgs a) {
     }
 }
 
+// Patch variant for rows of 64k pixels: a workgroup (4 waves) computes one
+// 64-pixel row segment x 32 output channels.  Per chunk of 32 input channels
+// it stages the input patch (rows y-1..y+1, pixels x0-1..x0+64, zero off the
+// image) and the 32 x 9 x 32 weights in LDS once, and every wave reads the
+// fragments of all 9 taps from there (wave w: pixels 16w..16w+15).  A patch
+// pixel occupies 80 bytes (32 channels + 8 pad) so that each group of 8
+// lanes of a 16-byte LDS read hits distinct banks.
+constexpr int kPatchPx = 66, kPatchStride = 40;  // pixels per patch row, BF16 per patch pixel
+__global__ __launch_bounds__(256) void conv3x3_patch_kernel(ConvArgs a) {
+    __shared__ __align__(16) bf16_t patch[3 * kPatchPx * kPatchStride];
+    __shared__ __align__(16) bf16_t wts[32 * 9 * kPatchStride];
+    const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+    const int i = l & 15, g = l >> 4;
+    const int segs = a.W / 64;
+    const int y = blockIdx.x / segs, x0 = (blockIdx.x % segs) * 64;
+    const int c0 = blockIdx.y * 32, b = blockIdx.z;
+    const size_t img = (size_t)b * a.H * a.W;
+    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+    for (int ci0 = 0; ci0 < a.Cin; ci0 += 32) {
+        __syncthreads();  // the previous chunk's readers are done
+        // input patch: 3 x 66 pixels x 4 runs of 8 channels
+        for (int e = t; e < 3 * kPatchPx * 4; e += 256) {
+            const int run = e & 3, px = (e >> 2) % kPatchPx, r = (e >> 2) / kPatchPx;
+            const int yy = y + r - 1, xx = x0 + px - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                v = *(const uint4*)(a.x + (img + (size_t)yy * a.W + xx) * a.Cin + ci0 + 8 * run);
+            *(uint4*)(patch + (r * kPatchPx + px) * kPatchStride + 8 * run) = v;
+        }
+        // weights: 32 output channels x 9 taps x 4 runs of 8 channels
+        for (int e = t; e < 32 * 9 * 4; e += 256) {
+            const int run = e & 3, ct = e >> 2;  // ct = co * 9 + tap
+            const int co = ct / 9, tap = ct % 9;
+            *(uint4*)(wts + ct * kPatchStride + 8 * run) =
+                *(const uint4*)(a.w + ((size_t)(c0 + co) * 9 + tap) * a.Cin + ci0 + 8 * run);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;  // patch row and pixel offset
+            const bf16x8 A = __builtin_bit_cast(
+                bf16x8, *(const uint4*)(patch + (dy * kPatchPx + 16 * wv + i + dx) * kPatchStride + 8 * g));
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const bf16x8 B = __builtin_bit_cast(
+                    bf16x8, *(const uint4*)(wts + ((16 * nt + i) * 9 + tap) * kPatchStride + 8 * g));
+                acc[nt] = mma(A, B, acc[nt]);
+            }
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int co = c0 + 16 * nt + i;
+        const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int x = x0 + 16 * wv + 4 * g + r;
+            float v = acc[nt][r] + bias;
+            if (a.relu) v = fmaxf(v, 0.0f);
+            const size_t o = (img + (size_t)y * a.W + x) * a.Cout + co;
+            if (a.y) a.y[o] = f2bf(v);
+            if (a.yf) a.yf[o] = v;
+        }
+    }
+}
+
 void conv3x3(const ConvArgs& a, hipStream_t st) {
     if (a.Cin % 32 || a.Cout % 32) throw HipError{hipErrorInvalidValue, "conv3x3: channels must be multiples of 32"};
+    if (a.W % 64 == 0) {
+        dim3 grid((unsigned)(a.H * (a.W / 64)), a.Cout / 32, a.B);
+        hipLaunchKernelGGL(conv3x3_patch_kernel, grid, dim3(256), 0, st, a);
+        FPM_LAUNCH_CHECK();
+        return;
+    }
     const int P = a.H * a.W;
     // 64-pixel waves once there are enough of them to fill the chip, else 32
     const int64_t waves64 = (int64_t)((P + 63) / 64) * (a.Cout / 32) * a.B;

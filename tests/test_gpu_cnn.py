@@ -54,7 +54,8 @@ def _inputs(n, batch, seed):
     return f, t
 
 
-CASES = [(1, 8, 0), (3, 16, 0), (4, 32, 0), (1, 8, 5), (3, 16, 5), (4, 32, 5)]
+# (levels, n, pupil terms); n = 64 runs the 64-pixel-row patch convolution on its first level
+CASES = [(1, 8, 0), (3, 16, 0), (4, 32, 0), (2, 64, 0), (1, 8, 5), (3, 16, 5), (4, 32, 5)]
 
 
 @pytest.mark.parametrize("levels,n,pk", CASES)

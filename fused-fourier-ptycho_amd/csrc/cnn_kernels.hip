@@ -352,8 +352,11 @@ void wgrad3x3(const bf16_t* gz_t, const bf16_t* xt, float* gW, int B, int H, int
     if (!wgrad3x3_supported(W, Cin, Cout) || B < 1 || H < 1)
         throw HipError{hipErrorInvalidValue, "wgrad3x3: W a multiple of 8, channels multiples of 32"};
     const int64_t Q = (int64_t)B * H * W;
-    // pixel ranges of 32-pixel chunks: about 64 ranges per output tile
-    const int64_t ks = std::max<int64_t>(512, (Q / 64 + 31) / 32 * 32);
+    // pixel ranges (multiples of 32 pixels, at least 512) so that a launch has
+    // about 2048 waves: the 32 x 32 layers have only 3 tiles to spread
+    const int64_t tiles = (int64_t)(Cout / 32) * 3 * (Cin / 32);
+    const int64_t splits = std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, Q / 512));
+    const int64_t ks = std::max<int64_t>(512, ((Q + splits - 1) / splits + 31) / 32 * 32);
     dim3 grid(Cout / 32, 3 * (Cin / 32), (unsigned)((Q + ks - 1) / ks));
     hipLaunchKernelGGL(wgrad3x3_kernel, grid, dim3(64), 0, st, gz_t, xt, gW, H, W, Cin, Q, ks);
     FPM_LAUNCH_CHECK();

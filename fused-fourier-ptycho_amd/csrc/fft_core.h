@@ -223,10 +223,19 @@ template <int NLINES, int LEN> struct LineSlowMap {
     __device__ static __forceinline__ void split(int b, int NB, int& line, int& j) { line = b / NB; j = b % NB; }
     __device__ static __forceinline__ int lds(int line, int idx) { return line * PADLEN + idx + (idx >> 4); }
 };
+// With 8 lines a 16-lane group of an 8-byte LDS access touches two point
+// indices -- idx and idx + 1 in the reads, idx and idx + 16 in a first pass's
+// writes -- so the 8-line map has no pad and swaps the low bit of idx when bit
+// 4 is set: the two indices then fill disjoint halves of the 16 bank pairs.
+// Wider maps (16+ lines) keep one pad word per point index.
 template <int NLINES, int LEN> struct LineFastMap {
-    static constexpr int LDS_WORDS = LEN * (NLINES + 1);
+    static constexpr bool SWZ = NLINES == 8;
+    static constexpr int LDS_WORDS = SWZ ? LEN * NLINES : LEN * (NLINES + 1);
     __device__ static __forceinline__ void split(int b, int NB, int& line, int& j) { line = b % NLINES; j = b / NLINES; }
-    __device__ static __forceinline__ int lds(int line, int idx) { return idx * (NLINES + 1) + line; }
+    __device__ static __forceinline__ int lds(int line, int idx) {
+        if constexpr (SWZ) return ((idx & ~1) | ((idx ^ (idx >> 4)) & 1)) * NLINES + line;
+        else return idx * (NLINES + 1) + line;
+    }
 };
 
 // ------------------------------------------------------------------ engines

@@ -129,76 +129,6 @@ def run_cpu_baseline(ds, args):
                       f"{n}^2 x {ds['L']} -> {N}^2 workload, timed {dt:.2f} s, scaled x{steps} steps/epoch"}
 
 
-def run_full_scale_gradient(ds, args):
-    """Accuracy at the full workload: the fused gradient of the first
-    minibatch (args.batch images) at the initial object and pupil, from both
-    FFT engines, against the FP64 oracle on the host cores (and the FP32
-    oracle for scale)."""
-    import numpy as np
-    import torch
-
-    import fpm_amd
-
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-
-    n, N, B, r = ds["n"], ds["N"], args.batch, ds["pratio"]
-    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
-    gpu = {}
-    for name, eng in (("fp32_engine", fpm_amd.FFT_F32), ("split_engine", fpm_amd.FFT_SPLIT_F16)):
-        GO, GP = torch.empty_like(ds["O0_dev"]), torch.empty_like(ds["P0_dev"])
-        torch.cuda.synchronize()
-        fpm_amd.gradient_device(ds["O0_dev"].data_ptr(), ds["P0_dev"].data_ptr(), ds["raw_dev"].data_ptr(), True,
-                                ds["led_dev"].data_ptr(), (N, N), (n, n), B, r, GO.data_ptr(), GP.data_ptr(),
-                                engine=eng)
-        torch.cuda.synchronize()
-        # device layout is the transpose of the MATLAB orientation
-        gpu[name] = (np.asfortranarray(GO.cpu().numpy().T), np.asfortranarray(GP.cpu().numpy().T))
-        del GO, GP
-    raw = ds["raw_dev"][:B].cpu().numpy()
-    mn, mx = ds["raw_min"], ds["raw_max"]
-    Y = np.sqrt((raw.astype(np.float32) - np.float32(mn)) / np.float32(mx - mn)).astype(np.float32)
-    Y = np.asfortranarray(Y.transpose(2, 1, 0))
-    O0 = np.asfortranarray(ds["O0_dev"].cpu().numpy().T)
-    P0 = np.asfortranarray(ds["P0_dev"].cpu().numpy().T)
-    led = np.asfortranarray(ds["led"][:, :B])
-    t0 = time.perf_counter()
-    GO64, GP64, _, margin = oracle.gradient(O0, P0, Y, led, r, prec="f64")
-    t64 = time.perf_counter() - t0
-    GO32, GP32, _, _ = oracle.gradient(O0, P0, Y, led, r, prec="f32")
-    out = {name: {"G_O": rel(go, GO64), "G_P": rel(gp, GP64)} for name, (go, gp) in gpu.items()}
-    out["fp32_oracle"] = {"G_O": rel(GO32, GO64), "G_P": rel(GP32, GP64)}
-    out.update({"images": B, "at": "initial object and pupil", "fp64_oracle_s": t64,
-                "oracle_threads": oracle.num_threads(),
-                "min_sign_margin": margin,
-                "note": "FP64 restatement of the same gradient on the host cores; sign decisions of the loss "
-                        "stencil closer than FP32 rounding to a tie flip in any FP32 computation"})
-    return out
-
-
-def fp64_check(result):
-    """The FP64 oracle (host cores) run from the same warm start, schedule,
-    images, LED windows and number of epochs as the GPU's warm-started solve:
-    rel-L2 of the GPU object and pupil against it, and, for scale, of the
-    FP32 oracle (the same algorithm in single precision on the host)."""
-    import numpy as np
-
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-
-    O0, P0, raw, led, A, r, batch, opt, O_gpu, P_gpu, epochs = result
-    mn, mx = int(raw.min()), int(raw.max())
-    Y = np.asfortranarray(np.sqrt((raw.astype(np.float32) - np.float32(mn)) / np.float32(mx - mn)).astype(np.float32))
-    t0 = time.perf_counter()
-    Oo, Po, _ = oracle.solve(O0, P0, Y, led, A, r, epochs, batch, opt, prec="f64", halve_from=13)
-    t64 = time.perf_counter() - t0
-    O32, P32, _ = oracle.solve(O0, P0, Y, led, A, r, epochs, batch, opt, prec="f32", halve_from=13)
-    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
-    return {"O": rel(O_gpu, Oo), "P": rel(P_gpu, Po), "fp32_oracle_vs_fp64": {"O": rel(O32, Oo), "P": rel(P32, Po)},
-            "gpu_vs_fp32_oracle": {"O": rel(O_gpu, O32), "P": rel(P_gpu, P32)}, "oracle_s": t64,
-            "oracle_threads": oracle.num_threads(), "epochs": epochs}
-
-
 TTL_TEST_SEEDS = tuple(range(77000, 77008))
 
 
@@ -238,12 +168,9 @@ def run_time_to_loss(args):
     res = {}
     for name, eng in (("fp32", fpm_amd.FFT_F32), ("split", fpm_amd.FFT_SPLIT_F16)):
         per = []
-        for k, seed in enumerate(TTL_TEST_SEEDS):
-            rr = warmstart.time_to_loss(net, n, r, seed, warm_engine=eng, pupil_model=pupil,
-                                        keep_result=not args.no_ttl_fp64 and name == "fp32" and k == 0)
+        for seed in TTL_TEST_SEEDS:
+            rr = warmstart.time_to_loss(net, n, r, seed, warm_engine=eng, pupil_model=pupil)
             rr.pop("loss_per_epoch")
-            if "result" in rr:
-                rr["warm_rel_l2_vs_fp64_oracle"] = fp64_check(rr.pop("result"))
             per.append(rr)
             log(f"[bench] time to loss ({name} warm solve, object {seed}): ratio {rr['ratio_vs_plain']:.3f}, "
                 f"{rr['warm_epochs_run']} epochs, reached {rr['warm_reaches_plain_loss']}")
@@ -330,8 +257,6 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-sweep", default="16,26,32",
                     help="comma-separated batch sizes also timed (same epochs, s/epoch each); empty: none")
-    ap.add_argument("--no-full-check", action="store_true",
-                    help="skip the full-scale one-minibatch gradient check against the FP64 oracle (host cores)")
     ap.add_argument("--no-stages", action="store_true", help="time without per-stage events")
     ap.add_argument("--engine", choices=["f32", "split"], default="f32",
                     help="FFT radix-16 stages: FP32 butterflies or split-FP16 MFMA products")
@@ -339,9 +264,6 @@ def main():
                     help="capture each epoch as a hipGraph and replay it (per-stage events off)")
     ap.add_argument("--no-time-to-loss", action="store_true", help="skip the warm-start time-to-loss leg")
     ap.add_argument("--ttl-samples", type=int, default=4096, help="synthetic training objects of the warm start")
-    ap.add_argument("--no-ttl-fp64", action="store_true",
-                    help="skip the FP64 oracle run (host cores) from the warm start that the GPU's warm-started "
-                         "result is compared with")
     ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
     ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
     ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")
@@ -487,9 +409,6 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, one minibatch step)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = run_cpu_baseline(ds, args)
-    if not args.no_full_check:
-        print("[bench] full-scale gradient vs the FP64 oracle (both FFT engines)", file=sys.stderr, flush=True)
-        out["full_scale_gradient_vs_fp64"] = run_full_scale_gradient(ds, args)
     if not (args.no_time_to_loss and args.no_tiles):
         del ds, Yd, Od, Pd
         torch.cuda.empty_cache()

@@ -433,7 +433,7 @@ LOSS_RTOL = 1e-3
 
 def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: int = 20, batch: int = 26,
                  opt=(5e-4, 0.999, 1e-4), engine: int = 0, warm_engine: Optional[int] = None, led_grid: int = 19,
-                 repeats: int = 3, keep_result: bool = False, pupil_sigma: Optional[float] = PUPIL_SIGMA,
+                 repeats: int = 3, pupil_sigma: Optional[float] = PUPIL_SIGMA,
                  pupil_model: Optional[ParallaxPupil] = None):
     """Plain solve from the reference's init (upsampled centre image, ideal
     pupil, the reference's 20-epoch schedule) against the warm form: network
@@ -445,9 +445,7 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
     (hipGraph replay would capture its epoch graphs again in every call,
     which 2-7 epochs do not amortise).  engine / warm_engine: FFT
     engine of the plain / warm-started solve.  Times are the best of
-    `repeats` runs.  keep_result adds "result": the warm solve's inputs and
-    outputs as host arrays in MATLAB orientation (O0, P0, raw uint16 stack,
-    LED windows, A, pratio, batch, opt, O, P) and its epoch count."""
+    `repeats` runs."""
     warm_engine = engine if warm_engine is None else warm_engine
     import time
 
@@ -479,7 +477,6 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
     ttl = t_inf + t_warm
     e_init = phase_aligned_rel_l2(O_net.cpu().numpy(), O_true)
     e_plain = phase_aligned_rel_l2(O_plain.cpu().numpy(), O_true)
-    P0w = ds["P0_dev"] if P_net is None else P_net
     out = {
         "seed": seed, "plain_epochs": epochs, "plain_final_loss": target, "plain_s": t_plain,
         "warm_inference_s": t_inf, "warm_solve_s": t_warm, "warm_epochs_run": ran,
@@ -498,9 +495,4 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
         "warm_time_to_plain_error_s": t_inf if e_init <= e_plain else None,
         "loss_per_epoch": {"plain": [float(v) for v in loss_plain], "warm": [float(v) for v in loss_warm]},
     }
-    if keep_result:
-        # device layout is the transpose of the MATLAB orientation the oracle takes
-        t = lambda x: np.asfortranarray(x.cpu().numpy().T)  # noqa: E731
-        raw = np.asfortranarray(ds["raw_dev"].cpu().numpy().transpose(2, 1, 0))
-        out["result"] = (t(O_net), t(P0w), raw, ds["led"], ds["A"], pratio, batch, opt, t(O_warm), t(P_warm), ran)
     return out

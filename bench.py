@@ -193,10 +193,12 @@ def run_tiles(args):
     """Uneven field-of-view tiles: 48 tiles of 128 and 256 low-res pixels
     (upsample 4) over a 1024-pixel field, 19 x 19 LEDs seen from each tile's
     centre, per-tile Zernike pupils (sigma 0.4 rad), batch 26, reconstructed
-    jointly (one hipGraph per epoch, autotuned stream count; the autotuning
-    call's time includes its trial epochs and captures, so the chosen count
-    is also timed fixed) against the same tiles solved one after another
-    with fpm_pure_v3; the autotuned call again on the split-FP16 engine."""
+    jointly: the tiles of each size in shared launches (two launch sequences),
+    one hipGraph per epoch, autotuned stream count (the autotuning call's time
+    includes its trial epochs and captures, so the chosen count is also timed
+    fixed); the same with one launch sequence per tile; the grouped call on
+    the split-FP16 engine; and the tiles solved one after another with
+    fpm_pure_v3."""
     import torch
 
     import fpm_amd
@@ -206,25 +208,24 @@ def run_tiles(args):
     tl = ft.make_tile_set(1024, (128, 256), 7, 4, seed=11, count=48)
     t_gen = time.perf_counter() - t0
     ep, opt = args.tile_epochs, (args.lr, 0.999, 1e-4)
-    # warm-up call (first launches of every kernel shape), then the timed joint call
-    ft.tiles_solve(tl, 4, 1, args.batch, opt, streams=1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rep, _ = ft.tiles_solve(tl, 4, ep, args.batch, opt, streams=0)
-    torch.cuda.synchronize()
-    t_joint = time.perf_counter() - t0
-    # steady state: the chosen stream count fixed (no tuning epochs, one capture per epoch graph)
-    t0 = time.perf_counter()
-    ft.tiles_solve(tl, 4, ep, args.batch, opt, streams=rep["streams"])
-    torch.cuda.synchronize()
-    t_fixed = time.perf_counter() - t0
-    # the tiles configuration's split-precision transforms: same call on the split engine
+
+    def timed(**kw):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep, _ = ft.tiles_solve(tl, 4, ep, args.batch, opt, **kw)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, rep
+
+    # warm-up calls (first launches of every kernel shape), then the timed calls
+    for launch in (ft.TILES_GROUPED, ft.TILES_PER_TILE):
+        ft.tiles_solve(tl, 4, 1, args.batch, opt, streams=1, tile_launch=launch)
     ft.tiles_solve(tl, 4, 1, args.batch, opt, streams=1, engine=fpm_amd.FFT_SPLIT_F16)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rep_s, _ = ft.tiles_solve(tl, 4, ep, args.batch, opt, streams=0, engine=fpm_amd.FFT_SPLIT_F16)
-    torch.cuda.synchronize()
-    t_split = time.perf_counter() - t0
+    t_joint, rep = timed(streams=0)
+    # steady state: the chosen stream count fixed (no tuning epochs, one capture per epoch graph)
+    t_fixed, _ = timed(streams=rep["streams"])
+    t_pt, rep_pt = timed(streams=0, tile_launch=ft.TILES_PER_TILE)
+    t_pt_fixed, _ = timed(streams=rep_pt["streams"], tile_launch=ft.TILES_PER_TILE)
+    t_split, rep_s = timed(streams=0, engine=fpm_amd.FFT_SPLIT_F16)
     t0 = time.perf_counter()
     for t in tl:
         N, n = t["O0"].shape[0], t["P0"].shape[0]
@@ -233,13 +234,24 @@ def run_tiles(args):
                              t["O"].data_ptr(), t["P"].data_ptr())
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
+    # dense algorithmic bytes of one epoch of all tiles (the byte model of the
+    # single solve: 104 N^2 per step, 138 n^2 per LED with uint16 images)
+    L = int(tl[0]["Y"].shape[0])
+    steps = (L + args.batch - 1) // args.batch
+    epoch_bytes = sum(steps * 104.0 * t["O0"].shape[0] ** 2 + L * 138.0 * t["P0"].shape[0] ** 2 for t in tl)
     return {"workload": "48 tiles of 128/256 px (upsample 4) over a 1024 px field, 361 LEDs per tile seen from its "
                         f"centre, per-tile pupils, batch {args.batch}, {ep} epochs",
-            "joint_s_per_epoch": t_joint / ep, "joint_fixed_streams_s_per_epoch": t_fixed / ep,
-            "joint_split_engine_s_per_epoch": t_split / ep, "split_streams_chosen": rep_s["streams"],
-            "sequential_s_per_epoch": t_seq / ep, "speedup": t_seq / t_joint, "speedup_fixed_streams": t_seq / t_fixed,
-            "streams_chosen": rep["streams"],
-            "autotune_epoch_ms": rep["candidates"], "generate_s": t_gen,
+            "grouped": {"s_per_epoch": t_joint / ep, "fixed_streams_s_per_epoch": t_fixed / ep,
+                        "streams_chosen": rep["streams"], "autotune_epoch_ms": rep["candidates"],
+                        "sequences": rep["sequences"], "grouped_tiles": rep["grouped_tiles"],
+                        "epoch_model_GBps": epoch_bytes / 1e9 / (t_fixed / ep)},
+            "per_tile": {"s_per_epoch": t_pt / ep, "fixed_streams_s_per_epoch": t_pt_fixed / ep,
+                         "streams_chosen": rep_pt["streams"], "autotune_epoch_ms": rep_pt["candidates"],
+                         "sequences": rep_pt["sequences"]},
+            "grouped_split_engine_s_per_epoch": t_split / ep, "split_streams_chosen": rep_s["streams"],
+            "sequential_s_per_epoch": t_seq / ep, "speedup_grouped_vs_sequential": t_seq / t_joint,
+            "speedup_grouped_vs_per_tile": t_pt / t_joint, "epoch_model_GB": epoch_bytes / 1e9,
+            "generate_s": t_gen,
             "note": "tile sides are powers of two; 192 px tiles need a radix-3 transform (not built)"}
 
 

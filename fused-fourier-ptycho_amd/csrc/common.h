@@ -110,6 +110,20 @@ inline PageLayout tiled_layout(uint32_t cols) {
     return l;
 }
 
+// Grouped launches of a multi-tile solve: the tiles of one size share every
+// launch, tile k owns block k of each working array (its pages, lines or
+// row-mask bytes, in tile order) and reads its own inputs through a device
+// table of these entries.
+struct TileIn {
+    const int2* led;     // LED windows (2 x L int32, column-major)
+    const void* y;       // image stack (the dtype of the group's YStack)
+    uint32_t ymn;        // uint16 stack: raw minimum
+    float yrange, yrcp;  // uint16 stack: (max - min) and its reciprocal
+    uint32_t pad;
+    const float* A;      // pupil mask
+    double* loss;        // running loss of the epoch, or null
+};
+
 // error plumbing (thread-local last error text)
 void set_error(const std::string& msg);
 const char* get_error();

@@ -21,9 +21,11 @@
 namespace fpm {
 
 // ------------------------------------------------------- contiguous pass
+constexpr int contig_threads(int len) { return len >= 16384 ? 1024 : (len >= 8192 ? 512 : 256); }
+constexpr int contig_lines(int len) { return contig_threads(len) * 16 / len; }
 template <int LEN> struct ContigCfg {
-    static constexpr int T = LEN >= 16384 ? 1024 : (LEN >= 8192 ? 512 : 256);
-    static constexpr int LINES = T * 16 / LEN;
+    static constexpr int T = contig_threads(LEN);
+    static constexpr int LINES = contig_lines(LEN);
     using Map = LineSlowMap<LINES, LEN>;
 };
 
@@ -80,6 +82,9 @@ struct StridedArgs {
     uint32_t in_g, in_s, out_g, out_s;
     uint32_t tw_step;
     const uint8_t* rowmask;  // optional: skip row tiles whose 16-row entries are all 0
+    // grouped tiles: page p's mask at rowmask + (p / mask_ppt) * mask_stride
+    // (mask_ppt 0: one mask for every page)
+    uint32_t mask_ppt, mask_stride;
 };
 
 static void set_layout(StridedArgs& a, const PageLayout& l) {
@@ -139,9 +144,10 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     const uint32_t tile = SRC == SRC_CROP ? blockIdx.z : blockIdx.x;
     const uint32_t page = SRC == SRC_CROP ? blockIdx.x : blockIdx.z;
     if (a.rowmask) {  // rows outside every LED window of the batch: nothing to do
+        const uint8_t* rmk = a.rowmask + (a.mask_ppt ? (size_t)(page / a.mask_ppt) * a.mask_stride : 0);
         bool any = false;
         for (uint32_t k = (tile * TR) / kMaskRows; k <= (tile * TR + TR - 1) / kMaskRows; ++k)
-            any |= a.rowmask[k] != 0;
+            any |= rmk[k] != 0;
         if (!any) return;  // uniform over the workgroup
     }
     tw = bind_twiddles(tw, twl, t, Cfg::T);
@@ -156,10 +162,34 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     const size_t roff = (size_t)(row >> a.sh) * a.tstride + (row & a.rmask);
     float2 v[16];
     uint32_t kl = 0, kt = 0;
+    // crop source of this page: its tile's spectrum, pupil and LED (grouped
+    // tiles: page k ppt + l is LED led0 + l of tile k)
+    const float2* __restrict__ cS = c.Shat;
+    const float2* __restrict__ cP = c.P;
     if constexpr (SRC == SRC_CROP) {
-        const int2 k = c.led[c.led0 + page];
+        const int2* cled = c.led;
+        uint32_t lp = page;
+        if (c.tab) {
+            const uint32_t tk = page / c.ppt;
+            lp = page - tk * c.ppt;
+            cS += (size_t)tk * c.Nr * c.Nc;
+            cP += (size_t)tk * c.nr * c.nc;
+            cled = c.tab[tk].led;
+        }
+        const int2 k = cled[c.led0 + lp];
         kl = (uint32_t)k.x;
         kt = (uint32_t)k.y;
+    }
+    // stitch sources (grouped tiles: page k is tile k's gradient)
+    const float2* __restrict__ sR = sa.R;
+    const float2* __restrict__ sP = sa.P;
+    const int2* __restrict__ sled = sa.led;
+    if constexpr (SRC == SRC_STITCH) {
+        if (sa.tab) {
+            sR += (size_t)page * sa.b * sa.nr * sa.nc;
+            sP += (size_t)page * sa.nr * sa.nc;
+            sled = sa.tab[page].led;
+        }
     }
     // stitch: window corners of the batch and this thread's row mask
     uint64_t rowhits = 0;
@@ -168,7 +198,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     if constexpr (SRC == SRC_STITCH) {
         const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
         for (uint32_t l = t; l < sa.b; l += Cfg::T) {
-            const int2 k = sa.led[sa.led0 + l];
+            const int2 k = sled[sa.led0 + l];
             const uint32_t wr = ((uint32_t)k.y - 1u + (sa.Nr >> 1)) & Rm;
             const uint32_t wc = ((uint32_t)k.x - 1u + (sa.Nc >> 1)) & Cm;
             win[l] = wr | (wc << 16);
@@ -202,12 +232,12 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 const uint32_t li = (srow - (w & 0xffffu)) & Rm;
                 const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
                 const uint32_t d = (col0 - (w >> 16)) & Cm;
-                const float2* __restrict__ Rl = sa.R + (size_t)l * plane + sa.nl.row_off(di);
+                const float2* __restrict__ Rl = sR + (size_t)l * plane + sa.nl.row_off(di);
 #pragma unroll
                 for (int s = 0; s < SL::GS0; ++s) {
                     const uint32_t lj = (d + (uint32_t)s * D) & Cm;
                     if (lj < sa.nc) {
-                        const float2 pp = sa.P[lj * sa.nr + li];
+                        const float2 pp = sP[lj * sa.nr + li];
                         if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0; R not read
                             const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
                             const float2 gw = cmulc(Rl[(size_t)sa.nl.perm(dj) * sa.nl.ld], pp);
@@ -231,9 +261,9 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                     const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
                     const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
                     const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
-                    const float2 pp = c.P[j * c.nr + i];
+                    const float2 pp = cP[j * c.nr + i];
                     float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
-                    if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(c.Shat[(size_t)c.pN(scl) * c.Nr + srw], pp);
+                    if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
                     v[q * SL::GS0 + s] = z;
                 } else {
                     v[q * SL::GS0 + s] = in[roff + col * a.ld];
@@ -367,12 +397,15 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // forward column transform(s): natural columns in `data` -> permuted (in
 // place) or natural (into nat_out) frequency columns
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
-                      hipStream_t st, float2* nat_out, const uint8_t* rowmask, const PageLayout* lay, int eng) {
+                      hipStream_t st, float2* nat_out, const uint8_t* rowmask, const PageLayout* lay, int eng,
+                      uint32_t mask_ppt) {
     Twiddles tw{twid};
     StridedArgs a{};
     set_layout(a, lay ? *lay : colmajor_layout(rows, p));
     a.page = (uint64_t)rows * cols;
     a.rowmask = rowmask;
+    a.mask_ppt = mask_ppt;
+    a.mask_stride = rows / kMaskRows;
     const uint32_t tiles = rows / kStridedRows;
     if (p.M2 == 1) {
         a.in = data;
@@ -407,16 +440,20 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
 // columns -> natural columns in `data`
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, const float2* nat_in, const CropArgs* crop, const StitchArgs* stitch,
-                      const uint8_t* rowmask, const PageLayout* lay, int eng) {
+                      const uint8_t* rowmask, const PageLayout* lay, int eng, uint32_t mask_ppt) {
     Twiddles tw{twid};
     StridedArgs a{};
     set_layout(a, lay ? *lay : colmajor_layout(rows, p));
     a.page = (uint64_t)rows * cols;
     a.rowmask = rowmask;
+    a.mask_ppt = mask_ppt;
+    a.mask_stride = rows / kMaskRows;
     const uint32_t tiles = rows / kStridedRows;
     Gather gs{crop, stitch};
-    if (stitch && (stitch->b > (uint32_t)kMaxStitchLeds || batch != 1))
-        throw HipError{hipErrorInvalidValue, "fused stitch supports one page and at most 64 LEDs"};
+    if (stitch && (stitch->b > (uint32_t)kMaxStitchLeds || (batch != 1 && !stitch->tab)))
+        throw HipError{hipErrorInvalidValue, "fused stitch supports one page per tile and at most 64 LEDs"};
+    if (crop && crop->tab && (crop->ppt == 0 || batch % crop->ppt != 0))
+        throw HipError{hipErrorInvalidValue, "grouped crop: the batch must be a multiple of the pages per tile"};
     // crop and stitch sources are addressed by natural column, like nat_in
     if (crop || stitch) nat_in = data;
     if (p.M2 == 1) {
@@ -503,12 +540,15 @@ template <bool NT> __device__ __forceinline__ void st_s(float* p, float v) {
     else *p = v;
 }
 
+// lines_per_tile > 0 (grouped tiles): the lines are several objects' back to
+// back, lines_per_tile each, with one row mask per object; a workgroup's
+// lines lie within one object (fused_update_groupable).
 template <int LEN, bool FWD, int ENG>
 __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2* __restrict__ G,
                                                                         float2* __restrict__ O,
                                                                         float* __restrict__ mO, uint32_t nlines,
                                                                         RmsArgs rms, const uint8_t* __restrict__ rowmask,
-                                                                        Twiddles tw) {
+                                                                        uint32_t lines_per_tile, Twiddles tw) {
     using Cfg = ContigCfg<LEN>;
     using Map = typename Cfg::Map;
     using SI = Slots<LEN, true, Cfg::T, Map, ENG>;
@@ -520,12 +560,14 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
     // batch): not read, taken as 0 -- the column passes skipped them
     __shared__ uint8_t rm[LEN / kMaskRows];
     const int t = threadIdx.x;
-    for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rowmask ? rowmask[k] : 1;
+    const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
+    const uint8_t* rmk = rowmask;
+    if (rmk && lines_per_tile) rmk += (line0 / lines_per_tile) * (size_t)(LEN / kMaskRows);
+    for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rmk ? rmk[k] : 1;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     EngCtx<ENG> ctx;
     ctx.init();
     const float lr = rms_lr(rms);
-    const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
     float2 v[16];
 #pragma unroll
     for (int q = 0; q < SI::NG0; ++q)
@@ -578,18 +620,19 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
 
 template <int ENG>
 static void launch_fused_update_eng(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                                    const uint8_t* rowmask, const float2* twid, hipStream_t st) {
+                                    const uint8_t* rowmask, const float2* twid, hipStream_t st, uint32_t ntiles) {
     Twiddles tw{twid};
-    const uint32_t nlines = g.Nc;
+    const uint32_t nlines = g.Nc * ntiles;
+    const uint32_t lpt = ntiles > 1 ? g.Nc : 0u;
 #define FPM_FUSED_CASE(L)                                                                                   \
     case L: {                                                                                               \
         dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                                \
         if (forward_after)                                                                                  \
             hipLaunchKernelGGL((fused_update_kernel<L, true, ENG>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
-                               rms, rowmask, tw);                                                           \
+                               rms, rowmask, lpt, tw);                                                      \
         else                                                                                                \
             hipLaunchKernelGGL((fused_update_kernel<L, false, ENG>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
-                               rms, rowmask, tw);                                                           \
+                               rms, rowmask, lpt, tw);                                                      \
         break;                                                                                              \
     }
     switch (g.Nr) {
@@ -611,10 +654,16 @@ static void launch_fused_update_eng(const Geom& g, float2* G, float2* O, float* 
 }
 
 void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng) {
-    if (eng == ENG_SPLIT16) launch_fused_update_eng<ENG_SPLIT16>(g, G, O, mO, rms, forward_after, rowmask, twid, st);
-    else launch_fused_update_eng<ENG_F32>(g, G, O, mO, rms, forward_after, rowmask, twid, st);
+                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng, uint32_t ntiles) {
+    if (ntiles > 1 && !fused_update_groupable(g))
+        throw HipError{hipErrorInvalidValue, "grouped fused update: a workgroup would span two objects"};
+    if (eng == ENG_SPLIT16)
+        launch_fused_update_eng<ENG_SPLIT16>(g, G, O, mO, rms, forward_after, rowmask, twid, st, ntiles);
+    else
+        launch_fused_update_eng<ENG_F32>(g, G, O, mO, rms, forward_after, rowmask, twid, st, ntiles);
 }
+
+bool fused_update_groupable(const Geom& g) { return g.Nc % (uint32_t)contig_lines((int)g.Nr) == 0; }
 
 // ------------------------ per-LED contiguous passes fused with the stencil
 // One workgroup owns K = LT - 2 interior columns of one LED page plus the
@@ -637,12 +686,16 @@ template <int LEN> struct StencilCfg {
     using Map = LineSlowMap<LT, LEN>;
 };
 
+// tab (grouped tiles): page k ppt + l is LED led0 + l of tile k, with tile
+// k's images, normalisation and loss slot from tab[k] and its row mask at
+// rowmask + k LEN/16.
 template <int LEN, int ENG>
 __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,
                                                             float inv_ratio, float gscale,
                                                             double* __restrict__ loss_acc,
-                                                            const uint8_t* __restrict__ rowmask, Twiddles tw) {
+                                                            const uint8_t* __restrict__ rowmask, Twiddles tw,
+                                                            const TileIn* __restrict__ tab, uint32_t ppt) {
     using Cfg = StencilCfg<LEN>;
     using Map = typename Cfg::Map;
     using SI = Slots<LEN, true, Cfg::T, Map, ENG>;
@@ -655,17 +708,29 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
     // spectrum rows known to be zero (pupil rows without support): not read
     __shared__ uint8_t rm[LEN / kMaskRows];
     const int t = threadIdx.x;
+    const uint32_t page = blockIdx.y;
+    uint32_t lpage = page;
+    if (tab) {
+        const uint32_t tk = page / ppt;
+        lpage = page - tk * ppt;
+        const TileIn ti = tab[tk];
+        y.data = ti.y;
+        y.mn = ti.ymn;
+        y.range = ti.yrange;
+        y.rcp_range = ti.yrcp;
+        loss_acc = ti.loss;
+        if (rowmask) rowmask += (size_t)tk * (LEN / kMaskRows);
+    }
     for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rowmask ? rowmask[k] : 1;
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     EngCtx<ENG> ctx;
     ctx.init();
-    const uint32_t page = blockIdx.y;
     const uint32_t j0 = blockIdx.x * Cfg::K;  // first interior column; line l is column j0 - 1 + l
     const uint32_t cmask = g.nc - 1;
     const size_t plane = (size_t)g.nr * g.nc;
     const float2* __restrict__ src = in + (size_t)page * plane;
     float2* __restrict__ dst = out + (size_t)page * plane;
-    const size_t ybase = (size_t)(led0 + page) * plane;
+    const size_t ybase = (size_t)(led0 + lpage) * plane;
     auto column = [&](int line) -> uint32_t { return (j0 - 1u + (uint32_t)line) & cmask; };
     auto interior = [&](int line) -> bool { return line >= 1 && line <= Cfg::K && j0 + (uint32_t)line - 1u < g.nc; };
     // space-domain pages have natural columns in either layout
@@ -756,7 +821,7 @@ bool stencil_fused_supported(const Geom& g) { return g.nr <= 2048; }
 template <int ENG>
 static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b,
                                      int pratio, float2* u_out, double* loss_acc, const uint8_t* rowmask,
-                                     const float2* twid, hipStream_t st) {
+                                     const float2* twid, hipStream_t st, const TileIn* tab, uint32_t ppt) {
     Twiddles tw{twid};
     const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
     const float gscale = (float)(pratio * pratio);
@@ -765,7 +830,7 @@ static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const 
     case L: {                                                                                             \
         dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                                   \
         hipLaunchKernelGGL((stencil_fused_kernel<L, ENG>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, led0, ratio, \
-                           inv_ratio, gscale, loss_acc, rowmask, tw);                                      \
+                           inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt);                            \
         break;                                                                                            \
     }
     switch (g.nr) {
@@ -785,11 +850,15 @@ static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const 
 
 void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
                           float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
-                          hipStream_t st, int eng) {
+                          hipStream_t st, int eng, const TileIn* tab, uint32_t ppt) {
+    if (tab && (ppt == 0 || b % ppt != 0))
+        throw HipError{hipErrorInvalidValue, "grouped stencil: the batch must be a multiple of the pages per tile"};
+    if (b > 65535) throw HipError{hipErrorInvalidValue, "fused stencil: at most 65535 pages per launch"};
     if (eng == ENG_SPLIT16)
-        launch_stencil_fused_eng<ENG_SPLIT16>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st);
+        launch_stencil_fused_eng<ENG_SPLIT16>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st, tab,
+                                              ppt);
     else
-        launch_stencil_fused_eng<ENG_F32>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st);
+        launch_stencil_fused_eng<ENG_F32>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st, tab, ppt);
 }
 
 }  // namespace fpm

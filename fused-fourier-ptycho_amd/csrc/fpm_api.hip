@@ -505,6 +505,158 @@ struct Solve {
     int parity_after(int cur) const { return (int)((cur + steps) & 1u); }
 };
 
+// Tiles of one size and image type run through shared launches: every stage
+// of a minibatch step takes all T tiles at once -- T b pages for the per-LED
+// stages (tile k's pages k b..), T pages (column passes, stitch) or T N_cols
+// lines (fused update) for the object side.  Tile k owns block k of every
+// working array (O, P, both spectra, moments, pages, row masks) and reads its
+// images, LED windows, pupil mask and loss slot through a device table.  Each
+// element is computed with the arithmetic of the one-tile Solve, so every
+// tile's result is the same bits as its own solve.
+struct GroupSolve {
+    Geom g{};
+    uint32_t T = 0, L = 0, B = 0, steps = 0, ne = 0;
+    int pr = 1;
+    const float2* tw = nullptr;
+    YStack y{};  // image type; each tile's stack and constants are in the table
+    RmsArgs rms{};
+    SolveState ss{};
+    DevBuf O, P, S0, S1, mO, mP, lat, rec, masks, state, tab;
+    std::vector<std::unique_ptr<DevBuf>> ybuf;
+    std::vector<TileIn> htab;
+    float2* spec[2] = {nullptr, nullptr};
+    uint8_t* mk[2] = {nullptr, nullptr};
+    uint8_t* pmask = nullptr;  // T pupil-support row masks (n side)
+    uint8_t* psup = nullptr;   // T pupil-support row sets
+    const TileIn* tabd = nullptr;
+
+    // tiles of one group can share launches
+    static bool groupable(const Geom& g, uint32_t B, int stitch) {
+        return stitch == FPM_STITCH_OWNER && B <= (uint32_t)64 && stencil_fused_supported(g) &&
+               fused_update_groupable(g);
+    }
+
+    void batch_of(uint32_t s, uint32_t& led0, uint32_t& b) const {
+        led0 = s * B;
+        b = std::min<uint32_t>(B, L - led0);
+    }
+
+    void init(const fpm_tile* tiles, const std::vector<int>& members, int ne_, float lr, float beta, float eps,
+              float clamp, int32_t halve_from, hipStream_t st) {
+        T = (uint32_t)members.size();
+        ne = (uint32_t)ne_;
+        if ((uint64_t)T * B > 65535u) invalid("a tile group holds more than 65535 pages per step");
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        steps = (L + B - 1) / B;
+        bool want_loss = false;
+        for (int t : members) want_loss |= tiles[t].loss_per_epoch != nullptr;
+        O.alloc(T * NN * sizeof(float2));
+        P.alloc(T * nn * sizeof(float2));
+        S0.alloc(T * NN * sizeof(float2));
+        S1.alloc(T * NN * sizeof(float2));
+        mO.alloc(T * NN * sizeof(float));
+        mP.alloc(T * nn * sizeof(float));
+        lat.alloc((size_t)T * B * nn * sizeof(float2));
+        rec.alloc((size_t)T * B * nn * sizeof(float2));
+        const size_t mrows = g.Nr / kMaskRows, prows = g.nr / kMaskRows;
+        masks.alloc(T * (2 * mrows + prows + g.nr));
+        mk[0] = masks.as<uint8_t>();
+        mk[1] = mk[0] + T * mrows;
+        pmask = mk[1] + T * mrows;
+        psup = pmask + T * prows;
+        spec[0] = S0.as<float2>();
+        spec[1] = S1.as<float2>();
+        state.alloc(64 + (want_loss ? (size_t)T * (1 + ne) * sizeof(double) : 0));
+        ss = SolveState{};
+        ss.lr = (float*)(state.as<char>() + 8);
+        ss.epoch = (uint32_t*)(state.as<char>() + 16);
+        ss.loss_cur = want_loss ? (double*)(state.as<char>() + 64) : nullptr;
+        ss.loss_hist = want_loss ? ss.loss_cur + T : nullptr;
+        ss.halve_from = halve_from > 0 ? (uint32_t)halve_from : 13u;
+        ss.ntiles = T;
+        ss.hist_stride = ne;
+        htab.resize(T);
+        for (uint32_t k = 0; k < T; ++k) {
+            const fpm_tile& tl = tiles[members[k]];
+            FPM_HIP(hipMemcpyAsync(O.as<float2>() + k * NN, tl.O0, NN * sizeof(float2), hipMemcpyDeviceToDevice, st));
+            FPM_HIP(hipMemcpyAsync(P.as<float2>() + k * nn, tl.P0, nn * sizeof(float2), hipMemcpyDeviceToDevice, st));
+            ybuf.push_back(std::make_unique<DevBuf>());
+            const YStack yk = prepare_y(tl.Y, tl.y_dtype, nn * (size_t)L, true, *ybuf.back(), st);
+            y.u16 = yk.u16;
+            htab[k] = TileIn{(const int2*)tl.led_idx, yk.data, yk.mn, yk.range, yk.rcp_range, 0u, tl.A,
+                             (want_loss && tl.loss_per_epoch) ? ss.loss_cur + k : nullptr};
+        }
+        tab.alloc(T * sizeof(TileIn));
+        FPM_HIP(hipMemcpyAsync(tab.p, htab.data(), T * sizeof(TileIn), hipMemcpyHostToDevice, st));
+        tabd = tab.as<TileIn>();
+        FPM_HIP(hipMemsetAsync(mO.p, 0, T * NN * sizeof(float), st));
+        FPM_HIP(hipMemsetAsync(mP.p, 0, T * nn * sizeof(float), st));
+        launch_solve_state_init(ss, lr, st);
+        rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
+        for (uint32_t k = 0; k < T; ++k)
+            launch_pupil_row_mask(P.as<float2>() + k * nn, tiles[members[k]].A, g.nr, g.nc, pmask + k * prows,
+                                  psup + (size_t)k * g.nr, st);
+        uint32_t led0, b;
+        batch_of(0, led0, b);
+        launch_row_mask(nullptr, led0, b, g.Nr, g.nr, psup, mk[0], st, tabd, T);
+        fft_contig(O.as<float2>(), spec[0], g.Nr, g.Nc * T, false, tw, st, g.eng);
+        fft_cols_forward(spec[0], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, mk[0], nullptr, g.eng, 1);
+    }
+
+    // the launches of Solve::enqueue_epoch + minibatch_backward (update
+    // mode, fused stencil, owner stitch inside the column pass), for all tiles
+    void enqueue_epoch(int cur, bool last_epoch, hipStream_t st) {
+        const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
+        float2* Pw = P.as<float2>();
+        float2* lp = lat.as<float2>();
+        float2* rp = rec.as<float2>();
+        uint32_t led0, b;
+        for (uint32_t s = 0; s < steps; ++s) {
+            batch_of(s, led0, b);
+            const uint32_t pages = T * b;
+            const CropArgs ca{spec[cur], Pw, nullptr, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN, tabd, b};
+            fft_cols_inverse(lp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, &ca, nullptr, pmask, &g.nl, g.eng, b);
+            launch_stencil_fused(g, lp, y, led0, pages, pr, rp, nullptr, pmask, tw, st, g.eng, tabd, b);
+            fft_cols_forward(rp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, pmask, &g.nl, g.eng, b);
+            const StitchArgs sa{rp, Pw, nullptr, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.nl, invN2, tabd};
+            fft_cols_inverse(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, nullptr, &sa, mk[cur], nullptr, g.eng,
+                             1);
+            launch_backproj(g, rp, spec[cur], nullptr, led0, b, Pw, spec[1 - cur], kBackprojPupil, mP.as<float>(),
+                            nullptr, rms, nullptr, st, tabd, T);
+            const bool last = last_epoch && (s == steps - 1);
+            if (!last) {
+                uint32_t nled0, nb;
+                batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
+                launch_row_mask(nullptr, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st, tabd, T);
+            }
+            launch_fused_update(g, spec[1 - cur], O.as<float2>(), mO.as<float>(), rms, !last, mk[cur], tw, st, g.eng,
+                                T);
+            if (!last) {
+                fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng, 1);
+                cur = 1 - cur;
+            }
+        }
+        launch_epoch_end(ss, st);
+    }
+
+    // results (and loss histories) to the callers' arrays
+    void finish(const fpm_tile* tiles, const std::vector<int>& members, int epochs, hipStream_t st) {
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        for (uint32_t k = 0; k < T; ++k) {
+            const fpm_tile& tl = tiles[members[k]];
+            FPM_HIP(hipMemcpyAsync(tl.O_out, O.as<float2>() + k * NN, NN * sizeof(float2), hipMemcpyDeviceToDevice,
+                                   st));
+            FPM_HIP(hipMemcpyAsync(tl.P_out, P.as<float2>() + k * nn, nn * sizeof(float2), hipMemcpyDeviceToDevice,
+                                   st));
+            if (tl.loss_per_epoch && ss.loss_hist)
+                FPM_HIP(hipMemcpyAsync(tl.loss_per_epoch, ss.loss_hist + (size_t)k * ne, (size_t)epochs * sizeof(double),
+                                       hipMemcpyDeviceToHost, st));
+        }
+    }
+
+    int parity_after(int cur) const { return (int)((cur + steps) & 1u); }
+};
+
 }  // namespace
 }  // namespace fpm
 
@@ -659,12 +811,16 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
         if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
         if (o.lr_halve_from < 0) invalid("lr_halve_from must be >= 0");
         if (o.stop_loss != 0.0 || o.epochs_run) invalid("stop_loss / epochs_run are options of fpm_pure_v3 only");
+        if (o.tile_launch != FPM_TILES_GROUPED && o.tile_launch != FPM_TILES_PER_TILE)
+            invalid("tile_launch must be FPM_TILES_GROUPED or FPM_TILES_PER_TILE");
         const int eng = engine_of(o.fft_engine);
         std::vector<Geom> geo(ntiles);
         for (int t = 0; t < ntiles; ++t) {
             const fpm_tile& tl = tiles[t];
             if (!tl.O0 || !tl.O_out || !tl.P0 || !tl.P_out || !tl.Y || !tl.led_idx || !tl.A)
                 invalid("tile " + std::to_string(t) + ": null array argument");
+            if (tl.y_dtype != FPM_Y_F32_AMPLITUDE && tl.y_dtype != FPM_Y_U16_RAW)
+                invalid("tile " + std::to_string(t) + ": y_dtype must be FPM_Y_F32_AMPLITUDE or FPM_Y_U16_RAW");
             geo[t] = make_geom(tl.N, tl.N, tl.n, tl.n);
             geo[t].eng = eng;
         }
@@ -677,10 +833,65 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
         hipStream_t st = sh.s;
         const uint32_t B = (uint32_t)std::min<int64_t>(bs, L);
         StageTimer none(nullptr, st);
-        std::vector<std::unique_ptr<Solve>> sv(ntiles);
-        std::vector<std::unique_ptr<DevBuf>> ybuf(ntiles);
-        std::vector<double> work(ntiles);
-        for (int t = 0; t < ntiles; ++t) {
+
+        // launch sequences: groups of tiles of one size and image type (two or
+        // more, groupable), and single tiles
+        std::vector<std::vector<int>> groups;
+        std::vector<int> singles;
+        {
+            std::map<std::pair<int64_t, int32_t>, std::vector<int>> by;
+            for (int t = 0; t < ntiles; ++t) {
+                if (o.tile_launch == FPM_TILES_GROUPED && GroupSolve::groupable(geo[t], B, o.stitch))
+                    by[{tiles[t].n, tiles[t].y_dtype}].push_back(t);
+                else
+                    singles.push_back(t);
+            }
+            for (auto& kv : by) {
+                if (kv.second.size() >= 2) groups.push_back(kv.second);
+                else singles.push_back(kv.second[0]);
+            }
+        }
+        std::vector<std::unique_ptr<GroupSolve>> gsv;
+        std::vector<std::unique_ptr<Solve>> sv;
+        std::vector<std::unique_ptr<DevBuf>> ybuf;
+        struct Seq {
+            std::function<void(int, bool, hipStream_t)> enqueue;
+            double work;
+        };
+        std::vector<Seq> seqs;
+        auto tile_work = [&](int t) {
+            const Geom& g = geo[t];
+            return (double)g.Nr * g.Nc + (double)B * g.nr * g.nc;
+        };
+        int grouped_tiles = 0;
+        for (const auto& members : groups) {
+            gsv.push_back(std::make_unique<GroupSolve>());
+            GroupSolve& gs = *gsv.back();
+            gs.g = geo[members[0]];
+            gs.L = (uint32_t)L;
+            gs.B = B;
+            gs.pr = pr;
+            gs.tw = tw;
+            double w = 0.0;
+            for (int t : members) w += tile_work(t);
+            if (ne > 0) {
+                gs.init(tiles, members, ne, lr, beta, eps, o.pupil_clamp, o.lr_halve_from, st);
+                seqs.push_back({[&gs](int cur, bool last, hipStream_t s) { gs.enqueue_epoch(cur, last, s); }, w});
+            } else {
+                for (int t : members) {
+                    const fpm_tile& tl = tiles[t];
+                    const Geom& g = geo[t];
+                    if ((const void*)tl.O0 != (void*)tl.O_out)
+                        FPM_HIP(hipMemcpyAsync(tl.O_out, tl.O0, (size_t)g.Nr * g.Nc * sizeof(float2),
+                                               hipMemcpyDeviceToDevice, st));
+                    if ((const void*)tl.P0 != (void*)tl.P_out)
+                        FPM_HIP(hipMemcpyAsync(tl.P_out, tl.P0, (size_t)g.nr * g.nc * sizeof(float2),
+                                               hipMemcpyDeviceToDevice, st));
+                }
+            }
+            grouped_tiles += (int)members.size();
+        }
+        for (int t : singles) {
             const fpm_tile& tl = tiles[t];
             const Geom& g = geo[t];
             const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
@@ -688,11 +899,11 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
                 FPM_HIP(hipMemcpyAsync(tl.O_out, tl.O0, NN * sizeof(float2), hipMemcpyDeviceToDevice, st));
             if ((const void*)tl.P0 != (void*)tl.P_out)
                 FPM_HIP(hipMemcpyAsync(tl.P_out, tl.P0, nn * sizeof(float2), hipMemcpyDeviceToDevice, st));
-            ybuf[t] = std::make_unique<DevBuf>();
-            sv[t] = std::make_unique<Solve>();
-            Solve& s = *sv[t];
+            ybuf.push_back(std::make_unique<DevBuf>());
+            sv.push_back(std::make_unique<Solve>());
+            Solve& s = *sv.back();
             s.g = g;
-            s.y = prepare_y(tl.Y, tl.y_dtype, nn * (size_t)L, true, *ybuf[t], st);
+            s.y = prepare_y(tl.Y, tl.y_dtype, nn * (size_t)L, true, *ybuf.back(), st);
             s.led = (const int2*)tl.led_idx;
             s.A = tl.A;
             s.O = (float2*)tl.O_out;
@@ -702,24 +913,35 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
             s.pr = pr;
             s.stitch = o.stitch;
             s.tw = tw;
-            if (ne > 0) s.init(ne, tl.loss_per_epoch != nullptr, lr, beta, eps, o.pupil_clamp, o.lr_halve_from, st, none);
-            work[t] = (double)NN + (double)B * (double)nn;
+            if (ne > 0) {
+                s.init(ne, tl.loss_per_epoch != nullptr, lr, beta, eps, o.pupil_clamp, o.lr_halve_from, st, none);
+                seqs.push_back(
+                    {[&s, &none](int cur, bool last, hipStream_t x) { s.enqueue_epoch(cur, last, x, none); },
+                     tile_work(t)});
+            }
         }
-        if (report) std::memset(report, 0, sizeof(*report));
+        if (report) {
+            std::memset(report, 0, sizeof(*report));
+            report->sequences = (int32_t)(groups.size() + singles.size());
+            report->grouped_tiles = grouped_tiles;
+        }
         if (ne > 0) {
-            // candidate stream counts; tiles largest first, each to the least loaded stream
+            const int nseq = (int)seqs.size();
+            const uint32_t steps = (uint32_t)((L + B - 1) / B);
+            // candidate stream counts (at most one per launch sequence);
+            // sequences largest first, each to the least loaded stream
             std::vector<int> cands;
             if (streams > 0) {
-                cands.push_back(std::min(streams, ntiles));
+                cands.push_back(std::min(streams, nseq));
             } else {
                 for (int k : {1, 2, 4, 8})
-                    if (k <= ntiles && (cands.empty() || cands.back() != k)) cands.push_back(k);
-                if (cands.back() < std::min(8, ntiles)) cands.push_back(std::min(8, ntiles));
+                    if (k <= nseq && (cands.empty() || cands.back() != k)) cands.push_back(k);
+                if (cands.back() < std::min(8, nseq)) cands.push_back(std::min(8, nseq));
             }
             const int kmax = *std::max_element(cands.begin(), cands.end());
-            std::vector<int> order(ntiles);
-            for (int t = 0; t < ntiles; ++t) order[t] = t;
-            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+            std::vector<int> order(nseq);
+            for (int q = 0; q < nseq; ++q) order[q] = q;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return seqs[a].work > seqs[b].work; });
             struct Streams {
                 std::vector<hipStream_t> s;
                 std::vector<hipEvent_t> e;
@@ -739,21 +961,21 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
                 xs.e.push_back(e);
             }
             auto stream_of = [&](int k) { return k == 0 ? st : xs.s[k - 1]; };
-            // all tiles' epoch on K streams: fork from st, join back into st
+            // all sequences' epoch on K streams: fork from st, join back into st
             auto enqueue_all = [&](int K, int cur, bool last) {
                 std::vector<std::vector<int>> bucket(K);
                 std::vector<double> load(K, 0.0);
-                for (int t : order) {
+                for (int q : order) {
                     const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-                    bucket[k].push_back(t);
-                    load[k] += work[t];
+                    bucket[k].push_back(q);
+                    load[k] += seqs[q].work;
                 }
                 if (K > 1) {
                     FPM_HIP(hipEventRecord(xs.e[0], st));
                     for (int k = 1; k < K; ++k) FPM_HIP(hipStreamWaitEvent(stream_of(k), xs.e[0], 0));
                 }
                 for (int k = 0; k < K; ++k)
-                    for (int t : bucket[k]) sv[t]->enqueue_epoch(cur, last, stream_of(k), none);
+                    for (int q : bucket[k]) seqs[q].enqueue(cur, last, stream_of(k));
                 if (K > 1) {
                     for (int k = 1; k < K; ++k) {
                         FPM_HIP(hipEventRecord(xs.e[k], stream_of(k)));
@@ -797,12 +1019,15 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
                 } else {
                     FPM_HIP(hipGraphLaunch(ex, st));
                 }
-                cur = sv[0]->parity_after(cur);
+                cur = (int)((cur + steps) & 1u);
             }
-            for (int t = 0; t < ntiles; ++t)
-                if (tiles[t].loss_per_epoch)
-                    FPM_HIP(hipMemcpyAsync(tiles[t].loss_per_epoch, sv[t]->ss.loss_hist, (size_t)ne * sizeof(double),
+            for (size_t q = 0; q < groups.size(); ++q) gsv[q]->finish(tiles, groups[q], ne, st);
+            for (size_t q = 0; q < singles.size(); ++q) {
+                const fpm_tile& tl = tiles[singles[q]];
+                if (tl.loss_per_epoch)
+                    FPM_HIP(hipMemcpyAsync(tl.loss_per_epoch, sv[q]->ss.loss_hist, (size_t)ne * sizeof(double),
                                            hipMemcpyDeviceToHost, st));
+            }
             if (report) {
                 report->streams = K;
                 report->tuned = tune ? (int32_t)std::min<size_t>(cands.size(), (size_t)ne) : 0;

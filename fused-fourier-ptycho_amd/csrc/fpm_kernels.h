@@ -73,6 +73,8 @@ struct SolveState {
     double* loss_cur;   // may be null (no loss requested)
     double* loss_hist;  // [epochs], may be null
     uint32_t halve_from = 13;  // lr halves after every epoch with index >= halve_from
+    // grouped tiles: loss_cur[k] and loss_hist[k * hist_stride + epoch] per tile k
+    uint32_t ntiles = 1, hist_stride = 0;
 };
 // lr = lr0, epoch = 0, loss_cur = 0 (a kernel, not a memset: graph-safe)
 void launch_solve_state_init(const SolveState& s, float lr0, hipStream_t st);
@@ -94,9 +96,11 @@ void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t le
 // kernels.cuh:384-410 is applied to P in place; otherwise the pupil gradient
 // is written to GP.
 enum { kBackprojPupil = 0, kBackprojAtomic = 1 };
+// tab (optional, ntiles entries): grouped tiles, tile k = grid row k with
+// R pages k b.., Shat, P, Gs and mP blocks k, LED windows and A from tab[k]
 void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const int2* led, uint32_t led0, uint32_t b,
                      float2* P, float2* Gs, int mode, float* mP, const float* A, RmsArgs rms, float2* GP,
-                     hipStream_t st);
+                     hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
 
 // owner-computes stitch: every element of Gs written as the sum, in LED order,
 // of R_l . conj(P) / (Nr Nc) over the windows covering it (zero elsewhere).
@@ -107,22 +111,32 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 // of n-side pages in one kernel (z_cols: pages after their inverse column
 // passes; u_out: pages before their forward column passes); nr <= 2048.
 // rowmask (optional, nr/16 bytes): input rows of 0-entries are taken as zero.
+// tab (optional): grouped tiles, b = ntiles x ppt pages, tile k = pages
+// k ppt..; its images, normalisation, loss slot from tab[k], its row mask at
+// rowmask + k nr/16.
 bool stencil_fused_supported(const Geom& g);
 void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
                           float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
-                          hipStream_t st, int eng = ENG_F32);
+                          hipStream_t st, int eng = ENG_F32, const TileIn* tab = nullptr, uint32_t ppt = 0);
 
 // contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
 // [+ contiguous forward pass of the new O, written back into G]
 // rowmask (optional, Nr/16 bytes): gradient rows of 0-entries are taken as zero
+// ntiles > 1: grouped tiles, G, O, mO hold ntiles objects back to back and
+// rowmask ntiles masks (fused_update_groupable must hold)
 void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng = ENG_F32);
+                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng = ENG_F32,
+                         uint32_t ntiles = 1);
+// every workgroup of the fused update stays within one object's lines
+bool fused_update_groupable(const Geom& g);
 
 // per 16-row tile of the N-side spectrum: 1 if a LED window of the batch
 // covers a row of the tile at a pupil row i with sup[i] != 0 (sup: nr bytes,
 // or nullptr for every pupil row); Nr / 16 bytes
+// tab (optional): grouped tiles, tile k's LED windows from tab[k], its
+// support at sup + k nr and its mask at mask + k Nr/16
 void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, const uint8_t* sup,
-                     uint8_t* mask, hipStream_t st);
+                     uint8_t* mask, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
 
 // pupil support: sup[i] (nr bytes) = 1 iff pupil row i holds P != 0 or A != 0;
 // mask (nr/16 bytes): the same per 16-row tile of the ifftshifted per-LED

@@ -113,16 +113,32 @@ void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t le
 // half of constrained_RMSprop_both (kernels.cuh:384-410).  One thread per
 // pupil pixel walks the batch in LED order: the pupil gradient sum stays in a
 // register (no atomics, deterministic) and the pupil update needs no second
-// kernel, since only this thread reads P at that pixel.
+// kernel, since only this thread reads P at that pixel.  Grouped tiles
+// (tab): grid row k is tile k, whose R pages are k b.., whose spectrum, pupil,
+// gradient and moment are block k of Shat, P, Gs, mP (GP) and whose LED
+// windows and A come from tab[k].
 __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __restrict__ R,
                                                        const float2* __restrict__ Shat, const int2* __restrict__ led,
                                                        uint32_t led0, uint32_t b, float2* __restrict__ P,
                                                        float2* __restrict__ Gs, int mode, float* __restrict__ mP,
                                                        const float* __restrict__ A, RmsArgs rms,
-                                                       float2* __restrict__ GP, float invN2) {
+                                                       float2* __restrict__ GP, float invN2,
+                                                       const TileIn* __restrict__ tab) {
     const uint32_t plane = g.nr * g.nc;
     const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
     if (pix >= plane) return;
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        const size_t NN = (size_t)g.Nr * g.Nc;
+        R += (size_t)tk * b * plane;
+        Shat += tk * NN;
+        P += (size_t)tk * plane;
+        if (Gs) Gs += tk * NN;
+        if (mP) mP += (size_t)tk * plane;
+        if (GP) GP += (size_t)tk * plane;
+        led = tab[tk].led;
+        A = tab[tk].A;
+    }
     const uint32_t i = pix & (g.nr - 1), j = pix >> g.lnr;
     const uint32_t di = (i + (g.nr >> 1)) & (g.nr - 1), dj = (j + (g.nc >> 1)) & (g.nc - 1);
     const size_t rpos = g.nl.off(di, dj);
@@ -175,11 +191,13 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
 
 void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const int2* led, uint32_t led0, uint32_t b,
                      float2* P, float2* Gs, int mode, float* mP, const float* A, RmsArgs rms, float2* GP,
-                     hipStream_t st) {
+                     hipStream_t st, const TileIn* tab, uint32_t ntiles) {
     const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
-    dim3 grid((g.plane() + 255) / 256);
+    if (!tab) ntiles = 1;
+    if (ntiles < 1 || ntiles > 65535) throw HipError{hipErrorInvalidValue, "back-projection: 1..65535 tiles"};
+    dim3 grid((g.plane() + 255) / 256, ntiles);
     hipLaunchKernelGGL(backproj_kernel, grid, dim3(256), 0, st, g, R, Shat, led, led0, b, P, Gs, mode, mP, A, rms, GP,
-                       invN2);
+                       invN2, tab);
     FPM_LAUNCH_CHECK();
 }
 
@@ -275,11 +293,19 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 // windows' P rows there are all 0) and the spectrum is never read (the crop
 // and the pupil kernel read it only where P != 0 or A != 0), so the column
 // passes skip them.
+// Grouped tiles (tab): grid row k is tile k, its LED windows from tab[k],
+// its support at sup + k nr, its mask at mask + k Nr/16.
 __global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ led, uint32_t led0, uint32_t b,
                                                        uint32_t Nr, uint32_t nr, const uint8_t* __restrict__ sup,
-                                                       uint8_t* __restrict__ mask) {
+                                                       uint8_t* __restrict__ mask, const TileIn* __restrict__ tab) {
     const uint32_t tile = blockIdx.x * 256u + threadIdx.x;
     if (tile >= Nr / kMaskRows) return;
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        led = tab[tk].led;
+        if (sup) sup += (size_t)tk * nr;
+        mask += (size_t)tk * (Nr / kMaskRows);
+    }
     const uint32_t Rm = Nr - 1, ra = tile * kMaskRows;
     uint8_t hit = 0;
     for (uint32_t l = 0; l < b && !hit; ++l) {
@@ -293,10 +319,12 @@ __global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ 
 }
 
 void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, const uint8_t* sup,
-                     uint8_t* mask, hipStream_t st) {
+                     uint8_t* mask, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
     const uint32_t tiles = Nr / kMaskRows;
-    hipLaunchKernelGGL(row_mask_kernel, dim3((tiles + 255) / 256), dim3(256), 0, st, led, led0, b, Nr, nr, sup,
-                       mask);
+    if (!tab) ntiles = 1;
+    if (ntiles < 1 || ntiles > 65535) throw HipError{hipErrorInvalidValue, "row mask: 1..65535 tiles"};
+    hipLaunchKernelGGL(row_mask_kernel, dim3((tiles + 255) / 256, ntiles), dim3(256), 0, st, led, led0, b, Nr, nr, sup,
+                       mask, tab);
     FPM_LAUNCH_CHECK();
 }
 
@@ -416,7 +444,8 @@ __global__ void solve_state_init_kernel(SolveState s, float lr0) {
     if (threadIdx.x == 0) {
         *s.lr = lr0;
         *s.epoch = 0u;
-        if (s.loss_cur) *s.loss_cur = 0.0;
+        if (s.loss_cur)
+            for (uint32_t k = 0; k < s.ntiles; ++k) s.loss_cur[k] = 0.0;
     }
 }
 
@@ -424,8 +453,10 @@ __global__ void epoch_end_kernel(SolveState s) {
     if (threadIdx.x == 0) {
         const uint32_t e = *s.epoch;
         if (s.loss_cur) {
-            if (s.loss_hist) s.loss_hist[e] = *s.loss_cur;
-            *s.loss_cur = 0.0;
+            for (uint32_t k = 0; k < s.ntiles; ++k) {
+                if (s.loss_hist) s.loss_hist[(size_t)k * s.hist_stride + e] = s.loss_cur[k];
+                s.loss_cur[k] = 0.0;
+            }
         }
         // float lr times double 0.5, as the reference's host loop does
         if (e >= s.halve_from) *s.lr = (float)((double)*s.lr * 0.5);

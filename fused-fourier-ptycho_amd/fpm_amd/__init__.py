@@ -86,7 +86,8 @@ class _Opts(ctypes.Structure):
         ("graph", ctypes.c_int32),
         ("fft_engine", ctypes.c_int32),
         ("lr_halve_from", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("tile_launch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("stop_loss", ctypes.c_double),
         ("epochs_run", ctypes.POINTER(ctypes.c_int32)),
     ]
@@ -137,7 +138,8 @@ def _check(rc: int) -> None:
 
 
 def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0.2, verbose=False,
-          graph=False, engine=FFT_F32, lr_halve_from=0, stop_loss=0.0, epochs_run=None) -> _Opts:
+          graph=False, engine=FFT_F32, lr_halve_from=0, stop_loss=0.0, epochs_run=None,
+          tile_launch=0) -> _Opts:
     o = _Opts()
     lib().fpm_opts_init(ctypes.byref(o))
     o.stitch = int(stitch)
@@ -151,6 +153,7 @@ def _opts(stitch=STITCH_OWNER, device=False, stream=None, loss_buf=None, clamp=0
     o.fft_engine = int(engine)
     o.lr_halve_from = int(lr_halve_from)
     o.stop_loss = float(stop_loss)
+    o.tile_launch = int(tile_launch)
     if epochs_run is not None:
         o.epochs_run = ctypes.pointer(epochs_run)
     return o

@@ -34,9 +34,14 @@ class Tile(ctypes.Structure):
                 ("A", ctypes.c_void_p), ("loss_per_epoch", ctypes.POINTER(ctypes.c_double))]
 
 
+# fpm_opts.tile_launch
+TILES_GROUPED = 0   # tiles of one size and image type share launches
+TILES_PER_TILE = 1  # one launch sequence per tile
+
+
 class TilesReport(ctypes.Structure):
     _fields_ = [("streams", ctypes.c_int32), ("tuned", ctypes.c_int32), ("cand", ctypes.c_int32 * 4),
-                ("cand_ms", ctypes.c_double * 4)]
+                ("cand_ms", ctypes.c_double * 4), ("sequences", ctypes.c_int32), ("grouped_tiles", ctypes.c_int32)]
 
 
 def _tiles_lib():
@@ -70,11 +75,13 @@ def led_offsets_at(objdx_um: float, objdy_um: float, n: int, pratio: int, *, wav
 
 
 def tiles_solve(tiles: Sequence[dict], pratio: int, epochs: int, batch: int, opt, *, streams: int = 0,
-                stitch=None, engine: int = 0, loss: bool = False, stream=None):
+                stitch=None, engine: int = 0, loss: bool = False, stream=None, tile_launch: int = TILES_GROUPED):
     """Jointly solve device-resident tiles.  Each tile dict holds CUDA
     tensors in the solve's device layout: O0, O (output), P0, P (output),
     Y (uint16 raw or float32 amplitudes), led (int32 L x 2 rows = 2 x L
-    column-major), A.  Returns (report dict, per-tile loss arrays or None)."""
+    column-major), A.  tile_launch: TILES_GROUPED (tiles of one size share
+    launches) or TILES_PER_TILE.  Returns (report dict, per-tile loss arrays
+    or None)."""
     import torch
 
     from . import STITCH_OWNER, Y_F32_AMPLITUDE, Y_U16_RAW, _opts
@@ -92,14 +99,16 @@ def tiles_solve(tiles: Sequence[dict], pratio: int, epochs: int, batch: int, opt
                       t["P"].data_ptr(), int(t["P0"].shape[0]), t["Y"].data_ptr(),
                       Y_U16_RAW if t["Y"].dtype == torch.uint16 else Y_F32_AMPLITUDE, t["led"].data_ptr(),
                       t["A"].data_ptr(), lb.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if loss else None)
-    o = _opts(stitch=STITCH_OWNER if stitch is None else stitch, device=True, stream=stream, engine=engine)
+    o = _opts(stitch=STITCH_OWNER if stitch is None else stitch, device=True, stream=stream, engine=engine,
+              tile_launch=tile_launch)
     rep = TilesReport()
     optv = (ctypes.c_double * 3)(*[float(v) for v in opt[:3]])
     torch.cuda.synchronize()
     _check(_tiles_lib().fpm_tiles_solve(arr, len(tiles), L, float(pratio), float(epochs), float(batch), optv,
                                         int(streams), ctypes.byref(o), ctypes.byref(rep)))
     report = {"streams": rep.streams, "tuned": rep.tuned,
-              "candidates": {int(rep.cand[k]): float(rep.cand_ms[k]) for k in range(rep.tuned)}}
+              "candidates": {int(rep.cand[k]): float(rep.cand_ms[k]) for k in range(rep.tuned)},
+              "sequences": rep.sequences, "grouped_tiles": rep.grouped_tiles}
     return report, ([lb[:int(epochs)] for lb in losses] if loss else None)
 
 

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FPM_AMD_ABI_VERSION 1
+#define FPM_AMD_ABI_VERSION 2
 
 /* status codes */
 #define FPM_OK              0
@@ -62,6 +62,10 @@ typedef struct { float re, im; } fpm_c32;
 #define FPM_STITCH_OWNER  1  /* owner-computes gather in fixed LED order: deterministic;
                                 the default (faster on MI355X: it is formed inside the
                                 first column pass of the gradient's inverse transform) */
+
+/* launches of fpm_tiles_solve (fpm_opts.tile_launch) */
+#define FPM_TILES_GROUPED  0  /* tiles of one size and image type share launches */
+#define FPM_TILES_PER_TILE 1  /* one launch sequence per tile */
 
 /* arithmetic of the radix-16 DFT stages of every transform of a call */
 #define FPM_FFT_F32       0  /* FP32 butterflies on the vector ALUs (the default) */
@@ -114,7 +118,9 @@ typedef struct {
     int32_t lr_halve_from;        /* lr halves after every epoch with index >= this;
                                      0 selects 13, the reference's schedule
                                      (cuFPM_pure_v3.cu:204-206: index > 12) */
-    int32_t reserved[2];
+    int32_t tile_launch;          /* fpm_tiles_solve: FPM_TILES_GROUPED (default) or
+                                     FPM_TILES_PER_TILE */
+    int32_t reserved;
     double  stop_loss;            /* > 0: fpm_pure_v3 stops after the first epoch whose
                                      loss (as loss_per_epoch defines it) is <= stop_loss;
                                      the host reads each epoch's loss (one stream
@@ -223,13 +229,22 @@ int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
  * jointly with the update of fpm_pure_v3.  Every tile has the same number of
  * images L and uses the same pratio, epochs, batch and RMSprop constants;
  * tile sides are powers of two (the transforms' rule).  All tile arrays are
- * DEVICE pointers (opts->pointers_are_device is ignored).  Each epoch of all
- * tiles is captured once as one hipGraph whose tiles are spread over
- * `streams` concurrent streams, largest tiles first (longest-processing-time
- * assignment); streams = 0 autotunes: the first epochs run with 1, 2, 4 and 8
- * streams in turn, each timed, and the fastest count runs the rest.  The
- * stream count changes only the overlap, never the arithmetic: results are
- * the same bits for every count.
+ * DEVICE pointers (opts->pointers_are_device is ignored).
+ * With opts->tile_launch = FPM_TILES_GROUPED (the default) the tiles of one
+ * size and image type run through shared launches: every stage of a
+ * minibatch step takes all of them at once (the library holds their working
+ * arrays back to back and reads each tile's images, LED windows and mask
+ * through a device table), so an epoch costs the launches of one solve per
+ * tile size instead of one per tile.  Tiles that cannot be grouped (atomic
+ * stitch, batch > 64, objects narrower than a fused-update workgroup) and
+ * every tile under FPM_TILES_PER_TILE run their own launch sequences.
+ * Each epoch of all launch sequences is captured once as one hipGraph whose
+ * sequences are spread over `streams` concurrent streams, largest first
+ * (longest-processing-time assignment); streams = 0 autotunes: the first
+ * epochs run with 1, 2, 4 and 8 streams in turn (at most one per sequence),
+ * each timed, and the fastest count runs the rest.  Grouping and the stream
+ * count change only the launches, never the arithmetic: every tile's result
+ * is the same bits as its own fpm_pure_v3 call.
  * ------------------------------------------------------------------------ */
 typedef struct {
     const fpm_c32* O0;       /* N x N spatial object */
@@ -250,6 +265,8 @@ typedef struct {
     int32_t tuned;           /* candidates timed (0: streams was given) */
     int32_t cand[4];         /* candidate stream counts */
     double  cand_ms[4];      /* measured epoch time of each candidate */
+    int32_t sequences;       /* launch sequences per epoch (groups + single tiles) */
+    int32_t grouped_tiles;   /* tiles that ran in shared launches */
 } fpm_tiles_report;
 
 int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pratio,

@@ -34,11 +34,35 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_opts_layout():
-    assert fpm_amd.lib().fpm_abi_version() == 1
+    assert fpm_amd.lib().fpm_abi_version() == 2
     o = fpm_amd._Opts()
     fpm_amd.lib().fpm_opts_init(ctypes.byref(o))
     assert o.struct_size == ctypes.sizeof(fpm_amd._Opts)
     assert abs(o.pupil_clamp - 0.2) < 1e-7 and o.stitch == fpm_amd.STITCH_OWNER
+    assert o.tile_launch == 0
+
+
+def test_ctypes_mirrors_match_the_header_layout(tmp_path):
+    """Every field offset and the size of fpm_opts, fpm_tile and
+    fpm_tiles_report as gcc lays out the header equal the ctypes mirrors'."""
+    from fpm_amd import tiles as ft
+    structs = {"fpm_opts": fpm_amd._Opts, "fpm_tile": ft.Tile, "fpm_tiles_report": ft.TilesReport}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 def _call_solve(nr=16, Nr=32, L=4, batch=2, led=None, epochs=1):
@@ -149,6 +173,11 @@ def test_tiles_solve_rejects_bad_arguments():
     o = fpm_amd._opts(stop_loss=1.0)
     assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 1, ctypes.byref(o), None) == -1
     assert "stop_loss" in L.fpm_last_error().decode()
+    # unknown tile launch mode
+    arr[0] = ft.Tile(p, p, 32, p, p, 16, p, 0, p, p, None)
+    o = fpm_amd._opts(tile_launch=2)
+    assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 1, ctypes.byref(o), None) == -1
+    assert "tile_launch" in L.fpm_last_error().decode()
 
 
 def test_cnn_create_rejects_bad_descriptors():

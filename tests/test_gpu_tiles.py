@@ -1,8 +1,11 @@
 """Multi-tile solve (fpm_tiles_solve): tiles of several sizes, each with its
 own pupil and LED windows, reconstructed in one hipGraph per epoch over
-concurrent streams.  The stream count only changes the overlap, so every
-schedule -- and the autotuned one -- gives the same bits as solving each
-tile alone with fpm_pure_v3 (owner-computes stitch: deterministic)."""
+concurrent streams, the tiles of one size in shared launches (or one launch
+sequence per tile).  Grouping and the stream count only change the launches,
+so every schedule -- and the autotuned one -- gives the same bits as solving
+each tile alone with fpm_pure_v3 (owner-computes stitch: deterministic).
+The per-epoch losses are FP64 sums of per-workgroup partial sums added
+atomically, in an order that may vary: equal to 1e-12."""
 import numpy as np
 import pytest
 
@@ -25,7 +28,8 @@ def _single(t, epochs, batch):
     O, P = torch.empty_like(t["O0"]), torch.empty_like(t["P0"])
     N, n = t["O0"].shape[0], t["P0"].shape[0]
     torch.cuda.synchronize()
-    loss = fpm_amd.solve_device(t["O0"].data_ptr(), t["P0"].data_ptr(), t["Y"].data_ptr(), True, t["led"].data_ptr(),
+    u16 = t["Y"].dtype == torch.uint16
+    loss = fpm_amd.solve_device(t["O0"].data_ptr(), t["P0"].data_ptr(), t["Y"].data_ptr(), u16, t["led"].data_ptr(),
                                 t["A"].data_ptr(), (N, N), (n, n), t["Y"].shape[0], 4, epochs, batch, OPT,
                                 O.data_ptr(), P.data_ptr(), loss=True)
     return O.cpu().numpy(), P.cpu().numpy(), loss
@@ -40,21 +44,46 @@ def test_tile_windows_differ_across_the_field():
     assert len({t["n"] for t in tl}) == 2
 
 
+@pytest.mark.parametrize("launch", [ft.TILES_GROUPED, ft.TILES_PER_TILE])
 @pytest.mark.parametrize("streams", [1, 3, 0])
-def test_tiles_solve_equals_per_tile_solves(streams):
+def test_tiles_solve_equals_per_tile_solves(streams, launch):
+    """9 tiles of two sizes, 49 LEDs in batches of 10 (a ragged last batch):
+    grouped, the 5 + 4 tiles of each size form two launch sequences."""
     tl = _tile_set()
     epochs, batch = 3, 10
-    rep, losses = ft.tiles_solve(tl, 4, epochs, batch, OPT, streams=streams, loss=True)
-    print(f"[tiles] streams={streams}: report {rep}", flush=True)
-    if streams == 0:
-        assert rep["tuned"] == 3 and rep["streams"] in rep["candidates"]
+    rep, losses = ft.tiles_solve(tl, 4, epochs, batch, OPT, streams=streams, loss=True, tile_launch=launch)
+    print(f"[tiles] streams={streams} launch={launch}: report {rep}", flush=True)
+    nseq = 2 if launch == ft.TILES_GROUPED else len(tl)
+    assert rep["sequences"] == nseq and rep["grouped_tiles"] == (len(tl) if launch == ft.TILES_GROUPED else 0)
+    if streams == 0:  # candidates 1, 2 (grouped) or 1, 2, 4, 8 (per tile), one per epoch
+        assert rep["tuned"] == (2 if launch == ft.TILES_GROUPED else 3) and rep["streams"] in rep["candidates"]
     else:
-        assert rep["streams"] == streams and rep["tuned"] == 0
+        assert rep["streams"] == min(streams, nseq) and rep["tuned"] == 0
     for k, t in enumerate(tl):
         O, P, l1 = _single(t, epochs, batch)
         assert np.array_equal(t["O"].cpu().numpy(), O), k
         assert np.array_equal(t["P"].cpu().numpy(), P), k
-        assert np.array_equal(losses[k], l1), k
+        np.testing.assert_allclose(losses[k], l1, rtol=1e-12, atol=0, err_msg=str(k))
+
+
+def test_grouped_tiles_mixed_image_types_and_a_lone_size():
+    """Tiles with float amplitudes group apart from uint16 ones, and a size
+    held by one tile runs alone: 3 sequences, every tile equal to its own
+    solve."""
+    tl = _tile_set()
+    small = [t for t in tl if t["n"] == 32][:4]
+    large = [t for t in tl if t["n"] == 64][:1]
+    for t in small[2:]:
+        raw = t["Y"].to(torch.float32)
+        t["Y"] = torch.sqrt((raw - raw.min()) / (raw.max() - raw.min())).contiguous()
+    sel = small + large
+    rep, _ = ft.tiles_solve(sel, 4, 2, 10, OPT, streams=2)
+    print(f"[tiles] mixed: report {rep}", flush=True)
+    assert rep["sequences"] == 3 and rep["grouped_tiles"] == 4
+    for k, t in enumerate(sel):
+        O, P, _ = _single(t, 2, 10)
+        assert np.array_equal(t["O"].cpu().numpy(), O), k
+        assert np.array_equal(t["P"].cpu().numpy(), P), k
 
 
 def _single_opts(t, epochs, batch, **kw):
@@ -69,9 +98,10 @@ def _single_opts(t, epochs, batch, **kw):
 
 def test_tiles_with_split_engine_equal_per_tile_solves():
     """The tile solve passes the FFT engine to every tile: same bits as each
-    tile's own split-engine solve."""
+    tile's own split-engine solve (grouped launches)."""
     tl = _tile_set()[:4]
-    ft.tiles_solve(tl, 4, 2, 10, OPT, streams=2, engine=fpm_amd.FFT_SPLIT_F16)
+    rep, _ = ft.tiles_solve(tl, 4, 2, 10, OPT, streams=2, engine=fpm_amd.FFT_SPLIT_F16)
+    assert rep["grouped_tiles"] == 4
     for k, t in enumerate(tl):
         O, P = _single_opts(t, 2, 10, engine=fpm_amd.FFT_SPLIT_F16)
         assert np.array_equal(t["O"].cpu().numpy(), O), k
@@ -85,7 +115,8 @@ def test_tiles_with_atomic_stitch_match_per_tile_solves():
     agree to about 1e-3 after 2 epochs, not bit for bit: each tile within
     1e-2 of its own atomic-stitch solve."""
     tl = _tile_set()[:4]
-    ft.tiles_solve(tl, 4, 2, 10, OPT, streams=3, stitch=fpm_amd.STITCH_ATOMIC)
+    rep, _ = ft.tiles_solve(tl, 4, 2, 10, OPT, streams=3, stitch=fpm_amd.STITCH_ATOMIC)
+    assert rep["grouped_tiles"] == 0 and rep["sequences"] == 4  # grouping needs the owner stitch
     for k, t in enumerate(tl):
         O, P = _single_opts(t, 2, 10, stitch=fpm_amd.STITCH_ATOMIC)
         o = t["O"].cpu().numpy()

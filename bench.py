@@ -255,6 +255,24 @@ def run_tiles(args):
             "note": "tile sides are powers of two; 192 px tiles need a radix-3 transform (not built)"}
 
 
+def run_tiles_grouped(args):
+    """The grouped call of the tiles leg alone (same tiles), fixed stream
+    count, one warm-up epoch then args.tile_epochs timed."""
+    import torch
+
+    from fpm_amd import tiles as ft
+
+    tl = ft.make_tile_set(1024, (128, 256), 7, 4, seed=11, count=48)
+    ep, opt = args.tile_epochs, (args.lr, 0.999, 1e-4)
+    ft.tiles_solve(tl, 4, 1, args.batch, opt, streams=args.tile_streams)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep, _ = ft.tiles_solve(tl, 4, ep, args.batch, opt, streams=args.tile_streams)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"s_per_epoch": dt / ep, "epochs": ep, "streams": rep["streams"], "sequences": rep["sequences"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +297,9 @@ def main():
     ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
     ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
     ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")
+    ap.add_argument("--tiles-only", action="store_true",
+                    help="only the grouped multi-tile solve, fixed streams (for profiling); prints its timing")
+    ap.add_argument("--tile-streams", type=int, default=2, help="stream count of --tiles-only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -299,6 +320,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU: the solve is single-device (the reference "
                                    "is single-GPU too); --gpus above 1 is not supported"}), flush=True)
         sys.exit(2)
+
+    if args.tiles_only:
+        print(json.dumps({"tiles_only": run_tiles_grouped(args)}), flush=True)
+        return
 
     import ctypes
 

@@ -293,38 +293,51 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 // windows' P rows there are all 0) and the spectrum is never read (the crop
 // and the pupil kernel read it only where P != 0 or A != 0), so the column
 // passes skip them.
-// Grouped tiles (tab): grid row k is tile k, its LED windows from tab[k],
-// its support at sup + k nr, its mask at mask + k Nr/16.
+// One workgroup = kRmEntries mask entries x 32 LED lanes: lane j tests LEDs
+// j, j + 32 (b <= 64 in a solve; more loop) against the 16 rows of its entry,
+// with the pupil support staged in LDS, and a ballot ORs the lanes.  Grouped
+// tiles (tab): grid row k is tile k, its LED windows from tab[k], its support
+// at sup + k nr, its mask at mask + k Nr/16.
+constexpr int kRmEntries = 8;
+constexpr int kRmMaxSup = 16384;  // pupil rows (the largest side)
 __global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ led, uint32_t led0, uint32_t b,
                                                        uint32_t Nr, uint32_t nr, const uint8_t* __restrict__ sup,
                                                        uint8_t* __restrict__ mask, const TileIn* __restrict__ tab) {
-    const uint32_t tile = blockIdx.x * 256u + threadIdx.x;
-    if (tile >= Nr / kMaskRows) return;
+    __shared__ uint8_t ssup[kRmMaxSup];
     if (tab) {
         const uint32_t tk = blockIdx.y;
         led = tab[tk].led;
         if (sup) sup += (size_t)tk * nr;
         mask += (size_t)tk * (Nr / kMaskRows);
     }
-    const uint32_t Rm = Nr - 1, ra = tile * kMaskRows;
-    uint8_t hit = 0;
-    for (uint32_t l = 0; l < b && !hit; ++l) {
-        const uint32_t wr = ((uint32_t)led[led0 + l].y - 1u + (Nr >> 1)) & Rm;
-        for (uint32_t r = 0; r < kMaskRows; ++r) {
-            const uint32_t i = (ra + r - wr) & Rm;
-            if (i < nr && (!sup || sup[i])) hit = 1;
+    if (sup)
+        for (uint32_t i = threadIdx.x; i < nr; i += 256) ssup[i] = sup[i];
+    __syncthreads();
+    const uint32_t e = blockIdx.x * kRmEntries + threadIdx.x / 32;
+    const uint32_t lane = threadIdx.x % 32;
+    const uint32_t Rm = Nr - 1, ra = e * kMaskRows;
+    bool hit = false;
+    if (e < Nr / kMaskRows) {
+        for (uint32_t l = lane; l < b; l += 32) {
+            const uint32_t wr = ((uint32_t)led[led0 + l].y - 1u + (Nr >> 1)) & Rm;
+            for (uint32_t r = 0; r < kMaskRows; ++r) {
+                const uint32_t i = (ra + r - wr) & Rm;
+                hit |= i < nr && (!sup || ssup[i]);
+            }
         }
     }
-    mask[tile] = hit;
+    const uint64_t bal = __ballot(hit);
+    if (e < Nr / kMaskRows && lane == 0) mask[e] = ((bal >> (threadIdx.x & 32u)) & 0xffffffffull) != 0 ? 1 : 0;
 }
 
 void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, const uint8_t* sup,
                      uint8_t* mask, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
-    const uint32_t tiles = Nr / kMaskRows;
+    const uint32_t entries = Nr / kMaskRows;
     if (!tab) ntiles = 1;
     if (ntiles < 1 || ntiles > 65535) throw HipError{hipErrorInvalidValue, "row mask: 1..65535 tiles"};
-    hipLaunchKernelGGL(row_mask_kernel, dim3((tiles + 255) / 256, ntiles), dim3(256), 0, st, led, led0, b, Nr, nr, sup,
-                       mask, tab);
+    if (nr > (uint32_t)kRmMaxSup) throw HipError{hipErrorInvalidValue, "row mask: pupil side above 16384"};
+    hipLaunchKernelGGL(row_mask_kernel, dim3((entries + kRmEntries - 1) / kRmEntries, ntiles), dim3(256), 0, st, led,
+                       led0, b, Nr, nr, sup, mask, tab);
     FPM_LAUNCH_CHECK();
 }
 
@@ -400,29 +413,65 @@ void launch_abs_scaled(const float2* z, size_t count, float scale, int squared, 
     FPM_LAUNCH_CHECK();
 }
 
-__global__ __launch_bounds__(256) void u16_minmax_kernel(const uint16_t* __restrict__ raw, size_t n,
-                                                         uint32_t* __restrict__ mm) {
+// min/max of a uint16 stack: 16-byte loads (8 values) over the aligned body,
+// single values for the unaligned head and the tail, a workgroup reduction
+// in LDS and one atomic pair per workgroup
+__device__ __forceinline__ void mm_u32pair(uint32_t w, uint32_t& lo, uint32_t& hi) {
+    const uint32_t a = w & 0xffffu, b = w >> 16;
+    lo = min(lo, min(a, b));
+    hi = max(hi, max(a, b));
+}
+
+__global__ __launch_bounds__(256) void u16_minmax_kernel(const uint16_t* __restrict__ raw, size_t n, size_t head,
+                                                         size_t nvec, uint32_t* __restrict__ mm) {
+    __shared__ uint32_t slo[4], shi[4];
     uint32_t lo = 0xffffu, hi = 0u;
-    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+    const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    const uint4* __restrict__ body = (const uint4*)(raw + head);
+    for (size_t k = tid; k < nvec; k += stride) {
+        const uint4 v = body[k];
+        mm_u32pair(v.x, lo, hi);
+        mm_u32pair(v.y, lo, hi);
+        mm_u32pair(v.z, lo, hi);
+        mm_u32pair(v.w, lo, hi);
+    }
+    // head (before the first 16-byte boundary) and tail (after the last full vector)
+    for (size_t k = tid; k < head; k += stride) {
         const uint32_t v = raw[k];
-        lo = v < lo ? v : lo;
-        hi = v > hi ? v : hi;
+        lo = min(lo, v);
+        hi = max(hi, v);
+    }
+    for (size_t k = head + nvec * 8 + tid; k < n; k += stride) {
+        const uint32_t v = raw[k];
+        lo = min(lo, v);
+        hi = max(hi, v);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t l2 = __shfl_down(lo, off, 64), h2 = __shfl_down(hi, off, 64);
-        lo = l2 < lo ? l2 : lo;
-        hi = h2 > hi ? h2 : hi;
+        lo = min(lo, (uint32_t)__shfl_down(lo, off, 64));
+        hi = max(hi, (uint32_t)__shfl_down(hi, off, 64));
     }
     if ((threadIdx.x & 63) == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            lo = min(lo, slo[w]);
+            hi = max(hi, shi[w]);
+        }
         atomicMin(&mm[0], lo);
         atomicMax(&mm[1], hi);
     }
 }
 
 void launch_u16_minmax(const uint16_t* raw, size_t count, uint32_t* minmax, hipStream_t st) {
-    const size_t blocks = std::min<size_t>((count + 255) / 256, 4096);
-    hipLaunchKernelGGL(u16_minmax_kernel, dim3((unsigned)blocks), dim3(256), 0, st, raw, count, minmax);
+    if ((uintptr_t)raw & 1u) throw HipError{hipErrorInvalidValue, "uint16 image stack is not 2-byte aligned"};
+    const size_t head = std::min<size_t>(count, ((16u - ((uintptr_t)raw & 15u)) & 15u) / 2u);
+    const size_t nvec = (count - head) / 8;
+    const size_t blocks = std::max<size_t>(1, std::min<size_t>((nvec + 255) / 256, 2048));
+    hipLaunchKernelGGL(u16_minmax_kernel, dim3((unsigned)blocks), dim3(256), 0, st, raw, count, head, nvec, minmax);
     FPM_LAUNCH_CHECK();
 }
 

@@ -225,6 +225,36 @@ def test_stop_loss_stops_after_the_epoch_that_reaches_it():
     assert np.array_equal(lossb[:4], l10[:4]) and np.all(np.isnan(lossb[4:]))
 
 
+def test_u16_min_max_every_alignment_and_tail():
+    """The device min/max of a uint16 stack (16-byte loads over the aligned
+    body, single values before and after it): every start offset in a
+    16-byte line, lengths with ragged tails, the extremes planted in the
+    head, the body and the tail in turn."""
+    import ctypes
+    torch = pytest.importorskip("torch")
+    base = np.random.default_rng(12).integers(1000, 2000, size=4096).astype(np.int32)
+    dev = torch.from_numpy(base).to(torch.uint16).cuda()
+    L = fpm_amd.lib()
+    for off in range(8):
+        for count in (1, 7, 8, 9, 100, 1001, 4096 - off):
+            for where in sorted({0, count // 2, count - 1}):
+                v = dev.clone()
+                v[off + where] = 7
+                v[off + (count - 1 - where)] = 60000
+                ref = v[off:off + count].cpu().numpy()
+                out = torch.empty(count, dtype=torch.float32, device="cuda")
+                lo, hi = ctypes.c_uint16(0), ctypes.c_uint16(0)
+                torch.cuda.synchronize()
+                rc = L.fpm_prepare_amplitude(ctypes.c_void_p(v.data_ptr() + 2 * off), count,
+                                             ctypes.c_void_p(out.data_ptr()), 1, None, ctypes.byref(lo),
+                                             ctypes.byref(hi))
+                if count == 1:  # a constant stack is rejected
+                    assert rc == -1
+                    continue
+                assert rc == 0, fpm_amd.lib().fpm_last_error()
+                assert (lo.value, hi.value) == (int(ref.min()), int(ref.max())), (off, count, where)
+
+
 def test_solve_u16_path_equals_prepared_f32():
     """Y as uint16 raw counts (normalised on the device) gives the same
     bits as Y prepared on the host the way MATLAB does in single."""

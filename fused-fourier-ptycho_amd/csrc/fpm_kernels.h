@@ -104,8 +104,10 @@ void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const i
 
 // owner-computes stitch: every element of Gs written as the sum, in LED order,
 // of R_l . conj(P) / (Nr Nc) over the windows covering it (zero elsewhere).
+// tab (optional): grouped tiles, grid z = tile k (R pages k b.., P and Gs
+// blocks k, LED windows from tab[k])
 void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b,
-                         float2* Gs, hipStream_t st);
+                         float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
 
 // contiguous inverse pass + loss stencil + contiguous forward pass of a batch
 // of n-side pages in one kernel (z_cols: pages after their inverse column

@@ -209,11 +209,20 @@ void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const i
 // gradient buffer needs no clearing.
 constexpr int kOwTR = 64, kOwTC = 16;
 
+// Grouped tiles (tab): grid z = tile k, its R pages k b.., pupil and
+// gradient blocks k, LED windows from tab[k].
 __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2* __restrict__ R,
                                                            const float2* __restrict__ P, const int2* __restrict__ led,
                                                            uint32_t led0, uint32_t b, float2* __restrict__ Gs,
-                                                           float invN2) {
+                                                           float invN2, const TileIn* __restrict__ tab) {
     __shared__ uint64_t hit[4];
+    if (tab) {
+        const uint32_t tk = blockIdx.z;
+        R += (size_t)tk * b * g.nr * g.nc;
+        P += (size_t)tk * g.nr * g.nc;
+        Gs += (size_t)tk * g.Nr * g.Nc;
+        led = tab[tk].led;
+    }
     const int t = threadIdx.x;
     const int r = t & 63, cg = t >> 6;
     const uint32_t ra = blockIdx.x * kOwTR, ca = blockIdx.y * kOwTC;
@@ -278,10 +287,12 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
 }
 
 void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b,
-                         float2* Gs, hipStream_t st) {
+                         float2* Gs, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
     const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
-    dim3 grid((g.Nr + kOwTR - 1) / kOwTR, (g.Nc + kOwTC - 1) / kOwTC);
-    hipLaunchKernelGGL(owner_stitch_kernel, grid, dim3(256), 0, st, g, R, P, led, led0, b, Gs, invN2);
+    if (!tab) ntiles = 1;
+    if (ntiles < 1 || ntiles > 65535) throw HipError{hipErrorInvalidValue, "owner stitch: 1..65535 tiles"};
+    dim3 grid((g.Nr + kOwTR - 1) / kOwTR, (g.Nc + kOwTC - 1) / kOwTC, ntiles);
+    hipLaunchKernelGGL(owner_stitch_kernel, grid, dim3(256), 0, st, g, R, P, led, led0, b, Gs, invN2, tab);
     FPM_LAUNCH_CHECK();
 }
 

@@ -24,13 +24,13 @@ def _tile_set():
     return ft.make_tile_set(1024, (32, 64), 3, 4, seed=3, led_grid=7)
 
 
-def _single(t, epochs, batch):
+def _single(t, epochs, batch, pratio=4):
     O, P = torch.empty_like(t["O0"]), torch.empty_like(t["P0"])
     N, n = t["O0"].shape[0], t["P0"].shape[0]
     torch.cuda.synchronize()
     u16 = t["Y"].dtype == torch.uint16
     loss = fpm_amd.solve_device(t["O0"].data_ptr(), t["P0"].data_ptr(), t["Y"].data_ptr(), u16, t["led"].data_ptr(),
-                                t["A"].data_ptr(), (N, N), (n, n), t["Y"].shape[0], 4, epochs, batch, OPT,
+                                t["A"].data_ptr(), (N, N), (n, n), t["Y"].shape[0], pratio, epochs, batch, OPT,
                                 O.data_ptr(), P.data_ptr(), loss=True)
     return O.cpu().numpy(), P.cpu().numpy(), loss
 
@@ -84,6 +84,21 @@ def test_grouped_tiles_mixed_image_types_and_a_lone_size():
         O, P, _ = _single(t, 2, 10)
         assert np.array_equal(t["O"].cpu().numpy(), O), k
         assert np.array_equal(t["P"].cpu().numpy(), P), k
+
+
+def test_grouped_tiles_above_1024_keep_the_fused_stitch():
+    """Objects above 1024^2 sum the owner stitch inside the column pass
+    (smaller ones in a separate kernel): 4 tiles of 256 px -> 2048^2 (upsample
+    8), grouped, equal to their own solves."""
+    tl = ft.make_tile_set(512, (256,), 2, 8, seed=5, led_grid=7)
+    rep, losses = ft.tiles_solve(tl, 8, 2, 10, OPT, streams=1, loss=True)
+    print(f"[tiles] 2048^2: report {rep}", flush=True)
+    assert rep["grouped_tiles"] == 4 and rep["sequences"] == 1
+    for k, t in enumerate(tl):
+        O, P, l1 = _single(t, 2, 10, pratio=8)
+        assert np.array_equal(t["O"].cpu().numpy(), O), k
+        assert np.array_equal(t["P"].cpu().numpy(), P), k
+        np.testing.assert_allclose(losses[k], l1, rtol=1e-12, atol=0, err_msg=str(k))
 
 
 def _single_opts(t, epochs, batch, **kw):

@@ -689,7 +689,11 @@ template <int LEN> struct StencilCfg {
 // tab (grouped tiles): page k ppt + l is LED led0 + l of tile k, with tile
 // k's images, normalisation and loss slot from tab[k] and its row mask at
 // rowmask + k LEN/16.
-template <int LEN, int ENG>
+// WHOLE: the workgroup holds a whole page (n_c = LT columns), every column
+// is interior and its neighbours are taken cyclically from the workgroup
+// (one workgroup per page instead of two with 2 interior columns in the
+// second: n = 128).
+template <int LEN, int ENG, bool WHOLE>
 __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,
                                                             float inv_ratio, float gscale,
@@ -725,14 +729,16 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     EngCtx<ENG> ctx;
     ctx.init();
-    const uint32_t j0 = blockIdx.x * Cfg::K;  // first interior column; line l is column j0 - 1 + l
+    const uint32_t j0 = WHOLE ? 0u : blockIdx.x * Cfg::K;  // first interior column; line l is column j0 - 1 + l
     const uint32_t cmask = g.nc - 1;
     const size_t plane = (size_t)g.nr * g.nc;
     const float2* __restrict__ src = in + (size_t)page * plane;
     float2* __restrict__ dst = out + (size_t)page * plane;
     const size_t ybase = (size_t)(led0 + lpage) * plane;
-    auto column = [&](int line) -> uint32_t { return (j0 - 1u + (uint32_t)line) & cmask; };
-    auto interior = [&](int line) -> bool { return line >= 1 && line <= Cfg::K && j0 + (uint32_t)line - 1u < g.nc; };
+    auto column = [&](int line) -> uint32_t { return WHOLE ? (uint32_t)line : (j0 - 1u + (uint32_t)line) & cmask; };
+    auto interior = [&](int line) -> bool {
+        return WHOLE || (line >= 1 && line <= Cfg::K && j0 + (uint32_t)line - 1u < g.nc);
+    };
     // space-domain pages have natural columns in either layout
     auto at = [&](int idx, int line) -> size_t { return g.nl.row_off((uint32_t)idx) + (size_t)column(line) * g.nl.ld; };
 
@@ -777,7 +783,8 @@ __global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __res
             if (interior(line)) {
                 const float* e = ef + line * LEN;
                 const float e0 = e[idx];
-                const float el = e[idx - LEN], er = e[idx + LEN];
+                const float el = WHOLE ? ef[((line + Cfg::LT - 1) & (Cfg::LT - 1)) * LEN + idx] : e[idx - LEN];
+                const float er = WHOLE ? ef[((line + 1) & (Cfg::LT - 1)) * LEN + idx] : e[idx + LEN];
                 const float eu = e[(idx - 1) & (LEN - 1)], ed = e[(idx + 1) & (LEN - 1)];
                 const float gval = (copysignf(1.0f, e0 - el) - copysignf(1.0f, er - e0) + copysignf(1.0f, e0 - eu) -
                                     copysignf(1.0f, ed - e0)) * gscale;
@@ -828,9 +835,14 @@ static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const 
     const float inv_ratio = exact_inverse_or_zero(ratio);
 #define FPM_STF_CASE(L)                                                                                   \
     case L: {                                                                                             \
-        dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                                   \
-        hipLaunchKernelGGL((stencil_fused_kernel<L, ENG>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, led0, ratio, \
-                           inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt);                            \
+        if (g.nc == (uint32_t)StencilCfg<L>::LT) {                                                        \
+            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, true>), dim3(1, b), dim3(1024), 0, st, z_cols, u_out, g, \
+                               y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt);       \
+        } else {                                                                                          \
+            dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                               \
+            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, false>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, \
+                               led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt);          \
+        }                                                                                                 \
         break;                                                                                            \
     }
     switch (g.nr) {

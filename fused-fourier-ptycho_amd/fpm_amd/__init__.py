@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = (
     "fpm_forward_intensity", "fpm_fft2", "fpm_fft2_engine", "fpm_last_error", "fpm_abi_version",
     "fpm_cnn_create", "fpm_cnn_destroy", "fpm_cnn_num_params", "fpm_cnn_get_params", "fpm_cnn_set_params",
     "fpm_cnn_forward", "fpm_cnn_gradient", "fpm_cnn_train_step", "fpm_led_offsets_at", "fpm_tiles_solve",
-    "fpm_parallax_shifts",
+    "fpm_parallax_shifts", "fpm_optimizer_step", "fpm_pupil_constraint",
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

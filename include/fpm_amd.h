@@ -24,6 +24,10 @@
  *                           (|IFFT(crop(FFT O) . P)|, the image model)
  *   fpm_fft2             <- the cufftExecC2C call sites of cuFPM_v3/cuFPM_pure_v3.cu:118-187
  *                           (unnormalised 2D C2C transforms)
+ *   fpm_optimizer_step   <- optimizer_*.step of codes/func_ware/optimizers/ (adam, RMSprop,
+ *                           sgd) and codes_v2/func_ware/optimizers/optimizer_yogi.m, the
+ *                           optimizers that drive fullyfusedFPM in codes/main_run_FPM.m:136-151
+ *   fpm_pupil_constraint <- the pupil amplitude constraint of codes/main_run_FPM.m:153-156
  */
 #ifndef FPM_AMD_H
 #define FPM_AMD_H
@@ -164,6 +168,39 @@ int fpm_fused_gradient(const fpm_c32* O, int64_t N_rows, int64_t N_cols,
                        const int32_t* led_idx, double pratio,
                        fpm_c32* G_O, fpm_c32* G_P, double* loss_out,
                        const fpm_opts* opts);
+
+/* Optimizers of the gradient-driven solve (fpm_fused_gradient in a loop, as
+ * codes/main_run_FPM.m:106-160 drives fullyfusedFPM): one step on DEVICE
+ * arrays of `count` complex elements, in place on param, with the
+ * reference's MATLAB arithmetic in single precision:
+ *   FPM_OPT_ADAM     m1 = b1 m1 + (1-b1) g;  m2 = b2 m2 + (1-b2)|g|^2;
+ *                    param -= lr (bias m1) / (sqrt(m2) + 1e-5),
+ *                    bias = sqrt(1 - b2^iter) / (1 - b1^iter)   (optimizer_adam.m:25-38)
+ *   FPM_OPT_YOGI     m1 as Adam; m2 -= (1-b2) sign(m2 - |g|^2) |g|^2;
+ *                    param -= lr m1 / (sqrt(m2) + 1e-5)          (optimizer_yogi.m:25-36)
+ *   FPM_OPT_RMSPROP  s = b1 + (1-b1)|g|^2 (no running average, as written);
+ *                    param -= lr g / (sqrt(s) + 1e-5)            (optimizer_RMSprop.m:28-46,
+ *                    centered = false, mum = 0 as every caller constructs it)
+ *   FPM_OPT_SGD      param -= lr g  (optimizer_sgd.m:16-21 subtracts an
+ *                    undefined `update`; the plain gradient step is meant)
+ * mom1 (complex) and mom2 (float) hold the moments (zero before the first
+ * step; unused by RMSprop and SGD, may be NULL there); iter counts from 1
+ * (Adam's bias correction).  Scalars are rounded to single as MATLAB does
+ * when a double meets a single array ((1 - b) is rounded once). */
+#define FPM_OPT_SGD     0
+#define FPM_OPT_ADAM    1
+#define FPM_OPT_YOGI    2
+#define FPM_OPT_RMSPROP 3
+int fpm_optimizer_step(int32_t kind, fpm_c32* param, const fpm_c32* grad, fpm_c32* mom1,
+                       float* mom2, int64_t count, double lr, double beta1, double beta2,
+                       int64_t iter, void* stream);
+
+/* The gradient-driven solve's pupil constraint (codes/main_run_FPM.m:153-156),
+ * in place on DEVICE arrays:
+ *   P = max(min(|P|, A + ac), A - ac) . sign(P) . A,   sign(P) = P / |P| (0 at 0).
+ * (fpm_pure_v3 applies its own clamp inside the pupil update.) */
+int fpm_pupil_constraint(fpm_c32* P, const float* A, int64_t count,
+                         double amplitude_constraint, void* stream);
 
 /* Optics set-up: LED illumination wave-vectors -> integer window offsets,
  * computed in FP64 (init_environment_rgb.m:33-134, main_run_FPM.m:23,53-55).

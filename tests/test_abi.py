@@ -180,6 +180,28 @@ def test_tiles_solve_rejects_bad_arguments():
     assert "tile_launch" in L.fpm_last_error().decode()
 
 
+def test_optimizer_entries_reject_bad_arguments():
+    """fpm_optimizer_step / fpm_pupil_constraint: unknown kind, null arrays,
+    missing moments, Adam without an iteration, negative counts:
+    FPM_E_INVALID before any device work."""
+    from fpm_amd import optimizers as fo
+    L = fo._olib()
+    buf = np.zeros(64, np.complex64)
+    p = buf.ctypes.data
+    assert L.fpm_optimizer_step(7, p, p, p, p, 4, 1e-3, 0.9, 0.999, 1, None) == -1
+    assert "kind" in L.fpm_last_error().decode()
+    assert L.fpm_optimizer_step(fo.OPT_SGD, None, p, None, None, 4, 1e-3, 0.9, 0.999, 1, None) == -1
+    assert L.fpm_optimizer_step(fo.OPT_ADAM, p, p, None, None, 4, 1e-3, 0.9, 0.999, 1, None) == -1
+    assert L.fpm_optimizer_step(fo.OPT_ADAM, p, p, p, p, 4, 1e-3, 0.9, 0.999, 0, None) == -1
+    assert L.fpm_optimizer_step(fo.OPT_YOGI, p, p, p, p, -1, 1e-3, 0.9, 0.999, 1, None) == -1
+    assert L.fpm_pupil_constraint(None, p, 4, 0.1, None) == -1
+    assert L.fpm_pupil_constraint(p, p, -2, 0.1, None) == -1
+    with pytest.raises(NotImplementedError):
+        fo.optimizer_RMSprop(0, 0, 0.9, 0.5, False, 1e-3)
+    with pytest.raises(ValueError):
+        fo.optimizer_adam(1, 0, 0.9, 0.999, 1e-3)
+
+
 def test_cnn_create_rejects_bad_descriptors():
     from fpm_amd import warmstart as ws
     L = ws._cnn_lib()

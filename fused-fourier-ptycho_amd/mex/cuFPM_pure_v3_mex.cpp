@@ -15,6 +15,7 @@
 #include "fpm_amd.h"
 
 #include <cstring>
+#include <initializer_list>
 
 namespace {
 
@@ -73,7 +74,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     o.pointers_are_device = dev ? 1 : 0;
     o.verbose = 1;  // the reference prints its parameters (cuFPM_pure_v3.cu:61,66)
 
-    mxArray *Oout, *Pout;
+    mxArray *Oout = nullptr, *Pout = nullptr;
     void *Op, *Pp;
     mxGPUArray *Og = nullptr, *Pg = nullptr;
     if (dev) {
@@ -93,7 +94,13 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                                mxGetScalar(prhs[6]), mxGetScalar(prhs[7]), opt, (fpm_c32*)Op, (fpm_c32*)Pp, &o);
     for (const In* v : {&O0, &P0, &Y, &led, &A})
         if (v->g) mxGPUDestroyGPUArray(v->g);
-    if (rc != FPM_OK) mexErrMsgIdAndTxt("fpm:solve", "%s", fpm_last_error());
+    if (rc != FPM_OK) {
+        if (Og) mxGPUDestroyGPUArray(Og);
+        if (Pg) mxGPUDestroyGPUArray(Pg);
+        if (Oout) mxDestroyArray(Oout);
+        if (Pout) mxDestroyArray(Pout);
+        mexErrMsgIdAndTxt("fpm:solve", "%s", fpm_last_error());
+    }
     if (dev) {
         plhs[0] = mxGPUCreateMxArrayOnGPU(Og);
         plhs[1] = mxGPUCreateMxArrayOnGPU(Pg);

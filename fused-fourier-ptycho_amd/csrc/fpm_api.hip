@@ -861,9 +861,15 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
                 else
                     singles.push_back(t);
             }
+            // a launch takes at most 65535 pages per grid dimension (T b of them)
+            const size_t max_group = std::max<size_t>(1, 65535 / B);
             for (auto& kv : by) {
-                if (kv.second.size() >= 2) groups.push_back(kv.second);
-                else singles.push_back(kv.second[0]);
+                const std::vector<int>& m = kv.second;
+                for (size_t a = 0; a < m.size(); a += max_group) {
+                    const size_t e = std::min(m.size(), a + max_group);
+                    if (e - a >= 2) groups.emplace_back(m.begin() + a, m.begin() + e);
+                    else singles.push_back(m[a]);
+                }
             }
         }
         std::vector<std::unique_ptr<GroupSolve>> gsv;

@@ -101,6 +101,21 @@ def test_grouped_tiles_above_1024_keep_the_fused_stitch():
         np.testing.assert_allclose(losses[k], l1, rtol=1e-12, atol=0, err_msg=str(k))
 
 
+def test_grouped_tiles_split_into_launch_sized_chunks():
+    """A launch holds at most 65535 pages per grid dimension: with batch 64
+    a group takes at most 1023 tiles, so 1030 tiles of one size run as two
+    groups (1023 + 7); tiles on both sides of the split equal their own
+    solves."""
+    tl = ft.make_tile_set(256, (16,), 33, 4, seed=2, led_grid=9, count=1030)
+    rep, _ = ft.tiles_solve(tl, 4, 1, 64, OPT, streams=1)
+    print(f"[tiles] 1030 tiles: report {rep}", flush=True)
+    assert rep["sequences"] == 2 and rep["grouped_tiles"] == 1030
+    for k in (0, 1022, 1023, 1029):
+        O, P, _ = _single(tl[k], 1, 64)
+        assert np.array_equal(tl[k]["O"].cpu().numpy(), O), k
+        assert np.array_equal(tl[k]["P"].cpu().numpy(), P), k
+
+
 def _single_opts(t, epochs, batch, **kw):
     O, P = torch.empty_like(t["O0"]), torch.empty_like(t["P0"])
     N, n = t["O0"].shape[0], t["P0"].shape[0]

@@ -679,8 +679,17 @@ bool fused_update_groupable(const Geom& g) { return g.Nc % (uint32_t)contig_line
 #ifndef FPM_STENCIL_MAG_LDS
 #define FPM_STENCIL_MAG_LDS 0
 #endif
+// Threads per workgroup: 1024 (16 waves, one workgroup per CU by LDS), but
+// 512 at LEN 256, where 32 columns (30 interior) per workgroup cover a
+// 256-column page with 288 transformed columns instead of 320 and two
+// workgroups share a CU (120 VGPRs, 70 KB of LDS; the 48-tile solve with
+// 24 tiles of n = 256 measured 22.07 -> 21.35 ms per epoch).  At larger LEN
+// fewer threads would only add halo columns.
+#ifndef FPM_STENCIL_T256
+#define FPM_STENCIL_T256 512
+#endif
 template <int LEN> struct StencilCfg {
-    static constexpr int T = 1024;
+    static constexpr int T = LEN == 256 ? FPM_STENCIL_T256 : 1024;
     static constexpr int LT = T * 16 / LEN;  // columns per workgroup
     static constexpr int K = LT - 2;          // interior columns
     using Map = LineSlowMap<LT, LEN>;
@@ -694,7 +703,7 @@ template <int LEN> struct StencilCfg {
 // (one workgroup per page instead of two with 2 interior columns in the
 // second: n = 128).
 template <int LEN, int ENG, bool WHOLE>
-__global__ __launch_bounds__(1024) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
+__global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,
                                                             float inv_ratio, float gscale,
                                                             double* __restrict__ loss_acc,
@@ -836,12 +845,12 @@ static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const 
 #define FPM_STF_CASE(L)                                                                                   \
     case L: {                                                                                             \
         if (g.nc == (uint32_t)StencilCfg<L>::LT) {                                                        \
-            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, true>), dim3(1, b), dim3(1024), 0, st, z_cols, u_out, g, \
-                               y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt);       \
+            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, true>), dim3(1, b), dim3(StencilCfg<L>::T), 0, st, z_cols, \
+                               u_out, g, y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt); \
         } else {                                                                                          \
             dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                               \
-            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, false>), grid, dim3(1024), 0, st, z_cols, u_out, g, y, \
-                               led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt);          \
+            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, false>), grid, dim3(StencilCfg<L>::T), 0, st, z_cols, \
+                               u_out, g, y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt); \
         }                                                                                                 \
         break;                                                                                            \
     }

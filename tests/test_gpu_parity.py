@@ -67,7 +67,7 @@ def test_forward_intensity_wrapped_windows():
 
 # -------------------------------------------------------------- gradient
 @pytest.mark.parametrize("stitch", [fpm_amd.STITCH_ATOMIC, fpm_amd.STITCH_OWNER])
-@pytest.mark.parametrize("n,r,grid", [(16, 2, 3), (32, 4, 5), (64, 16, 3), (128, 2, 3), (1024, 2, 3)])
+@pytest.mark.parametrize("n,r,grid", [(16, 2, 3), (32, 4, 5), (64, 16, 3), (128, 2, 3), (256, 2, 3), (1024, 2, 3)])
 def test_gradient_matches_fp64_oracle(stitch, n, r, grid):
     """One-minibatch gradient vs the FP64 oracle, rel-L2 <= 1e-5 (inputs with
     a >= 0.01 margin at every loss-stencil sign, so no kink is ambiguous).

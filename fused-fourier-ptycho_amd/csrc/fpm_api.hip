@@ -19,6 +19,7 @@
 #include "fft_core.h"
 #include "fft_kernels.h"
 #include "fpm_kernels.h"
+#include "generic.h"
 
 #include <algorithm>
 #include <cmath>
@@ -125,9 +126,9 @@ void check_device() {
 }
 
 void check_side(int64_t v, const char* what) {
-    if (!is_pow2(v) || v < 16 || v > 16384)
+    if (!((is_pow2(v) && v >= 16 && v <= 16384) || generic_side(v)))
         invalid(std::string(what) + " = " + std::to_string(v) +
-                    ": every image side must be a power of two in [16, 16384]",
+                    ": every image side must be a power of two in [16, 16384] or an even 2^a 3^b in [16, 4096]",
                 FPM_E_UNSUPPORTED);
 }
 
@@ -138,6 +139,30 @@ Geom make_geom(int64_t Nr, int64_t Nc, int64_t nr, int64_t nc) {
     check_side(nc, "n_cols");
     if (nr > Nr || nc > Nc) invalid("the pupil/images are larger than the object");
     Geom g;
+    if (!(is_pow2(Nr) && is_pow2(Nc) && is_pow2(nr) && is_pow2(nc))) {
+        // a side that is not a power of two: the generic path for all four
+        for (int64_t v : {Nr, Nc, nr, nc})
+            if (!generic_side(v))
+                invalid("with a side that is not a power of two every side must be an even 2^a 3^b in [16, 4096]",
+                        FPM_E_UNSUPPORTED);
+        g.nr = (uint32_t)nr;
+        g.nc = (uint32_t)nc;
+        g.Nr = (uint32_t)Nr;
+        g.Nc = (uint32_t)Nc;
+        g.lnr = 0;
+        g.generic = true;
+        ColPerm id{};
+        g.pn = id;
+        g.pN = id;
+        g.pn.M1 = g.nc;
+        g.pn.M2 = 1;
+        g.pn.C = g.nc;
+        g.pN.M1 = g.Nc;
+        g.pN.M2 = 1;
+        g.pN.C = g.Nc;
+        g.nl = colmajor_layout(g.nr, g.pn);
+        return g;
+    }
     g.nr = (uint32_t)nr;
     g.nc = (uint32_t)nc;
     g.Nr = (uint32_t)Nr;
@@ -410,7 +435,7 @@ struct Solve {
     const float2* tw = nullptr;
     RmsArgs rms{};
     SolveState ss{};
-    DevBuf S0, S1, mO, mP, state, masks;
+    DevBuf S0, S1, mO, mP, state, masks, ebuf;
     Workspace ws;
     float2* spec[2] = {nullptr, nullptr};
     uint8_t* mk[2] = {nullptr, nullptr};
@@ -428,6 +453,10 @@ struct Solve {
               hipStream_t st, StageTimer& tm) {
         const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
         steps = (L + B - 1) / B;
+        if (g.generic) {
+            init_generic(ne, want_loss, lr, beta, eps, clamp, halve_from, st);
+            return;
+        }
         S0.alloc(NN * sizeof(float2));
         S1.alloc(NN * sizeof(float2));
         mO.alloc(NN * sizeof(float));
@@ -476,6 +505,10 @@ struct Solve {
     // index and the loss slot live in device memory, so an epoch can be
     // captured and replayed.
     void enqueue_epoch(int cur, bool last_epoch, hipStream_t st, StageTimer& tm) {
+        if (g.generic) {
+            enqueue_epoch_generic(st);
+            return;
+        }
         uint32_t led0, b;
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
@@ -503,6 +536,54 @@ struct Solve {
 
     // spectrum buffer parity at the start of the next epoch
     int parity_after(int cur) const { return (int)((cur + steps) & 1u); }
+
+    // generic sides (generic.hip): S = DFT_N(O) every step into S0, the
+    // gradient spectrum and G_O in S1, the residuals in ebuf
+    void init_generic(int ne, bool want_loss, float lr, float beta, float eps, float clamp, int32_t halve_from,
+                      hipStream_t st) {
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        S0.alloc(NN * sizeof(float2));
+        S1.alloc(NN * sizeof(float2));
+        mO.alloc(NN * sizeof(float));
+        mP.alloc(nn * sizeof(float));
+        ws.lat.alloc(nn * B * sizeof(float2));
+        ws.rec.alloc(nn * B * sizeof(float2));
+        ebuf.alloc(nn * B * sizeof(float));
+        FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
+        FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
+        state.alloc(64 + (want_loss ? (size_t)ne * sizeof(double) : 0));
+        ss = SolveState{};
+        ss.lr = (float*)(state.as<char>() + 8);
+        ss.epoch = (uint32_t*)(state.as<char>() + 16);
+        ss.loss_cur = want_loss ? (double*)state.p : nullptr;
+        ss.loss_hist = want_loss ? (double*)(state.as<char>() + 64) : nullptr;
+        ss.halve_from = halve_from > 0 ? (uint32_t)halve_from : 13u;
+        launch_solve_state_init(ss, lr, st);
+        rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
+        generic_prepare(g);
+    }
+
+    void enqueue_epoch_generic(hipStream_t st) {
+        const size_t NN = (size_t)g.Nr * g.Nc;
+        float2* S = S0.as<float2>();
+        float2* G = S1.as<float2>();
+        float2* lat = ws.lat.as<float2>();
+        float2* rec = ws.rec.as<float2>();
+        uint32_t led0, b;
+        for (uint32_t s = 0; s < steps; ++s) {
+            batch_of(s, led0, b);
+            fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
+            generic_crop(S, P, led, led0, b, g, lat, st);
+            fft2_generic(lat, lat, g.nr, g.nc, b, true, st);
+            generic_stencil(lat, y, led0, b, pr, g, ebuf.as<float>(), rec, ss.loss_cur, st);
+            fft2_generic(rec, rec, g.nr, g.nc, b, false, st);
+            generic_stitch(rec, P, led, led0, b, g, G, st);  // reads this step's pupil
+            generic_pupil(rec, S, led, led0, b, g, P, mP.as<float>(), A, rms, nullptr, st);
+            fft2_generic(G, G, g.Nr, g.Nc, 1, true, st);
+            generic_update(G, O, mO.as<float>(), NN, rms, st);
+        }
+        launch_epoch_end(ss, st);
+    }
 };
 
 // Tiles of one size and image type run through shared launches: every stage
@@ -541,7 +622,7 @@ struct GroupSolve {
 
     // tiles of one group can share launches
     static bool groupable(const Geom& g, uint32_t B, int stitch) {
-        return stitch == FPM_STITCH_OWNER && B <= (uint32_t)64 && stencil_fused_supported(g) &&
+        return !g.generic && stitch == FPM_STITCH_OWNER && B <= (uint32_t)64 && stencil_fused_supported(g) &&
                fused_update_groupable(g);
     }
 
@@ -1110,13 +1191,6 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             FPM_HIP(hipMemsetAsync(lossd.p, 0, sizeof(double), st));
         }
         StageTimer tm(o.stage_times, st);
-        tm.begin(FPM_STAGE_SPECTRUM_INIT);
-        fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st, g.eng);
-        fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, nullptr, g.eng);
-        tm.end(1 + col_launches(g.pN));
-        minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
-                           o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
-                           loss_out ? lossd.as<double>() : nullptr, tw, st, tm, nullptr, nullptr);
         float2* GOd;
         if (isdev) {
             GOd = (float2*)G_O;
@@ -1124,7 +1198,34 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             gout.alloc(NN * sizeof(float2));
             GOd = gout.as<float2>();
         }
-        fft_contig(S1.as<float2>(), GOd, g.Nr, g.Nc, true, tw, st, g.eng);
+        if (g.generic) {
+            // generic sides (generic.hip): FP32 mixed-radix transforms, owner stitch
+            DevBuf eb;
+            eb.alloc(nn * (size_t)L * sizeof(float));
+            generic_prepare(g);
+            float2* S = S0.as<float2>();
+            float2* lat = ws.lat.as<float2>();
+            float2* rec = ws.rec.as<float2>();
+            const uint32_t b = (uint32_t)L;
+            fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
+            generic_crop(S, pcopy.as<float2>(), led, 0, b, g, lat, st);
+            fft2_generic(lat, lat, g.nr, g.nc, b, true, st);
+            generic_stencil(lat, y, 0, b, pr, g, eb.as<float>(), rec, loss_out ? lossd.as<double>() : nullptr, st);
+            fft2_generic(rec, rec, g.nr, g.nc, b, false, st);
+            generic_stitch(rec, pcopy.as<float2>(), led, 0, b, g, S1.as<float2>(), st);
+            generic_pupil(rec, S, led, 0, b, g, pcopy.as<float2>(), nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f},
+                          GPd, st);
+            fft2_generic(S1.as<float2>(), GOd, g.Nr, g.Nc, 1, true, st);
+        } else {
+            tm.begin(FPM_STAGE_SPECTRUM_INIT);
+            fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st, g.eng);
+            fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, nullptr, g.eng);
+            tm.end(1 + col_launches(g.pN));
+            minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
+                               o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
+                               loss_out ? lossd.as<double>() : nullptr, tw, st, tm, nullptr, nullptr);
+            fft_contig(S1.as<float2>(), GOd, g.Nr, g.Nc, true, tw, st, g.eng);
+        }
         if (!isdev) {
             FPM_HIP(hipMemcpyAsync(G_O, GOd, NN * sizeof(float2), hipMemcpyDeviceToHost, st));
             FPM_HIP(hipMemcpyAsync(G_P, GPd, nn * sizeof(float2), hipMemcpyDeviceToHost, st));
@@ -1164,15 +1265,26 @@ int fpm_forward_intensity(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, c
             outd.alloc(nn * (size_t)L * sizeof(float));
             od = outd.as<float>();
         }
-        fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st);
-        fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
         const float scale = (float)((double)nn * (double)pr * (double)pr);
-        for (int64_t l0 = 0; l0 < L; l0 += chunk) {
-            const uint32_t b = (uint32_t)std::min<int64_t>(chunk, L - l0);
-            const CropArgs ca{S0.as<float2>(), P, led, (uint32_t)l0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
-            fft_cols_inverse(lat.as<float2>(), g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca);
-            fft_contig(lat.as<float2>(), lat.as<float2>(), g.nr, g.nc * b, true, tw, st);
-            launch_abs_scaled(lat.as<float2>(), nn * b, scale, squared, od + nn * (size_t)l0, st);
+        if (g.generic) {
+            generic_prepare(g);
+            fft2_generic(O, S0.as<float2>(), g.Nr, g.Nc, 1, false, st);
+            for (int64_t l0 = 0; l0 < L; l0 += chunk) {
+                const uint32_t b = (uint32_t)std::min<int64_t>(chunk, L - l0);
+                generic_crop(S0.as<float2>(), P, led, (uint32_t)l0, b, g, lat.as<float2>(), st);
+                fft2_generic(lat.as<float2>(), lat.as<float2>(), g.nr, g.nc, b, true, st);
+                launch_abs_scaled(lat.as<float2>(), nn * b, scale, squared, od + nn * (size_t)l0, st);
+            }
+        } else {
+            fft_contig(O, S0.as<float2>(), g.Nr, g.Nc, false, tw, st);
+            fft_cols_forward(S0.as<float2>(), g.Nr, g.Nc, 1, g.pN, tw, st, nullptr);
+            for (int64_t l0 = 0; l0 < L; l0 += chunk) {
+                const uint32_t b = (uint32_t)std::min<int64_t>(chunk, L - l0);
+                const CropArgs ca{S0.as<float2>(), P, led, (uint32_t)l0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
+                fft_cols_inverse(lat.as<float2>(), g.nr, g.nc, b, g.pn, tw, st, nullptr, &ca);
+                fft_contig(lat.as<float2>(), lat.as<float2>(), g.nr, g.nc * b, true, tw, st);
+                launch_abs_scaled(lat.as<float2>(), nn * b, scale, squared, od + nn * (size_t)l0, st);
+            }
         }
         if (!isdev) FPM_HIP(hipMemcpyAsync(out, od, nn * (size_t)L * sizeof(float), hipMemcpyDeviceToHost, st));
         FPM_HIP(hipStreamSynchronize(st));
@@ -1212,7 +1324,13 @@ int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
             dst = dout.as<float2>();
         }
         const uint32_t R = (uint32_t)rows, C = (uint32_t)cols, Bt = (uint32_t)batch;
-        if (direction < 0) {
+        if (!is_pow2(rows) || !is_pow2(cols)) {
+            // a side that is not a power of two: the generic mixed-radix transform (FP32)
+            if (!generic_side(rows) || !generic_side(cols))
+                invalid("with a side that is not a power of two both sides must be even 2^a 3^b in [16, 4096]",
+                        FPM_E_UNSUPPORTED);
+            fft2_generic(src, dst, R, C, Bt, direction > 0, st);
+        } else if (direction < 0) {
             fft_contig(src, dst, R, C * Bt, false, tw, st, eng);
             if (p.M2 == 1) {
                 fft_cols_forward(dst, R, C, Bt, p, tw, st, nullptr, nullptr, nullptr, eng);

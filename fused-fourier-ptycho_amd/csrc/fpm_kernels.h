@@ -13,6 +13,7 @@ struct Geom {
     ColPerm pN;        // column permutation of the N-side spectrum
     PageLayout nl;     // layout of the per-LED pages of a minibatch (tiled or column-major with pn)
     int eng = ENG_F32; // FFT engine of every transform of the solve
+    bool generic = false;  // a side is not a power of two: the generic (mixed-radix, unfused) path
     uint32_t plane() const { return nr * nc; }
 };
 

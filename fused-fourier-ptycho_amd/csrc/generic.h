@@ -1,0 +1,39 @@
+// generic.h -- the solve's steps for even sides 2^a 3^b that are not powers
+// of two (generic.hip): mixed-radix transforms and unfused kernels in
+// natural order.
+#pragma once
+#include "common.h"
+#include "fpm_kernels.h"
+
+namespace fpm {
+
+// largest side of the generic path (its FFT holds up to 4096 points per
+// workgroup in LDS)
+constexpr int kGenericMaxSide = 4096;
+
+// even v = 2^a 3^b in [16, kGenericMaxSide] (powers of two included)
+bool generic_side(int64_t v);
+
+// unnormalised 2D C2C transforms of `batch` column-major rows x cols pages,
+// natural order in and out (in == out allowed); rows, cols generic sides
+void fft2_generic(const float2* in, float2* out, uint32_t rows, uint32_t cols, uint32_t batch, bool inverse,
+                  hipStream_t st);
+// the twiddle tables of the geometry's four lengths, made before any capture
+void generic_prepare(const Geom& g);
+
+// crop x pupil of LEDs led0.. into b ifftshifted natural pages
+void generic_crop(const float2* S, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
+                  float2* out, hipStream_t st);
+// residual (ebuf: b n_r n_c floats) and loss stencil: u from z; loss_acc may be null
+void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
+                     float* ebuf, float2* u, double* loss_acc, hipStream_t st);
+// owner-computes stitch of the b back-projections into G_S (every element written)
+void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
+                    float2* Gs, hipStream_t st);
+// pupil gradient; with mP and A the RMSprop + clamp update of P, else GP = gradient
+void generic_pupil(const float2* R, const float2* S, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
+                   float2* P, float* mP, const float* A, RmsArgs rms, float2* GP, hipStream_t st);
+// RMSprop of O from its gradient G (n elements)
+void generic_update(const float2* G, float2* O, float* mO, size_t n, RmsArgs rms, hipStream_t st);
+
+}  // namespace fpm

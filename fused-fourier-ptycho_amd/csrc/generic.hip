@@ -1,0 +1,409 @@
+// generic.hip -- the solve for even sides that are not powers of two
+// (2^a 3^b, at most 4096: field-of-view tiles of 192 px -> 768 px, and the
+// like), which the power-of-two kernels cannot take (their index maps are bit
+// masks and their FFT passes radix-16 slot maps).  The steps are those of
+// cuFPM_v3/cuFPM_pure_v3.cu:128-207 without the fusions, in natural order:
+//   S = DFT_N(O); crop x pupil (kernels.cuh:19-65); z = IDFT_n; loss stencil
+//   (kernels.cuh:197-275); R = DFT_n(u); owner-computes stitch and the pupil
+//   gradient / RMSprop + clamp (kernels.cuh:67-129, 384-410);
+//   G_O = IDFT_N(G_S); RMSprop of O (kernels.cuh:364-382).
+// The transforms are a self-sorting (Stockham) mixed-radix FFT with radices
+// 4, 2 and 3: pass (r, n, s) maps
+//   y[q + s (r p + k)] = (sum_j x[q + s (p + j n/r)] W_r^(j k)) W_n^(p k),
+// lines staged in LDS, twiddles from an FP64-generated table of W_len^k.
+#include "../../include/fpm_amd.h"
+#include "common.h"
+#include "fpm_kernels.h"
+#include "generic.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace fpm {
+
+bool generic_side(int64_t v) {
+    if (v < 16 || v > kGenericMaxSide || (v & 1)) return false;
+    while (v % 2 == 0) v /= 2;
+    while (v % 3 == 0) v /= 3;
+    return v == 1;
+}
+
+namespace {
+
+__device__ __forceinline__ float2 cmul_g(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc_g(float2 a, float2 b) {  // a conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+struct GenFFT {
+    const float2* in;
+    float2* out;
+    const float2* tw;  // W_len^k, k < len
+    uint64_t batch_stride, line_stride, es;
+    uint32_t len, nlines, lines_per_batch, tl, nf, inverse;
+    uint8_t fac[24];
+};
+
+// element e of line L at (L / lpb) batch_stride + (L % lpb) line_stride + e es
+__device__ __forceinline__ size_t gen_off(const GenFFT& a, uint32_t L, uint32_t e) {
+    return (size_t)(L / a.lines_per_batch) * a.batch_stride + (size_t)(L % a.lines_per_batch) * a.line_stride +
+           (size_t)e * a.es;
+}
+
+template <int R>
+__device__ __forceinline__ void gen_pass(const GenFFT& a, const float2* x, float2* y, uint32_t nl, uint32_t n,
+                                         uint32_t s) {
+    const uint32_t m = n / R, nb = a.len / R, step_r = a.len / R, step_n = a.len / n;
+    for (uint32_t bi = threadIdx.x; bi < nl * nb; bi += blockDim.x) {
+        const uint32_t l = bi / nb, b = bi - l * nb;
+        const uint32_t p = b / s, q = b - p * s;
+        const float2* xl = x + (size_t)l * a.len;
+        float2* yl = y + (size_t)l * a.len;
+        float2 v[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = xl[q + s * (p + j * m)];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            float2 acc = v[0];
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+                float2 w = a.tw[((j * k) % R) * step_r];
+                if (a.inverse) w.y = -w.y;
+                const float2 t = cmul_g(v[j], w);
+                acc.x += t.x;
+                acc.y += t.y;
+            }
+            float2 w = a.tw[((p * (uint32_t)k) % n) * step_n];
+            if (a.inverse) w.y = -w.y;
+            yl[q + s * (R * p + k)] = cmul_g(acc, w);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void fft_generic_kernel(GenFFT a) {
+    extern __shared__ float2 sm[];
+    float2* x = sm;
+    float2* y = sm + (size_t)a.tl * a.len;
+    const uint32_t L0 = blockIdx.x * a.tl;
+    const uint32_t nl = min(a.tl, a.nlines - L0);
+    const uint32_t tot = nl * a.len;
+    const bool line_fast = a.es != 1;  // strided lines: adjacent lines are adjacent in memory
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const uint32_t l = line_fast ? idx % nl : idx / a.len;
+        const uint32_t e = line_fast ? idx / nl : idx % a.len;
+        x[(size_t)l * a.len + e] = a.in[gen_off(a, L0 + l, e)];
+    }
+    __syncthreads();
+    uint32_t n = a.len, s = 1;
+    for (uint32_t f = 0; f < a.nf; ++f) {
+        const uint32_t r = a.fac[f];
+        if (r == 4) gen_pass<4>(a, x, y, nl, n, s);
+        else if (r == 2) gen_pass<2>(a, x, y, nl, n, s);
+        else gen_pass<3>(a, x, y, nl, n, s);
+        __syncthreads();
+        float2* t = x;
+        x = y;
+        y = t;
+        n /= r;
+        s *= r;
+    }
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const uint32_t l = line_fast ? idx % nl : idx / a.len;
+        const uint32_t e = line_fast ? idx / nl : idx % a.len;
+        a.out[gen_off(a, L0 + l, e)] = x[(size_t)l * a.len + e];
+    }
+}
+
+const float2* generic_twiddles(uint32_t len) {
+    static std::mutex mu;
+    static std::map<std::pair<int, uint32_t>, float2*> tables;
+    int dev = 0;
+    FPM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = tables.find({dev, len});
+    if (it != tables.end()) return it->second;
+    std::vector<float2> h(len);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint32_t k = 0; k < len; ++k) {
+        const double ang = -two_pi * (double)k / (double)len;
+        h[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    float2* d = nullptr;
+    FPM_HIP(hipMalloc(&d, sizeof(float2) * len));
+    FPM_HIP(hipMemcpy(d, h.data(), sizeof(float2) * len, hipMemcpyHostToDevice));
+    tables[{dev, len}] = d;
+    return d;
+}
+
+void set_factors(GenFFT& a, uint32_t len) {
+    uint32_t m = len;
+    a.nf = 0;
+    while (m % 4 == 0) { a.fac[a.nf++] = 4; m /= 4; }
+    while (m % 2 == 0) { a.fac[a.nf++] = 2; m /= 2; }
+    while (m % 3 == 0) { a.fac[a.nf++] = 3; m /= 3; }
+    if (m != 1 || a.nf > 24) throw HipError{hipErrorInvalidValue, "generic FFT: length " + std::to_string(len) +
+                                                                     " is not 2^a 3^b"};
+}
+
+void launch_generic_lines(GenFFT a, hipStream_t st) {
+    if (a.len < 2 || a.len > (uint32_t)kGenericMaxSide)
+        throw HipError{hipErrorInvalidValue, "generic FFT: length out of range"};
+    set_factors(a, a.len);
+    a.tw = generic_twiddles(a.len);
+    a.tl = std::max<uint32_t>(1, 4096u / a.len);
+    const size_t shmem = 2 * (size_t)a.tl * a.len * sizeof(float2);
+    const uint32_t blocks = (a.nlines + a.tl - 1) / a.tl;
+    hipLaunchKernelGGL(fft_generic_kernel, dim3(blocks), dim3(256), shmem, st, a);
+    FPM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------- solve kernels
+// crop x pupil into ifftshifted pages (kernels.cuh:38-64): page l pixel
+// (di, dj) = S(window of LED led0 + l at pupil pixel (i, j)) P(i, j),
+// i = (di + nr/2) mod nr, j = (dj + nc/2) mod nc; pages column-major, natural.
+__global__ __launch_bounds__(256) void gen_crop_kernel(const float2* __restrict__ S, const float2* __restrict__ P,
+                                                       const int2* __restrict__ led, uint32_t led0, uint32_t b,
+                                                       uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
+                                                       float2* __restrict__ out) {
+    const size_t plane = (size_t)nr * nc, total = plane * b;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total; k += (size_t)gridDim.x * 256) {
+        const uint32_t l = (uint32_t)(k / plane);
+        const uint32_t pix = (uint32_t)(k - (size_t)l * plane);
+        const uint32_t dj = pix / nr, di = pix - dj * nr;
+        const uint32_t i = (di + nr / 2) % nr, j = (dj + nc / 2) % nc;
+        const int2 kk = led[led0 + l];
+        const uint32_t srow = (i + (uint32_t)kk.y - 1u + Nr / 2) % Nr;
+        const uint32_t scol = (j + (uint32_t)kk.x - 1u + Nc / 2) % Nc;
+        const float2 pp = P[(size_t)j * nr + i];
+        float2 z = make_float2(0.f, 0.f);
+        if (pp.x != 0.0f || pp.y != 0.0f) z = cmul_g(S[(size_t)scol * Nr + srow], pp);
+        out[k] = z;
+    }
+}
+
+// e = |z| / (n_r n_c r^2) - y of every pixel of the b pages
+__global__ __launch_bounds__(256) void gen_residual_kernel(const float2* __restrict__ z, YStack y, uint32_t led0,
+                                                           size_t plane, size_t total, float ratio, float inv_ratio,
+                                                           float* __restrict__ e) {
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total; k += (size_t)gridDim.x * 256) {
+        const float2 v = z[k];
+        const float mag = sqrtf(v.x * v.x + v.y * v.y);
+        e[k] = scaled_mag(mag, ratio, inv_ratio) - read_y(y, (size_t)led0 * plane + k);
+    }
+}
+
+// the loss stencil (kernels.cuh:197-275) with cyclic neighbours:
+// g = sign(e - e_left) - sign(e_right - e) + sign(e - e_up) - sign(e_down - e)
+// (copysign semantics), u = r^2 g z/|z| (0 where |z| = 0), loss += |Dx e| + |Dy e|
+__global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restrict__ z, const float* __restrict__ e,
+                                                          uint32_t nr, uint32_t nc, size_t total, float gscale,
+                                                          float2* __restrict__ u, double* __restrict__ loss_acc) {
+    __shared__ double red[4];
+    const size_t plane = (size_t)nr * nc;
+    float loss32 = 0.0f;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total; k += (size_t)gridDim.x * 256) {
+        const size_t l = k / plane;
+        const uint32_t pix = (uint32_t)(k - l * plane);
+        const uint32_t j = pix / nr, i = pix - j * nr;
+        const float* el = e + l * plane;
+        const float e0 = el[(size_t)j * nr + i];
+        const float eL = el[(size_t)((j + nc - 1) % nc) * nr + i];
+        const float eR = el[(size_t)((j + 1) % nc) * nr + i];
+        const float eU = el[(size_t)j * nr + (i + nr - 1) % nr];
+        const float eD = el[(size_t)j * nr + (i + 1) % nr];
+        const float gval =
+            (copysignf(1.0f, e0 - eL) - copysignf(1.0f, eR - e0) + copysignf(1.0f, e0 - eU) - copysignf(1.0f, eD - e0)) *
+            gscale;
+        loss32 += fabsf(eR - e0) + fabsf(eD - e0);
+        const float2 v = z[k];
+        const float mag = sqrtf(v.x * v.x + v.y * v.y);
+        float2 o = make_float2(0.f, 0.f);
+        if (mag > 0.0f) {
+            const float inv = 1.0f / mag;
+            o = make_float2(v.x * inv * gval, v.y * inv * gval);
+        }
+        u[k] = o;
+    }
+    if (loss_acc) {
+        double loss = (double)loss32;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = loss;
+        __syncthreads();
+        if (threadIdx.x == 0) unsafeAtomicAdd(loss_acc, red[0] + red[1] + red[2] + red[3]);
+    }
+}
+
+// owner-computes stitch: G_S(row, col) = sum over the batch's LEDs, in LED
+// order, of R_l(fftshifted window pixel) conj(P) / (N_r N_c) (kernels.cuh:
+// 107-116 with the atomics replaced by a gather); every element written.
+__global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restrict__ R, const float2* __restrict__ P,
+                                                         const int2* __restrict__ led, uint32_t led0, uint32_t b,
+                                                         uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
+                                                         float invN2, float2* __restrict__ Gs) {
+    const size_t NN = (size_t)Nr * Nc, plane = (size_t)nr * nc;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < NN; k += (size_t)gridDim.x * 256) {
+        const uint32_t col = (uint32_t)(k / Nr), row = (uint32_t)(k - (size_t)col * Nr);
+        float2 v = make_float2(0.f, 0.f);
+        for (uint32_t l = 0; l < b; ++l) {
+            const int2 kk = led[led0 + l];
+            const uint32_t wr = ((uint32_t)kk.y - 1u + Nr / 2) % Nr;
+            const uint32_t wc = ((uint32_t)kk.x - 1u + Nc / 2) % Nc;
+            const uint32_t li = (row + Nr - wr) % Nr, lj = (col + Nc - wc) % Nc;
+            if (li < nr && lj < nc) {
+                const float2 pp = P[(size_t)lj * nr + li];
+                if (pp.x != 0.0f || pp.y != 0.0f) {
+                    const uint32_t di = (li + nr / 2) % nr, dj = (lj + nc / 2) % nc;
+                    const float2 gw = cmulc_g(R[(size_t)l * plane + (size_t)dj * nr + di], pp);
+                    v.x += gw.x * invN2;
+                    v.y += gw.y * invN2;
+                }
+            }
+        }
+        Gs[k] = v;
+    }
+}
+
+// pupil gradient sum_l R_l conj(S_l) in LED order (kernels.cuh:117-128), then
+// the RMSprop + clamp of kernels.cuh:384-410 (mP, A given) or GP = gradient
+__global__ __launch_bounds__(256) void gen_pupil_kernel(const float2* __restrict__ R, const float2* __restrict__ S,
+                                                        const int2* __restrict__ led, uint32_t led0, uint32_t b,
+                                                        uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
+                                                        float2* __restrict__ P, float* __restrict__ mP,
+                                                        const float* __restrict__ A, RmsArgs rms,
+                                                        float2* __restrict__ GP) {
+    const uint32_t plane = nr * nc;
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= plane) return;
+    const uint32_t j = pix / nr, i = pix - j * nr;
+    const uint32_t di = (i + nr / 2) % nr, dj = (j + nc / 2) % nc;
+    float2 gp = make_float2(0.f, 0.f);
+    for (uint32_t l = 0; l < b; ++l) {
+        const float2 Rv = R[(size_t)l * plane + (size_t)dj * nr + di];
+        const int2 kk = led[led0 + l];
+        const uint32_t srow = (i + (uint32_t)kk.y - 1u + Nr / 2) % Nr;
+        const uint32_t scol = (j + (uint32_t)kk.x - 1u + Nc / 2) % Nc;
+        const float2 gq = cmulc_g(Rv, S[(size_t)scol * Nr + srow]);
+        gp.x += gq.x;
+        gp.y += gq.y;
+    }
+    if (mP) {
+        const float2 p = P[pix];
+        float mu = gp.x * gp.x + gp.y * gp.y;
+        mu = rms.beta * mP[pix] + (1.0f - rms.beta) * mu;
+        const float sq = rms_lr(rms) / (sqrtf(mu) + rms.eps);
+        float2 w = p;
+        w.x -= gp.x * sq;
+        w.y -= gp.y * sq;
+        const float a = A[pix];
+        const float ang = atan2f(w.y, w.x);
+        float ab = sqrtf(w.x * w.x + w.y * w.y);
+        ab = a * fmaxf(fminf(ab, a + rms.clamp), a - rms.clamp);
+        w.x = cosf(ang) * ab;
+        w.y = sinf(ang) * ab;
+        P[pix] = w;
+        mP[pix] = mu;
+    } else {
+        GP[pix] = gp;
+    }
+}
+
+// RMSprop of O (kernels.cuh:364-382): m = beta m + (1-beta)|G|^2,
+// O -= lr G / (sqrt(m) + eps)
+__global__ __launch_bounds__(256) void gen_update_kernel(const float2* __restrict__ G, float2* __restrict__ O,
+                                                         float* __restrict__ mO, size_t n, RmsArgs rms) {
+    const float lr = rms_lr(rms);
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const float2 gr = G[k];
+        float mu = gr.x * gr.x + gr.y * gr.y;
+        mu = rms.beta * mO[k] + (1.0f - rms.beta) * mu;
+        const float sq = lr / (sqrtf(mu) + rms.eps);
+        float2 w = O[k];
+        w.x -= gr.x * sq;
+        w.y -= gr.y * sq;
+        O[k] = w;
+        mO[k] = mu;
+    }
+}
+
+unsigned gblocks(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1u << 16)); }
+
+}  // namespace
+
+void generic_prepare(const Geom& g) {
+    for (uint32_t len : {g.nr, g.nc, g.Nr, g.Nc}) (void)generic_twiddles(len);
+}
+
+void fft2_generic(const float2* in, float2* out, uint32_t rows, uint32_t cols, uint32_t batch, bool inverse,
+                  hipStream_t st) {
+    // along the contiguous index: one line per column of every page
+    GenFFT a{};
+    a.in = in;
+    a.out = out;
+    a.len = rows;
+    a.nlines = cols * batch;
+    a.lines_per_batch = a.nlines;
+    a.line_stride = rows;
+    a.es = 1;
+    a.batch_stride = 0;
+    a.inverse = inverse ? 1 : 0;
+    launch_generic_lines(a, st);
+    // along the strided index: one line per row of every page, in place
+    GenFFT c{};
+    c.in = out;
+    c.out = out;
+    c.len = cols;
+    c.nlines = rows * batch;
+    c.lines_per_batch = rows;
+    c.line_stride = 1;
+    c.es = rows;
+    c.batch_stride = (uint64_t)rows * cols;
+    c.inverse = a.inverse;
+    launch_generic_lines(c, st);
+}
+
+void generic_crop(const float2* S, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
+                  float2* out, hipStream_t st) {
+    hipLaunchKernelGGL(gen_crop_kernel, dim3(gblocks((size_t)g.nr * g.nc * b)), dim3(256), 0, st, S, P, led, led0, b,
+                       g.nr, g.nc, g.Nr, g.Nc, out);
+    FPM_LAUNCH_CHECK();
+}
+
+void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
+                     float* ebuf, float2* u, double* loss_acc, hipStream_t st) {
+    const size_t plane = (size_t)g.nr * g.nc, total = plane * b;
+    const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
+    const float inv_ratio = exact_inverse_or_zero(ratio);
+    hipLaunchKernelGGL(gen_residual_kernel, dim3(gblocks(total)), dim3(256), 0, st, z, y, led0, plane, total, ratio,
+                       inv_ratio, ebuf);
+    FPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gen_stencil_kernel, dim3(gblocks(total)), dim3(256), 0, st, z, ebuf, g.nr, g.nc, total,
+                       (float)(pratio * pratio), u, loss_acc);
+    FPM_LAUNCH_CHECK();
+}
+
+void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
+                    float2* Gs, hipStream_t st) {
+    hipLaunchKernelGGL(gen_stitch_kernel, dim3(gblocks((size_t)g.Nr * g.Nc)), dim3(256), 0, st, R, P, led, led0, b,
+                       g.nr, g.nc, g.Nr, g.Nc, 1.0f / (float)(g.Nr * g.Nc), Gs);
+    FPM_LAUNCH_CHECK();
+}
+
+void generic_pupil(const float2* R, const float2* S, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
+                   float2* P, float* mP, const float* A, RmsArgs rms, float2* GP, hipStream_t st) {
+    hipLaunchKernelGGL(gen_pupil_kernel, dim3((g.plane() + 255) / 256), dim3(256), 0, st, R, S, led, led0, b, g.nr,
+                       g.nc, g.Nr, g.Nc, P, mP, A, rms, GP);
+    FPM_LAUNCH_CHECK();
+}
+
+void generic_update(const float2* G, float2* O, float* mO, size_t n, RmsArgs rms, hipStream_t st) {
+    hipLaunchKernelGGL(gen_update_kernel, dim3(gblocks(n)), dim3(256), 0, st, G, O, mO, n, rms);
+    FPM_LAUNCH_CHECK();
+}
+
+}  // namespace fpm

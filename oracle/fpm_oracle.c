@@ -16,6 +16,7 @@
  * against an elementwise scalar restatement and the pupil-amplitude invariant.
  */
 #define _GNU_SOURCE
+#include <stdio.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>

@@ -26,12 +26,17 @@ typedef struct { REAL re, im; } CT;
 static inline CT FN(cmul)(CT a, CT b) { CT r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
 
 /* ------------------------------------------------------------------ FFT */
-/* Iterative radix-2 decimation-in-time on a contiguous line; twiddles are
- * computed in double and rounded once to REAL. */
+/* Iterative radix-2 decimation-in-time on a contiguous line for powers of
+ * two; for other even lengths 2^a 3^b a self-sorting (Stockham) mixed-radix
+ * transform with radices 4, 2, 3: pass (r, n, s) maps
+ *   y[q + s (r p + k)] = (sum_j x[q + s (p + j n/r)] W_r^(j k)) W_n^(p k).
+ * Twiddles are computed in double and rounded once to REAL. */
 typedef struct {
-    int n, lg;
+    int n, lg, pow2;
     int* rev;
-    CT* w; /* w[k] = exp(-2 pi i k / n), k < n/2 */
+    CT* w;    /* pow2: w[k] = exp(-2 pi i k / n), k <= n/2; else k < n */
+    int nf;
+    int fac[32];
 } FN(fftplan);
 
 static void FN(plan_init)(FN(fftplan) * p, int n) {
@@ -39,6 +44,23 @@ static void FN(plan_init)(FN(fftplan) * p, int n) {
     while ((1 << lg) < n) ++lg;
     p->n = n;
     p->lg = lg;
+    p->pow2 = (1 << lg) == n;
+    p->nf = 0;
+    if (!p->pow2) {
+        int m = n;
+        while (m % 4 == 0) { p->fac[p->nf++] = 4; m /= 4; }
+        while (m % 2 == 0) { p->fac[p->nf++] = 2; m /= 2; }
+        while (m % 3 == 0) { p->fac[p->nf++] = 3; m /= 3; }
+        if (m != 1) { fprintf(stderr, "oracle FFT: length %d is not 2^a 3^b\n", n); abort(); }
+        p->rev = NULL;
+        p->w = (CT*)malloc(sizeof(CT) * n);
+        for (int k = 0; k < n; ++k) {
+            const double a = -2.0 * M_PI * (double)k / (double)n;
+            p->w[k].re = (REAL)cos(a);
+            p->w[k].im = (REAL)sin(a);
+        }
+        return;
+    }
     p->rev = (int*)malloc(sizeof(int) * n);
     p->w = (CT*)malloc(sizeof(CT) * (n / 2 + 1));
     for (int i = 0; i < n; ++i) {
@@ -58,7 +80,46 @@ static void FN(plan_free)(FN(fftplan) * p) {
     free(p->w);
 }
 
+static void FN(fft_line_mixed)(const FN(fftplan) * p, CT* x, int inverse) {
+    const int L = p->n;
+    CT* buf = (CT*)malloc(sizeof(CT) * L);
+    CT *src = x, *dst = buf;
+    int n = L, s = 1;
+    for (int f = 0; f < p->nf; ++f) {
+        const int r = p->fac[f], m = n / r;
+        for (int b = 0; b < L / r; ++b) {
+            const int pp = b / s, q = b % s;
+            CT a[4];
+            for (int j = 0; j < r; ++j) a[j] = src[q + s * (pp + j * m)];
+            for (int k = 0; k < r; ++k) {
+                CT acc = {0, 0};
+                for (int j = 0; j < r; ++j) {
+                    CT w = p->w[((j * k) % r) * (L / r)];
+                    if (inverse) w.im = -w.im;
+                    const CT t = FN(cmul)(a[j], w);
+                    acc.re += t.re;
+                    acc.im += t.im;
+                }
+                CT w = p->w[((pp * k) % n) * (L / n)];
+                if (inverse) w.im = -w.im;
+                dst[q + s * (r * pp + k)] = FN(cmul)(acc, w);
+            }
+        }
+        CT* t = src;
+        src = dst;
+        dst = t;
+        n = m;
+        s *= r;
+    }
+    if (src != x) memcpy(x, src, sizeof(CT) * L);
+    free(buf);
+}
+
 static void FN(fft_line)(const FN(fftplan) * p, CT* x, int inverse) {
+    if (!p->pow2) {
+        FN(fft_line_mixed)(p, x, inverse);
+        return;
+    }
     const int n = p->n;
     for (int i = 0; i < n; ++i) {
         const int r = p->rev[i];
@@ -91,7 +152,8 @@ void FN(oracle_fft2)(CT* a, int rows, int cols, int batch, int inverse) {
 #pragma omp parallel for schedule(static)
     for (long line = 0; line < (long)cols * batch; ++line) FN(fft_line)(&pr, a + line * rows, inverse);
     /* strided index (each row), 8 rows per block gathered into a buffer */
-    const int RB = rows < 8 ? rows : 8;
+    int RB = 8; /* rows per block: a divisor of rows */
+    while (RB > 1 && rows % RB != 0) RB /= 2;
 #pragma omp parallel
     {
         CT* buf = (CT*)malloc(sizeof(CT) * (size_t)cols * RB);

@@ -75,10 +75,22 @@ def _call_solve(nr=16, Nr=32, L=4, batch=2, led=None, epochs=1):
     return fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, 2, epochs, batch, (1e-3, 0.999, 1e-4))
 
 
-def test_rejects_non_power_of_two():
-    with pytest.raises(fpm_amd.FPMError) as e:
+def test_rejects_sides_outside_both_paths():
+    """Sides must be powers of two or even 2^a 3^b (the generic path): sides
+    with a factor 5 are FPM_E_UNSUPPORTED before any device work."""
+    for nr, Nr in ((40, 80), (20, 60), (16, 80)):
+        with pytest.raises(fpm_amd.FPMError) as e:
+            _call_solve(nr=nr, Nr=Nr)
+        assert e.value.code == -2, (nr, Nr)
+
+
+def test_accepts_generic_sides_up_to_the_device():
+    """24 -> 48 (2^a 3^b) passes the size checks: without a GPU the call
+    fails only at the device check (FPM_E_NODEVICE), not as unsupported."""
+    try:
         _call_solve(nr=24, Nr=48)
-    assert e.value.code == -2
+    except fpm_amd.FPMError as e:
+        assert e.code == -5, e
 
 
 def test_rejects_bad_batch():
@@ -164,10 +176,10 @@ def test_tiles_solve_rejects_bad_arguments():
     assert "null" in L.fpm_last_error().decode()
     assert L.fpm_tiles_solve(arr, 0, 4, 2.0, 1.0, 2.0, optv, 0, None, None) == -1
     assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 65, None, None) == -1
-    # a tile side that is not a power of two
+    # a tile side with a factor 5 (neither a power of two nor 2^a 3^b)
     buf = np.zeros(4096, np.complex64)
     p = buf.ctypes.data
-    arr[0] = ft.Tile(p, p, 48, p, p, 24, p, 0, p, p, None)
+    arr[0] = ft.Tile(p, p, 40, p, p, 20, p, 0, p, p, None)
     assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 1, None, None) == -2
     # stop_loss is an option of the single solve only
     o = fpm_amd._opts(stop_loss=1.0)

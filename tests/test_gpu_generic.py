@@ -1,0 +1,146 @@
+"""Sides that are not powers of two (even 2^a 3^b: 48 px -> 192, 192 px ->
+768 and the like), which run the generic path (mixed-radix transforms,
+unfused kernels in natural order): the transform against numpy, the fused
+gradient against the FP64 oracle at <= 1e-5 (inputs whose loss-stencil signs
+all have a margin), one solve step against FP64 at <= 1e-6, a 20-epoch solve
+on real images as close to FP64 as the FP32 oracle, the image model against
+numpy, the graph
+replay against eager launches, and tiles of 32/48/64 px (48 generic) against
+their own solves, bit for bit."""
+import numpy as np
+import pytest
+
+import fpm_amd
+import oracle
+from fpm_amd import synth
+from fpm_amd import tiles as ft
+from helpers import margin_problem, rel_l2, small_leds
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.mark.parametrize("rows,cols,batch", [(192, 192, 2), (768, 768, 1), (48, 96, 3), (192, 256, 1), (24, 18, 4)])
+def test_generic_fft_matches_numpy(rows, cols, batch):
+    rng = np.random.default_rng(rows + cols)
+    x = np.asfortranarray((rng.standard_normal((rows, cols, batch)) + 1j * rng.standard_normal((rows, cols, batch)))
+                          .astype(np.complex64))
+    ref = np.fft.fft2(x.astype(np.complex128), axes=(0, 1))
+    got = np.stack([fpm_amd.fft2(x[:, :, k]) for k in range(batch)], axis=2)
+    assert rel_l2(got, ref) < 2e-6
+    inv = np.fft.ifft2(x.astype(np.complex128), axes=(0, 1)) * rows * cols
+    goti = np.stack([fpm_amd.fft2(x[:, :, k], inverse=True) for k in range(batch)], axis=2)
+    assert rel_l2(goti, inv) < 2e-6
+
+
+@pytest.mark.parametrize("n,r,grid", [(48, 4, 5), (24, 2, 3), (96, 2, 3)])
+def test_generic_gradient_matches_fp64_oracle(n, r, grid):
+    O, P, Y, led = margin_problem(n, r, grid)
+    GO, GP, loss, margin = oracle.gradient(O, P, Y, led, r)
+    assert margin > 5e-3
+    gO, gP, gl = fpm_amd.fused_gradient(O, P, Y, led, r)
+    print(f"[generic] gradient n={n} r={r}: G_O {rel_l2(gO, GO):.2e} G_P {rel_l2(gP, GP):.2e}", flush=True)
+    assert rel_l2(gO, GO) <= 1e-5
+    assert rel_l2(gP, GP) <= 1e-5
+    assert abs(gl - loss) <= 1e-5 * loss
+
+
+def test_generic_solve_one_step_matches_fp64_oracle():
+    n, r = 48, 4
+    O, P, Y, led = margin_problem(n, r, 5)
+    A, _ = synth.pupil(n)
+    L = led.shape[1]
+    opt = (2e-3, 0.999, 1e-4)
+    O1, P1 = fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, r, 1, L, opt)
+    Oo, Po, _ = oracle.solve(O, P, Y, led, A, r, 1, L, opt, prec="f64")
+    assert rel_l2(O1, Oo) < 1e-6
+    assert rel_l2(P1, Po) < 1e-6
+
+
+def test_generic_solve_epochs_on_margin_inputs_match_fp64():
+    """3 epochs over 25 LEDs in batches of 10 (ragged last batch), small
+    steps that keep the margin inputs' sign decisions: O, P and every epoch's
+    loss vs the FP64 oracle (the moments carry across steps and epochs)."""
+    n, r = 48, 4
+    O, P, Y, led = margin_problem(n, r, 5)
+    A, _ = synth.pupil(n)
+    opt = (2e-4, 0.999, 1e-4)
+    O1, P1, lg = fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, r, 3, 10, opt, loss=True)
+    Oo, Po, lo = oracle.solve(O, P, Y, led, A, r, 3, 10, opt, prec="f64")
+    print(f"[generic] 3 epochs margin: O {rel_l2(O1, Oo):.2e} P {rel_l2(P1, Po):.2e}", flush=True)
+    assert rel_l2(O1, Oo) < 1e-5
+    assert rel_l2(P1, Po) < 1e-5
+    np.testing.assert_allclose(lg, lo, rtol=1e-5)
+
+
+def _dataset(n, r, grid):
+    led = small_leds(n, r, grid)
+    A, Ptrue = synth.pupil(n)
+    amp = synth.object_numpy(n * r, r, 0.3, 11)
+    model = synth.image_model_numpy(np.asfortranarray(amp.astype(np.complex128)), Ptrue.astype(np.complex128), led, r)
+    Y = np.asfortranarray(model.astype(np.float32))
+    O0 = np.asfortranarray(synth.bilinear_upsample(Y[:, :, 0].astype(np.float64), r).astype(np.complex64))
+    return O0, np.asfortranarray(A.astype(np.complex64)), Y, led, A
+
+
+def test_generic_solve_20_epochs_as_close_to_fp64_as_the_fp32_oracle():
+    """20 epochs on real (noise-free model) images: loss-stencil signs near a
+    tie flip differently in any two FP32 computations (here two different FFT
+    algorithms), and the trajectories part chaotically: measured 9.6e-4 (O)
+    and 1.3e-3 (P) from FP64 against the FP32 oracle's 6.0e-4 and 5.0e-4.
+    Bars: below 2e-3 from FP64 and from the FP32 oracle, losses within 1e-3
+    (the exact regime is pinned by the margin tests)."""
+    n, r, grid = 48, 4, 7
+    O0, P0, Y, led, A = _dataset(n, r, grid)
+    opt = (5e-4, 0.999, 1e-4)
+    O, P, lg = fpm_amd.cuFPM_pure_v3(O0, P0, Y, led, A, r, 20, 10, opt, loss=True)
+    Oo, Po, lo = oracle.solve(O0, P0, Y, led, A, r, 20, 10, opt, prec="f32")
+    O64, P64, _ = oracle.solve(O0, P0, Y, led, A, r, 20, 10, opt, prec="f64")
+    e = {"O": rel_l2(O, O64), "P": rel_l2(P, P64)}
+    ref = {"O": rel_l2(Oo, O64), "P": rel_l2(Po, P64)}
+    print(f"[generic] 20 epochs vs FP64: GPU O {e['O']:.2e} P {e['P']:.2e}; FP32 oracle O {ref['O']:.2e} "
+          f"P {ref['P']:.2e}; GPU vs FP32 oracle O {rel_l2(O, Oo):.2e} P {rel_l2(P, Po):.2e}", flush=True)
+    for k in e:
+        assert e[k] <= 2e-3, (k, e[k], ref[k])
+    assert rel_l2(O, Oo) <= 2e-3 and rel_l2(P, Po) <= 2e-3
+    assert np.allclose(lg, lo, rtol=1e-3)
+    assert lg[-1] < lg[0]
+
+
+def test_generic_graph_replay_equals_eager():
+    n, r, grid = 48, 4, 7
+    O0, P0, Y, led, A = _dataset(n, r, grid)
+    opt = (5e-4, 0.999, 1e-4)
+    a = fpm_amd.cuFPM_pure_v3(O0, P0, Y, led, A, r, 4, 10, opt, loss=True)
+    b = fpm_amd.cuFPM_pure_v3(O0, P0, Y, led, A, r, 4, 10, opt, loss=True, graph=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-12)
+
+
+def test_generic_image_model_matches_numpy():
+    n, r = 48, 4
+    led = small_leds(n, r, 5)
+    _, P = synth.pupil(n)
+    O = np.asfortranarray(synth.object_numpy(n * r, r, 0.3, 5).astype(np.complex64))
+    got = fpm_amd.forward_intensity(O, P, led, r)
+    ref = synth.image_model_numpy(O.astype(np.complex128), P.astype(np.complex128), led, r)
+    assert rel_l2(got, ref) < 2e-6
+
+
+def test_generic_tiles_equal_their_own_solves():
+    """Tiles of 32, 48 and 64 px (the 48-px ones on the generic path, each its
+    own launch sequence) in one call: every tile equals its own solve."""
+    tl = ft.make_tile_set(1024, (32, 48, 64), 3, 4, seed=4, led_grid=7)
+    rep, _ = ft.tiles_solve(tl, 4, 2, 10, (1e-3, 0.999, 1e-4), streams=2)
+    print(f"[generic] tiles: report {rep}", flush=True)
+    n48 = sum(1 for t in tl if t["n"] == 48)
+    assert n48 == 3 and rep["grouped_tiles"] == len(tl) - n48
+    for k, t in enumerate(tl):
+        O, P = torch.empty_like(t["O0"]), torch.empty_like(t["P0"])
+        N, n = t["O0"].shape[0], t["P0"].shape[0]
+        torch.cuda.synchronize()
+        fpm_amd.solve_device(t["O0"].data_ptr(), t["P0"].data_ptr(), t["Y"].data_ptr(), True, t["led"].data_ptr(),
+                             t["A"].data_ptr(), (N, N), (n, n), t["Y"].shape[0], 4, 2, 10, (1e-3, 0.999, 1e-4),
+                             O.data_ptr(), P.data_ptr())
+        assert np.array_equal(t["O"].cpu().numpy(), O.cpu().numpy()), k
+        assert np.array_equal(t["P"].cpu().numpy(), P.cpu().numpy()), k

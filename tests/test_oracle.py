@@ -14,7 +14,8 @@ def _cplx(shape, rng=RNG):
     return np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
 
 
-@pytest.mark.parametrize("rows,cols,batch", [(16, 16, 1), (64, 32, 3), (32, 128, 2), (8, 512, 1)])
+@pytest.mark.parametrize("rows,cols,batch", [(16, 16, 1), (64, 32, 3), (32, 128, 2), (8, 512, 1),
+                                             (192, 192, 2), (48, 96, 1), (12, 24, 2), (768, 36, 1)])
 def test_oracle_fft_matches_numpy(rows, cols, batch):
     x = _cplx((rows, cols, batch))
     ref = np.fft.fft2(x, axes=(0, 1))
@@ -24,8 +25,9 @@ def test_oracle_fft_matches_numpy(rows, cols, batch):
     assert rel_l2(oracle.fft2(x, prec="f32"), ref) < 1e-6
 
 
-def test_oracle_fft_brute_force_dft():
-    n = 16
+@pytest.mark.parametrize("n", [16, 12, 18])
+def test_oracle_fft_brute_force_dft(n):
+    """Powers of two and the mixed-radix lengths 2^a 3^b (radices 4, 2, 3)."""
     x = _cplx((n, n))
     k = np.arange(n)
     W = np.exp(-2j * np.pi * np.outer(k, k) / n)

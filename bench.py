@@ -189,11 +189,17 @@ def run_time_to_loss(args):
             "warm_fft_engine": res}
 
 
+def _tile_sizes(args):
+    return tuple(int(v) for v in args.tile_sizes.split(","))
+
+
 def run_tiles(args):
-    """Uneven field-of-view tiles: 48 tiles of 128 and 256 low-res pixels
-    (upsample 4) over a 1024-pixel field, 19 x 19 LEDs seen from each tile's
-    centre, per-tile Zernike pupils (sigma 0.4 rad), batch 26, reconstructed
-    jointly: the tiles of each size in shared launches (two launch sequences),
+    """Uneven field-of-view tiles: 48 tiles of 128, 192 and 256 low-res
+    pixels (--tile-sizes; upsample 4) over a 1024-pixel field, 19 x 19 LEDs
+    seen from each tile's centre, per-tile Zernike pupils (sigma 0.4 rad),
+    batch 26, reconstructed jointly: the tiles of each power-of-two size in
+    shared launches, the 192-px tiles (mixed-radix sides) each in its own
+    launch sequence,
     one hipGraph per epoch, autotuned stream count (the autotuning call's time
     includes its trial epochs and captures, so the chosen count is also timed
     fixed); the same with one launch sequence per tile; the grouped call on
@@ -205,7 +211,7 @@ def run_tiles(args):
     from fpm_amd import tiles as ft
 
     t0 = time.perf_counter()
-    tl = ft.make_tile_set(1024, (128, 256), 7, 4, seed=11, count=48)
+    tl = ft.make_tile_set(1024, _tile_sizes(args), 7, 4, seed=11, count=48)
     t_gen = time.perf_counter() - t0
     ep, opt = args.tile_epochs, (args.lr, 0.999, 1e-4)
 
@@ -239,8 +245,9 @@ def run_tiles(args):
     L = int(tl[0]["Y"].shape[0])
     steps = (L + args.batch - 1) // args.batch
     epoch_bytes = sum(steps * 104.0 * t["O0"].shape[0] ** 2 + L * 138.0 * t["P0"].shape[0] ** 2 for t in tl)
-    return {"workload": "48 tiles of 128/256 px (upsample 4) over a 1024 px field, 361 LEDs per tile seen from its "
-                        f"centre, per-tile pupils, batch {args.batch}, {ep} epochs",
+    sizes = "/".join(str(v) for v in _tile_sizes(args))
+    return {"workload": f"48 tiles of {sizes} px (upsample 4) over a 1024 px field, 361 LEDs per tile seen from "
+                        f"its centre, per-tile pupils, batch {args.batch}, {ep} epochs",
             "grouped": {"s_per_epoch": t_joint / ep, "fixed_streams_s_per_epoch": t_fixed / ep,
                         "streams_chosen": rep["streams"], "autotune_epoch_ms": rep["candidates"],
                         "sequences": rep["sequences"], "grouped_tiles": rep["grouped_tiles"],
@@ -252,7 +259,8 @@ def run_tiles(args):
             "sequential_s_per_epoch": t_seq / ep, "speedup_grouped_vs_sequential": t_seq / t_joint,
             "speedup_grouped_vs_per_tile": t_pt / t_joint, "epoch_model_GB": epoch_bytes / 1e9,
             "generate_s": t_gen,
-            "note": "tile sides are powers of two; 192 px tiles need a radix-3 transform (not built)"}
+            "note": "sides that are not powers of two run the generic path (mixed-radix transforms, unfused "
+                    "kernels, one launch sequence per tile); the split engine applies to power-of-two sides"}
 
 
 def run_tiles_grouped(args):
@@ -262,7 +270,7 @@ def run_tiles_grouped(args):
 
     from fpm_amd import tiles as ft
 
-    tl = ft.make_tile_set(1024, (128, 256), 7, 4, seed=11, count=48)
+    tl = ft.make_tile_set(1024, _tile_sizes(args), 7, 4, seed=11, count=48)
     ep, opt = args.tile_epochs, (args.lr, 0.999, 1e-4)
     ft.tiles_solve(tl, 4, 1, args.batch, opt, streams=args.tile_streams)
     torch.cuda.synchronize()
@@ -297,6 +305,8 @@ def main():
     ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
     ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
     ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")
+    ap.add_argument("--tile-sizes", default="128,192,256",
+                    help="low-res tile sides of the multi-tile leg, cycled over the 48 tiles")
     ap.add_argument("--tiles-only", action="store_true",
                     help="only the grouped multi-tile solve, fixed streams (for profiling); prints its timing")
     ap.add_argument("--tile-streams", type=int, default=2, help="stream count of --tiles-only")

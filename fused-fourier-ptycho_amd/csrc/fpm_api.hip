@@ -602,7 +602,7 @@ struct GroupSolve {
     YStack y{};  // image type; each tile's stack and constants are in the table
     RmsArgs rms{};
     SolveState ss{};
-    DevBuf O, P, S0, S1, mO, mP, lat, rec, masks, state, tab;
+    DevBuf O, P, S0, S1, mO, mP, lat, rec, masks, state, tab, ebuf;
     std::vector<std::unique_ptr<DevBuf>> ybuf;
     std::vector<TileIn> htab;
     float2* spec[2] = {nullptr, nullptr};
@@ -622,7 +622,8 @@ struct GroupSolve {
 
     // tiles of one group can share launches
     static bool groupable(const Geom& g, uint32_t B, int stitch) {
-        return !g.generic && stitch == FPM_STITCH_OWNER && B <= (uint32_t)64 && stencil_fused_supported(g) &&
+        if (g.generic) return stitch == FPM_STITCH_OWNER;  // the generic kernels take a tile grid row
+        return stitch == FPM_STITCH_OWNER && B <= (uint32_t)64 && stencil_fused_supported(g) &&
                fused_update_groupable(g);
     }
 
@@ -683,6 +684,11 @@ struct GroupSolve {
         FPM_HIP(hipMemsetAsync(mP.p, 0, T * nn * sizeof(float), st));
         launch_solve_state_init(ss, lr, st);
         rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
+        if (g.generic) {  // no masks and no carried spectrum on the generic path
+            ebuf.alloc((size_t)T * B * nn * sizeof(float));
+            generic_prepare(g);
+            return;
+        }
         for (uint32_t k = 0; k < T; ++k)
             launch_pupil_row_mask(P.as<float2>() + k * nn, tiles[members[k]].A, g.nr, g.nc, pmask + k * prows,
                                   psup + (size_t)k * g.nr, st);
@@ -696,6 +702,10 @@ struct GroupSolve {
     // the launches of Solve::enqueue_epoch + minibatch_backward (update
     // mode, fused stencil, owner stitch inside the column pass), for all tiles
     void enqueue_epoch(int cur, bool last_epoch, hipStream_t st) {
+        if (g.generic) {
+            enqueue_epoch_generic(st);
+            return;
+        }
         const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
         float2* Pw = P.as<float2>();
         float2* lp = lat.as<float2>();
@@ -731,6 +741,33 @@ struct GroupSolve {
                 fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng, 1);
                 cur = 1 - cur;
             }
+        }
+        launch_epoch_end(ss, st);
+    }
+
+    // the generic steps (Solve::enqueue_epoch_generic) for all T tiles: T
+    // pages for the object-side transforms, T b for the per-LED ones, one
+    // grid row per tile for the crop, stencil, stitch and pupil kernels
+    void enqueue_epoch_generic(hipStream_t st) {
+        const size_t NN = (size_t)g.Nr * g.Nc;
+        float2* Ow = O.as<float2>();
+        float2* Pw = P.as<float2>();
+        float2* S = S0.as<float2>();
+        float2* G = S1.as<float2>();
+        float2* lp = lat.as<float2>();
+        float2* rp = rec.as<float2>();
+        uint32_t led0, b;
+        for (uint32_t s = 0; s < steps; ++s) {
+            batch_of(s, led0, b);
+            fft2_generic(Ow, S, g.Nr, g.Nc, T, false, st);
+            generic_crop(S, Pw, nullptr, led0, b, g, lp, st, tabd, T);
+            fft2_generic(lp, lp, g.nr, g.nc, T * b, true, st);
+            generic_stencil(lp, y, led0, b, pr, g, ebuf.as<float>(), rp, nullptr, st, tabd, T);
+            fft2_generic(rp, rp, g.nr, g.nc, T * b, false, st);
+            generic_stitch(rp, Pw, nullptr, led0, b, g, G, st, tabd, T);
+            generic_pupil(rp, S, nullptr, led0, b, g, Pw, mP.as<float>(), nullptr, rms, nullptr, st, tabd, T);
+            fft2_generic(G, G, g.Nr, g.Nc, T, true, st);
+            generic_update(G, Ow, mO.as<float>(), T * NN, rms, st);
         }
         launch_epoch_end(ss, st);
     }

@@ -22,17 +22,22 @@ void fft2_generic(const float2* in, float2* out, uint32_t rows, uint32_t cols, u
 void generic_prepare(const Geom& g);
 
 // crop x pupil of LEDs led0.. into b ifftshifted natural pages
+// tab (optional, ntiles entries, ntiles <= 65535): grouped tiles, tile k's
+// arrays at block k of each argument (pages: b per tile), its LED windows,
+// images, mask and loss slot from tab[k]; led / y / A / loss_acc unused then
 void generic_crop(const float2* S, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                  float2* out, hipStream_t st);
+                  float2* out, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
 // residual (ebuf: b n_r n_c floats) and loss stencil: u from z; loss_acc may be null
 void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
-                     float* ebuf, float2* u, double* loss_acc, hipStream_t st);
+                     float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab = nullptr,
+                     uint32_t ntiles = 1);
 // owner-computes stitch of the b back-projections into G_S (every element written)
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                    float2* Gs, hipStream_t st);
+                    float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
 // pupil gradient; with mP and A the RMSprop + clamp update of P, else GP = gradient
 void generic_pupil(const float2* R, const float2* S, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                   float2* P, float* mP, const float* A, RmsArgs rms, float2* GP, hipStream_t st);
+                   float2* P, float* mP, const float* A, RmsArgs rms, float2* GP, hipStream_t st,
+                   const TileIn* tab = nullptr, uint32_t ntiles = 1);
 // RMSprop of O from its gradient G (n elements)
 void generic_update(const float2* G, float2* O, float* mO, size_t n, RmsArgs rms, hipStream_t st);
 

@@ -55,10 +55,42 @@ __device__ __forceinline__ size_t gen_off(const GenFFT& a, uint32_t L, uint32_t 
            (size_t)e * a.es;
 }
 
+// radix-R DFT of v in place (forward: W_R = exp(-2 pi i / R); inverse: conjugate)
+template <int R> __device__ __forceinline__ void small_dft(float2* v, bool inv);
+template <> __device__ __forceinline__ void small_dft<2>(float2* v, bool) {
+    const float2 a = v[0], b = v[1];
+    v[0] = make_float2(a.x + b.x, a.y + b.y);
+    v[1] = make_float2(a.x - b.x, a.y - b.y);
+}
+template <> __device__ __forceinline__ void small_dft<4>(float2* v, bool inv) {
+    const float2 s02 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const float2 d02 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const float2 d13 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+    // forward: X1 = d02 - i d13, X3 = d02 + i d13 (inverse swaps them)
+    const float2 mi = inv ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);  // -+ i d13
+    v[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+    v[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+    v[1] = make_float2(d02.x + mi.x, d02.y + mi.y);
+    v[3] = make_float2(d02.x - mi.x, d02.y - mi.y);
+}
+template <> __device__ __forceinline__ void small_dft<3>(float2* v, bool inv) {
+    // X0 = a + b + c; X1,2 = a - (b + c)/2 -+ i (sqrt(3)/2)(b - c) (forward), +- for the inverse
+    const float c3 = 0.86602540378443864676f;
+    const float2 t = make_float2(v[1].x + v[2].x, v[1].y + v[2].y);
+    const float2 d = make_float2(v[1].x - v[2].x, v[1].y - v[2].y);
+    const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    const float sg = inv ? -1.0f : 1.0f;
+    const float2 r = make_float2(sg * c3 * d.y, -sg * c3 * d.x);  // -i sg c3 d
+    v[0] = make_float2(v[0].x + t.x, v[0].y + t.y);
+    v[1] = make_float2(m.x + r.x, m.y + r.y);
+    v[2] = make_float2(m.x - r.x, m.y - r.y);
+}
+
 template <int R>
 __device__ __forceinline__ void gen_pass(const GenFFT& a, const float2* x, float2* y, uint32_t nl, uint32_t n,
                                          uint32_t s) {
-    const uint32_t m = n / R, nb = a.len / R, step_r = a.len / R, step_n = a.len / n;
+    const uint32_t m = n / R, nb = a.len / R, step_n = a.len / n;
     for (uint32_t bi = threadIdx.x; bi < nl * nb; bi += blockDim.x) {
         const uint32_t l = bi / nb, b = bi - l * nb;
         const uint32_t p = b / s, q = b - p * s;
@@ -67,20 +99,13 @@ __device__ __forceinline__ void gen_pass(const GenFFT& a, const float2* x, float
         float2 v[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = xl[q + s * (p + j * m)];
+        small_dft<R>(v, a.inverse != 0);
+        yl[q + s * (R * p)] = v[0];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            float2 acc = v[0];
-#pragma unroll
-            for (int j = 1; j < R; ++j) {
-                float2 w = a.tw[((j * k) % R) * step_r];
-                if (a.inverse) w.y = -w.y;
-                const float2 t = cmul_g(v[j], w);
-                acc.x += t.x;
-                acc.y += t.y;
-            }
+        for (int k = 1; k < R; ++k) {
             float2 w = a.tw[((p * (uint32_t)k) % n) * step_n];
             if (a.inverse) w.y = -w.y;
-            yl[q + s * (R * p + k)] = cmul_g(acc, w);
+            yl[q + s * (R * p + k)] = cmul_g(v[k], w);
         }
     }
 }
@@ -163,14 +188,26 @@ void launch_generic_lines(GenFFT a, hipStream_t st) {
 }
 
 // ------------------------------------------------------- solve kernels
+// Every kernel runs one tile per grid row: tab (optional) gives grouped
+// tiles, tile k = blockIdx.y with its pages, spectra, pupil and moments at
+// block k of each array and its LED windows, images, mask and loss slot
+// from tab[k]; without tab, grid row 0 and the arguments as given.
+
 // crop x pupil into ifftshifted pages (kernels.cuh:38-64): page l pixel
 // (di, dj) = S(window of LED led0 + l at pupil pixel (i, j)) P(i, j),
 // i = (di + nr/2) mod nr, j = (dj + nc/2) mod nc; pages column-major, natural.
 __global__ __launch_bounds__(256) void gen_crop_kernel(const float2* __restrict__ S, const float2* __restrict__ P,
                                                        const int2* __restrict__ led, uint32_t led0, uint32_t b,
                                                        uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
-                                                       float2* __restrict__ out) {
+                                                       float2* __restrict__ out, const TileIn* __restrict__ tab) {
     const size_t plane = (size_t)nr * nc, total = plane * b;
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        S += (size_t)tk * Nr * Nc;
+        P += (size_t)tk * plane;
+        out += (size_t)tk * total;
+        led = tab[tk].led;
+    }
     for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total; k += (size_t)gridDim.x * 256) {
         const uint32_t l = (uint32_t)(k / plane);
         const uint32_t pix = (uint32_t)(k - (size_t)l * plane);
@@ -189,7 +226,17 @@ __global__ __launch_bounds__(256) void gen_crop_kernel(const float2* __restrict_
 // e = |z| / (n_r n_c r^2) - y of every pixel of the b pages
 __global__ __launch_bounds__(256) void gen_residual_kernel(const float2* __restrict__ z, YStack y, uint32_t led0,
                                                            size_t plane, size_t total, float ratio, float inv_ratio,
-                                                           float* __restrict__ e) {
+                                                           float* __restrict__ e, const TileIn* __restrict__ tab) {
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        z += (size_t)tk * total;
+        e += (size_t)tk * total;
+        const TileIn ti = tab[tk];
+        y.data = ti.y;
+        y.mn = ti.ymn;
+        y.range = ti.yrange;
+        y.rcp_range = ti.yrcp;
+    }
     for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total; k += (size_t)gridDim.x * 256) {
         const float2 v = z[k];
         const float mag = sqrtf(v.x * v.x + v.y * v.y);
@@ -202,8 +249,16 @@ __global__ __launch_bounds__(256) void gen_residual_kernel(const float2* __restr
 // (copysign semantics), u = r^2 g z/|z| (0 where |z| = 0), loss += |Dx e| + |Dy e|
 __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restrict__ z, const float* __restrict__ e,
                                                           uint32_t nr, uint32_t nc, size_t total, float gscale,
-                                                          float2* __restrict__ u, double* __restrict__ loss_acc) {
+                                                          float2* __restrict__ u, double* __restrict__ loss_acc,
+                                                          const TileIn* __restrict__ tab) {
     __shared__ double red[4];
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        z += (size_t)tk * total;
+        e += (size_t)tk * total;
+        u += (size_t)tk * total;
+        loss_acc = tab[tk].loss;
+    }
     const size_t plane = (size_t)nr * nc;
     float loss32 = 0.0f;
     for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total; k += (size_t)gridDim.x * 256) {
@@ -229,7 +284,7 @@ __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restri
         }
         u[k] = o;
     }
-    if (loss_acc) {
+    if (loss_acc) {  // uniform over the workgroup
         double loss = (double)loss32;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
@@ -245,8 +300,16 @@ __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restri
 __global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restrict__ R, const float2* __restrict__ P,
                                                          const int2* __restrict__ led, uint32_t led0, uint32_t b,
                                                          uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
-                                                         float invN2, float2* __restrict__ Gs) {
+                                                         float invN2, float2* __restrict__ Gs,
+                                                         const TileIn* __restrict__ tab) {
     const size_t NN = (size_t)Nr * Nc, plane = (size_t)nr * nc;
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        R += (size_t)tk * b * plane;
+        P += (size_t)tk * plane;
+        Gs += (size_t)tk * NN;
+        led = tab[tk].led;
+    }
     for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < NN; k += (size_t)gridDim.x * 256) {
         const uint32_t col = (uint32_t)(k / Nr), row = (uint32_t)(k - (size_t)col * Nr);
         float2 v = make_float2(0.f, 0.f);
@@ -276,10 +339,20 @@ __global__ __launch_bounds__(256) void gen_pupil_kernel(const float2* __restrict
                                                         uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
                                                         float2* __restrict__ P, float* __restrict__ mP,
                                                         const float* __restrict__ A, RmsArgs rms,
-                                                        float2* __restrict__ GP) {
+                                                        float2* __restrict__ GP, const TileIn* __restrict__ tab) {
     const uint32_t plane = nr * nc;
     const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
     if (pix >= plane) return;
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        R += (size_t)tk * b * plane;
+        S += (size_t)tk * Nr * Nc;
+        P += (size_t)tk * plane;
+        if (mP) mP += (size_t)tk * plane;
+        if (GP) GP += (size_t)tk * plane;
+        led = tab[tk].led;
+        A = tab[tk].A;
+    }
     const uint32_t j = pix / nr, i = pix - j * nr;
     const uint32_t di = (i + nr / 2) % nr, dj = (j + nc / 2) % nc;
     float2 gp = make_float2(0.f, 0.f);
@@ -368,36 +441,41 @@ void fft2_generic(const float2* in, float2* out, uint32_t rows, uint32_t cols, u
 }
 
 void generic_crop(const float2* S, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                  float2* out, hipStream_t st) {
-    hipLaunchKernelGGL(gen_crop_kernel, dim3(gblocks((size_t)g.nr * g.nc * b)), dim3(256), 0, st, S, P, led, led0, b,
-                       g.nr, g.nc, g.Nr, g.Nc, out);
+                  float2* out, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
+    if (!tab) ntiles = 1;
+    hipLaunchKernelGGL(gen_crop_kernel, dim3(gblocks((size_t)g.nr * g.nc * b), ntiles), dim3(256), 0, st, S, P, led,
+                       led0, b, g.nr, g.nc, g.Nr, g.Nc, out, tab);
     FPM_LAUNCH_CHECK();
 }
 
 void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
-                     float* ebuf, float2* u, double* loss_acc, hipStream_t st) {
+                     float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
+    if (!tab) ntiles = 1;
     const size_t plane = (size_t)g.nr * g.nc, total = plane * b;
     const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
     const float inv_ratio = exact_inverse_or_zero(ratio);
-    hipLaunchKernelGGL(gen_residual_kernel, dim3(gblocks(total)), dim3(256), 0, st, z, y, led0, plane, total, ratio,
-                       inv_ratio, ebuf);
+    hipLaunchKernelGGL(gen_residual_kernel, dim3(gblocks(total), ntiles), dim3(256), 0, st, z, y, led0, plane, total,
+                       ratio, inv_ratio, ebuf, tab);
     FPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gen_stencil_kernel, dim3(gblocks(total)), dim3(256), 0, st, z, ebuf, g.nr, g.nc, total,
-                       (float)(pratio * pratio), u, loss_acc);
+    hipLaunchKernelGGL(gen_stencil_kernel, dim3(gblocks(total), ntiles), dim3(256), 0, st, z, ebuf, g.nr, g.nc, total,
+                       (float)(pratio * pratio), u, loss_acc, tab);
     FPM_LAUNCH_CHECK();
 }
 
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                    float2* Gs, hipStream_t st) {
-    hipLaunchKernelGGL(gen_stitch_kernel, dim3(gblocks((size_t)g.Nr * g.Nc)), dim3(256), 0, st, R, P, led, led0, b,
-                       g.nr, g.nc, g.Nr, g.Nc, 1.0f / (float)(g.Nr * g.Nc), Gs);
+                    float2* Gs, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
+    if (!tab) ntiles = 1;
+    hipLaunchKernelGGL(gen_stitch_kernel, dim3(gblocks((size_t)g.Nr * g.Nc), ntiles), dim3(256), 0, st, R, P, led,
+                       led0, b, g.nr, g.nc, g.Nr, g.Nc, 1.0f / (float)(g.Nr * g.Nc), Gs, tab);
     FPM_LAUNCH_CHECK();
 }
 
 void generic_pupil(const float2* R, const float2* S, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                   float2* P, float* mP, const float* A, RmsArgs rms, float2* GP, hipStream_t st) {
-    hipLaunchKernelGGL(gen_pupil_kernel, dim3((g.plane() + 255) / 256), dim3(256), 0, st, R, S, led, led0, b, g.nr,
-                       g.nc, g.Nr, g.Nc, P, mP, A, rms, GP);
+                   float2* P, float* mP, const float* A, RmsArgs rms, float2* GP, hipStream_t st, const TileIn* tab,
+                   uint32_t ntiles) {
+    if (!tab) ntiles = 1;
+    hipLaunchKernelGGL(gen_pupil_kernel, dim3((g.plane() + 255) / 256, ntiles), dim3(256), 0, st, R, S, led, led0, b,
+                       g.nr, g.nc, g.Nr, g.Nc, P, mP, A, rms, GP, tab);
     FPM_LAUNCH_CHECK();
 }
 

@@ -127,14 +127,19 @@ def test_generic_image_model_matches_numpy():
     assert rel_l2(got, ref) < 2e-6
 
 
-def test_generic_tiles_equal_their_own_solves():
-    """Tiles of 32, 48 and 64 px (the 48-px ones on the generic path, each its
-    own launch sequence) in one call: every tile equals its own solve."""
+@pytest.mark.parametrize("launch", [ft.TILES_GROUPED, ft.TILES_PER_TILE])
+def test_generic_tiles_equal_their_own_solves(launch):
+    """Tiles of 32, 48 and 64 px (the 48-px ones on the generic path) in one
+    call, grouped per size (the generic kernels take a tile grid row) or one
+    launch sequence per tile: every tile equals its own solve."""
     tl = ft.make_tile_set(1024, (32, 48, 64), 3, 4, seed=4, led_grid=7)
-    rep, _ = ft.tiles_solve(tl, 4, 2, 10, (1e-3, 0.999, 1e-4), streams=2)
+    rep, _ = ft.tiles_solve(tl, 4, 2, 10, (1e-3, 0.999, 1e-4), streams=2, tile_launch=launch)
     print(f"[generic] tiles: report {rep}", flush=True)
-    n48 = sum(1 for t in tl if t["n"] == 48)
-    assert n48 == 3 and rep["grouped_tiles"] == len(tl) - n48
+    assert sum(1 for t in tl if t["n"] == 48) == 3
+    if launch == ft.TILES_GROUPED:
+        assert rep["grouped_tiles"] == len(tl) and rep["sequences"] == 3
+    else:
+        assert rep["grouped_tiles"] == 0 and rep["sequences"] == len(tl)
     for k, t in enumerate(tl):
         O, P = torch.empty_like(t["O0"]), torch.empty_like(t["P0"])
         N, n = t["O0"].shape[0], t["P0"].shape[0]

@@ -33,11 +33,34 @@ bool generic_side(int64_t v) {
 
 namespace {
 
+// v mod n: one subtraction for v < 2n, which every index sum below stays
+// under for LED offsets from fpm_led_offsets (-n/2 + 1 <= k <= N - n/2); a
+// caller's offsets outside that range take the division and stay in bounds
+__device__ __forceinline__ uint32_t wrap1(uint32_t v, uint32_t n) {
+    return v < n ? v : (v - n < n ? v - n : v % n);
+}
+
 __device__ __forceinline__ float2 cmul_g(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ float2 cmulc_g(float2 a, float2 b) {  // a conj(b)
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// x / d for a divisor fixed per launch: q = (umulhi(x, m) + x) >> s with
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1 (Granlund and
+// Montgomery's round-up method; exact for every 32-bit x)
+struct FastDiv {
+    uint32_t d, m, s;
+};
+FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f{d, 0, 0};
+    while ((1ull << f.s) < d) ++f.s;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
+    return f;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t x, const FastDiv& f) {
+    return (uint32_t)(((uint64_t)__umulhi(x, f.m) + x) >> f.s);
 }
 
 struct GenFFT {
@@ -46,13 +69,16 @@ struct GenFFT {
     const float2* tw;  // W_len^k, k < len
     uint64_t batch_stride, line_stride, es;
     uint32_t len, nlines, lines_per_batch, tl, nf, inverse;
+    FastDiv f_lpb, f_tl, f_len;
+    FastDiv f_nb[5];   // len / R for R = 2, 3, 4 (index R)
+    FastDiv f_s[24];   // the stride s of each pass
     uint8_t fac[24];
 };
 
 // element e of line L at (L / lpb) batch_stride + (L % lpb) line_stride + e es
 __device__ __forceinline__ size_t gen_off(const GenFFT& a, uint32_t L, uint32_t e) {
-    return (size_t)(L / a.lines_per_batch) * a.batch_stride + (size_t)(L % a.lines_per_batch) * a.line_stride +
-           (size_t)e * a.es;
+    const uint32_t pg = fdiv(L, a.f_lpb);
+    return (size_t)pg * a.batch_stride + (size_t)(L - pg * a.lines_per_batch) * a.line_stride + (size_t)e * a.es;
 }
 
 // radix-R DFT of v in place (forward: W_R = exp(-2 pi i / R); inverse: conjugate)
@@ -89,11 +115,11 @@ template <> __device__ __forceinline__ void small_dft<3>(float2* v, bool inv) {
 
 template <int R>
 __device__ __forceinline__ void gen_pass(const GenFFT& a, const float2* x, float2* y, uint32_t nl, uint32_t n,
-                                         uint32_t s) {
+                                         uint32_t s, const FastDiv& fs) {
     const uint32_t m = n / R, nb = a.len / R, step_n = a.len / n;
     for (uint32_t bi = threadIdx.x; bi < nl * nb; bi += blockDim.x) {
-        const uint32_t l = bi / nb, b = bi - l * nb;
-        const uint32_t p = b / s, q = b - p * s;
+        const uint32_t l = fdiv(bi, a.f_nb[R]), b = bi - l * nb;
+        const uint32_t p = fdiv(b, fs), q = b - p * s;
         const float2* xl = x + (size_t)l * a.len;
         float2* yl = y + (size_t)l * a.len;
         float2 v[R];
@@ -103,7 +129,7 @@ __device__ __forceinline__ void gen_pass(const GenFFT& a, const float2* x, float
         yl[q + s * (R * p)] = v[0];
 #pragma unroll
         for (int k = 1; k < R; ++k) {
-            float2 w = a.tw[((p * (uint32_t)k) % n) * step_n];
+            float2 w = a.tw[p * (uint32_t)k * step_n];  // p k < (n / R) R = n
             if (a.inverse) w.y = -w.y;
             yl[q + s * (R * p + k)] = cmul_g(v[k], w);
         }
@@ -118,18 +144,28 @@ __global__ __launch_bounds__(256) void fft_generic_kernel(GenFFT a) {
     const uint32_t nl = min(a.tl, a.nlines - L0);
     const uint32_t tot = nl * a.len;
     const bool line_fast = a.es != 1;  // strided lines: adjacent lines are adjacent in memory
+    // (line, element) of a load/store index: line fastest for strided lines
+    auto split = [&](uint32_t idx, uint32_t& l, uint32_t& e) {
+        if (line_fast) {
+            e = nl == a.tl ? fdiv(idx, a.f_tl) : idx / nl;
+            l = idx - e * nl;
+        } else {
+            l = fdiv(idx, a.f_len);
+            e = idx - l * a.len;
+        }
+    };
     for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-        const uint32_t l = line_fast ? idx % nl : idx / a.len;
-        const uint32_t e = line_fast ? idx / nl : idx % a.len;
+        uint32_t l, e;
+        split(idx, l, e);
         x[(size_t)l * a.len + e] = a.in[gen_off(a, L0 + l, e)];
     }
     __syncthreads();
     uint32_t n = a.len, s = 1;
     for (uint32_t f = 0; f < a.nf; ++f) {
         const uint32_t r = a.fac[f];
-        if (r == 4) gen_pass<4>(a, x, y, nl, n, s);
-        else if (r == 2) gen_pass<2>(a, x, y, nl, n, s);
-        else gen_pass<3>(a, x, y, nl, n, s);
+        if (r == 4) gen_pass<4>(a, x, y, nl, n, s, a.f_s[f]);
+        else if (r == 2) gen_pass<2>(a, x, y, nl, n, s, a.f_s[f]);
+        else gen_pass<3>(a, x, y, nl, n, s, a.f_s[f]);
         __syncthreads();
         float2* t = x;
         x = y;
@@ -138,8 +174,8 @@ __global__ __launch_bounds__(256) void fft_generic_kernel(GenFFT a) {
         s *= r;
     }
     for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-        const uint32_t l = line_fast ? idx % nl : idx / a.len;
-        const uint32_t e = line_fast ? idx / nl : idx % a.len;
+        uint32_t l, e;
+        split(idx, l, e);
         a.out[gen_off(a, L0 + l, e)] = x[(size_t)l * a.len + e];
     }
 }
@@ -181,6 +217,16 @@ void launch_generic_lines(GenFFT a, hipStream_t st) {
     set_factors(a, a.len);
     a.tw = generic_twiddles(a.len);
     a.tl = std::max<uint32_t>(1, 4096u / a.len);
+    a.f_lpb = make_fastdiv(a.lines_per_batch);
+    a.f_tl = make_fastdiv(a.tl);
+    a.f_len = make_fastdiv(a.len);
+    for (uint32_t r : {2u, 3u, 4u})
+        if (a.len % r == 0) a.f_nb[r] = make_fastdiv(a.len / r);
+    uint32_t s = 1;
+    for (uint32_t f = 0; f < a.nf; ++f) {
+        a.f_s[f] = make_fastdiv(s);
+        s *= a.fac[f];
+    }
     const size_t shmem = 2 * (size_t)a.tl * a.len * sizeof(float2);
     const uint32_t blocks = (a.nlines + a.tl - 1) / a.tl;
     hipLaunchKernelGGL(fft_generic_kernel, dim3(blocks), dim3(256), shmem, st, a);
@@ -212,10 +258,10 @@ __global__ __launch_bounds__(256) void gen_crop_kernel(const float2* __restrict_
         const uint32_t l = (uint32_t)(k / plane);
         const uint32_t pix = (uint32_t)(k - (size_t)l * plane);
         const uint32_t dj = pix / nr, di = pix - dj * nr;
-        const uint32_t i = (di + nr / 2) % nr, j = (dj + nc / 2) % nc;
+        const uint32_t i = wrap1(di + nr / 2, nr), j = wrap1(dj + nc / 2, nc);
         const int2 kk = led[led0 + l];
-        const uint32_t srow = (i + (uint32_t)kk.y - 1u + Nr / 2) % Nr;
-        const uint32_t scol = (j + (uint32_t)kk.x - 1u + Nc / 2) % Nc;
+        const uint32_t srow = wrap1(i + (uint32_t)kk.y - 1u + Nr / 2, Nr);
+        const uint32_t scol = wrap1(j + (uint32_t)kk.x - 1u + Nc / 2, Nc);
         const float2 pp = P[(size_t)j * nr + i];
         float2 z = make_float2(0.f, 0.f);
         if (pp.x != 0.0f || pp.y != 0.0f) z = cmul_g(S[(size_t)scol * Nr + srow], pp);
@@ -267,10 +313,12 @@ __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restri
         const uint32_t j = pix / nr, i = pix - j * nr;
         const float* el = e + l * plane;
         const float e0 = el[(size_t)j * nr + i];
-        const float eL = el[(size_t)((j + nc - 1) % nc) * nr + i];
-        const float eR = el[(size_t)((j + 1) % nc) * nr + i];
-        const float eU = el[(size_t)j * nr + (i + nr - 1) % nr];
-        const float eD = el[(size_t)j * nr + (i + 1) % nr];
+        const uint32_t jl = j == 0 ? nc - 1 : j - 1, jr = j + 1 == nc ? 0 : j + 1;
+        const uint32_t iu = i == 0 ? nr - 1 : i - 1, id = i + 1 == nr ? 0 : i + 1;
+        const float eL = el[(size_t)jl * nr + i];
+        const float eR = el[(size_t)jr * nr + i];
+        const float eU = el[(size_t)j * nr + iu];
+        const float eD = el[(size_t)j * nr + id];
         const float gval =
             (copysignf(1.0f, e0 - eL) - copysignf(1.0f, eR - e0) + copysignf(1.0f, e0 - eU) - copysignf(1.0f, eD - e0)) *
             gscale;
@@ -315,13 +363,14 @@ __global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restric
         float2 v = make_float2(0.f, 0.f);
         for (uint32_t l = 0; l < b; ++l) {
             const int2 kk = led[led0 + l];
-            const uint32_t wr = ((uint32_t)kk.y - 1u + Nr / 2) % Nr;
-            const uint32_t wc = ((uint32_t)kk.x - 1u + Nc / 2) % Nc;
-            const uint32_t li = (row + Nr - wr) % Nr, lj = (col + Nc - wc) % Nc;
+            const uint32_t wr = wrap1((uint32_t)kk.y - 1u + Nr / 2, Nr);
+            const uint32_t wc = wrap1((uint32_t)kk.x - 1u + Nc / 2, Nc);
+            const uint32_t li = row >= wr ? row - wr : row + Nr - wr;  // wr < Nr
+            const uint32_t lj = col >= wc ? col - wc : col + Nc - wc;
             if (li < nr && lj < nc) {
                 const float2 pp = P[(size_t)lj * nr + li];
                 if (pp.x != 0.0f || pp.y != 0.0f) {
-                    const uint32_t di = (li + nr / 2) % nr, dj = (lj + nc / 2) % nc;
+                    const uint32_t di = wrap1(li + nr / 2, nr), dj = wrap1(lj + nc / 2, nc);
                     const float2 gw = cmulc_g(R[(size_t)l * plane + (size_t)dj * nr + di], pp);
                     v.x += gw.x * invN2;
                     v.y += gw.y * invN2;
@@ -354,13 +403,13 @@ __global__ __launch_bounds__(256) void gen_pupil_kernel(const float2* __restrict
         A = tab[tk].A;
     }
     const uint32_t j = pix / nr, i = pix - j * nr;
-    const uint32_t di = (i + nr / 2) % nr, dj = (j + nc / 2) % nc;
+    const uint32_t di = wrap1(i + nr / 2, nr), dj = wrap1(j + nc / 2, nc);
     float2 gp = make_float2(0.f, 0.f);
     for (uint32_t l = 0; l < b; ++l) {
         const float2 Rv = R[(size_t)l * plane + (size_t)dj * nr + di];
         const int2 kk = led[led0 + l];
-        const uint32_t srow = (i + (uint32_t)kk.y - 1u + Nr / 2) % Nr;
-        const uint32_t scol = (j + (uint32_t)kk.x - 1u + Nc / 2) % Nc;
+        const uint32_t srow = wrap1(i + (uint32_t)kk.y - 1u + Nr / 2, Nr);
+        const uint32_t scol = wrap1(j + (uint32_t)kk.x - 1u + Nc / 2, Nc);
         const float2 gq = cmulc_g(Rv, S[(size_t)scol * Nr + srow]);
         gp.x += gq.x;
         gp.y += gq.y;

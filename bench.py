@@ -197,9 +197,8 @@ def run_tiles(args):
     """Uneven field-of-view tiles: 48 tiles of 128, 192 and 256 low-res
     pixels (--tile-sizes; upsample 4) over a 1024-pixel field, 19 x 19 LEDs
     seen from each tile's centre, per-tile Zernike pupils (sigma 0.4 rad),
-    batch 26, reconstructed jointly: the tiles of each power-of-two size in
-    shared launches, the 192-px tiles (mixed-radix sides) each in its own
-    launch sequence,
+    batch 26, reconstructed jointly: the tiles of each size in shared
+    launches (the 192-px tiles on the generic mixed-radix path),
     one hipGraph per epoch, autotuned stream count (the autotuning call's time
     includes its trial epochs and captures, so the chosen count is also timed
     fixed); the same with one launch sequence per tile; the grouped call on
@@ -260,7 +259,7 @@ def run_tiles(args):
             "speedup_grouped_vs_per_tile": t_pt / t_joint, "epoch_model_GB": epoch_bytes / 1e9,
             "generate_s": t_gen,
             "note": "sides that are not powers of two run the generic path (mixed-radix transforms, unfused "
-                    "kernels, one launch sequence per tile); the split engine applies to power-of-two sides"}
+                    "kernels, grouped like the others); the split engine applies to power-of-two sides"}
 
 
 def run_tiles_grouped(args):

@@ -354,39 +354,85 @@ __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restri
 // owner-computes stitch: G_S(row, col) = sum over the batch's LEDs, in LED
 // order, of R_l(fftshifted window pixel) conj(P) / (N_r N_c) (kernels.cuh:
 // 107-116 with the atomics replaced by a gather); every element written.
+// A workgroup owns a 64-row x 16-column block (thread: one row, 4 columns),
+// first keeps the LEDs whose windows meet the block (LDS bit mask), then
+// sums only those.
+constexpr uint32_t kGsTR = 64, kGsTC = 16;
 __global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restrict__ R, const float2* __restrict__ P,
                                                          const int2* __restrict__ led, uint32_t led0, uint32_t b,
                                                          uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
                                                          float invN2, float2* __restrict__ Gs,
                                                          const TileIn* __restrict__ tab) {
-    const size_t NN = (size_t)Nr * Nc, plane = (size_t)nr * nc;
+    __shared__ unsigned long long hit[4];
+    const size_t plane = (size_t)nr * nc;
     if (tab) {
         const uint32_t tk = blockIdx.y;
         R += (size_t)tk * b * plane;
         P += (size_t)tk * plane;
-        Gs += (size_t)tk * NN;
+        Gs += (size_t)tk * Nr * Nc;
         led = tab[tk].led;
     }
-    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < NN; k += (size_t)gridDim.x * 256) {
-        const uint32_t col = (uint32_t)(k / Nr), row = (uint32_t)(k - (size_t)col * Nr);
-        float2 v = make_float2(0.f, 0.f);
-        for (uint32_t l = 0; l < b; ++l) {
-            const int2 kk = led[led0 + l];
+    const uint32_t tiles_r = (Nr + kGsTR - 1) / kGsTR;
+    const uint32_t ra = (blockIdx.x % tiles_r) * kGsTR, ca = (blockIdx.x / tiles_r) * kGsTC;
+    const uint32_t t = threadIdx.x, r = t & 63u, cg = t >> 6;
+    const uint32_t row = ra + r;
+    float2 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (uint32_t c0 = 0; c0 < b; c0 += 256) {
+        if (t < 4) hit[t] = 0ull;
+        __syncthreads();
+        const uint32_t me = c0 + t;
+        if (me < b) {
+            const int2 kk = led[led0 + me];
             const uint32_t wr = wrap1((uint32_t)kk.y - 1u + Nr / 2, Nr);
             const uint32_t wc = wrap1((uint32_t)kk.x - 1u + Nc / 2, Nc);
-            const uint32_t li = row >= wr ? row - wr : row + Nr - wr;  // wr < Nr
-            const uint32_t lj = col >= wc ? col - wc : col + Nc - wc;
-            if (li < nr && lj < nc) {
-                const float2 pp = P[(size_t)lj * nr + li];
-                if (pp.x != 0.0f || pp.y != 0.0f) {
-                    const uint32_t di = wrap1(li + nr / 2, nr), dj = wrap1(lj + nc / 2, nc);
-                    const float2 gw = cmulc_g(R[(size_t)l * plane + (size_t)dj * nr + di], pp);
-                    v.x += gw.x * invN2;
-                    v.y += gw.y * invN2;
+            // window rows [wr, wr + nr) and block rows [ra, ra + 64) (mod N) meet
+            // iff one start lies inside the other range (conservative at the
+            // last, partial block)
+            const bool rows = (ra >= wr ? ra - wr : ra + Nr - wr) < nr || (wr >= ra ? wr - ra : wr + Nr - ra) < kGsTR;
+            const bool cols = (ca >= wc ? ca - wc : ca + Nc - wc) < nc || (wc >= ca ? wc - ca : wc + Nc - ca) < kGsTC;
+            if (rows && cols) atomicOr(&hit[t >> 6], 1ull << (t & 63u));
+        }
+        __syncthreads();
+        for (int w = 0; w < 4; ++w) {
+            unsigned long long m = hit[w];
+            while (m) {
+                const int bit = __ffsll(m) - 1;
+                m &= m - 1;
+                const uint32_t l = c0 + (uint32_t)w * 64u + (uint32_t)bit;
+                const int2 kk = led[led0 + l];
+                const uint32_t wr = wrap1((uint32_t)kk.y - 1u + Nr / 2, Nr);
+                const uint32_t wc = wrap1((uint32_t)kk.x - 1u + Nc / 2, Nc);
+                if (row >= Nr) continue;
+                const uint32_t li = row >= wr ? row - wr : row + Nr - wr;
+                if (li >= nr) continue;
+                const uint32_t di = wrap1(li + nr / 2, nr);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t col = ca + cg * 4u + (uint32_t)q;
+                    if (col >= Nc) continue;
+                    const uint32_t lj = col >= wc ? col - wc : col + Nc - wc;
+                    if (lj < nc) {
+                        const float2 pp = P[(size_t)lj * nr + li];
+                        if (pp.x != 0.0f || pp.y != 0.0f) {
+                            const uint32_t dj = wrap1(lj + nc / 2, nc);
+                            const float2 gw = cmulc_g(R[(size_t)l * plane + (size_t)dj * nr + di], pp);
+                            acc[q].x += gw.x * invN2;
+                            acc[q].y += gw.y * invN2;
+                        }
+                    }
                 }
             }
         }
-        Gs[k] = v;
+        __syncthreads();
+    }
+    if (row < Nr) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t col = ca + cg * 4u + (uint32_t)q;
+            if (col < Nc) Gs[(size_t)col * Nr + row] = acc[q];
+        }
     }
 }
 
@@ -523,8 +569,9 @@ void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
                     float2* Gs, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
     if (!tab) ntiles = 1;
-    hipLaunchKernelGGL(gen_stitch_kernel, dim3(gblocks((size_t)g.Nr * g.Nc), ntiles), dim3(256), 0, st, R, P, led,
-                       led0, b, g.nr, g.nc, g.Nr, g.Nc, 1.0f / (float)(g.Nr * g.Nc), Gs, tab);
+    const uint32_t blocks = ((g.Nr + kGsTR - 1) / kGsTR) * ((g.Nc + kGsTC - 1) / kGsTC);
+    hipLaunchKernelGGL(gen_stitch_kernel, dim3(blocks, ntiles), dim3(256), 0, st, R, P, led, led0, b, g.nr, g.nc,
+                       g.Nr, g.Nc, 1.0f / (float)(g.Nr * g.Nc), Gs, tab);
     FPM_LAUNCH_CHECK();
 }
 

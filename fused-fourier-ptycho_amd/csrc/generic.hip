@@ -25,6 +25,9 @@
 #ifndef FPM_GEN_CONTIG_POINTS
 #define FPM_GEN_CONTIG_POINTS 1024
 #endif
+#ifndef FPM_GEN_STRIDED_POINTS
+#define FPM_GEN_STRIDED_POINTS 2048
+#endif
 
 namespace fpm {
 
@@ -222,10 +225,12 @@ void launch_generic_lines(GenFFT a, hipStream_t st) {
     a.tw = generic_twiddles(a.len);
     // lines per workgroup: strided lines need several adjacent lines per
     // column read (coalescing), contiguous ones do not and take fewer
-    // points per workgroup (smaller LDS, more workgroups per CU; measured on
-    // the 48-tile solve with 16 tiles of 192 px: 4096 -> 1024 points,
-    // 39.6 -> 35.8 ms per epoch; 512 points 37.2)
-    a.tl = std::max<uint32_t>(1, (a.es == 1 ? (uint32_t)FPM_GEN_CONTIG_POINTS : 4096u) / a.len);
+    // points per workgroup (smaller LDS, more workgroups per CU).  Measured
+    // on the 48-tile solve with 16 tiles of 192 px: contiguous 4096 -> 1024
+    // points 39.6 -> 35.8 ms per epoch (512: 37.2); then, with the block-
+    // filtered stitch, strided 4096 -> 2048 points 33.5 -> 32.2 (1024: 35.6)
+    a.tl = std::max<uint32_t>(1, (a.es == 1 ? (uint32_t)FPM_GEN_CONTIG_POINTS : (uint32_t)FPM_GEN_STRIDED_POINTS) /
+                                     a.len);
     a.f_lpb = make_fastdiv(a.lines_per_batch);
     a.f_tl = make_fastdiv(a.tl);
     a.f_len = make_fastdiv(a.len);

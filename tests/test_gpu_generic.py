@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-@pytest.mark.parametrize("rows,cols,batch", [(192, 192, 2), (768, 768, 1), (48, 96, 3), (192, 256, 1), (24, 18, 4)])
+@pytest.mark.parametrize("rows,cols,batch", [(192, 192, 2), (768, 768, 1), (48, 96, 3), (192, 256, 1), (24, 18, 4),
+                                             (3072, 48, 1)])
 def test_generic_fft_matches_numpy(rows, cols, batch):
     rng = np.random.default_rng(rows + cols)
     x = np.asfortranarray((rng.standard_normal((rows, cols, batch)) + 1j * rng.standard_normal((rows, cols, batch)))

@@ -132,21 +132,100 @@ def test_solve_update_rule_scalar_restatement():
     assert (mag <= a * (a + 0.2) + 1e-12).all() and (mag >= a * (a - 0.2) - 1e-12).all()
 
 
-def test_lr_schedule_halves_after_epoch_12():
-    """With zero residual curvature the per-step displacement is lr-scaled:
-    run 16 epochs and check the loss trajectory is finite and the lr halving
-    is visible as a shrinking step (epochs 14, 15 move less than 13)."""
+def _state_problem():
+    """Margin inputs, one minibatch per epoch (batch = L): every sign of the
+    loss stencil is decided by >= 0.01, so with a tiny learning rate the
+    gradient stays constant to ~1e-5 relative over 16 steps."""
     n, r = 16, 2
     O, P, Y, led = margin_problem(n, r)
     A, _ = synth.pupil(n)
+    return n, r, O, P, Y, led, A
+
+
+def _closed_form_step(g, lr_e, beta, k, eps):
+    """RMSprop without bias correction from m_0 = 0 under a constant
+    gradient g: m_k = (1 - beta^k) |g|^2 exactly, so step k moves the
+    variable by lr_e g / (sqrt(1 - beta^k) |g| + eps).  This is the
+    recurrence of kernels.cuh:368-377 summed in closed form; it fails if
+    the m store is dropped (m_k = (1 - beta)|g|^2: steps stay equal) or if
+    the carried term is mis-scaled."""
+    return lr_e * g / (np.sqrt(1.0 - beta ** k) * np.abs(g) + eps)
+
+
+@pytest.mark.parametrize("prec,lr,tol", [("f64", 1e-7, 2e-4), ("f32", 3e-6, 5e-3)])
+def test_solve_carried_moments_closed_form(prec, lr, tol):
+    """Steps 1..4 of the oracle solve against the closed form above, for O
+    and for P (inside the pupil support |P| stays in A[A-0.2, A+0.2], so the
+    clamp is the identity and the same closed form holds).  Step k+1 / step
+    k is sqrt((1 - beta^k) / (1 - beta^(k+1))), 0.707 for k = 1; a solve
+    that forgot m between steps gives 1."""
+    n, r, O, P, Y, led, A = _state_problem()
     L = led.shape[1]
-    prev = O.astype(np.complex128)
-    steps = []
-    for ep in range(1, 17):
-        Oe, _, _ = oracle.solve(O, P, Y, led, A, r, ep, L, (1e-3, 0.999, 1e-4), prec="f64")
-        steps.append(np.linalg.norm(Oe - prev))
-        prev = Oe
-    assert steps[14] < 0.75 * steps[12] and steps[15] < 0.75 * steps[13]
+    beta = float(np.float32(0.999))
+    eps = float(np.float32(1e-4))
+    lr32 = float(np.float32(lr))
+    GO, GP, _, _ = oracle.gradient(O, P, Y, led, r)          # pinned by finite differences
+    prevO, prevP = O.astype(np.complex128), P.astype(np.complex128)
+    inside = A > 0
+    for k in range(1, 5):
+        Ok, Pk, _ = oracle.solve(O, P, Y, led, A, r, k, L, (lr, 0.999, 1e-4), prec=prec)
+        dO, dP = prevO - Ok, prevP - Pk
+        assert rel_l2(dO, _closed_form_step(GO, lr32, beta, k, eps)) < tol, k
+        assert rel_l2(dP[inside], _closed_form_step(GP[inside], lr32, beta, k, eps)) < tol, k
+        assert np.abs(Pk[~inside]).max() == 0.0
+        prevO, prevP = Ok.astype(np.complex128), Pk.astype(np.complex128)
+        if k == 2:
+            ratio = np.linalg.norm(dO) / step1
+            assert abs(ratio - np.sqrt((1 - beta) / (1 - beta ** 2))) < 5 * tol
+        step1 = np.linalg.norm(dO)
+
+
+def test_lr_halving_epoch_exact():
+    """cuFPM_pure_v3.cu:204-206 halves lr at the end of every epoch with
+    index > 12: epochs 0..13 step with lr, epoch 14 with lr/2, epoch 15 with
+    lr/4.  With one minibatch per epoch, each epoch's displacement is
+    checked against the closed form with that epoch's lr, so halving one
+    epoch early (after epoch 12) or late (after 14) fails."""
+    n, r, O, P, Y, led, A = _state_problem()
+    L = led.shape[1]
+    lr = 1e-7
+    beta = float(np.float32(0.999))
+    eps = float(np.float32(1e-4))
+    lr32 = float(np.float32(lr))
+    GO, _, _, _ = oracle.gradient(O, P, Y, led, r)
+    prev = None
+    disp = {}
+    for ep in range(11, 17):  # solve of `ep` epochs ends after epoch index ep-1
+        Oe, _, _ = oracle.solve(O, P, Y, led, A, r, ep, L, (lr, 0.999, 1e-4), prec="f64")
+        if prev is not None:
+            e = ep - 1  # displacement of epoch index e
+            lr_e = lr32 * (0.5 ** max(0, e - 13))
+            d = prev - Oe
+            assert rel_l2(d, _closed_form_step(GO, lr_e, beta, e + 1, eps)) < 2e-4, e
+            disp[e] = np.linalg.norm(d)
+        prev = Oe.astype(np.complex128)
+    f = lambda e: np.sqrt(1 - beta ** (e + 1))  # noqa: E731
+    assert abs(disp[12] / disp[11] - f(11) / f(12)) < 1e-3   # no halving before epoch 13
+    assert abs(disp[13] / disp[12] - f(12) / f(13)) < 1e-3   # epoch 13 still at full lr
+    assert abs(disp[14] / disp[13] - 0.5 * f(13) / f(14)) < 1e-3
+    assert abs(disp[15] / disp[14] - 0.5 * f(14) / f(15)) < 1e-3
+
+
+def test_lr_halve_from_option():
+    """The schedule option moves the first halved epoch: halve_from = h gives
+    lr/2 from epoch h+1 on (13 is the reference)."""
+    n, r, O, P, Y, led, A = _state_problem()
+    L = led.shape[1]
+    opt = (1e-7, 0.999, 1e-4)
+    o2, _, _ = oracle.solve(O, P, Y, led, A, r, 2, L, opt, prec="f64", halve_from=0)
+    o3, _, _ = oracle.solve(O, P, Y, led, A, r, 3, L, opt, prec="f64", halve_from=0)
+    o1, _, _ = oracle.solve(O, P, Y, led, A, r, 1, L, opt, prec="f64", halve_from=0)
+    d1 = np.linalg.norm(o2 - o1)
+    d2 = np.linalg.norm(o3 - o2)
+    beta = float(np.float32(0.999))
+    f = lambda k: np.sqrt(1 - beta ** k)  # noqa: E731
+    assert abs(d1 / np.linalg.norm(o1 - O.astype(np.complex128)) - 0.5 * f(1) / f(2)) < 1e-3
+    assert abs(d2 / d1 - 0.5 * f(2) / f(3)) < 1e-3
 
 
 def test_prepare_amplitude_oracle():

@@ -14,9 +14,6 @@
 
 #include "fpm_amd.h"
 
-#include <cstring>
-#include <initializer_list>
-
 namespace {
 
 struct In {
@@ -32,11 +29,12 @@ In view(const mxArray* a) {
     if (mxIsGPUArray(a)) {
         v.g = mxGPUCreateFromMxArray(a);
         v.ptr = mxGPUGetDataReadOnly(v.g);
-        const mwSize* d = mxGPUGetDimensions(v.g);
+        mwSize* d = (mwSize*)mxGPUGetDimensions(v.g);  // a copy: freed with mxFree
         const mwSize nd = mxGPUGetNumberOfDimensions(v.g);
         v.rows = d[0];
         v.cols = nd > 1 ? d[1] : 1;
         v.pages = nd > 2 ? d[2] : 1;
+        mxFree(d);
         v.cls = mxGPUGetClassID(v.g);
         v.complex = mxGPUGetComplexity(v.g) == mxCOMPLEX;
     } else {
@@ -52,22 +50,40 @@ In view(const mxArray* a) {
     return v;
 }
 
+void release(const In* const* vs, int count) {
+    for (int k = 0; k < count; ++k)
+        if (vs[k]->g) mxGPUDestroyGPUArray(vs[k]->g);
+}
+
+// releases the gpuArray views, then raises the MATLAB error (which does not return)
+[[noreturn]] void fail(const In* const* vs, int count, const char* id, const char* msg) {
+    release(vs, count);
+    mexErrMsgIdAndTxt(id, "%s", msg);
+    for (;;) {
+    }
+}
+
 }  // namespace
 
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     if (nrhs != 9) mexErrMsgIdAndTxt("fpm:nrhs", "cuFPM_pure_v3 takes 9 inputs");
     if (nlhs > 2) mexErrMsgIdAndTxt("fpm:nlhs", "cuFPM_pure_v3 returns at most 2 outputs");
-    mxInitGPU();
+    // MATLAB's GPU library (gpuArray) exists only on CUDA devices: on an
+    // MI355X host every input is a host array and mxInitGPU is not called
+    bool any_gpu = false;
+    for (int k = 0; k < 5; ++k) any_gpu = any_gpu || mxIsGPUArray(prhs[k]);
+    if (any_gpu) mxInitGPU();
     In O0 = view(prhs[0]), P0 = view(prhs[1]), Y = view(prhs[2]), led = view(prhs[3]), A = view(prhs[4]);
+    const In* all[] = {&O0, &P0, &Y, &led, &A};
     if (O0.cls != mxSINGLE_CLASS || !O0.complex || P0.cls != mxSINGLE_CLASS || !P0.complex)
-        mexErrMsgIdAndTxt("fpm:type", "O0 and P0 must be complex single");
-    if (led.cls != mxINT32_CLASS || A.cls != mxSINGLE_CLASS) mexErrMsgIdAndTxt("fpm:type", "ledIdx int32, A single");
+        fail(all, 5, "fpm:type", "O0 and P0 must be complex single");
+    if (led.cls != mxINT32_CLASS || A.cls != mxSINGLE_CLASS) fail(all, 5, "fpm:type", "ledIdx int32, A single");
     const int32_t ytype = Y.cls == mxUINT16_CLASS ? FPM_Y_U16_RAW : FPM_Y_F32_AMPLITUDE;
     const bool dev = O0.g && P0.g && Y.g && led.g && A.g;
-    if (!dev && (O0.g || P0.g || Y.g || led.g || A.g))
-        mexErrMsgIdAndTxt("fpm:mixed", "pass either all arrays as gpuArray or none");
-    const double* opt = mxGetPr(prhs[8]);
-    if (mxGetNumberOfElements(prhs[8]) < 3) mexErrMsgIdAndTxt("fpm:opt", "optimizer argument is [lr beta eps]");
+    if (!dev && (O0.g || P0.g || Y.g || led.g || A.g)) fail(all, 5, "fpm:mixed", "pass either all arrays as gpuArray or none");
+    if (mxGetNumberOfElements(prhs[8]) < 3 || !mxIsDouble(prhs[8]))
+        fail(all, 5, "fpm:opt", "optimizer argument is double [lr beta eps]");
+    const double* opt = mxGetDoubles(prhs[8]);
 
     fpm_opts o;
     fpm_opts_init(&o);
@@ -78,8 +94,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     void *Op, *Pp;
     mxGPUArray *Og = nullptr, *Pg = nullptr;
     if (dev) {
-        Og = mxGPUCreateGPUArray(2, mxGPUGetDimensions(O0.g), mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
-        Pg = mxGPUCreateGPUArray(2, mxGPUGetDimensions(P0.g), mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
+        const mwSize od[2] = {O0.rows, O0.cols}, pd[2] = {P0.rows, P0.cols};
+        Og = mxGPUCreateGPUArray(2, od, mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
+        Pg = mxGPUCreateGPUArray(2, pd, mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
         Op = mxGPUGetData(Og);
         Pp = mxGPUGetData(Pg);
     } else {
@@ -92,8 +109,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                                (int64_t)P0.rows, (int64_t)P0.cols, Y.ptr, ytype, (int64_t)Y.pages,
                                (const int32_t*)led.ptr, (const float*)A.ptr, mxGetScalar(prhs[5]),
                                mxGetScalar(prhs[6]), mxGetScalar(prhs[7]), opt, (fpm_c32*)Op, (fpm_c32*)Pp, &o);
-    for (const In* v : {&O0, &P0, &Y, &led, &A})
-        if (v->g) mxGPUDestroyGPUArray(v->g);
+    release(all, 5);
     if (rc != FPM_OK) {
         if (Og) mxGPUDestroyGPUArray(Og);
         if (Pg) mxGPUDestroyGPUArray(Pg);

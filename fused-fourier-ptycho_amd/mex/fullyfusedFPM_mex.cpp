@@ -35,11 +35,12 @@ In view(const mxArray* a) {
     if (mxIsGPUArray(a)) {
         v.g = mxGPUCreateFromMxArray(a);
         v.ptr = mxGPUGetDataReadOnly(v.g);
-        const mwSize* d = mxGPUGetDimensions(v.g);
+        mwSize* d = (mwSize*)mxGPUGetDimensions(v.g);  // a copy: freed with mxFree
         const mwSize nd = mxGPUGetNumberOfDimensions(v.g);
         v.rows = d[0];
         v.cols = nd > 1 ? d[1] : 1;
         v.pages = nd > 2 ? d[2] : 1;
+        mxFree(d);
         v.cls = mxGPUGetClassID(v.g);
         v.complex = mxGPUGetComplexity(v.g) == mxCOMPLEX;
     } else {
@@ -65,7 +66,11 @@ void release(const In* const* vs, int count) {
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     if (nrhs != 5) mexErrMsgIdAndTxt("fpm:nrhs", "fullyfusedFPM takes 5 inputs (w1, w2, Y, ledIdx, pratio)");
     if (nlhs > 2) mexErrMsgIdAndTxt("fpm:nlhs", "fullyfusedFPM returns at most 2 outputs");
-    mxInitGPU();
+    // MATLAB's GPU library (gpuArray) exists only on CUDA devices: on an
+    // MI355X host every input is a host array and mxInitGPU is not called
+    bool any_gpu = false;
+    for (int k = 0; k < 4; ++k) any_gpu = any_gpu || mxIsGPUArray(prhs[k]);
+    if (any_gpu) mxInitGPU();
     In w1 = view(prhs[0]), w2 = view(prhs[1]), Y = view(prhs[2]), led = view(prhs[3]);
     const In* all[] = {&w1, &w2, &Y, &led};
     if (w1.cls != mxSINGLE_CLASS || !w1.complex || w2.cls != mxSINGLE_CLASS || !w2.complex) {
@@ -90,8 +95,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     mxGPUArray *G1g = nullptr, *G2g = nullptr;
     void *g1p, *g2p;
     if (dev) {
-        G1g = mxGPUCreateGPUArray(2, mxGPUGetDimensions(w1.g), mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
-        G2g = mxGPUCreateGPUArray(2, mxGPUGetDimensions(w2.g), mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
+        const mwSize d1[2] = {w1.rows, w1.cols}, d2[2] = {w2.rows, w2.cols};
+        G1g = mxGPUCreateGPUArray(2, d1, mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
+        G2g = mxGPUCreateGPUArray(2, d2, mxSINGLE_CLASS, mxCOMPLEX, MX_GPU_DO_NOT_INITIALIZE);
         g1p = mxGPUGetData(G1g);
         g2p = mxGPUGetData(G2g);
     } else {

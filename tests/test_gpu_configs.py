@@ -1,0 +1,261 @@
+"""The workloads of the larger configurations, checked where they run:
+
+* 512^2 x 361 -> 4096^2, batch 26, 20 epochs (the drop-in GPU case) against
+  the FP32 CPU restatement at the end of the solve, and against FP64;
+* 48 field-of-view tiles of 128/192/256 px over a 1024 px (4096 px at
+  upsample 4) field with per-tile Zernike pupils: every tile of the joint
+  call equals its own fpm_pure_v3 call bit for bit, and a 192 -> 768 tile
+  (generic mixed-radix path) and a 256 -> 1024 tile agree with the oracle;
+* the 4-level, base-32 warm-start network on 256^2 features -> 1024^2:
+  forward and gradient against the numpy network oracle;
+* the generic path at n = 192, r = 4 (768) and on a non-square geometry
+  (48 x 96 -> 192 x 384) against the FP64 oracle on margin inputs."""
+import numpy as np
+import pytest
+
+import cnn_oracle
+import fpm_amd
+import oracle
+from fpm_amd import synth, warmstart
+from fpm_amd import tiles as ft
+from helpers import margin_field, margin_problem, rel_l2
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _host(t):
+    """device layout (row-major buffer of a column-major array) -> MATLAB-oriented host array"""
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+def _host_stack(t):
+    """(L, n_c, n_r) device stack -> (n_r, n_c, L) Fortran host array"""
+    return np.asfortranarray(t.cpu().numpy().transpose(2, 1, 0))
+
+
+# ----------------------------------------------------------------- 512 -> 4096, 20 epochs
+@pytest.mark.timeout(1200)
+def test_solve_512x361_to_4096_20_epochs_matches_fp32_cpu_path():
+    """512^2 x 361 uint16 -> 4096^2, batch 26, lr 5e-4, 20 epochs of the
+    reference's schedule (lr halved after each epoch > 12), from the same
+    init and batch partition: the GPU's O and P within rel-L2 1e-3 of the
+    FP32 CPU restatement at the end of the solve, no further from FP64 than
+    that restatement (1.5x + 1e-4), and every epoch's loss within 1e-3."""
+    n, r, ep, B = 512, 8, 20, 26
+    opt = (5e-4, 0.999, 1e-4)
+    ds = synth.make_device(n, r)
+    N = n * r
+    O, P = torch.empty_like(ds["O0_dev"]), torch.empty_like(ds["P0_dev"])
+    torch.cuda.synchronize()
+    lg = fpm_amd.solve_device(ds["O0_dev"].data_ptr(), ds["P0_dev"].data_ptr(), ds["raw_dev"].data_ptr(), True,
+                              ds["led_dev"].data_ptr(), ds["A_dev"].data_ptr(), (N, N), (n, n), ds["L"], r, ep, B,
+                              opt, O.data_ptr(), P.data_ptr(), loss=True)
+    torch.cuda.synchronize()
+    Og, Pg = _host(O), _host(P)
+    O0, P0, A = _host(ds["O0_dev"]), _host(ds["P0_dev"]), _host(ds["A_dev"])
+    Y = oracle.prepare_amplitude(_host_stack(ds["raw_dev"]))
+    led = np.asfortranarray(ds["led"])
+    del ds, O, P
+    torch.cuda.empty_cache()
+    O32, P32, l32 = oracle.solve(O0, P0, Y, led, A, r, ep, B, opt, prec="f32")
+    O64, P64, _ = oracle.solve(O0, P0, Y, led, A, r, ep, B, opt, prec="f64")
+    e32 = {"O": rel_l2(Og, O32), "P": rel_l2(Pg, P32)}
+    e64 = {"O": rel_l2(Og, O64), "P": rel_l2(Pg, P64)}
+    ref = {"O": rel_l2(O32, O64), "P": rel_l2(P32, P64)}
+    print(f"[cfg2-20ep] GPU vs FP32 CPU path: O {e32['O']:.2e} P {e32['P']:.2e}; GPU vs FP64: O {e64['O']:.2e} "
+          f"P {e64['P']:.2e}; FP32 CPU path vs FP64: O {ref['O']:.2e} P {ref['P']:.2e}; loss {lg[0]:.4e} -> "
+          f"{lg[-1]:.4e} (CPU {l32[-1]:.4e})", flush=True)
+    assert e32["O"] <= 1e-3 and e32["P"] <= 1e-3
+    for k in e64:
+        assert e64[k] <= 1.5 * ref[k] + 1e-4, k
+    np.testing.assert_allclose(lg, l32, rtol=1e-3)
+    assert lg[-1] < lg[0]
+
+
+# ----------------------------------------------------------------- 48 uneven tiles
+@pytest.fixture(scope="module")
+def tile_set_48():
+    # the benchmark's tiles: 16 each of 128, 192 and 256 px on a 7 x 7 lattice
+    # of a 1024 px field, 19 x 19 LEDs seen from every tile centre
+    return ft.make_tile_set(1024, (128, 192, 256), 7, 4, seed=11, count=48)
+
+
+def test_48_tiles_joint_solve_equals_per_tile_solves(tile_set_48):
+    """2 epochs, batch 26, autotuned stream count: the grouped call runs
+    three launch sequences (one per size, the 192-px group on the generic
+    path) and every tile's O and P equal its own fpm_pure_v3 call bit for
+    bit."""
+    tl = tile_set_48
+    assert len(tl) == 48 and sorted({t["n"] for t in tl}) == [128, 192, 256]
+    opt = (5e-4, 0.999, 1e-4)
+    rep, losses = ft.tiles_solve(tl, 4, 2, 26, opt, streams=0, loss=True)
+    print(f"[tiles-48] report {rep}", flush=True)
+    assert rep["sequences"] == 3 and rep["grouped_tiles"] == 48
+    for k, t in enumerate(tl):
+        O, P = torch.empty_like(t["O0"]), torch.empty_like(t["P0"])
+        N, n = t["O0"].shape[0], t["P0"].shape[0]
+        torch.cuda.synchronize()
+        l1 = fpm_amd.solve_device(t["O0"].data_ptr(), t["P0"].data_ptr(), t["Y"].data_ptr(), True,
+                                  t["led"].data_ptr(), t["A"].data_ptr(), (N, N), (n, n), t["Y"].shape[0], 4, 2, 26,
+                                  opt, O.data_ptr(), P.data_ptr(), loss=True)
+        assert torch.equal(t["O"], O), k
+        assert torch.equal(t["P"], P), k
+        np.testing.assert_allclose(losses[k], l1, rtol=1e-12, atol=0, err_msg=str(k))
+
+
+@pytest.fixture(scope="module")
+def tiles_48_20_epochs(tile_set_48):
+    """The joint call over all 48 tiles for the full 20-epoch schedule (lr
+    5e-4, halved after each epoch > 12, batch 26)."""
+    opt = (5e-4, 0.999, 1e-4)
+    rep, losses = ft.tiles_solve(tile_set_48, 4, 20, 26, opt, streams=0, loss=True)
+    return opt, {k: (_host(t["O"]), _host(t["P"]), losses[k]) for k, t in enumerate(tile_set_48)}
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("n", [192, 256])
+def test_48_tiles_one_tile_against_the_oracle(tile_set_48, tiles_48_20_epochs, n):
+    """One tile of the joint 20-epoch call (192 -> 768 on the generic path,
+    256 -> 1024 on the fused path; its own aberrated pupil and LED windows)
+    against the CPU restatement run alone on that tile from the same init and
+    batch partition: no further from FP64 than the FP32 restatement (1.5x +
+    1e-4), within 2x + 1e-4 of that distance from the FP32 restatement, and
+    per-epoch losses within 1e-3.  These tiles start from the ideal pupil
+    while their true pupils carry ~0.4 rad Zernike terms, so near-tie
+    loss-stencil signs flip differently in any two FP32 computations and the
+    trajectories part: measured on the 192-px tile, the FP32 restatement is
+    1.43e-2 (O) / 1.89e-2 (P) from FP64 after 20 epochs, the GPU 1.35e-2 /
+    1.77e-2 (the sign-synchronised gradient test bounds the arithmetic
+    error itself)."""
+    tl = tile_set_48
+    opt, res = tiles_48_20_epochs
+    k = next(i for i, t in enumerate(tl) if t["n"] == n)
+    t = tl[k]
+    Og, Pg, lg = res[k]
+    O0, P0, A = _host(t["O0"]), _host(t["P0"]), _host(t["A"])
+    Y = oracle.prepare_amplitude(_host_stack(t["Y"]))
+    led = _host(t["led"])
+    O32, P32, l32 = oracle.solve(O0, P0, Y, led, A, 4, 20, 26, opt, prec="f32")
+    O64, P64, _ = oracle.solve(O0, P0, Y, led, A, 4, 20, 26, opt, prec="f64")
+    e32 = {"O": rel_l2(Og, O32), "P": rel_l2(Pg, P32)}
+    e64 = {"O": rel_l2(Og, O64), "P": rel_l2(Pg, P64)}
+    ref = {"O": rel_l2(O32, O64), "P": rel_l2(P32, P64)}
+    print(f"[tiles-48] tile {k} ({n} -> {4 * n}), 20 epochs: vs FP32 oracle O {e32['O']:.2e} P {e32['P']:.2e}; "
+          f"vs FP64 O {e64['O']:.2e} P {e64['P']:.2e}; FP32 oracle vs FP64 O {ref['O']:.2e} P {ref['P']:.2e}; "
+          f"loss {lg[0]:.4e} -> {lg[-1]:.4e}", flush=True)
+    for key in e32:
+        assert e64[key] <= 1.5 * ref[key] + 1e-4, key
+        assert e32[key] <= 2.0 * ref[key] + 1e-4, key
+    np.testing.assert_allclose(lg, l32, rtol=1e-3)
+
+
+# ----------------------------------------------------------------- network at 256^2
+def test_warm_start_network_256_forward_and_gradient_match_oracle():
+    """The configuration's network: 4 levels, base 32 channels, 26 feature
+    planes of 256^2 -> amplitude/phase at 1024^2, one sample, random head
+    (so every layer's output and gradient is non-trivial).  Forward within
+    1e-2 and gradients within 2e-2 (whole vector) / 5e-2 (per tensor) of the
+    numpy oracle with BF16 rounding emulated; loss within 1e-3."""
+    n, levels = 256, 4
+    net = warmstart.WarmStartNet(n, 4, levels=levels, base=32, in_channels=26, max_batch=1, seed=3)
+    onet = cnn_oracle.Net(levels, 32, 26, 4, bf16=True)
+    p = net.get_params()
+    rng = np.random.default_rng(3)
+    ci, co, _ = onet.convs[-1]
+    k = onet.nconv - (co * 9 * ci + co)
+    p[k:k + co * 9 * ci] = rng.standard_normal(co * 9 * ci) * np.sqrt(2.0 / (9 * ci))
+    net.set_params(p)
+    p = net.get_params()
+    f = rng.random((1, 26, n, n)).astype(np.float32)
+    tgt = rng.standard_normal((1, 2, 4 * n, 4 * n)).astype(np.float32)
+    got = net.forward(torch.from_numpy(f).cuda()).cpu().numpy()
+    ref, _ = onet.forward(p.astype(np.float64), f.astype(np.float64))
+    ef = rel_l2(got, ref)
+    loss, g = net.gradient(torch.from_numpy(f).cuda(), torch.from_numpy(tgt).cuda())
+    lref, gref, _ = onet.loss_and_grad(p.astype(np.float64), f.astype(np.float64), tgt.astype(np.float64))
+    worst, k = 0.0, 0
+    for ci, cout, _ in onet.convs:
+        for size in (cout * 9 * ci, cout):
+            worst = max(worst, rel_l2(g[k:k + size], gref[k:k + size]))
+            k += size
+    assert k == g.size
+    print(f"[cnn-256] forward rel-L2 {ef:.2e}; loss {loss:.6f} vs {lref:.6f}; gradient whole {rel_l2(g, gref):.2e} "
+          f"worst tensor {worst:.2e}", flush=True)
+    net.close()
+    assert ef <= 1e-2
+    assert abs(loss - lref) <= 1e-3 * lref
+    assert rel_l2(g, gref) <= 2e-2 and worst <= 5e-2
+
+
+# ----------------------------------------------------------------- generic path, larger and non-square
+def test_generic_192_to_768_gradient_and_step_match_fp64():
+    """n = 192, r = 4 (768 = 3 * 256, the generic path) on margin inputs, 25
+    LEDs: the fused gradient within 1e-5 of FP64 and one solve step within
+    1e-6."""
+    n, r = 192, 4
+    O, P, Y, led = margin_problem(n, r, 5)
+    GO, GP, loss, margin = oracle.gradient(O, P, Y, led, r)
+    assert margin > 5e-3
+    gO, gP, gl = fpm_amd.fused_gradient(O, P, Y, led, r)
+    print(f"[generic-768] gradient G_O {rel_l2(gO, GO):.2e} G_P {rel_l2(gP, GP):.2e}", flush=True)
+    assert rel_l2(gO, GO) <= 1e-5 and rel_l2(gP, GP) <= 1e-5
+    assert abs(gl - loss) <= 1e-5 * loss
+    A, _ = synth.pupil(n)
+    L = led.shape[1]
+    opt = (2e-3, 0.999, 1e-4)
+    O1, P1 = fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, r, 1, L, opt)
+    Oo, Po, _ = oracle.solve(O, P, Y, led, A, r, 1, L, opt, prec="f64")
+    print(f"[generic-768] one step O {rel_l2(O1, Oo):.2e} P {rel_l2(P1, Po):.2e}", flush=True)
+    assert rel_l2(O1, Oo) < 1e-6 and rel_l2(P1, Po) < 1e-6
+
+
+def _rect_margin_problem(nr, nc, r, L, seed=5):
+    """Non-square geometry: object (nr r) x (nc r), pupil nr x nc (an
+    elliptical NA disk with a defocus phase), L LED windows around the centre
+    (kl indexes columns, kt rows), and images whose residual is a margin
+    field (every loss-stencil sign decided by >= 0.01)."""
+    rng = np.random.default_rng(seed)
+    Nr, Nc = nr * r, nc * r
+    spec = np.fft.fft2(rng.random((Nr, Nc)))
+    fy = np.fft.fftfreq(Nr)[:, None]
+    fx = np.fft.fftfreq(Nc)[None, :]
+    spec *= (fy ** 2 + fx ** 2) <= (0.5 / r) ** 2
+    amp = np.real(np.fft.ifft2(spec))
+    amp = 0.3 + 0.7 * (amp - amp.min()) / (amp.max() - amp.min())
+    O = np.asfortranarray((amp * np.exp(1j * rng.uniform(-0.5, 0.5, (Nr, Nc)))).astype(np.complex64))
+    yy = (np.arange(nr) - nr // 2)[:, None] / (0.4 * nr)
+    xx = (np.arange(nc) - nc // 2)[None, :] / (0.4 * nc)
+    rho2 = yy ** 2 + xx ** 2
+    A = np.asfortranarray((rho2 <= 1.0).astype(np.float32))
+    P = np.asfortranarray((A * np.exp(0.5j * np.sqrt(3) * (2 * rho2 - 1))).astype(np.complex64))
+    c_r, c_c = Nr // 2 - nr // 2 + 1, Nc // 2 - nc // 2 + 1
+    led = np.asfortranarray(np.stack([c_c + rng.integers(-nc // 2, nc // 2 + 1, L),
+                                      c_r + rng.integers(-nr // 2, nr // 2 + 1, L)]).astype(np.int32))
+    led[:, 0] = (c_c, c_r)
+    model = synth.image_model_numpy(O.astype(np.complex128), P.astype(np.complex128), led, r)
+    Y = np.empty((nr, nc, L), dtype=np.float32, order="F")
+    for l in range(L):
+        Y[:, :, l] = (model[:, :, l] - margin_field((nr, nc), rng)).astype(np.float32)
+    return O, P, Y, led, A
+
+
+@pytest.mark.parametrize("nr,nc", [(48, 96), (96, 48)])
+def test_generic_non_square_gradient_and_epochs_match_fp64(nr, nc):
+    """48 x 96 -> 192 x 384 and its transpose (the generic path; crop,
+    stitch and pupil with N_r != N_c), 12 LEDs: gradient within 1e-5 of
+    FP64; 2 epochs in batches of 5 (ragged last batch) within 1e-5."""
+    r, L = 4, 12
+    O, P, Y, led, A = _rect_margin_problem(nr, nc, r, L)
+    GO, GP, loss, margin = oracle.gradient(O, P, Y, led, r)
+    assert margin > 5e-3
+    gO, gP, gl = fpm_amd.fused_gradient(O, P, Y, led, r)
+    print(f"[generic-rect {nr}x{nc}] gradient G_O {rel_l2(gO, GO):.2e} G_P {rel_l2(gP, GP):.2e}", flush=True)
+    assert rel_l2(gO, GO) <= 1e-5 and rel_l2(gP, GP) <= 1e-5
+    assert abs(gl - loss) <= 1e-5 * loss
+    opt = (2e-4, 0.999, 1e-4)
+    O1, P1, lg = fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, r, 2, 5, opt, loss=True)
+    Oo, Po, lo = oracle.solve(O, P, Y, led, A, r, 2, 5, opt, prec="f64")
+    print(f"[generic-rect {nr}x{nc}] 2 epochs O {rel_l2(O1, Oo):.2e} P {rel_l2(P1, Po):.2e}", flush=True)
+    assert rel_l2(O1, Oo) < 1e-5 and rel_l2(P1, Po) < 1e-5
+    np.testing.assert_allclose(lg, lo, rtol=1e-5)

@@ -10,12 +10,16 @@ mex call is timed around cuFPM_pure_v3 (README.md:36 quotes the whole call).
 
 Also reported:
   stages     per-stage device time (HIP events on the solve's stream, inside
-             the timed region) and GB/s against the dense algorithmic byte
-             model (per step 104 N^2, per LED 138 n^2 with uint16 Y);
-  roofline   the dominant stage against MI355X's 8 TB/s HBM peak, and its
-             measured HBM traffic against a plain device copy timed here;
-  cpu_baseline the FP32 CPU restatement (oracle) on the host cores, one
-             minibatch step of the same workload scaled to s/epoch.
+             the timed region), GB/s against the dense byte model of the
+             unfused steps (per step 104 N^2, per LED 138 n^2 with uint16 Y)
+             and, for the fused kernels, against the bytes they must move;
+  roofline   the dominant kernel: the bytes it must move per launch over its
+             event-timed duration, against MI355X's 8 TB/s HBM peak and a
+             plain device copy timed here, and its measured HBM traffic
+             (rocprofv3 counters, copy-calibrated, from profiles/);
+  cpu_baseline the FP32 CPU restatement (oracle, OpenMP) on the host cores:
+             one epoch of the same workload, plus the 128^2 -> 512^2 case over
+             20 epochs and the 512^2 -> 4096^2 case over 2 epochs.
 """
 import argparse
 import json
@@ -63,8 +67,37 @@ def stage_alg_bytes(n, N, b_avg, y_bytes, stitch):
     return d
 
 
+def window_rows_per_step(led, n, N, batch):
+    """Mean number of object-spectrum rows that the LED windows of a
+    minibatch cover (union over its LEDs of rows (kt - 1 + N/2 + i) mod N,
+    i < n, cuFPM_v3/cuda/kernels.cuh:43-47): outside them the step's
+    spectrum gradient is zero by construction, so a kernel that reads the
+    gradient needs only these rows."""
+    import numpy as np
+
+    L = led.shape[1]
+    rows = []
+    for c in range(0, L, batch):
+        hit = np.zeros(N, dtype=bool)
+        for kt in led[1, c:c + batch]:
+            hit[(int(kt) - 1 + N // 2 + np.arange(n)) % N] = True
+        rows.append(int(hit.sum()))
+    return float(np.mean(rows))
+
+
+def stage_necessary_bytes(n, N, b_avg, y_bytes, win_rows):
+    """HBM bytes the fused kernels of a stage must move at least (their own
+    inputs and outputs once each; no credit for the stages they absorb):
+    fused_update reads O and m_O and the gradient rows inside the step's
+    windows and writes O, m_O and the next spectrum: 32 N^2 + 8 N rows; the
+    fused stencil reads the inverse-transformed pages' inputs and Y and
+    writes the pages: (8 + 8 + y) b n^2."""
+    NN, nn = float(N) * N, float(n) * n
+    return {"fused_update": 32 * NN + 8.0 * N * win_rows, "stencil": (16 + y_bytes) * b_avg * nn}
+
+
 # stage -> kernel whose per-dispatch HBM traffic stands for it in the PMC summary
-STAGE_KERNEL = {"fused_update": "fused_update_kernel<16384, true, 0>"}
+STAGE_KERNEL = {"fused_update": "fused_update_kernel<16384, true, 0>", "stencil": "stencil_fused_kernel<2048, 0>"}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_fpm_solve_16384_b26.md")
 
 
@@ -102,31 +135,74 @@ def measured_traffic(stage, n, N):
     return None
 
 
-def run_cpu_baseline(ds, args):
-    """FP32 oracle (OpenMP) on one minibatch step of the same workload."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_problem(ds, count=None):
+    """(O0, P0, Y, led, A) on the host from a synth.make_device dict: Y the
+    first `count` images (all if None) prepared as the solve prepares them."""
     import numpy as np
+
+    L = ds["L"] if count is None else count
+    raw = ds["raw_dev"][:L].cpu().numpy()  # (L, n, n) row-major = column-major pages
+    mn, mx = ds["raw_min"], ds["raw_max"]
+    Y = np.sqrt((raw.astype(np.float32) - np.float32(mn)) / np.float32(mx - mn)).astype(np.float32)
+    del raw
+    Y = np.asfortranarray(Y.transpose(2, 1, 0))
+    O0 = np.asfortranarray(ds["O0_dev"].cpu().numpy().T)
+    P0 = np.asfortranarray(ds["P0_dev"].cpu().numpy().T)
+    return O0, P0, Y, np.asfortranarray(ds["led"][:, :L]), np.asfortranarray(ds["A"])
+
+
+def run_cpu_baseline(ds, args):
+    """The FP32 CPU restatement (oracle, OpenMP over the host cores), timed
+    as the plan of the CPU baseline asks: one full epoch of the benchmark's
+    workload (all images, every minibatch step: the reported value), the
+    128^2 x 361 -> 512^2 case over its 20 epochs and the 512^2 x 361 ->
+    4096^2 case over 2 epochs (same generator, batch, optimizer)."""
     import torch
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    n, N, B = ds["n"], ds["N"], args.batch
-    raw = ds["raw_dev"][:B].cpu().numpy()  # (B, n, n) row-major = column-major pages
-    mn, mx = ds["raw_min"], ds["raw_max"]
-    Y = np.sqrt((raw.astype(np.float32) - np.float32(mn)) / np.float32(mx - mn)).astype(np.float32)
-    Y = np.asfortranarray(Y.transpose(2, 1, 0))
-    O0 = np.asfortranarray(ds["O0_dev"].cpu().numpy().T)
-    P0 = np.asfortranarray(ds["P0_dev"].cpu().numpy().T)
-    led = np.asfortranarray(ds["led"][:, :B])
-    A = np.asfortranarray(ds["A"])
+    from fpm_amd import synth
+
     oracle.lib()
+    opt = (args.lr, 0.999, 1e-4)
+    B = args.batch
+    n, N = ds["n"], ds["N"]
+    O0, P0, Y, led, A = _host_problem(ds)
     t0 = time.perf_counter()
-    oracle.solve(O0, P0, Y, led, A, ds["pratio"], 1, B, (args.lr, 0.999, 1e-4), prec="f32", max_steps=1)
-    dt = time.perf_counter() - t0
-    steps = (ds["L"] + B - 1) // B
-    return {"value": dt * steps, "unit": "s/epoch", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"one minibatch step ({B} of {ds['L']} images, FP32 C restatement, OpenMP) of the "
-                      f"{n}^2 x {ds['L']} -> {N}^2 workload, timed {dt:.2f} s, scaled x{steps} steps/epoch"}
+    oracle.solve(O0, P0, Y, led, A, ds["pratio"], 1, B, opt, prec="f32")
+    t_epoch = time.perf_counter() - t0
+    del O0, P0, Y
+    print(f"[bench] cpu baseline: one epoch of {n}^2 -> {N}^2 in {t_epoch:.1f} s", file=sys.stderr, flush=True)
+    plan = {}
+    for (sn, sr, epochs) in ((128, 4, 20), (512, 8, 2)):
+        small = synth.make_device(sn, sr, device="cuda:0")
+        hp = _host_problem(small)
+        del small
+        torch.cuda.empty_cache()
+        t0 = time.perf_counter()
+        oracle.solve(*hp[:4], hp[4], sr, epochs, B, opt, prec="f32")
+        dt = time.perf_counter() - t0
+        plan[f"{sn}^2x361->{sn * sr}^2"] = {"epochs": epochs, "s_total": round(dt, 3),
+                                            "s_per_epoch": round(dt / epochs, 4)}
+        print(f"[bench] cpu baseline: {sn}^2 -> {sn * sr}^2, {epochs} epochs in {dt:.1f} s", file=sys.stderr,
+              flush=True)
+    return {"value": t_epoch, "unit": "s/epoch", "cores": oracle.num_threads(), "kind": "port",
+            "cpu_model": _cpu_model(),
+            "sample": f"one full epoch ({(ds['L'] + B - 1) // B} minibatch steps over all {ds['L']} images) of the "
+                      f"{n}^2 x {ds['L']} -> {N}^2 workload, FP32 C restatement of the reference's solve, OpenMP "
+                      f"on {oracle.num_threads()} host threads",
+            "plan": plan}
 
 
 TTL_TEST_SEEDS = tuple(range(77000, 77008))
@@ -250,13 +326,13 @@ def run_tiles(args):
             "grouped": {"s_per_epoch": t_joint / ep, "fixed_streams_s_per_epoch": t_fixed / ep,
                         "streams_chosen": rep["streams"], "autotune_epoch_ms": rep["candidates"],
                         "sequences": rep["sequences"], "grouped_tiles": rep["grouped_tiles"],
-                        "epoch_model_GBps": epoch_bytes / 1e9 / (t_fixed / ep)},
+                        "epoch_dense_model_GBps": epoch_bytes / 1e9 / (t_fixed / ep)},
             "per_tile": {"s_per_epoch": t_pt / ep, "fixed_streams_s_per_epoch": t_pt_fixed / ep,
                          "streams_chosen": rep_pt["streams"], "autotune_epoch_ms": rep_pt["candidates"],
                          "sequences": rep_pt["sequences"]},
             "grouped_split_engine_s_per_epoch": t_split / ep, "split_streams_chosen": rep_s["streams"],
             "sequential_s_per_epoch": t_seq / ep, "speedup_grouped_vs_sequential": t_seq / t_joint,
-            "speedup_grouped_vs_per_tile": t_pt / t_joint, "epoch_model_GB": epoch_bytes / 1e9,
+            "speedup_grouped_vs_per_tile": t_pt / t_joint, "epoch_dense_model_GB": epoch_bytes / 1e9,
             "generate_s": t_gen,
             "note": "sides that are not powers of two run the generic path (mixed-radix transforms, unfused "
                     "kernels, grouped like the others); the split engine applies to power-of-two sides"}
@@ -403,8 +479,8 @@ def main():
                    "fft_engine": "split-fp16 MFMA (hi+lo, fp32 accumulate)" if args.engine == "split" else "fp32",
                    "solve_wall_s": t1 - t0,
                    "reference_wall_s_rtx3090_full_solve": 22.31},
-        "epoch_model_GB": epoch_bytes / 1e9,
-        "epoch_model_GBps": epoch_bytes / 1e9 / sec_per_epoch,
+        "epoch_dense_model_GB": epoch_bytes / 1e9,
+        "epoch_dense_model_GBps": epoch_bytes / 1e9 / sec_per_epoch,
     }
     if st is not None:
         alg = stage_alg_bytes(n, N, b_avg, y_bytes, args.stitch)
@@ -414,6 +490,8 @@ def main():
         if int(st.count[STAGE_NAMES.index("ifft_n_contig")]) == 0:
             # the contiguous passes run inside the stencil kernel
             alg["stencil"] += alg["ifft_n_contig"] + alg["fft_n_contig"]
+        win = window_rows_per_step(ds["led"], n, N, args.batch)
+        need = stage_necessary_bytes(n, N, b_avg, y_bytes, win)
         stages = {}
         for k, name in enumerate(STAGE_NAMES):
             cnt = int(st.count[k])
@@ -422,20 +500,29 @@ def main():
             ms_exec = st.ms[k] / cnt
             stages[name] = {"ms_total": round(st.ms[k], 4), "execs": cnt, "launches": int(st.launches[k]),
                             "ms_per_exec": round(ms_exec, 5),
-                            "alg_GBps": round(alg[name] / 1e9 / (ms_exec / 1e3), 1)}
+                            "dense_model_GBps": round(alg[name] / 1e9 / (ms_exec / 1e3), 1)}
+            if name in need:
+                stages[name]["necessary_GBps"] = round(need[name] / 1e9 / (ms_exec / 1e3), 1)
         out["stages"] = stages
         dom = max(stages, key=lambda s: stages[s]["ms_total"])
-        ach = stages[dom]["alg_GBps"]
+        ms_dom = stages[dom]["ms_per_exec"]
+        # achieved: the bytes the dominant kernel must move per launch (no
+        # credit for absorbed stages) over its event-timed launch duration
+        ach = need[dom] / 1e9 / (ms_dom / 1e3) if dom in need else None
         traffic = measured_traffic(dom, n, N)
-        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                           "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                           "alg_bytes_per_exec": alg[dom], "ms_per_exec": stages[dom]["ms_per_exec"]}
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1) if ach else None,
+                           "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBPS, 4) if ach else None, "traffic": traffic,
+                           "necessary_bytes_per_exec": need.get(dom), "window_rows_per_step": win,
+                           "ms_per_exec": ms_dom, "dense_model_bytes_per_exec": alg[dom],
+                           "dense_model_GBps": stages[dom]["dense_model_GBps"]}
         if traffic:
-            out["roofline"]["traffic_GBps"] = round(traffic / 1e9 / (stages[dom]["ms_per_exec"] / 1e3), 1)
-            # the measured denominator beside the spec peak: a plain device copy
-            cp = copy_bandwidth_gbps()
-            out["roofline"]["copy_GBps"] = round(cp, 1)
-            out["roofline"]["traffic_frac_of_copy"] = round(out["roofline"]["traffic_GBps"] / cp, 4)
+            out["roofline"]["traffic_GBps"] = round(traffic / 1e9 / (ms_dom / 1e3), 1)
+        # the measured denominator beside the spec peak: a plain device copy
+        cp = copy_bandwidth_gbps()
+        out["roofline"]["copy_GBps"] = round(cp, 1)
+        if ach:
+            out["roofline"]["frac_of_copy"] = round(ach / cp, 4)
     if args.batch_sweep:
         # the full-scale configuration's batch sizes: same call, same number of epochs, launched
         # without per-stage events (so every size, the bench line's included, is timed alike)
@@ -449,11 +536,11 @@ def main():
             dt = (time.perf_counter() - t0) / args.steps
             spe = (L + bs - 1) // bs
             sweep[str(bs)] = {"s_per_epoch": dt, "steps_per_epoch": spe,
-                              "epoch_model_GBps": (spe * 104.0 * N * N + L * (136.0 + y_bytes) * n * n) / 1e9 / dt}
+                              "epoch_dense_model_GBps": (spe * 104.0 * N * N + L * (136.0 + y_bytes) * n * n) / 1e9 / dt}
             print(f"[bench] batch {bs}: {dt * 1e3:.1f} ms/epoch", file=sys.stderr, flush=True)
         out["batch_sweep"] = sweep
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline (oracle, one minibatch step)", file=sys.stderr, flush=True)
+        print("[bench] cpu baseline (oracle: one epoch, and the two smaller cases)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = run_cpu_baseline(ds, args)
     if not (args.no_time_to_loss and args.no_tiles):
         del ds, Yd, Od, Pd

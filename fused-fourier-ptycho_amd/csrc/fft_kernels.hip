@@ -702,13 +702,17 @@ template <int LEN> struct StencilCfg {
 // is interior and its neighbours are taken cyclically from the workgroup
 // (one workgroup per page instead of two with 2 interior columns in the
 // second: n = 128).
-template <int LEN, int ENG, bool WHOLE>
+// SIGNS: also write the two sign decisions of every interior pixel to
+// signs (natural column-major pages: bit 0 = sign bit of er - e0, bit 1 =
+// of ed - e0), a separate instantiation so the solve's kernels are unchanged.
+template <int LEN, int ENG, bool WHOLE, bool SIGNS>
 __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,
                                                             float inv_ratio, float gscale,
                                                             double* __restrict__ loss_acc,
                                                             const uint8_t* __restrict__ rowmask, Twiddles tw,
-                                                            const TileIn* __restrict__ tab, uint32_t ppt) {
+                                                            const TileIn* __restrict__ tab, uint32_t ppt,
+                                                            uint8_t* __restrict__ signs) {
     using Cfg = StencilCfg<LEN>;
     using Map = typename Cfg::Map;
     using SI = Slots<LEN, true, Cfg::T, Map, ENG>;
@@ -798,6 +802,9 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
                 const float gval = (copysignf(1.0f, e0 - el) - copysignf(1.0f, er - e0) + copysignf(1.0f, e0 - eu) -
                                     copysignf(1.0f, ed - e0)) * gscale;
                 loss32 += fabsf(er - e0) + fabsf(ed - e0);
+                if constexpr (SIGNS)
+                    signs[(size_t)page * plane + (size_t)column(line) * LEN + idx] =
+                        (uint8_t)((signbit(er - e0) ? 1 : 0) | (signbit(ed - e0) ? 2 : 0));
                 const float2 z = v[q * SI::GSL + s];
                 const float mag = FPM_STENCIL_MAG_LDS ? mf[line * LEN + idx] : sqrtf(z.x * z.x + z.y * z.y);
                 if (mag > 0.0f) {
@@ -834,10 +841,11 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
 
 bool stencil_fused_supported(const Geom& g) { return g.nr <= 2048; }
 
-template <int ENG>
+template <int ENG, bool SIGNS>
 static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b,
                                      int pratio, float2* u_out, double* loss_acc, const uint8_t* rowmask,
-                                     const float2* twid, hipStream_t st, const TileIn* tab, uint32_t ppt) {
+                                     const float2* twid, hipStream_t st, const TileIn* tab, uint32_t ppt,
+                                     uint8_t* signs) {
     Twiddles tw{twid};
     const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
     const float gscale = (float)(pratio * pratio);
@@ -845,12 +853,14 @@ static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const 
 #define FPM_STF_CASE(L)                                                                                   \
     case L: {                                                                                             \
         if (g.nc == (uint32_t)StencilCfg<L>::LT) {                                                        \
-            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, true>), dim3(1, b), dim3(StencilCfg<L>::T), 0, st, z_cols, \
-                               u_out, g, y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt); \
+            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, true, SIGNS>), dim3(1, b), dim3(StencilCfg<L>::T), 0, st, \
+                               z_cols, u_out, g, y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt, \
+                               signs);                                                                    \
         } else {                                                                                          \
             dim3 grid((g.nc + StencilCfg<L>::K - 1) / StencilCfg<L>::K, b);                               \
-            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, false>), grid, dim3(StencilCfg<L>::T), 0, st, z_cols, \
-                               u_out, g, y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt); \
+            hipLaunchKernelGGL((stencil_fused_kernel<L, ENG, false, SIGNS>), grid, dim3(StencilCfg<L>::T), 0, st, \
+                               z_cols, u_out, g, y, led0, ratio, inv_ratio, gscale, loss_acc, rowmask, tw, tab, ppt, \
+                               signs);                                                                    \
         }                                                                                                 \
         break;                                                                                            \
     }
@@ -871,15 +881,26 @@ static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const 
 
 void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
                           float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
-                          hipStream_t st, int eng, const TileIn* tab, uint32_t ppt) {
+                          hipStream_t st, int eng, const TileIn* tab, uint32_t ppt, uint8_t* signs) {
     if (tab && (ppt == 0 || b % ppt != 0))
         throw HipError{hipErrorInvalidValue, "grouped stencil: the batch must be a multiple of the pages per tile"};
+    if (tab && signs) throw HipError{hipErrorInvalidValue, "fused stencil: sign output is not available for tiles"};
     if (b > 65535) throw HipError{hipErrorInvalidValue, "fused stencil: at most 65535 pages per launch"};
-    if (eng == ENG_SPLIT16)
-        launch_stencil_fused_eng<ENG_SPLIT16>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st, tab,
-                                              ppt);
-    else
-        launch_stencil_fused_eng<ENG_F32>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st, tab, ppt);
+    if (eng == ENG_SPLIT16) {
+        if (signs)
+            launch_stencil_fused_eng<ENG_SPLIT16, true>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid,
+                                                        st, tab, ppt, signs);
+        else
+            launch_stencil_fused_eng<ENG_SPLIT16, false>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid,
+                                                         st, tab, ppt, nullptr);
+    } else {
+        if (signs)
+            launch_stencil_fused_eng<ENG_F32, true>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st,
+                                                    tab, ppt, signs);
+        else
+            launch_stencil_fused_eng<ENG_F32, false>(g, z_cols, y, led0, b, pratio, u_out, loss_acc, rowmask, twid, st,
+                                                     tab, ppt, nullptr);
+    }
 }
 
 }  // namespace fpm

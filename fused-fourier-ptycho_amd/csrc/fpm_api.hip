@@ -128,7 +128,7 @@ void check_device() {
 void check_side(int64_t v, const char* what) {
     if (!((is_pow2(v) && v >= 16 && v <= 16384) || generic_side(v)))
         invalid(std::string(what) + " = " + std::to_string(v) +
-                    ": every image side must be a power of two in [16, 16384] or an even 2^a 3^b in [16, 4096]",
+                    ": every image side must be even and in [16, 16384]",
                 FPM_E_UNSUPPORTED);
 }
 
@@ -143,7 +143,7 @@ Geom make_geom(int64_t Nr, int64_t Nc, int64_t nr, int64_t nc) {
         // a side that is not a power of two: the generic path for all four
         for (int64_t v : {Nr, Nc, nr, nc})
             if (!generic_side(v))
-                invalid("with a side that is not a power of two every side must be an even 2^a 3^b in [16, 4096]",
+                invalid("with a side that is not a power of two every side must be even and in [16, 16384]",
                         FPM_E_UNSUPPORTED);
         g.nr = (uint32_t)nr;
         g.nc = (uint32_t)nc;
@@ -209,6 +209,18 @@ YStack prepare_y(const void* Y, int32_t y_dtype, size_t count, bool is_device, D
     y.range = (float)(h[1] - h[0]);
     y.rcp_range = 1.0f / y.range;
     return y;
+}
+
+// the generic path (sides that are not powers of two) runs FP32 mixed-radix
+// transforms and the owner-computes stitch only
+void check_generic_options(const Geom& g, int eng, int32_t stitch) {
+    if (!g.generic) return;
+    if (eng == ENG_SPLIT16)
+        invalid("the split-FP16 engine needs power-of-two sides (the generic path's transforms are FP32)",
+                FPM_E_UNSUPPORTED);
+    if (stitch == FPM_STITCH_ATOMIC)
+        invalid("the atomic stitch needs power-of-two sides (the generic path stitches owner-computes)",
+                FPM_E_UNSUPPORTED);
 }
 
 int engine_of(int32_t fft_engine) {
@@ -355,7 +367,7 @@ struct Workspace {
 void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2* led, uint32_t led0, uint32_t b,
                         const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
                         RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
-                        StageTimer& tm, const uint8_t* rowmask, const uint8_t* pmask) {
+                        StageTimer& tm, const uint8_t* rowmask, const uint8_t* pmask, uint8_t* signs = nullptr) {
     float2* lat = ws.lat.as<float2>();
     float2* rec = ws.rec.as<float2>();
     const bool fused = stencil_fused_supported(g);
@@ -372,14 +384,14 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     if (fused) {
         // contiguous inverse pass + stencil + contiguous forward pass in one kernel
         tm.begin(FPM_STAGE_STENCIL);
-        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, lat_mask, tw, st, g.eng);
+        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, lat_mask, tw, st, g.eng, nullptr, 0, signs);
         tm.end(1);
     } else {
         tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
         fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st, g.eng);
         tm.end(1);
         tm.begin(FPM_STAGE_STENCIL);
-        launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st);
+        launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st, signs);
         tm.end(1);
         tm.begin(FPM_STAGE_FFT_N_CONTIG_LO);
         fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st, g.eng);
@@ -506,7 +518,7 @@ struct Solve {
     // captured and replayed.
     void enqueue_epoch(int cur, bool last_epoch, hipStream_t st, StageTimer& tm) {
         if (g.generic) {
-            enqueue_epoch_generic(st);
+            enqueue_epoch_generic(st, tm);
             return;
         }
         uint32_t led0, b;
@@ -563,7 +575,12 @@ struct Solve {
         generic_prepare(g);
     }
 
-    void enqueue_epoch_generic(hipStream_t st) {
+    // Stage events: each whole 2D transform is recorded under its column-pass
+    // stage (FFT_N_COLS_HI: DFT_N(O); IFFT_N_COLS_LO / FFT_N_COLS_LO: the
+    // per-LED transforms; IFFT_N_COLS_HI: IDFT_N of the gradient), the
+    // residual + stencil under STENCIL and the RMSprop of O under
+    // FUSED_UPDATE.
+    void enqueue_epoch_generic(hipStream_t st, StageTimer& tm) {
         const size_t NN = (size_t)g.Nr * g.Nc;
         float2* S = S0.as<float2>();
         float2* G = S1.as<float2>();
@@ -572,15 +589,33 @@ struct Solve {
         uint32_t led0, b;
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
+            tm.begin(FPM_STAGE_FFT_N_COLS_HI);
             fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
+            tm.end(2);
+            tm.begin(FPM_STAGE_CROP);
             generic_crop(S, P, led, led0, b, g, lat, st);
+            tm.end(1);
+            tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
             fft2_generic(lat, lat, g.nr, g.nc, b, true, st);
+            tm.end(2);
+            tm.begin(FPM_STAGE_STENCIL);
             generic_stencil(lat, y, led0, b, pr, g, ebuf.as<float>(), rec, ss.loss_cur, st);
+            tm.end(2);
+            tm.begin(FPM_STAGE_FFT_N_COLS_LO);
             fft2_generic(rec, rec, g.nr, g.nc, b, false, st);
+            tm.end(2);
+            tm.begin(FPM_STAGE_STITCH_OWNER);
             generic_stitch(rec, P, led, led0, b, g, G, st);  // reads this step's pupil
+            tm.end(1);
+            tm.begin(FPM_STAGE_BACKPROJ);
             generic_pupil(rec, S, led, led0, b, g, P, mP.as<float>(), A, rms, nullptr, st);
+            tm.end(1);
+            tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
             fft2_generic(G, G, g.Nr, g.Nc, 1, true, st);
+            tm.end(2);
+            tm.begin(FPM_STAGE_FUSED_UPDATE);
             generic_update(G, O, mO.as<float>(), NN, rms, st);
+            tm.end(1);
         }
         launch_epoch_end(ss, st);
     }
@@ -835,6 +870,7 @@ int fpm_pure_v3(const fpm_c32* O0, int64_t N_rows, int64_t N_cols, const fpm_c32
         const float lr = (float)opt[0];
         Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
         g.eng = engine_of(o.fft_engine);
+        check_generic_options(g, g.eng, o.stitch);
         const float beta = (float)opt[1], eps = (float)opt[2];
         if (o.verbose) {
             std::printf("prhs4: %d, prhs5: %d, prhs6: %d \n", pr, ne, bs);
@@ -1191,10 +1227,11 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         if (!O_in || !P_in || !Y || !led_idx || !G_O || !G_P) invalid("null array argument");
         Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
         g.eng = engine_of(o.fft_engine);
+        if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
+        check_generic_options(g, g.eng, o.stitch);
         if (L < 1) invalid("L (number of images) must be >= 1");
         const int pr = (int)pratio;
         if (pr < 1) invalid("pratio must be >= 1");
-        if (o.stitch != FPM_STITCH_ATOMIC && o.stitch != FPM_STITCH_OWNER) invalid("unknown stitch mode");
         check_device();
         int dev = 0;
         FPM_HIP(hipGetDevice(&dev));
@@ -1235,6 +1272,17 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             gout.alloc(NN * sizeof(float2));
             GOd = gout.as<float2>();
         }
+        // the loss stencil's sign decisions (optional diagnostic output)
+        DevBuf sgn;
+        uint8_t* signs = nullptr;
+        if (o.stencil_signs) {
+            if (isdev) {
+                signs = o.stencil_signs;
+            } else {
+                sgn.alloc(nn * (size_t)L);
+                signs = sgn.as<uint8_t>();
+            }
+        }
         if (g.generic) {
             // generic sides (generic.hip): FP32 mixed-radix transforms, owner stitch
             DevBuf eb;
@@ -1247,7 +1295,8 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
             generic_crop(S, pcopy.as<float2>(), led, 0, b, g, lat, st);
             fft2_generic(lat, lat, g.nr, g.nc, b, true, st);
-            generic_stencil(lat, y, 0, b, pr, g, eb.as<float>(), rec, loss_out ? lossd.as<double>() : nullptr, st);
+            generic_stencil(lat, y, 0, b, pr, g, eb.as<float>(), rec, loss_out ? lossd.as<double>() : nullptr, st,
+                            nullptr, 1, signs);
             fft2_generic(rec, rec, g.nr, g.nc, b, false, st);
             generic_stitch(rec, pcopy.as<float2>(), led, 0, b, g, S1.as<float2>(), st);
             generic_pupil(rec, S, led, 0, b, g, pcopy.as<float2>(), nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f},
@@ -1260,7 +1309,7 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             tm.end(1 + col_launches(g.pN));
             minibatch_backward(g, S0.as<float2>(), pcopy.as<float2>(), led, 0, (uint32_t)L, y, pr, ws, S1.as<float2>(),
                                o.stitch, nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, GPd,
-                               loss_out ? lossd.as<double>() : nullptr, tw, st, tm, nullptr, nullptr);
+                               loss_out ? lossd.as<double>() : nullptr, tw, st, tm, nullptr, nullptr, signs);
             fft_contig(S1.as<float2>(), GOd, g.Nr, g.Nc, true, tw, st, g.eng);
         }
         if (!isdev) {
@@ -1268,6 +1317,8 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             FPM_HIP(hipMemcpyAsync(G_P, GPd, nn * sizeof(float2), hipMemcpyDeviceToHost, st));
         }
         if (loss_out) FPM_HIP(hipMemcpyAsync(loss_out, lossd.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        if (signs && !isdev)
+            FPM_HIP(hipMemcpyAsync(o.stencil_signs, signs, nn * (size_t)L, hipMemcpyDeviceToHost, st));
         FPM_HIP(hipStreamSynchronize(st));
         tm.flush();
     });
@@ -1342,6 +1393,9 @@ int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
         check_side(cols, "cols");
         if (batch < 1 || batch > 65535) invalid("batch must be in [1, 65535]");
         if (direction != -1 && direction != 1) invalid("direction must be -1 (forward) or +1 (inverse)");
+        if (eng == ENG_SPLIT16 && !(is_pow2(rows) && is_pow2(cols)))
+            invalid("the split-FP16 engine needs power-of-two sides (the generic transform is FP32)",
+                    FPM_E_UNSUPPORTED);
         check_device();
         int dev = 0;
         FPM_HIP(hipGetDevice(&dev));
@@ -1364,7 +1418,7 @@ int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
         if (!is_pow2(rows) || !is_pow2(cols)) {
             // a side that is not a power of two: the generic mixed-radix transform (FP32)
             if (!generic_side(rows) || !generic_side(cols))
-                invalid("with a side that is not a power of two both sides must be even 2^a 3^b in [16, 4096]",
+                invalid("with a side that is not a power of two both sides must be even and in [16, 16384]",
                         FPM_E_UNSUPPORTED);
             fft2_generic(src, dst, R, C, Bt, direction > 0, st);
         } else if (direction < 0) {

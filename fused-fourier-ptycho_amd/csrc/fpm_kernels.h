@@ -86,8 +86,11 @@ void launch_epoch_end(const SolveState& s, hipStream_t st);
 
 // feature-domain loss gradient (kernels.cuh:197-275): z (b pages, natural) and
 // images led0.. -> u.  loss_acc (device double, may be null) += sum |Dx e|+|Dy e|.
+// signs (optional): one byte per pixel of the b pages (page l pixel (i, j) at
+// l n_r n_c + j n_r + i), bit 0 = sign bit of e(i, j+1) - e(i, j), bit 1 = of
+// e(i+1, j) - e(i, j): the decisions the stencil took
 void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio,
-                    float2* u, double* loss_acc, hipStream_t st);
+                    float2* u, double* loss_acc, hipStream_t st, uint8_t* signs = nullptr);
 
 // back-projection (kernels.cuh:67-129): pupil gradient R conj(S) summed in
 // registers over the batch in LED order (S read back from the spectrum Shat
@@ -118,9 +121,11 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 // k ppt..; its images, normalisation, loss slot from tab[k], its row mask at
 // rowmask + k nr/16.
 bool stencil_fused_supported(const Geom& g);
+// signs (optional, not with tab): the stencil's decisions as launch_stencil writes them
 void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b, int pratio,
                           float2* u_out, double* loss_acc, const uint8_t* rowmask, const float2* twid,
-                          hipStream_t st, int eng = ENG_F32, const TileIn* tab = nullptr, uint32_t ppt = 0);
+                          hipStream_t st, int eng = ENG_F32, const TileIn* tab = nullptr, uint32_t ppt = 0,
+                          uint8_t* signs = nullptr);
 
 // contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
 // [+ contiguous forward pass of the new O, written back into G]

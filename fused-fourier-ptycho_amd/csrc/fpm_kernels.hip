@@ -28,7 +28,8 @@ constexpr int kStTR = 64, kStTC = 32, kStLD = kStTR + 2;
 
 __global__ __launch_bounds__(256) void stencil_kernel(Geom g, const float2* __restrict__ z, YStack y, uint32_t led0,
                                                       float ratio, float inv_ratio, float gscale,
-                                                      float2* __restrict__ u, double* __restrict__ loss_acc) {
+                                                      float2* __restrict__ u, double* __restrict__ loss_acc,
+                                                      uint8_t* __restrict__ signs) {
     __shared__ float es[(kStTC + 2) * kStLD];
     __shared__ double red[4];
     const int t = threadIdx.x;
@@ -85,6 +86,9 @@ __global__ __launch_bounds__(256) void stencil_kernel(Geom g, const float2* __re
                 o = make_float2(zz.x * inv * gval, zz.y * inv * gval);
             }
             u[page + (size_t)(col0 + c) * g.nr + (row0 + r)] = o;
+            if (signs)  // bit 0: sign of er - e0, bit 1: sign of ed - e0 (the copysign decisions above)
+                signs[page + (size_t)(col0 + c) * g.nr + (row0 + r)] =
+                    (uint8_t)((signbit(er - e0) ? 1 : 0) | (signbit(ed - e0) ? 2 : 0));
         }
     }
     if (loss_acc) {
@@ -97,14 +101,14 @@ __global__ __launch_bounds__(256) void stencil_kernel(Geom g, const float2* __re
 }
 
 void launch_stencil(const Geom& g, const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio,
-                    float2* u, double* loss_acc, hipStream_t st) {
+                    float2* u, double* loss_acc, hipStream_t st, uint8_t* signs) {
     // the reference forms both scales in integer arithmetic then casts
     const uint32_t plane = g.plane();
     const float ratio = (float)(plane * (uint32_t)(pratio * pratio));
     const float gscale = (float)(pratio * pratio);
     dim3 grid((g.nr + kStTR - 1) / kStTR, (g.nc + kStTC - 1) / kStTC, b);
     hipLaunchKernelGGL(stencil_kernel, grid, dim3(256), 0, st, g, z, y, led0, ratio, exact_inverse_or_zero(ratio), gscale,
-                       u, loss_acc);
+                       u, loss_acc, signs);
     FPM_LAUNCH_CHECK();
 }
 

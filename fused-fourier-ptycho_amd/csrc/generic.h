@@ -1,17 +1,16 @@
-// generic.h -- the solve's steps for even sides 2^a 3^b that are not powers
-// of two (generic.hip): mixed-radix transforms and unfused kernels in
-// natural order.
+// generic.h -- the solve's steps for even sides that are not powers of two
+// (generic.hip): mixed-radix transforms and unfused kernels in natural order.
 #pragma once
 #include "common.h"
 #include "fpm_kernels.h"
 
 namespace fpm {
 
-// largest side of the generic path (its FFT holds up to 4096 points per
-// workgroup in LDS)
-constexpr int kGenericMaxSide = 4096;
+// largest side of the generic path (its FFT holds a whole line, up to 16384
+// points = 128 KiB, in LDS)
+constexpr int kGenericMaxSide = 16384;
 
-// even v = 2^a 3^b in [16, kGenericMaxSide] (powers of two included)
+// even v in [16, kGenericMaxSide] (powers of two included)
 bool generic_side(int64_t v);
 
 // unnormalised 2D C2C transforms of `batch` column-major rows x cols pages,
@@ -27,10 +26,12 @@ void generic_prepare(const Geom& g);
 // images, mask and loss slot from tab[k]; led / y / A / loss_acc unused then
 void generic_crop(const float2* S, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
                   float2* out, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
-// residual (ebuf: b n_r n_c floats) and loss stencil: u from z; loss_acc may be null
+// residual (ebuf: b n_r n_c floats) and loss stencil: u from z; loss_acc may be null;
+// signs (optional, not with tab): the stencil's decisions, one byte per pixel
+// (bit 0: sign bit of e(i, j+1) - e(i, j), bit 1: of e(i+1, j) - e(i, j))
 void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
                      float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab = nullptr,
-                     uint32_t ntiles = 1);
+                     uint32_t ntiles = 1, uint8_t* signs = nullptr);
 // owner-computes stitch of the b back-projections into G_S (every element written)
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
                     float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);

@@ -1,16 +1,19 @@
-// generic.hip -- the solve for even sides that are not powers of two
-// (2^a 3^b, at most 4096: field-of-view tiles of 192 px -> 768 px, and the
-// like), which the power-of-two kernels cannot take (their index maps are bit
-// masks and their FFT passes radix-16 slot maps).  The steps are those of
+// generic.hip -- the solve for even sides that are not powers of two (up to
+// 16384: field-of-view tiles of 192 px -> 768 px, 200 px -> 1600 px, 1536 px
+// -> 12288 px, and the like -- any even side, as cuFFT takes any length),
+// which the power-of-two kernels cannot take (their index maps are bit masks
+// and their FFT passes radix-16 slot maps).  The steps are those of
 // cuFPM_v3/cuFPM_pure_v3.cu:128-207 without the fusions, in natural order:
 //   S = DFT_N(O); crop x pupil (kernels.cuh:19-65); z = IDFT_n; loss stencil
 //   (kernels.cuh:197-275); R = DFT_n(u); owner-computes stitch and the pupil
 //   gradient / RMSprop + clamp (kernels.cuh:67-129, 384-410);
 //   G_O = IDFT_N(G_S); RMSprop of O (kernels.cuh:364-382).
-// The transforms are a self-sorting (Stockham) mixed-radix FFT with radices
-// 4, 2 and 3: pass (r, n, s) maps
+// The transforms are a self-sorting (Stockham) mixed-radix FFT: radices 4, 2,
+// 3, 5 and 7 with hard-coded butterflies, any other prime factor as a direct
+// sum; pass (r, n, s) maps
 //   y[q + s (r p + k)] = (sum_j x[q + s (p + j n/r)] W_r^(j k)) W_n^(p k),
-// lines staged in LDS, twiddles from an FP64-generated table of W_len^k.
+// lines staged in LDS (each pass in place through registers), twiddles from
+// an FP64-generated table of W_len^k.
 #include "../../include/fpm_amd.h"
 #include "common.h"
 #include "fpm_kernels.h"
@@ -28,15 +31,16 @@
 #ifndef FPM_GEN_STRIDED_POINTS
 #define FPM_GEN_STRIDED_POINTS 2048
 #endif
+#ifndef FPM_GEN_SET_ATTR
+#define FPM_GEN_SET_ATTR 1
+#endif
+#ifndef FPM_GEN_LB
+#define FPM_GEN_LB 1024
+#endif
 
 namespace fpm {
 
-bool generic_side(int64_t v) {
-    if (v < 16 || v > kGenericMaxSide || (v & 1)) return false;
-    while (v % 2 == 0) v /= 2;
-    while (v % 3 == 0) v /= 3;
-    return v == 1;
-}
+bool generic_side(int64_t v) { return v >= 16 && v <= kGenericMaxSide && (v & 1) == 0; }
 
 namespace {
 
@@ -70,6 +74,11 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t x, const FastDiv& f) {
     return (uint32_t)(((uint64_t)__umulhi(x, f.m) + x) >> f.s);
 }
 
+constexpr int kGenMaxFactors = 32;
+// points of a line one thread holds in a pass at most: the workgroup has at
+// least (lines x length) / kGenPts threads
+constexpr int kGenPts = 16;
+
 struct GenFFT {
     const float2* in;
     float2* out;
@@ -77,15 +86,21 @@ struct GenFFT {
     uint64_t batch_stride, line_stride, es;
     uint32_t len, nlines, lines_per_batch, tl, nf, inverse;
     FastDiv f_lpb, f_tl, f_len;
-    FastDiv f_nb[5];   // len / R for R = 2, 3, 4 (index R)
-    FastDiv f_s[24];   // the stride s of each pass
-    uint8_t fac[24];
+    FastDiv f_nb[kGenMaxFactors];  // len / R of each pass
+    FastDiv f_s[kGenMaxFactors];   // the stride s of each pass
+    FastDiv f_r[kGenMaxFactors];   // R of each pass
+    uint16_t fac[kGenMaxFactors];
 };
 
 // element e of line L at (L / lpb) batch_stride + (L % lpb) line_stride + e es
 __device__ __forceinline__ size_t gen_off(const GenFFT& a, uint32_t L, uint32_t e) {
     const uint32_t pg = fdiv(L, a.f_lpb);
     return (size_t)pg * a.batch_stride + (size_t)(L - pg * a.lines_per_batch) * a.line_stride + (size_t)e * a.es;
+}
+
+// -i v (forward) or +i v (inverse)
+__device__ __forceinline__ float2 mul_mi(float2 v, bool inv) {
+    return inv ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
 }
 
 // radix-R DFT of v in place (forward: W_R = exp(-2 pi i / R); inverse: conjugate)
@@ -101,7 +116,7 @@ template <> __device__ __forceinline__ void small_dft<4>(float2* v, bool inv) {
     const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
     const float2 d13 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
     // forward: X1 = d02 - i d13, X3 = d02 + i d13 (inverse swaps them)
-    const float2 mi = inv ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);  // -+ i d13
+    const float2 mi = mul_mi(d13, inv);
     v[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
     v[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
     v[1] = make_float2(d02.x + mi.x, d02.y + mi.y);
@@ -113,77 +128,177 @@ template <> __device__ __forceinline__ void small_dft<3>(float2* v, bool inv) {
     const float2 t = make_float2(v[1].x + v[2].x, v[1].y + v[2].y);
     const float2 d = make_float2(v[1].x - v[2].x, v[1].y - v[2].y);
     const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
-    const float sg = inv ? -1.0f : 1.0f;
-    const float2 r = make_float2(sg * c3 * d.y, -sg * c3 * d.x);  // -i sg c3 d
+    const float2 r = mul_mi(make_float2(c3 * d.x, c3 * d.y), inv);
     v[0] = make_float2(v[0].x + t.x, v[0].y + t.y);
     v[1] = make_float2(m.x + r.x, m.y + r.y);
     v[2] = make_float2(m.x - r.x, m.y - r.y);
 }
-
+// odd R: pair x_j with x_{R-j}: t_j = x_j + x_{R-j}, d_j = x_j - x_{R-j};
+// X_k = x_0 + sum_j cos(2 pi j k / R) t_j -+ i sum_j sin(2 pi j k / R) d_j,
+// X_{R-k} the same with the sign of the i term flipped (forward: X_k takes -)
 template <int R>
-__device__ __forceinline__ void gen_pass(const GenFFT& a, const float2* x, float2* y, uint32_t nl, uint32_t n,
-                                         uint32_t s, const FastDiv& fs) {
-    const uint32_t m = n / R, nb = a.len / R, step_n = a.len / n;
-    for (uint32_t bi = threadIdx.x; bi < nl * nb; bi += blockDim.x) {
-        const uint32_t l = fdiv(bi, a.f_nb[R]), b = bi - l * nb;
-        const uint32_t p = fdiv(b, fs), q = b - p * s;
-        const float2* xl = x + (size_t)l * a.len;
-        float2* yl = y + (size_t)l * a.len;
-        float2 v[R];
+__device__ __forceinline__ void odd_dft(float2* v, bool inv, const float* c, const float* sn) {
+    constexpr int H = R / 2;
+    float2 t[H], d[H];
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = xl[q + s * (p + j * m)];
-        small_dft<R>(v, a.inverse != 0);
-        yl[q + s * (R * p)] = v[0];
-#pragma unroll
-        for (int k = 1; k < R; ++k) {
-            float2 w = a.tw[p * (uint32_t)k * step_n];  // p k < (n / R) R = n
-            if (a.inverse) w.y = -w.y;
-            yl[q + s * (R * p + k)] = cmul_g(v[k], w);
-        }
+    for (int j = 1; j <= H; ++j) {
+        t[j - 1] = make_float2(v[j].x + v[R - j].x, v[j].y + v[R - j].y);
+        d[j - 1] = make_float2(v[j].x - v[R - j].x, v[j].y - v[R - j].y);
     }
+    float2 x0 = v[0];
+    float2 s0 = x0;
+#pragma unroll
+    for (int j = 0; j < H; ++j) s0 = make_float2(s0.x + t[j].x, s0.y + t[j].y);
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        float2 ak = x0, bk = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const int m = (j * k) % R;  // cos(m th) = c[min(m, R-m)], sin(m th) = +-sn[min(m, R-m)]
+            const int mm = m <= H ? m : R - m;
+            const float cs = c[mm - 1], ss = m <= H ? sn[mm - 1] : -sn[mm - 1];
+            ak = make_float2(ak.x + cs * t[j - 1].x, ak.y + cs * t[j - 1].y);
+            bk = make_float2(bk.x + ss * d[j - 1].x, bk.y + ss * d[j - 1].y);
+        }
+        const float2 ib = mul_mi(bk, inv);
+        v[k] = make_float2(ak.x + ib.x, ak.y + ib.y);
+        v[R - k] = make_float2(ak.x - ib.x, ak.y - ib.y);
+    }
+    v[0] = s0;
+}
+template <> __device__ __forceinline__ void small_dft<5>(float2* v, bool inv) {
+    const float c[2] = {0.30901699437494742410f, -0.80901699437494742410f};  // cos(2 pi k / 5)
+    const float sn[2] = {0.95105651629515357212f, 0.58778525229247312917f};  // sin(2 pi k / 5)
+    odd_dft<5>(v, inv, c, sn);
+}
+template <> __device__ __forceinline__ void small_dft<7>(float2* v, bool inv) {
+    const float c[3] = {0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f};
+    const float sn[3] = {0.78183148246802980871f, 0.97492791218182360702f, 0.43388373911755812048f};
+    odd_dft<7>(v, inv, c, sn);
 }
 
-__global__ __launch_bounds__(256) void fft_generic_kernel(GenFFT a) {
+// One in-place pass (R, n, s) of a line set held in LDS x:
+//   x'[q + s (R p + k)] = (sum_j x[q + s (p + j n/R)] W_R^(j k)) W_n^(p k)
+// every thread computes its butterflies into registers, the workgroup
+// synchronises, then writes them back (one buffer instead of two, so a
+// 16384-point line fits the LDS)
+template <int R>
+__device__ __forceinline__ void gen_pass(const GenFFT& a, float2* x, uint32_t nl, uint32_t n, uint32_t s,
+                                         const FastDiv& fs, const FastDiv& fnb) {
+    constexpr int IT = (kGenPts + R - 1) / R;
+    const uint32_t m = n / R, nb = a.len / R, step_n = a.len / n, items = nl * nb;
+    float2 res[IT][R];
+    uint32_t dst[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const uint32_t bi = threadIdx.x + (uint32_t)it * blockDim.x;
+        dst[it] = 0xffffffffu;
+        if (bi < items) {
+            const uint32_t l = fdiv(bi, fnb), b = bi - l * nb;
+            const uint32_t p = fdiv(b, fs), q = b - p * s;
+            const float2* xl = x + (size_t)l * a.len;
+            float2 v[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[j] = xl[q + s * (p + j * m)];
+            small_dft<R>(v, a.inverse != 0);
+            res[it][0] = v[0];
+#pragma unroll
+            for (int k = 1; k < R; ++k) {
+                float2 w = a.tw[p * (uint32_t)k * step_n];  // p k < (n / R) R = n
+                if (a.inverse) w.y = -w.y;
+                res[it][k] = cmul_g(v[k], w);
+            }
+            dst[it] = l * a.len + q + s * (R * p);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it)
+        if (dst[it] != 0xffffffffu) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) x[dst[it] + s * (uint32_t)k] = res[it][k];
+        }
+    __syncthreads();
+}
+
+// A pass of any other (prime) radix R, one output per work item: X_k of
+// butterfly (p, q) is a direct R-point sum, W_R^(j k) = W_len^((j k mod R)
+// len / R) from the same table (R len / R products per line and pass)
+__device__ __forceinline__ void gen_pass_any(const GenFFT& a, float2* x, uint32_t nl, uint32_t n, uint32_t s, uint32_t R,
+                                          const FastDiv& fs, const FastDiv& fr) {
+    const uint32_t m = n / R, step_n = a.len / n, step_r = a.len / R, items = nl * a.len;
+    float2 res[kGenPts];
+    uint32_t dst[kGenPts];
+#pragma unroll
+    for (int it = 0; it < kGenPts; ++it) {
+        const uint32_t idx = threadIdx.x + (uint32_t)it * blockDim.x;
+        dst[it] = 0xffffffffu;
+        if (idx < items) {
+            const uint32_t l = fdiv(idx, a.f_len), e = idx - l * a.len;
+            const uint32_t b = fdiv(e, fr), k = e - b * R;  // butterfly b, output k
+            const uint32_t p = fdiv(b, fs), q = b - p * s;
+            const float2* xl = x + (size_t)l * a.len + q + s * p;
+            float2 acc = make_float2(0.f, 0.f);
+            uint32_t jk = 0;
+            for (uint32_t j = 0; j < R; ++j) {
+                float2 w = a.tw[jk * step_r];
+                if (a.inverse) w.y = -w.y;
+                const float2 t = cmul_g(xl[s * j * m], w);
+                acc = make_float2(acc.x + t.x, acc.y + t.y);
+                jk += k;
+                jk = jk >= R ? jk - R : jk;
+            }
+            float2 w = a.tw[p * k * step_n];
+            if (a.inverse) w.y = -w.y;
+            res[it] = cmul_g(acc, w);
+            dst[it] = l * a.len + q + s * (R * p + k);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kGenPts; ++it)
+        if (dst[it] != 0xffffffffu) x[dst[it]] = res[it];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(FPM_GEN_LB) void fft_generic_kernel(GenFFT a) {
     extern __shared__ float2 sm[];
     float2* x = sm;
-    float2* y = sm + (size_t)a.tl * a.len;
     const uint32_t L0 = blockIdx.x * a.tl;
     const uint32_t nl = min(a.tl, a.nlines - L0);
     const uint32_t tot = nl * a.len;
     const bool line_fast = a.es != 1;  // strided lines: adjacent lines are adjacent in memory
     // (line, element) of a load/store index: line fastest for strided lines
-    auto split = [&](uint32_t idx, uint32_t& l, uint32_t& e) {
+    auto split = [&](uint32_t idx) -> uint2 {
         if (line_fast) {
-            e = nl == a.tl ? fdiv(idx, a.f_tl) : idx / nl;
-            l = idx - e * nl;
-        } else {
-            l = fdiv(idx, a.f_len);
-            e = idx - l * a.len;
+            const uint32_t e = nl == a.tl ? fdiv(idx, a.f_tl) : idx / nl;
+            return make_uint2(idx - e * nl, e);
         }
+        const uint32_t l = fdiv(idx, a.f_len);
+        return make_uint2(l, idx - l * a.len);
     };
     for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-        uint32_t l, e;
-        split(idx, l, e);
-        x[(size_t)l * a.len + e] = a.in[gen_off(a, L0 + l, e)];
+        const uint2 le = split(idx);
+        x[(size_t)le.x * a.len + le.y] = a.in[gen_off(a, L0 + le.x, le.y)];
     }
     __syncthreads();
     uint32_t n = a.len, s = 1;
     for (uint32_t f = 0; f < a.nf; ++f) {
         const uint32_t r = a.fac[f];
-        if (r == 4) gen_pass<4>(a, x, y, nl, n, s, a.f_s[f]);
-        else if (r == 2) gen_pass<2>(a, x, y, nl, n, s, a.f_s[f]);
-        else gen_pass<3>(a, x, y, nl, n, s, a.f_s[f]);
-        __syncthreads();
-        float2* t = x;
-        x = y;
-        y = t;
+        switch (r) {
+            case 4: gen_pass<4>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
+            case 2: gen_pass<2>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
+            case 3: gen_pass<3>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
+            case 5: gen_pass<5>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
+            case 7: gen_pass<7>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
+            default: gen_pass_any(a, x, nl, n, s, r, a.f_s[f], a.f_r[f]); break;
+        }
         n /= r;
         s *= r;
     }
     for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-        uint32_t l, e;
-        split(idx, l, e);
-        a.out[gen_off(a, L0 + l, e)] = x[(size_t)l * a.len + e];
+        const uint2 le = split(idx);
+        a.out[gen_off(a, L0 + le.x, le.y)] = x[(size_t)le.x * a.len + le.y];
     }
 }
 
@@ -211,11 +326,21 @@ const float2* generic_twiddles(uint32_t len) {
 void set_factors(GenFFT& a, uint32_t len) {
     uint32_t m = len;
     a.nf = 0;
-    while (m % 4 == 0) { a.fac[a.nf++] = 4; m /= 4; }
-    while (m % 2 == 0) { a.fac[a.nf++] = 2; m /= 2; }
-    while (m % 3 == 0) { a.fac[a.nf++] = 3; m /= 3; }
-    if (m != 1 || a.nf > 24) throw HipError{hipErrorInvalidValue, "generic FFT: length " + std::to_string(len) +
-                                                                     " is not 2^a 3^b"};
+    auto take = [&](uint32_t r) {
+        while (m % r == 0) {
+            if (a.nf >= (uint32_t)kGenMaxFactors) throw HipError{hipErrorInvalidValue, "generic FFT: too many factors"};
+            a.fac[a.nf++] = (uint16_t)r;
+            m /= r;
+        }
+    };
+    take(4);
+    take(2);
+    take(3);
+    // 5, 7 hard-coded; other primes summed directly
+    for (uint32_t p = 5; p <= len; p += 2) {
+        if (m == 1) break;
+        take(p);
+    }
 }
 
 void launch_generic_lines(GenFFT a, hipStream_t st) {
@@ -234,16 +359,27 @@ void launch_generic_lines(GenFFT a, hipStream_t st) {
     a.f_lpb = make_fastdiv(a.lines_per_batch);
     a.f_tl = make_fastdiv(a.tl);
     a.f_len = make_fastdiv(a.len);
-    for (uint32_t r : {2u, 3u, 4u})
-        if (a.len % r == 0) a.f_nb[r] = make_fastdiv(a.len / r);
     uint32_t s = 1;
     for (uint32_t f = 0; f < a.nf; ++f) {
         a.f_s[f] = make_fastdiv(s);
+        a.f_nb[f] = make_fastdiv(a.len / a.fac[f]);
+        a.f_r[f] = make_fastdiv(a.fac[f]);
         s *= a.fac[f];
     }
-    const size_t shmem = 2 * (size_t)a.tl * a.len * sizeof(float2);
+    // threads: about 8 points each (at least 256 threads, at most kGenPts
+    // points each), a multiple of the wave
+    const uint32_t pts = a.tl * a.len;
+    const uint32_t threads = std::min<uint32_t>(FPM_GEN_LB, std::max<uint32_t>(256, (pts + 8 * 64 - 1) / (8 * 64) * 64));
+    if ((size_t)threads * kGenPts < pts) throw HipError{hipErrorInvalidValue, "generic FFT: line set too long"};
+    const size_t shmem = (size_t)pts * sizeof(float2);
+    static std::once_flag big_lds;
+    if (FPM_GEN_SET_ATTR) std::call_once(big_lds, [] {
+        // lines up to 16384 points (128 KiB) stage in LDS
+        FPM_HIP(hipFuncSetAttribute((const void*)fft_generic_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(kGenericMaxSide * sizeof(float2))));
+    });
     const uint32_t blocks = (a.nlines + a.tl - 1) / a.tl;
-    hipLaunchKernelGGL(fft_generic_kernel, dim3(blocks), dim3(256), shmem, st, a);
+    hipLaunchKernelGGL(fft_generic_kernel, dim3(blocks), dim3(threads), shmem, st, a);
     FPM_LAUNCH_CHECK();
 }
 
@@ -310,7 +446,7 @@ __global__ __launch_bounds__(256) void gen_residual_kernel(const float2* __restr
 __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restrict__ z, const float* __restrict__ e,
                                                           uint32_t nr, uint32_t nc, size_t total, float gscale,
                                                           float2* __restrict__ u, double* __restrict__ loss_acc,
-                                                          const TileIn* __restrict__ tab) {
+                                                          const TileIn* __restrict__ tab, uint8_t* __restrict__ signs) {
     __shared__ double red[4];
     if (tab) {
         const uint32_t tk = blockIdx.y;
@@ -337,6 +473,7 @@ __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restri
             (copysignf(1.0f, e0 - eL) - copysignf(1.0f, eR - e0) + copysignf(1.0f, e0 - eU) - copysignf(1.0f, eD - e0)) *
             gscale;
         loss32 += fabsf(eR - e0) + fabsf(eD - e0);
+        if (signs) signs[k] = (uint8_t)((signbit(eR - e0) ? 1 : 0) | (signbit(eD - e0) ? 2 : 0));
         const float2 v = z[k];
         const float mag = sqrtf(v.x * v.x + v.y * v.y);
         float2 o = make_float2(0.f, 0.f);
@@ -558,7 +695,9 @@ void generic_crop(const float2* S, const float2* P, const int2* led, uint32_t le
 }
 
 void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
-                     float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
+                     float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab, uint32_t ntiles,
+                     uint8_t* signs) {
+    if (tab && signs) throw HipError{hipErrorInvalidValue, "generic stencil: sign output is not available for tiles"};
     if (!tab) ntiles = 1;
     const size_t plane = (size_t)g.nr * g.nc, total = plane * b;
     const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
@@ -567,7 +706,7 @@ void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b
                        ratio, inv_ratio, ebuf, tab);
     FPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(gen_stencil_kernel, dim3(gblocks(total), ntiles), dim3(256), 0, st, z, ebuf, g.nr, g.nc, total,
-                       (float)(pratio * pratio), u, loss_acc, tab);
+                       (float)(pratio * pratio), u, loss_acc, tab, signs);
     FPM_LAUNCH_CHECK();
 }
 

@@ -90,6 +90,7 @@ class _Opts(ctypes.Structure):
         ("reserved", ctypes.c_int32),
         ("stop_loss", ctypes.c_double),
         ("epochs_run", ctypes.POINTER(ctypes.c_int32)),
+        ("stencil_signs", ctypes.c_void_p),
     ]
 
 
@@ -210,8 +211,12 @@ def cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio, numEpochs, batch, opt: Sequence[
     return O, P
 
 
-def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER, engine: int = FFT_F32):
-    """One fused gradient over all images: (G_O, G_P, loss).
+def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER, engine: int = FFT_F32,
+                   signs: bool = False):
+    """One fused gradient over all images: (G_O, G_P, loss), or (G_O, G_P,
+    loss, signs) with ``signs=True``: the loss stencil's sign decisions, uint8
+    n_r x n_c x L (bit 0: sign bit of e(i, j+1) - e(i, j), bit 1: of
+    e(i+1, j) - e(i, j); fpm_opts.stencil_signs).
 
     The gradient-only call of benchmark/cuFPM/fullyfusedFPM.cu:21-162 with the
     v3 loss: G_O = r^2 grad_O L and G_P = n^2 r^4 grad_P L (N = n r)."""
@@ -231,9 +236,14 @@ def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER, engin
     GP = np.empty_like(P, order="F")
     lossv = ctypes.c_double(0.0)
     o = _opts(stitch=stitch, engine=engine)
+    sg = np.zeros((nr, nc, L), dtype=np.uint8, order="F") if signs else None
+    if signs:
+        o.stencil_signs = sg.ctypes.data
     _check(lib().fpm_fused_gradient(_ptr(O), O.shape[0], O.shape[1], _ptr(P), nr, nc, _ptr(Yarr), ytype, L,
                                     _ptr(led), float(pratio), _ptr(GO), _ptr(GP), ctypes.byref(lossv),
                                     ctypes.byref(o)))
+    if signs:
+        return GO, GP, lossv.value, sg
     return GO, GP, lossv.value
 
 
@@ -310,10 +320,14 @@ def solve_device(O0_ptr: int, P0_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int
 
 def gradient_device(O_ptr: int, P_ptr: int, Y_ptr: int, y_u16: bool, led_ptr: int, N, n, L: int, pratio: float,
                     GO_ptr: int, GP_ptr: int, *, stitch: int = STITCH_OWNER, stream: Optional[int] = None,
-                    engine: int = FFT_F32) -> float:
-    """fpm_fused_gradient on device pointers; returns the loss."""
+                    engine: int = FFT_F32, signs_ptr: Optional[int] = None) -> float:
+    """fpm_fused_gradient on device pointers; returns the loss.  signs_ptr
+    (optional device pointer, n_r n_c L bytes) receives the loss stencil's
+    sign decisions (fpm_opts.stencil_signs)."""
     lossv = ctypes.c_double(0.0)
     o = _opts(stitch=stitch, device=True, stream=stream, engine=engine)
+    if signs_ptr:
+        o.stencil_signs = signs_ptr
     _check(lib().fpm_fused_gradient(O_ptr, N[0], N[1], P_ptr, n[0], n[1], Y_ptr, Y_U16_RAW if y_u16 else Y_F32_AMPLITUDE,
                                     L, led_ptr, float(pratio), GO_ptr, GP_ptr, ctypes.byref(lossv), ctypes.byref(o)))
     return lossv.value

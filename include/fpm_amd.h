@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define FPM_AMD_ABI_VERSION 2
+#define FPM_AMD_ABI_VERSION 3
 
 /* status codes */
 #define FPM_OK              0
@@ -49,9 +49,16 @@ extern "C" {
 #define FPM_E_NODEVICE     -5  /* no usable gfx950 device */
 
 /* shape rules: every image side (rows and columns of the object, pupil and
- * images) is a power of two in [16, 16384]; the object is at least as large
- * as the pupil in each dimension.  The reference accepts any even side
- * (cuFFT); sides that are not powers of two return FPM_E_UNSUPPORTED. */
+ * images) is even and in [16, 16384], and the object is at least as large as
+ * the pupil in each dimension (the reference takes any even side: cuFFT plans
+ * of any length, README.md:9, cuFPM_pure_v3.cu:118-126).  When all four sides
+ * are powers of two the solve runs the fused kernels; otherwise every call
+ * runs the generic path: FP32 mixed-radix transforms (radices 2, 3, 4, 5, 7
+ * as butterflies, any other prime factor as a direct sum) and unfused
+ * kernels, with the owner-computes stitch.  On the generic path
+ * fft_engine = FPM_FFT_SPLIT_F16 and stitch = FPM_STITCH_ATOMIC return
+ * FPM_E_UNSUPPORTED (fpm_tiles_solve runs such tiles with FP32 transforms and
+ * applies the split engine to its power-of-two tiles). */
 
 typedef struct { float re, im; } fpm_c32;
 
@@ -131,6 +138,18 @@ typedef struct {
                                      synchronisation per epoch); loss_per_epoch entries
                                      past the last epoch run are set to NaN */
     int32_t* epochs_run;          /* optional host int: epochs fpm_pure_v3 ran */
+    uint8_t* stencil_signs;       /* optional, fpm_fused_gradient only (host or device
+                                     like the other arrays): n_rows x n_cols x L bytes
+                                     receiving the loss stencil's sign decisions, pixel
+                                     (i, j) of image l at l n_r n_c + j n_r + i:
+                                     bit 0 = sign bit of e(i, j+1) - e(i, j), bit 1 =
+                                     sign bit of e(i+1, j) - e(i, j), cyclic, with
+                                     e = |z|/(n_r n_c r^2) - Y.  These are the two
+                                     copysign decisions per pixel that
+                                     kernels.cuh:234-253 combines; given to an FP64
+                                     computation they separate its distance from the
+                                     GPU's into arithmetic and near-tie flips.  No
+                                     counterpart in the reference. (ABI 3) */
 } fpm_opts;
 
 /* Fills *o with the defaults (owner-computes stitch, host pointers, silent,

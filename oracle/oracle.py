@@ -40,6 +40,9 @@ def lib() -> ctypes.CDLL:
             getattr(L, f"oracle_fft2_{s}").argtypes = [vp, i, i, i, i]
             getattr(L, f"oracle_gradient_{s}").argtypes = [vp, i, i, vp, i, i, vp, i, vp, i, vp, vp,
                                                            ctypes.POINTER(d), ctypes.POINTER(d)]
+            getattr(L, f"oracle_gradient_signs_{s}").argtypes = [vp, i, i, vp, i, i, vp, i, vp, i, vp, vp,
+                                                                 ctypes.POINTER(d), ctypes.POINTER(d), vp, vp,
+                                                                 ctypes.POINTER(ctypes.c_long)]
             getattr(L, f"oracle_loss_{s}").argtypes = [vp, i, i, vp, i, i, vp, i, vp, i]
             getattr(L, f"oracle_loss_{s}").restype = d
             getattr(L, f"oracle_solve_{s}").argtypes = [vp, i, i, vp, i, i, vp, i, vp, vp, i, i, i, f, f, f, f,
@@ -86,6 +89,32 @@ def gradient(O, P, Y, led, pratio, prec="f64"):
                                               P.shape[1], Y.ctypes.data, Y.shape[2], led.ctypes.data, int(pratio),
                                               GO.ctypes.data, GP.ctypes.data, ctypes.byref(loss), ctypes.byref(margin))
     return GO, GP, loss.value, margin.value
+
+
+def gradient_signs(O, P, Y, led, pratio, signs=None, prec="f64"):
+    """(G_O, G_P, loss, own_signs, flips) of one minibatch made of all pages
+    of Y with the loss stencil's sign decisions taken from `signs` (uint8 n_r
+    x n_c x L, bit 0: sign bit of e(i, j+1) - e(i, j), bit 1: of e(i+1, j) -
+    e(i, j)) instead of its own; own_signs are the decisions it would have
+    taken and flips the number of bits in which they differ from `signs`."""
+    O = _fa(O, _ct(prec))
+    P = _fa(P, _ct(prec))
+    Y = _fa(Y, np.float32)
+    if Y.ndim == 2:
+        Y = Y[:, :, None].copy(order="F")
+    led = _fa(led, np.int32)
+    GO = np.zeros_like(O, order="F")
+    GP = np.zeros_like(P, order="F")
+    own = np.zeros(Y.shape, dtype=np.uint8, order="F")
+    sg = None if signs is None else _fa(signs, np.uint8)
+    if sg is not None and sg.shape != Y.shape:
+        raise ValueError(f"signs must be {Y.shape}, got {sg.shape}")
+    loss, margin, flips = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_long(0)
+    getattr(lib(), f"oracle_gradient_signs_{prec}")(
+        O.ctypes.data, O.shape[0], O.shape[1], P.ctypes.data, P.shape[0], P.shape[1], Y.ctypes.data, Y.shape[2],
+        led.ctypes.data, int(pratio), GO.ctypes.data, GP.ctypes.data, ctypes.byref(loss), ctypes.byref(margin),
+        None if sg is None else sg.ctypes.data, own.ctypes.data, ctypes.byref(flips))
+    return GO, GP, loss.value, own, int(flips.value)
 
 
 def loss(O, P, Y, led, pratio, prec="f64"):

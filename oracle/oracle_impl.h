@@ -27,8 +27,9 @@ static inline CT FN(cmul)(CT a, CT b) { CT r = {a.re * b.re - a.im * b.im, a.re 
 
 /* ------------------------------------------------------------------ FFT */
 /* Iterative radix-2 decimation-in-time on a contiguous line for powers of
- * two; for other even lengths 2^a 3^b a self-sorting (Stockham) mixed-radix
- * transform with radices 4, 2, 3: pass (r, n, s) maps
+ * two; for other lengths a self-sorting (Stockham) mixed-radix transform
+ * with radices 4, 2, 3 and every other prime factor, each pass a direct
+ * r-point sum: pass (r, n, s) maps
  *   y[q + s (r p + k)] = (sum_j x[q + s (p + j n/r)] W_r^(j k)) W_n^(p k).
  * Twiddles are computed in double and rounded once to REAL. */
 typedef struct {
@@ -50,8 +51,8 @@ static void FN(plan_init)(FN(fftplan) * p, int n) {
         int m = n;
         while (m % 4 == 0) { p->fac[p->nf++] = 4; m /= 4; }
         while (m % 2 == 0) { p->fac[p->nf++] = 2; m /= 2; }
-        while (m % 3 == 0) { p->fac[p->nf++] = 3; m /= 3; }
-        if (m != 1) { fprintf(stderr, "oracle FFT: length %d is not 2^a 3^b\n", n); abort(); }
+        for (int q = 3; m > 1; q += 2)
+            while (m % q == 0) { p->fac[p->nf++] = q; m /= q; }
         p->rev = NULL;
         p->w = (CT*)malloc(sizeof(CT) * n);
         for (int k = 0; k < n; ++k) {
@@ -89,7 +90,7 @@ static void FN(fft_line_mixed)(const FN(fftplan) * p, CT* x, int inverse) {
         const int r = p->fac[f], m = n / r;
         for (int b = 0; b < L / r; ++b) {
             const int pp = b / s, q = b % s;
-            CT a[4];
+            CT a[r];
             for (int j = 0; j < r; ++j) a[j] = src[q + s * (pp + j * m)];
             for (int k = 0; k < r; ++k) {
                 CT acc = {0, 0};
@@ -179,10 +180,18 @@ void FN(oracle_fft2)(CT* a, int rows, int cols, int batch, int inverse) {
  * (float, page led0 + l), LED windows led[2*l] = kl, led[2*l+1] = kt.
  * Outputs: Gs (spectrum gradient, += into a zeroed N-side array), GP
  * (pupil gradient), loss (+=), margin (min |difference| at a sign decision).
+ * Sign decisions of the loss stencil, one byte per pixel of the b pages
+ * (page l pixel (i, j) at l nr nc + j nr + i): bit 0 = sign bit of
+ * e(i, j+1) - e(i, j), bit 1 = sign bit of e(i+1, j) - e(i, j) (cyclic), the
+ * two differences whose copysign values the stencil combines
+ * (kernels.cuh:234-253: sign(e - e_left) at (i, j) is bit 0 of (i, j-1)).
+ * force_signs (optional) replaces the stencil's own decisions with the
+ * given bits; own_signs (optional) receives the own decisions; flips
+ * (optional, +=) counts the bits in which they differ.
  */
 static void FN(minibatch)(const CT* Shat, int Nr, int Nc, const CT* P, int nr, int nc, const float* Y,
                           const int32_t* led, int led0, int b, int pratio, CT* Gs, CT* GP, double* loss,
-                          double* margin) {
+                          double* margin, const uint8_t* force_signs, uint8_t* own_signs, long* flips) {
     const long plane = (long)nr * nc;
     CT* sub = (CT*)malloc(sizeof(CT) * plane * b);
     CT* lat = (CT*)malloc(sizeof(CT) * plane * b);
@@ -209,11 +218,13 @@ static void FN(minibatch)(const CT* Shat, int Nr, int Nc, const CT* P, int nr, i
     const REAL ratio = (REAL)((uint32_t)plane * (uint32_t)(pratio * pratio));
     const REAL gscale = (REAL)(pratio * pratio);
     double lsum = 0.0, mmin = INFINITY;
-#pragma omp parallel for schedule(static) reduction(+ : lsum) reduction(min : mmin)
+    long nflip = 0;
+#pragma omp parallel for schedule(static) reduction(+ : lsum, nflip) reduction(min : mmin)
     for (int l = 0; l < b; ++l) {
         const float* y = Y + (long)(led0 + l) * plane;
         REAL* el = e + l * plane;
         const CT* z = lat + l * plane;
+        const uint8_t* fs = force_signs ? force_signs + l * plane : NULL;
         for (long p = 0; p < plane; ++p) el[p] = M_SQRT(z[p].re * z[p].re + z[p].im * z[p].im) / ratio - (REAL)y[p];
         for (int j = 0; j < nc; ++j)
             for (int i = 0; i < nr; ++i) {
@@ -222,8 +233,20 @@ static void FN(minibatch)(const CT* Shat, int Nr, int Nc, const CT* P, int nr, i
                 const REAL eR = el[(long)((j + 1) % nc) * nr + i];
                 const REAL eU = el[(long)j * nr + (i + nr - 1) % nr];
                 const REAL eD = el[(long)j * nr + (i + 1) % nr];
-                const REAL sxl = M_COPYSIGN((REAL)1, e0 - eL), sxr = M_COPYSIGN((REAL)1, eR - e0);
-                const REAL syu = M_COPYSIGN((REAL)1, e0 - eU), syd = M_COPYSIGN((REAL)1, eD - e0);
+                REAL sxl = M_COPYSIGN((REAL)1, e0 - eL), sxr = M_COPYSIGN((REAL)1, eR - e0);
+                REAL syu = M_COPYSIGN((REAL)1, e0 - eU), syd = M_COPYSIGN((REAL)1, eD - e0);
+                const uint8_t own = (uint8_t)((sxr < 0 ? 1 : 0) | (syd < 0 ? 2 : 0));
+                if (own_signs) own_signs[l * plane + (long)j * nr + i] = own;
+                if (fs) {
+                    const uint8_t here = fs[(long)j * nr + i];
+                    const uint8_t left = fs[(long)((j + nc - 1) % nc) * nr + i];
+                    const uint8_t up = fs[(long)j * nr + (i + nr - 1) % nr];
+                    sxr = (here & 1) ? (REAL)-1 : (REAL)1;
+                    syd = (here & 2) ? (REAL)-1 : (REAL)1;
+                    sxl = (left & 1) ? (REAL)-1 : (REAL)1;
+                    syu = (up & 2) ? (REAL)-1 : (REAL)1;
+                    nflip += ((here ^ own) & 1) + (((here ^ own) >> 1) & 1);
+                }
                 const REAL g = (sxl - sxr + syu - syd) * gscale;
                 lsum += fabs((double)(eR - e0)) + fabs((double)(eD - e0));
                 const double m1 = fabs((double)(eR - e0)), m2 = fabs((double)(eD - e0));
@@ -242,6 +265,7 @@ static void FN(minibatch)(const CT* Shat, int Nr, int Nc, const CT* P, int nr, i
     }
     if (loss) *loss += lsum;
     if (margin && mmin < *margin) *margin = mmin;
+    if (flips) *flips += nflip;
     FN(oracle_fft2)(u, nr, nc, b, 0);
 
     /* back-projection (kernels.cuh:87-128), sequential LED order */
@@ -271,9 +295,14 @@ static void FN(minibatch)(const CT* Shat, int Nr, int Nc, const CT* P, int nr, i
     free(e);
 }
 
-/* Gradient of one minibatch (all L images): G_O = IDFT(Gs), G_P. */
-void FN(oracle_gradient)(const CT* O, int Nr, int Nc, const CT* P, int nr, int nc, const float* Y, int L,
-                         const int32_t* led, int pratio, CT* G_O, CT* G_P, double* loss, double* margin) {
+/* Gradient of one minibatch (all L images) with the loss stencil's sign
+ * decisions imposed (force_signs, optional; layout in FN(minibatch)): G_O =
+ * IDFT(Gs), G_P; own_signs / flips as in FN(minibatch).  With the signs a
+ * GPU run took, the FP64 result differs from that run only by arithmetic
+ * (transforms, products, sums), not by near-tie decisions. */
+void FN(oracle_gradient_signs)(const CT* O, int Nr, int Nc, const CT* P, int nr, int nc, const float* Y, int L,
+                               const int32_t* led, int pratio, CT* G_O, CT* G_P, double* loss, double* margin,
+                               const uint8_t* force_signs, uint8_t* own_signs, long* flips) {
     const long NN = (long)Nr * Nc, nn = (long)nr * nc;
     CT* Shat = (CT*)malloc(sizeof(CT) * NN);
     memcpy(Shat, O, sizeof(CT) * NN);
@@ -282,9 +311,17 @@ void FN(oracle_gradient)(const CT* O, int Nr, int Nc, const CT* P, int nr, int n
     memset(G_P, 0, sizeof(CT) * nn);
     if (loss) *loss = 0.0;
     if (margin) *margin = INFINITY;
-    FN(minibatch)(Shat, Nr, Nc, P, nr, nc, Y, led, 0, L, pratio, G_O, G_P, loss, margin);
+    if (flips) *flips = 0;
+    FN(minibatch)(Shat, Nr, Nc, P, nr, nc, Y, led, 0, L, pratio, G_O, G_P, loss, margin, force_signs, own_signs,
+                  flips);
     FN(oracle_fft2)(G_O, Nr, Nc, 1, 1);
     free(Shat);
+}
+
+/* Gradient of one minibatch (all L images): G_O = IDFT(Gs), G_P. */
+void FN(oracle_gradient)(const CT* O, int Nr, int Nc, const CT* P, int nr, int nc, const float* Y, int L,
+                         const int32_t* led, int pratio, CT* G_O, CT* G_P, double* loss, double* margin) {
+    FN(oracle_gradient_signs)(O, Nr, Nc, P, nr, nc, Y, L, led, pratio, G_O, G_P, loss, margin, NULL, NULL, NULL);
 }
 
 /* Loss sum|Dx e| + |Dy e| of all L images at (O, P) (ret_loss.m). */
@@ -329,7 +366,7 @@ void FN(oracle_solve)(const CT* O0, int Nr, int Nc, const CT* P0, int nr, int nc
             FN(oracle_fft2)(Shat, Nr, Nc, 1, 0);
             memset(Gs, 0, sizeof(CT) * NN);
             memset(GP, 0, sizeof(CT) * nn);
-            FN(minibatch)(Shat, Nr, Nc, P, nr, nc, Y, led, curr, b, pratio, Gs, GP, &lsum, NULL);
+            FN(minibatch)(Shat, Nr, Nc, P, nr, nc, Y, led, curr, b, pratio, Gs, GP, &lsum, NULL, NULL, NULL, NULL);
             FN(oracle_fft2)(Gs, Nr, Nc, 1, 1);
             /* RMSprop of O (kernels.cuh:364-382) */
 #pragma omp parallel for schedule(static)

@@ -67,3 +67,33 @@ def texture(n, seed, shift=(0.0, 0.0), width=0.06):
     f *= np.exp(-(ky ** 2 + kx ** 2) / (2 * width ** 2))
     f *= np.exp(-2j * np.pi * (ky * shift[0] + kx * shift[1]))
     return np.real(np.fft.ifft2(f))
+
+
+def rect_margin_problem(nr, nc, r, L, seed=5):
+    """Non-square geometry: object (nr r) x (nc r), pupil nr x nc (an
+    elliptical NA disk with a defocus phase), L LED windows around the centre
+    (kl indexes columns, kt rows), and images whose residual is a margin
+    field (every loss-stencil sign decided by >= 0.01)."""
+    rng = np.random.default_rng(seed)
+    Nr, Nc = nr * r, nc * r
+    spec = np.fft.fft2(rng.random((Nr, Nc)))
+    fy = np.fft.fftfreq(Nr)[:, None]
+    fx = np.fft.fftfreq(Nc)[None, :]
+    spec *= (fy ** 2 + fx ** 2) <= (0.5 / r) ** 2
+    amp = np.real(np.fft.ifft2(spec))
+    amp = 0.3 + 0.7 * (amp - amp.min()) / (amp.max() - amp.min())
+    O = np.asfortranarray((amp * np.exp(1j * rng.uniform(-0.5, 0.5, (Nr, Nc)))).astype(np.complex64))
+    yy = (np.arange(nr) - nr // 2)[:, None] / (0.4 * nr)
+    xx = (np.arange(nc) - nc // 2)[None, :] / (0.4 * nc)
+    rho2 = yy ** 2 + xx ** 2
+    A = np.asfortranarray((rho2 <= 1.0).astype(np.float32))
+    P = np.asfortranarray((A * np.exp(0.5j * np.sqrt(3) * (2 * rho2 - 1))).astype(np.complex64))
+    c_r, c_c = Nr // 2 - nr // 2 + 1, Nc // 2 - nc // 2 + 1
+    led = np.asfortranarray(np.stack([c_c + rng.integers(-nc // 2, nc // 2 + 1, L),
+                                      c_r + rng.integers(-nr // 2, nr // 2 + 1, L)]).astype(np.int32))
+    led[:, 0] = (c_c, c_r)
+    model = synth.image_model_numpy(O.astype(np.complex128), P.astype(np.complex128), led, r)
+    Y = np.empty((nr, nc, L), dtype=np.float32, order="F")
+    for l in range(L):
+        Y[:, :, l] = (model[:, :, l] - margin_field((nr, nc), rng)).astype(np.float32)
+    return O, P, Y, led, A

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_opts_layout():
-    assert fpm_amd.lib().fpm_abi_version() == 2
+    assert fpm_amd.lib().fpm_abi_version() == 3
     o = fpm_amd._Opts()
     fpm_amd.lib().fpm_opts_init(ctypes.byref(o))
     assert o.struct_size == ctypes.sizeof(fpm_amd._Opts)
@@ -75,22 +75,45 @@ def _call_solve(nr=16, Nr=32, L=4, batch=2, led=None, epochs=1):
     return fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, 2, epochs, batch, (1e-3, 0.999, 1e-4))
 
 
-def test_rejects_sides_outside_both_paths():
-    """Sides must be powers of two or even 2^a 3^b (the generic path): sides
-    with a factor 5 are FPM_E_UNSUPPORTED before any device work."""
-    for nr, Nr in ((40, 80), (20, 60), (16, 80)):
+def test_rejects_sides_outside_the_domain():
+    """Every side even and in [16, 16384] (the reference takes any even side):
+    odd sides, sides below 16 and above 16384 are FPM_E_UNSUPPORTED before any
+    device work."""
+    for nr, Nr in ((15, 30), (16, 33), (8, 16), (16, 16386), (16, 32768)):
         with pytest.raises(fpm_amd.FPMError) as e:
             _call_solve(nr=nr, Nr=Nr)
         assert e.value.code == -2, (nr, Nr)
 
 
-def test_accepts_generic_sides_up_to_the_device():
-    """24 -> 48 (2^a 3^b) passes the size checks: without a GPU the call
-    fails only at the device check (FPM_E_NODEVICE), not as unsupported."""
+@pytest.mark.parametrize("nr,Nr", [(24, 48), (40, 80), (20, 60), (16, 80), (134, 268), (1536, 12288)])
+def test_accepts_any_even_side_up_to_the_device(nr, Nr):
+    """2^a 3^b, factors 5 and 7, a prime factor 67 and 12288-point sides pass
+    the size checks: without a GPU the call fails only at the device check
+    (FPM_E_NODEVICE), not as unsupported."""
     try:
-        _call_solve(nr=24, Nr=48)
+        _call_solve(nr=nr, Nr=Nr)
     except fpm_amd.FPMError as e:
         assert e.code == -5, e
+
+
+def test_generic_geometry_rejects_split_engine_and_atomic_stitch():
+    """On the generic path the split-FP16 engine and the atomic stitch are not
+    available: FPM_E_UNSUPPORTED with the reason, before any device work."""
+    O = np.zeros((80, 80), np.complex64, order="F")
+    P = np.zeros((40, 40), np.complex64, order="F")
+    Y = np.ones((40, 40, 2), np.float32, order="F")
+    led = np.ones((2, 2), np.int32, order="F")
+    for kw, word in (({"engine": fpm_amd.FFT_SPLIT_F16}, "split"), ({"stitch": fpm_amd.STITCH_ATOMIC}, "atomic")):
+        with pytest.raises(fpm_amd.FPMError) as e:
+            fpm_amd.fused_gradient(O, P, Y, led, 2, **kw)
+        assert e.value.code == -2 and word in str(e.value), kw
+        with pytest.raises(fpm_amd.FPMError) as e:
+            fpm_amd.cuFPM_pure_v3(O, P, Y, led, np.ones((40, 40), np.float32, order="F"), 2, 1, 2,
+                                  (1e-3, 0.999, 1e-4), **kw)
+        assert e.value.code == -2 and word in str(e.value), kw
+    with pytest.raises(fpm_amd.FPMError) as e:
+        fpm_amd.fft2(O, engine=fpm_amd.FFT_SPLIT_F16)
+    assert e.value.code == -2 and "split" in str(e.value)
 
 
 def test_rejects_bad_batch():
@@ -176,10 +199,10 @@ def test_tiles_solve_rejects_bad_arguments():
     assert "null" in L.fpm_last_error().decode()
     assert L.fpm_tiles_solve(arr, 0, 4, 2.0, 1.0, 2.0, optv, 0, None, None) == -1
     assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 65, None, None) == -1
-    # a tile side with a factor 5 (neither a power of two nor 2^a 3^b)
+    # an odd tile side (outside the reference's domain of even sides)
     buf = np.zeros(4096, np.complex64)
     p = buf.ctypes.data
-    arr[0] = ft.Tile(p, p, 40, p, p, 20, p, 0, p, p, None)
+    arr[0] = ft.Tile(p, p, 42, p, p, 21, p, 0, p, p, None)
     assert L.fpm_tiles_solve(arr, 1, 4, 2.0, 1.0, 2.0, optv, 1, None, None) == -2
     # stop_loss is an option of the single solve only
     o = fpm_amd._opts(stop_loss=1.0)

@@ -1,17 +1,22 @@
 """Accuracy against the FP64 oracle at the workloads the benchmark runs:
-the full 2048^2 x 361 -> 16384^2 gradient of one minibatch, and a short
-solve from an aberrated-pupil start at the warm-start size.  With real data
-some loss-stencil sign decisions lie within FP32 rounding of a tie, so every
-FP32 computation (the GPU's and the oracle's own FP32 build) drifts from FP64
-by about 1e-3 here; the bar is that both GPU engines stay as close to FP64
-as the FP32 oracle does (a multiple of its distance to FP64, plus 1e-5, and
-below an absolute cap)."""
+the full 2048^2 x 361 -> 16384^2 gradient of one minibatch (and the 512^2 x
+361 -> 4096^2 one), and a short solve from an aberrated-pupil start at the
+warm-start size.  With real data some loss-stencil sign decisions lie within
+FP32 rounding of a tie, so every FP32 computation (the GPU's and the
+oracle's own FP32 build) takes a few of them differently from FP64 and its
+gradient lands about 1e-3 from FP64's.  Two bars separate the causes: (1)
+sign-synchronised -- the FP64 oracle given the GPU's own sign decisions
+(fpm_opts.stencil_signs) -- the GPU gradient is within 1e-5 of FP64, so the
+arithmetic (transforms, products, sums) meets the 1e-5 bar at full scale;
+(2) unsynchronised, both GPU engines stay as close to FP64 as the FP32
+oracle does (a multiple of its distance, plus 1e-5, below a cap)."""
 import numpy as np
 import pytest
 
 import fpm_amd
 import oracle
 from fpm_amd import synth, warmstart
+from helpers import margin_problem, rect_margin_problem, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -39,27 +44,52 @@ def _prepared(ds, count):
     return np.asfortranarray(Y.transpose(2, 1, 0))
 
 
-def test_full_scale_gradient_as_close_to_fp64_as_the_fp32_oracle():
-    """2048^2 x 361 -> 16384^2: the fused gradient of the first minibatch (26
-    images) at the initial object and pupil, from both FFT engines, against
-    the FP64 oracle on the host cores (about 3 s on 16 threads).  G_O's
-    distance comes from many near-tie sign flips and is stable (about 1.3e-3
-    for the oracle and both engines): 1.25x.  G_P's comes from a few flips
-    that carry much of its norm, so it varies by about 2x between any two
-    FP32 computations (2.2e-4 to 6e-4 measured): 3x.  Cap 2e-3 for both."""
-    n, r, B = 2048, 8, 26
+@pytest.mark.parametrize("case", ["fused", "whole_page", "generic", "plain_stencil"])
+def test_stencil_signs_equal_the_fp64_decisions_on_margin_inputs(case):
+    """On margin inputs (every sign decided by >= 0.01) the sign bits the GPU
+    exports (fpm_opts.stencil_signs) are exactly the FP64 oracle's own
+    decisions, pixel for pixel (the export's index map), for the fused
+    stencil kernel (n = 64 and a whole-page workgroup at n = 128), the generic
+    path (n = 48) and the plain stencil kernel (4096 x 32 pupil: rows beyond
+    the fused kernel's 2048); the gradient matches FP64 within 1e-5."""
+    if case == "fused":
+        O, P, Y, led = margin_problem(64, 4, 5)
+        r = 4
+    elif case == "whole_page":
+        O, P, Y, led = margin_problem(128, 2, 3)
+        r = 2
+    elif case == "generic":
+        O, P, Y, led = margin_problem(48, 4, 5)
+        r = 4
+    else:
+        O, P, Y, led, _ = rect_margin_problem(4096, 32, 2, 6)
+        r = 2
+    gO, gP, _, sg = fpm_amd.fused_gradient(O, P, Y, led, r, signs=True)
+    GO, GP, _, own, flips = oracle.gradient_signs(O, P, Y, led, r, signs=sg)
+    print(f"[signs] {case}: flips {flips} of {2 * sg.size}; G_O {rel_l2(gO, GO):.2e} G_P {rel_l2(gP, GP):.2e}",
+          flush=True)
+    assert flips == 0 and np.array_equal(sg, own)
+    assert rel_l2(gO, GO) <= 1e-5 and rel_l2(gP, GP) <= 1e-5
+
+
+def _sign_synchronised(n, r, B, label, cap_unsync):
+    """The fused gradient of the first minibatch (B images) of the n^2 x 361
+    -> (n r)^2 synthetic workload at the initial object and pupil, from both
+    FFT engines with their sign decisions; the FP64 oracle with each engine's
+    decisions imposed, its own FP64 gradient and the FP32 oracle's."""
     ds = synth.make_device(n, r)
     N = n * r
     gpu = {}
     for name, eng in (("fp32", fpm_amd.FFT_F32), ("split", fpm_amd.FFT_SPLIT_F16)):
         GO, GP = torch.empty_like(ds["O0_dev"]), torch.empty_like(ds["P0_dev"])
+        sg = torch.empty((B, n, n), dtype=torch.uint8, device=GO.device)
         torch.cuda.synchronize()
         fpm_amd.gradient_device(ds["O0_dev"].data_ptr(), ds["P0_dev"].data_ptr(), ds["raw_dev"].data_ptr(), True,
                                 ds["led_dev"].data_ptr(), (N, N), (n, n), B, r, GO.data_ptr(), GP.data_ptr(),
-                                engine=eng)
+                                engine=eng, signs_ptr=sg.data_ptr())
         torch.cuda.synchronize()
-        gpu[name] = (_host(GO), _host(GP))
-        del GO, GP
+        gpu[name] = (_host(GO), _host(GP), np.asfortranarray(sg.cpu().numpy().transpose(2, 1, 0)))
+        del GO, GP, sg
     Y = _prepared(ds, B)
     O0, P0 = _host(ds["O0_dev"]), _host(ds["P0_dev"])
     led = np.asfortranarray(ds["led"][:, :B])
@@ -68,12 +98,46 @@ def test_full_scale_gradient_as_close_to_fp64_as_the_fp32_oracle():
     GO64, GP64, _, margin = oracle.gradient(O0, P0, Y, led, r, prec="f64")
     GO32, GP32, _, _ = oracle.gradient(O0, P0, Y, led, r, prec="f32")
     ref = {"G_O": _rel(GO32, GO64), "G_P": _rel(GP32, GP64)}
-    print(f"[fp64-full] FP32 oracle vs FP64: G_O {ref['G_O']:.2e} G_P {ref['G_P']:.2e} (smallest sign margin "
+    print(f"[{label}] FP32 oracle vs FP64: G_O {ref['G_O']:.2e} G_P {ref['G_P']:.2e} (smallest sign margin "
           f"{margin:.1e})", flush=True)
-    for name, (go, gp) in gpu.items():
+    decisions = 2 * Y.size
+    for name, (go, gp, sg) in gpu.items():
+        SO, SP, _, _, flips = oracle.gradient_signs(O0, P0, Y, led, r, signs=sg, prec="f64")
+        FO, FP, _, _, _ = oracle.gradient_signs(O0, P0, Y, led, r, signs=sg, prec="f32")
+        es = {"G_O": _rel(go, SO), "G_P": _rel(gp, SP)}
+        floor = {"G_O": _rel(FO, SO), "G_P": _rel(FP, SP)}
         e = {"G_O": _rel(go, GO64), "G_P": _rel(gp, GP64)}
-        print(f"[fp64-full] {name} engine vs FP64: G_O {e['G_O']:.2e} G_P {e['G_P']:.2e}", flush=True)
-        _close_enough(name, e, ref, {"G_O": 1.25, "G_P": 3.0}, 2e-3)
+        print(f"[{label}] {name} engine: sign-synchronised vs FP64 G_O {es['G_O']:.2e} G_P {es['G_P']:.2e} "
+              f"(FP32 oracle with the same signs: G_O {floor['G_O']:.2e} G_P {floor['G_P']:.2e}); {flips} of "
+              f"{decisions} decisions ({flips / decisions:.1e}) differ from FP64's; unsynchronised "
+              f"G_O {e['G_O']:.2e} G_P {e['G_P']:.2e}", flush=True)
+        assert es["G_O"] <= 1e-5, (name, es)
+        # G_P = sum over the batch of R conj(S) cancels strongly: any FP32
+        # computation of R and S (transforms) lands ~1e-5..2e-5 from FP64 there
+        assert es["G_P"] <= max(1e-5, 1.5 * floor["G_P"]), (name, es, floor)
+        assert flips <= 1e-4 * decisions, (name, flips)
+        _close_enough(name, e, ref, {"G_O": 1.25, "G_P": 3.0}, cap_unsync)
+
+
+@pytest.mark.timeout(900)
+def test_full_scale_gradient_sign_synchronised_within_1e5_of_fp64():
+    """2048^2 x 361 -> 16384^2, first minibatch (26 images): each engine's
+    G_O within 1e-5 of the FP64 oracle that takes the engine's own sign
+    decisions (fewer than 1e-4 of the decisions differ from FP64's), G_P
+    within 1e-5 or 1.5x the FP32 oracle's distance with the same signs
+    (G_P = sum_l R_l conj(S_l) cancels: measured 1.2e-5 on the GPU, 2.2e-5
+    for the FP32 restatement at 512 -> 4096, with FP32 or FP64 summation);
+    unsynchronised, G_O within 1.25x and G_P within 3x of the FP32 oracle's
+    distance to FP64 (G_O's ~1.3e-3 comes from many near-tie flips and is
+    stable; G_P's from a few flips that carry much of its norm, so any two
+    FP32 computations differ by ~2x there), cap 2e-3."""
+    _sign_synchronised(2048, 8, 26, "fp64-full", 2e-3)
+
+
+@pytest.mark.timeout(600)
+def test_512_to_4096_gradient_sign_synchronised_within_1e5_of_fp64():
+    """The same at 512^2 x 361 -> 4096^2 (first minibatch of 26)."""
+    _sign_synchronised(512, 8, 26, "fp64-4096", 5e-3)
 
 
 def test_short_solve_from_an_aberrated_pupil_as_close_to_fp64_as_the_fp32_oracle():

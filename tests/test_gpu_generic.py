@@ -14,7 +14,7 @@ import fpm_amd
 import oracle
 from fpm_amd import synth
 from fpm_amd import tiles as ft
-from helpers import margin_problem, rel_l2, small_leds
+from helpers import margin_problem, rect_margin_problem, rel_l2, small_leds
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -150,3 +150,50 @@ def test_generic_tiles_equal_their_own_solves(launch):
                              O.data_ptr(), P.data_ptr())
         assert np.array_equal(t["O"].cpu().numpy(), O.cpu().numpy()), k
         assert np.array_equal(t["P"].cpu().numpy(), P.cpu().numpy()), k
+
+
+# ------------------------------------------------- any even side up to 16384
+@pytest.mark.parametrize("rows,cols,batch", [(200, 40, 2), (20, 98, 1), (134, 22, 3), (254, 16, 1),
+                                             (12288, 16, 1), (16, 12288, 1), (1600, 200, 1), (16, 8186, 1)])
+def test_generic_fft_radix_5_7_and_prime_factors_match_numpy(rows, cols, batch):
+    """Lengths with factors 5 and 7 (hard-coded butterflies), primes 67, 127
+    and 4093 (direct sums: 134 = 2 67, 254 = 2 127, 8186 = 2 4093) and lines
+    of 12288 points (one line per workgroup, 96 KiB of LDS)."""
+    rng = np.random.default_rng(rows * 7 + cols)
+    x = np.asfortranarray((rng.standard_normal((rows, cols, batch)) + 1j * rng.standard_normal((rows, cols, batch)))
+                          .astype(np.complex64))
+    ref = np.fft.fft2(x.astype(np.complex128), axes=(0, 1))
+    got = np.stack([fpm_amd.fft2(x[:, :, k]) for k in range(batch)], axis=2)
+    inv = np.fft.ifft2(x.astype(np.complex128), axes=(0, 1)) * rows * cols
+    goti = np.stack([fpm_amd.fft2(x[:, :, k], inverse=True) for k in range(batch)], axis=2)
+    print(f"[generic-fft] {rows} x {cols}: forward {rel_l2(got, ref):.2e} inverse {rel_l2(goti, inv):.2e}", flush=True)
+    # a direct p-point sum in FP32 adds ~sqrt(p) eps per pass
+    tol = 2e-6 if max(rows, cols) <= 12288 and cols != 8186 else 2e-5
+    assert rel_l2(got, ref) < tol and rel_l2(goti, inv) < tol
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("n,r,grid", [(200, 8, 3), (134, 2, 3), (1536, 8, 3)])
+def test_generic_any_even_side_gradient_and_step_match_fp64(n, r, grid):
+    """200 -> 1600 (2^6 5^2), 134 -> 268 (4 67: a prime factor summed
+    directly) and 1536 -> 12288 (3 4096: 12288-point lines) on margin inputs:
+    the fused gradient within 1e-5 of FP64 and one solve step within 1e-6.
+    (The 12288 object is a band-limited random field with LED windows near
+    the centre: the synthetic specimen's 20 Gaussians cost minutes there.)"""
+    if n == 1536:
+        O, P, Y, led, _ = rect_margin_problem(n, n, r, 6)
+    else:
+        O, P, Y, led = margin_problem(n, r, grid)
+    GO, GP, loss, margin = oracle.gradient(O, P, Y, led, r)
+    assert margin > 5e-3
+    gO, gP, gl = fpm_amd.fused_gradient(O, P, Y, led, r)
+    print(f"[generic-any] {n} -> {n * r}: gradient G_O {rel_l2(gO, GO):.2e} G_P {rel_l2(gP, GP):.2e}", flush=True)
+    assert rel_l2(gO, GO) <= 1e-5 and rel_l2(gP, GP) <= 1e-5
+    assert abs(gl - loss) <= 1e-5 * loss
+    A, _ = synth.pupil(n)
+    L = led.shape[1]
+    opt = (2e-3, 0.999, 1e-4)
+    O1, P1 = fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, r, 1, L, opt)
+    Oo, Po, _ = oracle.solve(O, P, Y, led, A, r, 1, L, opt, prec="f64")
+    print(f"[generic-any] {n} -> {n * r}: one step O {rel_l2(O1, Oo):.2e} P {rel_l2(P1, Po):.2e}", flush=True)
+    assert rel_l2(O1, Oo) < 1e-6 and rel_l2(P1, Po) < 1e-6

@@ -15,7 +15,8 @@ def _cplx(shape, rng=RNG):
 
 
 @pytest.mark.parametrize("rows,cols,batch", [(16, 16, 1), (64, 32, 3), (32, 128, 2), (8, 512, 1),
-                                             (192, 192, 2), (48, 96, 1), (12, 24, 2), (768, 36, 1)])
+                                             (192, 192, 2), (48, 96, 1), (12, 24, 2), (768, 36, 1),
+                                             (200, 40, 1), (14, 98, 2), (134, 22, 1), (254, 12, 1), (12288, 4, 1)])
 def test_oracle_fft_matches_numpy(rows, cols, batch):
     x = _cplx((rows, cols, batch))
     ref = np.fft.fft2(x, axes=(0, 1))
@@ -25,9 +26,10 @@ def test_oracle_fft_matches_numpy(rows, cols, batch):
     assert rel_l2(oracle.fft2(x, prec="f32"), ref) < 1e-6
 
 
-@pytest.mark.parametrize("n", [16, 12, 18])
+@pytest.mark.parametrize("n", [16, 12, 18, 10, 14, 22, 26, 34])
 def test_oracle_fft_brute_force_dft(n):
-    """Powers of two and the mixed-radix lengths 2^a 3^b (radices 4, 2, 3)."""
+    """Powers of two and mixed-radix lengths (radices 4, 2, 3, 5, 7 and the
+    primes 11, 13, 17 as direct sums)."""
     x = _cplx((n, n))
     k = np.arange(n)
     W = np.exp(-2j * np.pi * np.outer(k, k) / n)
@@ -254,3 +256,60 @@ def test_led_offsets_closed_forms():
         mkl, mkt = pos[mk]
         assert kl - c == -(mkl - c) and kt - c == -(mkt - c)
     assert sorted(k.tolist()) == list(range(1, 362))
+
+
+def _numpy_gradient_with_signs(O, P, Y, led, r, signs):
+    """Independent numpy restatement (np.fft, vectorised index maps) of one
+    minibatch gradient with the loss stencil's right / down difference signs
+    taken from `signs` (bit 0 / bit 1 set: negative)."""
+    Nr, Nc = O.shape
+    nr, nc = P.shape
+    S = np.fft.fft2(O)
+    GS = np.zeros_like(S)
+    GP = np.zeros_like(P, dtype=np.complex128)
+    ii = np.arange(nr)[:, None]
+    jj = np.arange(nc)[None, :]
+    for l in range(led.shape[1]):
+        kl, kt = int(led[0, l]), int(led[1, l])
+        rows = (ii + kt - 1 + Nr // 2) % Nr
+        cols = (jj + kl - 1 + Nc // 2) % Nc
+        sub = S[rows, cols]
+        z = np.fft.ifft2(np.fft.ifftshift(sub * P)) * (nr * nc)
+        sx = np.where(signs[:, :, l] & 1, -1.0, 1.0)
+        sy = np.where(signs[:, :, l] & 2, -1.0, 1.0)
+        g = r * r * (np.roll(sx, 1, axis=1) - sx + np.roll(sy, 1, axis=0) - sy)
+        mag = np.abs(z)
+        u = np.where(mag > 0, g * z / np.where(mag > 0, mag, 1), 0)
+        R = np.fft.fftshift(np.fft.fft2(u))
+        np.add.at(GS, (np.broadcast_to(rows, (nr, nc)), np.broadcast_to(cols, (nr, nc))), R * np.conj(P) / (Nr * Nc))
+        GP += R * np.conj(sub)
+    return np.fft.ifft2(GS) * (Nr * Nc), GP
+
+
+def test_gradient_with_imposed_signs():
+    """oracle.gradient_signs: (1) its own decisions imposed reproduce
+    oracle.gradient bit for bit, with zero flips; (2) every decision flipped
+    negates the gradient exactly (u is linear in the stencil value); (3)
+    random decisions imposed match the numpy restatement; (4) the FP32
+    oracle's decisions on margin inputs equal the FP64 ones (no flips)."""
+    n, r = 16, 2
+    O, P, Y, led = margin_problem(n, r)
+    GO, GP, loss, _ = oracle.gradient(O, P, Y, led, r)
+    g1 = oracle.gradient_signs(O, P, Y, led, r)
+    own = g1[3]
+    assert own.shape == Y.shape and own.max() <= 3 and 0 < own.mean() < 3
+    g2 = oracle.gradient_signs(O, P, Y, led, r, signs=own)
+    assert np.array_equal(g2[0], GO) and np.array_equal(g2[1], GP) and g2[4] == 0
+    assert abs(g2[2] - loss) <= 1e-12 * loss  # an OpenMP reduction: order may vary
+    g3 = oracle.gradient_signs(O, P, Y, led, r, signs=own ^ 3)
+    assert np.array_equal(g3[0], -GO) and np.array_equal(g3[1], -GP)
+    assert g3[4] == 2 * own.size
+    rng = np.random.default_rng(5)
+    rnd = np.asfortranarray(rng.integers(0, 4, size=Y.shape).astype(np.uint8))
+    g4 = oracle.gradient_signs(O, P, Y, led, r, signs=rnd)
+    nO, nP = _numpy_gradient_with_signs(O.astype(np.complex128), P.astype(np.complex128), Y, led, r, rnd)
+    assert rel_l2(g4[0], nO) < 1e-12 and rel_l2(g4[1], nP) < 1e-12
+    flips = int(np.unpackbits((own ^ rnd)[..., None], axis=-1).sum())
+    assert g4[4] == flips
+    f32 = oracle.gradient_signs(O, P, Y, led, r, prec="f32")
+    assert np.array_equal(f32[3], own)

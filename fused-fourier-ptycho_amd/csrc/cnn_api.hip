@@ -467,6 +467,18 @@ int fpm_cnn_set_params(fpm_cnn* h, const float* in) {
     });
 }
 
+// The network's stream is its own (non-blocking); its inputs are often
+// written by work queued on the null stream just before the call (torch's
+// default stream): wait for that work first.
+static void after_null_stream(hipStream_t st) {
+    hipEvent_t e = nullptr;
+    FPM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    hipError_t r = hipEventRecord(e, nullptr);
+    if (r == hipSuccess) r = hipStreamWaitEvent(st, e, 0);
+    (void)hipEventDestroy(e);
+    FPM_HIP(r);
+}
+
 int fpm_cnn_forward(fpm_cnn* h, const float* feats, int64_t batch, float* out, float* pupil_out) {
     return guarded("fpm_cnn_forward", [&] {
         Net* net = (Net*)h;
@@ -475,6 +487,7 @@ int fpm_cnn_forward(fpm_cnn* h, const float* feats, int64_t batch, float* out, f
         const int B = (int)batch;
         if (pupil_out && net->K == 0)
             throw fpm::HipError{hipErrorInvalidValue, "pupil_out given to a network without a pupil head"};
+        after_null_stream(net->st);
         load_features(*net, feats, B);
         forward(*net, B, pupil_out);
         head_shuffle(net->head_f.as<float>(), out, B, net->d.n, net->d.n, net->d.up, net->st);
@@ -487,6 +500,7 @@ int fpm_cnn_gradient(fpm_cnn* h, const float* feats, const float* targets, const
     return guarded("fpm_cnn_gradient", [&] {
         Net* net = (Net*)h;
         check_call(net, feats, targets, pupil_targets, batch);
+        after_null_stream(net->st);
         const double loss = loss_and_grad(*net, feats, targets, pupil_targets, (int)batch);
         if (loss_out) *loss_out = loss;
         if (grad_out) {
@@ -501,6 +515,7 @@ int fpm_cnn_train_step(fpm_cnn* h, const float* feats, const float* targets, con
     return guarded("fpm_cnn_train_step", [&] {
         Net* net = (Net*)h;
         check_call(net, feats, targets, pupil_targets, batch);
+        after_null_stream(net->st);
         const double loss = loss_and_grad(*net, feats, targets, pupil_targets, (int)batch);
         net->step += 1;
         const float b1 = 0.9f, b2 = 0.999f;

@@ -37,6 +37,11 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* get_error() { return g_last_error.c_str(); }
 
+// W_16384^m, m < 16384: FP64-generated, rounded once to FP32; one copy per
+// GPU in a static device array (no allocation), filled on the first call that
+// needs it on that device
+static __device__ __align__(256) float2 g_twiddles[kTwiddleLen];
+
 const float2* twiddle_table(int device) {
     static std::mutex mu;
     static std::map<int, float2*> tables;
@@ -49,11 +54,11 @@ const float2* twiddle_table(int device) {
         const double a = -two_pi * (double)m / (double)kTwiddleLen;
         h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
-    float2* d = nullptr;
-    FPM_HIP(hipMalloc(&d, sizeof(float2) * kTwiddleLen));
+    void* d = nullptr;
+    FPM_HIP(hipGetSymbolAddress(&d, HIP_SYMBOL(g_twiddles)));
     FPM_HIP(hipMemcpy(d, h.data(), sizeof(float2) * kTwiddleLen, hipMemcpyHostToDevice));
-    tables[device] = d;
-    return d;
+    tables[device] = (float2*)d;
+    return (const float2*)d;
 }
 
 namespace {
@@ -87,17 +92,27 @@ struct DevBuf {
     }
 };
 
-// stream: caller's, or one created for the call (default priority)
+// stream: the caller's, or (NULL) one created for the call (default
+// priority) that first waits for the work already queued on the null stream
+// (torch's default stream, host-side work of a MATLAB session), so inputs
+// that work is still producing are read only once they are complete; the
+// created stream is non-blocking so that epochs can be captured as graphs
 struct StreamHolder {
     hipStream_t s = nullptr;
     bool own = false;
     explicit StreamHolder(void* user) {
         if (user) {
             s = (hipStream_t)user;
-        } else {
-            FPM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-            own = true;
+            return;
         }
+        FPM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        own = true;
+        hipEvent_t e = nullptr;
+        FPM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        hipError_t r = hipEventRecord(e, nullptr);
+        if (r == hipSuccess) r = hipStreamWaitEvent(s, e, 0);
+        (void)hipEventDestroy(e);
+        FPM_HIP(r);
     }
     ~StreamHolder() {
         if (own && s) (void)hipStreamDestroy(s);
@@ -353,7 +368,15 @@ inline int col_launches(const ColPerm& p) { return p.M2 == 1 ? 1 : 2; }
 
 // Per-minibatch gradient pipeline shared by the solve and the gradient entry.
 struct Workspace {
-    DevBuf lat, rec;
+    float2* lat = nullptr;  // b pages each (b <= the batch)
+    float2* rec = nullptr;
+    DevBuf own_lat, own_rec;
+    void alloc(size_t bytes_each) {
+        own_lat.alloc(bytes_each);
+        own_rec.alloc(bytes_each);
+        lat = own_lat.as<float2>();
+        rec = own_rec.as<float2>();
+    }
 };
 
 // Shat (full spectrum, column-permuted) -> per-LED back-projection, the
@@ -368,8 +391,8 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
                         const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
                         RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
                         StageTimer& tm, const uint8_t* rowmask, const uint8_t* pmask, uint8_t* signs = nullptr) {
-    float2* lat = ws.lat.as<float2>();
-    float2* rec = ws.rec.as<float2>();
+    float2* lat = ws.lat;
+    float2* rec = ws.rec;
     const bool fused = stencil_fused_supported(g);
     // pupil-support pruning needs the masked read of the fused stencil kernel
     // and the skip of the pupil kernel (update mode, A given)
@@ -473,8 +496,7 @@ struct Solve {
         S1.alloc(NN * sizeof(float2));
         mO.alloc(NN * sizeof(float));
         mP.alloc(nn * sizeof(float));
-        ws.lat.alloc(nn * B * sizeof(float2));
-        ws.rec.alloc(nn * B * sizeof(float2));
+        ws.alloc(nn * B * sizeof(float2));
         FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
         FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
         // device-resident solve state: lr, epoch counter, FP64 loss slots
@@ -558,8 +580,7 @@ struct Solve {
         S1.alloc(NN * sizeof(float2));
         mO.alloc(NN * sizeof(float));
         mP.alloc(nn * sizeof(float));
-        ws.lat.alloc(nn * B * sizeof(float2));
-        ws.rec.alloc(nn * B * sizeof(float2));
+        ws.alloc(nn * B * sizeof(float2));
         ebuf.alloc(nn * B * sizeof(float));
         FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
         FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
@@ -584,8 +605,8 @@ struct Solve {
         const size_t NN = (size_t)g.Nr * g.Nc;
         float2* S = S0.as<float2>();
         float2* G = S1.as<float2>();
-        float2* lat = ws.lat.as<float2>();
-        float2* rec = ws.rec.as<float2>();
+        float2* lat = ws.lat;
+        float2* rec = ws.rec;
         uint32_t led0, b;
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
@@ -1249,8 +1270,7 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         const YStack y = prepare_y(Y, y_dtype, nn * (size_t)L, isdev, hY, st);
         const int2* led = (const int2*)upload(led_idx, 2 * (size_t)L, isdev, hLed, st);
         Workspace ws;
-        ws.lat.alloc(nn * L * sizeof(float2));
-        ws.rec.alloc(nn * L * sizeof(float2));
+        ws.alloc(nn * L * sizeof(float2));
         S0.alloc(NN * sizeof(float2));
         S1.alloc(NN * sizeof(float2));
         float2* GPd;
@@ -1289,8 +1309,8 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
             eb.alloc(nn * (size_t)L * sizeof(float));
             generic_prepare(g);
             float2* S = S0.as<float2>();
-            float2* lat = ws.lat.as<float2>();
-            float2* rec = ws.rec.as<float2>();
+            float2* lat = ws.lat;
+            float2* rec = ws.rec;
             const uint32_t b = (uint32_t)L;
             fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
             generic_crop(S, pcopy.as<float2>(), led, 0, b, g, lat, st);

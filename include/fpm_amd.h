@@ -115,7 +115,9 @@ typedef struct {
     int32_t stitch;               /* FPM_STITCH_*; default OWNER */
     int32_t pointers_are_device;  /* 1: every array argument is a device pointer */
     int32_t verbose;              /* 1: print the two parameter lines v3 prints */
-    void*   stream;               /* hipStream_t to run on; NULL = library stream */
+    void*   stream;               /* hipStream_t to run on; NULL = a library stream
+                                     that starts after the work queued on the
+                                     null stream when the call is made */
     double* loss_per_epoch;       /* optional host array [epochs]: FP64 sum over the
                                      epoch's minibatches of sum|Dx e| + |Dy e| */
     float   pupil_clamp;          /* pupil amplitude band (v3 hard-codes 0.2);

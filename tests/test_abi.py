@@ -312,3 +312,4 @@ def test_parallax_pupil_save_load_roundtrip(tmp_path):
     m.save(path)
     m2 = ws.ParallaxPupil.load(path)
     assert m2.planes == [0, 2, 5] and m2.iters == 3 and np.array_equal(m2.W, m.W)
+

@@ -113,8 +113,10 @@ def _sign_synchronised(n, r, B, label, cap_unsync):
               f"G_O {e['G_O']:.2e} G_P {e['G_P']:.2e}", flush=True)
         assert es["G_O"] <= 1e-5, (name, es)
         # G_P = sum over the batch of R conj(S) cancels strongly: any FP32
-        # computation of R and S (transforms) lands ~1e-5..2e-5 from FP64 there
-        assert es["G_P"] <= max(1e-5, 1.5 * floor["G_P"]), (name, es, floor)
+        # computation of R and S (transforms) lands ~1e-5..3e-5 from FP64
+        # there; the split engine's 22-bit products add up to ~2x more
+        f = 1.5 if name == "fp32" else 3.0
+        assert es["G_P"] <= max(1e-5, f * floor["G_P"]), (name, es, floor)
         assert flips <= 1e-4 * decisions, (name, flips)
         _close_enough(name, e, ref, {"G_O": 1.25, "G_P": 3.0}, cap_unsync)
 

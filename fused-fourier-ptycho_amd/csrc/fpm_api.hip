@@ -1344,6 +1344,78 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
     });
 }
 
+// Scratch of fpm_fused_gradient_noalloc: the two object spectra, the two sets
+// of per-LED pages and the pupil copy that fpm_fused_gradient allocates, in
+// one static device array.  Each GPU the library runs on holds its own copy
+// until the process ends; every call on a GPU shares that GPU's copy.
+static __device__ __align__(256) unsigned char scratch[FPM_NOALLOC_SCRATCH_BYTES];
+
+// Address of this device's copy of `scratch`: looked up once per device (with
+// that device current) on the entry point's first call there.  In host code
+// the array's name is only a stand-in; kernels and copies take this address.
+unsigned char* noalloc_scratch(int device) {
+    static std::mutex mu;
+    static std::map<int, unsigned char*> ptrs;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(device);
+    if (it != ptrs.end()) return it->second;
+    void* p = nullptr;
+    FPM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) invalid("hipGetSymbolAddress returned no address for the scratch array", FPM_E_HIP);
+    ptrs[device] = (unsigned char*)p;
+    return (unsigned char*)p;
+}
+
+int fpm_fused_gradient_noalloc(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, const fpm_c32* P_in,
+                               int64_t n_rows, int64_t n_cols, const float* Y, int64_t L, const int32_t* led_idx,
+                               double pratio, fpm_c32* G_O, fpm_c32* G_P, double* loss_out, void* stream) {
+    return run_guarded("fpm_fused_gradient_noalloc", [&] {
+        if (!O_in || !P_in || !Y || !led_idx || !G_O || !G_P) invalid("null array argument");
+        Geom g = make_geom(N_rows, N_cols, n_rows, n_cols);
+        if (g.generic)
+            invalid("fpm_fused_gradient_noalloc takes power-of-two sides only (the generic path allocates its "
+                    "twiddle tables)", FPM_E_UNSUPPORTED);
+        g.eng = ENG_F32;
+        if (L < 1) invalid("L (number of images) must be >= 1");
+        const int pr = (int)pratio;
+        if (pr < 1) invalid("pratio must be >= 1");
+        const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+        auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t spec = up(NN * sizeof(float2)), pages = up(nn * (size_t)L * sizeof(float2));
+        const size_t need = 2 * spec + 2 * pages + up(nn * sizeof(float2));
+        if (need > (size_t)FPM_NOALLOC_SCRATCH_BYTES)
+            invalid("this call needs " + std::to_string(need) + " bytes of scratch (2 N_rows N_cols + 2 L n_rows n_cols + "
+                    "n_rows n_cols complex values); the static scratch holds " +
+                    std::to_string((size_t)FPM_NOALLOC_SCRATCH_BYTES));
+        check_device();
+        int dev = 0;
+        FPM_HIP(hipGetDevice(&dev));
+        unsigned char* base = noalloc_scratch(dev);
+        const float2* tw = twiddle_table(dev);
+        hipStream_t st = (hipStream_t)stream;
+        float2* S0 = (float2*)base;
+        float2* S1 = (float2*)(base + spec);
+        Workspace ws;
+        ws.lat = (float2*)(base + 2 * spec);
+        ws.rec = (float2*)(base + 2 * spec + pages);
+        float2* pcopy = (float2*)(base + 2 * spec + 2 * pages);
+        // P is read-only here; the back-projection takes a mutable pointer
+        FPM_HIP(hipMemcpyAsync(pcopy, P_in, nn * sizeof(float2), hipMemcpyDeviceToDevice, st));
+        // the loss accumulates into the caller's slot: cleared by a fill kernel
+        if (loss_out) launch_zero((float2*)loss_out, 1, st);
+        YStack y{};
+        y.data = Y;
+        y.u16 = 0;
+        StageTimer tm(nullptr, st);
+        fft_contig((const float2*)O_in, S0, g.Nr, g.Nc, false, tw, st, g.eng);
+        fft_cols_forward(S0, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, nullptr, g.eng);
+        minibatch_backward(g, S0, pcopy, (const int2*)led_idx, 0, (uint32_t)L, y, pr, ws, S1, FPM_STITCH_OWNER,
+                           nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f}, (float2*)G_P, loss_out, tw, st, tm, nullptr,
+                           nullptr);
+        fft_contig(S1, (float2*)G_O, g.Nr, g.Nc, true, tw, st, g.eng);
+    });
+}
+
 int fpm_forward_intensity(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, const fpm_c32* P_in, int64_t n_rows,
                           int64_t n_cols, int64_t L, const int32_t* led_idx, double pratio, int32_t squared,
                           float* out, int32_t pointers_are_device, void* stream) {

@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = (
     "fpm_forward_intensity", "fpm_fft2", "fpm_fft2_engine", "fpm_last_error", "fpm_abi_version",
     "fpm_cnn_create", "fpm_cnn_destroy", "fpm_cnn_num_params", "fpm_cnn_get_params", "fpm_cnn_set_params",
     "fpm_cnn_forward", "fpm_cnn_gradient", "fpm_cnn_train_step", "fpm_led_offsets_at", "fpm_tiles_solve",
-    "fpm_parallax_shifts", "fpm_optimizer_step", "fpm_pupil_constraint",
+    "fpm_parallax_shifts", "fpm_optimizer_step", "fpm_pupil_constraint", "fpm_fused_gradient_noalloc",
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -125,6 +125,8 @@ def lib() -> ctypes.CDLL:
     L.fpm_fft2.restype = ctypes.c_int
     L.fpm_fft2_engine.argtypes = [vp, vp, i64, i64, i64, i32, i32, i32, vp]
     L.fpm_fft2_engine.restype = ctypes.c_int
+    L.fpm_fused_gradient_noalloc.argtypes = [vp, i64, i64, vp, i64, i64, vp, i64, vp, dbl, vp, vp, vp, vp]
+    L.fpm_fused_gradient_noalloc.restype = ctypes.c_int
     L.fpm_last_error.argtypes = []
     L.fpm_last_error.restype = ctypes.c_char_p
     L.fpm_abi_version.argtypes = []

@@ -190,6 +190,28 @@ int fpm_fused_gradient(const fpm_c32* O, int64_t N_rows, int64_t N_cols,
                        fpm_c32* G_O, fpm_c32* G_P, double* loss_out,
                        const fpm_opts* opts);
 
+/* fpm_fused_gradient for callers that cannot allocate device memory.  No
+ * counterpart in the reference: it serves code that must not call hipMalloc
+ * (work under HIP graph capture, serving loops, optimizer loops that forbid
+ * allocation, e.g. a gradient-driven solve driving this entry every step).
+ * Every array is a DEVICE pointer: O, P, Y (single amplitudes, as v3 takes),
+ * led_idx, G_O, G_P and the optional loss (one device double, cleared by a
+ * fill kernel, then the FP64 loss sum|Dx e| + |Dy e|).  Power-of-two sides,
+ * FP32 transforms, owner-computes stitch: the same kernels and bits as
+ * fpm_fused_gradient with those options.  Queued on `stream` (NULL: the null
+ * stream) and not waited for.  The working buffers (two N_rows x N_cols
+ * spectra, two sets of L per-LED pages, a pupil copy: 16 (N_rows N_cols +
+ * L n_rows n_cols) + 8 n_rows n_cols bytes, each rounded up to 256) live in a
+ * static device array of FPM_NOALLOC_SCRATCH_BYTES (64 MiB), one per GPU,
+ * held until the process ends: e.g. a 1024^2 object with 26 images of 256^2
+ * fits; larger calls return FPM_E_INVALID.  Calls that overlap on one GPU
+ * (several host threads or streams) race on that array. */
+#define FPM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int fpm_fused_gradient_noalloc(const fpm_c32* O, int64_t N_rows, int64_t N_cols,
+                               const fpm_c32* P, int64_t n_rows, int64_t n_cols,
+                               const float* Y, int64_t L, const int32_t* led_idx, double pratio,
+                               fpm_c32* G_O, fpm_c32* G_P, double* loss, void* stream);
+
 /* Optimizers of the gradient-driven solve (fpm_fused_gradient in a loop, as
  * codes/main_run_FPM.m:106-160 drives fullyfusedFPM): one step on DEVICE
  * arrays of `count` complex elements, in place on param, with the

@@ -313,3 +313,15 @@ def test_parallax_pupil_save_load_roundtrip(tmp_path):
     m2 = ws.ParallaxPupil.load(path)
     assert m2.planes == [0, 2, 5] and m2.iters == 3 and np.array_equal(m2.W, m.W)
 
+
+def test_noalloc_gradient_rejects_a_call_beyond_its_scratch_before_device_work():
+    """fpm_fused_gradient_noalloc checks the call's scratch need against the
+    static array (FPM_NOALLOC_SCRATCH_BYTES) before touching a device: a
+    2048^2 object needs 64 MiB of spectra alone."""
+    L = fpm_amd.lib()
+    buf = np.zeros(16, np.complex64)
+    p = buf.ctypes.data
+    rc = L.fpm_fused_gradient_noalloc(p, 2048, 2048, p, 512, 512, p, 26, p, 4.0, p, p, None, None)
+    assert rc == -1 and "scratch" in L.fpm_last_error().decode()
+    rc = L.fpm_fused_gradient_noalloc(p, 96, 96, p, 48, 48, p, 4, p, 2.0, p, p, None, None)
+    assert rc == -2 and "power-of-two" in L.fpm_last_error().decode()

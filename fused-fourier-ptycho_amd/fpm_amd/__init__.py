@@ -26,7 +26,7 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 __all__ = [
-    "FPMError", "lib", "library_path", "cuFPM_pure_v3", "fused_gradient", "led_offsets",
+    "FPMError", "lib", "library_path", "cuFPM_pure_v3", "solve_FPM", "fused_gradient", "led_offsets",
     "prepare_amplitude", "forward_intensity", "fft2", "solve_device", "gradient_device",
     "STITCH_ATOMIC", "STITCH_OWNER", "Y_F32_AMPLITUDE", "Y_U16_RAW", "EXPORTED_SYMBOLS",
 ]
@@ -211,6 +211,19 @@ def cuFPM_pure_v3(O0, P0, Y, ledIdx, A, pratio, numEpochs, batch, opt: Sequence[
     if loss:
         return O, P, lossb[:ne]
     return O, P
+
+
+def solve_FPM(target, pupil, obsY, ledidx, circ, *, pratio: float = 8, epoch: float = 20, batchSize: float = 26,
+              learningRate: float = 0.0005, beta: float = 0.999, eps: float = 1e-4, **kw):
+    """The MATLAB caller of the mex, solve_FPM(parms) of
+    codes_v2/main_run_FPM.m:77-107, with its name-value defaults (pratio 8,
+    epoch 20, batchSize 26, learningRate 5e-4, beta 0.999, eps 1e-4): returns
+    (w1, w2) = cuFPM_pure_v3(target, pupil, obsY, ledidx, circ, pratio, epoch,
+    batchSize, [learningRate beta eps]); further keywords go to
+    :func:`cuFPM_pure_v3`."""
+    circ = np.asarray(circ, dtype=np.float32)
+    return cuFPM_pure_v3(target, pupil, obsY, ledidx, circ, pratio, epoch, batchSize,
+                         (learningRate, beta, eps), **kw)
 
 
 def fused_gradient(O, P, Y, ledIdx, pratio, *, stitch: int = STITCH_OWNER, engine: int = FFT_F32,

@@ -66,21 +66,27 @@ def export_result(O: np.ndarray, P: np.ndarray, prefix: str, raw_centre: Optiona
 
 
 def psnr(x: np.ndarray, ref: np.ndarray, peak: Optional[float] = None) -> float:
-    """Peak signal-to-noise ratio in dB (peak defaults to max(ref))."""
+    """Peak signal-to-noise ratio in dB.  peak defaults to 1, the range MATLAB's
+    psnr(A, ref) takes for single or double images, as the reference's log
+    calls it (benchmark/cuFPM_recon_2025_circular.m:193: psnr(gt,
+    single(abs(w2))))."""
     x = np.asarray(x, dtype=np.float64)
     ref = np.asarray(ref, dtype=np.float64)
     mse = float(np.mean((x - ref) ** 2))
-    pk = float(ref.max()) if peak is None else peak
+    pk = 1.0 if peak is None else peak
     return float("inf") if mse == 0 else 10.0 * np.log10(pk * pk / mse)
 
 
 def ssim(x: np.ndarray, ref: np.ndarray, data_range: Optional[float] = None) -> float:
-    """Mean SSIM with an 11x11 Gaussian window (sigma 1.5), K1 0.01, K2 0.03."""
+    """Mean SSIM with an 11x11 Gaussian window (sigma 1.5, replicated
+    borders), K1 0.01, K2 0.03.  data_range defaults to 1, the range MATLAB's
+    ssim takes for single or double images (benchmark/cuFPM_recon_2025_
+    circular.m:194)."""
     from scipy.ndimage import gaussian_filter
 
     x = np.asarray(x, dtype=np.float64)
     y = np.asarray(ref, dtype=np.float64)
-    L = float(y.max() - y.min()) if data_range is None else data_range
+    L = 1.0 if data_range is None else data_range
     c1, c2 = (0.01 * L) ** 2, (0.03 * L) ** 2
     f = lambda a: gaussian_filter(a, 1.5, truncate=3.5, mode="nearest")  # 11-tap window
     mx, my = f(x), f(y)

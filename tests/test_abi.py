@@ -325,3 +325,22 @@ def test_noalloc_gradient_rejects_a_call_beyond_its_scratch_before_device_work()
     assert rc == -1 and "scratch" in L.fpm_last_error().decode()
     rc = L.fpm_fused_gradient_noalloc(p, 96, 96, p, 48, 48, p, 4, p, 2.0, p, p, None, None)
     assert rc == -2 and "power-of-two" in L.fpm_last_error().decode()
+
+
+def test_solve_FPM_passes_the_matlab_defaults(monkeypatch):
+    """solve_FPM (codes_v2/main_run_FPM.m:77-107) forwards its name-value
+    defaults pratio 8, epoch 20, batchSize 26 and [5e-4 0.999 1e-4] to the
+    mex call, and overrides by keyword."""
+    seen = {}
+
+    def fake(*args, **kw):
+        seen["args"], seen["kw"] = args, kw
+        return args[0], args[1]
+
+    monkeypatch.setattr(fpm_amd, "cuFPM_pure_v3", fake)
+    O, P = np.zeros((64, 64), np.complex64), np.zeros((16, 16), np.complex64)
+    fpm_amd.solve_FPM(O, P, np.zeros((16, 16, 3), np.float32), np.ones((2, 3), np.int32), np.ones((16, 16)))
+    assert seen["args"][5:8] == (8, 20, 26) and seen["args"][8] == (0.0005, 0.999, 1e-4)
+    assert seen["args"][4].dtype == np.float32
+    fpm_amd.solve_FPM(O, P, None, None, np.ones((16, 16)), batchSize=16, learningRate=1e-3, graph=True)
+    assert seen["args"][7] == 16 and seen["args"][8][0] == 1e-3 and seen["kw"] == {"graph": True}

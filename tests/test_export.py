@@ -36,8 +36,22 @@ def test_psnr_ssim_basic():
     assert abs(export.ssim(ref, ref) - 1.0) < 1e-12
     noisy = ref + 0.01 * rng.standard_normal(ref.shape)
     p = export.psnr(noisy, ref)
-    assert 35 < p < 45  # 10 log10(max^2 / 1e-4) ~ 40 dB
+    assert 35 < p < 45  # 10 log10(1 / 1e-4) ~ 40 dB
     assert export.ssim(noisy, ref) < 1.0
+
+
+def test_psnr_ssim_use_matlabs_range_for_floating_images():
+    """MATLAB's psnr(A, ref) and ssim(A, ref) on single/double images take
+    the class range 1 as peak / dynamic range, whatever the images' own
+    maximum: an offset of 0.1 on an image in [0, 0.5] is 20 dB exactly
+    (10 log10(1 / 0.01)), not 10 log10(0.25 / 0.01)."""
+    rng = np.random.default_rng(3)
+    ref = 0.5 * rng.random((64, 64))
+    assert abs(export.psnr(ref + 0.1, ref) - 20.0) < 1e-9
+    assert abs(export.psnr(ref + 0.1, ref, peak=0.5) - 10 * np.log10(0.25 / 0.01)) < 1e-9
+    x = ref + 0.05 * rng.standard_normal(ref.shape)
+    assert export.ssim(x, ref) == export.ssim(x, ref, data_range=1.0)
+    assert export.ssim(x, ref) != export.ssim(x, ref, data_range=float(ref.max() - ref.min()))
 
 
 def test_epoch_log_save(tmp_path):

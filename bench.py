@@ -254,6 +254,24 @@ def run_time_to_loss(args):
         res[name] = {"mean_ratio_vs_plain": float(np.mean(ratios)), "max_ratio_vs_plain": float(np.max(ratios)),
                      "all_reached": all(p["warm_reaches_plain_loss"] for p in per),
                      "mean_epochs_saved": float(np.mean([p["epochs_saved"] for p in per])), "objects": per}
+    # ablation: what the network and the parallax pupil each buy (FP32 engine)
+    abl = {k: [] for k in warmstart.ABLATION_STARTS}
+    for seed in TTL_TEST_SEEDS:
+        rr, _, _ = warmstart.ablation(net, pupil, n, r, seed)
+        for k in warmstart.ABLATION_STARTS:
+            abl[k].append(dict(rr["starts"][k], plain_final_rel_l2_vs_truth=rr["plain"]["final_rel_l2_vs_truth"]))
+        log(f"[bench] ablation, object {seed}: " + ", ".join(
+            f"{k} {rr['starts'][k]['epochs_run']} ep ratio {rr['starts'][k]['ratio_vs_plain']}"
+            for k in warmstart.ABLATION_STARTS))
+    ablation = {}
+    for k, per in abl.items():
+        ratios = [p["ratio_vs_plain"] for p in per if p["ratio_vs_plain"] is not None]
+        ablation[k] = {"reached": sum(p["reached"] for p in per), "objects": len(per),
+                       "mean_ratio_vs_plain_of_reached": float(np.mean(ratios)) if ratios else None,
+                       "mean_epochs_run": float(np.mean([p["epochs_run"] for p in per])),
+                       "mean_final_rel_l2_vs_truth": float(np.mean([p["final_rel_l2_vs_truth"] for p in per])),
+                       "mean_plain_final_rel_l2_vs_truth": float(np.mean(
+                           [p["plain_final_rel_l2_vs_truth"] for p in per]))}
     net.close()
     return {"workload": f"361 uint16 images {n}x{n} -> {n * r}x{n * r}, batch 26, pupils with 5 Zernike terms "
                         f"~ N(0, {warmstart.PUPIL_SIGMA}) rad, plain 20 epochs (FP32 engine, reference schedule, "
@@ -262,7 +280,8 @@ def run_time_to_loss(args):
             "training": {"objects": args.ttl_samples, "epochs": args.ttl_epochs, "batch": 8, "generate_s": t_gen,
                          "train_s": t_train, "loss_first_last": [hist[0], hist[-1]],
                          "pupil_fit_s": t_fit, "pupil_fit_rms_rad": fit_rms},
-            "warm_fft_engine": res}
+            "warm_fft_engine": res,
+            "ablation_fp32_engine": ablation}
 
 
 def _tile_sizes(args):

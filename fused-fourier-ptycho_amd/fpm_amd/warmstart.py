@@ -496,3 +496,58 @@ def time_to_loss(net: WarmStartNet, n: int, pratio: int, seed: int, *, epochs: i
         "loss_per_epoch": {"plain": [float(v) for v in loss_plain], "warm": [float(v) for v in loss_warm]},
     }
     return out
+
+
+ABLATION_STARTS = ("net_object_ideal_pupil", "plain_object_parallax_pupil", "net_object_parallax_pupil")
+
+
+def ablation(net: WarmStartNet, pupil_model: "ParallaxPupil", n: int, pratio: int, seed: int, *, epochs: int = 20,
+             batch: int = 26, opt=(5e-4, 0.999, 1e-4), engine: int = 0, led_grid: int = 19, repeats: int = 3,
+             pupil_sigma: Optional[float] = PUPIL_SIGMA):
+    """What each part of the warm start buys on one held-out object: the
+    plain solve (plain object, ideal pupil, 20 epochs) sets the target loss;
+    then three starts each run the same solve stopped at that loss -- the
+    network's object with the ideal pupil, the plain object with the parallax
+    pupil, and both (the warm form).  Per start: epochs run, whether the
+    target was reached within the 20 epochs, inference + solve time over the
+    plain solve's time, and the phase-aligned rel-L2 of the final object
+    against the true object.  The objects and pupils of each start are also
+    returned (device tensors) for accuracy checks."""
+    import time
+
+    import torch
+
+    feats, _, ds = make_sample(n, pratio, seed, led_grid=led_grid, pupil_sigma=pupil_sigma)
+    O_true = ds["O_true_dev"].cpu().numpy()
+    t_plain, loss_plain, O_plain, P_plain = min((_timed_solve(ds, ds["O0_dev"], epochs, batch, opt, engine)
+                                                 for _ in range(repeats)), key=lambda r: r[0])
+    stop = float(loss_plain[-1]) * (1.0 + LOSS_RTOL)
+
+    def timed(fn):
+        best, val = float("inf"), None
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            val = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best, val
+
+    t_net, O_net = timed(lambda: warm_start(net, feats, ds["O0_dev"].real)[0])
+    t_par, P_par = timed(lambda: pupil_from_coeffs(pupil_model.predict(feats[None])[0], ds["A_dev"]))
+    starts = {"net_object_ideal_pupil": (O_net, None, t_net),
+              "plain_object_parallax_pupil": (ds["O0_dev"], P_par, t_par),
+              "net_object_parallax_pupil": (O_net, P_par, t_net + t_par)}
+    out = {"seed": seed, "plain": {"s": t_plain, "epochs": epochs, "final_loss": float(loss_plain[-1]),
+                                   "final_rel_l2_vs_truth": phase_aligned_rel_l2(O_plain.cpu().numpy(), O_true)},
+           "starts": {}}
+    results = {"plain": (ds["O0_dev"], ds["P0_dev"], epochs, O_plain, P_plain)}
+    for name, (O0, P0, t_inf) in starts.items():
+        t, loss, Of, Pf = min((_timed_solve(ds, O0, epochs, batch, opt, engine, 0, P0, stop_loss=stop)
+                               for _ in range(repeats)), key=lambda r: r[0])
+        reached = bool(loss[-1] <= stop)
+        out["starts"][name] = {"epochs_run": len(loss), "reached": reached, "inference_s": t_inf, "solve_s": t,
+                               "ratio_vs_plain": (t_inf + t) / t_plain if reached else None,
+                               "final_rel_l2_vs_truth": phase_aligned_rel_l2(Of.cpu().numpy(), O_true)}
+        results[name] = (O0, ds["P0_dev"] if P0 is None else P0, len(loss), Of, Pf)
+    return out, results, ds

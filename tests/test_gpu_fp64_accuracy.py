@@ -167,3 +167,39 @@ def test_short_solve_from_an_aberrated_pupil_as_close_to_fp64_as_the_fp32_oracle
               f"O {_rel(O, O32):.2e} P {_rel(P, P32):.2e}", flush=True)
         f = 1.25 if name == "fp32" else 1.5
         _close_enough(name, e, ref, {"O": f, "P": f}, 5e-3)
+
+
+@pytest.mark.timeout(900)
+def test_warm_start_ablation_starts_as_close_to_fp64_as_the_fp32_oracle():
+    """256^2 x 361 -> 1024^2 (the warm-start configuration), one held-out
+    object: each start of the ablation -- plain object + ideal pupil (20
+    epochs), network object + ideal pupil, plain object + parallax pupil,
+    network object + parallax pupil (each stopped at the plain solve's final
+    loss) -- against the FP64 oracle run from the same start for the same
+    epochs: O and P no further from FP64 than the FP32 oracle (1.5x + 1e-4).
+    The network here is trained briefly (256 objects, 2 epochs): the check
+    is the solve's accuracy from each start, the time-to-loss of each start
+    with the fully trained network is the benchmark's ablation."""
+    n, r, B = 256, 4, 26
+    opt = (5e-4, 0.999, 1e-4)
+    F, T, Z = warmstart.make_training_set(256, n, r, seed0=1000, pupil_sigma=warmstart.PUPIL_SIGMA)
+    net = warmstart.WarmStartNet(n, r, levels=4, base=32, max_batch=8, seed=0)
+    warmstart.train(net, F, T, epochs=2, batch=8, lr=5e-4, lr_final=5e-5)
+    _, _, nai = fpm_amd.led_offsets(n, r, wavelength_um=synth.WAVELENGTH_UM, na=synth.NA, pixel_um=synth.PIXEL_UM,
+                                    led_pitch_um=synth.LED_PITCH_UM, led_height_um=synth.LED_HEIGHT_UM)
+    pupil = warmstart.ParallaxPupil(warmstart.brightfield_planes(nai, synth.NA))
+    pupil.fit(F, Z)
+    del F, T, Z
+    out, res, ds = warmstart.ablation(net, pupil, n, r, 77000, repeats=1)
+    net.close()
+    Y, led, A = _prepared(ds, ds["L"]), np.asfortranarray(ds["led"]), np.asfortranarray(ds["A"])
+    for name, (O0, P0, ep, Of, Pf) in res.items():
+        O0h, P0h = _host(O0), _host(P0)
+        O64, P64, _ = oracle.solve(O0h, P0h, Y, led, A, r, ep, B, opt, prec="f64")
+        O32, P32, _ = oracle.solve(O0h, P0h, Y, led, A, r, ep, B, opt, prec="f32")
+        e = {"O": _rel(_host(Of), O64), "P": _rel(_host(Pf), P64)}
+        ref = {"O": _rel(O32, O64), "P": _rel(P32, P64)}
+        info = out["starts"].get(name, {})
+        print(f"[ablation] {name}: {ep} epochs (reached {info.get('reached', True)}); GPU vs FP64 O {e['O']:.2e} "
+              f"P {e['P']:.2e}; FP32 oracle vs FP64 O {ref['O']:.2e} P {ref['P']:.2e}", flush=True)
+        _close_enough(name, e, ref, {"O": 1.5, "P": 1.5}, 2e-2)

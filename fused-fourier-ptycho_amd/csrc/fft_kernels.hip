@@ -618,12 +618,246 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
     }
 }
 
+// ------------- 16384-point lines, two workgroups per CU (half-size exchange)
+// A 16384-point line exchanged through LDS in one piece takes 136 KiB, so
+// only one workgroup fits a CU and its load, transform and store phases
+// cannot overlap another's (a contiguous pass of 16384-point lines measured
+// 3.9 TB/s against 5.2 TB/s for 8192-point lines, two workgroups per CU).
+// Here a 512-thread workgroup runs the 1024-thread slot maps of the
+// one-piece kernel for two "virtual threads" per thread (A = t, B = t + 512)
+// and exchanges through a buffer of half a line, in two phases: in every
+// radix-16 write of both plans (inverse 4,16,16,16; forward 16,16,16,4)
+// virtual threads below 512 write exactly the indices below 8192, and in
+// the radix-4 write (inverse pass 0) groups 0-1 do; every read takes points
+// s < R/2 from the lower half.  Phase 0 moves the lower half, phase 1 the
+// upper, so 71 KiB of LDS per workgroup and two workgroups per CU.  Same
+// slots, twiddles and butterflies as the one-piece kernel (its arithmetic,
+// up to where the compiler contracts products into FMAs).  Measured at
+// 16384^2, batch 26: 2.13 -> 1.88 ms per launch.
+#ifndef FPM_FUSED16K_HALF
+#define FPM_FUSED16K_HALF 1
+#endif
+namespace half16k {
+constexpr int LEN = 16384, TV = 1024, HALF = LEN / 2;
+using HMap = LineSlowMap<1, HALF>;  // the half buffer (one pad word per 16)
+template <bool INV, int P> struct PassG {
+    static constexpr int R = Plan<LEN, INV>::radix(P), L = Plan<LEN, INV>::span(P), NB = LEN / R, NG = 16 / R;
+};
+__device__ __forceinline__ int hofs(int idx) { return HMap::lds(0, idx & (HALF - 1)); }
+// the thread index through an empty asm: index arithmetic that depends on it
+// is formed where it is used instead of hoisted to the kernel start and kept
+// (spilled) across the transform
+__device__ __forceinline__ int fresh(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// writes of pass P's outputs of virtual thread tv, groups [q0, q1)
+template <bool INV, int P>
+__device__ __forceinline__ void write_groups(const float2 (&v)[16], float2* lds, int tv, int q0, int q1) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        if (q < q0 || q >= q1) continue;
+        const int j = tv + TV * q;
+        const int base = (j / G::L) * G::L * G::R + (j % G::L);
+#pragma unroll
+        for (int s = 0; s < G::R; ++s) lds[hofs(base + s * G::L)] = v[q * G::R + s];
+    }
+}
+// reads of pass P's inputs s in [s0, s1) of virtual thread tv into d[q * R + s]
+template <bool INV, int P>
+__device__ __forceinline__ void read_points(float2 (&d)[16], const float2* lds, int tv, int s0, int s1) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        const int j = tv + TV * q;
+#pragma unroll
+        for (int s = 0; s < G::R; ++s)
+            if (s >= s0 && s < s1) d[q * G::R + s] = lds[hofs(j + s * G::NB)];
+    }
+}
+// twiddles and DFT of pass P on inputs already in v (pass_from_lds without the reads)
+template <bool INV, int P>
+__device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles& tw) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        if constexpr (G::L > 1) {
+            const uint32_t k = (uint32_t)((tv + TV * q) % G::L);
+            constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+            float2 w[16];
+            twiddle_powers<G::R, INV>(tw, k * step, w);
+#pragma unroll
+            for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], w[s]);
+        }
+        dft<G::R, INV>(&v[q * G::R]);
+    }
+}
+// outputs of pass P (in va / vb) -> inputs of pass P + 1, through the half
+// buffer; the caller's LDS reads of any earlier exchange must be complete
+template <bool INV, int P>
+__device__ __forceinline__ void exchange(float2 (&va)[16], float2 (&vb)[16], float2* lds, int t) {
+    using W = PassG<INV, P>;
+    using Rd = PassG<INV, P + 1>;
+    constexpr int H = Rd::R / 2;  // read points s < H sit in the lower half
+    const int ta = t, tb = t + TV / 2;
+    if constexpr (W::R == 4) {
+        // groups 0-1 of both virtual threads hold the lower half
+        write_groups<INV, P>(va, lds, ta, 0, 2);
+        write_groups<INV, P>(vb, lds, tb, 0, 2);
+        __syncthreads();
+        static_assert(Rd::R == 16 && Rd::NG == 1, "inverse pass 1 is radix 16");
+        read_points<INV, P + 1>(va, lds, ta, 0, H);  // registers 0-7: the groups just written
+        read_points<INV, P + 1>(vb, lds, tb, 0, H);
+        __syncthreads();
+        write_groups<INV, P>(va, lds, ta, 2, 4);
+        write_groups<INV, P>(vb, lds, tb, 2, 4);
+        __syncthreads();
+        read_points<INV, P + 1>(va, lds, ta, H, Rd::R);
+        read_points<INV, P + 1>(vb, lds, tb, H, Rd::R);
+    } else {
+        static_assert(W::R == 16, "radix-16 writes");
+        // A (t < 512) writes the lower half, B the upper
+        float2 tmp[16];
+        write_groups<INV, P>(va, lds, ta, 0, 1);
+        __syncthreads();
+        read_points<INV, P + 1>(va, lds, ta, 0, H);   // va is written out: free
+        read_points<INV, P + 1>(tmp, lds, tb, 0, H);  // vb still holds B's outputs
+        __syncthreads();
+        write_groups<INV, P>(vb, lds, tb, 0, 1);
+        __syncthreads();
+        read_points<INV, P + 1>(va, lds, ta, H, Rd::R);
+        read_points<INV, P + 1>(vb, lds, tb, H, Rd::R);
+#pragma unroll
+        for (int q = 0; q < Rd::NG; ++q)
+#pragma unroll
+            for (int s = 0; s < H; ++s) vb[q * Rd::R + s] = tmp[q * Rd::R + s];
+    }
+}
+}  // namespace half16k
+
+template <bool FWD>
+__global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restrict__ G, float2* __restrict__ O,
+                                                                 float* __restrict__ mO, uint32_t nlines, RmsArgs rms,
+                                                                 const uint8_t* __restrict__ rowmask,
+                                                                 uint32_t lines_per_tile, Twiddles tw) {
+    using namespace half16k;
+    using MapG = LineSlowMap<1, LEN>;  // slot arithmetic of the one-piece kernel (no LDS)
+    using SI = Slots<LEN, true, TV, MapG, ENG_F32>;
+    using SF = Slots<LEN, false, TV, MapG, ENG_F32>;
+    __shared__ float2 lds[HMap::LDS_WORDS];
+    __shared__ float2 twl[2 * kTwLo];
+    __shared__ uint8_t rm[LEN / kMaskRows];
+    const int t = threadIdx.x;
+    const size_t line = blockIdx.x;  // one line per workgroup; the grid is nlines wide
+    const uint8_t* rmk = rowmask;
+    if (rmk && lines_per_tile) rmk += (line / lines_per_tile) * (size_t)(LEN / kMaskRows);
+    for (int k = t; k < LEN / kMaskRows; k += 512) rm[k] = rmk ? rmk[k] : 1;
+    tw = bind_twiddles(tw, twl, t, 512);
+    const float lr = rms_lr(rms);
+    float2 va[16], vb[16];
+    auto load_g = [&](float2 (&v)[16], int tv) {
+#pragma unroll
+        for (int q = 0; q < SI::NG0; ++q)
+#pragma unroll
+            for (int s = 0; s < SI::GS0; ++s) {
+                int ln, idx;
+                SI::in(tv, q, s, ln, idx);
+                v[q * SI::GS0 + s] = rm[idx / kMaskRows] ? G[line * LEN + idx] : make_float2(0.f, 0.f);
+            }
+    };
+    load_g(va, t);
+    load_g(vb, t + 512);
+    first_pass_dft<LEN, true, TV>(va);
+    first_pass_dft<LEN, true, TV>(vb);
+    exchange<true, 0>(va, vb, lds, t);
+    compute<true, 1>(va, t, tw);
+    compute<true, 1>(vb, t + 512, tw);
+    __syncthreads();
+    exchange<true, 1>(va, vb, lds, t);
+    compute<true, 2>(va, t, tw);
+    compute<true, 2>(vb, t + 512, tw);
+    __syncthreads();
+    exchange<true, 2>(va, vb, lds, t);
+    compute<true, 3>(va, t, tw);
+    compute<true, 3>(vb, t + 512, tw);
+    // RMSprop of O (cuFPM_v3/cuda/kernels.cuh:364-382) on the inverse's outputs
+    auto update = [&](float2 (&v)[16], int tv) {
+        tv = fresh(tv);
+#pragma unroll
+        for (int q = 0; q < SI::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SI::GSL; ++s) {
+                int ln, idx;
+                SI::out(tv, q, s, ln, idx);
+                const size_t k = line * LEN + idx;
+                const float2 gr = v[q * SI::GSL + s];
+                constexpr bool NT = FPM_NT_UPDATE >= 1;
+                float mu = gr.x * gr.x + gr.y * gr.y;
+                mu = rms.beta * ld_s<NT>(mO + k) + (1.0f - rms.beta) * mu;
+                const float sq = lr / (sqrtf(mu) + rms.eps);
+                float2 w = ld_s<NT>(O + k);
+                w.x -= gr.x * sq;
+                w.y -= gr.y * sq;
+                st_s<NT>(O + k, w);
+                st_s<NT>(mO + k, mu);
+                v[q * SI::GSL + s] = w;
+            }
+    };
+    update(va, t);
+    update(vb, t + 512);
+    if constexpr (FWD) {
+        first_pass_dft<LEN, false, TV>(va);
+        first_pass_dft<LEN, false, TV>(vb);
+        __syncthreads();
+        exchange<false, 0>(va, vb, lds, t);
+        compute<false, 1>(va, t, tw);
+        compute<false, 1>(vb, t + 512, tw);
+        __syncthreads();
+        exchange<false, 1>(va, vb, lds, t);
+        compute<false, 2>(va, t, tw);
+        compute<false, 2>(vb, t + 512, tw);
+        __syncthreads();
+        exchange<false, 2>(va, vb, lds, t);
+        compute<false, 3>(va, t, tw);
+        compute<false, 3>(vb, t + 512, tw);
+        auto store = [&](const float2 (&v)[16], int tv) {
+            tv = fresh(tv);
+#pragma unroll
+            for (int q = 0; q < SF::NGL; ++q)
+#pragma unroll
+                for (int s = 0; s < SF::GSL; ++s) {
+                    int ln, idx;
+                    SF::out(tv, q, s, ln, idx);
+                    st_s<(FPM_NT_UPDATE >= 2)>(G + line * LEN + idx, v[q * SF::GSL + s]);
+                }
+        };
+        store(va, t);
+        store(vb, t + 512);
+    }
+    (void)nlines;
+}
+
 template <int ENG>
 static void launch_fused_update_eng(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
                                     const uint8_t* rowmask, const float2* twid, hipStream_t st, uint32_t ntiles) {
     Twiddles tw{twid};
     const uint32_t nlines = g.Nc * ntiles;
     const uint32_t lpt = ntiles > 1 ? g.Nc : 0u;
+    if (FPM_FUSED16K_HALF && ENG == ENG_F32 && g.Nr == 16384) {
+        if (forward_after)
+            hipLaunchKernelGGL((fused_update16k_kernel<true>), dim3(nlines), dim3(512), 0, st, G, O, mO, nlines, rms,
+                               rowmask, lpt, tw);
+        else
+            hipLaunchKernelGGL((fused_update16k_kernel<false>), dim3(nlines), dim3(512), 0, st, G, O, mO, nlines, rms,
+                               rowmask, lpt, tw);
+        FPM_LAUNCH_CHECK();
+        return;
+    }
 #define FPM_FUSED_CASE(L)                                                                                   \
     case L: {                                                                                               \
         dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                                \

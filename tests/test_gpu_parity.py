@@ -366,3 +366,24 @@ def test_synthetic_generator_is_reproducible():
     del a
     b = synth.make_device(512, 8)
     assert torch.equal(ra, b["raw_dev"].cpu()) and torch.equal(oa, b["O0_dev"].cpu())
+
+
+@pytest.mark.parametrize("epochs,lr,tol", [(1, 2e-3, 1e-6), (2, 2e-4, 1e-5)])
+def test_solve_with_16384_point_lines_matches_fp64(epochs, lr, tol):
+    """An object of 16384 x 256 (pupil 2048 x 32, r 8) on margin inputs, one
+    minibatch per epoch: the object update runs on 16384-point lines (two
+    512-thread workgroups per CU, half-line LDS exchange).  One step (the
+    update without a following transform) and two steps (the first update
+    also forms the next step's spectrum, which the second step's gradient
+    reads) against the FP64 oracle."""
+    from helpers import rect_margin_problem
+    O, P, Y, led, A = rect_margin_problem(2048, 32, 8, 6)
+    GO, GP, _, margin = oracle.gradient(O, P, Y, led, 8)
+    assert margin > 5e-3
+    L = led.shape[1]
+    opt = (lr, 0.999, 1e-4)
+    O1, P1, lg = fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, 8, epochs, L, opt, loss=True)
+    Oo, Po, lo = oracle.solve(O, P, Y, led, A, 8, epochs, L, opt, prec="f64")
+    print(f"[16384-lines] {epochs} epochs: O {rel_l2(O1, Oo):.2e} P {rel_l2(P1, Po):.2e}", flush=True)
+    assert rel_l2(O1, Oo) < tol and rel_l2(P1, Po) < tol
+    np.testing.assert_allclose(lg, lo, rtol=1e-5)

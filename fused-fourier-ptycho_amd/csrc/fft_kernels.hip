@@ -23,6 +23,14 @@ namespace fpm {
 // ------------------------------------------------------- contiguous pass
 constexpr int contig_threads(int len) { return len >= 16384 ? 1024 : (len >= 8192 ? 512 : 256); }
 constexpr int contig_lines(int len) { return contig_threads(len) * 16 / len; }
+// the thread index through an empty asm: index arithmetic that depends on it
+// is formed where it is used instead of hoisted to the kernel start and kept
+// (spilled) across the transform
+__device__ __forceinline__ int fresh(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 template <int LEN> struct ContigCfg {
     static constexpr int T = contig_threads(LEN);
     static constexpr int LINES = contig_lines(LEN);
@@ -644,13 +652,6 @@ template <bool INV, int P> struct PassG {
     static constexpr int R = Plan<LEN, INV>::radix(P), L = Plan<LEN, INV>::span(P), NB = LEN / R, NG = 16 / R;
 };
 __device__ __forceinline__ int hofs(int idx) { return HMap::lds(0, idx & (HALF - 1)); }
-// the thread index through an empty asm: index arithmetic that depends on it
-// is formed where it is used instead of hoisted to the kernel start and kept
-// (spilled) across the transform
-__device__ __forceinline__ int fresh(int v) {
-    asm volatile("" : "+v"(v));
-    return v;
-}
 
 // writes of pass P's outputs of virtual thread tv, groups [q0, q1)
 template <bool INV, int P>
@@ -1007,12 +1008,13 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
     float* ef = (float*)lds;
     float* mf = ef + Cfg::LT * LEN;
     static_assert(2 * Cfg::LT * LEN <= 2 * Map::LDS_WORDS, "e and |z| must fit the exchange buffer");
+    int tt = fresh(t);
 #pragma unroll
     for (int q = 0; q < SI::NGL; ++q)
 #pragma unroll
         for (int s = 0; s < SI::GSL; ++s) {
             int line, idx;
-            SI::out(t, q, s, line, idx);
+            SI::out(tt, q, s, line, idx);
             const float2 z = v[q * SI::GSL + s];
             const float mag = sqrtf(z.x * z.x + z.y * z.y);
             if constexpr (FPM_STENCIL_MAG_LDS) mf[line * LEN + idx] = mag;
@@ -1020,12 +1022,13 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
         }
     __syncthreads();
     float loss32 = 0.0f;  // 32 terms per thread at most; the workgroup sum is FP64
+    tt = fresh(t);
 #pragma unroll
     for (int q = 0; q < SI::NGL; ++q)
 #pragma unroll
         for (int s = 0; s < SI::GSL; ++s) {
             int line, idx;
-            SI::out(t, q, s, line, idx);
+            SI::out(tt, q, s, line, idx);
             float2 o = make_float2(0.f, 0.f);
             if (interior(line)) {
                 const float* e = ef + line * LEN;
@@ -1051,12 +1054,13 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
     first_pass_dft<LEN, false, Cfg::T, ENG>(v, ctx);
     __syncthreads();
     remaining_passes<LEN, false, Cfg::T, Map, ENG>(v, lds, t, tw, ctx);
+    tt = fresh(t);
 #pragma unroll
     for (int q = 0; q < SF::NGL; ++q)
 #pragma unroll
         for (int s = 0; s < SF::GSL; ++s) {
             int line, idx;
-            SF::out(t, q, s, line, idx);
+            SF::out(tt, q, s, line, idx);
             if (interior(line)) dst[at(idx, line)] = v[q * SF::GSL + s];
         }
     if (loss_acc) {

@@ -1077,6 +1077,300 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
     }
 }
 
+// -------- the fused stencil on 2048-point columns, two workgroups per CU
+// stencil_fused_kernel<2048> keeps 8 columns of 2048 points in LDS (136 KiB):
+// one workgroup per CU.  Here 512 threads run its 1024-thread slot maps for
+// two virtual threads each (A = t, B = t + 512) and exchange through half
+// the LDS, 4 columns at a time: in its radix-16 passes A owns columns 0-3
+// and B columns 4-7, in its radix-8 passes group 0 of both owns columns 0-3
+// and group 1 columns 4-7, so phase 0 of every exchange moves columns 0-3
+// and phase 1 columns 4-7; the residuals e of all 8 columns (64 KiB of
+// floats) fit the same half buffer for the stencil.  Same slots, twiddles,
+// butterflies and stencil as stencil_fused_kernel<2048> (whose arithmetic
+// it is, up to where the compiler contracts products into FMAs).  121
+// VGPRs, no spills, 70 KiB LDS; on 26 pages of 2048^2 points 1.04 -> 0.88
+// ms per launch.
+#ifndef FPM_STENCIL_HALF
+#define FPM_STENCIL_HALF 1
+#endif
+namespace halfst {
+constexpr int LEN = 2048, TV = 1024, NLINES = 8;
+using FMap = LineSlowMap<NLINES, LEN>;  // the slot maps of stencil_fused_kernel<2048>
+using HMap = LineSlowMap<NLINES / 2, LEN>;
+template <bool INV, int P> struct PassG {
+    static constexpr int R = Plan<LEN, INV>::radix(P), L = Plan<LEN, INV>::span(P), NB = LEN / R, NG = 16 / R;
+};
+__device__ __forceinline__ int hofs(int line, int idx) { return HMap::lds(line & (NLINES / 2 - 1), idx); }
+
+template <bool INV, int P>
+__device__ __forceinline__ void write_groups(const float2 (&v)[16], float2* lds, int tv, int q0, int q1) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        if (q < q0 || q >= q1) continue;
+        const int b = tv + TV * q, line = b / G::NB, j = b % G::NB;
+        const int base = (j / G::L) * G::L * G::R + (j % G::L);
+#pragma unroll
+        for (int s = 0; s < G::R; ++s) lds[hofs(line, base + s * G::L)] = v[q * G::R + s];
+    }
+}
+// inputs of pass P of groups [q0, q1), points [s0, s1), into d[q * R + s]
+template <bool INV, int P>
+__device__ __forceinline__ void read_groups(float2* d, const float2* lds, int tv, int q0, int q1, int s0, int s1) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        if (q < q0 || q >= q1) continue;
+        const int b = tv + TV * q, line = b / G::NB, j = b % G::NB;
+#pragma unroll
+        for (int s = 0; s < G::R; ++s)
+            if (s >= s0 && s < s1) d[q * G::R + s] = lds[hofs(line, j + s * G::NB)];
+    }
+}
+template <bool INV, int P>
+__device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles& tw) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        if constexpr (G::L > 1) {
+            const int j = (tv + TV * q) % G::NB;
+            const uint32_t k = (uint32_t)(j % G::L);
+            constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+            float2 w[16];
+            twiddle_powers<G::R, INV>(tw, k * step, w);
+#pragma unroll
+            for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], w[s]);
+        }
+        dft<G::R, INV>(&v[q * G::R]);
+    }
+}
+// outputs of pass P -> inputs of pass P + 1 through the half buffer; the LDS
+// reads of any earlier use of the buffer must be complete
+template <bool INV, int P>
+__device__ __forceinline__ void exchange(float2 (&va)[16], float2 (&vb)[16], float2* lds, int t) {
+    using W = PassG<INV, P>;
+    using Rd = PassG<INV, P + 1>;
+    const int ta = t, tb = t + TV / 2;
+    if constexpr (W::R == 16 && Rd::R == 16) {
+        // A's columns 0-3, then B's columns 4-7
+        write_groups<INV, P>(va, lds, ta, 0, 1);
+        __syncthreads();
+        read_groups<INV, P + 1>(va, lds, ta, 0, 1, 0, 16);
+        __syncthreads();
+        write_groups<INV, P>(vb, lds, tb, 0, 1);
+        __syncthreads();
+        read_groups<INV, P + 1>(vb, lds, tb, 0, 1, 0, 16);
+    } else if constexpr (W::R == 8) {
+        // radix-8 writers: group 0 of both holds columns 0-3; A reads all 16
+        // of its (radix-16) inputs there, 8 of them into registers whose
+        // group-1 outputs are still to be written
+        static_assert(Rd::R == 16, "inverse pass 1 is radix 16");
+        float2 tmp[16];
+        write_groups<INV, P>(va, lds, ta, 0, 1);
+        write_groups<INV, P>(vb, lds, tb, 0, 1);
+        __syncthreads();
+        read_groups<INV, P + 1>(va, lds, ta, 0, 1, 0, 8);
+        read_groups<INV, P + 1>(tmp, lds, ta, 0, 1, 8, 16);
+        __syncthreads();
+        write_groups<INV, P>(va, lds, ta, 1, 2);
+        write_groups<INV, P>(vb, lds, tb, 1, 2);
+        __syncthreads();
+        read_groups<INV, P + 1>(vb, lds, tb, 0, 1, 0, 16);
+#pragma unroll
+        for (int s = 8; s < 16; ++s) va[s] = tmp[s];
+    } else {
+        // radix-16 writers, radix-8 readers: A writes columns 0-3, where group
+        // 0 of both reads; B's outputs wait in vb
+        static_assert(W::R == 16 && Rd::R == 8, "forward pass 2 is radix 8");
+        float2 tmp[16];
+        write_groups<INV, P>(va, lds, ta, 0, 1);
+        __syncthreads();
+        read_groups<INV, P + 1>(va, lds, ta, 0, 1, 0, 8);
+        read_groups<INV, P + 1>(tmp, lds, tb, 0, 1, 0, 8);
+        __syncthreads();
+        write_groups<INV, P>(vb, lds, tb, 0, 1);
+        __syncthreads();
+        read_groups<INV, P + 1>(va, lds, ta, 1, 2, 0, 8);
+        read_groups<INV, P + 1>(vb, lds, tb, 1, 2, 0, 8);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) vb[s] = tmp[s];
+    }
+}
+}  // namespace halfst
+
+__global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __restrict__ in, float2* __restrict__ out,
+                                                             Geom g, YStack y, uint32_t led0, float ratio,
+                                                             float inv_ratio, float gscale,
+                                                             double* __restrict__ loss_acc,
+                                                             const uint8_t* __restrict__ rowmask, Twiddles tw,
+                                                             const TileIn* __restrict__ tab, uint32_t ppt) {
+    using namespace halfst;
+    using SI = Slots<LEN, true, TV, FMap, ENG_F32>;
+    using SF = Slots<LEN, false, TV, FMap, ENG_F32>;
+    constexpr int K = NLINES - 2;  // interior columns
+    __shared__ float2 lds[HMap::LDS_WORDS];
+    __shared__ float2 twl[2 * kTwLo];
+    __shared__ double red[512 / 64];
+    __shared__ uint8_t rm[LEN / kMaskRows];
+    const int t = threadIdx.x;
+    const uint32_t page = blockIdx.y;
+    uint32_t lpage = page;
+    if (tab) {
+        const uint32_t tk = page / ppt;
+        lpage = page - tk * ppt;
+        const TileIn ti = tab[tk];
+        y.data = ti.y;
+        y.mn = ti.ymn;
+        y.range = ti.yrange;
+        y.rcp_range = ti.yrcp;
+        loss_acc = ti.loss;
+        if (rowmask) rowmask += (size_t)tk * (LEN / kMaskRows);
+    }
+    for (int k = t; k < LEN / kMaskRows; k += 512) rm[k] = rowmask ? rowmask[k] : 1;
+    tw = bind_twiddles(tw, twl, t, 512);
+    const uint32_t j0 = blockIdx.x * K;  // first interior column; line l is column j0 - 1 + l
+    const uint32_t cmask = g.nc - 1;
+    const size_t plane = (size_t)g.nr * g.nc;
+    const float2* __restrict__ src = in + (size_t)page * plane;
+    float2* __restrict__ dst = out + (size_t)page * plane;
+    const size_t ybase = (size_t)(led0 + lpage) * plane;
+    auto column = [&](int line) -> uint32_t { return (j0 - 1u + (uint32_t)line) & cmask; };
+    auto interior = [&](int line) -> bool { return line >= 1 && line <= K && j0 + (uint32_t)line - 1u < g.nc; };
+    auto at = [&](int idx, int line) -> size_t { return g.nl.row_off((uint32_t)idx) + (size_t)column(line) * g.nl.ld; };
+
+    float2 va[16], vb[16];
+    auto load = [&](float2 (&v)[16], int tv) {
+        tv = fresh(tv);
+#pragma unroll
+        for (int q = 0; q < SI::NG0; ++q)
+#pragma unroll
+            for (int s = 0; s < SI::GS0; ++s) {
+                int line, idx;
+                SI::in(tv, q, s, line, idx);
+                v[q * SI::GS0 + s] = rm[idx / kMaskRows] ? src[at(idx, line)] : make_float2(0.f, 0.f);
+            }
+    };
+    load(va, t);
+    load(vb, t + 512);
+    first_pass_dft<LEN, true, TV>(va);
+    first_pass_dft<LEN, true, TV>(vb);
+    exchange<true, 0>(va, vb, lds, t);
+    compute<true, 1>(va, t, tw);
+    compute<true, 1>(vb, t + 512, tw);
+    __syncthreads();
+    exchange<true, 1>(va, vb, lds, t);
+    compute<true, 2>(va, t, tw);
+    compute<true, 2>(vb, t + 512, tw);
+    __syncthreads();
+    // e of every column of the workgroup, in LDS (the half buffer as floats)
+    float* ef = (float*)lds;
+    static_assert(NLINES * LEN <= 2 * HMap::LDS_WORDS, "e must fit the half buffer");
+    // Y of this page through a global pointer and 32-bit offsets (a page
+    // holds 2^22 points), the uniform stack-type branch outside the loop
+    using GU16 = const __attribute__((address_space(1))) uint16_t*;
+    using GF32 = const __attribute__((address_space(1))) float*;
+    const char* ypage = (const char*)y.data + ybase * (y.u16 ? 2 : 4);
+    auto residual = [&](const float2 (&v)[16], int tv) {
+        tv = fresh(tv);
+        auto one = [&](auto u16) {
+#pragma unroll
+            for (int q = 0; q < SI::NGL; ++q)
+#pragma unroll
+                for (int s = 0; s < SI::GSL; ++s) {
+                    int line, idx;
+                    SI::out(tv, q, s, line, idx);
+                    const uint32_t off = column(line) * (uint32_t)LEN + (uint32_t)idx;
+                    float yv;
+                    if constexpr (decltype(u16)::value) {
+                        const uint32_t raw = ((GU16)ypage)[off];
+                        yv = sqrtf(div_u16((float)(raw - y.mn), y.range, y.rcp_range));
+                    } else {
+                        yv = ((GF32)ypage)[off];
+                    }
+                    const float2 z = v[q * SI::GSL + s];
+                    const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                    ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) - yv;
+                }
+        };
+        if (y.u16)
+            one(std::true_type{});
+        else
+            one(std::false_type{});
+    };
+    residual(va, t);
+    residual(vb, t + 512);
+    __syncthreads();
+    // FP32 partial sums of 32 terms per virtual thread, as in the 1024-thread kernel
+    float loss_a = 0.0f, loss_b = 0.0f;
+    auto stencil = [&](float2 (&v)[16], int tv, float& loss32) {
+        tv = fresh(tv);
+#pragma unroll
+        for (int q = 0; q < SI::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SI::GSL; ++s) {
+                int line, idx;
+                SI::out(tv, q, s, line, idx);
+                float2 o = make_float2(0.f, 0.f);
+                if (interior(line)) {
+                    const float* e = ef + line * LEN;
+                    const float e0 = e[idx];
+                    const float el = e[idx - LEN], er = e[idx + LEN];
+                    const float eu = e[(idx - 1) & (LEN - 1)], ed = e[(idx + 1) & (LEN - 1)];
+                    const float gval = (copysignf(1.0f, e0 - el) - copysignf(1.0f, er - e0) +
+                                        copysignf(1.0f, e0 - eu) - copysignf(1.0f, ed - e0)) * gscale;
+                    loss32 += fabsf(er - e0) + fabsf(ed - e0);
+                    const float2 z = v[q * SI::GSL + s];
+                    const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                    if (mag > 0.0f) {
+                        const float inv = 1.0f / mag;
+                        o = make_float2(z.x * inv * gval, z.y * inv * gval);
+                    }
+                }
+                v[q * SI::GSL + s] = o;
+            }
+    };
+    stencil(va, t, loss_a);
+    stencil(vb, t + 512, loss_b);
+    first_pass_dft<LEN, false, TV>(va);
+    first_pass_dft<LEN, false, TV>(vb);
+    __syncthreads();
+    exchange<false, 0>(va, vb, lds, t);
+    compute<false, 1>(va, t, tw);
+    compute<false, 1>(vb, t + 512, tw);
+    __syncthreads();
+    exchange<false, 1>(va, vb, lds, t);
+    compute<false, 2>(va, t, tw);
+    compute<false, 2>(vb, t + 512, tw);
+    auto store = [&](const float2 (&v)[16], int tv) {
+        tv = fresh(tv);
+#pragma unroll
+        for (int q = 0; q < SF::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SF::GSL; ++s) {
+                int line, idx;
+                SF::out(tv, q, s, line, idx);
+                if (interior(line)) dst[at(idx, line)] = v[q * SF::GSL + s];
+            }
+    };
+    store(va, t);
+    store(vb, t + 512);
+    if (loss_acc) {
+        double loss = (double)loss_a + (double)loss_b;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+        if ((t & 63) == 0) red[t >> 6] = loss;
+        __syncthreads();
+        if (t == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < 512 / 64; ++w) sum += red[w];
+            unsafeAtomicAdd(loss_acc, sum);
+        }
+    }
+}
+
 bool stencil_fused_supported(const Geom& g) { return g.nr <= 2048; }
 
 template <int ENG, bool SIGNS>
@@ -1088,6 +1382,13 @@ static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const 
     const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
     const float gscale = (float)(pratio * pratio);
     const float inv_ratio = exact_inverse_or_zero(ratio);
+    if (FPM_STENCIL_HALF && ENG == ENG_F32 && !SIGNS && g.nr == 2048 && g.nc != (uint32_t)StencilCfg<2048>::LT) {
+        dim3 grid((g.nc + 5) / 6, b);
+        hipLaunchKernelGGL(stencil_half_kernel, grid, dim3(512), 0, st, z_cols, u_out, g, y, led0, ratio, inv_ratio,
+                           gscale, loss_acc, rowmask, tw, tab, ppt);
+        FPM_LAUNCH_CHECK();
+        return;
+    }
 #define FPM_STF_CASE(L)                                                                                   \
     case L: {                                                                                             \
         if (g.nc == (uint32_t)StencilCfg<L>::LT) {                                                        \

@@ -17,6 +17,8 @@ Also reported:
              event-timed duration, against MI355X's 8 TB/s HBM peak and a
              plain device copy timed here, and its measured HBM traffic
              (rocprofv3 counters, copy-calibrated, from profiles/);
+  fft_reference the library's 2D DFT beside rocFFT (torch.fft) on the
+             16384^2 spectrum and 26 pages of 2048^2, same buffers and stream;
   cpu_baseline the FP32 CPU restatement (oracle, OpenMP) on the host cores:
              one epoch of the same workload, plus the 128^2 -> 512^2 case over
              20 epochs and the 512^2 -> 4096^2 case over 2 epochs.
@@ -133,6 +135,53 @@ def measured_traffic(stage, n, N):
         if len(cells) == 4 and cells[0].endswith(kern):
             return (float(cells[2]) + float(cells[3])) * 1e9
     return None
+
+
+def run_fft_reference(stream, reps=5):
+    """The library's unnormalised 2D DFT beside rocFFT (torch.fft) on the same
+    device buffers and stream: the object spectrum (16384^2) and one
+    minibatch of per-LED pages (26 x 2048^2), forward, best of `reps`, HIP
+    events on the stream both run on; also their relative L2 distance."""
+    import torch
+
+    import fpm_amd
+
+    res = {}
+    for rows, cols, batch in ((16384, 16384, 1), (2048, 2048, 26)):
+        g = torch.Generator(device="cuda:0").manual_seed(7)
+        x = torch.complex(torch.randn((batch, cols, rows), generator=g, device="cuda:0"),
+                          torch.randn((batch, cols, rows), generator=g, device="cuda:0"))
+        y = torch.empty_like(x)
+
+        def ours():
+            fpm_amd._check(fpm_amd.lib().fpm_fft2_engine(x.data_ptr(), y.data_ptr(), rows, cols, batch, -1,
+                                                         fpm_amd.FFT_F32, 1, stream.cuda_stream))
+
+        def timed(fn):
+            fn()
+            best = float("inf")
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            return best
+
+        with torch.cuda.stream(stream):
+            ms_ours = timed(ours)
+            ref = torch.fft.fft2(x)
+            ms_roc = timed(lambda: torch.fft.fft2(x))
+            rel = ((y - ref).abs().double().pow(2).sum() / ref.abs().double().pow(2).sum()).sqrt().item()
+        moved = 2.0 * x.numel() * 8  # one read + one write of the pages
+        res[f"{batch}x{rows}x{cols}"] = {"ms": round(ms_ours, 4), "rocfft_ms": round(ms_roc, 4),
+                                         "GBps_read_write_once": round(moved / 1e6 / ms_ours, 1),
+                                         "rocfft_GBps_read_write_once": round(moved / 1e6 / ms_roc, 1),
+                                         "rel_l2_vs_rocfft": rel}
+        del x, y, ref
+        torch.cuda.empty_cache()
+    return res
 
 
 def _cpu_model():
@@ -398,6 +447,7 @@ def main():
     ap.add_argument("--ttl-samples", type=int, default=4096, help="synthetic training objects of the warm start")
     ap.add_argument("--ttl-epochs", type=int, default=8, help="training epochs of the warm start")
     ap.add_argument("--no-tiles", action="store_true", help="skip the multi-tile leg")
+    ap.add_argument("--no-fft-reference", action="store_true", help="skip the rocFFT (torch.fft) comparison")
     ap.add_argument("--tile-epochs", type=int, default=10, help="epochs of the multi-tile leg")
     ap.add_argument("--tile-sizes", default="128,192,256",
                     help="low-res tile sides of the multi-tile leg, cycled over the 48 tiles")
@@ -558,6 +608,9 @@ def main():
                               "epoch_dense_model_GBps": (spe * 104.0 * N * N + L * (136.0 + y_bytes) * n * n) / 1e9 / dt}
             print(f"[bench] batch {bs}: {dt * 1e3:.1f} ms/epoch", file=sys.stderr, flush=True)
         out["batch_sweep"] = sweep
+    if not args.no_fft_reference and (n, N) == (2048, 16384):
+        out["fft_reference"] = run_fft_reference(stream)
+        print(f"[bench] fft vs rocFFT: {out['fft_reference']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle: one epoch, and the two smaller cases)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = run_cpu_baseline(ds, args)

@@ -132,6 +132,10 @@ template <int LEN, int TR> struct StridedCfg {
 //             it (the owner-computes stitch), so the gradient spectrum is
 //             never written, read back or cleared.
 enum { SRC_PLAIN = 0, SRC_CROP = 1, SRC_STITCH = 2 };
+#ifndef FPM_STITCH_XCD
+#define FPM_STITCH_XCD 1
+#endif
+
 constexpr int kMaxStitchLeds = 64;  // one bit per LED of the batch
 
 template <int LEN, int TR, bool INV, bool TWID, int SRC, int ENG>
@@ -149,8 +153,22 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     const int t = threadIdx.x;
     // the crop launches pages fastest (grid x) so the LED pages that share a
     // pupil tile run back to back and the pupil tile stays in cache
-    const uint32_t tile = SRC == SRC_CROP ? blockIdx.z : blockIdx.x;
+    uint32_t tile = SRC == SRC_CROP ? blockIdx.z : blockIdx.x;
     const uint32_t page = SRC == SRC_CROP ? blockIdx.x : blockIdx.z;
+    uint32_t g = blockIdx.y;
+    if constexpr (SRC == SRC_STITCH && FPM_STITCH_XCD) {
+        // workgroup i runs on XCD i mod 8: give each XCD a contiguous run of
+        // (tile, group) pairs, so that row tiles that share the cache lines
+        // at their boundaries (the windows start at any row) share an L2
+        // (16384^2, batch 26: 0.81 -> 0.78 ms for the two column passes)
+        const uint32_t nb = gridDim.x * gridDim.y;
+        if ((nb & 7u) == 0) {
+            const uint32_t b = blockIdx.x + gridDim.x * blockIdx.y;
+            const uint32_t lb = (b & 7u) * (nb >> 3) + (b >> 3);
+            tile = lb % gridDim.x;
+            g = lb / gridDim.x;
+        }
+    }
     if (a.rowmask) {  // rows outside every LED window of the batch: nothing to do
         const uint8_t* rmk = a.rowmask + (a.mask_ppt ? (size_t)(page / a.mask_ppt) * a.mask_stride : 0);
         bool any = false;
@@ -161,7 +179,6 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     EngCtx<ENG> ctx;
     ctx.init();
-    const uint32_t g = blockIdx.y;
     const size_t base = (size_t)page * a.page;
     const float2* __restrict__ in = a.in + base;
     float2* __restrict__ out = a.out + base;

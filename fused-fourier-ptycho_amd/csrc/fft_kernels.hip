@@ -13,6 +13,8 @@
 // Every pass reads and writes each element once (16 B / element / pass).
 // These replace the cufftPlan2d / cufftPlanMany transforms of
 // cuFPM_v3/cuFPM_pure_v3.cu:118-126 (unnormalised C2C, cuFFT sign convention).
+#include <type_traits>
+
 #include "fft_core.h"
 #include "common.h"
 #include "fft_kernels.h"
@@ -1025,18 +1027,28 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
     float* ef = (float*)lds;
     float* mf = ef + Cfg::LT * LEN;
     static_assert(2 * Cfg::LT * LEN <= 2 * Map::LDS_WORDS, "e and |z| must fit the exchange buffer");
+    // Y of this page through a global pointer and 32-bit offsets (a page
+    // holds at most 2^22 points), the uniform stack-type branch outside the loop
+    const char* ypage = (const char*)y.data + ybase * (y.u16 ? 2 : 4);
     int tt = fresh(t);
+    auto residual = [&](auto u16) {
 #pragma unroll
-    for (int q = 0; q < SI::NGL; ++q)
+        for (int q = 0; q < SI::NGL; ++q)
 #pragma unroll
-        for (int s = 0; s < SI::GSL; ++s) {
-            int line, idx;
-            SI::out(tt, q, s, line, idx);
-            const float2 z = v[q * SI::GSL + s];
-            const float mag = sqrtf(z.x * z.x + z.y * z.y);
-            if constexpr (FPM_STENCIL_MAG_LDS) mf[line * LEN + idx] = mag;
-            ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) - read_y(y, ybase + (size_t)column(line) * LEN + idx);
-        }
+            for (int s = 0; s < SI::GSL; ++s) {
+                int line, idx;
+                SI::out(tt, q, s, line, idx);
+                const float2 z = v[q * SI::GSL + s];
+                const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                if constexpr (FPM_STENCIL_MAG_LDS) mf[line * LEN + idx] = mag;
+                ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) -
+                                       read_y_page<decltype(u16)::value>(ypage, column(line) * (uint32_t)LEN + (uint32_t)idx, y);
+            }
+    };
+    if (y.u16)
+        residual(std::true_type{});
+    else
+        residual(std::false_type{});
     __syncthreads();
     float loss32 = 0.0f;  // 32 terms per thread at most; the workgroup sum is FP64
     tt = fresh(t);
@@ -1287,8 +1299,6 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
     static_assert(NLINES * LEN <= 2 * HMap::LDS_WORDS, "e must fit the half buffer");
     // Y of this page through a global pointer and 32-bit offsets (a page
     // holds 2^22 points), the uniform stack-type branch outside the loop
-    using GU16 = const __attribute__((address_space(1))) uint16_t*;
-    using GF32 = const __attribute__((address_space(1))) float*;
     const char* ypage = (const char*)y.data + ybase * (y.u16 ? 2 : 4);
     auto residual = [&](const float2 (&v)[16], int tv) {
         tv = fresh(tv);
@@ -1299,14 +1309,8 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
                 for (int s = 0; s < SI::GSL; ++s) {
                     int line, idx;
                     SI::out(tv, q, s, line, idx);
-                    const uint32_t off = column(line) * (uint32_t)LEN + (uint32_t)idx;
-                    float yv;
-                    if constexpr (decltype(u16)::value) {
-                        const uint32_t raw = ((GU16)ypage)[off];
-                        yv = sqrtf(div_u16((float)(raw - y.mn), y.range, y.rcp_range));
-                    } else {
-                        yv = ((GF32)ypage)[off];
-                    }
+                    const float yv = read_y_page<decltype(u16)::value>(
+                        ypage, column(line) * (uint32_t)LEN + (uint32_t)idx, y);
                     const float2 z = v[q * SI::GSL + s];
                     const float mag = sqrtf(z.x * z.x + z.y * z.y);
                     ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) - yv;

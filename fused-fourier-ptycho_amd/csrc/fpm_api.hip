@@ -572,7 +572,7 @@ struct Solve {
     int parity_after(int cur) const { return (int)((cur + steps) & 1u); }
 
     // generic sides (generic.hip): S = DFT_N(O) every step into S0, the
-    // gradient spectrum and G_O in S1, the residuals in ebuf
+    // gradient spectrum and G_O in S1, the residuals in ebuf (unfused pages only)
     void init_generic(int ne, bool want_loss, float lr, float beta, float eps, float clamp, int32_t halve_from,
                       hipStream_t st) {
         const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
@@ -581,7 +581,7 @@ struct Solve {
         mO.alloc(NN * sizeof(float));
         mP.alloc(nn * sizeof(float));
         ws.alloc(nn * B * sizeof(float2));
-        ebuf.alloc(nn * B * sizeof(float));
+        if (!generic_fused_pages(g, false)) ebuf.alloc(nn * B * sizeof(float));
         FPM_HIP(hipMemsetAsync(mO.p, 0, NN * sizeof(float), st));
         FPM_HIP(hipMemsetAsync(mP.p, 0, nn * sizeof(float), st));
         state.alloc(64 + (want_loss ? (size_t)ne * sizeof(double) : 0));
@@ -594,6 +594,7 @@ struct Solve {
         launch_solve_state_init(ss, lr, st);
         rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
         generic_prepare(g);
+        generic_tune(g, 1, B, O, S0.as<float2>(), P, led, nullptr, ws.lat, st);
     }
 
     // Stage events: each whole 2D transform is recorded under its column-pass
@@ -607,24 +608,32 @@ struct Solve {
         float2* G = S1.as<float2>();
         float2* lat = ws.lat;
         float2* rec = ws.rec;
+        const bool fused = generic_fused_pages(g, false);
         uint32_t led0, b;
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
             tm.begin(FPM_STAGE_FFT_N_COLS_HI);
             fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
             tm.end(2);
-            tm.begin(FPM_STAGE_CROP);
-            generic_crop(S, P, led, led0, b, g, lat, st);
-            tm.end(1);
-            tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
-            fft2_generic(lat, lat, g.nr, g.nc, b, true, st);
-            tm.end(2);
+            if (fused) {
+                tm.begin(FPM_STAGE_IFFT_N_COLS_LO);  // crop x pupil inside the row pass
+                generic_inverse_pages(S, P, led, led0, b, g, lat, st, nullptr, 1, true);
+                tm.end(1);
+            } else {
+                tm.begin(FPM_STAGE_CROP);
+                generic_crop(S, P, led, led0, b, g, lat, st);
+                tm.end(1);
+                tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
+                fft2_generic(lat, lat, g.nr, g.nc, b, true, st);
+                tm.end(2);
+            }
             tm.begin(FPM_STAGE_STENCIL);
-            generic_stencil(lat, y, led0, b, pr, g, ebuf.as<float>(), rec, ss.loss_cur, st);
-            tm.end(2);
+            generic_stencil_pages(lat, y, led0, b, pr, g, ebuf.as<float>(), rec, ss.loss_cur, st, nullptr, 1, nullptr,
+                                  fused);
+            tm.end(fused ? 1 : 2);
             tm.begin(FPM_STAGE_FFT_N_COLS_LO);
-            fft2_generic(rec, rec, g.nr, g.nc, b, false, st);
-            tm.end(2);
+            generic_forward_pages(rec, b, g, st, fused);
+            tm.end(fused ? 1 : 2);
             tm.begin(FPM_STAGE_STITCH_OWNER);
             generic_stitch(rec, P, led, led0, b, g, G, st);  // reads this step's pupil
             tm.end(1);
@@ -741,8 +750,10 @@ struct GroupSolve {
         launch_solve_state_init(ss, lr, st);
         rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
         if (g.generic) {  // no masks and no carried spectrum on the generic path
-            ebuf.alloc((size_t)T * B * nn * sizeof(float));
+            if (!generic_fused_pages(g, false)) ebuf.alloc((size_t)T * B * nn * sizeof(float));
             generic_prepare(g);
+            generic_tune(g, T, B, O.as<float2>(), S0.as<float2>(), P.as<float2>(), nullptr, tabd, lat.as<float2>(),
+                         st);
             return;
         }
         for (uint32_t k = 0; k < T; ++k)
@@ -812,14 +823,14 @@ struct GroupSolve {
         float2* G = S1.as<float2>();
         float2* lp = lat.as<float2>();
         float2* rp = rec.as<float2>();
+        const bool fused = generic_fused_pages(g, false);
         uint32_t led0, b;
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
             fft2_generic(Ow, S, g.Nr, g.Nc, T, false, st);
-            generic_crop(S, Pw, nullptr, led0, b, g, lp, st, tabd, T);
-            fft2_generic(lp, lp, g.nr, g.nc, T * b, true, st);
-            generic_stencil(lp, y, led0, b, pr, g, ebuf.as<float>(), rp, nullptr, st, tabd, T);
-            fft2_generic(rp, rp, g.nr, g.nc, T * b, false, st);
+            generic_inverse_pages(S, Pw, nullptr, led0, b, g, lp, st, tabd, T, fused);
+            generic_stencil_pages(lp, y, led0, b, pr, g, ebuf.as<float>(), rp, nullptr, st, tabd, T, nullptr, fused);
+            generic_forward_pages(rp, T * b, g, st, fused);
             generic_stitch(rp, Pw, nullptr, led0, b, g, G, st, tabd, T);
             generic_pupil(rp, S, nullptr, led0, b, g, Pw, mP.as<float>(), nullptr, rms, nullptr, st, tabd, T);
             fft2_generic(G, G, g.Nr, g.Nc, T, true, st);
@@ -1306,18 +1317,18 @@ int fpm_fused_gradient(const fpm_c32* O_in, int64_t N_rows, int64_t N_cols, cons
         if (g.generic) {
             // generic sides (generic.hip): FP32 mixed-radix transforms, owner stitch
             DevBuf eb;
-            eb.alloc(nn * (size_t)L * sizeof(float));
+            if (!generic_fused_pages(g, signs != nullptr)) eb.alloc(nn * (size_t)L * sizeof(float));
             generic_prepare(g);
             float2* S = S0.as<float2>();
             float2* lat = ws.lat;
             float2* rec = ws.rec;
             const uint32_t b = (uint32_t)L;
+            const bool fused = generic_fused_pages(g, signs != nullptr);
             fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
-            generic_crop(S, pcopy.as<float2>(), led, 0, b, g, lat, st);
-            fft2_generic(lat, lat, g.nr, g.nc, b, true, st);
-            generic_stencil(lat, y, 0, b, pr, g, eb.as<float>(), rec, loss_out ? lossd.as<double>() : nullptr, st,
-                            nullptr, 1, signs);
-            fft2_generic(rec, rec, g.nr, g.nc, b, false, st);
+            generic_inverse_pages(S, pcopy.as<float2>(), led, 0, b, g, lat, st, nullptr, 1, fused);
+            generic_stencil_pages(lat, y, 0, b, pr, g, eb.as<float>(), rec, loss_out ? lossd.as<double>() : nullptr, st,
+                                  nullptr, 1, signs, fused);
+            generic_forward_pages(rec, b, g, st, fused);
             generic_stitch(rec, pcopy.as<float2>(), led, 0, b, g, S1.as<float2>(), st);
             generic_pupil(rec, S, led, 0, b, g, pcopy.as<float2>(), nullptr, nullptr, RmsArgs{0.f, 0.f, 0.f, 0.f},
                           GPd, st);

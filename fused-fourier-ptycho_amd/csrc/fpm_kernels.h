@@ -49,6 +49,19 @@ __device__ __forceinline__ float read_y(const YStack& y, size_t off) {
     return ((const float*)y.data)[off];
 }
 
+// read_y at element `off` of one page (`page` = the page's first element,
+// the stack type fixed at compile time): loads through a global-space
+// pointer and a 32-bit offset, so a page's loads need no 64-bit addresses
+template <bool U16>
+__device__ __forceinline__ float read_y_page(const char* page, uint32_t off, const YStack& y) {
+    if constexpr (U16) {
+        const uint32_t raw = ((const __attribute__((address_space(1))) uint16_t*)page)[off];
+        return sqrtf(div_u16((float)(raw - y.mn), y.range, y.rcp_range));
+    } else {
+        return ((const __attribute__((address_space(1))) float*)page)[off];
+    }
+}
+
 // |z| / ratio as the reference's absC (cuFPM_v3/cuda/kernels.cuh:5-9); when
 // ratio = n_r n_c r^2 is a power of two, inv_ratio = 1/ratio is exact and the
 // product equals the quotient; otherwise inv_ratio is 0 and the division runs

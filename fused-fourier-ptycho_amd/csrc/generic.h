@@ -19,6 +19,15 @@ void fft2_generic(const float2* in, float2* out, uint32_t rows, uint32_t cols, u
                   hipStream_t st);
 // the twiddle tables of the geometry's four lengths, made before any capture
 void generic_prepare(const Geom& g);
+// Launch shapes of the solve's transform passes (T tiles, batch B): each
+// pass not yet tuned on this device is timed with candidate shapes (points
+// per workgroup x points per thread) on the given buffers -- O (T objects),
+// S (T spectra), P, led or tab, pages (T B per-LED pages), read only; the
+// outputs go to a scratch buffer -- and the fastest is kept for every later
+// launch of that (length, line kind, source).  Results do not depend on
+// the shape.  Synchronises st; does nothing while st is capturing.
+void generic_tune(const Geom& g, uint32_t T, uint32_t B, const float2* O, const float2* S, const float2* P,
+                  const int2* led, const TileIn* tab, const float2* pages, hipStream_t st);
 
 // crop x pupil of LEDs led0.. into b ifftshifted natural pages
 // tab (optional, ntiles entries, ntiles <= 65535): grouped tiles, tile k's
@@ -32,6 +41,22 @@ void generic_crop(const float2* S, const float2* P, const int2* led, uint32_t le
 void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
                      float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab = nullptr,
                      uint32_t ntiles = 1, uint8_t* signs = nullptr);
+// The per-LED pages of a minibatch step, in three stages: lat = IDFT_n(crop x
+// pupil), rec = the loss stencil's u, then rec = DFT_n(rec).  fused
+// (generic_fused_pages): the crop is the source of the first row pass, and
+// the inverse column pass, residual, stencil and forward column pass run as
+// one kernel on column sets (lat holds the pages between the row pass and
+// the columns; ebuf unused); else generic_crop, fft2_generic,
+// generic_stencil, fft2_generic.  The same arithmetic per element either
+// way, with the row and column passes in the other order.
+bool generic_fused_pages(const Geom& g, bool signs);
+void generic_inverse_pages(const float2* S, const float2* P, const int2* led, uint32_t led0, uint32_t b,
+                           const Geom& g, float2* lat, hipStream_t st, const TileIn* tab, uint32_t ntiles,
+                           bool fused);
+void generic_stencil_pages(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
+                           float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab,
+                           uint32_t ntiles, uint8_t* signs, bool fused);
+void generic_forward_pages(float2* rec, uint32_t pages, const Geom& g, hipStream_t st, bool fused);
 // owner-computes stitch of the b back-projections into G_S (every element written)
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
                     float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #ifndef FPM_GEN_CONTIG_POINTS
@@ -36,6 +37,17 @@
 #endif
 #ifndef FPM_GEN_LB
 #define FPM_GEN_LB 1024
+#endif
+// crop in the first row pass and the fused column steps (gen_cols_fused_kernel)
+#ifndef FPM_GEN_FUSED
+#define FPM_GEN_FUSED 1
+#endif
+#ifndef FPM_GEN_FUSED_MAX
+#define FPM_GEN_FUSED_MAX 2048
+#endif
+// launch shapes of the transform passes measured at a solve's set-up
+#ifndef FPM_GEN_TUNE
+#define FPM_GEN_TUNE 1
 #endif
 
 namespace fpm {
@@ -184,7 +196,7 @@ template <> __device__ __forceinline__ void small_dft<7>(float2* v, bool inv) {
 // 16384-point line fits the LDS)
 template <int R>
 __device__ __forceinline__ void gen_pass(const GenFFT& a, float2* x, uint32_t nl, uint32_t n, uint32_t s,
-                                         const FastDiv& fs, const FastDiv& fnb) {
+                                         const FastDiv& fs, const FastDiv& fnb, bool inv) {
     constexpr int IT = (kGenPts + R - 1) / R;
     const uint32_t m = n / R, nb = a.len / R, step_n = a.len / n, items = nl * nb;
     float2 res[IT][R];
@@ -200,12 +212,12 @@ __device__ __forceinline__ void gen_pass(const GenFFT& a, float2* x, uint32_t nl
             float2 v[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) v[j] = xl[q + s * (p + j * m)];
-            small_dft<R>(v, a.inverse != 0);
+            small_dft<R>(v, inv);
             res[it][0] = v[0];
 #pragma unroll
             for (int k = 1; k < R; ++k) {
                 float2 w = a.tw[p * (uint32_t)k * step_n];  // p k < (n / R) R = n
-                if (a.inverse) w.y = -w.y;
+                if (inv) w.y = -w.y;
                 res[it][k] = cmul_g(v[k], w);
             }
             dst[it] = l * a.len + q + s * (R * p);
@@ -225,7 +237,7 @@ __device__ __forceinline__ void gen_pass(const GenFFT& a, float2* x, uint32_t nl
 // butterfly (p, q) is a direct R-point sum, W_R^(j k) = W_len^((j k mod R)
 // len / R) from the same table (R len / R products per line and pass)
 __device__ __forceinline__ void gen_pass_any(const GenFFT& a, float2* x, uint32_t nl, uint32_t n, uint32_t s, uint32_t R,
-                                          const FastDiv& fs, const FastDiv& fr) {
+                                          const FastDiv& fs, const FastDiv& fr, bool inv) {
     const uint32_t m = n / R, step_n = a.len / n, step_r = a.len / R, items = nl * a.len;
     float2 res[kGenPts];
     uint32_t dst[kGenPts];
@@ -242,14 +254,14 @@ __device__ __forceinline__ void gen_pass_any(const GenFFT& a, float2* x, uint32_
             uint32_t jk = 0;
             for (uint32_t j = 0; j < R; ++j) {
                 float2 w = a.tw[jk * step_r];
-                if (a.inverse) w.y = -w.y;
+                if (inv) w.y = -w.y;
                 const float2 t = cmul_g(xl[s * j * m], w);
                 acc = make_float2(acc.x + t.x, acc.y + t.y);
                 jk += k;
                 jk = jk >= R ? jk - R : jk;
             }
             float2 w = a.tw[p * k * step_n];
-            if (a.inverse) w.y = -w.y;
+            if (inv) w.y = -w.y;
             res[it] = cmul_g(acc, w);
             dst[it] = l * a.len + q + s * (R * p + k);
         }
@@ -261,7 +273,60 @@ __device__ __forceinline__ void gen_pass_any(const GenFFT& a, float2* x, uint32_
     __syncthreads();
 }
 
-__global__ __launch_bounds__(FPM_GEN_LB) void fft_generic_kernel(GenFFT a) {
+// every pass of the transform of the nl lines in x (LDS), in place
+__device__ __forceinline__ void gen_passes(const GenFFT& a, float2* x, uint32_t nl, bool inv) {
+    uint32_t n = a.len, s = 1;
+    for (uint32_t f = 0; f < a.nf; ++f) {
+        const uint32_t r = a.fac[f];
+        switch (r) {
+            case 4: gen_pass<4>(a, x, nl, n, s, a.f_s[f], a.f_nb[f], inv); break;
+            case 2: gen_pass<2>(a, x, nl, n, s, a.f_s[f], a.f_nb[f], inv); break;
+            case 3: gen_pass<3>(a, x, nl, n, s, a.f_s[f], a.f_nb[f], inv); break;
+            case 5: gen_pass<5>(a, x, nl, n, s, a.f_s[f], a.f_nb[f], inv); break;
+            case 7: gen_pass<7>(a, x, nl, n, s, a.f_s[f], a.f_nb[f], inv); break;
+            default: gen_pass_any(a, x, nl, n, s, r, a.f_s[f], a.f_r[f], inv); break;
+        }
+        n /= r;
+        s *= r;
+    }
+}
+
+// The crop x pupil as the source of a row pass (kernels.cuh:38-64): line L
+// is row di of page pg = L / n_r of the batch (pages of tile k = pg / b at
+// b k .. b k + b - 1 with tab), element dj its column; the value is
+// ifftshift(S P) of LED led0 + pg mod b, formed as gen_crop_kernel forms it.
+struct GenCrop {
+    const float2* S;
+    const float2* P;
+    const int2* led;
+    const TileIn* tab;
+    uint32_t led0, b, nr, nc, Nr, Nc;
+    FastDiv f_b;
+};
+__device__ __forceinline__ float2 gen_crop_value(const GenCrop& c, uint32_t pg, uint32_t di, uint32_t dj) {
+    const float2* S = c.S;
+    const float2* P = c.P;
+    const int2* led = c.led;
+    uint32_t l = pg;
+    if (c.tab) {
+        const uint32_t tk = fdiv(pg, c.f_b);
+        l = pg - tk * c.b;
+        S += (size_t)tk * c.Nr * c.Nc;
+        P += (size_t)tk * c.nr * c.nc;
+        led = c.tab[tk].led;
+    }
+    const uint32_t i = wrap1(di + c.nr / 2, c.nr), j = wrap1(dj + c.nc / 2, c.nc);
+    const int2 kk = led[c.led0 + l];
+    const uint32_t srow = wrap1(i + (uint32_t)kk.y - 1u + c.Nr / 2, c.Nr);
+    const uint32_t scol = wrap1(j + (uint32_t)kk.x - 1u + c.Nc / 2, c.Nc);
+    const float2 pp = P[(size_t)j * c.nr + i];
+    float2 z = make_float2(0.f, 0.f);
+    if (pp.x != 0.0f || pp.y != 0.0f) z = cmul_g(S[(size_t)scol * c.Nr + srow], pp);
+    return z;
+}
+
+template <bool CROP>
+__global__ __launch_bounds__(FPM_GEN_LB) void fft_generic_kernel(GenFFT a, GenCrop c) {
     extern __shared__ float2 sm[];
     float2* x = sm;
     const uint32_t L0 = blockIdx.x * a.tl;
@@ -279,23 +344,17 @@ __global__ __launch_bounds__(FPM_GEN_LB) void fft_generic_kernel(GenFFT a) {
     };
     for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
         const uint2 le = split(idx);
-        x[(size_t)le.x * a.len + le.y] = a.in[gen_off(a, L0 + le.x, le.y)];
+        float2 v;
+        if constexpr (CROP) {
+            const uint32_t L = L0 + le.x, pg = fdiv(L, a.f_lpb);
+            v = gen_crop_value(c, pg, L - pg * a.lines_per_batch, le.y);
+        } else {
+            v = a.in[gen_off(a, L0 + le.x, le.y)];
+        }
+        x[(size_t)le.x * a.len + le.y] = v;
     }
     __syncthreads();
-    uint32_t n = a.len, s = 1;
-    for (uint32_t f = 0; f < a.nf; ++f) {
-        const uint32_t r = a.fac[f];
-        switch (r) {
-            case 4: gen_pass<4>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
-            case 2: gen_pass<2>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
-            case 3: gen_pass<3>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
-            case 5: gen_pass<5>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
-            case 7: gen_pass<7>(a, x, nl, n, s, a.f_s[f], a.f_nb[f]); break;
-            default: gen_pass_any(a, x, nl, n, s, r, a.f_s[f], a.f_r[f]); break;
-        }
-        n /= r;
-        s *= r;
-    }
+    gen_passes(a, x, nl, a.inverse != 0);
     for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
         const uint2 le = split(idx);
         a.out[gen_off(a, L0 + le.x, le.y)] = x[(size_t)le.x * a.len + le.y];
@@ -343,21 +402,13 @@ void set_factors(GenFFT& a, uint32_t len) {
     }
 }
 
-void launch_generic_lines(GenFFT a, hipStream_t st) {
+// the plan of a length: factors, twiddles, divisors (not tl)
+void plan_generic(GenFFT& a) {
     if (a.len < 2 || a.len > (uint32_t)kGenericMaxSide)
         throw HipError{hipErrorInvalidValue, "generic FFT: length out of range"};
     set_factors(a, a.len);
     a.tw = generic_twiddles(a.len);
-    // lines per workgroup: strided lines need several adjacent lines per
-    // column read (coalescing), contiguous ones do not and take fewer
-    // points per workgroup (smaller LDS, more workgroups per CU).  Measured
-    // on the 48-tile solve with 16 tiles of 192 px: contiguous 4096 -> 1024
-    // points 39.6 -> 35.8 ms per epoch (512: 37.2); then, with the block-
-    // filtered stitch, strided 4096 -> 2048 points 33.5 -> 32.2 (1024: 35.6)
-    a.tl = std::max<uint32_t>(1, (a.es == 1 ? (uint32_t)FPM_GEN_CONTIG_POINTS : (uint32_t)FPM_GEN_STRIDED_POINTS) /
-                                     a.len);
-    a.f_lpb = make_fastdiv(a.lines_per_batch);
-    a.f_tl = make_fastdiv(a.tl);
+    a.f_lpb = make_fastdiv(a.lines_per_batch ? a.lines_per_batch : 1);
     a.f_len = make_fastdiv(a.len);
     uint32_t s = 1;
     for (uint32_t f = 0; f < a.nf; ++f) {
@@ -366,20 +417,77 @@ void launch_generic_lines(GenFFT a, hipStream_t st) {
         a.f_r[f] = make_fastdiv(a.fac[f]);
         s *= a.fac[f];
     }
-    // threads: about 8 points each (at least 256 threads, at most kGenPts
-    // points each), a multiple of the wave
-    const uint32_t pts = a.tl * a.len;
-    const uint32_t threads = std::min<uint32_t>(FPM_GEN_LB, std::max<uint32_t>(256, (pts + 8 * 64 - 1) / (8 * 64) * 64));
+}
+
+// threads of a workgroup holding pts points: about dens points each (at
+// least 256 threads, at most kGenPts points each), a multiple of the wave
+uint32_t gen_threads(uint32_t pts, uint32_t dens = 8) {
+    const uint32_t threads =
+        std::min<uint32_t>(FPM_GEN_LB, std::max<uint32_t>(256, (pts + dens * 64 - 1) / (dens * 64) * 64));
     if ((size_t)threads * kGenPts < pts) throw HipError{hipErrorInvalidValue, "generic FFT: line set too long"};
+    return threads;
+}
+
+// ---- launch shapes of the transform passes and their autotuner
+// A pass's shape is the points per workgroup (tl = max(1, pts / len) lines)
+// and the points per thread.  Neither changes any result: every butterfly,
+// twiddle and crop product is computed the same way whichever workgroup
+// and thread hold it, so a shape is chosen for speed alone.  Defaults
+// (measured on the 48-tile solve with 16 tiles of 192 px): contiguous
+// lines 1024 points (4096 -> 1024: 39.6 -> 35.8 ms per epoch; 512: 37.2),
+// strided lines 2048 (4096 -> 2048: 33.5 -> 32.2; 1024: 35.6), 8 points per
+// thread.  generic_tune replaces them per device and (length, line kind,
+// crop source) with the fastest of a candidate set timed on a solve's own
+// buffers at its set-up.
+struct GenShape {
+    uint32_t pts, dens;
+};
+struct GenKey {
+    int dev;
+    uint32_t len;
+    bool strided, crop;
+    bool operator<(const GenKey& o) const {
+        return std::tie(dev, len, strided, crop) < std::tie(o.dev, o.len, o.strided, o.crop);
+    }
+};
+std::mutex g_shape_mu;
+std::map<GenKey, GenShape> g_shapes;
+
+GenKey key_of(const GenFFT& a, bool crop) {
+    int dev = 0;
+    FPM_HIP(hipGetDevice(&dev));
+    return GenKey{dev, a.len, a.es != 1, crop};
+}
+GenShape shape_of(const GenFFT& a, bool crop) {
+    if (FPM_GEN_TUNE) {
+        std::lock_guard<std::mutex> lock(g_shape_mu);
+        auto it = g_shapes.find(key_of(a, crop));
+        if (it != g_shapes.end()) return it->second;
+    }
+    return GenShape{a.es == 1 ? (uint32_t)FPM_GEN_CONTIG_POINTS : (uint32_t)FPM_GEN_STRIDED_POINTS, 8};
+}
+
+void launch_generic_lines(GenFFT a, hipStream_t st, const GenCrop* crop = nullptr, const GenShape* force = nullptr) {
+    plan_generic(a);
+    const GenShape sh = force ? *force : shape_of(a, crop != nullptr);
+    a.tl = std::max<uint32_t>(1, sh.pts / a.len);
+    a.f_tl = make_fastdiv(a.tl);
+    const uint32_t pts = a.tl * a.len;
+    const uint32_t threads = gen_threads(pts, sh.dens);
     const size_t shmem = (size_t)pts * sizeof(float2);
     static std::once_flag big_lds;
     if (FPM_GEN_SET_ATTR) std::call_once(big_lds, [] {
         // lines up to 16384 points (128 KiB) stage in LDS
-        FPM_HIP(hipFuncSetAttribute((const void*)fft_generic_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+        FPM_HIP(hipFuncSetAttribute((const void*)fft_generic_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(kGenericMaxSide * sizeof(float2))));
+        FPM_HIP(hipFuncSetAttribute((const void*)fft_generic_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(kGenericMaxSide * sizeof(float2))));
     });
     const uint32_t blocks = (a.nlines + a.tl - 1) / a.tl;
-    hipLaunchKernelGGL(fft_generic_kernel, dim3(blocks), dim3(threads), shmem, st, a);
+    if (crop)
+        hipLaunchKernelGGL(fft_generic_kernel<true>, dim3(blocks), dim3(threads), shmem, st, a, *crop);
+    else
+        hipLaunchKernelGGL(fft_generic_kernel<false>, dim3(blocks), dim3(threads), shmem, st, a, GenCrop{});
     FPM_LAUNCH_CHECK();
 }
 
@@ -490,6 +598,107 @@ __global__ __launch_bounds__(256) void gen_stencil_kernel(const float2* __restri
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = loss;
         __syncthreads();
         if (threadIdx.x == 0) unsafeAtomicAdd(loss_acc, red[0] + red[1] + red[2] + red[3]);
+    }
+}
+
+// The per-LED column steps in one kernel: a workgroup holds lt whole
+// columns (n_r points each) of one page in LDS -- columns j0 - 1 .. j0 + lt
+// - 2 (cyclic), the first and last as neighbours only -- and runs the
+// inverse column transform, the residual e (in LDS beside the columns), the
+// loss stencil of gen_stencil_kernel on its lt - 2 interior columns, and the
+// forward column transform, then writes the interior columns.  The row
+// transforms (the crop's inverse pass before, the forward pass after) run
+// as separate passes.  Grid: x = page (b per tile) x column group, y = tile.
+struct GenCols {
+    const float2* z;
+    float2* out;
+    YStack y;
+    const TileIn* tab;
+    double* loss;
+    uint32_t led0, b, nr, nc, lt, groups;
+    float ratio, inv_ratio, gscale;
+};
+__global__ __launch_bounds__(FPM_GEN_LB) void gen_cols_fused_kernel(GenFFT a, GenCols c) {
+    extern __shared__ float2 sm[];
+    __shared__ double red[FPM_GEN_LB / 64];
+    const uint32_t lt = c.lt, len = c.nr, tot = lt * len;
+    float2* x = sm;
+    float* ef = (float*)(sm + tot);
+    const uint32_t page = blockIdx.x / c.groups, grp = blockIdx.x - page * c.groups;
+    const size_t plane = (size_t)c.nr * c.nc;
+    size_t pbase = (size_t)page * plane;
+    YStack y = c.y;
+    double* loss_acc = c.loss;
+    if (c.tab) {
+        const uint32_t tk = blockIdx.y;
+        pbase += (size_t)tk * c.b * plane;
+        const TileIn ti = c.tab[tk];
+        y.data = ti.y;
+        y.mn = ti.ymn;
+        y.range = ti.yrange;
+        y.rcp_range = ti.yrcp;
+        loss_acc = ti.loss;
+    }
+    const float2* __restrict__ z = c.z + pbase;
+    float2* __restrict__ out = c.out + pbase;
+    const size_t ybase = (size_t)(c.led0 + page) * plane;
+    const uint32_t j0 = grp * (lt - 2);  // first interior column; line l is column j0 - 1 + l (mod n_c)
+    auto column = [&](uint32_t line) -> uint32_t {
+        const uint32_t v = j0 + line + c.nc - 1u;  // lt <= n_c + 2: v < 3 n_c
+        return v >= c.nc ? (v - c.nc >= c.nc ? v - 2u * c.nc : v - c.nc) : v;
+    };
+    auto interior = [&](uint32_t line) -> bool { return line >= 1 && line + 1 < lt && j0 + line - 1u < c.nc; };
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const uint32_t l = fdiv(idx, a.f_len), e = idx - l * len;
+        x[idx] = z[(size_t)column(l) * len + e];
+    }
+    __syncthreads();
+    gen_passes(a, x, lt, true);
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const uint32_t l = fdiv(idx, a.f_len), e = idx - l * len;
+        const float2 v = x[idx];
+        const float mag = sqrtf(v.x * v.x + v.y * v.y);
+        ef[idx] = scaled_mag(mag, c.ratio, c.inv_ratio) - read_y(y, ybase + (size_t)column(l) * len + e);
+    }
+    __syncthreads();
+    float loss32 = 0.0f;
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const uint32_t l = fdiv(idx, a.f_len), i = idx - l * len;
+        if (!interior(l)) continue;
+        const float e0 = ef[idx];
+        const float eL = ef[idx - len], eR = ef[idx + len];
+        const float eU = ef[l * len + (i == 0 ? len - 1 : i - 1)];
+        const float eD = ef[l * len + (i + 1 == len ? 0 : i + 1)];
+        const float gval =
+            (copysignf(1.0f, e0 - eL) - copysignf(1.0f, eR - e0) + copysignf(1.0f, e0 - eU) - copysignf(1.0f, eD - e0)) *
+            c.gscale;
+        loss32 += fabsf(eR - e0) + fabsf(eD - e0);
+        const float2 v = x[idx];
+        const float mag = sqrtf(v.x * v.x + v.y * v.y);
+        float2 o = make_float2(0.f, 0.f);
+        if (mag > 0.0f) {
+            const float inv = 1.0f / mag;
+            o = make_float2(v.x * inv * gval, v.y * inv * gval);
+        }
+        x[idx] = o;  // only this thread reads x[idx] in this loop
+    }
+    __syncthreads();
+    gen_passes(a, x, lt, false);
+    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const uint32_t l = fdiv(idx, a.f_len), e = idx - l * len;
+        if (interior(l)) out[(size_t)column(l) * len + e] = x[idx];
+    }
+    if (loss_acc) {  // uniform over the workgroup
+        double loss = (double)loss32;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = loss;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum = 0.0;
+            for (uint32_t w = 0; w < blockDim.x / 64; ++w) sum += red[w];
+            unsafeAtomicAdd(loss_acc, sum);
+        }
     }
 }
 
@@ -658,6 +867,128 @@ void generic_prepare(const Geom& g) {
     for (uint32_t len : {g.nr, g.nc, g.Nr, g.Nc}) (void)generic_twiddles(len);
 }
 
+namespace {
+// the fastest shape of pass a (its output redirected to scratch), timed over
+// three launches per candidate after one untimed launch
+GenShape tune_pass(GenFFT a, const GenCrop* crop, float2* scratch, hipStream_t st) {
+    a.out = scratch;
+    std::vector<GenShape> cand;
+    std::vector<std::pair<uint32_t, uint32_t>> seen;  // (lines per workgroup, threads)
+    for (uint32_t pts : {512u, 1024u, 2048u, 4096u, 8192u, 16384u})
+        for (uint32_t dens : {4u, 8u, 16u}) {
+            const uint32_t tl = std::max<uint32_t>(1, pts / a.len);
+            if ((size_t)tl * a.len > (size_t)kGenericMaxSide) continue;
+            if (tl > 1 && tl > a.nlines) continue;
+            const uint32_t p = tl * a.len;
+            const uint32_t th = std::min<uint32_t>(FPM_GEN_LB, std::max<uint32_t>(256, (p + dens * 64 - 1) / (dens * 64) * 64));
+            if ((size_t)th * kGenPts < p) continue;
+            if (std::find(seen.begin(), seen.end(), std::make_pair(tl, th)) != seen.end()) continue;
+            seen.emplace_back(tl, th);
+            cand.push_back(GenShape{pts, dens});
+        }
+    hipEvent_t e0, e1;
+    FPM_HIP(hipEventCreate(&e0));
+    FPM_HIP(hipEventCreate(&e1));
+    GenShape best = shape_of(a, crop != nullptr);
+    float best_ms = 1e30f;
+    for (const GenShape& c : cand) {
+        launch_generic_lines(a, st, crop, &c);
+        FPM_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; ++r) launch_generic_lines(a, st, crop, &c);
+        FPM_HIP(hipEventRecord(e1, st));
+        FPM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FPM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    FPM_HIP(hipEventDestroy(e0));
+    FPM_HIP(hipEventDestroy(e1));
+    return best;
+}
+}  // namespace
+
+void generic_tune(const Geom& g, uint32_t T, uint32_t B, const float2* O, const float2* S, const float2* P,
+                  const int2* led, const TileIn* tab, const float2* pages, hipStream_t st) {
+    if (!FPM_GEN_TUNE) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FPM_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return;  // timing needs a live stream
+    const bool fused = generic_fused_pages(g, false);
+    const size_t NN = (size_t)g.Nr * g.Nc, nn = (size_t)g.nr * g.nc;
+    // the passes of a step as the solve launches them (in: its own buffers)
+    std::vector<std::pair<GenFFT, bool>> passes;  // (pass, crop source)
+    GenFFT a{};
+    a.in = O;  // DFT_N of the objects: contiguous lines, then strided
+    a.len = g.Nr;
+    a.nlines = g.Nc * T;
+    a.lines_per_batch = a.nlines;
+    a.line_stride = g.Nr;
+    a.es = 1;
+    passes.emplace_back(a, false);
+    a = GenFFT{};
+    a.in = S;
+    a.len = g.Nc;
+    a.nlines = g.Nr * T;
+    a.lines_per_batch = g.Nr;
+    a.line_stride = 1;
+    a.es = g.Nr;
+    a.batch_stride = NN;
+    passes.emplace_back(a, false);
+    a = GenFFT{};  // per-LED row passes (crop source, then the forward pass)
+    a.in = pages;
+    a.len = g.nc;
+    a.nlines = g.nr * T * B;
+    a.lines_per_batch = g.nr;
+    a.line_stride = 1;
+    a.es = g.nr;
+    a.batch_stride = nn;
+    a.inverse = 1;
+    if (fused) passes.emplace_back(a, true);
+    a.inverse = 0;
+    passes.emplace_back(a, false);
+    if (!fused) {  // per-LED column passes
+        a = GenFFT{};
+        a.in = pages;
+        a.len = g.nr;
+        a.nlines = g.nc * T * B;
+        a.lines_per_batch = a.nlines;
+        a.line_stride = g.nr;
+        a.es = 1;
+        passes.emplace_back(a, false);
+    }
+    GenCrop c{S, P, led, tab, 0, B, g.nr, g.nc, g.Nr, g.Nc, make_fastdiv(B)};
+    std::vector<std::pair<GenKey, std::pair<GenFFT, bool>>> todo;
+    {
+        std::lock_guard<std::mutex> lock(g_shape_mu);
+        for (auto& pc : passes) {
+            const GenKey k = key_of(pc.first, pc.second);
+            if (g_shapes.count(k) == 0 &&
+                std::none_of(todo.begin(), todo.end(), [&](const auto& t) { return !(t.first < k) && !(k < t.first); }))
+                todo.emplace_back(k, pc);
+        }
+    }
+    if (todo.empty()) return;
+    size_t most = 0;
+    for (auto& t : todo) most = std::max<size_t>(most, (size_t)t.second.first.nlines * t.second.first.len);
+    float2* scratch = nullptr;
+    FPM_HIP(hipMalloc(&scratch, most * sizeof(float2)));
+    try {
+        for (auto& t : todo) {
+            const GenShape best = tune_pass(t.second.first, t.second.second ? &c : nullptr, scratch, st);
+            std::lock_guard<std::mutex> lock(g_shape_mu);
+            g_shapes[t.first] = best;
+        }
+    } catch (...) {
+        (void)hipFree(scratch);
+        throw;
+    }
+    FPM_HIP(hipStreamSynchronize(st));
+    FPM_HIP(hipFree(scratch));
+}
+
 void fft2_generic(const float2* in, float2* out, uint32_t rows, uint32_t cols, uint32_t batch, bool inverse,
                   hipStream_t st) {
     // along the contiguous index: one line per column of every page
@@ -708,6 +1039,100 @@ void generic_stencil(const float2* z, const YStack& y, uint32_t led0, uint32_t b
     hipLaunchKernelGGL(gen_stencil_kernel, dim3(gblocks(total), ntiles), dim3(256), 0, st, z, ebuf, g.nr, g.nc, total,
                        (float)(pratio * pratio), u, loss_acc, tab, signs);
     FPM_LAUNCH_CHECK();
+}
+
+// columns per workgroup of the fused column steps: about 4096 points (at
+// least 8 columns: a quarter or less halo) and at most 12288 points (12 B
+// each in LDS: the columns and their residuals); 0 when 3 columns do not fit
+uint32_t gen_cols_lines(const Geom& g) {
+    uint32_t lt = std::max<uint32_t>(4096 / g.nr, std::min<uint32_t>(8, 12288 / g.nr));
+    lt = std::min<uint32_t>(lt, g.nc + 2);
+    return lt >= 3 ? lt : 0;
+}
+
+bool generic_fused_pages(const Geom& g, bool signs) {
+    return FPM_GEN_FUSED && !signs && g.nr <= (uint32_t)FPM_GEN_FUSED_MAX && gen_cols_lines(g) != 0;
+}
+
+void generic_inverse_pages(const float2* S, const float2* P, const int2* led, uint32_t led0, uint32_t b,
+                           const Geom& g, float2* lat, hipStream_t st, const TileIn* tab, uint32_t ntiles,
+                           bool fused) {
+    if (!tab) ntiles = 1;
+    if (!fused) {
+        generic_crop(S, P, led, led0, b, g, lat, st, tab, ntiles);
+        fft2_generic(lat, lat, g.nr, g.nc, ntiles * b, true, st);
+        return;
+    }
+    // the row pass with the crop as its source, natural pages out
+    GenFFT r{};
+    r.out = lat;
+    r.len = g.nc;
+    r.nlines = g.nr * ntiles * b;
+    r.lines_per_batch = g.nr;
+    r.line_stride = 1;
+    r.es = g.nr;
+    r.batch_stride = (uint64_t)g.nr * g.nc;
+    r.inverse = 1;
+    GenCrop c{S, P, led, tab, led0, b, g.nr, g.nc, g.Nr, g.Nc, make_fastdiv(b)};
+    launch_generic_lines(r, st, &c);
+}
+
+void generic_stencil_pages(const float2* z, const YStack& y, uint32_t led0, uint32_t b, int pratio, const Geom& g,
+                           float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab,
+                           uint32_t ntiles, uint8_t* signs, bool fused) {
+    if (!fused) {
+        generic_stencil(z, y, led0, b, pratio, g, ebuf, u, loss_acc, st, tab, ntiles, signs);
+        return;
+    }
+    if (signs) throw HipError{hipErrorInvalidValue, "generic fused columns: no sign output"};
+    if (!tab) ntiles = 1;
+    GenFFT a{};
+    a.len = g.nr;
+    a.lines_per_batch = 1;
+    plan_generic(a);
+    GenCols c{};
+    c.z = z;
+    c.out = u;
+    c.y = y;
+    c.tab = tab;
+    c.loss = loss_acc;
+    c.led0 = led0;
+    c.b = b;
+    c.nr = g.nr;
+    c.nc = g.nc;
+    c.lt = gen_cols_lines(g);
+    c.groups = (g.nc + c.lt - 3) / (c.lt - 2);
+    c.ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
+    c.inv_ratio = exact_inverse_or_zero(c.ratio);
+    c.gscale = (float)(pratio * pratio);
+    const uint32_t pts = c.lt * g.nr;
+    const size_t shmem = (size_t)pts * (sizeof(float2) + sizeof(float));
+    static std::once_flag big_lds;
+    std::call_once(big_lds, [] {
+        FPM_HIP(hipFuncSetAttribute((const void*)gen_cols_fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(12288 * (sizeof(float2) + sizeof(float)))));
+    });
+    if ((size_t)b * c.groups > 0x7fffffffu) throw HipError{hipErrorInvalidValue, "generic fused columns: grid too large"};
+    hipLaunchKernelGGL(gen_cols_fused_kernel, dim3(b * c.groups, ntiles), dim3(gen_threads(pts)), shmem, st, a, c);
+    FPM_LAUNCH_CHECK();
+}
+
+void generic_forward_pages(float2* rec, uint32_t pages, const Geom& g, hipStream_t st, bool fused) {
+    if (!fused) {
+        fft2_generic(rec, rec, g.nr, g.nc, pages, false, st);
+        return;
+    }
+    GenFFT r{};
+    r.in = rec;
+    r.out = rec;
+    r.len = g.nc;
+    r.nlines = g.nr * pages;
+    r.lines_per_batch = g.nr;
+    r.line_stride = 1;
+    r.es = g.nr;
+    r.batch_stride = (uint64_t)g.nr * g.nc;
+    r.inverse = 0;
+    launch_generic_lines(r, st);
 }
 
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,

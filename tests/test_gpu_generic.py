@@ -197,3 +197,23 @@ def test_generic_any_even_side_gradient_and_step_match_fp64(n, r, grid):
     Oo, Po, _ = oracle.solve(O, P, Y, led, A, r, 1, L, opt, prec="f64")
     print(f"[generic-any] {n} -> {n * r}: one step O {rel_l2(O1, Oo):.2e} P {rel_l2(P1, Po):.2e}", flush=True)
     assert rel_l2(O1, Oo) < 1e-6 and rel_l2(P1, Po) < 1e-6
+
+
+def test_generic_launch_shapes_do_not_change_results():
+    """A solve's set-up times candidate launch shapes of its transform passes
+    and keeps the fastest per (length, line kind, source); the shape only
+    spreads lines over workgroups and threads, so a transform of lengths no
+    other test uses gives the same bits before (default shapes) and after a
+    solve of that geometry (tuned shapes)."""
+    n, r = 210, 4  # 2 3 5 7 -> 840 = 2^3 3 5 7: lengths of no other test
+    rng = np.random.default_rng(840)
+    x = np.asfortranarray((rng.standard_normal((n * r, n * r, 2)) +
+                           1j * rng.standard_normal((n * r, n * r, 2))).astype(np.complex64))
+    before = fpm_amd.fft2(x)
+    O, P, Y, led = margin_problem(n, r, 3)
+    A, _ = synth.pupil(n)
+    fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, r, 1, led.shape[1], (1e-3, 0.999, 1e-4))
+    after = fpm_amd.fft2(x)
+    assert np.array_equal(np.ascontiguousarray(before).view(np.uint32), np.ascontiguousarray(after).view(np.uint32))
+    inv = fpm_amd.fft2(after, inverse=True) / (n * r) ** 2
+    assert rel_l2(inv, x) < 3e-6

@@ -99,7 +99,7 @@ def stage_necessary_bytes(n, N, b_avg, y_bytes, win_rows):
 
 
 # stage -> kernel whose per-dispatch HBM traffic stands for it in the PMC summary
-STAGE_KERNEL = {"fused_update": "fused_update_kernel<16384, true, 0>", "stencil": "stencil_fused_kernel<2048, 0>"}
+STAGE_KERNEL = {"fused_update": "fused_update16k_kernel<true>", "stencil": "stencil_half_kernel"}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_fpm_solve_16384_b26.md")
 
 

@@ -401,6 +401,11 @@ void FN(oracle_solve)(const CT* O0, int Nr, int Nc, const CT* P0, int nr, int nc
         }
         if (loss_per_epoch) loss_per_epoch[ep] = lsum;
         if (ep >= halve_from) lr = (float)((double)lr * 0.5);
+        /* long solves name their progress when FPM_ORACLE_PROGRESS is set */
+        if (getenv("FPM_ORACLE_PROGRESS")) {
+            fprintf(stderr, "[oracle] solve %dx%d epoch %d/%d done\n", Nr, Nc, ep + 1, epochs);
+            fflush(stderr);
+        }
     }
     free(mO);
     free(mP);

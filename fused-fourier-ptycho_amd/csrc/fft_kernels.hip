@@ -13,6 +13,7 @@
 // Every pass reads and writes each element once (16 B / element / pass).
 // These replace the cufftPlan2d / cufftPlanMany transforms of
 // cuFPM_v3/cuFPM_pure_v3.cu:118-126 (unnormalised C2C, cuFFT sign convention).
+#include <algorithm>
 #include <type_traits>
 
 #include "fft_core.h"
@@ -1393,6 +1394,259 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
 }
 
 bool stencil_fused_supported(const Geom& g) { return g.nr <= 2048; }
+
+// ------------------------------------ whole per-LED pipeline of small pages
+// One workgroup runs every per-LED stage of a 128 x 128 page: the crop x
+// pupil gathered from the object spectrum (kernels.cuh:19-65, as the
+// SRC_CROP source of fft_strided_kernel forms it), the inverse transform
+// along the columns index, the inverse along the rows index, the residual
+// and loss stencil with cyclic neighbours (kernels.cuh:197-275, as the
+// whole-page stencil_fused_kernel<128> computes them), the forward
+// transform along the rows index and the forward along the columns index;
+// only R = DFT_n(u) of the rows with pupil support is written.  The page
+// (128 KiB) stays in LDS between the directions: a transform along the
+// columns index uses LineFastMap<128, 128> (lines = rows), one along the
+// rows index LineSlowMap<128, 128> (lines = columns), and each change of
+// direction is one LDS write in the one map and one read in the other.  It
+// replaces three launches (crop + inverse column pass, fused stencil,
+// forward column pass) and their two HBM round trips of the pages.
+// PERSIST: a grid of at most one workgroup per CU walks the pages
+// (page = blockIdx.x, blockIdx.x + gridDim.x, ...).
+constexpr int kPageLen = 128;
+#ifndef FPM_PAGE_FUSED
+#define FPM_PAGE_FUSED 1
+#endif
+#ifndef FPM_PAGE_PERSIST
+#define FPM_PAGE_PERSIST 0
+#endif
+
+template <int ENG, bool PERSIST>
+__global__ __launch_bounds__(1024, 4) void page_fused_kernel(CropArgs c, float2* __restrict__ out, Geom g, YStack y0,
+                                                           float ratio, float inv_ratio, float gscale,
+                                                           double* __restrict__ loss0,
+                                                           const uint8_t* __restrict__ rowmask0, Twiddles tw,
+                                                           const TileIn* __restrict__ tab, uint32_t npages) {
+    constexpr int LEN = kPageLen, T = 1024;
+    using FMap = LineFastMap<LEN, LEN>;
+    using SMap = LineSlowMap<LEN, LEN>;
+    using SJI = Slots<LEN, true, T, FMap, ENG>;   // inverse along the columns index
+    using SCI = Slots<LEN, true, T, SMap, ENG>;   // inverse along the rows index
+    using SCF = Slots<LEN, false, T, SMap, ENG>;  // forward along the rows index
+    using SJF = Slots<LEN, false, T, FMap, ENG>;  // forward along the columns index
+    static_assert(SCI::RL == SCF::R0, "inverse last pass and forward first pass must share the slot map");
+    constexpr int WORDS = FMap::LDS_WORDS > SMap::LDS_WORDS ? FMap::LDS_WORDS : SMap::LDS_WORDS;
+    __shared__ float2 lds[WORDS];
+    __shared__ float2 twl[2 * kTwLo];
+    __shared__ double red[T / 64];
+    __shared__ uint8_t rm[LEN / kMaskRows];
+    const int t = threadIdx.x;
+    tw = bind_twiddles(tw, twl, t, T);
+    EngCtx<ENG> ctx;
+    ctx.init();
+    constexpr uint32_t plane = (uint32_t)LEN * LEN;
+    constexpr uint32_t h = LEN / 2, m = LEN - 1;
+    for (uint32_t page = blockIdx.x; page < npages; page += PERSIST ? gridDim.x : npages) {
+        // this page's tile (grouped tiles: page k ppt + l is LED led0 + l of tile k)
+        uint32_t lp = page;
+        const float2* __restrict__ cS = c.Shat;
+        const float2* __restrict__ cP = c.P;
+        const int2* cled = c.led;
+        YStack y = y0;
+        double* loss_acc = loss0;
+        const uint8_t* rowmask = rowmask0;
+        if (tab) {
+            const uint32_t tk = page / c.ppt;
+            lp = page - tk * c.ppt;
+            cS += (size_t)tk * c.Nr * c.Nc;
+            cP += (size_t)tk * plane;
+            const TileIn ti = tab[tk];
+            cled = ti.led;
+            y.data = ti.y;
+            y.mn = ti.ymn;
+            y.range = ti.yrange;
+            y.rcp_range = ti.yrcp;
+            loss_acc = ti.loss;
+            if (rowmask) rowmask += (size_t)tk * (LEN / kMaskRows);
+        }
+        if (PERSIST) __syncthreads();  // the previous page's last reads of rm and red
+        for (int k = t; k < LEN / kMaskRows; k += T) rm[k] = rowmask ? rowmask[k] : 1;
+        const int2 kk = cled[c.led0 + lp];
+        const uint32_t kl = (uint32_t)kk.x, kt = (uint32_t)kk.y;
+        // crop x pupil at (row di, natural column dj): ifftshift(S P)
+        float2 v[16];
+        int tt = fresh(t);
+#pragma unroll
+        for (int q = 0; q < SJI::NG0; ++q)
+#pragma unroll
+            for (int s = 0; s < SJI::GS0; ++s) {
+                int line, idx;
+                SJI::in(tt, q, s, line, idx);
+                const uint32_t i = ((uint32_t)line + h) & m, j = ((uint32_t)idx + h) & m;
+                const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
+                const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
+                const float2 pp = cP[j * LEN + i];
+                float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
+                if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
+                v[q * SJI::GS0 + s] = z;
+            }
+        first_pass_dft<LEN, true, T, ENG>(v, ctx);
+        if (PERSIST) __syncthreads();  // the previous page's last reads of lds
+        remaining_passes<LEN, true, T, FMap, ENG>(v, lds, fresh(t), tw, ctx);
+        // columns index done: on to the rows index through LDS
+        __syncthreads();
+        tt = fresh(t);
+#pragma unroll
+        for (int q = 0; q < SJI::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SJI::GSL; ++s) {
+                int line, idx;
+                SJI::out(tt, q, s, line, idx);
+                lds[SMap::lds(idx, line)] = v[q * SJI::GSL + s];
+            }
+        __syncthreads();
+        tt = fresh(t);
+#pragma unroll
+        for (int q = 0; q < SCI::NG0; ++q)
+#pragma unroll
+            for (int s = 0; s < SCI::GS0; ++s) {
+                int line, idx;
+                SCI::in(tt, q, s, line, idx);
+                v[q * SCI::GS0 + s] = lds[SMap::lds(line, idx)];
+            }
+        first_pass_dft<LEN, true, T, ENG>(v, ctx);
+        __syncthreads();
+        remaining_passes<LEN, true, T, SMap, ENG>(v, lds, fresh(t), tw, ctx);
+        __syncthreads();
+        // residual e of every column into LDS (floats), then the stencil
+        float* ef = (float*)lds;
+        const char* ypage = (const char*)y.data + (size_t)(c.led0 + lp) * plane * (y.u16 ? 2 : 4);
+        tt = fresh(t);
+        auto residual = [&](auto u16) {
+#pragma unroll
+            for (int q = 0; q < SCI::NGL; ++q)
+#pragma unroll
+                for (int s = 0; s < SCI::GSL; ++s) {
+                    int line, idx;
+                    SCI::out(tt, q, s, line, idx);
+                    const float2 z = v[q * SCI::GSL + s];
+                    const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                    ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) -
+                                           read_y_page<decltype(u16)::value>(ypage, (uint32_t)line * LEN + (uint32_t)idx, y);
+                }
+        };
+        if (y.u16)
+            residual(std::true_type{});
+        else
+            residual(std::false_type{});
+        __syncthreads();
+        float loss32 = 0.0f;
+        tt = fresh(t);
+#pragma unroll
+        for (int q = 0; q < SCI::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SCI::GSL; ++s) {
+                int line, idx;
+                SCI::out(tt, q, s, line, idx);
+                const float* e = ef + line * LEN;
+                const float e0 = e[idx];
+                const float el = ef[((line + LEN - 1) & m) * LEN + idx];
+                const float er = ef[((line + 1) & m) * LEN + idx];
+                const float eu = e[(idx - 1) & m], ed = e[(idx + 1) & m];
+                const float gval = (copysignf(1.0f, e0 - el) - copysignf(1.0f, er - e0) + copysignf(1.0f, e0 - eu) -
+                                    copysignf(1.0f, ed - e0)) * gscale;
+                loss32 += fabsf(er - e0) + fabsf(ed - e0);
+                const float2 z = v[q * SCI::GSL + s];
+                const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                float2 o = make_float2(0.f, 0.f);
+                if (mag > 0.0f) {
+                    const float inv = 1.0f / mag;
+                    o = make_float2(z.x * inv * gval, z.y * inv * gval);
+                }
+                v[q * SCI::GSL + s] = o;
+            }
+        first_pass_dft<LEN, false, T, ENG>(v, ctx);
+        __syncthreads();
+        remaining_passes<LEN, false, T, SMap, ENG>(v, lds, fresh(t), tw, ctx);
+        // rows index done: on to the columns index through LDS
+        __syncthreads();
+        tt = fresh(t);
+#pragma unroll
+        for (int q = 0; q < SCF::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SCF::GSL; ++s) {
+                int line, idx;
+                SCF::out(tt, q, s, line, idx);
+                lds[FMap::lds(idx, line)] = v[q * SCF::GSL + s];
+            }
+        __syncthreads();
+        tt = fresh(t);
+#pragma unroll
+        for (int q = 0; q < SJF::NG0; ++q)
+#pragma unroll
+            for (int s = 0; s < SJF::GS0; ++s) {
+                int line, idx;
+                SJF::in(tt, q, s, line, idx);
+                v[q * SJF::GS0 + s] = lds[FMap::lds(line, idx)];
+            }
+        first_pass_dft<LEN, false, T, ENG>(v, ctx);
+        __syncthreads();
+        remaining_passes<LEN, false, T, FMap, ENG>(v, lds, fresh(t), tw, ctx);
+        float2* __restrict__ dst = out + (size_t)page * plane;
+        tt = fresh(t);
+#pragma unroll
+        for (int q = 0; q < SJF::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SJF::GSL; ++s) {
+                int line, idx;
+                SJF::out(tt, q, s, line, idx);
+                if (rm[line / kMaskRows]) dst[g.nl.off((uint32_t)line, (uint32_t)idx)] = v[q * SJF::GSL + s];
+            }
+        if (loss_acc) {
+            double loss = (double)loss32;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
+            if ((t & 63) == 0) red[t >> 6] = loss;
+            __syncthreads();
+            if (t == 0) {
+                double sum = 0.0;
+                for (int w = 0; w < T / 64; ++w) sum += red[w];
+                unsafeAtomicAdd(loss_acc, sum);
+            }
+        }
+    }
+}
+
+bool page_fused_supported(const Geom& g) {
+    return FPM_PAGE_FUSED && !g.generic && g.nr == (uint32_t)kPageLen && g.nc == (uint32_t)kPageLen;
+}
+
+void launch_page_fused(const Geom& g, const CropArgs& ca, float2* rec, const YStack& y, uint32_t pages, int pratio,
+                       double* loss_acc, const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng) {
+    if (!page_fused_supported(g)) throw HipError{hipErrorInvalidValue, "fused page pipeline: pages of 128 x 128 only"};
+    if (ca.tab && (ca.ppt == 0 || pages % ca.ppt != 0))
+        throw HipError{hipErrorInvalidValue, "fused page pipeline: the pages must be a multiple of the pages per tile"};
+    if (pages == 0) return;
+    Twiddles tw{twid};
+    const float ratio = (float)(g.plane() * (uint32_t)(pratio * pratio));
+    const float gscale = (float)(pratio * pratio);
+    const float inv_ratio = exact_inverse_or_zero(ratio);
+    uint32_t grid = pages;
+    if (FPM_PAGE_PERSIST) {
+        int dev = 0, cus = 0;
+        FPM_HIP(hipGetDevice(&dev));
+        FPM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        grid = std::min<uint32_t>(pages, (uint32_t)std::max(cus, 1));
+    }
+#define FPM_PF_LAUNCH(E)                                                                                         \
+    hipLaunchKernelGGL((page_fused_kernel<E, FPM_PAGE_PERSIST != 0>), dim3(grid), dim3(1024), 0, st, ca, rec, g, y, \
+                       ratio, inv_ratio, gscale, loss_acc, rowmask, tw, ca.tab, pages)
+    if (eng == ENG_SPLIT16)
+        FPM_PF_LAUNCH(ENG_SPLIT16);
+    else
+        FPM_PF_LAUNCH(ENG_F32);
+#undef FPM_PF_LAUNCH
+    FPM_LAUNCH_CHECK();
+}
 
 template <int ENG, bool SIGNS>
 static void launch_stencil_fused_eng(const Geom& g, const float2* z_cols, const YStack& y, uint32_t led0, uint32_t b,

@@ -401,28 +401,35 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     const uint8_t* rec_mask = prune ? pmask : nullptr;
     // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
     const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
-    tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
-    fft_cols_inverse(lat, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, &ca, nullptr, lat_mask, &g.nl, g.eng);
-    tm.end(col_launches(g.nl.perm));
-    if (fused) {
-        // contiguous inverse pass + stencil + contiguous forward pass in one kernel
+    if (page_fused_supported(g) && !signs) {
+        // 128 x 128 pages: every per-LED stage in one kernel
         tm.begin(FPM_STAGE_STENCIL);
-        launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, lat_mask, tw, st, g.eng, nullptr, 0, signs);
+        launch_page_fused(g, ca, rec, y, b, pratio, loss_acc, rec_mask, tw, st, g.eng);
         tm.end(1);
     } else {
-        tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
-        fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st, g.eng);
-        tm.end(1);
-        tm.begin(FPM_STAGE_STENCIL);
-        launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st, signs);
-        tm.end(1);
-        tm.begin(FPM_STAGE_FFT_N_CONTIG_LO);
-        fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st, g.eng);
-        tm.end(1);
+        tm.begin(FPM_STAGE_IFFT_N_COLS_LO);
+        fft_cols_inverse(lat, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, &ca, nullptr, lat_mask, &g.nl, g.eng);
+        tm.end(col_launches(g.nl.perm));
+        if (fused) {
+            // contiguous inverse pass + stencil + contiguous forward pass in one kernel
+            tm.begin(FPM_STAGE_STENCIL);
+            launch_stencil_fused(g, lat, y, led0, b, pratio, rec, loss_acc, lat_mask, tw, st, g.eng, nullptr, 0, signs);
+            tm.end(1);
+        } else {
+            tm.begin(FPM_STAGE_IFFT_N_CONTIG_LO);
+            fft_contig(lat, lat, g.nr, g.nc * b, true, tw, st, g.eng);
+            tm.end(1);
+            tm.begin(FPM_STAGE_STENCIL);
+            launch_stencil(g, lat, y, led0, b, pratio, rec, loss_acc, st, signs);
+            tm.end(1);
+            tm.begin(FPM_STAGE_FFT_N_CONTIG_LO);
+            fft_contig(rec, rec, g.nr, g.nc * b, false, tw, st, g.eng);
+            tm.end(1);
+        }
+        tm.begin(FPM_STAGE_FFT_N_COLS_LO);
+        fft_cols_forward(rec, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, rec_mask, &g.nl, g.eng);
+        tm.end(col_launches(g.nl.perm));
     }
-    tm.begin(FPM_STAGE_FFT_N_COLS_LO);
-    fft_cols_forward(rec, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, rec_mask, &g.nl, g.eng);
-    tm.end(col_launches(g.nl.perm));
     // stitch into Gs, pupil gradient / update, then the column passes of IDFT_N(Gs)
     if (stitch == FPM_STITCH_OWNER && b <= 64) {
         // owner-computes stitch inside the first column pass of IDFT_N; the
@@ -782,9 +789,14 @@ struct GroupSolve {
             batch_of(s, led0, b);
             const uint32_t pages = T * b;
             const CropArgs ca{spec[cur], Pw, nullptr, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN, tabd, b};
-            fft_cols_inverse(lp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, &ca, nullptr, pmask, &g.nl, g.eng, b);
-            launch_stencil_fused(g, lp, y, led0, pages, pr, rp, nullptr, pmask, tw, st, g.eng, tabd, b);
-            fft_cols_forward(rp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, pmask, &g.nl, g.eng, b);
+            if (page_fused_supported(g)) {
+                launch_page_fused(g, ca, rp, y, pages, pr, nullptr, pmask, tw, st, g.eng);
+            } else {
+                fft_cols_inverse(lp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, &ca, nullptr, pmask, &g.nl, g.eng,
+                                 b);
+                launch_stencil_fused(g, lp, y, led0, pages, pr, rp, nullptr, pmask, tw, st, g.eng, tabd, b);
+                fft_cols_forward(rp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, pmask, &g.nl, g.eng, b);
+            }
             if (separate_stitch()) {
                 launch_owner_stitch(g, rp, Pw, nullptr, led0, b, spec[1 - cur], st, tabd, T);
                 fft_cols_inverse(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, nullptr, nullptr, mk[cur], nullptr,

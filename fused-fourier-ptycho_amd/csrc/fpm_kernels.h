@@ -140,6 +140,17 @@ void launch_stencil_fused(const Geom& g, const float2* z_cols, const YStack& y, 
                           hipStream_t st, int eng = ENG_F32, const TileIn* tab = nullptr, uint32_t ppt = 0,
                           uint8_t* signs = nullptr);
 
+// the whole per-LED pipeline of 128 x 128 pages in one kernel: crop x pupil
+// of ca's spectrum, both inverse passes, the residual and loss stencil, both
+// forward passes; rec receives R (rows of rowmask's nonzero 16-row entries;
+// all rows without a mask) in g.nl's layout.  ca.tab / ca.ppt: grouped
+// tiles (pages = ntiles x ppt; images, normalisation, loss slot and row mask
+// of tile k as for launch_stencil_fused).
+struct CropArgs;
+bool page_fused_supported(const Geom& g);
+void launch_page_fused(const Geom& g, const CropArgs& ca, float2* rec, const YStack& y, uint32_t pages, int pratio,
+                       double* loss_acc, const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng);
+
 // contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
 // [+ contiguous forward pass of the new O, written back into G]
 // rowmask (optional, Nr/16 bytes): gradient rows of 0-entries are taken as zero

@@ -82,6 +82,40 @@ def test_gradient_matches_fp64_oracle(stitch, n, r, grid):
     assert abs(gl - loss) <= 1e-5 * loss
 
 
+@pytest.mark.parametrize("engine", [fpm_amd.FFT_F32, fpm_amd.FFT_SPLIT_F16])
+def test_page_pipeline_kernel_matches_the_three_kernel_pipeline(engine):
+    """128 x 128 pages run every per-LED stage in one kernel (crop x pupil,
+    both inverse passes, residual and stencil, both forward passes); the
+    gradient entry with sign export keeps the three-kernel pipeline (crop +
+    column pass, fused stencil, column pass).  Same transforms and stencil:
+    G_O, G_P and the loss agree to FP32 rounding, and both are within 1e-5 of
+    FP64 on margin inputs."""
+    O, P, Y, led = margin_problem(128, 2, 3)
+    GO, GP, loss, _ = oracle.gradient(O, P, Y, led, 2)
+    a = fpm_amd.fused_gradient(O, P, Y, led, 2, engine=engine)
+    b = fpm_amd.fused_gradient(O, P, Y, led, 2, engine=engine, signs=True)
+    print(f"[page-kernel] engine {engine}: G_O {rel_l2(a[0], b[0]):.2e} G_P {rel_l2(a[1], b[1]):.2e} apart; "
+          f"vs FP64 {rel_l2(a[0], GO):.2e} / {rel_l2(a[1], GP):.2e}", flush=True)
+    assert rel_l2(a[0], b[0]) <= 1e-6 and rel_l2(a[1], b[1]) <= 1e-6
+    assert abs(a[2] - b[2]) <= 1e-6 * abs(b[2])
+    for g in (a, b):
+        assert rel_l2(g[0], GO) <= 1e-5 and rel_l2(g[1], GP) <= 1e-5
+
+
+def test_solve_one_step_with_128_pages_matches_fp64_oracle():
+    """One minibatch step at n = 128, r = 2 (the one-kernel page pipeline in
+    update mode: only rows with pupil support are written): O and P vs FP64."""
+    n, r = 128, 2
+    O, P, Y, led = margin_problem(n, r, 3)
+    A, _ = synth.pupil(n)
+    L = led.shape[1]
+    opt = (2e-3, 0.999, 1e-4)
+    O1, P1 = fpm_amd.cuFPM_pure_v3(O, P, Y, led, A, r, 1, L, opt)
+    Oo, Po, _ = oracle.solve(O, P, Y, led, A, r, 1, L, opt, prec="f64")
+    assert rel_l2(O1, Oo) < 1e-6
+    assert rel_l2(P1, Po) < 1e-6
+
+
 def test_owner_stitch_is_bitwise_deterministic():
     O, P, Y, led = margin_problem(32, 4, 5)
     a = fpm_amd.fused_gradient(O, P, Y, led, 4, stitch=fpm_amd.STITCH_OWNER)

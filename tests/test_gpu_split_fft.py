@@ -70,7 +70,7 @@ def test_split_fft2_zero_and_delta():
     assert np.max(np.abs(got - 1.0)) < 1e-6
 
 
-@pytest.mark.parametrize("n,r,grid", [(32, 4, 5), (64, 16, 3), (1024, 2, 3)])
+@pytest.mark.parametrize("n,r,grid", [(32, 4, 5), (64, 16, 3), (128, 2, 3), (1024, 2, 3)])
 def test_split_gradient_vs_fp64_oracle(n, r, grid):
     """One-minibatch gradient with every transform on the split engine vs
     the FP64 oracle: rel-L2 <= 1e-5, the bar of the FP32 form."""

@@ -18,12 +18,14 @@
 #include "common.h"
 #include "fpm_kernels.h"
 #include "generic.h"
+#include "fft_core.h"
 
 #include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #ifndef FPM_GEN_CONTIG_POINTS
@@ -44,6 +46,10 @@
 #endif
 #ifndef FPM_GEN_FUSED_MAX
 #define FPM_GEN_FUSED_MAX 2048
+#endif
+// lines of 3 2^k points as a radix-3 step + three power-of-two transforms (fft3m_kernel)
+#ifndef FPM_GEN_3M
+#define FPM_GEN_3M 1
 #endif
 // launch shapes of the transform passes measured at a solve's set-up
 #ifndef FPM_GEN_TUNE
@@ -361,6 +367,190 @@ __global__ __launch_bounds__(FPM_GEN_LB) void fft_generic_kernel(GenFFT a, GenCr
     }
 }
 
+// ---- lines of N = 3 M points (M = 2^k, 16 <= M <= 4096): 192, 384, 768, 1536 ...
+// Cooley-Tukey with N = 3 x M, n = n2 + M n1, k = k1 + 3 k2:
+//   X[k1 + 3 k2] = sum_n2 W_M^(n2 k2) (W_N^(n2 k1) sum_n1 x[n2 + M n1] W_3^(n1 k1))
+// A workgroup stages NL = P / M lines in LDS (natural order, coalesced from
+// global memory whichever way the lines lie), forms the radix-3 step
+// (small_dft<3>'s arithmetic) and the twiddle W_N^(n2 k1) (the FP64-made
+// table of W_N) straight into the first-pass registers of 3 NL sub-lines of
+// M points, runs the power-of-two Stockham passes of fft_core.h on them
+// (radix 16 in registers, one LDS exchange per pass instead of the generic
+// engine's one per radix-4/2/3 factor) and stages X[k1 + 3 k2] back in
+// natural order for a coalesced store.  3 P / 16 threads, 3 P (17/16) words
+// of LDS: P = 1024 for contiguous lines (192 threads, 26 KiB: several
+// workgroups per CU overlap their load, transform and store phases), P =
+// 16 M for strided lines (16 adjacent lines make 128-byte runs).
+// The transform of nl <= NL lines of N = 3 M points staged in LDS (natural
+// order, pitch N + 1), in place: radix-3 step + twiddle into the first-pass
+// registers of the 3 NL sub-lines, the power-of-two passes, X[k1 + 3 k2]
+// back to the staged layout.  twN: W_N^k, k < N.  Ends with a barrier.
+template <int M, int NL> struct Lines3m {
+    static constexpr int N = 3 * M, SUB = 3 * NL, T = SUB * M / 16, LD = N + 1;
+    using Map = LineSlowMap<SUB, M>;
+    static constexpr int WORDS = Map::LDS_WORDS > NL * LD ? Map::LDS_WORDS : NL * LD;
+};
+template <int M, int NL, bool INV>
+__device__ __forceinline__ void fft3m_lines(float2* lds, uint32_t nl, const float2* __restrict__ twN,
+                                            const Twiddles& tw, int t) {
+    using C = Lines3m<M, NL>;
+    constexpr int T = C::T, LD = C::LD;
+    using SL = Slots<M, INV, T, typename C::Map, ENG_F32>;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < SL::NG0; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::GS0; ++s) {
+            int line, n2;
+            SL::in(t, q, s, line, n2);
+            const uint32_t l = (uint32_t)line / 3u, k1 = (uint32_t)line - 3u * l;
+            float2 y = make_float2(0.f, 0.f);
+            if (l < nl) {
+                const float2* xl = lds + l * LD + n2;
+                float2 x3[3] = {xl[0], xl[M], xl[2 * M]};
+                small_dft<3>(x3, INV);
+                y = k1 == 0 ? x3[0] : (k1 == 1 ? x3[1] : x3[2]);
+                if (k1 != 0) {
+                    float2 w = twN[(uint32_t)n2 * k1];  // n2 k1 < 2 M < N
+                    if (INV) w.y = -w.y;
+                    y = cmul_g(y, w);
+                }
+            }
+            v[q * SL::GS0 + s] = y;
+        }
+    first_pass_dft<M, INV, T>(v);
+    __syncthreads();
+    remaining_passes<M, INV, T, typename C::Map>(v, lds, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SL::NGL; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::GSL; ++s) {
+            int line, k2;
+            SL::out(t, q, s, line, k2);
+            const uint32_t l = (uint32_t)line / 3u, k1 = (uint32_t)line - 3u * l;
+            if (l < nl) lds[l * LD + k1 + 3u * (uint32_t)k2] = v[q * SL::GSL + s];
+        }
+    __syncthreads();
+}
+
+template <int M, int P, bool INV, bool CROP>
+__global__ __launch_bounds__(3 * P / 16) void fft3m_kernel(GenFFT a, GenCrop c, Twiddles tw) {
+    constexpr int N = 3 * M, NL = P / M, T = 3 * P / 16, LD = N + 1;
+    static_assert(NL >= 1 && NL * N == 16 * T, "16 points per thread");
+    __shared__ float2 lds[Lines3m<M, NL>::WORDS];
+    __shared__ float2 twl[2 * kTwLo];
+    const int t = threadIdx.x;
+    tw = bind_twiddles(tw, twl, t, T);
+    const uint32_t L0 = blockIdx.x * (uint32_t)NL;
+    const uint32_t nl = min((uint32_t)NL, a.nlines - L0);
+    const uint32_t tot = nl * (uint32_t)N;
+    const bool line_fast = a.es != 1;  // strided lines: adjacent lines adjacent in memory
+    // (line, element) of a staging index: line fastest for strided lines
+    auto split = [&](uint32_t idx, uint32_t& l, uint32_t& e) {
+        if (line_fast) {
+            e = nl == (uint32_t)NL ? idx / (uint32_t)NL : idx / nl;
+            l = idx - e * nl;
+        } else {
+            l = idx / (uint32_t)N;
+            e = idx - l * (uint32_t)N;
+        }
+    };
+    auto load = [&](uint32_t l, uint32_t e) -> float2 {
+        if constexpr (CROP) {
+            const uint32_t L = L0 + l, pg = fdiv(L, a.f_lpb);
+            return gen_crop_value(c, pg, L - pg * a.lines_per_batch, e);
+        } else {
+            return a.in[gen_off(a, L0 + l, e)];
+        }
+    };
+    // a full workgroup moves exactly 16 points per thread: unrolled, so every
+    // thread keeps its 16 loads in flight at once
+    if (nl == (uint32_t)NL) {
+        float2 buf[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint32_t l, e;
+            split((uint32_t)(t + i * T), l, e);
+            buf[i] = load(l, e);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint32_t l, e;
+            split((uint32_t)(t + i * T), l, e);
+            lds[l * LD + e] = buf[i];
+        }
+    } else {
+        for (uint32_t idx = t; idx < tot; idx += T) {
+            uint32_t l, e;
+            split(idx, l, e);
+            lds[l * LD + e] = load(l, e);
+        }
+    }
+    __syncthreads();
+    fft3m_lines<M, NL, INV>(lds, nl, a.tw, tw, t);
+    if (nl == (uint32_t)NL) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint32_t l, e;
+            split((uint32_t)(t + i * T), l, e);
+            a.out[gen_off(a, L0 + l, e)] = lds[l * LD + e];
+        }
+    } else {
+        for (uint32_t idx = t; idx < tot; idx += T) {
+            uint32_t l, e;
+            split(idx, l, e);
+            a.out[gen_off(a, L0 + l, e)] = lds[l * LD + e];
+        }
+    }
+}
+
+// M of a length that the 3 x 2^k kernel takes (0: none); strided lines only
+// up to 768 points (16 lines of a workgroup make 128-byte runs)
+uint32_t len3m(uint32_t len, bool strided) {
+    if (!FPM_GEN_3M || len % 3 != 0) return 0;
+    const uint32_t m = len / 3;
+    if (m < 16 || m > 4096 || (m & (m - 1)) != 0) return 0;
+    if (strided && m > 256) return 0;
+    return m;
+}
+
+template <int M, int P, bool INV, bool CROP>
+void launch_3m_shape(const GenFFT& a, const GenCrop& c, const Twiddles& tw, hipStream_t st) {
+    constexpr int NL = P / M;
+    hipLaunchKernelGGL((fft3m_kernel<M, P, INV, CROP>), dim3((a.nlines + NL - 1) / NL), dim3(3 * P / 16), 0, st, a, c,
+                       tw);
+}
+
+template <bool INV, bool CROP>
+void launch_3m(uint32_t m, const GenFFT& a, const GenCrop& c, hipStream_t st) {
+    int dev = 0;
+    FPM_HIP(hipGetDevice(&dev));
+    const Twiddles tw{twiddle_table(dev)};
+    const bool strided = a.es != 1;
+#define FPM_3M_CASE(MM, PC, PS)                                                    \
+    case MM:                                                                       \
+        if (strided)                                                               \
+            launch_3m_shape<MM, PS, INV, CROP>(a, c, tw, st);                      \
+        else                                                                       \
+            launch_3m_shape<MM, PC, INV, CROP>(a, c, tw, st);                      \
+        break;
+    switch (m) {
+        FPM_3M_CASE(16, 1024, 1024)
+        FPM_3M_CASE(32, 1024, 1024)
+        FPM_3M_CASE(64, 1024, 1024)
+        FPM_3M_CASE(128, 1024, 2048)
+        FPM_3M_CASE(256, 1024, 4096)
+        FPM_3M_CASE(512, 1024, 1024)
+        FPM_3M_CASE(1024, 1024, 1024)
+        FPM_3M_CASE(2048, 2048, 2048)
+        FPM_3M_CASE(4096, 4096, 4096)
+        default: throw HipError{hipErrorInvalidValue, "3 x 2^k transform: unsupported length"};
+    }
+#undef FPM_3M_CASE
+    FPM_LAUNCH_CHECK();
+}
+
 const float2* generic_twiddles(uint32_t len) {
     static std::mutex mu;
     static std::map<std::pair<int, uint32_t>, float2*> tables;
@@ -469,6 +659,14 @@ GenShape shape_of(const GenFFT& a, bool crop) {
 
 void launch_generic_lines(GenFFT a, hipStream_t st, const GenCrop* crop = nullptr, const GenShape* force = nullptr) {
     plan_generic(a);
+    if (const uint32_t m = len3m(a.len, a.es != 1)) {
+        const GenCrop c = crop ? *crop : GenCrop{};
+        if (a.inverse)
+            crop ? launch_3m<true, true>(m, a, c, st) : launch_3m<true, false>(m, a, c, st);
+        else
+            crop ? launch_3m<false, true>(m, a, c, st) : launch_3m<false, false>(m, a, c, st);
+        return;
+    }
     const GenShape sh = force ? *force : shape_of(a, crop != nullptr);
     a.tl = std::max<uint32_t>(1, sh.pts / a.len);
     a.f_tl = make_fastdiv(a.tl);
@@ -871,6 +1069,7 @@ namespace {
 // the fastest shape of pass a (its output redirected to scratch), timed over
 // three launches per candidate after one untimed launch
 GenShape tune_pass(GenFFT a, const GenCrop* crop, float2* scratch, hipStream_t st) {
+    if (len3m(a.len, a.es != 1)) return shape_of(a, crop != nullptr);  // fixed shape: nothing to tune
     a.out = scratch;
     std::vector<GenShape> cand;
     std::vector<std::pair<uint32_t, uint32_t>> seen;  // (lines per workgroup, threads)

@@ -34,6 +34,26 @@ def test_generic_fft_matches_numpy(rows, cols, batch):
     assert rel_l2(goti, inv) < 2e-6
 
 
+@pytest.mark.parametrize("rows,cols,batch", [(48, 96, 3), (96, 48, 2), (384, 24, 1), (1536, 20, 1), (6144, 18, 1),
+                                             (12288, 16, 1), (20, 768, 2), (26, 384, 1), (30, 1536, 1), (50, 192, 1)])
+def test_3x2k_line_kernel_matches_numpy(rows, cols, batch):
+    """Lines of 3 x 2^k points run a radix-3 step into three power-of-two
+    transforms (fft3m_kernel): every M from 16 to 4096 along the contiguous
+    index (48 .. 12288 rows), M <= 256 along the strided index (48 .. 768
+    columns; 1536 columns take the generic kernel), workgroups holding fewer
+    lines than their capacity (24 lines of 48, 50 rows of 192 ...), both
+    directions, against numpy in FP64."""
+    rng = np.random.default_rng(rows * 7 + cols)
+    x = np.asfortranarray((rng.standard_normal((rows, cols, batch)) + 1j * rng.standard_normal((rows, cols, batch)))
+                          .astype(np.complex64))
+    ref = np.fft.fft2(x.astype(np.complex128), axes=(0, 1))
+    got = np.stack([fpm_amd.fft2(x[:, :, k]) for k in range(batch)], axis=2)
+    assert rel_l2(got, ref) < 2e-6
+    inv = np.fft.ifft2(x.astype(np.complex128), axes=(0, 1)) * rows * cols
+    goti = np.stack([fpm_amd.fft2(x[:, :, k], inverse=True) for k in range(batch)], axis=2)
+    assert rel_l2(goti, inv) < 2e-6
+
+
 @pytest.mark.parametrize("n,r,grid", [(48, 4, 5), (24, 2, 3), (96, 2, 3)])
 def test_generic_gradient_matches_fp64_oracle(n, r, grid):
     O, P, Y, led = margin_problem(n, r, grid)

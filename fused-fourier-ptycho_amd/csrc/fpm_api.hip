@@ -1217,10 +1217,46 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
             std::vector<double> cand_ms(cands.size(), 0.0);
             int K = tune ? -1 : cands[0];
             int cur = 0;
+            // With tuning, epoch 0 runs the sequences one after another on the
+            // call's stream with an event after each: their measured times
+            // replace the size model in the largest-first assignment of every
+            // candidate stream count timed after it.
+            const bool measure = tune && nseq > 1;
+            const int skip = measure ? 1 : 0;
             for (int e = 0; e < ne; ++e) {
                 const bool last = e == ne - 1;
-                const bool timing = K < 0 && e < (int)cands.size();
-                const int Ke = timing ? cands[e] : K;
+                if (measure && e == 0) {
+                    struct Evs {
+                        std::vector<hipEvent_t> v;
+                        ~Evs() {
+                            for (auto x : v) (void)hipEventDestroy(x);
+                        }
+                    } ev;
+                    for (int q = 0; q <= nseq; ++q) {
+                        hipEvent_t x;
+                        FPM_HIP(hipEventCreate(&x));
+                        ev.v.push_back(x);
+                    }
+                    FPM_HIP(hipEventRecord(ev.v[0], st));
+                    for (int i = 0; i < nseq; ++i) {
+                        seqs[order[i]].enqueue(cur, last, st);
+                        FPM_HIP(hipEventRecord(ev.v[i + 1], st));
+                    }
+                    FPM_HIP(hipEventSynchronize(ev.v[nseq]));
+                    for (int i = 0; i < nseq; ++i) {
+                        float ms = 0.f;
+                        FPM_HIP(hipEventElapsedTime(&ms, ev.v[i], ev.v[i + 1]));
+                        seqs[order[i]].work = (double)ms;
+                    }
+                    std::stable_sort(order.begin(), order.end(),
+                                     [&](int a, int b) { return seqs[a].work > seqs[b].work; });
+                    if (last) K = 1;
+                    cur = (int)((cur + steps) & 1u);
+                    continue;
+                }
+                const int ce = e - skip;  // candidate index
+                const bool timing = K < 0 && ce < (int)cands.size();
+                const int Ke = timing ? cands[ce] : K;
                 const int key = (Ke * 2 + cur) * 2 + (last ? 1 : 0);
                 hipGraphExec_t ex = graphs.get(key, st, [&] { enqueue_all(Ke, cur, last); });
                 if (timing) {
@@ -1230,9 +1266,9 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
                     FPM_HIP(hipEventSynchronize(tb));
                     float ms = 0.f;
                     FPM_HIP(hipEventElapsedTime(&ms, ta, tb));
-                    cand_ms[e] = ms;
-                    if (e == (int)cands.size() - 1 || last) {
-                        const int nt = e + 1;
+                    cand_ms[ce] = ms;
+                    if (ce == (int)cands.size() - 1 || last) {
+                        const int nt = ce + 1;
                         K = cands[(int)(std::min_element(cand_ms.begin(), cand_ms.begin() + nt) - cand_ms.begin())];
                     }
                 } else {
@@ -1249,7 +1285,7 @@ int fpm_tiles_solve(const fpm_tile* tiles, int32_t ntiles, int64_t L, double pra
             }
             if (report) {
                 report->streams = K;
-                report->tuned = tune ? (int32_t)std::min<size_t>(cands.size(), (size_t)ne) : 0;
+                report->tuned = tune ? (int32_t)std::min<size_t>(cands.size(), (size_t)std::max(0, ne - skip)) : 0;
                 for (size_t k = 0; k < cands.size() && k < 4; ++k) {
                     report->cand[k] = cands[k];
                     report->cand_ms[k] = cand_ms[k];

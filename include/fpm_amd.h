@@ -321,8 +321,10 @@ int fpm_fft2_engine(const fpm_c32* in, fpm_c32* out, int64_t rows, int64_t cols,
  * Each epoch of all launch sequences is captured once as one hipGraph whose
  * sequences are spread over `streams` concurrent streams, largest first
  * (longest-processing-time assignment); streams = 0 autotunes: the first
- * epochs run with 1, 2, 4 and 8 streams in turn (at most one per sequence),
- * each timed, and the fastest count runs the rest.  Grouping and the stream
+ * epoch runs the sequences one after another and times each, and these
+ * measured times order and assign them from then on; the next epochs run
+ * with 1, 2, 4 and 8 streams in turn (at most one per sequence), each timed,
+ * and the fastest count runs the rest.  Grouping and the stream
  * count change only the launches, never the arithmetic: every tile's result
  * is the same bits as its own fpm_pure_v3 call.
  * ------------------------------------------------------------------------ */

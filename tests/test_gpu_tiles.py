@@ -55,8 +55,10 @@ def test_tiles_solve_equals_per_tile_solves(streams, launch):
     print(f"[tiles] streams={streams} launch={launch}: report {rep}", flush=True)
     nseq = 2 if launch == ft.TILES_GROUPED else len(tl)
     assert rep["sequences"] == nseq and rep["grouped_tiles"] == (len(tl) if launch == ft.TILES_GROUPED else 0)
-    if streams == 0:  # candidates 1, 2 (grouped) or 1, 2, 4, 8 (per tile), one per epoch
-        assert rep["tuned"] == (2 if launch == ft.TILES_GROUPED else 3) and rep["streams"] in rep["candidates"]
+    if streams == 0:
+        # epoch 0 times each sequence; then candidates 1, 2 (grouped) or 1, 2, 4, 8 (per
+        # tile), one per epoch: the 2 epochs left time two of them
+        assert rep["tuned"] == 2 and rep["streams"] in rep["candidates"]
     else:
         assert rep["streams"] == min(streams, nseq) and rep["tuned"] == 0
     for k, t in enumerate(tl):

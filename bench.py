@@ -91,11 +91,12 @@ def stage_necessary_bytes(n, N, b_avg, y_bytes, win_rows):
     """HBM bytes the fused kernels of a stage must move at least (their own
     inputs and outputs once each; no credit for the stages they absorb):
     fused_update reads O and m_O and the gradient rows inside the step's
-    windows and writes O, m_O and the next spectrum: 32 N^2 + 8 N rows; the
-    fused stencil reads the inverse-transformed pages' inputs and Y and
-    writes the pages: (8 + 8 + y) b n^2."""
+    windows and writes O, m_O and the next spectrum's rows inside the next
+    step's windows: 24 N^2 + 8 N (rows + next rows), with the mean window
+    rows for both; the fused stencil reads the inverse-transformed pages'
+    inputs and Y and writes the pages: (8 + 8 + y) b n^2."""
     NN, nn = float(N) * N, float(n) * n
-    return {"fused_update": 32 * NN + 8.0 * N * win_rows, "stencil": (16 + y_bytes) * b_avg * nn}
+    return {"fused_update": 24 * NN + 16.0 * N * win_rows, "stencil": (16 + y_bytes) * b_avg * nn}
 
 
 # stage -> kernel whose per-dispatch HBM traffic stands for it in the PMC summary

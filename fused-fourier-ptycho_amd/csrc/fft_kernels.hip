@@ -576,6 +576,7 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
                                                                         float2* __restrict__ O,
                                                                         float* __restrict__ mO, uint32_t nlines,
                                                                         RmsArgs rms, const uint8_t* __restrict__ rowmask,
+                                                                        const uint8_t* __restrict__ nextmask,
                                                                         uint32_t lines_per_tile, Twiddles tw) {
     using Cfg = ContigCfg<LEN>;
     using Map = typename Cfg::Map;
@@ -587,11 +588,20 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
     // rows of the gradient known to be zero (outside every LED window of the
     // batch): not read, taken as 0 -- the column passes skipped them
     __shared__ uint8_t rm[LEN / kMaskRows];
+    // rows of the next spectrum that the next step's column passes process
+    // (its LED windows): the only rows of the forward pass's output written
+    __shared__ uint8_t nm[LEN / kMaskRows];
     const int t = threadIdx.x;
     const size_t line0 = (size_t)blockIdx.x * Cfg::LINES;
     const uint8_t* rmk = rowmask;
-    if (rmk && lines_per_tile) rmk += (line0 / lines_per_tile) * (size_t)(LEN / kMaskRows);
-    for (int k = t; k < LEN / kMaskRows; k += Cfg::T) rm[k] = rmk ? rmk[k] : 1;
+    const uint8_t* nmk = nextmask;
+    const size_t mofs = lines_per_tile ? (line0 / lines_per_tile) * (size_t)(LEN / kMaskRows) : 0;
+    if (rmk) rmk += mofs;
+    if (nmk) nmk += mofs;
+    for (int k = t; k < LEN / kMaskRows; k += Cfg::T) {
+        rm[k] = rmk ? rmk[k] : 1;
+        nm[k] = nmk ? nmk[k] : 1;
+    }
     tw = bind_twiddles(tw, twl, t, Cfg::T);
     EngCtx<ENG> ctx;
     ctx.init();
@@ -641,7 +651,8 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
                 int line, idx;
                 SF::out(t, q, s, line, idx);
                 const size_t gl = line0 + line;
-                if (gl < nlines) st_s<(FPM_NT_UPDATE >= 2)>(G + gl * LEN + idx, v[q * SF::GSL + s]);
+                if (gl < nlines && nm[idx / kMaskRows])
+                    st_s<(FPM_NT_UPDATE >= 2)>(G + gl * LEN + idx, v[q * SF::GSL + s]);
             }
     }
 }
@@ -765,6 +776,7 @@ template <bool FWD>
 __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restrict__ G, float2* __restrict__ O,
                                                                  float* __restrict__ mO, uint32_t nlines, RmsArgs rms,
                                                                  const uint8_t* __restrict__ rowmask,
+                                                                 const uint8_t* __restrict__ nextmask,
                                                                  uint32_t lines_per_tile, Twiddles tw) {
     using namespace half16k;
     using MapG = LineSlowMap<1, LEN>;  // slot arithmetic of the one-piece kernel (no LDS)
@@ -772,12 +784,18 @@ __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restr
     using SF = Slots<LEN, false, TV, MapG, ENG_F32>;
     __shared__ float2 lds[HMap::LDS_WORDS];
     __shared__ float2 twl[2 * kTwLo];
+    // per 16-row tile: bit 0 = gradient rows read (else taken as 0), bit 1 =
+    // rows of the next spectrum written (the next step's LED windows)
     __shared__ uint8_t rm[LEN / kMaskRows];
     const int t = threadIdx.x;
     const size_t line = blockIdx.x;  // one line per workgroup; the grid is nlines wide
     const uint8_t* rmk = rowmask;
-    if (rmk && lines_per_tile) rmk += (line / lines_per_tile) * (size_t)(LEN / kMaskRows);
-    for (int k = t; k < LEN / kMaskRows; k += 512) rm[k] = rmk ? rmk[k] : 1;
+    const uint8_t* nmk = nextmask;
+    const size_t mofs = lines_per_tile ? (line / lines_per_tile) * (size_t)(LEN / kMaskRows) : 0;
+    if (rmk) rmk += mofs;
+    if (nmk) nmk += mofs;
+    for (int k = t; k < LEN / kMaskRows; k += 512)
+        rm[k] = (uint8_t)(((rmk ? rmk[k] : 1) != 0 ? 1 : 0) | ((nmk ? nmk[k] : 1) != 0 ? 2 : 0));
     tw = bind_twiddles(tw, twl, t, 512);
     const float lr = rms_lr(rms);
     float2 va[16], vb[16];
@@ -788,7 +806,7 @@ __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restr
             for (int s = 0; s < SI::GS0; ++s) {
                 int ln, idx;
                 SI::in(tv, q, s, ln, idx);
-                v[q * SI::GS0 + s] = rm[idx / kMaskRows] ? G[line * LEN + idx] : make_float2(0.f, 0.f);
+                v[q * SI::GS0 + s] = (rm[idx / kMaskRows] & 1) ? G[line * LEN + idx] : make_float2(0.f, 0.f);
             }
     };
     load_g(va, t);
@@ -854,7 +872,7 @@ __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restr
                 for (int s = 0; s < SF::GSL; ++s) {
                     int ln, idx;
                     SF::out(tv, q, s, ln, idx);
-                    st_s<(FPM_NT_UPDATE >= 2)>(G + line * LEN + idx, v[q * SF::GSL + s]);
+                    if (rm[idx / kMaskRows] & 2) st_s<(FPM_NT_UPDATE >= 2)>(G + line * LEN + idx, v[q * SF::GSL + s]);
                 }
         };
         store(va, t);
@@ -865,17 +883,18 @@ __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restr
 
 template <int ENG>
 static void launch_fused_update_eng(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                                    const uint8_t* rowmask, const float2* twid, hipStream_t st, uint32_t ntiles) {
+                                    const uint8_t* rowmask, const uint8_t* nextmask, const float2* twid,
+                                    hipStream_t st, uint32_t ntiles) {
     Twiddles tw{twid};
     const uint32_t nlines = g.Nc * ntiles;
     const uint32_t lpt = ntiles > 1 ? g.Nc : 0u;
     if (FPM_FUSED16K_HALF && ENG == ENG_F32 && g.Nr == 16384) {
         if (forward_after)
             hipLaunchKernelGGL((fused_update16k_kernel<true>), dim3(nlines), dim3(512), 0, st, G, O, mO, nlines, rms,
-                               rowmask, lpt, tw);
+                               rowmask, nextmask, lpt, tw);
         else
             hipLaunchKernelGGL((fused_update16k_kernel<false>), dim3(nlines), dim3(512), 0, st, G, O, mO, nlines, rms,
-                               rowmask, lpt, tw);
+                               rowmask, nullptr, lpt, tw);
         FPM_LAUNCH_CHECK();
         return;
     }
@@ -884,10 +903,10 @@ static void launch_fused_update_eng(const Geom& g, float2* G, float2* O, float* 
         dim3 grid((nlines + ContigCfg<L>::LINES - 1) / ContigCfg<L>::LINES);                                \
         if (forward_after)                                                                                  \
             hipLaunchKernelGGL((fused_update_kernel<L, true, ENG>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
-                               rms, rowmask, lpt, tw);                                                      \
+                               rms, rowmask, nextmask, lpt, tw);                                            \
         else                                                                                                \
             hipLaunchKernelGGL((fused_update_kernel<L, false, ENG>), grid, dim3(ContigCfg<L>::T), 0, st, G, O, mO, nlines, \
-                               rms, rowmask, lpt, tw);                                                      \
+                               rms, rowmask, nullptr, lpt, tw);                                             \
         break;                                                                                              \
     }
     switch (g.Nr) {
@@ -909,13 +928,14 @@ static void launch_fused_update_eng(const Geom& g, float2* G, float2* O, float* 
 }
 
 void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng, uint32_t ntiles) {
+                         const uint8_t* rowmask, const uint8_t* nextmask, const float2* twid, hipStream_t st, int eng,
+                         uint32_t ntiles) {
     if (ntiles > 1 && !fused_update_groupable(g))
         throw HipError{hipErrorInvalidValue, "grouped fused update: a workgroup would span two objects"};
     if (eng == ENG_SPLIT16)
-        launch_fused_update_eng<ENG_SPLIT16>(g, G, O, mO, rms, forward_after, rowmask, twid, st, ntiles);
+        launch_fused_update_eng<ENG_SPLIT16>(g, G, O, mO, rms, forward_after, rowmask, nextmask, twid, st, ntiles);
     else
-        launch_fused_update_eng<ENG_F32>(g, G, O, mO, rms, forward_after, rowmask, twid, st, ntiles);
+        launch_fused_update_eng<ENG_F32>(g, G, O, mO, rms, forward_after, rowmask, nextmask, twid, st, ntiles);
 }
 
 bool fused_update_groupable(const Geom& g) { return g.Nc % (uint32_t)contig_lines((int)g.Nr) == 0; }
@@ -1091,7 +1111,7 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
         for (int s = 0; s < SF::GSL; ++s) {
             int line, idx;
             SF::out(tt, q, s, line, idx);
-            if (interior(line)) dst[at(idx, line)] = v[q * SF::GSL + s];
+            if (interior(line) && rm[idx / kMaskRows]) dst[at(idx, line)] = v[q * SF::GSL + s];
         }
     if (loss_acc) {
         double loss = (double)loss32;
@@ -1374,7 +1394,7 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
             for (int s = 0; s < SF::GSL; ++s) {
                 int line, idx;
                 SF::out(tv, q, s, line, idx);
-                if (interior(line)) dst[at(idx, line)] = v[q * SF::GSL + s];
+                if (interior(line) && rm[idx / kMaskRows]) dst[at(idx, line)] = v[q * SF::GSL + s];
             }
     };
     store(va, t);

@@ -562,7 +562,8 @@ struct Solve {
                 launch_row_mask(led, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st);
             }
             tm.begin(FPM_STAGE_FUSED_UPDATE);
-            launch_fused_update(g, spec[1 - cur], O, mO.as<float>(), rms, !last, mk[cur], tw, st, g.eng);
+            launch_fused_update(g, spec[1 - cur], O, mO.as<float>(), rms, !last, mk[cur], last ? nullptr : mk[1 - cur],
+                                tw, st, g.eng);
             tm.end(1);
             if (!last) {
                 tm.begin(FPM_STAGE_FFT_N_COLS_HI);
@@ -814,7 +815,8 @@ struct GroupSolve {
                 batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
                 launch_row_mask(nullptr, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st, tabd, T);
             }
-            launch_fused_update(g, spec[1 - cur], O.as<float2>(), mO.as<float>(), rms, !last, mk[cur], tw, st, g.eng,
+            launch_fused_update(g, spec[1 - cur], O.as<float2>(), mO.as<float>(), rms, !last, mk[cur],
+                                last ? nullptr : mk[1 - cur], tw, st, g.eng,
                                 T);
             if (!last) {
                 fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng, 1);

@@ -129,7 +129,8 @@ void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const 
 // contiguous inverse pass + loss stencil + contiguous forward pass of a batch
 // of n-side pages in one kernel (z_cols: pages after their inverse column
 // passes; u_out: pages before their forward column passes); nr <= 2048.
-// rowmask (optional, nr/16 bytes): input rows of 0-entries are taken as zero.
+// rowmask (optional, nr/16 bytes): input rows of 0-entries are taken as zero and
+// output rows of 0-entries are not written (the forward column pass skips them).
 // tab (optional): grouped tiles, b = ntiles x ppt pages, tile k = pages
 // k ppt..; its images, normalisation, loss slot from tab[k], its row mask at
 // rowmask + k nr/16.
@@ -154,11 +155,14 @@ void launch_page_fused(const Geom& g, const CropArgs& ca, float2* rec, const YSt
 // contiguous inverse pass of the object gradient + RMSprop of O (kernels.cuh:364-382)
 // [+ contiguous forward pass of the new O, written back into G]
 // rowmask (optional, Nr/16 bytes): gradient rows of 0-entries are taken as zero
+// nextmask (optional, Nr/16 bytes): only rows of the forward pass's output in
+// 16-row tiles of nonzero entries are written -- the rows the next step's
+// column passes process (its LED windows); the others are left as they were
 // ntiles > 1: grouped tiles, G, O, mO hold ntiles objects back to back and
-// rowmask ntiles masks (fused_update_groupable must hold)
+// rowmask / nextmask ntiles masks (fused_update_groupable must hold)
 void launch_fused_update(const Geom& g, float2* G, float2* O, float* mO, RmsArgs rms, bool forward_after,
-                         const uint8_t* rowmask, const float2* twid, hipStream_t st, int eng = ENG_F32,
-                         uint32_t ntiles = 1);
+                         const uint8_t* rowmask, const uint8_t* nextmask, const float2* twid, hipStream_t st,
+                         int eng = ENG_F32, uint32_t ntiles = 1);
 // every workgroup of the fused update stays within one object's lines
 bool fused_update_groupable(const Geom& g);
 

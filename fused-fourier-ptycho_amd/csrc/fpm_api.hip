@@ -547,7 +547,7 @@ struct Solve {
     // captured and replayed.
     void enqueue_epoch(int cur, bool last_epoch, hipStream_t st, StageTimer& tm) {
         if (g.generic) {
-            enqueue_epoch_generic(st, tm);
+            enqueue_epoch_generic(cur, last_epoch, st, tm);
             return;
         }
         uint32_t led0, b;
@@ -601,8 +601,13 @@ struct Solve {
         ss.halve_from = halve_from > 0 ? (uint32_t)halve_from : 13u;
         launch_solve_state_init(ss, lr, st);
         rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
+        spec[0] = S0.as<float2>();
+        spec[1] = S1.as<float2>();
         generic_prepare(g);
         generic_tune(g, 1, B, O, S0.as<float2>(), P, led, nullptr, ws.lat, st);
+        // with the fused update the spectrum is carried from step to step
+        // (S0 / S1 alternate as on the power-of-two path): the first one here
+        if (generic_update_fusable(g)) fft2_generic(O, spec[0], g.Nr, g.Nc, 1, false, st);
     }
 
     // Stage events: each whole 2D transform is recorded under its column-pass
@@ -610,19 +615,26 @@ struct Solve {
     // per-LED transforms; IFFT_N_COLS_HI: IDFT_N of the gradient), the
     // residual + stencil under STENCIL and the RMSprop of O under
     // FUSED_UPDATE.
-    void enqueue_epoch_generic(hipStream_t st, StageTimer& tm) {
+    // With the fused update (N_rows = 3 x 2^k) the step's spectrum comes from
+    // the previous step (spec[cur]) and the gradient's strided inverse pass,
+    // the fused contiguous inverse + RMSprop + contiguous forward pass and the
+    // strided forward pass leave the next one in spec[1 - cur].
+    void enqueue_epoch_generic(int cur, bool last_epoch, hipStream_t st, StageTimer& tm) {
         const size_t NN = (size_t)g.Nr * g.Nc;
-        float2* S = S0.as<float2>();
-        float2* G = S1.as<float2>();
+        const bool fuse_update = generic_update_fusable(g);
         float2* lat = ws.lat;
         float2* rec = ws.rec;
         const bool fused = generic_fused_pages(g, false);
         uint32_t led0, b;
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
-            tm.begin(FPM_STAGE_FFT_N_COLS_HI);
-            fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
-            tm.end(2);
+            float2* S = fuse_update ? spec[cur] : S0.as<float2>();
+            float2* G = fuse_update ? spec[1 - cur] : S1.as<float2>();
+            if (!fuse_update) {
+                tm.begin(FPM_STAGE_FFT_N_COLS_HI);
+                fft2_generic(O, S, g.Nr, g.Nc, 1, false, st);
+                tm.end(2);
+            }
             if (fused) {
                 tm.begin(FPM_STAGE_IFFT_N_COLS_LO);  // crop x pupil inside the row pass
                 generic_inverse_pages(S, P, led, led0, b, g, lat, st, nullptr, 1, true);
@@ -648,6 +660,22 @@ struct Solve {
             tm.begin(FPM_STAGE_BACKPROJ);
             generic_pupil(rec, S, led, led0, b, g, P, mP.as<float>(), A, rms, nullptr, st);
             tm.end(1);
+            if (fuse_update) {
+                const bool last = last_epoch && (s == steps - 1);
+                tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
+                generic_strided_pass(G, g.Nr, g.Nc, 1, true, st);
+                tm.end(1);
+                tm.begin(FPM_STAGE_FUSED_UPDATE);
+                generic_update_fused(G, O, mO.as<float>(), g, 1, rms, !last, st);
+                tm.end(1);
+                if (!last) {
+                    tm.begin(FPM_STAGE_FFT_N_COLS_HI);
+                    generic_strided_pass(G, g.Nr, g.Nc, 1, false, st);
+                    tm.end(1);
+                    cur = 1 - cur;
+                }
+                continue;
+            }
             tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
             fft2_generic(G, G, g.Nr, g.Nc, 1, true, st);
             tm.end(2);
@@ -757,11 +785,12 @@ struct GroupSolve {
         FPM_HIP(hipMemsetAsync(mP.p, 0, T * nn * sizeof(float), st));
         launch_solve_state_init(ss, lr, st);
         rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
-        if (g.generic) {  // no masks and no carried spectrum on the generic path
+        if (g.generic) {  // no masks; the spectrum is carried only with the fused update
             if (!generic_fused_pages(g, false)) ebuf.alloc((size_t)T * B * nn * sizeof(float));
             generic_prepare(g);
             generic_tune(g, T, B, O.as<float2>(), S0.as<float2>(), P.as<float2>(), nullptr, tabd, lat.as<float2>(),
                          st);
+            if (generic_update_fusable(g)) fft2_generic(O.as<float2>(), spec[0], g.Nr, g.Nc, T, false, st);
             return;
         }
         for (uint32_t k = 0; k < T; ++k)
@@ -778,7 +807,7 @@ struct GroupSolve {
     // mode, fused stencil, owner stitch inside the column pass), for all tiles
     void enqueue_epoch(int cur, bool last_epoch, hipStream_t st) {
         if (g.generic) {
-            enqueue_epoch_generic(st);
+            enqueue_epoch_generic(cur, last_epoch, st);
             return;
         }
         const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
@@ -829,24 +858,35 @@ struct GroupSolve {
     // the generic steps (Solve::enqueue_epoch_generic) for all T tiles: T
     // pages for the object-side transforms, T b for the per-LED ones, one
     // grid row per tile for the crop, stencil, stitch and pupil kernels
-    void enqueue_epoch_generic(hipStream_t st) {
+    void enqueue_epoch_generic(int cur, bool last_epoch, hipStream_t st) {
         const size_t NN = (size_t)g.Nr * g.Nc;
+        const bool fuse_update = generic_update_fusable(g);
         float2* Ow = O.as<float2>();
         float2* Pw = P.as<float2>();
-        float2* S = S0.as<float2>();
-        float2* G = S1.as<float2>();
         float2* lp = lat.as<float2>();
         float2* rp = rec.as<float2>();
         const bool fused = generic_fused_pages(g, false);
         uint32_t led0, b;
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
-            fft2_generic(Ow, S, g.Nr, g.Nc, T, false, st);
+            float2* S = fuse_update ? spec[cur] : S0.as<float2>();
+            float2* G = fuse_update ? spec[1 - cur] : S1.as<float2>();
+            if (!fuse_update) fft2_generic(Ow, S, g.Nr, g.Nc, T, false, st);
             generic_inverse_pages(S, Pw, nullptr, led0, b, g, lp, st, tabd, T, fused);
             generic_stencil_pages(lp, y, led0, b, pr, g, ebuf.as<float>(), rp, nullptr, st, tabd, T, nullptr, fused);
             generic_forward_pages(rp, T * b, g, st, fused);
             generic_stitch(rp, Pw, nullptr, led0, b, g, G, st, tabd, T);
             generic_pupil(rp, S, nullptr, led0, b, g, Pw, mP.as<float>(), nullptr, rms, nullptr, st, tabd, T);
+            if (fuse_update) {
+                const bool last = last_epoch && (s == steps - 1);
+                generic_strided_pass(G, g.Nr, g.Nc, T, true, st);
+                generic_update_fused(G, Ow, mO.as<float>(), g, T, rms, !last, st);
+                if (!last) {
+                    generic_strided_pass(G, g.Nr, g.Nc, T, false, st);
+                    cur = 1 - cur;
+                }
+                continue;
+            }
             fft2_generic(G, G, g.Nr, g.Nc, T, true, st);
             generic_update(G, Ow, mO.as<float>(), T * NN, rms, st);
         }

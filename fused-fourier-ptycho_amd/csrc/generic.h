@@ -66,5 +66,15 @@ void generic_pupil(const float2* R, const float2* S, const int2* led, uint32_t l
                    const TileIn* tab = nullptr, uint32_t ntiles = 1);
 // RMSprop of O from its gradient G (n elements)
 void generic_update(const float2* G, float2* O, float* mO, size_t n, RmsArgs rms, hipStream_t st);
+// the strided-index pass of a 2D transform alone (in place)
+void generic_strided_pass(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, bool inverse, hipStream_t st);
+// N_rows = 3 x 2^k: the object update fused with the contiguous passes --
+// G (ntiles gradient spectra after their strided inverse pass) -> G_O along
+// the contiguous index, RMSprop of O, and with forward the contiguous pass
+// of DFT_N(O) written back into G (its strided pass then completes the next
+// step's spectrum)
+bool generic_update_fusable(const Geom& g);
+void generic_update_fused(float2* G, float2* O, float* mO, const Geom& g, uint32_t ntiles, RmsArgs rms, bool forward,
+                          hipStream_t st);
 
 }  // namespace fpm

@@ -51,6 +51,10 @@
 #ifndef FPM_GEN_3M
 #define FPM_GEN_3M 1
 #endif
+// object update fused with the contiguous passes for N_rows = 3 2^k (gen_update3m_kernel)
+#ifndef FPM_GEN_FUSED_UPDATE
+#define FPM_GEN_FUSED_UPDATE 1
+#endif
 // launch shapes of the transform passes measured at a solve's set-up
 #ifndef FPM_GEN_TUNE
 #define FPM_GEN_TUNE 1
@@ -1057,6 +1061,85 @@ __global__ __launch_bounds__(256) void gen_update_kernel(const float2* __restric
     }
 }
 
+// The object update on columns of N = 3 M points: the last (contiguous)
+// pass of G_O = IDFT_N(G_S) on the 3 x 2^k transform, RMSprop of O on its
+// outputs (gen_update_kernel's arithmetic, kernels.cuh:364-382) and, with
+// FWD, the first (contiguous) pass of DFT_N of the new O, written back into
+// G -- as the power-of-two path's fused update does: G_O never reaches HBM
+// and the next step's spectrum starts here.  Contiguous lines of one
+// workgroup are one contiguous run of nl N elements.
+template <int M, int P, bool FWD>
+__global__ __launch_bounds__(3 * P / 16) void gen_update3m_kernel(float2* __restrict__ G, float2* __restrict__ O,
+                                                                float* __restrict__ mO, uint32_t nlines, RmsArgs rms,
+                                                                const float2* __restrict__ twN, Twiddles tw) {
+    constexpr int N = 3 * M, NL = P / M, T = 3 * P / 16, LD = N + 1;
+    static_assert(NL >= 1 && NL * N == 16 * T, "16 points per thread");
+    __shared__ float2 lds[Lines3m<M, NL>::WORDS];
+    __shared__ float2 twl[2 * kTwLo];
+    const int t = threadIdx.x;
+    tw = bind_twiddles(tw, twl, t, T);
+    const uint32_t L0 = blockIdx.x * (uint32_t)NL;
+    const uint32_t nl = min((uint32_t)NL, nlines - L0);
+    const uint32_t tot = nl * (uint32_t)N;
+    const bool full = nl == (uint32_t)NL;
+    float2* __restrict__ g = G + (size_t)L0 * N;
+    float2* __restrict__ o = O + (size_t)L0 * N;
+    float* __restrict__ m = mO + (size_t)L0 * N;
+    if (full) {
+        float2 buf[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) buf[k] = g[t + k * T];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t idx = (uint32_t)(t + k * T), l = idx / (uint32_t)N;
+            lds[l * LD + (idx - l * (uint32_t)N)] = buf[k];
+        }
+    } else {
+        for (uint32_t idx = t; idx < tot; idx += T) {
+            const uint32_t l = idx / (uint32_t)N;
+            lds[l * LD + (idx - l * (uint32_t)N)] = g[idx];
+        }
+    }
+    __syncthreads();
+    fft3m_lines<M, NL, true>(lds, nl, twN, tw, t);
+    const float lr = rms_lr(rms);
+    auto update = [&](uint32_t idx) {
+        const uint32_t l = idx / (uint32_t)N, a = l * LD + (idx - l * (uint32_t)N);
+        const float2 gr = lds[a];
+        float mu = gr.x * gr.x + gr.y * gr.y;
+        mu = rms.beta * m[idx] + (1.0f - rms.beta) * mu;
+        const float sq = lr / (sqrtf(mu) + rms.eps);
+        float2 w = o[idx];
+        w.x -= gr.x * sq;
+        w.y -= gr.y * sq;
+        o[idx] = w;
+        m[idx] = mu;
+        lds[a] = w;  // only this thread touches lds[a] in this phase
+    };
+    if (full) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) update((uint32_t)(t + k * T));
+    } else {
+        for (uint32_t idx = t; idx < tot; idx += T) update(idx);
+    }
+    if constexpr (FWD) {
+        __syncthreads();
+        fft3m_lines<M, NL, false>(lds, nl, twN, tw, t);
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t idx = (uint32_t)(t + k * T), l = idx / (uint32_t)N;
+                g[idx] = lds[l * LD + (idx - l * (uint32_t)N)];
+            }
+        } else {
+            for (uint32_t idx = t; idx < tot; idx += T) {
+                const uint32_t l = idx / (uint32_t)N;
+                g[idx] = lds[l * LD + (idx - l * (uint32_t)N)];
+            }
+        }
+    }
+}
+
 unsigned gblocks(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1u << 16)); }
 
 }  // namespace
@@ -1186,6 +1269,57 @@ void generic_tune(const Geom& g, uint32_t T, uint32_t B, const float2* O, const 
     }
     FPM_HIP(hipStreamSynchronize(st));
     FPM_HIP(hipFree(scratch));
+}
+
+void generic_strided_pass(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, bool inverse, hipStream_t st) {
+    GenFFT c{};
+    c.in = data;
+    c.out = data;
+    c.len = cols;
+    c.nlines = rows * batch;
+    c.lines_per_batch = rows;
+    c.line_stride = 1;
+    c.es = rows;
+    c.batch_stride = (uint64_t)rows * cols;
+    c.inverse = inverse ? 1 : 0;
+    launch_generic_lines(c, st);
+}
+
+bool generic_update_fusable(const Geom& g) { return FPM_GEN_FUSED_UPDATE && g.generic && len3m(g.Nr, false) != 0; }
+
+void generic_update_fused(float2* G, float2* O, float* mO, const Geom& g, uint32_t ntiles, RmsArgs rms, bool forward,
+                          hipStream_t st) {
+    const uint32_t m = len3m(g.Nr, false);
+    if (!m) throw HipError{hipErrorInvalidValue, "fused generic update: N_rows is not 3 x 2^k"};
+    const float2* twN = generic_twiddles(g.Nr);
+    int dev = 0;
+    FPM_HIP(hipGetDevice(&dev));
+    const Twiddles tw{twiddle_table(dev)};
+    const uint32_t nlines = g.Nc * ntiles;
+    auto go = [&](auto mm, auto pp) {
+        constexpr int MM = decltype(mm)::value, PP = decltype(pp)::value, NL = PP / MM;
+        const dim3 grid((nlines + NL - 1) / NL);
+        if (forward)
+            hipLaunchKernelGGL((gen_update3m_kernel<MM, PP, true>), grid, dim3(3 * PP / 16), 0, st, G, O, mO, nlines,
+                               rms, twN, tw);
+        else
+            hipLaunchKernelGGL((gen_update3m_kernel<MM, PP, false>), grid, dim3(3 * PP / 16), 0, st, G, O, mO, nlines,
+                               rms, twN, tw);
+    };
+    using std::integral_constant;
+    switch (m) {
+        case 16: go(integral_constant<int, 16>{}, integral_constant<int, 1024>{}); break;
+        case 32: go(integral_constant<int, 32>{}, integral_constant<int, 1024>{}); break;
+        case 64: go(integral_constant<int, 64>{}, integral_constant<int, 1024>{}); break;
+        case 128: go(integral_constant<int, 128>{}, integral_constant<int, 1024>{}); break;
+        case 256: go(integral_constant<int, 256>{}, integral_constant<int, 1024>{}); break;
+        case 512: go(integral_constant<int, 512>{}, integral_constant<int, 1024>{}); break;
+        case 1024: go(integral_constant<int, 1024>{}, integral_constant<int, 1024>{}); break;
+        case 2048: go(integral_constant<int, 2048>{}, integral_constant<int, 2048>{}); break;
+        case 4096: go(integral_constant<int, 4096>{}, integral_constant<int, 4096>{}); break;
+        default: throw HipError{hipErrorInvalidValue, "fused generic update: unsupported length"};
+    }
+    FPM_LAUNCH_CHECK();
 }
 
 void fft2_generic(const float2* in, float2* out, uint32_t rows, uint32_t cols, uint32_t batch, bool inverse,

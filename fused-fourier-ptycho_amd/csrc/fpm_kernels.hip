@@ -154,6 +154,8 @@ __global__ __launch_bounds__(256) void backproj_kernel(Geom g, const float2* __r
     // those rows).  The gradient-only call (mP == nullptr) computes every pixel.
     const bool skip = mP && p_zero && A[pix] == 0.0f;
     float2 gp = make_float2(0.f, 0.f);
+    // unrolled: several LEDs' loads in flight at once (the sums keep LED order)
+#pragma unroll 4
     for (uint32_t ell = 0; ell < (skip ? 0u : b); ++ell) {
         const size_t page = (size_t)ell * plane;
         const float2 Rv = R[page + rpos];

@@ -17,6 +17,8 @@ Also reported:
              event-timed duration, against MI355X's 8 TB/s HBM peak and a
              plain device copy timed here, and its measured HBM traffic
              (rocprofv3 counters, copy-calibrated, from profiles/);
+  batch_autotune the opt-in minibatch autotuner (fpm_amd.tune): seconds to
+             the batch-26 solve's final loss for each candidate size;
   fft_reference the library's 2D DFT beside rocFFT (torch.fft) on the
              16384^2 spectrum and 26 pages of 2048^2, same buffers and stream;
   cpu_baseline the FP32 CPU restatement (oracle, OpenMP) on the host cores:
@@ -440,6 +442,9 @@ def main():
     ap.add_argument("--batch-sweep", default="16,26,32",
                     help="comma-separated batch sizes also timed (same epochs, s/epoch each); empty: none")
     ap.add_argument("--no-stages", action="store_true", help="time without per-stage events")
+    ap.add_argument("--batch-autotune", default="16,26,32",
+                    help="candidate batch sizes of the time-to-loss batch autotuner, run with the batch sweep "
+                         "('' to skip)")
     ap.add_argument("--engine", choices=["f32", "split"], default="f32",
                     help="FFT radix-16 stages: FP32 butterflies or split-FP16 MFMA products")
     ap.add_argument("--graph", action="store_true",
@@ -609,6 +614,20 @@ def main():
                               "epoch_dense_model_GBps": (spe * 104.0 * N * N + L * (136.0 + y_bytes) * n * n) / 1e9 / dt}
             print(f"[bench] batch {bs}: {dt * 1e3:.1f} ms/epoch", file=sys.stderr, flush=True)
         out["batch_sweep"] = sweep
+    if args.batch_sweep and args.batch_autotune:
+        # the opt-in minibatch autotuner (a batch-size study, so it runs with the batch
+        # sweep): time to the batch-26 solve's final loss per candidate size
+        from fpm_amd import tune
+
+        cands = tuple(int(v) for v in args.batch_autotune.split(","))
+        best, rep = tune.autotune_batch(ds["O0_dev"].data_ptr(), ds["P0_dev"].data_ptr(), Yd.data_ptr(), y_u16,
+                                        ds["led_dev"].data_ptr(), ds["A_dev"].data_ptr(), (N, N), (n, n), L,
+                                        args.pratio, opt, Od.data_ptr(), Pd.data_ptr(), candidates=cands,
+                                        ref_batch=args.batch, epochs=args.steps, stream=stream.cuda_stream)
+        out["batch_autotune"] = {"chosen": best, "target_loss": rep[args.batch]["losses"][-1],
+                                 "candidates": {str(b): {"s_to_target": round(r["s"], 4), "epochs": r["epochs"],
+                                                         "reached": r["reached"]} for b, r in sorted(rep.items())}}
+        print(f"[bench] batch autotune: chosen {best}", file=sys.stderr, flush=True)
     if not args.no_fft_reference and (n, N) == (2048, 16384):
         out["fft_reference"] = run_fft_reference(stream)
         print(f"[bench] fft vs rocFFT: {out['fft_reference']}", file=sys.stderr, flush=True)

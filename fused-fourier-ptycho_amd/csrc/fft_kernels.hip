@@ -676,6 +676,9 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
 #ifndef FPM_FUSED16K_HALF
 #define FPM_FUSED16K_HALF 1
 #endif
+#ifndef FPM_TWIDDLE_TABLES
+#define FPM_TWIDDLE_TABLES 1
+#endif
 namespace half16k {
 constexpr int LEN = 16384, TV = 1024, HALF = LEN / 2;
 using HMap = LineSlowMap<1, HALF>;  // the half buffer (one pad word per 16)
@@ -711,20 +714,47 @@ __device__ __forceinline__ void read_points(float2 (&d)[16], const float2* lds, 
             if (s >= s0 && s < s1) d[q * G::R + s] = lds[hofs(j + s * G::NB)];
     }
 }
+// Pass P's twiddles W^(s k step) (k < L, s < R) from a table in LDS when its
+// span L is at most 16: L (R - 1) values, made once per workgroup by
+// twiddle_powers -- the same values bit for bit -- at tab[(s - 1) L + k].
+template <bool INV, int P> constexpr bool tabled() {
+    return FPM_TWIDDLE_TABLES && PassG<INV, P>::L > 1 && PassG<INV, P>::L <= 16;
+}
+template <bool INV, int P> constexpr int table_words() {
+    return tabled<INV, P>() ? PassG<INV, P>::L * (PassG<INV, P>::R - 1) : 0;
+}
+template <bool INV, int P>
+__device__ __forceinline__ void fill_table(float2* tab, int t, const Twiddles& tw) {
+    using G = PassG<INV, P>;
+    if constexpr (tabled<INV, P>()) {
+        if (t < G::L) {
+            constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+            float2 w[16];
+            twiddle_powers<G::R, INV>(tw, (uint32_t)t * step, w);
+#pragma unroll
+            for (int s = 1; s < G::R; ++s) tab[(s - 1) * G::L + t] = w[s];
+        }
+    }
+}
 // twiddles and DFT of pass P on inputs already in v (pass_from_lds without the reads)
 template <bool INV, int P>
-__device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles& tw) {
+__device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles& tw, const float2* tab = nullptr) {
     using G = PassG<INV, P>;
     tv = fresh(tv);
 #pragma unroll
     for (int q = 0; q < G::NG; ++q) {
         if constexpr (G::L > 1) {
             const uint32_t k = (uint32_t)((tv + TV * q) % G::L);
-            constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
-            float2 w[16];
-            twiddle_powers<G::R, INV>(tw, k * step, w);
+            if constexpr (tabled<INV, P>()) {
 #pragma unroll
-            for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], w[s]);
+                for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], tab[(s - 1) * G::L + k]);
+            } else {
+                constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+                float2 w[16];
+                twiddle_powers<G::R, INV>(tw, k * step, w);
+#pragma unroll
+                for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], w[s]);
+            }
         }
         dft<G::R, INV>(&v[q * G::R]);
     }
@@ -797,6 +827,13 @@ __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restr
     for (int k = t; k < LEN / kMaskRows; k += 512)
         rm[k] = (uint8_t)(((rmk ? rmk[k] : 1) != 0 ? 1 : 0) | ((nmk ? nmk[k] : 1) != 0 ? 2 : 0));
     tw = bind_twiddles(tw, twl, t, 512);
+    // twiddle tables of the two small-span passes (inverse pass 1: 4 x 15,
+    // forward pass 1: 16 x 15 values); the exchanges before their first use
+    // synchronise the workgroup
+    __shared__ float2 ttab_i[table_words<true, 1>() > 0 ? table_words<true, 1>() : 1];
+    __shared__ float2 ttab_f[FWD && table_words<false, 1>() > 0 ? table_words<false, 1>() : 1];
+    fill_table<true, 1>(ttab_i, t, tw);
+    if constexpr (FWD) fill_table<false, 1>(ttab_f, t, tw);
     const float lr = rms_lr(rms);
     float2 va[16], vb[16];
     auto load_g = [&](float2 (&v)[16], int tv) {
@@ -814,8 +851,8 @@ __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restr
     first_pass_dft<LEN, true, TV>(va);
     first_pass_dft<LEN, true, TV>(vb);
     exchange<true, 0>(va, vb, lds, t);
-    compute<true, 1>(va, t, tw);
-    compute<true, 1>(vb, t + 512, tw);
+    compute<true, 1>(va, t, tw, ttab_i);
+    compute<true, 1>(vb, t + 512, tw, ttab_i);
     __syncthreads();
     exchange<true, 1>(va, vb, lds, t);
     compute<true, 2>(va, t, tw);
@@ -854,8 +891,8 @@ __global__ __launch_bounds__(512, 4) void fused_update16k_kernel(float2* __restr
         first_pass_dft<LEN, false, TV>(vb);
         __syncthreads();
         exchange<false, 0>(va, vb, lds, t);
-        compute<false, 1>(va, t, tw);
-        compute<false, 1>(vb, t + 512, tw);
+        compute<false, 1>(va, t, tw, ttab_f);
+        compute<false, 1>(vb, t + 512, tw, ttab_f);
         __syncthreads();
         exchange<false, 1>(va, vb, lds, t);
         compute<false, 2>(va, t, tw);
@@ -1179,8 +1216,30 @@ __device__ __forceinline__ void read_groups(float2* d, const float2* lds, int tv
             if (s >= s0 && s < s1) d[q * G::R + s] = lds[hofs(line, j + s * G::NB)];
     }
 }
+// Pass P's twiddles W^(s k step) (k < L, s < R) from a table in LDS when its
+// span L is at most 16: L (R - 1) values, made once per workgroup by
+// twiddle_powers -- the same values bit for bit -- at tab[(s - 1) L + k].
+template <bool INV, int P> constexpr bool tabled() {
+    return FPM_TWIDDLE_TABLES && PassG<INV, P>::L > 1 && PassG<INV, P>::L <= 16;
+}
+template <bool INV, int P> constexpr int table_words() {
+    return tabled<INV, P>() ? PassG<INV, P>::L * (PassG<INV, P>::R - 1) : 0;
+}
 template <bool INV, int P>
-__device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles& tw) {
+__device__ __forceinline__ void fill_table(float2* tab, int t, const Twiddles& tw) {
+    using G = PassG<INV, P>;
+    if constexpr (tabled<INV, P>()) {
+        if (t < G::L) {
+            constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+            float2 w[16];
+            twiddle_powers<G::R, INV>(tw, (uint32_t)t * step, w);
+#pragma unroll
+            for (int s = 1; s < G::R; ++s) tab[(s - 1) * G::L + t] = w[s];
+        }
+    }
+}
+template <bool INV, int P>
+__device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles& tw, const float2* tab = nullptr) {
     using G = PassG<INV, P>;
     tv = fresh(tv);
 #pragma unroll
@@ -1188,11 +1247,16 @@ __device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles&
         if constexpr (G::L > 1) {
             const int j = (tv + TV * q) % G::NB;
             const uint32_t k = (uint32_t)(j % G::L);
-            constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
-            float2 w[16];
-            twiddle_powers<G::R, INV>(tw, k * step, w);
+            if constexpr (tabled<INV, P>()) {
 #pragma unroll
-            for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], w[s]);
+                for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], tab[(s - 1) * G::L + k]);
+            } else {
+                constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+                float2 w[16];
+                twiddle_powers<G::R, INV>(tw, k * step, w);
+#pragma unroll
+                for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], w[s]);
+            }
         }
         dft<G::R, INV>(&v[q * G::R]);
     }
@@ -1281,6 +1345,13 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
     }
     for (int k = t; k < LEN / kMaskRows; k += 512) rm[k] = rowmask ? rowmask[k] : 1;
     tw = bind_twiddles(tw, twl, t, 512);
+    // twiddle tables of the two small-span passes (inverse pass 1: 8 x 15,
+    // forward pass 1: 16 x 15 values); the exchanges before their first use
+    // synchronise the workgroup
+    __shared__ float2 ttab_i[table_words<true, 1>() > 0 ? table_words<true, 1>() : 1];
+    __shared__ float2 ttab_f[table_words<false, 1>() > 0 ? table_words<false, 1>() : 1];
+    fill_table<true, 1>(ttab_i, t, tw);
+    fill_table<false, 1>(ttab_f, t, tw);
     const uint32_t j0 = blockIdx.x * K;  // first interior column; line l is column j0 - 1 + l
     const uint32_t cmask = g.nc - 1;
     const size_t plane = (size_t)g.nr * g.nc;
@@ -1308,8 +1379,8 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
     first_pass_dft<LEN, true, TV>(va);
     first_pass_dft<LEN, true, TV>(vb);
     exchange<true, 0>(va, vb, lds, t);
-    compute<true, 1>(va, t, tw);
-    compute<true, 1>(vb, t + 512, tw);
+    compute<true, 1>(va, t, tw, ttab_i);
+    compute<true, 1>(vb, t + 512, tw, ttab_i);
     __syncthreads();
     exchange<true, 1>(va, vb, lds, t);
     compute<true, 2>(va, t, tw);
@@ -1380,8 +1451,8 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
     first_pass_dft<LEN, false, TV>(vb);
     __syncthreads();
     exchange<false, 0>(va, vb, lds, t);
-    compute<false, 1>(va, t, tw);
-    compute<false, 1>(vb, t + 512, tw);
+    compute<false, 1>(va, t, tw, ttab_f);
+    compute<false, 1>(vb, t + 512, tw, ttab_f);
     __syncthreads();
     exchange<false, 1>(va, vb, lds, t);
     compute<false, 2>(va, t, tw);

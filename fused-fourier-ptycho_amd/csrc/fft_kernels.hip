@@ -679,6 +679,9 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
 #ifndef FPM_TWIDDLE_TABLES
 #define FPM_TWIDDLE_TABLES 1
 #endif
+#ifndef FPM_LINEAR_LDS
+#define FPM_LINEAR_LDS 1
+#endif
 namespace half16k {
 constexpr int LEN = 16384, TV = 1024, HALF = LEN / 2;
 using HMap = LineSlowMap<1, HALF>;  // the half buffer (one pad word per 16)
@@ -686,6 +689,13 @@ template <bool INV, int P> struct PassG {
     static constexpr int R = Plan<LEN, INV>::radix(P), L = Plan<LEN, INV>::span(P), NB = LEN / R, NG = 16 / R;
 };
 __device__ __forceinline__ int hofs(int idx) { return HMap::lds(0, idx & (HALF - 1)); }
+// Padded distance of d points: hofs(i + d) == hofs(i) + pad_dist(d) whenever
+// i + d stays in i's half and (i mod 16) + (d mod 16) < 16.  A pass's group
+// sits in one aligned block of L R <= HALF points, with base mod L < L and the
+// steps multiples of L (L a power of two), and a read point j + s NB has
+// j < NB, so both exchanges meet it; the addresses of one group are then one
+// register plus constants that fold into the LDS instructions' offsets.
+__device__ __forceinline__ constexpr int pad_dist(int d) { return d + (d >> 4); }
 
 // writes of pass P's outputs of virtual thread tv, groups [q0, q1)
 template <bool INV, int P>
@@ -697,8 +707,14 @@ __device__ __forceinline__ void write_groups(const float2 (&v)[16], float2* lds,
         if (q < q0 || q >= q1) continue;
         const int j = tv + TV * q;
         const int base = (j / G::L) * G::L * G::R + (j % G::L);
+#if FPM_LINEAR_LDS
+        float2* dst = lds + hofs(base);
+#pragma unroll
+        for (int s = 0; s < G::R; ++s) dst[pad_dist(s * G::L)] = v[q * G::R + s];
+#else
 #pragma unroll
         for (int s = 0; s < G::R; ++s) lds[hofs(base + s * G::L)] = v[q * G::R + s];
+#endif
     }
 }
 // reads of pass P's inputs s in [s0, s1) of virtual thread tv into d[q * R + s]
@@ -709,9 +725,16 @@ __device__ __forceinline__ void read_points(float2 (&d)[16], const float2* lds, 
 #pragma unroll
     for (int q = 0; q < G::NG; ++q) {
         const int j = tv + TV * q;
+#if FPM_LINEAR_LDS
+        const float2* src = lds + hofs(j);
+#pragma unroll
+        for (int s = 0; s < G::R; ++s)
+            if (s >= s0 && s < s1) d[q * G::R + s] = src[pad_dist((s * G::NB) & (HALF - 1))];
+#else
 #pragma unroll
         for (int s = 0; s < G::R; ++s)
             if (s >= s0 && s < s1) d[q * G::R + s] = lds[hofs(j + s * G::NB)];
+#endif
     }
 }
 // Pass P's twiddles W^(s k step) (k < L, s < R) from a table in LDS when its
@@ -1198,8 +1221,14 @@ __device__ __forceinline__ void write_groups(const float2 (&v)[16], float2* lds,
         if (q < q0 || q >= q1) continue;
         const int b = tv + TV * q, line = b / G::NB, j = b % G::NB;
         const int base = (j / G::L) * G::L * G::R + (j % G::L);
+#if FPM_LINEAR_LDS
+        float2* dst = lds + hofs(line, base);
+#pragma unroll
+        for (int s = 0; s < G::R; ++s) dst[half16k::pad_dist(s * G::L)] = v[q * G::R + s];
+#else
 #pragma unroll
         for (int s = 0; s < G::R; ++s) lds[hofs(line, base + s * G::L)] = v[q * G::R + s];
+#endif
     }
 }
 // inputs of pass P of groups [q0, q1), points [s0, s1), into d[q * R + s]
@@ -1211,9 +1240,16 @@ __device__ __forceinline__ void read_groups(float2* d, const float2* lds, int tv
     for (int q = 0; q < G::NG; ++q) {
         if (q < q0 || q >= q1) continue;
         const int b = tv + TV * q, line = b / G::NB, j = b % G::NB;
+#if FPM_LINEAR_LDS
+        const float2* src = lds + hofs(line, j);
+#pragma unroll
+        for (int s = 0; s < G::R; ++s)
+            if (s >= s0 && s < s1) d[q * G::R + s] = src[half16k::pad_dist(s * G::NB)];
+#else
 #pragma unroll
         for (int s = 0; s < G::R; ++s)
             if (s >= s0 && s < s1) d[q * G::R + s] = lds[hofs(line, j + s * G::NB)];
+#endif
     }
 }
 // Pass P's twiddles W^(s k step) (k < L, s < R) from a table in LDS when its

@@ -29,9 +29,11 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-// a * conj(b)
+// a * conj(b) with its roundings written out (each part: the first product
+// fused onto the rounded second), so that every kernel that forms the stitch
+// or pupil-gradient terms gets the same bits whatever the compiler contracts
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
-    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 // a * (-i) for the forward direction, a * (+i) for the inverse one

@@ -37,6 +37,9 @@ struct StitchArgs {
     PageLayout nl;  // layout of the R pages
     float invN2;
     const TileIn* tab = nullptr;
+    // optional: pupil row i holds P != 0 only in columns [c0, c1), pcols[i] =
+    // c0 | c1 << 16 (launch_pupil_row_mask); grouped tiles: nr words per tile
+    const uint32_t* pcols = nullptr;
 };
 
 // one LEN-point transform per contiguous line (LEN = len, nlines lines)

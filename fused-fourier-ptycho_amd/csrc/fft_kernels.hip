@@ -212,17 +212,21 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     const float2* __restrict__ sR = sa.R;
     const float2* __restrict__ sP = sa.P;
     const int2* __restrict__ sled = sa.led;
+    const uint32_t* __restrict__ spc = sa.pcols;
     if constexpr (SRC == SRC_STITCH) {
         if (sa.tab) {
             sR += (size_t)page * sa.b * sa.nr * sa.nc;
             sP += (size_t)page * sa.nr * sa.nc;
             sled = sa.tab[page].led;
+            if (spc) spc += (size_t)page * sa.nr;
         }
     }
-    // stitch: window corners of the batch and this thread's row mask
+    // stitch: window corners of the batch, the pupil-support column interval
+    // of each (LED, tile row) and this thread's row mask
     uint64_t rowhits = 0;
     uint32_t srow = 0;
     __shared__ uint32_t win[SRC == SRC_STITCH ? kMaxStitchLeds : 1];
+    __shared__ uint32_t wcc[SRC == SRC_STITCH ? kMaxStitchLeds * TR : 1];
     if constexpr (SRC == SRC_STITCH) {
         const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
         for (uint32_t l = t; l < sa.b; l += Cfg::T) {
@@ -232,9 +236,16 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             win[l] = wr | (wc << 16);
         }
         __syncthreads();
+        for (uint32_t e = t; e < sa.b * TR; e += Cfg::T) {
+            const uint32_t l = e / TR, li = (tile * TR + e % TR - (win[l] & 0xffffu)) & Rm;
+            wcc[e] = li < sa.nr ? (spc ? spc[li] : sa.nc << 16) : 0u;  // 0: row outside the window
+        }
+        __syncthreads();
         srow = tile * TR + (uint32_t)(t % TR);  // LineFast: one row per thread
-        for (uint32_t l = 0; l < sa.b; ++l)
-            if (((srow - (win[l] & 0xffffu)) & Rm) < sa.nr) rowhits |= 1ull << l;
+        for (uint32_t l = 0; l < sa.b; ++l) {
+            const uint32_t cc = wcc[l * TR + t % TR];
+            if ((cc >> 16) > (cc & 0xffffu)) rowhits |= 1ull << l;  // pupil rows without support add nothing
+        }
     }
     if constexpr (SRC == SRC_STITCH) {
         // A thread's slots of butterfly q sit at columns col0 + s*D (s < R0),
@@ -261,14 +272,23 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
                 const uint32_t d = (col0 - (w >> 16)) & Cm;
                 const float2* __restrict__ Rl = sR + (size_t)l * plane + sa.nl.row_off(di);
+                // P = 0 outside the row's support interval [c0, c0 + cw) (within the window)
+                const uint32_t cc = wcc[l * TR + t % TR];
+                const uint32_t c0 = cc & 0xffffu, cw = (cc >> 16) - c0;
 #pragma unroll
                 for (int s = 0; s < SL::GS0; ++s) {
                     const uint32_t lj = (d + (uint32_t)s * D) & Cm;
-                    if (lj < sa.nc) {
+                    if (lj - c0 < cw) {
+                        const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
+                        const float2* rp = Rl + (size_t)sa.nl.perm(dj) * sa.nl.ld;
+                        // with the support intervals R is read beside P (inside
+                        // the pupil disk P is nonzero), not after it
+                        float2 rv = make_float2(0.f, 0.f);
+                        if (spc) rv = *rp;
                         const float2 pp = sP[lj * sa.nr + li];
-                        if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0; R not read
-                            const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
-                            const float2 gw = cmulc(Rl[(size_t)sa.nl.perm(dj) * sa.nl.ld], pp);
+                        if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0
+                            if (!spc) rv = *rp;
+                            const float2 gw = cmulc(rv, pp);
                             v[q * SL::GS0 + s].x += gw.x * sa.invN2;
                             v[q * SL::GS0 + s].y += gw.y * sa.invN2;
                         }

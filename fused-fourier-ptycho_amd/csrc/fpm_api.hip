@@ -387,10 +387,13 @@ struct Workspace {
 // (optional, nr/16 bytes): the pupil-support row mask of the per-LED spectra
 // (launch_pupil_row_mask); with it the per-LED column passes skip pupil rows
 // without support (update mode only: the pupil kernel then skips them too).
+// pcols (optional): the pupil-support column interval of each pupil row
+// (launch_pupil_row_mask); the fused owner stitch then reads P only inside it.
 void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2* led, uint32_t led0, uint32_t b,
                         const YStack& y, int pratio, Workspace& ws, float2* Gs, int stitch, float* mP, const float* A,
                         RmsArgs rms, float2* GP, double* loss_acc, const float2* tw, hipStream_t st,
-                        StageTimer& tm, const uint8_t* rowmask, const uint8_t* pmask, uint8_t* signs = nullptr) {
+                        StageTimer& tm, const uint8_t* rowmask, const uint8_t* pmask, uint8_t* signs = nullptr,
+                        const uint32_t* pcols = nullptr) {
     float2* lat = ws.lat;
     float2* rec = ws.rec;
     const bool fused = stencil_fused_supported(g);
@@ -434,7 +437,8 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     if (stitch == FPM_STITCH_OWNER && b <= 64) {
         // owner-computes stitch inside the first column pass of IDFT_N; the
         // pupil update must come after it (the stitch reads this step's pupil)
-        const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.nl, 1.0f / (float)(g.Nr * g.Nc)};
+        const StitchArgs sa{rec, P, led, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.nl, 1.0f / (float)(g.Nr * g.Nc), nullptr,
+                            pcols};
         tm.begin(FPM_STAGE_IFFT_N_COLS_HI);
         fft_cols_inverse(Gs, g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, nullptr, &sa, rowmask, nullptr, g.eng);
         tm.end(col_launches(g.pN));
@@ -483,6 +487,7 @@ struct Solve {
     uint8_t* mk[2] = {nullptr, nullptr};
     uint8_t* pmask = nullptr;  // pupil-support row mask (n side)
     uint8_t* psup = nullptr;   // pupil-support rows
+    uint32_t* pcols = nullptr; // pupil-support column interval per pupil row
 
     void batch_of(uint32_t s, uint32_t& led0, uint32_t& b) const {
         led0 = s * B;
@@ -520,16 +525,18 @@ struct Solve {
         // outside them the spectrum gradient is zero and the spectrum is
         // never read, so the N-side column passes skip those row tiles.
         const size_t mrows = g.Nr / kMaskRows;
-        masks.alloc(2 * mrows + g.nr / kMaskRows + g.nr);
+        const size_t cols_at = (2 * mrows + g.nr / kMaskRows + g.nr + 3) & ~(size_t)3;
+        masks.alloc(cols_at + g.nr * sizeof(uint32_t));
         pmask = masks.as<uint8_t>() + 2 * mrows;
         psup = pmask + g.nr / kMaskRows;
+        pcols = (uint32_t*)(masks.as<uint8_t>() + cols_at);
         mk[0] = masks.as<uint8_t>();
         mk[1] = masks.as<uint8_t>() + mrows;
         spec[0] = S0.as<float2>();
         spec[1] = S1.as<float2>();
         // rows where P != 0 or A != 0: the update keeps P = 0 wherever
         // A = 0, so the masks of the initial pupil hold for every step
-        launch_pupil_row_mask(P, A, g.nr, g.nc, pmask, psup, st);
+        launch_pupil_row_mask(P, A, g.nr, g.nc, pmask, psup, st, pcols);
         uint32_t led0, b;
         batch_of(0, led0, b);
         launch_row_mask(led, led0, b, g.Nr, g.nr, psup, mk[0], st);
@@ -554,7 +561,7 @@ struct Solve {
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
             minibatch_backward(g, spec[cur], P, led, led0, b, y, pr, ws, spec[1 - cur], stitch, mP.as<float>(), A,
-                               rms, nullptr, ss.loss_cur, tw, st, tm, mk[cur], pmask);
+                               rms, nullptr, ss.loss_cur, tw, st, tm, mk[cur], pmask, nullptr, pcols);
             const bool last = last_epoch && (s == steps - 1);
             if (!last) {  // the next minibatch: s + 1, or the first one of the next epoch
                 uint32_t nled0, nb;
@@ -710,6 +717,7 @@ struct GroupSolve {
     uint8_t* mk[2] = {nullptr, nullptr};
     uint8_t* pmask = nullptr;  // T pupil-support row masks (n side)
     uint8_t* psup = nullptr;   // T pupil-support row sets
+    uint32_t* pcols = nullptr; // T pupil-support column intervals (nr words each)
     const TileIn* tabd = nullptr;
 
     // The owner stitch as its own kernel (coalesced gathers along the rows,
@@ -751,11 +759,13 @@ struct GroupSolve {
         lat.alloc((size_t)T * B * nn * sizeof(float2));
         rec.alloc((size_t)T * B * nn * sizeof(float2));
         const size_t mrows = g.Nr / kMaskRows, prows = g.nr / kMaskRows;
-        masks.alloc(T * (2 * mrows + prows + g.nr));
+        const size_t cols_at = (T * (2 * mrows + prows + g.nr) + 3) & ~(size_t)3;
+        masks.alloc(cols_at + (size_t)T * g.nr * sizeof(uint32_t));
         mk[0] = masks.as<uint8_t>();
         mk[1] = mk[0] + T * mrows;
         pmask = mk[1] + T * mrows;
         psup = pmask + T * prows;
+        pcols = (uint32_t*)(masks.as<uint8_t>() + cols_at);
         spec[0] = S0.as<float2>();
         spec[1] = S1.as<float2>();
         state.alloc(64 + (want_loss ? (size_t)T * (1 + ne) * sizeof(double) : 0));
@@ -795,7 +805,7 @@ struct GroupSolve {
         }
         for (uint32_t k = 0; k < T; ++k)
             launch_pupil_row_mask(P.as<float2>() + k * nn, tiles[members[k]].A, g.nr, g.nc, pmask + k * prows,
-                                  psup + (size_t)k * g.nr, st);
+                                  psup + (size_t)k * g.nr, st, pcols + (size_t)k * g.nr);
         uint32_t led0, b;
         batch_of(0, led0, b);
         launch_row_mask(nullptr, led0, b, g.Nr, g.nr, psup, mk[0], st, tabd, T);
@@ -832,7 +842,7 @@ struct GroupSolve {
                 fft_cols_inverse(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, nullptr, nullptr, mk[cur], nullptr,
                                  g.eng, 1);
             } else {
-                const StitchArgs sa{rp, Pw, nullptr, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.nl, invN2, tabd};
+                const StitchArgs sa{rp, Pw, nullptr, led0, b, g.nr, g.nc, g.Nr, g.Nc, g.nl, invN2, tabd, pcols};
                 fft_cols_inverse(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, nullptr, &sa, mk[cur], nullptr,
                                  g.eng, 1);
             }

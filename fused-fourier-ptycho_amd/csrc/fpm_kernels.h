@@ -177,9 +177,12 @@ void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, ui
 // pupil support: sup[i] (nr bytes) = 1 iff pupil row i holds P != 0 or A != 0;
 // mask (nr/16 bytes): the same per 16-row tile of the ifftshifted per-LED
 // spectra -- the only rows where S P can be nonzero and where R is read.
+// cols (optional, nr words): cols[i] = c0 | c1 << 16 with [c0, c1) the
+// smallest column interval of pupil row i holding every P != 0 and A != 0
+// (0 when the row has none), so P = 0 outside it at every step.
 // Valid for every step of a solve with the clamped update.
 void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, uint8_t* sup,
-                           hipStream_t st);
+                           hipStream_t st, uint32_t* cols = nullptr);
 
 // fill with zeros (count complex elements)
 void launch_zero(float2* p, size_t count, hipStream_t st);

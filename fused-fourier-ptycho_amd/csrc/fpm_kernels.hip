@@ -369,22 +369,41 @@ void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, ui
 // pupil hold for every later step.
 __global__ __launch_bounds__(1024) void pupil_row_mask_kernel(const float2* __restrict__ P,
                                                               const float* __restrict__ A, uint32_t nr, uint32_t nc,
-                                                              uint8_t* __restrict__ mask, uint8_t* __restrict__ sup) {
+                                                              uint8_t* __restrict__ mask, uint8_t* __restrict__ sup,
+                                                              uint32_t* __restrict__ cols) {
     __shared__ int any[kMaskRows];
+    __shared__ uint32_t lo[kMaskRows], hi[kMaskRows];
     const uint32_t t = threadIdx.x, tile = blockIdx.x;
-    if (t < kMaskRows) any[t] = 0;
+    if (t < kMaskRows) {
+        any[t] = 0;
+        lo[t] = 0xffffffffu;
+        hi[t] = 0;
+    }
     __syncthreads();
     // 1024 is a multiple of 16: every thread stays on spectrum row t mod 16
     const uint32_t di = tile * kMaskRows + t % kMaskRows;
     const uint32_t i = (di + (nr >> 1)) & (nr - 1);
     int h = 0;
+    uint32_t jlo = 0xffffffffu, jhi = 0;
     for (uint32_t j = t / kMaskRows; j < nc; j += 1024 / kMaskRows) {
         const float2 p = P[(size_t)j * nr + i];
-        h |= (p.x != 0.0f) | (p.y != 0.0f) | (A[(size_t)j * nr + i] != 0.0f);
+        if ((p.x != 0.0f) | (p.y != 0.0f) | (A[(size_t)j * nr + i] != 0.0f)) {
+            h = 1;
+            jlo = min(jlo, j);
+            jhi = j + 1;
+        }
     }
-    if (h) atomicOr(&any[t % kMaskRows], 1);
+    if (h) {
+        atomicOr(&any[t % kMaskRows], 1);
+        atomicMin(&lo[t % kMaskRows], jlo);
+        atomicMax(&hi[t % kMaskRows], jhi);
+    }
     __syncthreads();
-    if (t < kMaskRows) sup[(tile * kMaskRows + t + (nr >> 1)) & (nr - 1)] = (uint8_t)any[t];
+    if (t < kMaskRows) {
+        const uint32_t row = (tile * kMaskRows + t + (nr >> 1)) & (nr - 1);
+        sup[row] = (uint8_t)any[t];
+        if (cols) cols[row] = any[t] ? lo[t] | hi[t] << 16 : 0u;
+    }
     if (t == 0) {
         int m = 0;
         for (int r = 0; r < kMaskRows; ++r) m |= any[r];
@@ -393,8 +412,9 @@ __global__ __launch_bounds__(1024) void pupil_row_mask_kernel(const float2* __re
 }
 
 void launch_pupil_row_mask(const float2* P, const float* A, uint32_t nr, uint32_t nc, uint8_t* mask, uint8_t* sup,
-                           hipStream_t st) {
-    hipLaunchKernelGGL(pupil_row_mask_kernel, dim3(nr / kMaskRows), dim3(1024), 0, st, P, A, nr, nc, mask, sup);
+                           hipStream_t st, uint32_t* cols) {
+    if (cols && nc > 65535u) throw HipError{hipErrorInvalidValue, "pupil column intervals: pupil side above 65535"};
+    hipLaunchKernelGGL(pupil_row_mask_kernel, dim3(nr / kMaskRows), dim3(1024), 0, st, P, A, nr, nc, mask, sup, cols);
     FPM_LAUNCH_CHECK();
 }
 

@@ -76,8 +76,8 @@ __device__ __forceinline__ uint32_t wrap1(uint32_t v, uint32_t n) {
 __device__ __forceinline__ float2 cmul_g(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 cmulc_g(float2 a, float2 b) {  // a conj(b)
-    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+__device__ __forceinline__ float2 cmulc_g(float2 a, float2 b) {  // a conj(b), the roundings of fpm::cmulc
+    return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
 }
 
 // x / d for a divisor fixed per launch: q = (umulhi(x, m) + x) >> s with

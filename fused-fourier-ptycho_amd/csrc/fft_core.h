@@ -35,6 +35,10 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
     return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
 }
+// a * b with the roundings of cmulc written out (the crop x pupil products)
+__device__ __forceinline__ float2 cmulx(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 // a * (-i) for the forward direction, a * (+i) for the inverse one
 template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a) {

@@ -19,6 +19,9 @@ struct CropArgs {
     ColPerm pN;
     const TileIn* tab = nullptr;
     uint32_t ppt = 0;
+    // optional: pupil row i holds P != 0 only in columns [c0, c1), pcols[i] =
+    // c0 | c1 << 16 (launch_pupil_row_mask); grouped tiles: nr words per tile
+    const uint32_t* pcols = nullptr;
 };
 
 // Owner-computes stitch formed inside the first inverse column pass of the

@@ -194,8 +194,12 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     // tiles: page k ppt + l is LED led0 + l of tile k)
     const float2* __restrict__ cS = c.Shat;
     const float2* __restrict__ cP = c.P;
+    // crop: the pupil-support column interval [pc0, pc0 + pcw) of this
+    // thread's pupil row (every slot of thread t lies in row t mod TR)
+    uint32_t pc0 = 0, pcw = 0xffffffffu;
     if constexpr (SRC == SRC_CROP) {
         const int2* cled = c.led;
+        const uint32_t* cpc = c.pcols;
         uint32_t lp = page;
         if (c.tab) {
             const uint32_t tk = page / c.ppt;
@@ -203,10 +207,18 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             cS += (size_t)tk * c.Nr * c.Nc;
             cP += (size_t)tk * c.nr * c.nc;
             cled = c.tab[tk].led;
+            if (cpc) cpc += (size_t)tk * c.nr;
         }
         const int2 k = cled[c.led0 + lp];
         kl = (uint32_t)k.x;
         kt = (uint32_t)k.y;
+        if (cpc) {
+            int line, idx;
+            SL::in(t, 0, 0, line, idx);
+            const uint32_t cc = cpc[(tile * TR + (uint32_t)line + (c.nr >> 1)) & (c.nr - 1)];
+            pc0 = cc & 0xffffu;
+            pcw = (cc >> 16) - pc0;
+        }
     }
     // stitch sources (grouped tiles: page k is tile k's gradient)
     const float2* __restrict__ sR = sa.R;
@@ -226,7 +238,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     uint64_t rowhits = 0;
     uint32_t srow = 0;
     __shared__ uint32_t win[SRC == SRC_STITCH ? kMaxStitchLeds : 1];
-    __shared__ uint32_t wcc[SRC == SRC_STITCH ? kMaxStitchLeds * TR : 1];
+    extern __shared__ uint32_t wcc[];  // stitch: b TR words, sized at launch
     if constexpr (SRC == SRC_STITCH) {
         const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
         for (uint32_t l = t; l < sa.b; l += Cfg::T) {
@@ -257,12 +269,12 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
         constexpr uint32_t NB0 = LEN / SL::R0;
 #pragma unroll
         for (int k = 0; k < 16; ++k) v[k] = make_float2(0.f, 0.f);
+        const uint32_t D = NB0 * a.in_s;
 #pragma unroll
         for (int q = 0; q < SL::NG0; ++q) {
             int line, idx0;
             SL::in(t, q, 0, line, idx0);
             const uint32_t col0 = g * a.in_g + (uint32_t)idx0 * a.in_s;
-            const uint32_t D = NB0 * a.in_s;
             uint64_t m = rowhits;
             while (m) {
                 const int l = __ffsll((unsigned long long)m) - 1;
@@ -309,9 +321,18 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                     const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
                     const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
                     const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
-                    const float2 pp = cP[j * c.nr + i];
                     float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
-                    if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
+                    if (j - pc0 < pcw) {
+                        // with the support interval S is read beside P, not after it
+                        const float2* sp = cS + (size_t)c.pN(scl) * c.Nr + srw;
+                        float2 sv = make_float2(0.f, 0.f);
+                        if (c.pcols) sv = *sp;
+                        const float2 pp = cP[j * c.nr + i];
+                        if (pp.x != 0.0f || pp.y != 0.0f) {
+                            if (!c.pcols) sv = *sp;
+                            z = cmulx(sv, pp);
+                        }
+                    }
                     v[q * SL::GS0 + s] = z;
                 } else {
                     v[q * SL::GS0 + s] = in[roff + col * a.ld];
@@ -380,7 +401,8 @@ static void launch_strided_one(const StridedArgs& a, Gather gs, dim3 grid, Twidd
             return;
         }
         if (gs.stitch) {
-            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_STITCH, ENG>), grid, blk, 0, st, a, c, s, tw);
+            hipLaunchKernelGGL((fft_strided_kernel<L, TR, INV, TWID, SRC_STITCH, ENG>), grid, blk,
+                               (size_t)s.b * TR * sizeof(uint32_t), st, a, c, s, tw);
             return;
         }
     }
@@ -1633,7 +1655,7 @@ __global__ __launch_bounds__(1024, 4) void page_fused_kernel(CropArgs c, float2*
                 const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
                 const float2 pp = cP[j * LEN + i];
                 float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
-                if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
+                if (pp.x != 0.0f || pp.y != 0.0f) z = cmulx(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
                 v[q * SJI::GS0 + s] = z;
             }
         first_pass_dft<LEN, true, T, ENG>(v, ctx);

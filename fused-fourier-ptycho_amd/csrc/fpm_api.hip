@@ -403,7 +403,7 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     const uint8_t* lat_mask = prune ? pmask : nullptr;
     const uint8_t* rec_mask = prune ? pmask : nullptr;
     // crop x pupil (kernels.cuh:19-65) is formed inside the first pass
-    const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN};
+    const CropArgs ca{Shat, P, led, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN, nullptr, 0, pcols};
     if (page_fused_supported(g) && !signs) {
         // 128 x 128 pages: every per-LED stage in one kernel
         tm.begin(FPM_STAGE_STENCIL);
@@ -828,7 +828,7 @@ struct GroupSolve {
         for (uint32_t s = 0; s < steps; ++s) {
             batch_of(s, led0, b);
             const uint32_t pages = T * b;
-            const CropArgs ca{spec[cur], Pw, nullptr, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN, tabd, b};
+            const CropArgs ca{spec[cur], Pw, nullptr, led0, g.nr, g.nc, g.Nr, g.Nc, g.pN, tabd, b, pcols};
             if (page_fused_supported(g)) {
                 launch_page_fused(g, ca, rp, y, pages, pr, nullptr, pmask, tw, st, g.eng);
             } else {

@@ -1245,6 +1245,9 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
 #ifndef FPM_STENCIL_HALF
 #define FPM_STENCIL_HALF 1
 #endif
+#ifndef FPM_STENCIL_SKIP_HALO
+#define FPM_STENCIL_SKIP_HALO 1
+#endif
 namespace halfst {
 constexpr int LEN = 2048, TV = 1024, NLINES = 8;
 using FMap = LineSlowMap<NLINES, LEN>;  // the slot maps of stencil_fused_kernel<2048>
@@ -1525,12 +1528,24 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
     };
     stencil(va, t, loss_a);
     stencil(vb, t + 512, loss_b);
-    first_pass_dft<LEN, false, TV>(va);
-    first_pass_dft<LEN, false, TV>(vb);
+#if FPM_STENCIL_SKIP_HALO
+    // The forward plan's first two passes (radix 16, 128 butterflies per
+    // line) keep virtual thread tv on line tv / 128: the halo lines' inputs
+    // are zero and their outputs are not stored, so the waves of a halo line
+    // skip them (wave-uniform; every line is its own transform, so the other
+    // lines' bits do not change).  Each SIMD holds one such half-idle wave.
+    static_assert(SI::NGL == 1 && SI::GSL == 16 && PassG<false, 0>::R == 16 && PassG<false, 1>::R == 16,
+                  "forward passes 0 and 1 are radix 16 on the inverse's output layout");
+    const bool run_a = interior(t / (LEN / 16)), run_b = interior((t + 512) / (LEN / 16));
+#else
+    const bool run_a = true, run_b = true;
+#endif
+    if (run_a) first_pass_dft<LEN, false, TV>(va);
+    if (run_b) first_pass_dft<LEN, false, TV>(vb);
     __syncthreads();
     exchange<false, 0>(va, vb, lds, t);
-    compute<false, 1>(va, t, tw, ttab_f);
-    compute<false, 1>(vb, t + 512, tw, ttab_f);
+    if (run_a) compute<false, 1>(va, t, tw, ttab_f);
+    if (run_b) compute<false, 1>(vb, t + 512, tw, ttab_f);
     __syncthreads();
     exchange<false, 1>(va, vb, lds, t);
     compute<false, 2>(va, t, tw);

@@ -26,18 +26,16 @@ namespace fpm {
 // ------------------------------------------------------------------ complex
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Complex products with their roundings written out (each part: the first
+// product fused onto the rounded second), so that every kernel -- and every
+// slot map of one transform -- gets the same bits whatever the compiler
+// would contract in its context.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
-// a * conj(b) with its roundings written out (each part: the first product
-// fused onto the rounded second), so that every kernel that forms the stitch
-// or pupil-gradient terms gets the same bits whatever the compiler contracts
+// a * conj(b)
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
     return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
-}
-// a * b with the roundings of cmulc written out (the crop x pupil products)
-__device__ __forceinline__ float2 cmulx(float2 a, float2 b) {
-    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 // a * (-i) for the forward direction, a * (+i) for the inverse one

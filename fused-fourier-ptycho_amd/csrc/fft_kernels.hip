@@ -141,6 +141,60 @@ enum { SRC_PLAIN = 0, SRC_CROP = 1, SRC_STITCH = 2 };
 
 constexpr int kMaxStitchLeds = 64;  // one bit per LED of the batch
 
+// Crop x pupil source of one page (SRC_CROP): its tile's spectrum, pupil
+// and LED (grouped tiles: page k ppt + l is LED led0 + l of tile k), and the
+// pupil-support column interval [pc0, pc0 + pcw) of one pupil row (a thread's
+// slots of a strided pass all lie in one row).
+struct CropSrc {
+    const float2* __restrict__ S;
+    const float2* __restrict__ P;
+    uint32_t kl, kt, pc0, pcw;
+    __device__ __forceinline__ void init(const CropArgs& c, uint32_t page, uint32_t di) {
+        S = c.Shat;
+        P = c.P;
+        const int2* cled = c.led;
+        const uint32_t* cpc = c.pcols;
+        uint32_t lp = page;
+        if (c.tab) {
+            const uint32_t tk = page / c.ppt;
+            lp = page - tk * c.ppt;
+            S += (size_t)tk * c.Nr * c.Nc;
+            P += (size_t)tk * c.nr * c.nc;
+            cled = c.tab[tk].led;
+            if (cpc) cpc += (size_t)tk * c.nr;
+        }
+        const int2 k = cled[c.led0 + lp];
+        kl = (uint32_t)k.x;
+        kt = (uint32_t)k.y;
+        pc0 = 0;
+        pcw = 0xffffffffu;
+        if (cpc) {
+            const uint32_t cc = cpc[(di + (c.nr >> 1)) & (c.nr - 1)];
+            pc0 = cc & 0xffffu;
+            pcw = (cc >> 16) - pc0;
+        }
+    }
+    // point (row di, natural column dj) of ifftshift(S P), cuFPM_v3/cuda/kernels.cuh:38-64
+    __device__ __forceinline__ float2 at(const CropArgs& c, uint32_t di, uint32_t dj) const {
+        const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
+        const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
+        const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
+        float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
+        if (j - pc0 < pcw) {
+            // with the support interval S is read beside P, not after it
+            const float2* sp = S + (size_t)c.pN(scl) * c.Nr + srw;
+            float2 sv = make_float2(0.f, 0.f);
+            if (c.pcols) sv = *sp;
+            const float2 pp = P[j * c.nr + i];
+            if (pp.x != 0.0f || pp.y != 0.0f) {
+                if (!c.pcols) sv = *sp;
+                z = cmul(sv, pp);
+            }
+        }
+        return z;
+    }
+};
+
 template <int LEN, int TR, bool INV, bool TWID, int SRC, int ENG>
 __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs a, CropArgs c, StitchArgs sa,
                                                                    Twiddles tw) {
@@ -189,37 +243,8 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     const uint32_t row = tile * TR + (uint32_t)(t % TR);
     const size_t roff = (size_t)(row >> a.sh) * a.tstride + (row & a.rmask);
     float2 v[16];
-    uint32_t kl = 0, kt = 0;
-    // crop source of this page: its tile's spectrum, pupil and LED (grouped
-    // tiles: page k ppt + l is LED led0 + l of tile k)
-    const float2* __restrict__ cS = c.Shat;
-    const float2* __restrict__ cP = c.P;
-    // crop: the pupil-support column interval [pc0, pc0 + pcw) of this
-    // thread's pupil row (every slot of thread t lies in row t mod TR)
-    uint32_t pc0 = 0, pcw = 0xffffffffu;
-    if constexpr (SRC == SRC_CROP) {
-        const int2* cled = c.led;
-        const uint32_t* cpc = c.pcols;
-        uint32_t lp = page;
-        if (c.tab) {
-            const uint32_t tk = page / c.ppt;
-            lp = page - tk * c.ppt;
-            cS += (size_t)tk * c.Nr * c.Nc;
-            cP += (size_t)tk * c.nr * c.nc;
-            cled = c.tab[tk].led;
-            if (cpc) cpc += (size_t)tk * c.nr;
-        }
-        const int2 k = cled[c.led0 + lp];
-        kl = (uint32_t)k.x;
-        kt = (uint32_t)k.y;
-        if (cpc) {
-            int line, idx;
-            SL::in(t, 0, 0, line, idx);
-            const uint32_t cc = cpc[(tile * TR + (uint32_t)line + (c.nr >> 1)) & (c.nr - 1)];
-            pc0 = cc & 0xffffu;
-            pcw = (cc >> 16) - pc0;
-        }
-    }
+    CropSrc cs;
+    if constexpr (SRC == SRC_CROP) cs.init(c, page, row);
     // stitch sources (grouped tiles: page k is tile k's gradient)
     const float2* __restrict__ sR = sa.R;
     const float2* __restrict__ sP = sa.P;
@@ -317,23 +342,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 SL::in(t, q, s, line, idx);
                 const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
                 if constexpr (SRC == SRC_CROP) {
-                    const uint32_t di = tile * TR + line, dj = (uint32_t)col;
-                    const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
-                    const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
-                    const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
-                    float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
-                    if (j - pc0 < pcw) {
-                        // with the support interval S is read beside P, not after it
-                        const float2* sp = cS + (size_t)c.pN(scl) * c.Nr + srw;
-                        float2 sv = make_float2(0.f, 0.f);
-                        if (c.pcols) sv = *sp;
-                        const float2 pp = cP[j * c.nr + i];
-                        if (pp.x != 0.0f || pp.y != 0.0f) {
-                            if (!c.pcols) sv = *sp;
-                            z = cmulx(sv, pp);
-                        }
-                    }
-                    v[q * SL::GS0 + s] = z;
+                    v[q * SL::GS0 + s] = cs.at(c, tile * TR + line, (uint32_t)col);
                 } else {
                     v[q * SL::GS0 + s] = in[roff + col * a.ld];
                 }
@@ -352,6 +361,223 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
             out[roff + col * a.ld] = r;
         }
+}
+
+// 2048-point strided passes of 8-row tiles (the per-LED column passes at
+// n = 2048) as two 512-thread workgroups per CU.  Each thread runs the slots
+// of virtual threads t and t + 512 of fft_strided_kernel<2048, 8> and the
+// exchanges go through half of the tile's LDS buffer, points [0, 1024) of
+// the 8 rows and then [1024, 2048): 64 KiB instead of 128, so one
+// workgroup's gathers and stores overlap the other's passes.  In the
+// row-fast slot map a radix-16 pass's outputs of virtual thread A (t < 512)
+// are the lower half of every row and B's the upper half, a radix-8 pass's
+// group 0 is the lower half, and the inputs s < R/2 of every butterfly of the
+// next pass lie in the lower half.  Same passes, twiddles and order as the
+// one-workgroup kernel: the same bits.
+#ifndef FPM_STRIDED_HALF
+#define FPM_STRIDED_HALF 1
+#endif
+// twiddles of the first twiddled pass from an LDS table (the half-buffer kernels)
+#ifndef FPM_TWIDDLE_TABLES
+#define FPM_TWIDDLE_TABLES 1
+#endif
+namespace halfsd {
+constexpr int LEN = 2048, NL = 8, TV = 1024, HALF = LEN / 2;
+using FMap = LineFastMap<NL, LEN>;   // the slot arithmetic of fft_strided_kernel<2048, 8>
+using HMap = LineFastMap<NL, HALF>;  // the half buffer
+template <bool INV, int P> struct PassG {
+    static constexpr int R = Plan<LEN, INV>::radix(P), L = Plan<LEN, INV>::span(P), NB = LEN / R, NG = 16 / R;
+};
+__device__ __forceinline__ int hofs(int line, int idx) { return HMap::lds(line, idx & (HALF - 1)); }
+
+// writes of pass P's outputs of virtual thread tv, groups [q0, q1)
+template <bool INV, int P>
+__device__ __forceinline__ void write_groups(const float2 (&v)[16], float2* lds, int tv, int q0, int q1) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        if (q < q0 || q >= q1) continue;
+        const int b = tv + TV * q, line = b % NL, j = b / NL;
+        const int base = (j / G::L) * G::L * G::R + (j % G::L);
+#pragma unroll
+        for (int s = 0; s < G::R; ++s) lds[hofs(line, base + s * G::L)] = v[q * G::R + s];
+    }
+}
+// reads of pass P's inputs s in [s0, s1) of virtual thread tv into d[q * R + s]
+template <bool INV, int P>
+__device__ __forceinline__ void read_points(float2 (&d)[16], const float2* lds, int tv, int s0, int s1) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        const int b = tv + TV * q, line = b % NL, j = b / NL;
+#pragma unroll
+        for (int s = 0; s < G::R; ++s)
+            if (s >= s0 && s < s1) d[q * G::R + s] = lds[hofs(line, j + s * G::NB)];
+    }
+}
+// Pass P's twiddles from an LDS table when its span is at most 16 (values
+// of twiddle_powers, made once per workgroup), else from twiddle_powers.
+template <bool INV, int P> constexpr bool tabled() {
+    return FPM_TWIDDLE_TABLES && PassG<INV, P>::L > 1 && PassG<INV, P>::L <= 16;
+}
+template <bool INV, int P> constexpr int table_words() {
+    return tabled<INV, P>() ? PassG<INV, P>::L * (PassG<INV, P>::R - 1) : 0;
+}
+template <bool INV, int P>
+__device__ __forceinline__ void fill_table(float2* tab, int t, const Twiddles& tw) {
+    using G = PassG<INV, P>;
+    if constexpr (tabled<INV, P>()) {
+        if (t < G::L) {
+            constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+            float2 w[16];
+            twiddle_powers<G::R, INV>(tw, (uint32_t)t * step, w);
+#pragma unroll
+            for (int s = 1; s < G::R; ++s) tab[(s - 1) * G::L + t] = w[s];
+        }
+    }
+}
+template <bool INV, int P>
+__device__ __forceinline__ void compute(float2 (&v)[16], int tv, const Twiddles& tw, const float2* tab = nullptr) {
+    using G = PassG<INV, P>;
+    tv = fresh(tv);
+#pragma unroll
+    for (int q = 0; q < G::NG; ++q) {
+        if constexpr (G::L > 1) {
+            const uint32_t k = (uint32_t)(((tv + TV * q) / NL) % G::L);
+            if constexpr (tabled<INV, P>()) {
+#pragma unroll
+                for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], tab[(s - 1) * G::L + k]);
+            } else {
+                constexpr uint32_t step = kTwiddleLen / (G::L * G::R);
+                float2 w[16];
+                twiddle_powers<G::R, INV>(tw, k * step, w);
+#pragma unroll
+                for (int s = 1; s < G::R; ++s) v[q * G::R + s] = cmul(v[q * G::R + s], w[s]);
+            }
+        }
+        dft<G::R, INV>(&v[q * G::R]);
+    }
+}
+// outputs of pass P (in va / vb) -> inputs of pass P + 1 through the half
+// buffer; the caller's LDS reads of any earlier exchange must be complete
+template <bool INV, int P>
+__device__ __forceinline__ void exchange(float2 (&va)[16], float2 (&vb)[16], float2* lds, int t) {
+    using W = PassG<INV, P>;
+    using Rd = PassG<INV, P + 1>;
+    constexpr int H = Rd::R / 2;  // read points s < H sit in the lower half
+    const int ta = t, tb = t + TV / 2;
+    if constexpr (W::NG > 1) {
+        // small-radix writers: groups [0, NG/2) of both virtual threads hold the lower half
+        static_assert(Rd::NG == 1, "a radix-16 pass follows the small one");
+        write_groups<INV, P>(va, lds, ta, 0, W::NG / 2);
+        write_groups<INV, P>(vb, lds, tb, 0, W::NG / 2);
+        __syncthreads();
+        read_points<INV, P + 1>(va, lds, ta, 0, H);  // registers [0, H): the groups just written
+        read_points<INV, P + 1>(vb, lds, tb, 0, H);
+        __syncthreads();
+        write_groups<INV, P>(va, lds, ta, W::NG / 2, W::NG);
+        write_groups<INV, P>(vb, lds, tb, W::NG / 2, W::NG);
+        __syncthreads();
+        read_points<INV, P + 1>(va, lds, ta, H, Rd::R);
+        read_points<INV, P + 1>(vb, lds, tb, H, Rd::R);
+    } else {
+        // radix-16 writers: A writes the lower half, B the upper
+        float2 tmp[16];
+        write_groups<INV, P>(va, lds, ta, 0, 1);
+        __syncthreads();
+        read_points<INV, P + 1>(va, lds, ta, 0, H);   // va is written out: free
+        read_points<INV, P + 1>(tmp, lds, tb, 0, H);  // vb still holds B's outputs
+        __syncthreads();
+        write_groups<INV, P>(vb, lds, tb, 0, 1);
+        __syncthreads();
+        read_points<INV, P + 1>(va, lds, ta, H, Rd::R);
+        read_points<INV, P + 1>(vb, lds, tb, H, Rd::R);
+#pragma unroll
+        for (int q = 0; q < Rd::NG; ++q)
+#pragma unroll
+            for (int s = 0; s < H; ++s) vb[q * Rd::R + s] = tmp[q * Rd::R + s];
+    }
+}
+}  // namespace halfsd
+
+template <bool INV, bool TWID, int SRC>
+__global__ __launch_bounds__(512, 4) void fft_strided_half_kernel(StridedArgs a, CropArgs c, Twiddles tw) {
+    using namespace halfsd;
+    constexpr int TR = NL;
+    using SL = Slots<LEN, INV, TV, FMap, ENG_F32>;
+    static_assert(Plan<LEN, INV>::NPASS == 3, "2048 points: three passes");
+    __shared__ float2 lds[HMap::LDS_WORDS];
+    __shared__ float2 twl[2 * kTwLo];
+    const int t = threadIdx.x;
+    // the crop launches pages fastest (grid x), as fft_strided_kernel
+    const uint32_t tile = SRC == SRC_CROP ? blockIdx.z : blockIdx.x;
+    const uint32_t page = SRC == SRC_CROP ? blockIdx.x : blockIdx.z;
+    const uint32_t g = blockIdx.y;
+    if (a.rowmask) {  // rows outside every LED window of the batch: nothing to do
+        const uint8_t* rmk = a.rowmask + (a.mask_ppt ? (size_t)(page / a.mask_ppt) * a.mask_stride : 0);
+        bool any = false;
+        for (uint32_t k = (tile * TR) / kMaskRows; k <= (tile * TR + TR - 1) / kMaskRows; ++k)
+            any |= rmk[k] != 0;
+        if (!any) return;  // uniform over the workgroup
+    }
+    tw = bind_twiddles(tw, twl, t, 512);
+    // the table of the first twiddled pass; the exchange before its first use
+    // synchronises the workgroup
+    __shared__ float2 ttab[table_words<INV, 1>() > 0 ? table_words<INV, 1>() : 1];
+    fill_table<INV, 1>(ttab, t, tw);
+    const size_t base = (size_t)page * a.page;
+    const float2* __restrict__ in = a.in + base;
+    float2* __restrict__ out = a.out + base;
+    // row-fast slots: every slot of virtual threads t and t + 512 lies in row t mod 8
+    const uint32_t row = tile * TR + (uint32_t)(t % TR);
+    const size_t roff = (size_t)(row >> a.sh) * a.tstride + (row & a.rmask);
+    CropSrc cs;
+    if constexpr (SRC == SRC_CROP) cs.init(c, page, row);
+    float2 va[16], vb[16];
+    auto load = [&](float2 (&v)[16], int tv) {
+        tv = fresh(tv);
+#pragma unroll
+        for (int q = 0; q < SL::NG0; ++q)
+#pragma unroll
+            for (int s = 0; s < SL::GS0; ++s) {
+                int line, idx;
+                SL::in(tv, q, s, line, idx);
+                const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
+                if constexpr (SRC == SRC_CROP)
+                    v[q * SL::GS0 + s] = cs.at(c, row, (uint32_t)col);
+                else
+                    v[q * SL::GS0 + s] = in[roff + col * a.ld];
+            }
+    };
+    load(va, t);
+    load(vb, t + TV / 2);
+    first_pass_dft<LEN, INV, TV>(va);
+    first_pass_dft<LEN, INV, TV>(vb);
+    exchange<INV, 0>(va, vb, lds, t);
+    compute<INV, 1>(va, t, tw, ttab);
+    compute<INV, 1>(vb, t + TV / 2, tw, ttab);
+    __syncthreads();
+    exchange<INV, 1>(va, vb, lds, t);
+    compute<INV, 2>(va, t, tw);
+    compute<INV, 2>(vb, t + TV / 2, tw);
+    auto store = [&](const float2 (&v)[16], int tv) {
+        tv = fresh(tv);
+#pragma unroll
+        for (int q = 0; q < SL::NGL; ++q)
+#pragma unroll
+            for (int s = 0; s < SL::GSL; ++s) {
+                int line, idx;
+                SL::out(tv, q, s, line, idx);
+                float2 r = v[q * SL::GSL + s];
+                if constexpr (TWID) r = cmul(r, tw.get<INV>(g * (uint32_t)idx * a.tw_step));
+                const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
+                out[roff + col * a.ld] = r;
+            }
+    };
+    store(va, t);
+    store(vb, t + TV / 2);
 }
 
 // -------------------------------------------------------------- launchers
@@ -393,6 +619,19 @@ static void launch_strided_one(const StridedArgs& a, Gather gs, dim3 grid, Twidd
     const CropArgs c = gs.crop ? *gs.crop : CropArgs{};
     const StitchArgs s = gs.stitch ? *gs.stitch : StitchArgs{};
     const dim3 blk(StridedCfg<L, TR>::T);
+    if constexpr (FPM_STRIDED_HALF && L == halfsd::LEN && TR == halfsd::NL && ENG == ENG_F32) {
+        if (!gs.stitch) {
+            if constexpr (INV) {
+                if (gs.crop) {
+                    hipLaunchKernelGGL((fft_strided_half_kernel<INV, TWID, SRC_CROP>), dim3(grid.z, grid.y, grid.x),
+                                       dim3(512), 0, st, a, c, tw);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((fft_strided_half_kernel<INV, TWID, SRC_PLAIN>), grid, dim3(512), 0, st, a, c, tw);
+            return;
+        }
+    }
     if constexpr (INV) {
         if (gs.crop) {
             const dim3 pages_first(grid.z, grid.y, grid.x);
@@ -717,9 +956,6 @@ __global__ __launch_bounds__(ContigCfg<LEN>::T) void fused_update_kernel(float2*
 // 16384^2, batch 26: 2.13 -> 1.88 ms per launch.
 #ifndef FPM_FUSED16K_HALF
 #define FPM_FUSED16K_HALF 1
-#endif
-#ifndef FPM_TWIDDLE_TABLES
-#define FPM_TWIDDLE_TABLES 1
 #endif
 #ifndef FPM_LINEAR_LDS
 #define FPM_LINEAR_LDS 1
@@ -1670,7 +1906,7 @@ __global__ __launch_bounds__(1024, 4) void page_fused_kernel(CropArgs c, float2*
                 const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
                 const float2 pp = cP[j * LEN + i];
                 float2 z = make_float2(0.f, 0.f);  // P = 0: S P = +-0, S not read
-                if (pp.x != 0.0f || pp.y != 0.0f) z = cmulx(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
+                if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
                 v[q * SJI::GS0 + s] = z;
             }
         first_pass_dft<LEN, true, T, ENG>(v, ctx);

@@ -73,8 +73,8 @@ __device__ __forceinline__ uint32_t wrap1(uint32_t v, uint32_t n) {
     return v < n ? v : (v - n < n ? v - n : v % n);
 }
 
-__device__ __forceinline__ float2 cmul_g(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+__device__ __forceinline__ float2 cmul_g(float2 a, float2 b) {  // the roundings of fpm::cmul
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cmulc_g(float2 a, float2 b) {  // a conj(b), the roundings of fpm::cmulc
     return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));

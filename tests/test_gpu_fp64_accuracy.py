@@ -118,7 +118,14 @@ def _sign_synchronised(n, r, B, label, cap_unsync):
         f = 1.5 if name == "fp32" else 3.0
         assert es["G_P"] <= max(1e-5, f * floor["G_P"]), (name, es, floor)
         assert flips <= 1e-4 * decisions, (name, flips)
-        _close_enough(name, e, ref, {"G_O": 1.25, "G_P": 3.0}, cap_unsync)
+        # Unsynchronised, G_O's distance comes from many near-tie flips and is
+        # stable (within 1.25x of the FP32 oracle's); G_P's comes from the few
+        # flips that carry much of its norm, so which decisions flip sets it
+        # (split engine at 512 -> 4096: 1e-4 .. 7e-4 between builds that differ
+        # only in where products are fused): the cap only there.  The
+        # arithmetic is bounded by the synchronised bars above.
+        _close_enough(name, {"G_O": e["G_O"]}, ref, {"G_O": 1.25}, cap_unsync)
+        assert e["G_P"] <= cap_unsync, (name, e)
 
 
 @pytest.mark.timeout(900)
@@ -129,10 +136,10 @@ def test_full_scale_gradient_sign_synchronised_within_1e5_of_fp64():
     within 1e-5 or 1.5x the FP32 oracle's distance with the same signs
     (G_P = sum_l R_l conj(S_l) cancels: measured 1.2e-5 on the GPU, 2.2e-5
     for the FP32 restatement at 512 -> 4096, with FP32 or FP64 summation);
-    unsynchronised, G_O within 1.25x and G_P within 3x of the FP32 oracle's
-    distance to FP64 (G_O's ~1.3e-3 comes from many near-tie flips and is
-    stable; G_P's from a few flips that carry much of its norm, so any two
-    FP32 computations differ by ~2x there), cap 2e-3."""
+    unsynchronised, G_O within 1.25x of the FP32 oracle's distance to FP64
+    (its ~1.3e-3 comes from many near-tie flips and is stable) and both
+    within the cap, 2e-3 (G_P's distance comes from a few flips that carry
+    much of its norm, so which decisions flip sets it)."""
     _sign_synchronised(2048, 8, 26, "fp64-full", 2e-3)
 
 

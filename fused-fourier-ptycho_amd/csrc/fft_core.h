@@ -110,32 +110,40 @@ template <bool INV> __device__ __forceinline__ void dft4(float2& a0, float2& a1,
     a3 = csub(t1, t3);
 }
 
-// exp(-+ 2 pi i m / 8) applied to a (m in 1..3)
-template <bool INV, int M> __device__ __forceinline__ float2 w8(float2 a) {
-    constexpr float h = 0.70710678118654752440f;
-    if constexpr (M == 1) {  // (1 - i)/sqrt2 forward
-        return INV ? make_float2(h * (a.x - a.y), h * (a.x + a.y)) : make_float2(h * (a.x + a.y), h * (a.y - a.x));
-    } else if constexpr (M == 2) {
-        return mul_mi<INV>(a);
-    } else {  // M == 3: (-1 - i)/sqrt2 forward
-        return INV ? make_float2(-h * (a.x + a.y), h * (a.x - a.y)) : make_float2(h * (a.y - a.x), -h * (a.x + a.y));
-    }
-}
-
-// radix-8 as 4 x 2: n = 2 n1 + n2, k = k1 + 4 k2
+// radix-8 as 4 x 2: n = 2 n1 + n2, k = k1 + 4 k2.  The W8^(n2 k1) twiddles
+// of k1 = 1 and 3 are h (1 -+ i) multiples; each is fused into its radix-2
+// butterfly as fmaf(+-h, s, a) with the roundings written out (left to the
+// compiler, whether the product was fused into the butterfly's add followed
+// the surrounding code, so two builds of one transform could differ).
 template <bool INV> __device__ __forceinline__ void dft8(float2* a) {
+    constexpr float h = 0.70710678118654752440f;
     // DFT4 over n1 for each n2
     dft4<INV>(a[0], a[2], a[4], a[6]);
     dft4<INV>(a[1], a[3], a[5], a[7]);
-    // twiddle W8^(n2 k1), n2 = 1
-    a[3] = w8<INV, 1>(a[3]);
-    a[5] = w8<INV, 2>(a[5]);
-    a[7] = w8<INV, 3>(a[7]);
-    // DFT2 over n2 for each k1
-    dft2<INV>(a[0], a[1]);
-    dft2<INV>(a[2], a[3]);
+    // k1 = 1: a3 <- a3 W8 = h (s1, s2); (a2, a3) <- (a2 + a3, a2 - a3)
+    {
+        const float x = a[3].x, y = a[3].y;
+        const float s1 = INV ? x - y : x + y, s2 = INV ? x + y : y - x;
+        const float2 p = a[2];
+        a[2] = make_float2(fmaf(h, s1, p.x), fmaf(h, s2, p.y));
+        a[3] = make_float2(fmaf(-h, s1, p.x), fmaf(-h, s2, p.y));
+    }
+    // k1 = 2: W8^2 = -+i, exact
+    a[5] = mul_mi<INV>(a[5]);
     dft2<INV>(a[4], a[5]);
-    dft2<INV>(a[6], a[7]);
+    // k1 = 3: a7 <- a7 W8^3 = h (s1, s2) with signs folded into +-h
+    {
+        const float x = a[7].x, y = a[7].y;
+        const float2 p = a[6];
+        if constexpr (INV) {  // W8^-3 = (-1 + i)/sqrt2: (-h (x + y), h (x - y))
+            a[6] = make_float2(fmaf(-h, x + y, p.x), fmaf(h, x - y, p.y));
+            a[7] = make_float2(fmaf(h, x + y, p.x), fmaf(-h, x - y, p.y));
+        } else {  // W8^3 = (-1 - i)/sqrt2: (h (y - x), -h (x + y))
+            a[6] = make_float2(fmaf(h, y - x, p.x), fmaf(-h, x + y, p.y));
+            a[7] = make_float2(fmaf(-h, y - x, p.x), fmaf(h, x + y, p.y));
+        }
+    }
+    dft2<INV>(a[0], a[1]);
     // result X[k1 + 4 k2] sits at a[2 k1 + k2]; reorder to natural
     float2 t1 = a[1], t2 = a[2], t3 = a[3], t4 = a[4], t5 = a[5], t6 = a[6];
     a[1] = t2; a[2] = t4; a[3] = t6; a[4] = t1; a[5] = t3; a[6] = t5;

@@ -58,9 +58,12 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // needs p = identity (one pass, cols <= 2048).
 // mask_ppt > 0: one row mask (rows/16 bytes) per mask_ppt pages, back to back
 // (grouped tiles); 0: the one mask serves every page.
+// colmask (optional, one page): the last pass stores only the natural
+// columns whose 16-column group has colmask != 0 (Nc / 16 bytes)
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, float2* nat_out, const uint8_t* rowmask = nullptr,
-                      const PageLayout* lay = nullptr, int eng = ENG_F32, uint32_t mask_ppt = 0);
+                      const PageLayout* lay = nullptr, int eng = ENG_F32, uint32_t mask_ppt = 0,
+                      const uint8_t* colmask = nullptr);
 // With crop, the input is not read: the first pass gathers ifftshift(S P)
 // of each page's LED window (natural frequency columns) instead.
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,

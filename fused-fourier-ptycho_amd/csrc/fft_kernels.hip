@@ -96,6 +96,10 @@ struct StridedArgs {
     // grouped tiles: page p's mask at rowmask + (p / mask_ppt) * mask_stride
     // (mask_ppt 0: one mask for every page)
     uint32_t mask_ppt, mask_stride;
+    // optional: outputs stored only in natural columns c (= g cm_g + idx cm_s)
+    // whose 16-column group has colmask[c / 16] != 0 (one page)
+    const uint8_t* colmask;
+    uint32_t cm_g, cm_s;
 };
 
 static void set_layout(StridedArgs& a, const PageLayout& l) {
@@ -359,7 +363,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             float2 r = v[q * SL::GSL + s];
             if constexpr (TWID) r = cmul(r, tw.get<INV>(g * (uint32_t)idx * a.tw_step));
             const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
-            out[roff + col * a.ld] = r;
+            if (!a.colmask || a.colmask[(g * a.cm_g + (uint32_t)idx * a.cm_s) / kMaskRows]) out[roff + col * a.ld] = r;
         }
 }
 
@@ -573,7 +577,8 @@ __global__ __launch_bounds__(512, 4) void fft_strided_half_kernel(StridedArgs a,
                 float2 r = v[q * SL::GSL + s];
                 if constexpr (TWID) r = cmul(r, tw.get<INV>(g * (uint32_t)idx * a.tw_step));
                 const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
-                out[roff + col * a.ld] = r;
+                if (!a.colmask || a.colmask[(g * a.cm_g + (uint32_t)idx * a.cm_s) / kMaskRows])
+                    out[roff + col * a.ld] = r;
             }
     };
     store(va, t);
@@ -707,7 +712,8 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // place) or natural (into nat_out) frequency columns
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, float2* nat_out, const uint8_t* rowmask, const PageLayout* lay, int eng,
-                      uint32_t mask_ppt) {
+                      uint32_t mask_ppt, const uint8_t* colmask) {
+    if (colmask && batch != 1) throw HipError{hipErrorInvalidValue, "column mask: one page"};
     Twiddles tw{twid};
     StridedArgs a{};
     set_layout(a, lay ? *lay : colmajor_layout(rows, p));
@@ -721,6 +727,9 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
         a.out = nat_out ? nat_out : data;
         a.in_g = a.out_g = 0;
         a.in_s = a.out_s = 1;
+        a.colmask = colmask;  // natural column idx
+        a.cm_g = 0;
+        a.cm_s = 1;
         launch_strided_len<false, false>((int)cols, a, dim3(tiles, 1, batch), tw, st, eng);
         return;
     }
@@ -731,7 +740,8 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
     a.in_s = a.out_s = p.M2;
     a.tw_step = kTwiddleLen / cols;
     launch_strided_len<false, true>((int)p.M1, a, dim3(tiles, p.M2, batch), tw, st, eng);
-    // pass B: length M2 over the contiguous block of group k1
+    // pass B: length M2 over the contiguous block of group k1; output idx of
+    // group k1 is natural column k1 + M1 idx
     a.in_g = p.M2;
     a.in_s = 1;
     if (nat_out) {
@@ -742,6 +752,9 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
         a.out_g = p.M2;
         a.out_s = 1;
     }
+    a.colmask = colmask;
+    a.cm_g = 1;
+    a.cm_s = p.M1;
     launch_strided_len<false, false>((int)p.M2, a, dim3(tiles, p.M1, batch), tw, st, eng);
 }
 

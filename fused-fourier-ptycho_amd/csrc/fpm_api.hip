@@ -488,6 +488,8 @@ struct Solve {
     uint8_t* pmask = nullptr;  // pupil-support row mask (n side)
     uint8_t* psup = nullptr;   // pupil-support rows
     uint32_t* pcols = nullptr; // pupil-support column interval per pupil row
+    uint8_t* csup = nullptr;   // pupil columns inside some row's support interval
+    uint8_t* cmask = nullptr;  // window column mask of the next batch (N side)
 
     void batch_of(uint32_t s, uint32_t& led0, uint32_t& b) const {
         led0 = s * B;
@@ -524,11 +526,13 @@ struct Solve {
         // Row masks of the window rows of the current and the next batch:
         // outside them the spectrum gradient is zero and the spectrum is
         // never read, so the N-side column passes skip those row tiles.
-        const size_t mrows = g.Nr / kMaskRows;
-        const size_t cols_at = (2 * mrows + g.nr / kMaskRows + g.nr + 3) & ~(size_t)3;
+        const size_t mrows = g.Nr / kMaskRows, mcols = g.Nc / kMaskRows;
+        const size_t cols_at = (2 * mrows + g.nr / kMaskRows + g.nr + mcols + g.nc + 3) & ~(size_t)3;
         masks.alloc(cols_at + g.nr * sizeof(uint32_t));
         pmask = masks.as<uint8_t>() + 2 * mrows;
         psup = pmask + g.nr / kMaskRows;
+        cmask = psup + g.nr;
+        csup = cmask + mcols;
         pcols = (uint32_t*)(masks.as<uint8_t>() + cols_at);
         mk[0] = masks.as<uint8_t>();
         mk[1] = masks.as<uint8_t>() + mrows;
@@ -537,12 +541,16 @@ struct Solve {
         // rows where P != 0 or A != 0: the update keeps P = 0 wherever
         // A = 0, so the masks of the initial pupil hold for every step
         launch_pupil_row_mask(P, A, g.nr, g.nc, pmask, psup, st, pcols);
+        launch_pupil_col_support(pcols, g.nr, g.nc, csup, st);
         uint32_t led0, b;
         batch_of(0, led0, b);
         launch_row_mask(led, led0, b, g.Nr, g.nr, psup, mk[0], st);
+        // the spectrum is read only inside the batch's windows (rows and
+        // columns at the pupil support): its last column pass stores those columns
+        launch_row_mask(led, led0, b, g.Nc, g.nc, csup, cmask, st, nullptr, 1, 1);
         tm.begin(FPM_STAGE_SPECTRUM_INIT);
         fft_contig(O, spec[0], g.Nr, g.Nc, false, tw, st, g.eng);
-        fft_cols_forward(spec[0], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[0], nullptr, g.eng);
+        fft_cols_forward(spec[0], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[0], nullptr, g.eng, 0, cmask);
         tm.end(1 + col_launches(g.pN));
     }
 
@@ -567,6 +575,7 @@ struct Solve {
                 uint32_t nled0, nb;
                 batch_of(s + 1 < steps ? s + 1 : 0, nled0, nb);
                 launch_row_mask(led, nled0, nb, g.Nr, g.nr, psup, mk[1 - cur], st);
+                launch_row_mask(led, nled0, nb, g.Nc, g.nc, csup, cmask, st, nullptr, 1, 1);
             }
             tm.begin(FPM_STAGE_FUSED_UPDATE);
             launch_fused_update(g, spec[1 - cur], O, mO.as<float>(), rms, !last, mk[cur], last ? nullptr : mk[1 - cur],
@@ -574,7 +583,8 @@ struct Solve {
             tm.end(1);
             if (!last) {
                 tm.begin(FPM_STAGE_FFT_N_COLS_HI);
-                fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng);
+                fft_cols_forward(spec[1 - cur], g.Nr, g.Nc, 1, g.pN, tw, st, nullptr, mk[1 - cur], nullptr, g.eng, 0,
+                                 cmask);
                 tm.end(col_launches(g.pN));
                 cur = 1 - cur;
             }

@@ -170,9 +170,15 @@ bool fused_update_groupable(const Geom& g);
 // covers a row of the tile at a pupil row i with sup[i] != 0 (sup: nr bytes,
 // or nullptr for every pupil row); Nr / 16 bytes
 // tab (optional): grouped tiles, tile k's LED windows from tab[k], its
-// support at sup + k nr and its mask at mask + k Nr/16
+// support at sup + k nr and its mask at mask + k Nr/16.
+// axis 1: the same for 16-column groups of the spectrum (Nr, nr, sup the
+// column count, pupil columns and their support; the LEDs' kl offsets)
 void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, const uint8_t* sup,
-                     uint8_t* mask, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
+                     uint8_t* mask, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1,
+                     int axis = 0);
+// csup (nc bytes) = 1 for the pupil columns inside the support interval of
+// some pupil row (cols: nr words from launch_pupil_row_mask)
+void launch_pupil_col_support(const uint32_t* cols, uint32_t nr, uint32_t nc, uint8_t* csup, hipStream_t st);
 
 // pupil support: sup[i] (nr bytes) = 1 iff pupil row i holds P != 0 or A != 0;
 // mask (nr/16 bytes): the same per 16-row tile of the ifftshifted per-LED

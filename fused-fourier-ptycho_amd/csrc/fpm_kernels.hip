@@ -319,7 +319,8 @@ constexpr int kRmEntries = 8;
 constexpr int kRmMaxSup = 16384;  // pupil rows (the largest side)
 __global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ led, uint32_t led0, uint32_t b,
                                                        uint32_t Nr, uint32_t nr, const uint8_t* __restrict__ sup,
-                                                       uint8_t* __restrict__ mask, const TileIn* __restrict__ tab) {
+                                                       uint8_t* __restrict__ mask, const TileIn* __restrict__ tab,
+                                                       int axis) {
     __shared__ uint8_t ssup[kRmMaxSup];
     if (tab) {
         const uint32_t tk = blockIdx.y;
@@ -336,7 +337,8 @@ __global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ 
     bool hit = false;
     if (e < Nr / kMaskRows) {
         for (uint32_t l = lane; l < b; l += 32) {
-            const uint32_t wr = ((uint32_t)led[led0 + l].y - 1u + (Nr >> 1)) & Rm;
+            const int2 k = led[led0 + l];
+            const uint32_t wr = ((uint32_t)(axis ? k.x : k.y) - 1u + (Nr >> 1)) & Rm;
             for (uint32_t r = 0; r < kMaskRows; ++r) {
                 const uint32_t i = (ra + r - wr) & Rm;
                 hit |= i < nr && (!sup || ssup[i]);
@@ -348,13 +350,33 @@ __global__ __launch_bounds__(256) void row_mask_kernel(const int2* __restrict__ 
 }
 
 void launch_row_mask(const int2* led, uint32_t led0, uint32_t b, uint32_t Nr, uint32_t nr, const uint8_t* sup,
-                     uint8_t* mask, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
+                     uint8_t* mask, hipStream_t st, const TileIn* tab, uint32_t ntiles, int axis) {
     const uint32_t entries = Nr / kMaskRows;
     if (!tab) ntiles = 1;
     if (ntiles < 1 || ntiles > 65535) throw HipError{hipErrorInvalidValue, "row mask: 1..65535 tiles"};
     if (nr > (uint32_t)kRmMaxSup) throw HipError{hipErrorInvalidValue, "row mask: pupil side above 16384"};
     hipLaunchKernelGGL(row_mask_kernel, dim3((entries + kRmEntries - 1) / kRmEntries, ntiles), dim3(256), 0, st, led,
-                       led0, b, Nr, nr, sup, mask, tab);
+                       led0, b, Nr, nr, sup, mask, tab, axis);
+    FPM_LAUNCH_CHECK();
+}
+
+// csup[j] = 1 iff pupil column j lies in the support interval of some pupil
+// row (cols from launch_pupil_row_mask): a superset of the columns holding a
+// P != 0 or an A != 0, at every step of a solve.
+__global__ __launch_bounds__(256) void pupil_col_support_kernel(const uint32_t* __restrict__ cols, uint32_t nr,
+                                                                uint32_t nc, uint8_t* __restrict__ csup) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= nc) return;
+    uint8_t hit = 0;
+    for (uint32_t i = 0; i < nr; ++i) {
+        const uint32_t cc = cols[i], c0 = cc & 0xffffu;
+        hit |= (j - c0 < (cc >> 16) - c0) ? 1 : 0;
+    }
+    csup[j] = hit;
+}
+
+void launch_pupil_col_support(const uint32_t* cols, uint32_t nr, uint32_t nc, uint8_t* csup, hipStream_t st) {
+    hipLaunchKernelGGL(pupil_col_support_kernel, dim3((nc + 255) / 256), dim3(256), 0, st, cols, nr, nc, csup);
     FPM_LAUNCH_CHECK();
 }
 

@@ -268,6 +268,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     uint32_t srow = 0;
     __shared__ uint32_t win[SRC == SRC_STITCH ? kMaxStitchLeds : 1];
     extern __shared__ uint32_t wcc[];  // stitch: b TR words, sized at launch
+    __shared__ uint64_t colhits[SRC == SRC_STITCH ? LEN : 1];
     if constexpr (SRC == SRC_STITCH) {
         const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
         for (uint32_t l = t; l < sa.b; l += Cfg::T) {
@@ -281,6 +282,15 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             const uint32_t l = e / TR, li = (tile * TR + e % TR - (win[l] & 0xffffu)) & Rm;
             wcc[e] = li < sa.nr ? (spc ? spc[li] : sa.nc << 16) : 0u;  // 0: row outside the window
         }
+        // the LEDs whose window covers each of the LEN column positions of the
+        // workgroup's lines (position idx: column g in_g + idx in_s)
+        for (uint32_t idx = t; idx < (uint32_t)LEN; idx += Cfg::T) {
+            const uint32_t col = g * a.in_g + idx * a.in_s;
+            uint64_t mk = 0;
+            for (uint32_t l = 0; l < sa.b; ++l)
+                if (((col - (win[l] >> 16)) & Cm) < sa.nc) mk |= 1ull << l;
+            colhits[idx] = mk;
+        }
         __syncthreads();
         srow = tile * TR + (uint32_t)(t % TR);  // LineFast: one row per thread
         for (uint32_t l = 0; l < sa.b; ++l) {
@@ -289,54 +299,46 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
         }
     }
     if constexpr (SRC == SRC_STITCH) {
-        // A thread's slots of butterfly q sit at columns col0 + s*D (s < R0),
-        // D = (LEN/R0) * in_s: loop over the LEDs covering the thread's row
-        // (in LED order) and, per LED, test each slot's column against the
-        // window -- the LED's row offset and page are computed once.
+        // Each slot sums, in LED order, the LEDs whose window covers both its
+        // row (rowhits) and its column (colhits of its column position): the
+        // work follows the actual window hits.
         const uint32_t Rm = sa.Nr - 1, Cm = sa.Nc - 1;
         const uint32_t plane = sa.nr * sa.nc;
-        constexpr uint32_t NB0 = LEN / SL::R0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = make_float2(0.f, 0.f);
-        const uint32_t D = NB0 * a.in_s;
+        for (int q = 0; q < SL::NG0; ++q)
 #pragma unroll
-        for (int q = 0; q < SL::NG0; ++q) {
-            int line, idx0;
-            SL::in(t, q, 0, line, idx0);
-            const uint32_t col0 = g * a.in_g + (uint32_t)idx0 * a.in_s;
-            uint64_t m = rowhits;
-            while (m) {
-                const int l = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                const uint32_t w = win[l];
-                const uint32_t li = (srow - (w & 0xffffu)) & Rm;
-                const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
-                const uint32_t d = (col0 - (w >> 16)) & Cm;
-                const float2* __restrict__ Rl = sR + (size_t)l * plane + sa.nl.row_off(di);
-                // P = 0 outside the row's support interval [c0, c0 + cw) (within the window)
-                const uint32_t cc = wcc[l * TR + t % TR];
-                const uint32_t c0 = cc & 0xffffu, cw = (cc >> 16) - c0;
-#pragma unroll
-                for (int s = 0; s < SL::GS0; ++s) {
-                    const uint32_t lj = (d + (uint32_t)s * D) & Cm;
-                    if (lj - c0 < cw) {
+            for (int s = 0; s < SL::GS0; ++s) {
+                int line, idx;
+                SL::in(t, q, s, line, idx);
+                const uint32_t col = g * a.in_g + (uint32_t)idx * a.in_s;
+                float2 acc = make_float2(0.f, 0.f);
+                uint64_t m = rowhits & colhits[idx];
+                while (m) {
+                    const int l = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const uint32_t w = win[l];
+                    const uint32_t li = (srow - (w & 0xffffu)) & Rm;
+                    const uint32_t lj = (col - (w >> 16)) & Cm;  // < nc: the window covers col
+                    const uint32_t cc = wcc[l * TR + t % TR];
+                    const uint32_t c0 = cc & 0xffffu;
+                    if (lj - c0 < (cc >> 16) - c0) {  // P = 0 outside the row's support interval
+                        const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
                         const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
-                        const float2* rp = Rl + (size_t)sa.nl.perm(dj) * sa.nl.ld;
-                        // with the support intervals R is read beside P (inside
-                        // the pupil disk P is nonzero), not after it
+                        const float2* rp = sR + (size_t)l * plane + sa.nl.row_off(di) + (size_t)sa.nl.perm(dj) * sa.nl.ld;
                         float2 rv = make_float2(0.f, 0.f);
-                        if (spc) rv = *rp;
+                        if (spc) rv = *rp;  // R beside P inside the interval
                         const float2 pp = sP[lj * sa.nr + li];
                         if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0
                             if (!spc) rv = *rp;
                             const float2 gw = cmulc(rv, pp);
-                            v[q * SL::GS0 + s].x += gw.x * sa.invN2;
-                            v[q * SL::GS0 + s].y += gw.y * sa.invN2;
+                            acc.x = fmaf(gw.x, sa.invN2, acc.x);
+                            acc.y = fmaf(gw.y, sa.invN2, acc.y);
                         }
                     }
                 }
+                v[q * SL::GS0 + s] = acc;
             }
-        }
+
     } else {
 #pragma unroll
         for (int q = 0; q < SL::NG0; ++q)

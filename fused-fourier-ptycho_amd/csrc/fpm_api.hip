@@ -449,7 +449,7 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
     }
     if (stitch == FPM_STITCH_OWNER) {
         tm.begin(FPM_STAGE_STITCH_OWNER);
-        launch_owner_stitch(g, rec, P, led, led0, b, Gs, st);
+        launch_owner_stitch(g, rec, P, led, led0, b, Gs, st, nullptr, 1, pcols);
         tm.end(1);
         tm.begin(FPM_STAGE_BACKPROJ);
         launch_backproj(g, rec, Shat, led, led0, b, P, Gs, kBackprojPupil, mP, A, rms, GP, st);
@@ -848,7 +848,7 @@ struct GroupSolve {
                 fft_cols_forward(rp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, pmask, &g.nl, g.eng, b);
             }
             if (separate_stitch()) {
-                launch_owner_stitch(g, rp, Pw, nullptr, led0, b, spec[1 - cur], st, tabd, T);
+                launch_owner_stitch(g, rp, Pw, nullptr, led0, b, spec[1 - cur], st, tabd, T, pcols);
                 fft_cols_inverse(spec[1 - cur], g.Nr, g.Nc, T, g.pN, tw, st, nullptr, nullptr, nullptr, mk[cur], nullptr,
                                  g.eng, 1);
             } else {

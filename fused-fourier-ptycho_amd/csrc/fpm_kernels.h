@@ -123,8 +123,11 @@ void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const i
 // of R_l . conj(P) / (Nr Nc) over the windows covering it (zero elsewhere).
 // tab (optional): grouped tiles, grid z = tile k (R pages k b.., P and Gs
 // blocks k, LED windows from tab[k])
+// pcols (optional): the pupil-support column interval of each pupil row
+// (launch_pupil_row_mask; grouped tiles: nr words per tile), P read only inside
 void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b,
-                         float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
+                         float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1,
+                         const uint32_t* pcols = nullptr);
 
 // contiguous inverse pass + loss stencil + contiguous forward pass of a batch
 // of n-side pages in one kernel (z_cols: pages after their inverse column

@@ -220,7 +220,8 @@ constexpr int kOwTR = 64, kOwTC = 16;
 __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2* __restrict__ R,
                                                            const float2* __restrict__ P, const int2* __restrict__ led,
                                                            uint32_t led0, uint32_t b, float2* __restrict__ Gs,
-                                                           float invN2, const TileIn* __restrict__ tab) {
+                                                           float invN2, const TileIn* __restrict__ tab,
+                                                           const uint32_t* __restrict__ pcols) {
     __shared__ uint64_t hit[4];
     if (tab) {
         const uint32_t tk = blockIdx.z;
@@ -228,6 +229,7 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
         P += (size_t)tk * g.nr * g.nc;
         Gs += (size_t)tk * g.Nr * g.Nc;
         led = tab[tk].led;
+        if (pcols) pcols += (size_t)tk * g.nr;
     }
     const int t = threadIdx.x;
     const int r = t & 63, cg = t >> 6;
@@ -263,19 +265,31 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
                 const uint32_t wr = ((uint32_t)k.y - 1u + (g.Nr >> 1)) & Rm;
                 const uint32_t wc = ((uint32_t)k.x - 1u + (g.Nc >> 1)) & Cm;
                 const uint32_t li = (row - wr) & Rm;
+                if (li >= g.nr) continue;
+                // P = 0 outside pupil row li's support interval [c0, c0 + cw);
+                // inside it R is loaded beside P
+                uint32_t c0 = 0, cw = g.nc;
+                if (pcols) {
+                    const uint32_t cc = pcols[li];
+                    c0 = cc & 0xffffu;
+                    cw = (cc >> 16) - c0;
+                }
+                const uint32_t di = (li + (g.nr >> 1)) & (g.nr - 1);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t col = ca + cg * 4 + q;
                     const uint32_t lj = (col - wc) & Cm;
-                    if (li < g.nr && lj < g.nc) {
-                        const uint32_t di = (li + (g.nr >> 1)) & (g.nr - 1);
+                    if (lj - c0 < cw) {
                         const uint32_t dj = (lj + (g.nc >> 1)) & (g.nc - 1);
+                        const float2* rp = R + (size_t)ell * plane + g.nl.off(di, dj);
+                        float2 Rv = make_float2(0.f, 0.f);
+                        if (pcols) Rv = *rp;
                         const float2 pp = P[lj * g.nr + li];
                         if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0
-                            const float2 Rv = R[(size_t)ell * plane + g.nl.off(di, dj)];
+                            if (!pcols) Rv = *rp;
                             const float2 gw = cmulc(Rv, pp);
-                            acc[q].x += gw.x * invN2;
-                            acc[q].y += gw.y * invN2;
+                            acc[q].x = fmaf(gw.x, invN2, acc[q].x);
+                            acc[q].y = fmaf(gw.y, invN2, acc[q].y);
                         }
                     }
                 }
@@ -293,12 +307,12 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
 }
 
 void launch_owner_stitch(const Geom& g, const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b,
-                         float2* Gs, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
+                         float2* Gs, hipStream_t st, const TileIn* tab, uint32_t ntiles, const uint32_t* pcols) {
     const float invN2 = 1.0f / (float)(g.Nr * g.Nc);
     if (!tab) ntiles = 1;
     if (ntiles < 1 || ntiles > 65535) throw HipError{hipErrorInvalidValue, "owner stitch: 1..65535 tiles"};
     dim3 grid((g.Nr + kOwTR - 1) / kOwTR, (g.Nc + kOwTC - 1) / kOwTC, ntiles);
-    hipLaunchKernelGGL(owner_stitch_kernel, grid, dim3(256), 0, st, g, R, P, led, led0, b, Gs, invN2, tab);
+    hipLaunchKernelGGL(owner_stitch_kernel, grid, dim3(256), 0, st, g, R, P, led, led0, b, Gs, invN2, tab, pcols);
     FPM_LAUNCH_CHECK();
 }
 

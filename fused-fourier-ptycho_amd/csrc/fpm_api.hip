@@ -620,6 +620,10 @@ struct Solve {
         rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
         spec[0] = S0.as<float2>();
         spec[1] = S1.as<float2>();
+        // pupil-support column intervals (the stitch reads P only inside them)
+        masks.alloc(g.nr * sizeof(uint32_t));
+        pcols = masks.as<uint32_t>();
+        generic_pupil_cols(P, A, g, pcols, st);
         generic_prepare(g);
         generic_tune(g, 1, B, O, S0.as<float2>(), P, led, nullptr, ws.lat, st);
         // with the fused update the spectrum is carried from step to step
@@ -672,7 +676,7 @@ struct Solve {
             generic_forward_pages(rec, b, g, st, fused);
             tm.end(fused ? 1 : 2);
             tm.begin(FPM_STAGE_STITCH_OWNER);
-            generic_stitch(rec, P, led, led0, b, g, G, st);  // reads this step's pupil
+            generic_stitch(rec, P, led, led0, b, g, G, st, nullptr, 1, pcols);  // reads this step's pupil
             tm.end(1);
             tm.begin(FPM_STAGE_BACKPROJ);
             generic_pupil(rec, S, led, led0, b, g, P, mP.as<float>(), A, rms, nullptr, st);
@@ -805,7 +809,8 @@ struct GroupSolve {
         FPM_HIP(hipMemsetAsync(mP.p, 0, T * nn * sizeof(float), st));
         launch_solve_state_init(ss, lr, st);
         rms = RmsArgs{lr, beta, eps, clamp, ss.lr};
-        if (g.generic) {  // no masks; the spectrum is carried only with the fused update
+        if (g.generic) {  // no row masks; the spectrum is carried only with the fused update
+            generic_pupil_cols(P.as<float2>(), nullptr, g, pcols, st, tabd, T);
             if (!generic_fused_pages(g, false)) ebuf.alloc((size_t)T * B * nn * sizeof(float));
             generic_prepare(g);
             generic_tune(g, T, B, O.as<float2>(), S0.as<float2>(), P.as<float2>(), nullptr, tabd, lat.as<float2>(),
@@ -895,7 +900,7 @@ struct GroupSolve {
             generic_inverse_pages(S, Pw, nullptr, led0, b, g, lp, st, tabd, T, fused);
             generic_stencil_pages(lp, y, led0, b, pr, g, ebuf.as<float>(), rp, nullptr, st, tabd, T, nullptr, fused);
             generic_forward_pages(rp, T * b, g, st, fused);
-            generic_stitch(rp, Pw, nullptr, led0, b, g, G, st, tabd, T);
+            generic_stitch(rp, Pw, nullptr, led0, b, g, G, st, tabd, T, pcols);
             generic_pupil(rp, S, nullptr, led0, b, g, Pw, mP.as<float>(), nullptr, rms, nullptr, st, tabd, T);
             if (fuse_update) {
                 const bool last = last_epoch && (s == steps - 1);

@@ -57,9 +57,15 @@ void generic_stencil_pages(const float2* z, const YStack& y, uint32_t led0, uint
                            float* ebuf, float2* u, double* loss_acc, hipStream_t st, const TileIn* tab,
                            uint32_t ntiles, uint8_t* signs, bool fused);
 void generic_forward_pages(float2* rec, uint32_t pages, const Geom& g, hipStream_t st, bool fused);
-// owner-computes stitch of the b back-projections into G_S (every element written)
+// owner-computes stitch of the b back-projections into G_S (every element
+// written); pcols (optional): pupil-support column intervals (generic_pupil_cols)
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                    float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1);
+                    float2* Gs, hipStream_t st, const TileIn* tab = nullptr, uint32_t ntiles = 1,
+                    const uint32_t* pcols = nullptr);
+// cols (nr words per tile) = the smallest column interval of each pupil row
+// holding every P != 0 and A != 0 (A per tile from tab when grouped)
+void generic_pupil_cols(const float2* P, const float* A, const Geom& g, uint32_t* cols, hipStream_t st,
+                        const TileIn* tab = nullptr, uint32_t ntiles = 1);
 // pupil gradient; with mP and A the RMSprop + clamp update of P, else GP = gradient
 void generic_pupil(const float2* R, const float2* S, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
                    float2* P, float* mP, const float* A, RmsArgs rms, float2* GP, hipStream_t st,

@@ -911,11 +911,38 @@ __global__ __launch_bounds__(FPM_GEN_LB) void gen_cols_fused_kernel(GenFFT a, Ge
 // first keeps the LEDs whose windows meet the block (LDS bit mask), then
 // sums only those.
 constexpr uint32_t kGsTR = 64, kGsTC = 16;
+// cols[i] = c0 | c1 << 16: the smallest column interval of pupil row i that
+// holds every P != 0 and A != 0 (0: none); P = 0 outside it at every step of
+// a solve (the clamped update keeps P = 0 where A = 0).  Grouped tiles: grid
+// row k is tile k (its pupil block, its A from tab[k], its nr words).
+__global__ __launch_bounds__(256) void gen_pupil_cols_kernel(const float2* __restrict__ P, const float* __restrict__ A,
+                                                             uint32_t nr, uint32_t nc, uint32_t* __restrict__ cols,
+                                                             const TileIn* __restrict__ tab) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (tab) {
+        const uint32_t tk = blockIdx.y;
+        P += (size_t)tk * nr * nc;
+        A = tab[tk].A;
+        cols += (size_t)tk * nr;
+    }
+    if (i >= nr) return;
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint32_t j = 0; j < nc; ++j) {
+        const float2 p = P[(size_t)j * nr + i];
+        if (p.x != 0.0f || p.y != 0.0f || A[(size_t)j * nr + i] != 0.0f) {
+            lo = min(lo, j);
+            hi = j + 1;
+        }
+    }
+    cols[i] = hi ? lo | hi << 16 : 0u;
+}
+
 __global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restrict__ R, const float2* __restrict__ P,
                                                          const int2* __restrict__ led, uint32_t led0, uint32_t b,
                                                          uint32_t nr, uint32_t nc, uint32_t Nr, uint32_t Nc,
                                                          float invN2, float2* __restrict__ Gs,
-                                                         const TileIn* __restrict__ tab) {
+                                                         const TileIn* __restrict__ tab,
+                                                         const uint32_t* __restrict__ pcols) {
     __shared__ unsigned long long hit[4];
     const size_t plane = (size_t)nr * nc;
     if (tab) {
@@ -924,6 +951,7 @@ __global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restric
         P += (size_t)tk * plane;
         Gs += (size_t)tk * Nr * Nc;
         led = tab[tk].led;
+        if (pcols) pcols += (size_t)tk * nr;
     }
     const uint32_t tiles_r = (Nr + kGsTR - 1) / kGsTR;
     const uint32_t ra = (blockIdx.x % tiles_r) * kGsTR, ca = (blockIdx.x / tiles_r) * kGsTC;
@@ -961,18 +989,30 @@ __global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restric
                 const uint32_t li = row >= wr ? row - wr : row + Nr - wr;
                 if (li >= nr) continue;
                 const uint32_t di = wrap1(li + nr / 2, nr);
+                // P = 0 outside pupil row li's support interval [c0, c0 + cw);
+                // inside it R is loaded beside P
+                uint32_t c0 = 0, cw = nc;
+                if (pcols) {
+                    const uint32_t cc = pcols[li];
+                    c0 = cc & 0xffffu;
+                    cw = (cc >> 16) - c0;
+                }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t col = ca + cg * 4u + (uint32_t)q;
                     if (col >= Nc) continue;
                     const uint32_t lj = col >= wc ? col - wc : col + Nc - wc;
-                    if (lj < nc) {
+                    if (lj - c0 < cw) {
+                        const uint32_t dj = wrap1(lj + nc / 2, nc);
+                        const float2* rp = R + (size_t)l * plane + (size_t)dj * nr + di;
+                        float2 rv = make_float2(0.f, 0.f);
+                        if (pcols) rv = *rp;
                         const float2 pp = P[(size_t)lj * nr + li];
                         if (pp.x != 0.0f || pp.y != 0.0f) {
-                            const uint32_t dj = wrap1(lj + nc / 2, nc);
-                            const float2 gw = cmulc_g(R[(size_t)l * plane + (size_t)dj * nr + di], pp);
-                            acc[q].x += gw.x * invN2;
-                            acc[q].y += gw.y * invN2;
+                            if (!pcols) rv = *rp;
+                            const float2 gw = cmulc_g(rv, pp);
+                            acc[q].x = fmaf(gw.x, invN2, acc[q].x);
+                            acc[q].y = fmaf(gw.y, invN2, acc[q].y);
                         }
                     }
                 }
@@ -1468,12 +1508,21 @@ void generic_forward_pages(float2* rec, uint32_t pages, const Geom& g, hipStream
     launch_generic_lines(r, st);
 }
 
+void generic_pupil_cols(const float2* P, const float* A, const Geom& g, uint32_t* cols, hipStream_t st,
+                        const TileIn* tab, uint32_t ntiles) {
+    if (!tab) ntiles = 1;
+    if (g.nc > 65535u) throw HipError{hipErrorInvalidValue, "pupil column intervals: pupil side above 65535"};
+    hipLaunchKernelGGL(gen_pupil_cols_kernel, dim3((g.nr + 255) / 256, ntiles), dim3(256), 0, st, P, A, g.nr, g.nc,
+                       cols, tab);
+    FPM_LAUNCH_CHECK();
+}
+
 void generic_stitch(const float2* R, const float2* P, const int2* led, uint32_t led0, uint32_t b, const Geom& g,
-                    float2* Gs, hipStream_t st, const TileIn* tab, uint32_t ntiles) {
+                    float2* Gs, hipStream_t st, const TileIn* tab, uint32_t ntiles, const uint32_t* pcols) {
     if (!tab) ntiles = 1;
     const uint32_t blocks = ((g.Nr + kGsTR - 1) / kGsTR) * ((g.Nc + kGsTC - 1) / kGsTC);
     hipLaunchKernelGGL(gen_stitch_kernel, dim3(blocks, ntiles), dim3(256), 0, st, R, P, led, led0, b, g.nr, g.nc,
-                       g.Nr, g.Nc, 1.0f / (float)(g.Nr * g.Nc), Gs, tab);
+                       g.Nr, g.Nc, 1.0f / (float)(g.Nr * g.Nc), Gs, tab, pcols);
     FPM_LAUNCH_CHECK();
 }
 

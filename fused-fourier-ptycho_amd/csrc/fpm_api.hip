@@ -792,12 +792,42 @@ struct GroupSolve {
         ss.ntiles = T;
         ss.hist_stride = ne;
         htab.resize(T);
+        // uint16 stacks (on the device): every tile's min / max reduction is
+        // launched first and read back once (prepare_y's steps and checks)
+        DevBuf mm;
+        mm.alloc((size_t)T * 2 * sizeof(uint32_t));
+        std::vector<uint32_t> mmh(2 * (size_t)T);
+        for (uint32_t k = 0; k < T; ++k) {
+            mmh[2 * k] = 0xffffu;
+            mmh[2 * k + 1] = 0u;
+        }
+        FPM_HIP(hipMemcpyAsync(mm.p, mmh.data(), mmh.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        std::vector<YStack> ys(T);
         for (uint32_t k = 0; k < T; ++k) {
             const fpm_tile& tl = tiles[members[k]];
             FPM_HIP(hipMemcpyAsync(O.as<float2>() + k * NN, tl.O0, NN * sizeof(float2), hipMemcpyDeviceToDevice, st));
             FPM_HIP(hipMemcpyAsync(P.as<float2>() + k * nn, tl.P0, nn * sizeof(float2), hipMemcpyDeviceToDevice, st));
-            ybuf.push_back(std::make_unique<DevBuf>());
-            const YStack yk = prepare_y(tl.Y, tl.y_dtype, nn * (size_t)L, true, *ybuf.back(), st);
+            if (tl.y_dtype == FPM_Y_U16_RAW) {
+                ys[k].data = tl.Y;
+                ys[k].u16 = 1;
+                launch_u16_minmax((const uint16_t*)tl.Y, nn * (size_t)L, mm.as<uint32_t>() + 2 * k, st);
+            } else {
+                ybuf.push_back(std::make_unique<DevBuf>());
+                ys[k] = prepare_y(tl.Y, tl.y_dtype, nn * (size_t)L, true, *ybuf.back(), st);
+            }
+        }
+        FPM_HIP(hipMemcpyAsync(mmh.data(), mm.p, mmh.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        FPM_HIP(hipStreamSynchronize(st));
+        for (uint32_t k = 0; k < T; ++k) {
+            const fpm_tile& tl = tiles[members[k]];
+            YStack& yk = ys[k];
+            if (yk.u16) {
+                if (mmh[2 * k + 1] <= mmh[2 * k])
+                    invalid("uint16 image stack is constant (max == min): amplitudes undefined");
+                yk.mn = mmh[2 * k];
+                yk.range = (float)(mmh[2 * k + 1] - mmh[2 * k]);
+                yk.rcp_range = 1.0f / yk.range;
+            }
             y.u16 = yk.u16;
             htab[k] = TileIn{(const int2*)tl.led_idx, yk.data, yk.mn, yk.range, yk.rcp_range, 0u, tl.A,
                              (want_loss && tl.loss_per_epoch) ? ss.loss_cur + k : nullptr};

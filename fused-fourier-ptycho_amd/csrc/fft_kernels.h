@@ -60,10 +60,14 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // (grouped tiles); 0: the one mask serves every page.
 // colmask (optional, one page): the last pass stores only the natural
 // columns whose 16-column group has colmask != 0 (Nc / 16 bytes)
+// pcols (optional, one pass): pupil-support column intervals c0 | c1 << 16 of
+// the rows (launch_pupil_row_mask's, indexed by the pupil row of the
+// ifftshifted page and tested on the pupil column); each row stores only the
+// points inside its interval (with mask_ppt: rows intervals per mask_ppt pages)
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, float2* nat_out, const uint8_t* rowmask = nullptr,
                       const PageLayout* lay = nullptr, int eng = ENG_F32, uint32_t mask_ppt = 0,
-                      const uint8_t* colmask = nullptr);
+                      const uint8_t* colmask = nullptr, const uint32_t* pcols = nullptr);
 // With crop, the input is not read: the first pass gathers ifftshift(S P)
 // of each page's LED window (natural frequency columns) instead.
 void fft_cols_inverse(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,

@@ -100,6 +100,28 @@ struct StridedArgs {
     // whose 16-column group has colmask[c / 16] != 0 (one page)
     const uint8_t* colmask;
     uint32_t cm_g, cm_s;
+    // optional (one-pass forward transforms of per-LED spectra): row r stores
+    // only the natural columns c whose pupil column (c + nc/2) mod nc lies in
+    // the support interval pcols[(r + nr/2) mod nr] (c0 | c1 << 16) -- the
+    // only points the stitch and the pupil update read; grouped tiles: page
+    // p's intervals at pcols + (p / mask_ppt) * pc_nr
+    const uint32_t* pcols;
+    uint32_t pc_nr, pc_nc;
+};
+
+// [c0, c0 + w) of the stored columns of one row (everything without intervals)
+struct StoreSpan {
+    uint32_t c0 = 0, w = 0xffffffffu, cm = 0, half = 0;
+    __device__ __forceinline__ void init(const StridedArgs& a, uint32_t page, uint32_t row) {
+        if (!a.pcols) return;
+        const uint32_t* pc = a.pcols + (a.mask_ppt ? (size_t)(page / a.mask_ppt) * a.pc_nr : 0);
+        const uint32_t cc = pc[(row + (a.pc_nr >> 1)) & (a.pc_nr - 1)];
+        c0 = cc & 0xffffu;
+        w = (cc >> 16) - c0;  // an empty row holds 0: w = 0
+        cm = a.pc_nc - 1;
+        half = a.pc_nc >> 1;
+    }
+    __device__ __forceinline__ bool keep(uint32_t col) const { return ((col + half) & cm) - c0 < w; }
 };
 
 static void set_layout(StridedArgs& a, const PageLayout& l) {
@@ -249,6 +271,8 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
     float2 v[16];
     CropSrc cs;
     if constexpr (SRC == SRC_CROP) cs.init(c, page, row);
+    StoreSpan span;
+    span.init(a, page, row);
     // stitch sources (grouped tiles: page k is tile k's gradient)
     const float2* __restrict__ sR = sa.R;
     const float2* __restrict__ sP = sa.P;
@@ -365,7 +389,8 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
             float2 r = v[q * SL::GSL + s];
             if constexpr (TWID) r = cmul(r, tw.get<INV>(g * (uint32_t)idx * a.tw_step));
             const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
-            if (!a.colmask || a.colmask[(g * a.cm_g + (uint32_t)idx * a.cm_s) / kMaskRows]) out[roff + col * a.ld] = r;
+            if ((!a.colmask || a.colmask[(g * a.cm_g + (uint32_t)idx * a.cm_s) / kMaskRows]) && span.keep((uint32_t)col))
+                out[roff + col * a.ld] = r;
         }
 }
 
@@ -541,6 +566,8 @@ __global__ __launch_bounds__(512, 4) void fft_strided_half_kernel(StridedArgs a,
     const size_t roff = (size_t)(row >> a.sh) * a.tstride + (row & a.rmask);
     CropSrc cs;
     if constexpr (SRC == SRC_CROP) cs.init(c, page, row);
+    StoreSpan span;
+    span.init(a, page, row);
     float2 va[16], vb[16];
     auto load = [&](float2 (&v)[16], int tv) {
         tv = fresh(tv);
@@ -579,7 +606,8 @@ __global__ __launch_bounds__(512, 4) void fft_strided_half_kernel(StridedArgs a,
                 float2 r = v[q * SL::GSL + s];
                 if constexpr (TWID) r = cmul(r, tw.get<INV>(g * (uint32_t)idx * a.tw_step));
                 const size_t col = (size_t)g * a.out_g + (size_t)idx * a.out_s;
-                if (!a.colmask || a.colmask[(g * a.cm_g + (uint32_t)idx * a.cm_s) / kMaskRows])
+                if ((!a.colmask || a.colmask[(g * a.cm_g + (uint32_t)idx * a.cm_s) / kMaskRows]) &&
+                    span.keep((uint32_t)col))
                     out[roff + col * a.ld] = r;
             }
     };
@@ -714,7 +742,7 @@ void fft_contig(const float2* in, float2* out, uint32_t len, uint32_t nlines, bo
 // place) or natural (into nat_out) frequency columns
 void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch, const ColPerm& p, const float2* twid,
                       hipStream_t st, float2* nat_out, const uint8_t* rowmask, const PageLayout* lay, int eng,
-                      uint32_t mask_ppt, const uint8_t* colmask) {
+                      uint32_t mask_ppt, const uint8_t* colmask, const uint32_t* pcols) {
     if (colmask && batch != 1) throw HipError{hipErrorInvalidValue, "column mask: one page"};
     Twiddles tw{twid};
     StridedArgs a{};
@@ -724,6 +752,13 @@ void fft_cols_forward(float2* data, uint32_t rows, uint32_t cols, uint32_t batch
     a.mask_ppt = mask_ppt;
     a.mask_stride = rows / kMaskRows;
     const uint32_t tiles = rows / kStridedRows;
+    if (pcols) {
+        if (p.M2 != 1 || rows > 65536u || cols > 65536u)
+            throw HipError{hipErrorInvalidValue, "pupil column intervals: one-pass pages of at most 65536 columns"};
+        a.pcols = pcols;
+        a.pc_nr = rows;
+        a.pc_nc = cols;
+    }
     if (p.M2 == 1) {
         a.in = data;
         a.out = nat_out ? nat_out : data;

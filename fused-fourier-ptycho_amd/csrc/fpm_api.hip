@@ -430,7 +430,9 @@ void minibatch_backward(const Geom& g, const float2* Shat, float2* P, const int2
             tm.end(1);
         }
         tm.begin(FPM_STAGE_FFT_N_COLS_LO);
-        fft_cols_forward(rec, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, rec_mask, &g.nl, g.eng);
+        // R is read only inside the pupil-support intervals (stitch, pupil update)
+        fft_cols_forward(rec, g.nr, g.nc, b, g.nl.perm, tw, st, nullptr, rec_mask, &g.nl, g.eng, 0, nullptr,
+                         prune ? pcols : nullptr);
         tm.end(col_launches(g.nl.perm));
     }
     // stitch into Gs, pupil gradient / update, then the column passes of IDFT_N(Gs)
@@ -880,7 +882,8 @@ struct GroupSolve {
                 fft_cols_inverse(lp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, &ca, nullptr, pmask, &g.nl, g.eng,
                                  b);
                 launch_stencil_fused(g, lp, y, led0, pages, pr, rp, nullptr, pmask, tw, st, g.eng, tabd, b);
-                fft_cols_forward(rp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, pmask, &g.nl, g.eng, b);
+                fft_cols_forward(rp, g.nr, g.nc, pages, g.nl.perm, tw, st, nullptr, pmask, &g.nl, g.eng, b, nullptr,
+                                 pcols);
             }
             if (separate_stitch()) {
                 launch_owner_stitch(g, rp, Pw, nullptr, led0, b, spec[1 - cur], st, tabd, T, pcols);

@@ -219,7 +219,48 @@ struct CropSrc {
         }
         return z;
     }
+    // at() in two steps, so that a loop can issue several points' loads
+    // before the first product: fetch loads S beside P inside the interval
+    // (without intervals: everywhere; every address is in bounds) and leaves
+    // zeros outside, form gives at()'s value from the pair
+    __device__ __forceinline__ void fetch(const CropArgs& c, uint32_t di, uint32_t dj, float2& sv, float2& pp) const {
+        const uint32_t i = (di + (c.nr >> 1)) & (c.nr - 1), j = (dj + (c.nc >> 1)) & (c.nc - 1);
+        sv = make_float2(0.f, 0.f);
+        pp = make_float2(0.f, 0.f);
+        if (j - pc0 < pcw) {
+            const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
+            const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
+            sv = S[(size_t)c.pN(scl) * c.Nr + srw];
+            pp = P[j * c.nr + i];
+        }
+    }
+    __device__ __forceinline__ static float2 form(float2 sv, float2 pp) {
+        return (pp.x != 0.0f || pp.y != 0.0f) ? cmul(sv, pp) : make_float2(0.f, 0.f);  // P = 0: +0, S unused
+    }
 };
+#ifndef FPM_CROP_CHUNK
+#define FPM_CROP_CHUNK 16  // points of a thread whose crop loads are issued together
+#endif
+
+// v[k] = at() of a thread's 16 input slots, gathered FPM_CROP_CHUNK points at a time
+template <class SL, class Col>
+__device__ __forceinline__ void crop_gather(float2 (&v)[16], const CropSrc& cs, const CropArgs& c, int tv,
+                                            uint32_t row, Col col_of) {
+    static_assert(SL::NG0 * SL::GS0 == 16 && 16 % FPM_CROP_CHUNK == 0, "16 input slots in whole chunks");
+    constexpr int CH = FPM_CROP_CHUNK;
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += CH) {
+        float2 sv[CH], pv[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            int line, idx;
+            SL::in(tv, (k0 + u) / SL::GS0, (k0 + u) % SL::GS0, line, idx);
+            cs.fetch(c, row, col_of(idx), sv[u], pv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) v[k0 + u] = CropSrc::form(sv[u], pv[u]);
+    }
+}
 
 template <int LEN, int TR, bool INV, bool TWID, int SRC, int ENG>
 __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs a, CropArgs c, StitchArgs sa,
@@ -363,6 +404,9 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 v[q * SL::GS0 + s] = acc;
             }
 
+    } else if constexpr (SRC == SRC_CROP) {
+        // every slot of the thread lies in row `row` (line t mod TR)
+        crop_gather<SL>(v, cs, c, t, row, [&](int idx) { return g * a.in_g + (uint32_t)idx * a.in_s; });
     } else {
 #pragma unroll
         for (int q = 0; q < SL::NG0; ++q)
@@ -371,11 +415,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 int line, idx;
                 SL::in(t, q, s, line, idx);
                 const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
-                if constexpr (SRC == SRC_CROP) {
-                    v[q * SL::GS0 + s] = cs.at(c, tile * TR + line, (uint32_t)col);
-                } else {
-                    v[q * SL::GS0 + s] = in[roff + col * a.ld];
-                }
+                v[q * SL::GS0 + s] = in[roff + col * a.ld];
             }
     }
     first_pass_dft<LEN, INV, Cfg::T, ENG>(v, ctx);
@@ -571,18 +611,19 @@ __global__ __launch_bounds__(512, 4) void fft_strided_half_kernel(StridedArgs a,
     float2 va[16], vb[16];
     auto load = [&](float2 (&v)[16], int tv) {
         tv = fresh(tv);
+        if constexpr (SRC == SRC_CROP) {
+            crop_gather<SL>(v, cs, c, tv, row, [&](int idx) { return g * a.in_g + (uint32_t)idx * a.in_s; });
+        } else {
 #pragma unroll
-        for (int q = 0; q < SL::NG0; ++q)
+            for (int q = 0; q < SL::NG0; ++q)
 #pragma unroll
-            for (int s = 0; s < SL::GS0; ++s) {
-                int line, idx;
-                SL::in(tv, q, s, line, idx);
-                const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
-                if constexpr (SRC == SRC_CROP)
-                    v[q * SL::GS0 + s] = cs.at(c, row, (uint32_t)col);
-                else
+                for (int s = 0; s < SL::GS0; ++s) {
+                    int line, idx;
+                    SL::in(tv, q, s, line, idx);
+                    const size_t col = (size_t)g * a.in_g + (size_t)idx * a.in_s;
                     v[q * SL::GS0 + s] = in[roff + col * a.ld];
-            }
+                }
+        }
     };
     load(va, t);
     load(vb, t + TV / 2);
@@ -1368,6 +1409,54 @@ template <int LEN> struct StencilCfg {
 // SIGNS: also write the two sign decisions of every interior pixel to
 // signs (natural column-major pages: bit 0 = sign bit of er - e0, bit 1 =
 // of ed - e0), a separate instantiation so the solve's kernels are unchanged.
+// Residual e = |z| / ratio - y of a virtual thread's output slots (cuFPM_v3/cuda/kernels.cuh:210-222
+// with the stack's amplitudes): load_y_slots issues every Y load of the slots
+// together, residual_slots then converts them and forms e in LDS, and
+// with_y_mode takes the uniform choices (stack type; |z| / ratio, or the exact
+// product |z| (1/ratio) when ratio is a power of two) once, outside the loops,
+// so that no branch between the slots holds their loads back.  The operations
+// are read_y_page's and scaled_mag's.
+template <class SL, bool U16, int NP, class YOff>
+__device__ __forceinline__ void load_y_slots(uint32_t (&raw)[NP], int tv, const char* ypage, YOff yoff) {
+    static_assert(NP == SL::NGL * SL::GSL, "one word per output slot");
+#pragma unroll
+    for (int q = 0; q < SL::NGL; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::GSL; ++s) {
+            int line, idx;
+            SL::out(tv, q, s, line, idx);
+            raw[q * SL::GSL + s] = load_y_page<U16>(ypage, yoff(line, idx));
+        }
+}
+template <class SL, bool U16, bool QUOT, int NP, class OnMag>
+__device__ __forceinline__ void residual_slots(const float2 (&v)[NP], const uint32_t (&raw)[NP], int tv, float* ef,
+                                               uint32_t len, const YStack& y, float ratio, float inv_ratio,
+                                               OnMag on_mag) {
+#pragma unroll
+    for (int q = 0; q < SL::NGL; ++q)
+#pragma unroll
+        for (int s = 0; s < SL::GSL; ++s) {
+            int line, idx;
+            SL::out(tv, q, s, line, idx);
+            const float2 z = v[q * SL::GSL + s];
+            const float mag = sqrtf(z.x * z.x + z.y * z.y);
+            on_mag(line, idx, mag);
+            const float sm = QUOT ? mag / ratio : mag * inv_ratio;
+            ef[line * len + idx] = sm - y_of_raw<U16>(raw[q * SL::GSL + s], y);
+        }
+}
+template <class F>
+__device__ __forceinline__ void with_y_mode(const YStack& y, float inv_ratio, F f) {
+    const bool quot = !(inv_ratio > 0.0f);  // scaled_mag's test
+    if (y.u16) {
+        if (quot) f(std::true_type{}, std::true_type{});
+        else f(std::true_type{}, std::false_type{});
+    } else {
+        if (quot) f(std::false_type{}, std::true_type{});
+        else f(std::false_type{}, std::false_type{});
+    }
+}
+
 template <int LEN, int ENG, bool WHOLE, bool SIGNS>
 __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,
@@ -1440,24 +1529,38 @@ __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(co
     // holds at most 2^22 points), the uniform stack-type branch outside the loop
     const char* ypage = (const char*)y.data + ybase * (y.u16 ? 2 : 4);
     int tt = fresh(t);
-    auto residual = [&](auto u16) {
-#pragma unroll
-        for (int q = 0; q < SI::NGL; ++q)
-#pragma unroll
-            for (int s = 0; s < SI::GSL; ++s) {
-                int line, idx;
-                SI::out(tt, q, s, line, idx);
-                const float2 z = v[q * SI::GSL + s];
-                const float mag = sqrtf(z.x * z.x + z.y * z.y);
-                if constexpr (FPM_STENCIL_MAG_LDS) mf[line * LEN + idx] = mag;
-                ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) -
-                                       read_y_page<decltype(u16)::value>(ypage, column(line) * (uint32_t)LEN + (uint32_t)idx, y);
-            }
+    auto yoff = [&](int line, int idx) { return column(line) * (uint32_t)LEN + (uint32_t)idx; };
+    auto on_mag = [&](int line, int idx, float mag) {
+        if constexpr (FPM_STENCIL_MAG_LDS) mf[line * LEN + idx] = mag;
     };
-    if (y.u16)
-        residual(std::true_type{});
-    else
-        residual(std::false_type{});
+    if constexpr (ENG == ENG_F32) {
+        with_y_mode(y, inv_ratio, [&](auto u16, auto quot) {
+            constexpr bool U16 = decltype(u16)::value;
+            uint32_t raw[SI::NGL * SI::GSL];
+            load_y_slots<SI, U16>(raw, tt, ypage, yoff);
+            residual_slots<SI, U16, decltype(quot)::value>(v, raw, tt, ef, LEN, y, ratio, inv_ratio, on_mag);
+        });
+    } else {
+        // the split engine's register budget: one Y load per point as it is used
+        auto residual = [&](auto u16) {
+#pragma unroll
+            for (int q = 0; q < SI::NGL; ++q)
+#pragma unroll
+                for (int s = 0; s < SI::GSL; ++s) {
+                    int line, idx;
+                    SI::out(tt, q, s, line, idx);
+                    const float2 z = v[q * SI::GSL + s];
+                    const float mag = sqrtf(z.x * z.x + z.y * z.y);
+                    on_mag(line, idx, mag);
+                    ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) -
+                                           read_y_page<decltype(u16)::value>(ypage, yoff(line, idx), y);
+                }
+        };
+        if (y.u16)
+            residual(std::true_type{});
+        else
+            residual(std::false_type{});
+    }
     __syncthreads();
     float loss32 = 0.0f;  // 32 terms per thread at most; the workgroup sum is FP64
     tt = fresh(t);
@@ -1759,29 +1862,17 @@ __global__ __launch_bounds__(512, 4) void stencil_half_kernel(const float2* __re
     // Y of this page through a global pointer and 32-bit offsets (a page
     // holds 2^22 points), the uniform stack-type branch outside the loop
     const char* ypage = (const char*)y.data + ybase * (y.u16 ? 2 : 4);
-    auto residual = [&](const float2 (&v)[16], int tv) {
-        tv = fresh(tv);
-        auto one = [&](auto u16) {
-#pragma unroll
-            for (int q = 0; q < SI::NGL; ++q)
-#pragma unroll
-                for (int s = 0; s < SI::GSL; ++s) {
-                    int line, idx;
-                    SI::out(tv, q, s, line, idx);
-                    const float yv = read_y_page<decltype(u16)::value>(
-                        ypage, column(line) * (uint32_t)LEN + (uint32_t)idx, y);
-                    const float2 z = v[q * SI::GSL + s];
-                    const float mag = sqrtf(z.x * z.x + z.y * z.y);
-                    ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) - yv;
-                }
-        };
-        if (y.u16)
-            one(std::true_type{});
-        else
-            one(std::false_type{});
-    };
-    residual(va, t);
-    residual(vb, t + 512);
+    auto yoff = [&](int line, int idx) { return column(line) * (uint32_t)LEN + (uint32_t)idx; };
+    auto no_mag = [](int, int, float) {};
+    with_y_mode(y, inv_ratio, [&](auto u16, auto quot) {
+        constexpr bool U16 = decltype(u16)::value, Q = decltype(quot)::value;
+        uint32_t ra[SI::NGL * SI::GSL], rb[SI::NGL * SI::GSL];
+        // both virtual threads' loads before either residual: 32 in flight
+        load_y_slots<SI, U16>(ra, fresh(t), ypage, yoff);
+        load_y_slots<SI, U16>(rb, fresh(t + 512), ypage, yoff);
+        residual_slots<SI, U16, Q>(va, ra, fresh(t), ef, LEN, y, ratio, inv_ratio, no_mag);
+        residual_slots<SI, U16, Q>(vb, rb, fresh(t + 512), ef, LEN, y, ratio, inv_ratio, no_mag);
+    });
     __syncthreads();
     // FP32 partial sums of 32 terms per virtual thread, as in the 1024-thread kernel
     float loss_a = 0.0f, loss_b = 0.0f;
@@ -1991,23 +2082,14 @@ __global__ __launch_bounds__(1024, 4) void page_fused_kernel(CropArgs c, float2*
         float* ef = (float*)lds;
         const char* ypage = (const char*)y.data + (size_t)(c.led0 + lp) * plane * (y.u16 ? 2 : 4);
         tt = fresh(t);
-        auto residual = [&](auto u16) {
-#pragma unroll
-            for (int q = 0; q < SCI::NGL; ++q)
-#pragma unroll
-                for (int s = 0; s < SCI::GSL; ++s) {
-                    int line, idx;
-                    SCI::out(tt, q, s, line, idx);
-                    const float2 z = v[q * SCI::GSL + s];
-                    const float mag = sqrtf(z.x * z.x + z.y * z.y);
-                    ef[line * LEN + idx] = scaled_mag(mag, ratio, inv_ratio) -
-                                           read_y_page<decltype(u16)::value>(ypage, (uint32_t)line * LEN + (uint32_t)idx, y);
-                }
-        };
-        if (y.u16)
-            residual(std::true_type{});
-        else
-            residual(std::false_type{});
+        with_y_mode(y, inv_ratio, [&](auto u16, auto quot) {
+            constexpr bool U16 = decltype(u16)::value;
+            uint32_t raw[SCI::NGL * SCI::GSL];
+            load_y_slots<SCI, U16>(raw, tt, ypage,
+                                   [&](int line, int idx) { return (uint32_t)line * LEN + (uint32_t)idx; });
+            residual_slots<SCI, U16, decltype(quot)::value>(v, raw, tt, ef, LEN, y, ratio, inv_ratio,
+                                                            [](int, int, float) {});
+        });
         __syncthreads();
         float loss32 = 0.0f;
         tt = fresh(t);

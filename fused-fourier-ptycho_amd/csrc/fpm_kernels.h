@@ -53,13 +53,20 @@ __device__ __forceinline__ float read_y(const YStack& y, size_t off) {
 // the stack type fixed at compile time): loads through a global-space
 // pointer and a 32-bit offset, so a page's loads need no 64-bit addresses
 template <bool U16>
+__device__ __forceinline__ uint32_t load_y_page(const char* page, uint32_t off) {
+    if constexpr (U16) return ((const __attribute__((address_space(1))) uint16_t*)page)[off];
+    else return __float_as_uint(((const __attribute__((address_space(1))) float*)page)[off]);
+}
+// the amplitude of a raw word of load_y_page (split from the load, so that a
+// loop can issue all of its loads before the first conversion)
+template <bool U16>
+__device__ __forceinline__ float y_of_raw(uint32_t raw, const YStack& y) {
+    if constexpr (U16) return sqrtf(div_u16((float)(raw - y.mn), y.range, y.rcp_range));
+    else return __uint_as_float(raw);
+}
+template <bool U16>
 __device__ __forceinline__ float read_y_page(const char* page, uint32_t off, const YStack& y) {
-    if constexpr (U16) {
-        const uint32_t raw = ((const __attribute__((address_space(1))) uint16_t*)page)[off];
-        return sqrtf(div_u16((float)(raw - y.mn), y.range, y.rcp_range));
-    } else {
-        return ((const __attribute__((address_space(1))) float*)page)[off];
-    }
+    return y_of_raw<U16>(load_y_page<U16>(page, off), y);
 }
 
 // |z| / ratio as the reference's absC (cuFPM_v3/cuda/kernels.cuh:5-9); when

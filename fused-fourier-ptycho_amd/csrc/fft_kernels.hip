@@ -166,6 +166,9 @@ enum { SRC_PLAIN = 0, SRC_CROP = 1, SRC_STITCH = 2 };
 #endif
 
 constexpr int kMaxStitchLeds = 64;  // one bit per LED of the batch
+#ifndef FPM_STITCH_PAIRS
+#define FPM_STITCH_PAIRS 2  // hits of a slot whose loads the owner stitch issues together (0: one at a time)
+#endif
 
 // Crop x pupil source of one page (SRC_CROP): its tile's spectrum, pupil
 // and LED (grouped tiles: page k ppt + l is LED led0 + l of tile k), and the
@@ -378,6 +381,46 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                 const uint32_t col = g * a.in_g + (uint32_t)idx * a.in_s;
                 float2 acc = make_float2(0.f, 0.f);
                 uint64_t m = rowhits & colhits[idx];
+#if FPM_STITCH_PAIRS
+                // the slot's hits FPM_STITCH_PAIRS at a time: their LEDs' R and P
+                // loads are issued before the first product; the sum keeps LED order
+                auto fetch = [&](int l, float2& rv, float2& pp) {
+                    rv = make_float2(0.f, 0.f);
+                    pp = make_float2(0.f, 0.f);
+                    const uint32_t w = win[l];
+                    const uint32_t li = (srow - (w & 0xffffu)) & Rm;
+                    const uint32_t lj = (col - (w >> 16)) & Cm;  // < nc: the window covers col
+                    const uint32_t cc = wcc[l * TR + t % TR];
+                    const uint32_t c0 = cc & 0xffffu;
+                    if (lj - c0 < (cc >> 16) - c0) {  // P = 0 outside the row's support interval
+                        const uint32_t di = (li + (sa.nr >> 1)) & (sa.nr - 1);
+                        const uint32_t dj = (lj + (sa.nc >> 1)) & (sa.nc - 1);
+                        rv = sR[(size_t)l * plane + sa.nl.row_off(di) + (size_t)sa.nl.perm(dj) * sa.nl.ld];
+                        pp = sP[lj * sa.nr + li];
+                    }
+                };
+                auto add = [&](float2 rv, float2 pp) {
+                    if (pp.x != 0.0f || pp.y != 0.0f) {  // P = 0 contributes +-0
+                        const float2 gw = cmulc(rv, pp);
+                        acc.x = fmaf(gw.x, sa.invN2, acc.x);
+                        acc.y = fmaf(gw.y, sa.invN2, acc.y);
+                    }
+                };
+                constexpr int K = FPM_STITCH_PAIRS;
+                while (m) {
+                    float2 r[K], pv[K];
+#pragma unroll
+                    for (int u = 0; u < K; ++u) {
+                        r[u] = pv[u] = make_float2(0.f, 0.f);
+                        if (m) {
+                            fetch(__ffsll((unsigned long long)m) - 1, r[u], pv[u]);
+                            m &= m - 1;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < K; ++u) add(r[u], pv[u]);
+                }
+#else
                 while (m) {
                     const int l = __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
@@ -401,6 +444,7 @@ __global__ __launch_bounds__(LEN * TR / 16) void fft_strided_kernel(StridedArgs 
                         }
                     }
                 }
+#endif
                 v[q * SL::GS0 + s] = acc;
             }
 

@@ -214,6 +214,9 @@ void launch_backproj(const Geom& g, const float2* R, const float2* Shat, const i
 // LED order.  Every element is written (zeros outside all windows), so the
 // gradient buffer needs no clearing.
 constexpr int kOwTR = 64, kOwTC = 16;
+#ifndef FPM_STITCH_PAIRS
+#define FPM_STITCH_PAIRS 2  // nonzero: each LED's four column loads issued before their products
+#endif
 
 // Grouped tiles (tab): grid z = tile k, its R pages k b.., pupil and
 // gradient blocks k, LED windows from tab[k].
@@ -275,6 +278,26 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
                     cw = (cc >> 16) - c0;
                 }
                 const uint32_t di = (li + (g.nr >> 1)) & (g.nr - 1);
+#if FPM_STITCH_PAIRS
+                // the four columns' R and P loads issued before their products
+                float2 Rq[4], Pq[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Rq[q] = Pq[q] = make_float2(0.f, 0.f);
+                    const uint32_t lj = (ca + cg * 4 + q - wc) & Cm;
+                    if (lj - c0 < cw) {  // (without intervals R is read wherever P is)
+                        Rq[q] = R[(size_t)ell * plane + g.nl.off(di, (lj + (g.nc >> 1)) & (g.nc - 1))];
+                        Pq[q] = P[lj * g.nr + li];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (Pq[q].x != 0.0f || Pq[q].y != 0.0f) {  // P = 0 contributes +-0
+                        const float2 gw = cmulc(Rq[q], Pq[q]);
+                        acc[q].x = fmaf(gw.x, invN2, acc[q].x);
+                        acc[q].y = fmaf(gw.y, invN2, acc[q].y);
+                    }
+#else
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t col = ca + cg * 4 + q;
@@ -293,6 +316,7 @@ __global__ __launch_bounds__(256) void owner_stitch_kernel(Geom g, const float2*
                         }
                     }
                 }
+#endif
             }
         }
         __syncthreads();

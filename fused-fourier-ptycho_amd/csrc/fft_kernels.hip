@@ -1489,18 +1489,6 @@ __device__ __forceinline__ void residual_slots(const float2 (&v)[NP], const uint
             ef[line * len + idx] = sm - y_of_raw<U16>(raw[q * SL::GSL + s], y);
         }
 }
-template <class F>
-__device__ __forceinline__ void with_y_mode(const YStack& y, float inv_ratio, F f) {
-    const bool quot = !(inv_ratio > 0.0f);  // scaled_mag's test
-    if (y.u16) {
-        if (quot) f(std::true_type{}, std::true_type{});
-        else f(std::true_type{}, std::false_type{});
-    } else {
-        if (quot) f(std::false_type{}, std::true_type{});
-        else f(std::false_type{}, std::false_type{});
-    }
-}
-
 template <int LEN, int ENG, bool WHOLE, bool SIGNS>
 __global__ __launch_bounds__(StencilCfg<LEN>::T, 4) void stencil_fused_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             Geom g, YStack y, uint32_t led0, float ratio,

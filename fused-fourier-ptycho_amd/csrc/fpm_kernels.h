@@ -1,6 +1,7 @@
 // fpm_kernels.h -- host launchers of the FPM hot-path kernels.
 #pragma once
 #include "common.h"
+#include <type_traits>
 
 namespace fpm {
 
@@ -74,6 +75,27 @@ __device__ __forceinline__ float read_y_page(const char* page, uint32_t off, con
 // product equals the quotient; otherwise inv_ratio is 0 and the division runs
 __device__ __forceinline__ float scaled_mag(float mag, float ratio, float inv_ratio) {
     return inv_ratio > 0.0f ? mag * inv_ratio : mag / ratio;
+}
+
+// f(U16, QUOT) for the uniform choices of a residual loop, taken once outside
+// it: the stack type, and scaled_mag's quotient (QUOT) or exact product
+template <class F>
+__device__ __forceinline__ void with_y_mode(const YStack& y, float inv_ratio, F f) {
+    const bool quot = !(inv_ratio > 0.0f);  // scaled_mag's test
+    if (y.u16) {
+        if (quot) f(std::true_type{}, std::true_type{});
+        else f(std::true_type{}, std::false_type{});
+    } else {
+        if (quot) f(std::false_type{}, std::true_type{});
+        else f(std::false_type{}, std::false_type{});
+    }
+}
+
+// the raw word of read_y at element `off` (uint16 count or float bits); y_of_raw converts it
+template <bool U16>
+__device__ __forceinline__ uint32_t load_y_raw(const YStack& y, size_t off) {
+    if constexpr (U16) return ((const uint16_t*)y.data)[off];
+    else return __float_as_uint(((const float*)y.data)[off]);
 }
 
 // RMSprop + pupil constraint constants of one step.  lr_dev (optional): the

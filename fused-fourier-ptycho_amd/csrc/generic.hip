@@ -856,12 +856,35 @@ __global__ __launch_bounds__(FPM_GEN_LB) void gen_cols_fused_kernel(GenFFT a, Ge
     }
     __syncthreads();
     gen_passes(a, x, lt, true);
-    for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-        const uint32_t l = fdiv(idx, a.f_len), e = idx - l * len;
-        const float2 v = x[idx];
-        const float mag = sqrtf(v.x * v.x + v.y * v.y);
-        ef[idx] = scaled_mag(mag, c.ratio, c.inv_ratio) - read_y(y, ybase + (size_t)column(l) * len + e);
-    }
+    // the image loads of GY of a thread's points are issued before their
+    // conversions, and the uniform choices are taken outside the loop (the
+    // operations of read_y and scaled_mag)
+    with_y_mode(y, c.inv_ratio, [&](auto u16, auto quot) {
+        constexpr bool U16 = decltype(u16)::value, Q = decltype(quot)::value;
+        constexpr uint32_t GY = 8;
+        for (uint32_t i0 = threadIdx.x; i0 < tot; i0 += GY * blockDim.x) {
+            uint32_t raw[GY];
+#pragma unroll
+            for (uint32_t u = 0; u < GY; ++u) {
+                const uint32_t idx = i0 + u * blockDim.x;
+                raw[u] = 0;
+                if (idx < tot) {
+                    const uint32_t l = fdiv(idx, a.f_len), e = idx - l * len;
+                    raw[u] = load_y_raw<U16>(y, ybase + (size_t)column(l) * len + e);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < GY; ++u) {
+                const uint32_t idx = i0 + u * blockDim.x;
+                if (idx < tot) {
+                    const float2 v = x[idx];
+                    const float mag = sqrtf(v.x * v.x + v.y * v.y);
+                    const float sm = Q ? mag / c.ratio : mag * c.inv_ratio;
+                    ef[idx] = sm - y_of_raw<U16>(raw[u], y);
+                }
+            }
+        }
+    });
     __syncthreads();
     float loss32 = 0.0f;
     for (uint32_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {

@@ -2068,6 +2068,31 @@ __global__ __launch_bounds__(1024, 4) void page_fused_kernel(CropArgs c, float2*
         // crop x pupil at (row di, natural column dj): ifftshift(S P)
         float2 v[16];
         int tt = fresh(t);
+        if constexpr (ENG == ENG_F32) {
+            // every P load of the thread, then the S loads where P != 0, then
+            // the products: two round trips instead of one per point
+            float2 pv[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                int line, idx;
+                SJI::in(tt, k / SJI::GS0, k % SJI::GS0, line, idx);
+                const uint32_t i = ((uint32_t)line + h) & m, j = ((uint32_t)idx + h) & m;
+                pv[k] = cP[j * LEN + i];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                int line, idx;
+                SJI::in(tt, k / SJI::GS0, k % SJI::GS0, line, idx);
+                const uint32_t i = ((uint32_t)line + h) & m, j = ((uint32_t)idx + h) & m;
+                const uint32_t srw = (i + kt - 1u + (c.Nr >> 1)) & (c.Nr - 1);
+                const uint32_t scl = (j + kl - 1u + (c.Nc >> 1)) & (c.Nc - 1);
+                v[k] = make_float2(0.f, 0.f);
+                if (pv[k].x != 0.0f || pv[k].y != 0.0f) v[k] = cS[(size_t)c.pN(scl) * c.Nr + srw];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k)  // P = 0: S P = +-0, S not read
+                v[k] = (pv[k].x != 0.0f || pv[k].y != 0.0f) ? cmul(v[k], pv[k]) : make_float2(0.f, 0.f);
+        } else {
 #pragma unroll
         for (int q = 0; q < SJI::NG0; ++q)
 #pragma unroll
@@ -2082,6 +2107,7 @@ __global__ __launch_bounds__(1024, 4) void page_fused_kernel(CropArgs c, float2*
                 if (pp.x != 0.0f || pp.y != 0.0f) z = cmul(cS[(size_t)c.pN(scl) * c.Nr + srw], pp);
                 v[q * SJI::GS0 + s] = z;
             }
+        }
         first_pass_dft<LEN, true, T, ENG>(v, ctx);
         if (PERSIST) __syncthreads();  // the previous page's last reads of lds
         remaining_passes<LEN, true, T, FMap, ENG>(v, lds, fresh(t), tw, ctx);

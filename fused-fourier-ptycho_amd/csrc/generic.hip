@@ -313,6 +313,29 @@ struct GenCrop {
     uint32_t led0, b, nr, nc, Nr, Nc;
     FastDiv f_b;
 };
+// the P and S elements behind gen_crop_value (S is read only where P != 0)
+struct GenCropAt {
+    const float2* p;
+    const float2* s;
+};
+__device__ __forceinline__ GenCropAt gen_crop_at(const GenCrop& c, uint32_t pg, uint32_t di, uint32_t dj) {
+    const float2* S = c.S;
+    const float2* P = c.P;
+    const int2* led = c.led;
+    uint32_t l = pg;
+    if (c.tab) {
+        const uint32_t tk = fdiv(pg, c.f_b);
+        l = pg - tk * c.b;
+        S += (size_t)tk * c.Nr * c.Nc;
+        P += (size_t)tk * c.nr * c.nc;
+        led = c.tab[tk].led;
+    }
+    const uint32_t i = wrap1(di + c.nr / 2, c.nr), j = wrap1(dj + c.nc / 2, c.nc);
+    const int2 kk = led[c.led0 + l];
+    const uint32_t srow = wrap1(i + (uint32_t)kk.y - 1u + c.Nr / 2, c.Nr);
+    const uint32_t scol = wrap1(j + (uint32_t)kk.x - 1u + c.Nc / 2, c.Nc);
+    return GenCropAt{P + (size_t)j * c.nr + i, S + (size_t)scol * c.Nr + srow};
+}
 __device__ __forceinline__ float2 gen_crop_value(const GenCrop& c, uint32_t pg, uint32_t di, uint32_t dj) {
     const float2* S = c.S;
     const float2* P = c.P;
@@ -472,11 +495,33 @@ __global__ __launch_bounds__(3 * P / 16) void fft3m_kernel(GenFFT a, GenCrop c, 
     // thread keeps its 16 loads in flight at once
     if (nl == (uint32_t)NL) {
         float2 buf[16];
+        if constexpr (CROP) {
+            // the crop's P loads, then its S loads where P != 0, then the
+            // products (gen_crop_value's operations): two round trips
+            float2 pv[16];
+            auto at = [&](int i) {
+                uint32_t l, e;
+                split((uint32_t)(t + i * T), l, e);
+                const uint32_t L = L0 + l, pg = fdiv(L, a.f_lpb);
+                return gen_crop_at(c, pg, L - pg * a.lines_per_batch, e);
+            };
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            uint32_t l, e;
-            split((uint32_t)(t + i * T), l, e);
-            buf[i] = load(l, e);
+            for (int i = 0; i < 16; ++i) pv[i] = *at(i).p;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                buf[i] = make_float2(0.f, 0.f);
+                if (pv[i].x != 0.0f || pv[i].y != 0.0f) buf[i] = *at(i).s;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                buf[i] = (pv[i].x != 0.0f || pv[i].y != 0.0f) ? cmul_g(buf[i], pv[i]) : make_float2(0.f, 0.f);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                uint32_t l, e;
+                split((uint32_t)(t + i * T), l, e);
+                buf[i] = load(l, e);
+            }
         }
 #pragma unroll
         for (int i = 0; i < 16; ++i) {

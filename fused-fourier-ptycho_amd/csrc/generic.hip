@@ -1065,25 +1065,28 @@ __global__ __launch_bounds__(256) void gen_stitch_kernel(const float2* __restric
                     c0 = cc & 0xffffu;
                     cw = (cc >> 16) - c0;
                 }
+                // the four columns' R and P loads issued before their products
+                // (without intervals R is read wherever P is)
+                float2 Rq[4], Pq[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
+                    Rq[q] = Pq[q] = make_float2(0.f, 0.f);
                     const uint32_t col = ca + cg * 4u + (uint32_t)q;
                     if (col >= Nc) continue;
                     const uint32_t lj = col >= wc ? col - wc : col + Nc - wc;
                     if (lj - c0 < cw) {
                         const uint32_t dj = wrap1(lj + nc / 2, nc);
-                        const float2* rp = R + (size_t)l * plane + (size_t)dj * nr + di;
-                        float2 rv = make_float2(0.f, 0.f);
-                        if (pcols) rv = *rp;
-                        const float2 pp = P[(size_t)lj * nr + li];
-                        if (pp.x != 0.0f || pp.y != 0.0f) {
-                            if (!pcols) rv = *rp;
-                            const float2 gw = cmulc_g(rv, pp);
-                            acc[q].x = fmaf(gw.x, invN2, acc[q].x);
-                            acc[q].y = fmaf(gw.y, invN2, acc[q].y);
-                        }
+                        Rq[q] = R[(size_t)l * plane + (size_t)dj * nr + di];
+                        Pq[q] = P[(size_t)lj * nr + li];
                     }
                 }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (Pq[q].x != 0.0f || Pq[q].y != 0.0f) {  // P = 0 contributes +-0
+                        const float2 gw = cmulc_g(Rq[q], Pq[q]);
+                        acc[q].x = fmaf(gw.x, invN2, acc[q].x);
+                        acc[q].y = fmaf(gw.y, invN2, acc[q].y);
+                    }
             }
         }
         __syncthreads();
